@@ -1,0 +1,316 @@
+// Golden-vector harness for the reference PIC++ implicit step.
+//
+// Test infrastructure only: links against the unmodified reference objects
+// (compiled from PIC++/*.cpp by oracle/Makefile.ref) and drives the public
+// Simulation methods in the order of the reference step loop
+// (PIC++/simulation.cpp:94-123 pre-loop, :205-390 loop body, implicit branch,
+// smoothingCount = 0), dumping every per-phase array the hot path produces.
+//
+// Input  (binary, little endian): see read_input() below.
+// Output (binary): a sequence of named float64 arrays, read by
+//                  tests/golden/make_golden.py.
+#include <mpi.h>
+#include <cstdio>
+#include <cstdlib>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "simulation.h"
+#include "particle.h"
+#include "vector3d.h"
+#include "matrix3d.h"
+
+namespace {
+
+FILE* g_out = nullptr;
+
+void dump(const char* name, const std::vector<double>& data, std::vector<uint64_t> dims) {
+	uint32_t len = (uint32_t)std::string(name).size();
+	fwrite(&len, 4, 1, g_out);
+	fwrite(name, 1, len, g_out);
+	uint32_t nd = (uint32_t)dims.size();
+	fwrite(&nd, 4, 1, g_out);
+	fwrite(dims.data(), 8, nd, g_out);
+	fwrite(data.data(), 8, data.size(), g_out);
+}
+
+std::string tag(const char* base, int step) {
+	char buf[128];
+	snprintf(buf, sizeof(buf), "s%d/%s", step, base);
+	return std::string(buf);
+}
+
+void dump_node_vec(Simulation& s, Vector3d*** f, const std::string& name) {
+	int NX = s.xnumberAdded + 1, NY = s.ynumberAdded + 1, NZ = s.znumberAdded + 1;
+	std::vector<double> d;
+	d.reserve((size_t)NX * NY * NZ * 3);
+	for (int i = 0; i < NX; ++i)
+		for (int j = 0; j < NY; ++j)
+			for (int k = 0; k < NZ; ++k)
+				for (int l = 0; l < 3; ++l) d.push_back(f[i][j][k][l]);
+	dump(name.c_str(), d, {(uint64_t)NX, (uint64_t)NY, (uint64_t)NZ, 3});
+}
+
+void dump_node_mat(Simulation& s, Matrix3d*** f, const std::string& name) {
+	int NX = s.xnumberAdded + 1, NY = s.ynumberAdded + 1, NZ = s.znumberAdded + 1;
+	std::vector<double> d;
+	for (int i = 0; i < NX; ++i)
+		for (int j = 0; j < NY; ++j)
+			for (int k = 0; k < NZ; ++k)
+				for (int a = 0; a < 3; ++a)
+					for (int b = 0; b < 3; ++b) d.push_back(f[i][j][k].matrix[a][b]);
+	dump(name.c_str(), d, {(uint64_t)NX, (uint64_t)NY, (uint64_t)NZ, 3, 3});
+}
+
+void dump_cell_vec(Simulation& s, Vector3d*** f, const std::string& name) {
+	int NX = s.xnumberAdded, NY = s.ynumberAdded, NZ = s.znumberAdded;
+	std::vector<double> d;
+	for (int i = 0; i < NX; ++i)
+		for (int j = 0; j < NY; ++j)
+			for (int k = 0; k < NZ; ++k)
+				for (int l = 0; l < 3; ++l) d.push_back(f[i][j][k][l]);
+	dump(name.c_str(), d, {(uint64_t)NX, (uint64_t)NY, (uint64_t)NZ, 3});
+}
+
+void dump_cell_mat(Simulation& s, Matrix3d*** f, const std::string& name) {
+	int NX = s.xnumberAdded, NY = s.ynumberAdded, NZ = s.znumberAdded;
+	std::vector<double> d;
+	for (int i = 0; i < NX; ++i)
+		for (int j = 0; j < NY; ++j)
+			for (int k = 0; k < NZ; ++k)
+				for (int a = 0; a < 3; ++a)
+					for (int b = 0; b < 3; ++b) d.push_back(f[i][j][k].matrix[a][b]);
+	dump(name.c_str(), d, {(uint64_t)NX, (uint64_t)NY, (uint64_t)NZ, 3, 3});
+}
+
+void dump_cell_scalar(Simulation& s, double*** f, const std::string& name) {
+	int NX = s.xnumberAdded, NY = s.ynumberAdded, NZ = s.znumberAdded;
+	std::vector<double> d;
+	for (int i = 0; i < NX; ++i)
+		for (int j = 0; j < NY; ++j)
+			for (int k = 0; k < NZ; ++k) d.push_back(f[i][j][k]);
+	dump(name.c_str(), d, {(uint64_t)NX, (uint64_t)NY, (uint64_t)NZ});
+}
+
+void dump_rhs(Simulation& s, const std::string& name) {
+	int NX = s.xnumberAdded, NY = s.ynumberAdded, NZ = s.znumberAdded;
+	std::vector<double> d;
+	for (int i = 0; i < NX; ++i)
+		for (int j = 0; j < NY; ++j)
+			for (int k = 0; k < NZ; ++k)
+				for (int l = 0; l < 3; ++l) d.push_back(s.maxwellEquationRightPart[i][j][k][l]);
+	dump(name.c_str(), d, {(uint64_t)NX, (uint64_t)NY, (uint64_t)NZ, 3});
+}
+
+// Particles in the reference's current vector order: id, x, y, z, px, py, pz.
+void dump_particles(Simulation& s, const std::string& name) {
+	std::vector<double> d;
+	for (size_t p = 0; p < s.particles.size(); ++p) {
+		Particle* q = s.particles[p];
+		Vector3d m = q->getMomentum();
+		d.push_back((double)q->number);
+		d.push_back(q->coordinates.x);
+		d.push_back(q->coordinates.y);
+		d.push_back(q->coordinates.z);
+		d.push_back(m.x);
+		d.push_back(m.y);
+		d.push_back(m.z);
+	}
+	dump(name.c_str(), d, {(uint64_t)s.particles.size(), 7});
+}
+
+void dump_alpha(Simulation& s, const std::string& name) {
+	std::vector<double> d;
+	for (size_t p = 0; p < s.particles.size(); ++p) {
+		Particle* q = s.particles[p];
+		d.push_back((double)q->number);
+		for (int a = 0; a < 3; ++a)
+			for (int b = 0; b < 3; ++b) d.push_back(q->rotationTensor.matrix[a][b]);
+	}
+	dump(name.c_str(), d, {(uint64_t)s.particles.size(), 10});
+}
+
+void dump_species(Simulation& s, int step, const std::vector<int>& types) {
+	for (size_t t = 0; t < types.size(); ++t) {
+		int tn = types[t];
+		char nb[64];
+		snprintf(nb, sizeof(nb), "conc%zu", t);
+		dump_cell_scalar(s, s.particleConcentrations[tn], tag(nb, step));
+		snprintf(nb, sizeof(nb), "energy%zu", t);
+		dump_cell_scalar(s, s.particleEnergies[tn], tag(nb, step));
+		snprintf(nb, sizeof(nb), "bulkv%zu", t);
+		dump_cell_vec(s, s.particleBulkVelocities[tn], tag(nb, step));
+	}
+}
+
+void dump_energy(Simulation& s, int step) {
+	std::vector<double> d = {s.particleEnergy, s.electricFieldEnergy, s.magneticFieldEnergy, s.energy,
+	                         s.globalMomentum.x, s.globalMomentum.y, s.globalMomentum.z};
+	dump(tag("energy", step).c_str(), d, {7});
+}
+
+template <class T>
+T rd(FILE* f) {
+	T v;
+	if (fread(&v, sizeof(T), 1, f) != 1) {
+		fprintf(stderr, "short input\n");
+		exit(2);
+	}
+	return v;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+	MPI_Init(&argc, &argv);
+	if (argc < 3) {
+		fprintf(stderr, "usage: ref_harness input.bin output.bin\n");
+		return 2;
+	}
+	FILE* in = fopen(argv[1], "rb");
+	if (!in) {
+		fprintf(stderr, "cannot open %s\n", argv[1]);
+		return 2;
+	}
+	// ---- input ----
+	int nx = rd<int32_t>(in), ny = rd<int32_t>(in), nz = rd<int32_t>(in);
+	double dx = rd<double>(in), dt = rd<double>(in);
+	int nsteps = rd<int32_t>(in);
+	int nspecies = rd<int32_t>(in);
+	std::vector<int> sp_type(nspecies);
+	std::vector<double> sp_mass(nspecies), sp_charge(nspecies), sp_weight(nspecies);
+	for (int s = 0; s < nspecies; ++s) {
+		sp_type[s] = rd<int32_t>(in);
+		sp_mass[s] = rd<double>(in);
+		sp_charge[s] = rd<double>(in);
+		sp_weight[s] = rd<double>(in);
+	}
+	int64_t np = rd<int64_t>(in);
+
+	int dims[3] = {1, 1, 1};
+	int period[3] = {1, 1, 1};
+	MPI_Comm comm;
+	MPI_Cart_create(MPI_COMM_WORLD, 3, dims, period, 0, &comm);
+
+	int ppb[8] = {1, 1, 0, 0, 0, 0, 0, 0};
+	double conc[8] = {1, 1, 0, 0, 0, 0, 0, 0};
+	// 37-argument constructor with the IMPLICIT solver (PIC++/initialization.cpp:506-516).
+	Simulation* sim = new Simulation(nx, ny, nz, dx, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0,
+	                                 nsteps + 10, 1e30, 1000000, 1, 1000000, 1000000, 0, 0.1, false, 8, ppb, conc,
+	                                 1, 1, 0, dt, 0.01, 1.0, 1.0, IMPLICIT, comm);
+	Simulation& s = *sim;
+	s.timing = false;
+	s.createArrays();
+	s.initialize();
+	s.deltaT = dt;
+
+	// Fields: E on (N+1)^3 nodes, B on N^3 cells, both ghosted.
+	int NX = s.xnumberAdded, NY = s.ynumberAdded, NZ = s.znumberAdded;
+	for (int i = 0; i <= NX; ++i)
+		for (int j = 0; j <= NY; ++j)
+			for (int k = 0; k <= NZ; ++k) {
+				Vector3d e;
+				e.x = rd<double>(in);
+				e.y = rd<double>(in);
+				e.z = rd<double>(in);
+				s.Efield[i][j][k] = e;
+				s.newEfield[i][j][k] = e;
+				s.tempEfield[i][j][k] = e;
+			}
+	for (int i = 0; i < NX; ++i)
+		for (int j = 0; j < NY; ++j)
+			for (int k = 0; k < NZ; ++k) {
+				Vector3d b;
+				b.x = rd<double>(in);
+				b.y = rd<double>(in);
+				b.z = rd<double>(in);
+				s.Bfield[i][j][k] = b;
+				s.newBfield[i][j][k] = b;
+			}
+	for (int64_t p = 0; p < np; ++p) {
+		int sp = rd<int32_t>(in);
+		double x = rd<double>(in), y = rd<double>(in), z = rd<double>(in);
+		double px = rd<double>(in), py = rd<double>(in), pz = rd<double>(in);
+		Particle* q = new Particle((int)p, sp_mass[sp], sp_charge[sp] > 0 ? 1 : -1, sp_charge[sp], sp_weight[sp],
+		                           (ParticleTypes)sp_type[sp], x, y, z, px, py, pz, s.deltaX / 2, s.deltaY / 2,
+		                           s.deltaZ / 2, dt);
+		s.particles.push_back(q);
+	}
+	fclose(in);
+	s.particlesNumber = (int)np;
+
+	g_out = fopen(argv[2], "wb");
+	std::vector<double> meta = {(double)NX, (double)NY, (double)NZ, s.deltaX, s.deltaT, s.xgrid[0], s.ygrid[0],
+	                            s.zgrid[0], s.theta, s.eta, s.cellVolume};
+	dump("meta", meta, {meta.size()});
+
+	// ---- pre-loop (PIC++/simulation.cpp:86-123, implicit branch) ----
+	s.exchangeEfield();
+	s.exchangeGeneralBfield(s.Bfield);
+	s.exchangeGeneralBfield(s.newBfield);
+	s.updateParticleCorrelationMaps();
+	s.evaluateParticlesRotationTensor();
+	s.updateElectroMagneticParameters();
+	s.updateDensityParameters();
+	s.updateFields();
+	s.currentIteration = 0;
+	s.updateEnergy();
+	dump_particles(s, tag("particles", 0));
+	dump_node_vec(s, s.Efield, tag("E", 0));
+	dump_cell_vec(s, s.Bfield, tag("B", 0));
+	dump_cell_scalar(s, s.chargeDensity, tag("rho", 0));
+	dump_species(s, 0, sp_type);
+	dump_energy(s, 0);
+
+	// ---- loop body (PIC++/simulation.cpp:205-390, implicit branch, smoothingCount = 0) ----
+	for (int step = 1; step <= nsteps; ++step) {
+		bool full = (step <= 2);
+		s.evaluateParticlesRotationTensor();
+		if (full) dump_alpha(s, tag("alpha", step));
+		s.updateElectroMagneticParameters();
+		if (full) {
+			dump_node_vec(s, s.electricFlux, tag("J", step));
+			dump_node_mat(s, s.dielectricTensor, tag("mu", step));
+			dump_cell_scalar(s, s.chargeDensityHat, tag("rhohat", step));
+			dump_cell_mat(s, s.pressureTensor, tag("Pi", step));
+			dump_node_vec(s, s.divPressureTensor, tag("divPi", step));
+			s.evaluateMaxwellEquationMatrix();
+			dump_rhs(s, tag("rhs", step));
+		}
+		s.evaluateElectricField();
+		s.exchangeEfield();
+		s.evaluateMagneticField();
+		s.exchangeGeneralBfield(s.newBfield);
+		s.exchangeGeneralEfield(s.tempEfield);
+		s.exchangeGeneralEfield(s.newEfield);
+		if (full) {
+			dump_node_vec(s, s.tempEfield, tag("Etheta", step));
+			dump_node_vec(s, s.newEfield, tag("Enew", step));
+			dump_cell_vec(s, s.newBfield, tag("Bnew", step));
+		}
+		s.moveParticles();
+		s.eraseEscapedPaticles();
+		s.exchangeParticles();
+		s.updateParticleCorrelationMaps();
+		s.updateDensityParameters();
+		s.exchangeGeneralEfield(s.newEfield);
+		s.updateFields();
+		s.exchangeGeneralEfield(s.newEfield);
+		s.exchangeGeneralEfield(s.Efield);
+		s.exchangeGeneralBfield(s.Bfield);
+		s.removeEscapedParticles();
+		s.time += s.deltaT;
+		s.currentIteration++;
+		s.updateEnergy();
+		dump_particles(s, tag("particles", step));
+		dump_node_vec(s, s.Efield, tag("E", step));
+		dump_cell_vec(s, s.Bfield, tag("B", step));
+		dump_cell_scalar(s, s.chargeDensity, tag("rho", step));
+		if (full) dump_species(s, step, sp_type);
+		dump_energy(s, step);
+	}
+	fclose(g_out);
+	MPI_Finalize();
+	return 0;
+}

@@ -1,0 +1,239 @@
+"""Workload definitions and seeded synthetic initial states.
+
+Units are the reference's normalised Gaussian units with c = 1 (PIC++ keeps
+c = 1, ``speed_of_light_normalized`` is commented out throughout, e.g.
+PIC++/mover.cpp:234): lengths in electron skin depths d_e = c/omega_pe, times
+in 1/omega_pe, masses in electron masses. With electron density n_e = 1 the
+plasma frequency omega_pe^2 = 4 pi n_e q_e^2 / m_e = 1 fixes |q_e| = 1/sqrt(4 pi).
+Particle weight is the number of physical particles a macro-particle carries,
+n V_cell / ppc, as in the reference's weight = concentration/ppc * volume
+(PIC++/mover.cpp:935-937 uses the same product for injected particles).
+
+The grid is the reference's ghosted grid: n physical cells per axis, two ghost
+layers each side (N = n + 4), node 0 at coordinate L - 2 dx with L = n dx
+(PIC++/initialization.cpp:71-202), so owned particles sit in [L, 2L).
+
+Initial states are synthetic: the reference seeds rand() with time(NULL)
+(PIC++/random.cpp:35, main.cpp:73), so its states cannot be reproduced; every
+state here is generated from an explicit seed.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+Q_E = 1.0 / math.sqrt(4.0 * math.pi)  # |electron charge| in these units
+
+# PIC++ ParticleTypes (PIC++/particleTypes.h:4)
+ELECTRON, PROTON, POSITRON = 0, 1, 2
+
+
+@dataclasses.dataclass
+class Species:
+    name: str
+    ptype: int            # reference ParticleTypes index
+    mass: float           # in electron masses
+    charge: float         # signed, in these units
+    ppc: int              # macro-particles per cell
+    temperature: float    # kT / (m c^2) of this species in its rest frame
+    density: float = 1.0  # physical density in n_e units
+    drift_gamma: float = 1.0  # bulk Lorentz factor of the drift
+    drift_axis: int = 2
+    drift_sign: int = 1
+
+    @property
+    def weight_per_volume(self) -> float:
+        return self.density / self.ppc
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    n: Tuple[int, int, int]
+    dx: float
+    dt: float
+    steps: int
+    species: List[Species]
+    b0: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    e0: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    seed: int = 42
+    gmres_tolerance: float = 1e-8
+
+    @property
+    def N(self) -> Tuple[int, int, int]:
+        return tuple(v + 4 for v in self.n)
+
+    @property
+    def cell_volume(self) -> float:
+        return self.dx ** 3
+
+    @property
+    def particles_total(self) -> int:
+        return sum(s.ppc for s in self.species) * int(np.prod(self.n))
+
+    def weight(self, s: Species) -> float:
+        return s.density * self.cell_volume / s.ppc
+
+    def origin(self) -> Tuple[float, float, float]:
+        """Coordinate of node 0 per axis: L - 2 dx (PIC++/initialization.cpp:194)."""
+        return tuple(v * self.dx + (-2) * self.dx for v in self.n)
+
+
+def electron_proton(ppc: int, theta_e: float, mass_ratio: float, theta_p: Optional[float] = None) -> List[Species]:
+    """Electron-proton plasma at a common temperature unless theta_p is given."""
+    if theta_p is None:
+        theta_p = theta_e / mass_ratio  # same kT
+    return [
+        Species("electron", ELECTRON, 1.0, -Q_E, ppc, theta_e),
+        Species("proton", PROTON, mass_ratio, Q_E, ppc, theta_p),
+    ]
+
+
+def b_for_gyro(omega_ratio: float) -> float:
+    """|B| that gives omega_ce / omega_pe = omega_ratio for m_e = 1, |q_e| = Q_E."""
+    return omega_ratio / Q_E
+
+
+def config1(n=(32, 32, 8), ppc=20, steps=50, seed=42) -> Workload:
+    """Plain FP64 implicit step, the reference's CPU-runnable case: periodic 32x32x8,
+    dx = 1 d_e, e-/p+ with m_p/m_e = 100, 20 ppc per species, kT/mc^2 = 0.01,
+    omega_ce/omega_pe = 0.5 along x, dt = 0.1/omega_pe, 50 steps, seed 42."""
+    return Workload("implicit-e-p-%dx%dx%d-ppc%d" % (n[0], n[1], n[2], ppc), tuple(n), 1.0, 0.1, steps,
+                    electron_proton(ppc, 0.01, 100.0), b0=(b_for_gyro(0.5), 0.0, 0.0), seed=seed)
+
+
+def config2(n=(128, 128, 128), ppc=64, steps=200, seed=42) -> Workload:
+    """Same physics at one-rank production size: 128^3 cells, 64 ppc per species."""
+    return Workload("implicit-e-p-%dx%dx%d-ppc%d" % (n[0], n[1], n[2], ppc), tuple(n), 1.0, 0.1, steps,
+                    electron_proton(ppc, 0.01, 100.0), b0=(b_for_gyro(0.5), 0.0, 0.0), seed=seed)
+
+
+def mini(n=(6, 5, 5), ppc=4, steps=5, seed=7) -> Workload:
+    """Small case for golden vectors and fast parity tests."""
+    return Workload("mini-%dx%dx%d-ppc%d" % (n[0], n[1], n[2], ppc), tuple(n), 1.0, 0.1, steps,
+                    electron_proton(ppc, 0.01, 100.0), b0=(b_for_gyro(0.5), 0.0, 0.0), seed=seed)
+
+
+def weibel(n=(192, 192, 192), ppc=32, steps=300, seed=7) -> Workload:
+    """Counterstreaming e-/e+ beams along +/-z, drift gamma 5, kT/mc^2 = 0.01 in the beam frame."""
+    sp = []
+    for name, ptype, q in (("electron", ELECTRON, -Q_E), ("positron", POSITRON, Q_E)):
+        for sign in (1, -1):
+            sp.append(Species("%s%+d" % (name, sign), ptype, 1.0, q, ppc, 0.01, density=0.5,
+                              drift_gamma=5.0, drift_axis=2, drift_sign=sign))
+    return Workload("weibel-%d^3-ppc%d" % (n[0], ppc), tuple(n), 1.0, 0.1, steps, sp, seed=seed)
+
+
+# ------------------------------------------------------------------ sampling
+def _mj_table(theta: float, npts: int = 65536):
+    """Tabulated CDF of |u| for the Maxwell-Juettner distribution
+    f(u) du ~ u^2 exp(-(gamma - 1)/theta) du, gamma = sqrt(1 + u^2)."""
+    gmax = 1.0 + 60.0 * theta
+    umax = math.sqrt(gmax * gmax - 1.0)
+    u = np.linspace(0.0, umax, npts)
+    g = np.sqrt(1.0 + u * u)
+    # (gamma - 1) in the compensated form u^2/(gamma+1) keeps small-theta tails exact
+    f = u * u * np.exp(-(u * u / (g + 1.0)) / theta)
+    cdf = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(u))])
+    cdf /= cdf[-1]
+    return u, cdf
+
+
+def sample_maxwell_juettner(rng: np.random.Generator, theta: float, count: int) -> np.ndarray:
+    """Isotropic Maxwell-Juettner momenta u = p/(m c), shape (count, 3)."""
+    ut, cdf = _mj_table(theta)
+    mag = np.interp(rng.random(count), cdf, ut)
+    cost = 2.0 * rng.random(count) - 1.0
+    phi = 2.0 * math.pi * rng.random(count)
+    sint = np.sqrt(np.maximum(0.0, 1.0 - cost * cost))
+    return np.stack([mag * sint * np.cos(phi), mag * sint * np.sin(phi), mag * cost], axis=1)
+
+
+def boost(rng: np.random.Generator, u: np.ndarray, gamma_d: float, axis: int, sign: int) -> np.ndarray:
+    """Boost rest-frame momenta along `axis` with the volume-transform flipping
+    method (Zenitani 2015, Phys. Plasmas 22, 042116)."""
+    if gamma_d <= 1.0:
+        return u
+    beta = sign * math.sqrt(1.0 - 1.0 / (gamma_d * gamma_d))
+    g = np.sqrt(1.0 + np.sum(u * u, axis=1))
+    ua = u[:, axis].copy()
+    flip = (-beta * ua / g) > rng.random(len(ua))
+    ua[flip] = -ua[flip]
+    out = u.copy()
+    out[:, axis] = gamma_d * (ua + beta * g)
+    return out
+
+
+@dataclasses.dataclass
+class InitialState:
+    species: np.ndarray    # int32 (np,)
+    ids: np.ndarray        # int32 (np,)
+    x: np.ndarray          # float64 (np, 3) absolute coordinates
+    p: np.ndarray          # float64 (np, 3) momentum m*gamma*v
+    E: np.ndarray          # float64 (N+1, N+1, N+1, 3) ghosted nodes
+    B: np.ndarray          # float64 (N, N, N, 3) ghosted cells
+
+
+def periodic_fill_nodes(owned: np.ndarray) -> np.ndarray:
+    """Ghosted node array (N+1 per axis) from owned nodes 2..n+1, periodic."""
+    n = owned.shape[:3]
+    idx = [np.mod(np.arange(v + 5) - 2, v) for v in n]
+    return owned[np.ix_(idx[0], idx[1], idx[2])]
+
+
+def periodic_fill_cells(owned: np.ndarray) -> np.ndarray:
+    """Ghosted cell array (N per axis) from owned cells 2..n+1, periodic."""
+    n = owned.shape[:3]
+    idx = [np.mod(np.arange(v + 4) - 2, v) for v in n]
+    return owned[np.ix_(idx[0], idx[1], idx[2])]
+
+
+def make_initial_state(w: Workload, perturb: float = 0.0) -> InitialState:
+    """Seeded initial state: positions uniform within each cell, momenta
+    Maxwell-Juettner (boosted for drifting species), uniform E0/B0 plus an optional
+    smooth perturbation of relative size `perturb` (used by parity tests so that
+    gathers see non-uniform fields)."""
+    rng = np.random.default_rng(w.seed)
+    n = np.array(w.n)
+    ncell = int(np.prod(n))
+    lo = n * w.dx  # first owned coordinate per axis
+    sps, xs, ps = [], [], []
+    cells = np.stack(np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij"), -1).reshape(-1, 3)
+    for si, s in enumerate(w.species):
+        cnt = ncell * s.ppc
+        c = np.repeat(cells, s.ppc, axis=0)
+        x = lo + (c + rng.random((cnt, 3))) * w.dx
+        u = sample_maxwell_juettner(rng, s.temperature, cnt)
+        u = boost(rng, u, s.drift_gamma, s.drift_axis, s.drift_sign)
+        sps.append(np.full(cnt, si, np.int32))
+        xs.append(x)
+        ps.append(u * s.mass)
+    sp = np.concatenate(sps)
+    x = np.concatenate(xs)
+    p = np.concatenate(ps)
+    # fields on owned nodes (node i at lo + i dx) and cells (centre lo + (i + 1/2) dx)
+    Eo = np.zeros(tuple(n) + (3,))
+    Bo = np.zeros(tuple(n) + (3,))
+    Eo[...] = np.array(w.e0)
+    Bo[...] = np.array(w.b0)
+    if perturb:
+        grid_n = np.meshgrid(*[np.arange(v) / v for v in n], indexing="ij")
+        grid_c = np.meshgrid(*[(np.arange(v) + 0.5) / v for v in n], indexing="ij")
+        b0 = max(1.0, float(np.linalg.norm(w.b0)))
+        for c in range(3):
+            ph = 2 * math.pi * (grid_n[0] * (c + 1) + grid_n[1] * 1 + grid_n[2] * (2 - c % 2))
+            Eo[..., c] += perturb * b0 * np.sin(ph + 0.3 * c)
+            ph = 2 * math.pi * (grid_c[0] * 1 + grid_c[1] * (c % 2 + 1) + grid_c[2] * 1)
+            Bo[..., c] += perturb * b0 * np.cos(ph + 0.7 * c)
+    return InitialState(sp, np.arange(len(sp), dtype=np.int32), x, p, periodic_fill_nodes(Eo), periodic_fill_cells(Bo))
+
+
+def species_arrays(w: Workload):
+    """(mass, charge, weight, ptype) arrays per species."""
+    return (np.array([s.mass for s in w.species], np.float64),
+            np.array([s.charge for s in w.species], np.float64),
+            np.array([w.weight(s) for s in w.species], np.float64),
+            np.array([s.ptype for s in w.species], np.int32))
