@@ -1,0 +1,171 @@
+/*
+ * picpp_amd — C ABI of the MI355X-native implicit PIC hot path.
+ *
+ * One context holds one periodic subdomain (one reference MPI rank at its real
+ * shape) resident in device memory. Each entry point replaces one Simulation
+ * method of the reference PIC++ code and is called in the reference's order
+ * (PIC++/simulation.cpp:205-390). All entry points return 0 on success and a
+ * negative PIC_ERR_* code on failure (the reference exits on errors instead:
+ * e.g. PIC++/particle.cpp:295-304); pic_last_error() describes the failure.
+ *
+ * Grid layout (as the reference's ghosted arrays, PIC++/initialization.cpp:6618-7222):
+ *   N = n + 4 cells per axis, nodes 0..N, cells 0..N-1, owned indices 2..n+1,
+ *   node i at x0 + i*dx with x0 = L - 2*dx, L = n*dx (PIC++/initialization.cpp:194).
+ *   Field arrays are component planes: a node field is [ncomp][(N0+1)(N1+1)(N2+1)]
+ *   with node id (i*(N1+1) + j)*(N2+1) + k; a cell field is [ncomp][N0*N1*N2].
+ * Units: c = 1, Gaussian (the reference's normalised units).
+ */
+#ifndef PICPP_AMD_H
+#define PICPP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pic_ctx pic_ctx;
+
+enum {
+	PIC_OK = 0,
+	PIC_ERR_ARGUMENT = -1,
+	PIC_ERR_DEVICE = -2,
+	PIC_ERR_MEMORY = -3,
+	PIC_ERR_PARTICLE = -4, /* particle outside the box or moved more than half a cell */
+	PIC_ERR_SOLVER = -5
+};
+
+enum { PIC_PRECISION_FP64 = 0, PIC_PRECISION_FP32 = 1 };
+
+/* GMRES stopping rules */
+enum {
+	/* the reference's rule: Givens residual estimate <= max(tol / (3 nx ny nz), 1e-15),
+	   at most 48 Arnoldi steps, no restart (PIC++/specialmath.cpp:427-481) */
+	PIC_STOP_REFERENCE = 0,
+	/* restarted GMRES until the true residual ||b - A x|| / ||b|| <= tol */
+	PIC_STOP_TRUE_RESIDUAL = 1
+};
+
+/* orthogonalisation inside the Arnoldi process */
+enum {
+	PIC_ORTHO_MGS = 0, /* modified Gram-Schmidt, one dot + axpy per basis vector (PIC++/specialmath.cpp:295-332) */
+	PIC_ORTHO_CGS2 = 1 /* classical Gram-Schmidt applied twice, batched over the basis */
+};
+
+typedef struct {
+	int nx, ny, nz;       /* physical cells per axis, each >= 5 */
+	double dx, dy, dz;    /* cell sizes */
+	double dt;            /* time step (deltaT) */
+	int nspecies;         /* <= 8 (PIC++ supports 8 types, initialization.cpp:596-606) */
+	double mass[8];       /* per species */
+	double charge[8];
+	double weight[8];     /* physical particles per macro-particle */
+	long long capacity;   /* maximum number of macro-particles */
+	int precision;        /* PIC_PRECISION_* of the particle mover and deposit */
+	int stop_rule;        /* PIC_STOP_* */
+	int ortho;            /* PIC_ORTHO_* */
+	double gmres_tolerance; /* maxErrorLevel (PIC++/constants.h:31), 1e-8 */
+	int gmres_max_iterations; /* maxGMRESIterations (PIC++/constants.h:55), 50 */
+} pic_config;
+
+typedef struct {
+	int iterations;           /* Arnoldi steps taken (all restarts) */
+	int restarts;
+	double estimated_residual; /* Givens estimate relative to ||b|| */
+	double true_residual;      /* ||b - A x|| / ||b|| when computed, else -1 */
+	double seconds;            /* wall time of the solve on the stream */
+} pic_solve_info;
+
+/* which arrays pic_download_array / pic_device_array address */
+enum {
+	PIC_ARRAY_E = 0,       /* Efield, nodes x 3 */
+	PIC_ARRAY_ETHETA = 1,  /* tempEfield E^{n+theta}, nodes x 3 */
+	PIC_ARRAY_ENEW = 2,    /* newEfield, nodes x 3 */
+	PIC_ARRAY_B = 3,       /* Bfield, cells x 3 */
+	PIC_ARRAY_BNEW = 4,    /* newBfield, cells x 3 */
+	PIC_ARRAY_J = 5,       /* electricFlux (J hat), nodes x 3 */
+	PIC_ARRAY_MU = 6,      /* dielectricTensor, nodes x 9 (row-major 3x3) */
+	PIC_ARRAY_DIVPI = 7,   /* divPressureTensor, nodes x 3 */
+	PIC_ARRAY_RHOHAT = 8,  /* chargeDensityHat, cells */
+	PIC_ARRAY_PI = 9,      /* pressureTensor, cells x 9 */
+	PIC_ARRAY_DENSITY = 10,/* updateDensityParameters: per species [concentration, energy, bulk v x3], then rho; cells x (5S+1) */
+	PIC_ARRAY_RHS = 11,    /* Maxwell right part, owned nodes x 3 (compact n0*n1*n2) */
+	PIC_ARRAY_SOLUTION = 12/* last GMRES solution, owned nodes x 3 */
+};
+
+/* Context lifetime. `stream` may be NULL (the library creates one). */
+int pic_create(const pic_config* cfg, void* stream, pic_ctx** out);
+void pic_destroy(pic_ctx* ctx);
+const char* pic_last_error(const pic_ctx* ctx);
+
+/* Host <-> device transfer (synchronous). Field arrays are ghosted component planes. */
+int pic_upload_fields(pic_ctx* ctx, const double* E, const double* B);
+/* Particles: species[np], ids[np], x[3][np], p[3][np] (component planes, absolute
+   coordinates in [L, 2L] per axis, momentum p = m gamma v). */
+int pic_upload_particles(pic_ctx* ctx, long long np, const int* species, const int* ids, const double* x, const double* p);
+long long pic_num_particles(const pic_ctx* ctx);
+/* Particles in the context's current (cell-sorted) order. */
+int pic_download_particles(pic_ctx* ctx, int* species, int* ids, double* x, double* p);
+/* Returns the element count of the array (out may be NULL to query it). */
+long long pic_download_array(pic_ctx* ctx, int which, double* out);
+/* Device pointer and element count of an array (no copy). */
+int pic_device_array(pic_ctx* ctx, int which, double** ptr, long long* count);
+
+/* ---- Simulation-level phases, in the reference's step order ---- */
+
+/* Pre-loop sequence (PIC++/simulation.cpp:86-123): ghost fill, particle sort,
+   updateDensityParameters, updateFields, updateEnergy. */
+int pic_initialize(pic_ctx* ctx);
+/* evaluateParticlesRotationTensor (PIC++/mover.cpp:810-841) +
+   updateElectroMagneticParameters (PIC++/momentum_collecting.cpp:365-758) */
+int pic_rotation_and_moments(pic_ctx* ctx);
+/* right part of createInternalEquation (PIC++/fields.cpp:2635-2715) */
+int pic_field_rhs(pic_ctx* ctx);
+/* evaluateElectricField (PIC++/fields.cpp:152-376): GMRES on the matrix-free operator
+   of createInternalEquation{X,Y,Z} (fields.cpp:886-2633), generalizedMinimalResidualMethod
+   (PIC++/specialmath.cpp:367-671); writes E^{n+theta} and newE. Calls pic_field_rhs. */
+int pic_maxwell_solve(pic_ctx* ctx, pic_solve_info* info);
+/* evaluateMagneticField (PIC++/fields.cpp:2717-2773) */
+int pic_update_b(pic_ctx* ctx);
+/* moveParticles (PIC++/mover.cpp:21-55, :194-491) + eraseEscapedPaticles/exchangeParticles
+   periodic wrap (mover.cpp:167-192, :1002-1210) + updateParticleCorrelationMaps +
+   updateDensityParameters (PIC++/momentum_collecting.cpp:760-1020) */
+int pic_push(pic_ctx* ctx);
+/* updateFields (PIC++/fields.cpp:422-439) */
+int pic_commit_fields(pic_ctx* ctx);
+/* updateEnergy (PIC++/simulation.cpp:1506-1640): out[7] = particle, E, B, total energy,
+   momentum x, y, z */
+int pic_energy(pic_ctx* ctx, double* out7);
+/* one full implicit step: all of the above in the reference's order */
+int pic_step(pic_ctx* ctx, pic_solve_info* info);
+
+/* The free-function solver boundary: generalizedMinimalResidualMethod
+   (PIC++/specialmath.h:14-17) on the context's current operator (dielectric tensor of
+   the last pic_rotation_and_moments). rhs and x are device pointers to owned-node
+   vectors [3][n0*n1*n2]; rhs is preserved. */
+int pic_gmres_solve(pic_ctx* ctx, const double* rhs, double* x, pic_solve_info* info);
+/* y = A v on device vectors [3][n0*n1*n2] (multiplySpecialMatrixVector, specialmath.cpp:202-252) */
+int pic_apply_operator(pic_ctx* ctx, const double* v, double* y);
+
+/* Blocks until the context's stream is idle. */
+int pic_synchronize(pic_ctx* ctx);
+
+/* Particles already in device memory, grouped by species (counts[s] particles of species
+   s, in species order): x[3][np], p[3][np] component planes, ids[np] or NULL. Copied and
+   sorted on the context's stream. Replaces the reference's particle construction
+   (PIC++/initialization.cpp:8145-8234) for inputs generated on the device. */
+int pic_upload_particles_device(pic_ctx* ctx, const long long* counts, const double* x, const double* p, const int* ids);
+
+/* Per-kernel timing with HIP events on the context's stream, the counterpart of the
+   reference's `timing` prints (PIC++/simulation.cpp:156-160, fields.cpp:189-205).
+   pic_profile_enable resets the statistics. pic_profile_read fills up to max_entries
+   rows: names (32 chars each), total milliseconds, launch counts and the algorithmic
+   bytes of those launches; it returns the number of rows. */
+int pic_profile_enable(pic_ctx* ctx, int on);
+int pic_profile_read(pic_ctx* ctx, int max_entries, char* names, double* total_ms, long long* counts, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PICPP_AMD_H */

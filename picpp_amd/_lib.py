@@ -1,0 +1,98 @@
+"""ctypes binding of libpicpp_amd.so (the C ABI declared in include/picpp_amd.h).
+
+The library is loaded from picpp_amd/lib/ (built in-tree by picpp_amd.build). There is
+no fallback: if the library is missing or fails to load, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpicpp_amd.so")
+
+PIC_OK = 0
+ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "particle", -5: "solver"}
+
+PRECISION_FP64, PRECISION_FP32 = 0, 1
+STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
+ORTHO_MGS, ORTHO_CGS2 = 0, 1
+
+ARRAYS = {"E": 0, "Etheta": 1, "Enew": 2, "B": 3, "Bnew": 4, "J": 5, "mu": 6, "divPi": 7, "rhohat": 8, "Pi": 9,
+          "density": 10, "rhs": 11, "solution": 12}
+
+# Every symbol the header declares, with (restype, argtypes).
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_V = ctypes.c_void_p
+
+
+class PicConfig(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("dt", ctypes.c_double), ("nspecies", ctypes.c_int),
+                ("mass", ctypes.c_double * 8), ("charge", ctypes.c_double * 8), ("weight", ctypes.c_double * 8),
+                ("capacity", ctypes.c_longlong), ("precision", ctypes.c_int), ("stop_rule", ctypes.c_int),
+                ("ortho", ctypes.c_int), ("gmres_tolerance", ctypes.c_double), ("gmres_max_iterations", ctypes.c_int)]
+
+
+class PicSolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("restarts", ctypes.c_int), ("estimated_residual", ctypes.c_double),
+                ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+SIGNATURES = {
+    "pic_create": (ctypes.c_int, [ctypes.POINTER(PicConfig), _V, ctypes.POINTER(_V)]),
+    "pic_destroy": (None, [_V]),
+    "pic_last_error": (ctypes.c_char_p, [_V]),
+    "pic_upload_fields": (ctypes.c_int, [_V, _D, _D]),
+    "pic_upload_particles": (ctypes.c_int, [_V, ctypes.c_longlong, _I, _I, _D, _D]),
+    "pic_num_particles": (ctypes.c_longlong, [_V]),
+    "pic_download_particles": (ctypes.c_int, [_V, _I, _I, _D, _D]),
+    "pic_download_array": (ctypes.c_longlong, [_V, ctypes.c_int, _D]),
+    "pic_device_array": (ctypes.c_int, [_V, ctypes.c_int, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong)]),
+    "pic_initialize": (ctypes.c_int, [_V]),
+    "pic_rotation_and_moments": (ctypes.c_int, [_V]),
+    "pic_field_rhs": (ctypes.c_int, [_V]),
+    "pic_maxwell_solve": (ctypes.c_int, [_V, ctypes.POINTER(PicSolveInfo)]),
+    "pic_update_b": (ctypes.c_int, [_V]),
+    "pic_push": (ctypes.c_int, [_V]),
+    "pic_commit_fields": (ctypes.c_int, [_V]),
+    "pic_energy": (ctypes.c_int, [_V, _D]),
+    "pic_step": (ctypes.c_int, [_V, ctypes.POINTER(PicSolveInfo)]),
+    "pic_gmres_solve": (ctypes.c_int, [_V, _D, _D, ctypes.POINTER(PicSolveInfo)]),
+    "pic_apply_operator": (ctypes.c_int, [_V, _D, _D]),
+    "pic_synchronize": (ctypes.c_int, [_V]),
+    "pic_upload_particles_device": (ctypes.c_int, [_V, ctypes.POINTER(ctypes.c_longlong), _D, _D, _I]),
+    "pic_profile_enable": (ctypes.c_int, [_V, ctypes.c_int]),
+    "pic_profile_read": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_char_p, _D, ctypes.POINTER(ctypes.c_longlong), _D]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libpicpp_amd.so is not built (run python -m picpp_amd.build): %s" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class PicError(RuntimeError):
+    pass
+
+
+def check(rc: int, ctx=None, what: str = ""):
+    if rc < 0:
+        msg = load().pic_last_error(ctx).decode() if ctx else ""
+        raise PicError("%s failed (%s): %s" % (what, ERRORS.get(rc, rc), msg))
+    return rc

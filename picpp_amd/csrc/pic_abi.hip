@@ -1,0 +1,920 @@
+// C ABI of the implicit PIC hot path: context, transfers and the Simulation-level phases.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/picpp_amd.h"
+#include "pic_device.hpp"
+#include "pic_kernels.hpp"
+
+using namespace picgpu;
+
+namespace {
+constexpr double kCprime2 = 1.025 * 1.025;  // speed_of_light_correction_sqr (PIC++/constants.h:46)
+
+struct DeviceError {
+	int code;
+	std::string what;
+};
+
+#define HIP_CHECK(expr)                                                                          \
+	do {                                                                                         \
+		hipError_t e_ = (expr);                                                                  \
+		if (e_ != hipSuccess) throw DeviceError{PIC_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+	} while (0)
+
+template <class T>
+T* dalloc(size_t n) {
+	void* p = nullptr;
+	if (n == 0) n = 1;
+	hipError_t e = hipMalloc(&p, n * sizeof(T));
+	if (e != hipSuccess) throw DeviceError{PIC_ERR_MEMORY, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed"};
+	return static_cast<T*>(p);
+}
+
+}  // namespace
+
+struct pic_ctx {
+	pic_config cfg;
+	Grid g;
+	SpeciesTable sp;
+	hipStream_t stream = nullptr;
+	bool own_stream = false;
+	std::string error;
+	double theta = 0.5, eta = 0.5;  // initialTheta, eta (PIC++/constants.h:41, initialization.cpp:625)
+
+	// ghosted fields (component planes)
+	double *E = nullptr, *Etheta = nullptr, *Enew = nullptr, *B = nullptr, *Bnew = nullptr;
+	double *J = nullptr, *mu = nullptr, *divPi = nullptr, *rhohat = nullptr, *Pi = nullptr, *dens = nullptr;
+	// scratch of the deterministic deposits
+	double *node_scratch = nullptr, *rho_scratch = nullptr, *pi_scratch = nullptr, *dens_scratch = nullptr;
+	// particles
+	ParticleArrays P{}, Palt{};
+	long long capacity = 0;
+	uint32_t *keys = nullptr, *keys_sorted = nullptr, *perm = nullptr, *perm_sorted = nullptr;
+	int *start = nullptr, *counts = nullptr;
+	long long nkeys = 0;
+	int end_bit = 0;
+	void* cub_temp = nullptr;
+	size_t cub_temp_bytes = 0;
+	std::vector<int> species_start_h;
+	int* species_start = nullptr;
+	int* status = nullptr;
+	// Krylov solver
+	double *rhs = nullptr, *muc = nullptr, *xsol = nullptr, *work = nullptr, *work2 = nullptr, *partial = nullptr, *dscal = nullptr, *ydev = nullptr;
+	std::vector<double*> V;
+	double** Vdev = nullptr;
+	int max_basis = 0;
+	// energies
+	double* epartial = nullptr;
+	double* eout = nullptr;
+	// per-kernel timing with events on the context's stream (the reference's `timing` prints)
+	struct Pending {
+		std::string name;
+		hipEvent_t a, b;
+		double bytes;
+	};
+	struct Stat {
+		double ms = 0, bytes = 0;
+		long long count = 0;
+	};
+	bool profiling = false;
+	std::vector<hipEvent_t> ev_pool;
+	size_t ev_next = 0;
+	std::vector<Pending> pending;
+	std::map<std::string, Stat> prof;
+
+	long long nown3() const { return 3 * g.nown; }
+};
+
+namespace {
+
+int fail(pic_ctx* c, const DeviceError& e) {
+	if (c) c->error = e.what;
+	return e.code;
+}
+
+void flush_profile(pic_ctx* c) {
+	if (c->pending.empty()) return;
+	HIP_CHECK(hipStreamSynchronize(c->stream));
+	for (auto& p : c->pending) {
+		float ms = 0;
+		HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+		auto& st = c->prof[p.name];
+		st.ms += ms;
+		st.bytes += p.bytes;
+		st.count += 1;
+	}
+	c->pending.clear();
+	c->ev_next = 0;
+}
+
+// Records events around the enclosed launches when profiling is on; `bytes` is the
+// algorithmic HBM traffic of one launch.
+struct Timed {
+	pic_ctx* c;
+	const char* name;
+	double bytes;
+	hipEvent_t a{}, b{};
+	bool on;
+	Timed(pic_ctx* c_, const char* n, double by) : c(c_), name(n), bytes(by), on(c_->profiling) {
+		if (!on) return;
+		if (c->ev_next + 2 > c->ev_pool.size()) flush_profile(c);
+		a = c->ev_pool[c->ev_next++];
+		b = c->ev_pool[c->ev_next++];
+		HIP_CHECK(hipEventRecord(a, c->stream));
+	}
+	~Timed() {
+		if (!on) return;
+		if (hipEventRecord(b, c->stream) == hipSuccess) c->pending.push_back({name, a, b, bytes});
+	}
+};
+
+void check_status(pic_ctx* c) {
+	int st = 0;
+	HIP_CHECK(hipMemcpyAsync(&st, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+	HIP_CHECK(hipStreamSynchronize(c->stream));
+	if (st) {
+		HIP_CHECK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+		std::string why;
+		if (st & 1) why += " particle moved more than one box length;";
+		if (st & 2) why += " particle maps inconsistent (moved more than half a cell?);";
+		if (st & 4) why += " particle outside the owned region;";
+		if (st & 8) why += " particle stencil outside the ghost layers;";
+		throw DeviceError{PIC_ERR_PARTICLE, "particle check failed:" + why};
+	}
+}
+
+// Sort particles by key (stable radix sort: deterministic), rebuild start[].
+void sort_particles(pic_ctx* c) {
+	long long n = c->P.count;
+	hipStream_t s = c->stream;
+	if (n > 0) {
+		{
+			Timed t(c, "sort_radix", (double)n * 16);
+			launch_iota(c->perm, n, s);
+			size_t bytes = c->cub_temp_bytes;
+			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_temp, bytes, c->keys, c->keys_sorted, c->perm, c->perm_sorted, (int)n,
+			                                             0, c->end_bit, s));
+		}
+		{
+			Timed t(c, "permute", (double)n * (2 * 52 + 4));
+			launch_permute(c->P, c->Palt, c->perm_sorted, s);
+		}
+		std::swap(c->P, c->Palt);
+	}
+	Timed t(c, "segments", (double)n * 4 + (double)(c->nkeys + 1) * 12);
+	HIP_CHECK(hipMemsetAsync(c->counts, 0, (c->nkeys + 1) * sizeof(int), s));
+	launch_count_keys(c->keys_sorted, n, c->counts, s);
+	size_t bytes = c->cub_temp_bytes;
+	HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start, (int)(c->nkeys + 1), s));
+}
+
+void density(pic_ctx* c) {
+	hipStream_t s = c->stream;
+	int nq = c->sp.count * 5 + 1;
+	double np = (double)c->P.count;
+	{
+		Timed t(c, "density", np * 48 + (double)c->g.ncells * nq * 8);
+		launch_density(c->g, c->sp, c->P, c->start, c->dens_scratch, s);
+	}
+	{
+		Timed t(c, "density_assemble", (double)c->g.ncells * nq * 8 * 9);
+		launch_density_assemble(c->g, c->dens_scratch, nq, c->dens, s);
+	}
+	{
+		Timed t(c, "folds", 0);
+		launch_fold_cells(c->g, c->dens, nq, 4, 3, s);  // sumCellParameters{X,Y,Z}, sumCellVectorParameters{X,Y,Z}
+		launch_bulk_divide(c->g, c->dens, c->sp.count, s);
+	}
+}
+
+void moments(pic_ctx* c) {
+	hipStream_t s = c->stream;
+	const Grid& g = c->g;
+	double np = (double)c->P.count;
+	{
+		Timed t(c, "node_moments", np * 48 + (double)g.nnodes * 12 * 8);
+		launch_node_moments(g, c->sp, c->P, c->start, c->E, c->B, c->cfg.dt, c->theta, c->node_scratch, s);
+	}
+	{
+		Timed t(c, "node_assemble", (double)g.ncells * 96 * 8 + (double)g.nnodes * 12 * 8);
+		launch_node_assemble(g, c->node_scratch, c->J, c->mu, s);
+	}
+	{
+		Timed t(c, "cell_moments", np * 48 + (double)g.ncells * 10 * 8);
+		launch_cell_moments(g, c->sp, c->P, c->start, c->E, c->B, c->cfg.dt, c->rho_scratch, c->pi_scratch, s);
+	}
+	{
+		Timed t(c, "cell_assemble", (double)g.ncells * 9 * 8);
+		launch_cell_assemble(g, c->rho_scratch, c->rhohat, s);
+	}
+	{
+		Timed t(c, "pressure_reduce", (double)g.ncells * 48 * 8);
+		HIP_CHECK(hipMemsetAsync(c->Pi, 0, 9 * g.ncells * sizeof(double), s));
+		launch_pressure_reduce(g, c->pi_scratch, c->Pi, s);
+	}
+	Timed tf(c, "folds", 0);
+	launch_fold_cells(g, c->rhohat, 1, 4, 3, s);  // sumChargeDensityHat{X,Y,Z}
+	launch_fold_cells(g, c->Pi, 9, 3, 2, s);      // sumCellMatrixParameters{X,Y,Z}
+	launch_fold_nodes(g, c->J, 3, s);             // sumNodeVectorParameters{X,Y,Z}
+	launch_div_pressure_update_j(g, c->Pi, c->J, c->divPi, c->eta, c->cfg.dt, s);
+	launch_div_flux_update_rho(g, c->J, c->rhohat, c->cfg.dt, c->theta, s);
+	launch_fold_nodes(g, c->mu, 9, s);            // sumNodeMatrixParameters{X,Y,Z}
+}
+
+double operator_C(const pic_ctx* c) { return (c->theta * c->cfg.dt) * (c->theta * c->cfg.dt) * kCprime2; }
+
+// Givens-updated least squares of the reference GMRES (PIC++/specialmath.cpp:487-600).
+struct Givens {
+	std::vector<std::vector<double>> H, Q, R, oldQ, oldR;
+	std::vector<double> y;
+	double rel = 1.0;
+	void update(int nn, const std::vector<double>& hcol) {
+		// hcol: column nn-2 of the Hessenberg matrix, entries 0..nn-1
+		std::vector<std::vector<double>> Hn(nn, std::vector<double>(nn - 1, 0.0));
+		for (int r = 0; r < nn - 1; ++r)
+			for (int cc = 0; cc < nn - 2; ++cc) Hn[r][cc] = H[r][cc];
+		for (int r = 0; r < nn; ++r) Hn[r][nn - 2] = hcol[r];
+		H = Hn;
+		if (nn == 2) {
+			double rho = H[0][0], sigma = H[1][0];
+			double module = std::sqrt(rho * rho + sigma * sigma);
+			double cs = rho / module, sn = sigma / module;
+			Q = {{cs, sn}, {-sn, cs}};
+			R = {{module}, {0}};
+		} else {
+			R.assign(nn, std::vector<double>(nn - 1, 0.0));
+			for (int r = 0; r < nn - 1; ++r)
+				for (int cc = 0; cc < nn - 2; ++cc) R[r][cc] = oldR[r][cc];
+			Q.assign(nn, std::vector<double>(nn, 0.0));
+			for (int r = 0; r < nn - 1; ++r)
+				for (int cc = 0; cc < nn - 1; ++cc) Q[r][cc] = oldQ[r][cc];
+			Q[nn - 1][nn - 1] = 1;
+			for (int r = 0; r < nn; ++r) {
+				R[r][nn - 2] = 0;
+				for (int cc = 0; cc < nn; ++cc) R[r][nn - 2] += Q[r][cc] * H[cc][nn - 2];
+			}
+			double rho = R[nn - 2][nn - 2], sigma = R[nn - 1][nn - 2];
+			double module = std::sqrt(rho * rho + sigma * sigma);
+			double cs = rho / module, sn = sigma / module;
+			R[nn - 2][nn - 2] = module;
+			R[nn - 1][nn - 2] = 0;
+			for (int cc = 0; cc < nn - 1; ++cc) {
+				Q[nn - 2][cc] = cs * oldQ[nn - 2][cc];
+				Q[nn - 1][cc] = -sn * oldQ[nn - 2][cc];
+			}
+			Q[nn - 2][nn - 1] = sn;
+			Q[nn - 1][nn - 1] = cs;
+		}
+		double beta = 1.0;
+		y.assign(nn - 1, 0.0);
+		for (int i = nn - 2; i >= 0; --i) {
+			y[i] = beta * Q[i][0];
+			for (int j = nn - 2; j > i; --j) y[i] -= R[i][j] * y[j];
+			if (R[i][i] > 0)
+				y[i] /= R[i][i];
+			else
+				y[i] = 0;
+		}
+		oldQ = Q;
+		oldR = R;
+	}
+};
+
+// One unrestarted GMRES cycle on A x = b (x0 = 0), as generalizedMinimalResidualMethod:
+// b is normalised in place and restored. Returns Arnoldi steps; rel = Givens estimate.
+int gmres_cycle(pic_ctx* c, double* b, double* x, double target, int max_iterations, int dim, double& rel) {
+	hipStream_t s = c->stream;
+	long long len = c->nown3();
+	double* d_norm = c->dscal + 0;
+	double* d_normr = c->dscal + 1;
+	double* d_h = c->dscal + 8;
+	launch_dot(b, b, len, c->partial, d_norm, true, s);
+	double norm = 0;
+	HIP_CHECK(hipMemcpyAsync(&norm, d_norm, sizeof(double), hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	rel = 0;
+	if (norm == 0) {
+		HIP_CHECK(hipMemsetAsync(x, 0, len * sizeof(double), s));
+		return 0;
+	}
+	launch_div_copy(b, c->V[0], d_norm, len, s);
+	launch_dot(b, b, len, c->partial, d_normr, true, s);
+	double normr = 0;
+	HIP_CHECK(hipMemcpyAsync(&normr, d_normr, sizeof(double), hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	const double C = operator_C(c);
+	Givens gv;
+	int nn = 2;
+	rel = 1.0;
+	int limit = std::min(std::min(max_iterations, dim + 3), c->max_basis + 1);
+	std::vector<double> hcol;
+	while (rel > target && nn < limit) {
+		int j = nn - 2;
+		double* w = c->V[j + 1];
+		{
+			Timed t(c, "apply_operator", (double)c->g.nown * 15 * 8);
+			launch_apply_operator(c->g, c->muc, c->V[j], C, w, s);
+		}
+		{
+			// algorithmic bytes: MGS reads v_m and w twice and writes w once per basis vector
+			Timed tg(c, "orthogonalize", (double)len * 8 * (c->cfg.ortho == PIC_ORTHO_MGS ? 5.0 * (j + 1) + 3 : 6.0 * (j + 1) + 3));
+			if (c->cfg.ortho == PIC_ORTHO_MGS) {
+				for (int m = 0; m <= j; ++m) {
+					launch_dot(c->V[m], w, len, c->partial, d_h + m, false, s);
+					launch_axpy_dev(w, c->V[m], d_h + m, len, s);
+				}
+			} else {
+				// CGS2: two passes of h = V^T w, w -= V h (same span; reorthogonalised)
+				for (int pass = 0; pass < 2; ++pass) {
+					double* hp = d_h + (pass == 0 ? 0 : 64);
+					for (int m = 0; m <= j; ++m)
+						launch_dot(c->V[m], w, len, c->partial + (m + 1) * kDotBlocks, hp + m, false, s);
+					for (int m = 0; m <= j; ++m) launch_axpy_dev(w, c->V[m], hp + m, len, s);
+				}
+			}
+			launch_dot(w, w, len, c->partial, d_h + j + 1, true, s);
+			launch_normalize_dev(w, d_h + j + 1, len, s);
+		}
+		hcol.assign(nn, 0.0);
+		if (c->cfg.ortho == PIC_ORTHO_MGS) {
+			HIP_CHECK(hipMemcpyAsync(hcol.data(), d_h, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+		} else {
+			std::vector<double> h1(nn), h2(nn);
+			HIP_CHECK(hipMemcpyAsync(h1.data(), d_h, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipMemcpyAsync(h2.data(), d_h + 64, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+			for (int m = 0; m <= j; ++m) hcol[m] = h1[m] + h2[m];
+			hcol[j + 1] = h1[j + 1];
+		}
+		gv.update(nn, hcol);
+		rel = std::fabs(1.0 * gv.Q[nn - 1][0]) / normr;
+		nn++;
+	}
+	int steps = nn - 2;
+	HIP_CHECK(hipMemcpyAsync(c->ydev, gv.y.data(), gv.y.size() * sizeof(double), hipMemcpyHostToDevice, s));
+	Timed t(c, "combine", (double)len * 8 * (gv.y.size() + 1));
+	launch_combine(x, c->Vdev, c->ydev, (int)gv.y.size(), d_norm, len, s);
+	return steps;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pic_last_error(const pic_ctx* c) { return c ? c->error.c_str() : "null context"; }
+
+int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
+	if (!cfg || !out) return PIC_ERR_ARGUMENT;
+	*out = nullptr;
+	if (cfg->nx < 5 || cfg->ny < 5 || cfg->nz < 5 || cfg->nspecies < 1 || cfg->nspecies > kMaxSpecies || cfg->dt <= 0 ||
+	    cfg->dx <= 0 || cfg->dy <= 0 || cfg->dz <= 0 || cfg->capacity < 0)
+		return PIC_ERR_ARGUMENT;
+	if (cfg->precision != PIC_PRECISION_FP64) return PIC_ERR_ARGUMENT;
+	pic_ctx* c = new pic_ctx();
+	try {
+		c->cfg = *cfg;
+		if (c->cfg.gmres_max_iterations <= 0) c->cfg.gmres_max_iterations = 50;
+		if (c->cfg.gmres_tolerance <= 0) c->cfg.gmres_tolerance = 1e-8;
+		Grid& g = c->g;
+		int n[3] = {cfg->nx, cfg->ny, cfg->nz};
+		double d[3] = {cfg->dx, cfg->dy, cfg->dz};
+		for (int a = 0; a < 3; ++a) {
+			g.n[a] = n[a];
+			g.N[a] = n[a] + 4;
+			g.d[a] = d[a];
+			g.L[a] = n[a] * d[a];
+			g.x0[a] = g.L[a] + (-2) * d[a];
+		}
+		g.volume = d[0] * d[1] * d[2];
+		g.nnodes = (long long)(g.N[0] + 1) * (g.N[1] + 1) * (g.N[2] + 1);
+		g.ncells = (long long)g.N[0] * g.N[1] * g.N[2];
+		g.nown = (long long)n[0] * n[1] * n[2];
+		c->sp.count = cfg->nspecies;
+		for (int s = 0; s < kMaxSpecies; ++s) {
+			c->sp.mass[s] = s < cfg->nspecies ? cfg->mass[s] : 1.0;
+			c->sp.charge[s] = s < cfg->nspecies ? cfg->charge[s] : 0.0;
+			c->sp.weight[s] = s < cfg->nspecies ? cfg->weight[s] : 0.0;
+		}
+		if (stream) {
+			c->stream = static_cast<hipStream_t>(stream);
+		} else {
+			HIP_CHECK(hipStreamCreate(&c->stream));
+			c->own_stream = true;
+		}
+		c->nkeys = (long long)cfg->nspecies * g.ncells * 8;
+		if (c->nkeys >= (1LL << 32) - 1) throw DeviceError{PIC_ERR_ARGUMENT, "grid too large for 32-bit particle keys"};
+		c->end_bit = 1;
+		while ((1LL << c->end_bit) < c->nkeys) ++c->end_bit;
+		c->E = dalloc<double>(3 * g.nnodes);
+		c->Etheta = dalloc<double>(3 * g.nnodes);
+		c->Enew = dalloc<double>(3 * g.nnodes);
+		c->B = dalloc<double>(3 * g.ncells);
+		c->Bnew = dalloc<double>(3 * g.ncells);
+		c->J = dalloc<double>(3 * g.nnodes);
+		c->mu = dalloc<double>(9 * g.nnodes);
+		c->divPi = dalloc<double>(3 * g.nnodes);
+		c->rhohat = dalloc<double>(g.ncells);
+		c->Pi = dalloc<double>(9 * g.ncells);
+		c->dens = dalloc<double>((5 * cfg->nspecies + 1) * g.ncells);
+		c->node_scratch = dalloc<double>(96 * g.ncells);
+		c->rho_scratch = dalloc<double>(8 * g.ncells);
+		c->pi_scratch = dalloc<double>(48 * g.ncells);
+		c->dens_scratch = dalloc<double>((5 * cfg->nspecies + 1) * 8 * g.ncells);
+		for (double* p : {c->E, c->Etheta, c->Enew, c->J, c->divPi}) HIP_CHECK(hipMemset(p, 0, 3 * g.nnodes * sizeof(double)));
+		HIP_CHECK(hipMemset(c->mu, 0, 9 * g.nnodes * sizeof(double)));
+		for (double* p : {c->B, c->Bnew}) HIP_CHECK(hipMemset(p, 0, 3 * g.ncells * sizeof(double)));
+		HIP_CHECK(hipMemset(c->rhohat, 0, g.ncells * sizeof(double)));
+		HIP_CHECK(hipMemset(c->Pi, 0, 9 * g.ncells * sizeof(double)));
+		HIP_CHECK(hipMemset(c->dens, 0, (5 * cfg->nspecies + 1) * g.ncells * sizeof(double)));
+		c->capacity = cfg->capacity;
+		for (ParticleArrays* P : {&c->P, &c->Palt}) {
+			for (int a = 0; a < 3; ++a) {
+				P->x[a] = dalloc<double>(c->capacity);
+				P->p[a] = dalloc<double>(c->capacity);
+			}
+			P->id = dalloc<int>(c->capacity);
+			P->count = 0;
+		}
+		c->keys = dalloc<uint32_t>(c->capacity);
+		c->keys_sorted = dalloc<uint32_t>(c->capacity);
+		c->perm = dalloc<uint32_t>(c->capacity);
+		c->perm_sorted = dalloc<uint32_t>(c->capacity);
+		c->start = dalloc<int>(c->nkeys + 1);
+		c->counts = dalloc<int>(c->nkeys + 1);
+		HIP_CHECK(hipMemset(c->start, 0, (c->nkeys + 1) * sizeof(int)));
+		size_t b1 = 0, b2 = 0;
+		HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c->keys, c->keys_sorted, c->perm, c->perm_sorted,
+		                                             (int)std::max<long long>(c->capacity, 1), 0, c->end_bit, c->stream));
+		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->counts, c->start, (int)(c->nkeys + 1), c->stream));
+		c->cub_temp_bytes = std::max(b1, b2);
+		c->cub_temp = dalloc<char>(c->cub_temp_bytes);
+		c->species_start = dalloc<int>(kMaxSpecies + 1);
+		c->species_start_h.assign(kMaxSpecies + 1, 0);
+		HIP_CHECK(hipMemset(c->species_start, 0, (kMaxSpecies + 1) * sizeof(int)));
+		c->status = dalloc<int>(1);
+		HIP_CHECK(hipMemset(c->status, 0, sizeof(int)));
+		// Krylov storage: up to max_iterations basis vectors
+		c->max_basis = std::min(120, std::max(2, c->cfg.gmres_max_iterations));
+		long long len = 3 * g.nown;
+		c->rhs = dalloc<double>(len);
+		c->muc = dalloc<double>(9 * g.nown);
+		c->xsol = dalloc<double>(len);
+		c->work = dalloc<double>(len);
+		c->work2 = dalloc<double>(len);
+		HIP_CHECK(hipMemset(c->xsol, 0, len * sizeof(double)));
+		c->V.resize(c->max_basis + 1);
+		for (auto& v : c->V) v = dalloc<double>(len);
+		c->Vdev = dalloc<double*>(c->V.size());
+		HIP_CHECK(hipMemcpy(c->Vdev, c->V.data(), c->V.size() * sizeof(double*), hipMemcpyHostToDevice));
+		c->partial = dalloc<double>((size_t)kDotBlocks * 128);
+		c->dscal = dalloc<double>(256);
+		c->ydev = dalloc<double>(128);
+		c->epartial = dalloc<double>(9 * kEnergyBlocks);
+		c->eout = dalloc<double>(16);
+		HIP_CHECK(hipDeviceSynchronize());
+	} catch (const DeviceError& e) {
+		std::string msg = e.what;
+		int code = e.code;
+		pic_destroy(c);
+		(void)msg;
+		return code;
+	}
+	*out = c;
+	return PIC_OK;
+}
+
+void pic_destroy(pic_ctx* c) {
+	if (!c) return;
+	hipDeviceSynchronize();
+	auto f = [](void* p) {
+		if (p) hipFree(p);
+	};
+	for (void* p : {(void*)c->E, (void*)c->Etheta, (void*)c->Enew, (void*)c->B, (void*)c->Bnew, (void*)c->J, (void*)c->mu,
+	                (void*)c->divPi, (void*)c->rhohat, (void*)c->Pi, (void*)c->dens, (void*)c->node_scratch,
+	                (void*)c->rho_scratch, (void*)c->pi_scratch, (void*)c->dens_scratch, (void*)c->keys, (void*)c->keys_sorted,
+	                (void*)c->perm, (void*)c->perm_sorted, (void*)c->start, (void*)c->counts, c->cub_temp,
+	                (void*)c->species_start, (void*)c->status, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
+	                (void*)c->Vdev, (void*)c->partial, (void*)c->dscal, (void*)c->ydev, (void*)c->epartial, (void*)c->eout})
+		f(p);
+	for (ParticleArrays* P : {&c->P, &c->Palt}) {
+		for (int a = 0; a < 3; ++a) {
+			f(P->x[a]);
+			f(P->p[a]);
+		}
+		f(P->id);
+	}
+	for (double* v : c->V) f(v);
+	for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+	if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+	delete c;
+}
+
+int pic_upload_fields(pic_ctx* c, const double* E, const double* B) {
+	if (!c || !E || !B) return PIC_ERR_ARGUMENT;
+	try {
+		const Grid& g = c->g;
+		HIP_CHECK(hipMemcpyAsync(c->E, E, 3 * g.nnodes * sizeof(double), hipMemcpyHostToDevice, c->stream));
+		HIP_CHECK(hipMemcpyAsync(c->Etheta, E, 3 * g.nnodes * sizeof(double), hipMemcpyHostToDevice, c->stream));
+		HIP_CHECK(hipMemcpyAsync(c->Enew, E, 3 * g.nnodes * sizeof(double), hipMemcpyHostToDevice, c->stream));
+		HIP_CHECK(hipMemcpyAsync(c->B, B, 3 * g.ncells * sizeof(double), hipMemcpyHostToDevice, c->stream));
+		HIP_CHECK(hipMemcpyAsync(c->Bnew, B, 3 * g.ncells * sizeof(double), hipMemcpyHostToDevice, c->stream));
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_upload_particles(pic_ctx* c, long long np, const int* species, const int* ids, const double* x, const double* p) {
+	if (!c || np < 0 || np > c->capacity || (np > 0 && (!species || !x || !p))) {
+		if (c) c->error = "pic_upload_particles: bad arguments or capacity exceeded";
+		return PIC_ERR_ARGUMENT;
+	}
+	try {
+		// group by species on the host so species ranges are contiguous before the key sort
+		std::vector<long long> order(np);
+		std::vector<int> cnt(c->sp.count + 1, 0);
+		for (long long i = 0; i < np; ++i) {
+			if (species[i] < 0 || species[i] >= c->sp.count) throw DeviceError{PIC_ERR_ARGUMENT, "species index out of range"};
+			cnt[species[i] + 1]++;
+		}
+		for (int s = 0; s < c->sp.count; ++s) cnt[s + 1] += cnt[s];
+		std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+		for (long long i = 0; i < np; ++i) order[pos[species[i]]++] = i;
+		std::vector<double> buf(np);
+		std::vector<int> ib(np);
+		for (int a = 0; a < 3; ++a) {
+			for (long long i = 0; i < np; ++i) buf[i] = x[a * np + order[i]];
+			HIP_CHECK(hipMemcpy(c->P.x[a], buf.data(), np * sizeof(double), hipMemcpyHostToDevice));
+			for (long long i = 0; i < np; ++i) buf[i] = p[a * np + order[i]];
+			HIP_CHECK(hipMemcpy(c->P.p[a], buf.data(), np * sizeof(double), hipMemcpyHostToDevice));
+		}
+		for (long long i = 0; i < np; ++i) ib[i] = ids ? ids[order[i]] : (int)order[i];
+		HIP_CHECK(hipMemcpy(c->P.id, ib.data(), np * sizeof(int), hipMemcpyHostToDevice));
+		c->P.count = np;
+		c->Palt.count = np;
+		c->species_start_h.assign(kMaxSpecies + 1, (int)np);
+		for (int s = 0; s <= c->sp.count; ++s) c->species_start_h[s] = cnt[s];
+		HIP_CHECK(hipMemcpy(c->species_start, c->species_start_h.data(), (kMaxSpecies + 1) * sizeof(int), hipMemcpyHostToDevice));
+		launch_keys(c->g, c->sp, c->P, c->species_start, c->keys, c->status, c->stream);
+		check_status(c);
+		sort_particles(c);
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_upload_particles_device(pic_ctx* c, const long long* counts, const double* x, const double* p, const int* ids) {
+	if (!c || !counts || !x || !p) return PIC_ERR_ARGUMENT;
+	try {
+		long long np = 0;
+		std::vector<int> cnt(c->sp.count + 1, 0);
+		for (int s = 0; s < c->sp.count; ++s) {
+			if (counts[s] < 0) throw DeviceError{PIC_ERR_ARGUMENT, "negative particle count"};
+			np += counts[s];
+			cnt[s + 1] = (int)np;
+		}
+		if (np > c->capacity) throw DeviceError{PIC_ERR_ARGUMENT, "particle capacity exceeded"};
+		hipStream_t s = c->stream;
+		for (int a = 0; a < 3; ++a) {
+			HIP_CHECK(hipMemcpyAsync(c->P.x[a], x + a * np, np * sizeof(double), hipMemcpyDeviceToDevice, s));
+			HIP_CHECK(hipMemcpyAsync(c->P.p[a], p + a * np, np * sizeof(double), hipMemcpyDeviceToDevice, s));
+		}
+		if (ids) {
+			HIP_CHECK(hipMemcpyAsync(c->P.id, ids, np * sizeof(int), hipMemcpyDeviceToDevice, s));
+		} else {
+			launch_iota(reinterpret_cast<uint32_t*>(c->P.id), np, s);
+		}
+		c->P.count = np;
+		c->Palt.count = np;
+		c->species_start_h.assign(kMaxSpecies + 1, (int)np);
+		for (int q = 0; q <= c->sp.count; ++q) c->species_start_h[q] = cnt[q];
+		HIP_CHECK(hipMemcpyAsync(c->species_start, c->species_start_h.data(), (kMaxSpecies + 1) * sizeof(int),
+		                         hipMemcpyHostToDevice, s));
+		launch_keys(c->g, c->sp, c->P, c->species_start, c->keys, c->status, s);
+		check_status(c);
+		sort_particles(c);
+		HIP_CHECK(hipStreamSynchronize(s));
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+long long pic_num_particles(const pic_ctx* c) { return c ? c->P.count : -1; }
+
+int pic_download_particles(pic_ctx* c, int* species, int* ids, double* x, double* p) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		long long n = c->P.count;
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+		for (int a = 0; a < 3; ++a) {
+			if (x) HIP_CHECK(hipMemcpy(x + a * n, c->P.x[a], n * sizeof(double), hipMemcpyDeviceToHost));
+			if (p) HIP_CHECK(hipMemcpy(p + a * n, c->P.p[a], n * sizeof(double), hipMemcpyDeviceToHost));
+		}
+		if (ids) HIP_CHECK(hipMemcpy(ids, c->P.id, n * sizeof(int), hipMemcpyDeviceToHost));
+		if (species)
+			for (int s = 0; s < c->sp.count; ++s)
+				for (long long i = c->species_start_h[s]; i < c->species_start_h[s + 1]; ++i) species[i] = s;
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_device_array(pic_ctx* c, int which, double** ptr, long long* count) {
+	if (!c || !ptr || !count) return PIC_ERR_ARGUMENT;
+	const Grid& g = c->g;
+	switch (which) {
+	case PIC_ARRAY_E: *ptr = c->E; *count = 3 * g.nnodes; break;
+	case PIC_ARRAY_ETHETA: *ptr = c->Etheta; *count = 3 * g.nnodes; break;
+	case PIC_ARRAY_ENEW: *ptr = c->Enew; *count = 3 * g.nnodes; break;
+	case PIC_ARRAY_B: *ptr = c->B; *count = 3 * g.ncells; break;
+	case PIC_ARRAY_BNEW: *ptr = c->Bnew; *count = 3 * g.ncells; break;
+	case PIC_ARRAY_J: *ptr = c->J; *count = 3 * g.nnodes; break;
+	case PIC_ARRAY_MU: *ptr = c->mu; *count = 9 * g.nnodes; break;
+	case PIC_ARRAY_DIVPI: *ptr = c->divPi; *count = 3 * g.nnodes; break;
+	case PIC_ARRAY_RHOHAT: *ptr = c->rhohat; *count = g.ncells; break;
+	case PIC_ARRAY_PI: *ptr = c->Pi; *count = 9 * g.ncells; break;
+	case PIC_ARRAY_DENSITY: *ptr = c->dens; *count = (5 * c->sp.count + 1) * g.ncells; break;
+	case PIC_ARRAY_RHS: *ptr = c->rhs; *count = 3 * g.nown; break;
+	case PIC_ARRAY_SOLUTION: *ptr = c->xsol; *count = 3 * g.nown; break;
+	default: c->error = "unknown array id"; return PIC_ERR_ARGUMENT;
+	}
+	return PIC_OK;
+}
+
+long long pic_download_array(pic_ctx* c, int which, double* out) {
+	double* ptr = nullptr;
+	long long n = 0;
+	int rc = pic_device_array(c, which, &ptr, &n);
+	if (rc) return rc;
+	if (!out) return n;
+	try {
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+		HIP_CHECK(hipMemcpy(out, ptr, n * sizeof(double), hipMemcpyDeviceToHost));
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return n;
+}
+
+int pic_initialize(pic_ctx* c) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		const Grid& g = c->g;
+		hipStream_t s = c->stream;
+		launch_ghost_fill(g, c->E, 3, true, s);
+		launch_ghost_fill(g, c->Etheta, 3, true, s);
+		launch_ghost_fill(g, c->Enew, 3, true, s);
+		launch_ghost_fill(g, c->B, 3, false, s);
+		launch_ghost_fill(g, c->Bnew, 3, false, s);
+		density(c);
+		HIP_CHECK(hipMemcpyAsync(c->E, c->Enew, 3 * g.nnodes * sizeof(double), hipMemcpyDeviceToDevice, s));
+		HIP_CHECK(hipMemcpyAsync(c->B, c->Bnew, 3 * g.ncells * sizeof(double), hipMemcpyDeviceToDevice, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_rotation_and_moments(pic_ctx* c) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		moments(c);
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_field_rhs(pic_ctx* c) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		Timed t(c, "rhs", (double)c->g.nown * (3 + 3 + 3 + 1 + 3 + 9 + 9) * 8);
+		launch_rhs(c->g, c->E, c->B, c->J, c->rhohat, c->theta, c->cfg.dt, c->rhs, c->stream);
+		launch_pack_mu(c->g, c->mu, c->muc, c->stream);
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+static int solve_impl(pic_ctx* c, double* b, double* x, pic_solve_info* info) {
+	hipStream_t s = c->stream;
+	auto t0 = std::chrono::steady_clock::now();
+	int dim = 3 * c->g.n[0] * c->g.n[1] * c->g.n[2];
+	double rel = 0;
+	pic_solve_info inf{0, 0, 0, -1, 0};
+	long long len = c->nown3();
+	if (c->cfg.stop_rule == PIC_STOP_REFERENCE) {
+		// generalizedMinimalResidualMethod: b is normalised in place there; a copy keeps rhs intact here
+		HIP_CHECK(hipMemcpyAsync(c->work, b, len * sizeof(double), hipMemcpyDeviceToDevice, s));
+		double target = std::max(c->cfg.gmres_tolerance / dim, 1e-15);
+		inf.iterations = gmres_cycle(c, c->work, x, target, c->cfg.gmres_max_iterations, dim, rel);
+		inf.estimated_residual = rel;
+	} else {
+		// restarted GMRES on the residual until ||b - A x|| / ||b|| <= tol
+		const double C = operator_C(c);
+		HIP_CHECK(hipMemsetAsync(x, 0, len * sizeof(double), s));
+		launch_dot(b, b, len, c->partial, c->dscal + 4, true, s);
+		double bnorm = 0;
+		HIP_CHECK(hipMemcpyAsync(&bnorm, c->dscal + 4, sizeof(double), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		double tr = 0, rnorm = bnorm;
+		HIP_CHECK(hipMemcpyAsync(c->work, b, len * sizeof(double), hipMemcpyDeviceToDevice, s));
+		while (bnorm > 0) {
+			tr = rnorm / bnorm;
+			if (tr <= c->cfg.gmres_tolerance || inf.restarts >= 50) break;
+			double target = std::max(0.5 * c->cfg.gmres_tolerance * bnorm / rnorm, 1e-15);
+			int it = gmres_cycle(c, c->work, c->work2, target, c->cfg.gmres_max_iterations, dim, rel);
+			inf.iterations += it;
+			inf.restarts += 1;
+			launch_add(x, c->work2, len, s);
+			launch_apply_operator(c->g, c->muc, x, C, c->work2, s);
+			launch_sub(c->work, b, c->work2, len, s);  // r = b - A x
+			launch_dot(c->work, c->work, len, c->partial, c->dscal + 5, true, s);
+			HIP_CHECK(hipMemcpyAsync(&rnorm, c->dscal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+			if (it == 0) break;
+		}
+		inf.true_residual = bnorm > 0 ? rnorm / bnorm : 0.0;
+		inf.estimated_residual = rel;
+		if (inf.true_residual > c->cfg.gmres_tolerance)
+			throw DeviceError{PIC_ERR_SOLVER, "GMRES did not reach the requested true residual"};
+	}
+	HIP_CHECK(hipStreamSynchronize(s));
+	inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	if (info) *info = inf;
+	return PIC_OK;
+}
+
+int pic_maxwell_solve(pic_ctx* c, pic_solve_info* info) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		int rc = pic_field_rhs(c);
+		if (rc) return rc;
+		rc = solve_impl(c, c->rhs, c->xsol, info);
+		if (rc) return rc;
+		Timed t(c, "unpack_e", (double)c->g.nnodes * 9 * 8);
+		launch_unpack_e(c->g, c->xsol, c->E, c->theta, c->Etheta, c->Enew, c->stream);
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_gmres_solve(pic_ctx* c, const double* rhs, double* x, pic_solve_info* info) {
+	if (!c || !rhs || !x) return PIC_ERR_ARGUMENT;
+	try {
+		return solve_impl(c, const_cast<double*>(rhs), x, info);
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+}
+
+int pic_apply_operator(pic_ctx* c, const double* v, double* y) {
+	if (!c || !v || !y) return PIC_ERR_ARGUMENT;
+	try {
+		launch_pack_mu(c->g, c->mu, c->muc, c->stream);
+		launch_apply_operator(c->g, c->muc, v, operator_C(c), y, c->stream);
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_update_b(pic_ctx* c) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		Timed t(c, "update_b", (double)c->g.ncells * 6 * 8 + (double)c->g.nnodes * 3 * 8);
+		launch_update_b(c->g, c->Etheta, c->B, c->cfg.dt, c->Bnew, c->stream);
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_push(pic_ctx* c) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		{
+			Timed t(c, "push", (double)c->P.count * 100 + ((double)c->g.nnodes * 6 + (double)c->g.ncells * 3) * 8);
+			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->status,
+			            c->stream);
+		}
+		check_status(c);
+		sort_particles(c);
+		density(c);
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_commit_fields(pic_ctx* c) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		const Grid& g = c->g;
+		HIP_CHECK(hipMemcpyAsync(c->E, c->Enew, 3 * g.nnodes * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+		HIP_CHECK(hipMemcpyAsync(c->B, c->Bnew, 3 * g.ncells * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_energy(pic_ctx* c, double* out7) {
+	if (!c || !out7) return PIC_ERR_ARGUMENT;
+	try {
+		hipStream_t s = c->stream;
+		launch_field_energy(c->g, c->E, c->B, c->epartial, s);
+		launch_particle_energy(c->sp, c->P, c->species_start, c->epartial + 5 * kEnergyBlocks, s);
+		launch_rows_final(c->epartial, kEnergyBlocks, 9, c->eout, s);
+		double h[9];
+		HIP_CHECK(hipMemcpyAsync(h, c->eout, 9 * sizeof(double), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		out7[0] = h[5];
+		out7[1] = h[0];
+		out7[2] = h[1];
+		out7[3] = h[5] + h[0] + h[1];
+		out7[4] = h[2] + h[6];
+		out7[5] = h[3] + h[7];
+		out7[6] = h[4] + h[8];
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_step(pic_ctx* c, pic_solve_info* info) {
+	int rc;
+	if ((rc = pic_rotation_and_moments(c))) return rc;
+	if ((rc = pic_maxwell_solve(c, info))) return rc;
+	if ((rc = pic_update_b(c))) return rc;
+	if ((rc = pic_push(c))) return rc;
+	return pic_commit_fields(c);
+}
+
+int pic_profile_enable(pic_ctx* c, int on) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		flush_profile(c);
+		c->prof.clear();
+		if (on && c->ev_pool.empty()) {
+			c->ev_pool.resize(4096);
+			for (auto& e : c->ev_pool) HIP_CHECK(hipEventCreate(&e));
+		}
+		c->profiling = on != 0;
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_profile_read(pic_ctx* c, int max_entries, char* names, double* total_ms, long long* counts, double* bytes) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		flush_profile(c);
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	int i = 0;
+	for (auto& kv : c->prof) {
+		if (i >= max_entries) break;
+		if (names) {
+			std::strncpy(names + 32 * i, kv.first.c_str(), 31);
+			names[32 * i + 31] = 0;
+		}
+		if (total_ms) total_ms[i] = kv.second.ms;
+		if (counts) counts[i] = kv.second.count;
+		if (bytes) bytes[i] = kv.second.bytes;
+		++i;
+	}
+	return i;
+}
+
+int pic_synchronize(pic_ctx* c) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	hipError_t e = hipStreamSynchronize(c->stream);
+	if (e != hipSuccess) {
+		c->error = hipGetErrorString(e);
+		return PIC_ERR_DEVICE;
+	}
+	return PIC_OK;
+}
+
+}  // extern "C"

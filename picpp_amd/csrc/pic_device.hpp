@@ -1,0 +1,163 @@
+// Device-side geometry and per-particle arithmetic shared by the HIP kernels.
+//
+// Layout (ghosted, as the reference lays out its grid, PIC++/initialization.cpp:6618-7222):
+//   N = n + 4 cells per axis; nodes 0..N (N+1 per axis), cells 0..N-1;
+//   node i sits at x0 + i*dx with x0 = L - 2 dx (PIC++/initialization.cpp:194);
+//   owned nodes and cells are 2..n+1 per axis.
+// Grid arrays are structure-of-arrays: component c of a node field lives at
+// field[c * nnodes + node_id], node_id = (i*(N1+1) + j)*(N2+1) + k.
+// GMRES vectors are compact over owned nodes: vec[c * nown + (i*n1 + j)*n2 + k].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace picgpu {
+
+constexpr int kMaxSpecies = 8;
+
+struct Grid {
+	int n[3];
+	int N[3];
+	double d[3];
+	double x0[3];
+	double L[3];
+	double volume;
+	long long nnodes, ncells, nown;
+};
+
+struct SpeciesTable {
+	int count;
+	double mass[kMaxSpecies];
+	double charge[kMaxSpecies];
+	double weight[kMaxSpecies];
+};
+
+__host__ __device__ inline long long node_id(const Grid& g, int i, int j, int k) {
+	return ((long long)i * (g.N[1] + 1) + j) * (g.N[2] + 1) + k;
+}
+__host__ __device__ inline long long cell_id(const Grid& g, int i, int j, int k) {
+	return ((long long)i * g.N[1] + j) * g.N[2] + k;
+}
+__host__ __device__ inline long long own_id(const Grid& g, int i, int j, int k) {
+	return ((long long)i * g.n[1] + j) * g.n[2] + k;
+}
+// periodic owned index (0..n-1) of a ghosted index
+__host__ __device__ inline int wrap_owned(int i, int n) {
+	int r = (i - 2) % n;
+	return r < 0 ? r + n : r;
+}
+
+// correlationBspline, splineOrder 0 (PIC++/correlation.cpp:798-835).
+__device__ inline double bspline(double x, double hw, double left, double right) {
+	if (x < left - hw) return 0.0;
+	if (x > right + hw) return 0.0;
+	double c;
+	if (x < left + hw) {
+		c = 0.5 * (x + hw - left) / hw;
+	} else if (x > right - hw) {
+		c = 0.5 * (right - (x - hw)) / hw;
+	} else {
+		c = 1.0;
+	}
+	if (c < 0.0 && c > -1e-14) c = 0.0;
+	return c;
+}
+
+// One axis of the node and cell correlation maps
+// (updateCorrelationMapNodeX / updateCorrelationMapCellX, PIC++/correlation.cpp:1112-1145, :1283-1312).
+struct AxisMap {
+	int nbase;     // first node index
+	int cbase;     // first cell index
+	double nw[2];  // node weights
+	double cw[2];  // cell weights
+};
+
+__device__ inline AxisMap axis_map(double x, double x0, double d) {
+	AxisMap m;
+	double hw = d / 2;
+	int cc = (int)floor((x - x0) / d);
+	double mid = ((x0 + cc * d) + (x0 + (cc + 1) * d)) / 2;
+	m.cbase = cc + ((x < mid) ? -1 : 0);
+	m.cw[0] = bspline(x, hw, x0 + m.cbase * d, x0 + (m.cbase + 1) * d);
+	m.cw[1] = bspline(x, hw, x0 + (m.cbase + 1) * d, x0 + (m.cbase + 2) * d);
+	int nc = (int)floor(((x - x0) / d) + 0.5);
+	m.nbase = nc + ((x < x0 + nc * d) ? -1 : 0);
+	m.nw[0] = bspline(x, hw, x0 + (m.nbase - 0.5) * d, x0 + (m.nbase + 0.5) * d);
+	m.nw[1] = bspline(x, hw, x0 + (m.nbase + 1 - 0.5) * d, x0 + (m.nbase + 1 + 0.5) * d);
+	return m;
+}
+
+// Sort key of a particle: (species, node-map cell, octant). The octant bit per axis is
+// 1 when the cell-map base equals the node-map base (upper half of the cell), so the
+// particles of one key share both stencils.
+__device__ inline uint32_t particle_key(const Grid& g, int sp, const AxisMap* m) {
+	int bx = m[0].cbase - m[0].nbase + 1;
+	int by = m[1].cbase - m[1].nbase + 1;
+	int bz = m[2].cbase - m[2].nbase + 1;
+	long long cid = cell_id(g, m[0].nbase, m[1].nbase, m[2].nbase);
+	return (uint32_t)((((long long)sp * g.ncells + cid) << 3) | (bx << 2) | (by << 1) | bz);
+}
+
+// correlationGeneralEfield / correlationBfield (PIC++/correlation.cpp:23-58, :103-164):
+// 8-point gathers, x outermost, in the reference's summation order.
+__device__ inline void gather_node(const Grid& g, const double* __restrict__ f, const AxisMap* m, double out[3]) {
+	out[0] = out[1] = out[2] = 0.0;
+	for (int i = 0; i < 2; ++i)
+		for (int j = 0; j < 2; ++j)
+			for (int k = 0; k < 2; ++k) {
+				long long id = node_id(g, m[0].nbase + i, m[1].nbase + j, m[2].nbase + k);
+				double c = m[0].nw[i] * m[1].nw[j] * m[2].nw[k];
+				out[0] = out[0] + f[id] * c;
+				out[1] = out[1] + f[g.nnodes + id] * c;
+				out[2] = out[2] + f[2 * g.nnodes + id] * c;
+			}
+}
+
+__device__ inline void gather_cell(const Grid& g, const double* __restrict__ f, const AxisMap* m, double out[3]) {
+	out[0] = out[1] = out[2] = 0.0;
+	for (int i = 0; i < 2; ++i)
+		for (int j = 0; j < 2; ++j)
+			for (int k = 0; k < 2; ++k) {
+				long long id = cell_id(g, m[0].cbase + i, m[1].cbase + j, m[2].cbase + k);
+				double c = m[0].cw[i] * m[1].cw[j] * m[2].cw[k];
+				out[0] = out[0] + f[id] * c;
+				out[1] = out[1] + f[g.ncells + id] * c;
+				out[2] = out[2] + f[2 * g.ncells + id] * c;
+			}
+}
+
+// Particle::gammaFactor and getVelocity (PIC++/particle.cpp:216-235, :381-390).
+__device__ inline void velocity_gamma(const double p[3], double m, double v[3], double& g) {
+	double p2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+	g = sqrt(p2 / (m * m) + 1.0);
+	v[0] = p[0] / (m * g);
+	v[1] = p[1] / (m * g);
+	v[2] = p[2] / (m * g);
+}
+
+// evaluateAlphaRotationTensor (PIC++/mover.cpp:851-908): row-major 3x3.
+// alpha_ij = [delta_ij + bs^2 B_i B_j - bs sum_l eps_jil B_l] / (G (1 + bs^2 B^2)),
+// G = beta (E.v) + gamma, bs = beta / G.
+__device__ inline void alpha_tensor(double beta, const double v[3], double gamma, const double E[3], const double B[3],
+                                    double a[9]) {
+	// Levi-Civita eps[j][i][l] as indexed by the reference (PIC++/initialization.cpp:779-793).
+	constexpr double eps[3][3][3] = {{{0, 0, 0}, {0, 0, 1}, {0, -1, 0}},
+	                                 {{0, 0, -1}, {0, 0, 0}, {1, 0, 0}},
+	                                 {{0, 1, 0}, {-1, 0, 0}, {0, 0, 0}}};
+	double G = (beta * (E[0] * v[0] + E[1] * v[1] + E[2] * v[2])) + gamma;
+	double bs = beta / G;
+	double b2 = bs * bs;
+	double den = G * (1 + b2 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]));
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j) {
+			double r = (i == j ? 1.0 : 0.0) + (b2 * B[i] * B[j]);
+#pragma unroll
+			for (int l = 0; l < 3; ++l)
+				if (eps[j][i][l] != 0) r -= (bs * eps[j][i][l] * B[l]);
+			a[i * 3 + j] = r / den;
+		}
+}
+
+}  // namespace picgpu

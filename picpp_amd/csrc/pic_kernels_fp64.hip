@@ -1,0 +1,1110 @@
+// FP64 kernels of the implicit step (the path held to the reference's arithmetic).
+//
+// Particles are stored structure-of-arrays and sorted by key = (species, node-map
+// cell, octant); `start[key]` is the first particle of each key. Deposition is
+// deterministic without atomics: one lane owns one cell (node-map moments) or one
+// dual cell (cell-map moments), walks its particles in sorted order and keeps the
+// 8 corner sums in registers; a second pass gathers the 8 corner sums of the cells
+// around each grid point in a fixed order.
+//
+// FP contraction is off in this file so the per-particle arithmetic follows the
+// reference operation by operation.
+#pragma clang fp contract(off)
+
+#include "pic_device.hpp"
+#include "pic_kernels.hpp"
+
+namespace picgpu {
+
+constexpr double kPi = 3.141592653589793115997963468544185161590576171875;  // 4*atan2(1,1)
+constexpr double kCprime = 1.025;                                              // PIC++/constants.h:45
+constexpr double kCprime2 = 1.025 * 1.025;
+
+__device__ inline void load_particle(const ParticleArrays& P, long long i, double x[3], double p[3]) {
+	x[0] = P.x[0][i];
+	x[1] = P.x[1][i];
+	x[2] = P.x[2][i];
+	p[0] = P.p[0][i];
+	p[1] = P.p[1][i];
+	p[2] = P.p[2][i];
+}
+
+// Maps of the three axes. Stencil bases are clamped into the ghosted arrays so that no
+// access can leave them; returns false when a clamp was needed (the particle was
+// further than the ghost layers allow, which the callers report as an error).
+__device__ inline bool maps3(const Grid& g, const double x[3], AxisMap m[3]) {
+	bool ok = true;
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		m[a] = axis_map(x[a], g.x0[a], g.d[a]);
+		if (!(m[a].nbase >= 0 && m[a].nbase <= g.N[a] - 1)) {
+			m[a].nbase = m[a].nbase < 0 ? 0 : g.N[a] - 1;
+			ok = false;
+		}
+		if (!(m[a].cbase >= 0 && m[a].cbase <= g.N[a] - 2)) {
+			m[a].cbase = m[a].cbase < 0 ? 0 : g.N[a] - 2;
+			ok = false;
+		}
+	}
+	return ok;
+}
+
+// ------------------------------------------------------------------ moments at x^n
+// evaluateParticlesRotationTensor + the node loop of updateElectroMagneticParameters
+// (PIC++/mover.cpp:810-841, PIC++/momentum_collecting.cpp:397-480): per cell, the 8
+// corner sums of J (3) and of the dielectric tensor mu (9).
+__global__ void __launch_bounds__(128) k_node_moments(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ start,
+                                                      const double* __restrict__ E, const double* __restrict__ B,
+                                                      double dt, double theta, double* __restrict__ out) {
+	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.ncells) return;
+	double acc[8][12];
+#pragma unroll
+	for (int a = 0; a < 8; ++a)
+#pragma unroll
+		for (int q = 0; q < 12; ++q) acc[a][q] = 0.0;
+	for (int s = 0; s < sp.count; ++s) {
+		long long key0 = ((long long)s * g.ncells + c) << 3;
+		int lo = start[key0], hi = start[key0 + 8];
+		double m = sp.mass[s], q = sp.charge[s], w = sp.weight[s];
+		double beta = 0.5 * q * dt / m;
+		double omega = w * theta * dt * dt * 2 * kPi * q * q / m;
+		double pc = q * w;
+		for (int i = lo; i < hi; ++i) {
+			double x[3], p[3];
+			load_particle(P, i, x, p);
+			AxisMap mp[3];
+			maps3(g, x, mp);
+			double Ef[3], Bf[3], v[3], gm, a[9];
+			gather_node(g, E, mp, Ef);
+			gather_cell(g, B, mp, Bf);
+			velocity_gamma(p, m, v, gm);
+			alpha_tensor(beta, v, gm, Ef, Bf, a);
+			double u[3] = {v[0] * gm, v[1] * gm, v[2] * gm};
+			double rv[3];
+#pragma unroll
+			for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
+#pragma unroll
+			for (int ci = 0; ci < 2; ++ci)
+#pragma unroll
+				for (int cj = 0; cj < 2; ++cj)
+#pragma unroll
+					for (int ck = 0; ck < 2; ++ck) {
+						int corner = ci * 4 + cj * 2 + ck;
+						double corr = mp[0].nw[ci] * mp[1].nw[cj] * mp[2].nw[ck] / g.volume;
+						double f = pc * corr;
+#pragma unroll
+						for (int r = 0; r < 3; ++r) acc[corner][r] += rv[r] * f;
+						double o = omega * corr;
+#pragma unroll
+						for (int r = 0; r < 9; ++r) acc[corner][3 + r] = acc[corner][3 + r] - a[r] * o;
+					}
+		}
+	}
+#pragma unroll
+	for (int a = 0; a < 8; ++a)
+#pragma unroll
+		for (int q = 0; q < 12; ++q) out[((long long)q * 8 + a) * g.ncells + c] = acc[a][q];
+}
+
+// Node values from the corner sums of the 8 cells around each node, written to the
+// ghosted node arrays J (3 planes) and mu (9 planes).
+__global__ void k_node_assemble(Grid g, const double* __restrict__ cs, double* __restrict__ J, double* __restrict__ mu) {
+	long long nid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (nid >= g.nnodes) return;
+	int k = (int)(nid % (g.N[2] + 1));
+	int j = (int)((nid / (g.N[2] + 1)) % (g.N[1] + 1));
+	int i = (int)(nid / ((long long)(g.N[2] + 1) * (g.N[1] + 1)));
+	double s[12];
+#pragma unroll
+	for (int q = 0; q < 12; ++q) s[q] = 0.0;
+	for (int a = 0; a < 2; ++a)
+		for (int b = 0; b < 2; ++b)
+			for (int cc = 0; cc < 2; ++cc) {
+				int ci = i - a, cj = j - b, ck = k - cc;
+				if (ci < 0 || cj < 0 || ck < 0 || ci >= g.N[0] || cj >= g.N[1] || ck >= g.N[2]) continue;
+				long long cid = cell_id(g, ci, cj, ck);
+				int corner = a * 4 + b * 2 + cc;
+#pragma unroll
+				for (int q = 0; q < 12; ++q) s[q] += cs[((long long)q * 8 + corner) * g.ncells + cid];
+			}
+#pragma unroll
+	for (int q = 0; q < 3; ++q) J[q * g.nnodes + nid] = s[q];
+#pragma unroll
+	for (int q = 0; q < 9; ++q) mu[q * g.nnodes + nid] = s[3 + q];
+}
+
+// Cell loop of updateElectroMagneticParameters (PIC++/momentum_collecting.cpp:521-584):
+// per dual cell b (cell-map base), the 8 corner sums of rhohat and the pressure tensor
+// contributions. The reference writes the pressure tensor at [curI][j][k] with the
+// stencil indices j, k (momentum_collecting.cpp:565, :577); the 48 partials kept here
+// are indexed (x stencil index, j, k, 6 symmetric components).
+__global__ void __launch_bounds__(128) k_cell_moments(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ start,
+                                                      const double* __restrict__ E, const double* __restrict__ B,
+                                                      double dt, double* __restrict__ rho_out, double* __restrict__ pi_out) {
+	long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= g.ncells) return;
+	int bk = (int)(b % g.N[2]);
+	int bj = (int)((b / g.N[2]) % g.N[1]);
+	int bi = (int)(b / ((long long)g.N[2] * g.N[1]));
+	double ar[8], ap[8][6];
+#pragma unroll
+	for (int t = 0; t < 8; ++t) {
+		ar[t] = 0.0;
+#pragma unroll
+		for (int q = 0; q < 6; ++q) ap[t][q] = 0.0;
+	}
+	for (int s = 0; s < sp.count; ++s) {
+		double m = sp.mass[s], q = sp.charge[s], w = sp.weight[s];
+		double beta = 0.5 * q * dt / m;
+		double pc = q * w;
+		for (int seg = 0; seg < 8; ++seg) {
+			int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
+			int mi = bi + ax, mj = bj + ay, mk = bk + az;
+			if (mi >= g.N[0] || mj >= g.N[1] || mk >= g.N[2]) continue;
+			long long key = (((long long)s * g.ncells + cell_id(g, mi, mj, mk)) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) | (1 - az);
+			int lo = start[key], hi = start[key + 1];
+			for (int i = lo; i < hi; ++i) {
+				double x[3], p[3];
+				load_particle(P, i, x, p);
+				AxisMap mp[3];
+				maps3(g, x, mp);
+				double Ef[3], Bf[3], v[3], gm, a[9];
+				gather_node(g, E, mp, Ef);
+				gather_cell(g, B, mp, Bf);
+				velocity_gamma(p, m, v, gm);
+				alpha_tensor(beta, v, gm, Ef, Bf, a);
+				double u[3] = {v[0] * gm, v[1] * gm, v[2] * gm};
+				double rv[3];
+#pragma unroll
+				for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
+				double T[6] = {rv[0] * rv[0], rv[0] * rv[1], rv[0] * rv[2], rv[1] * rv[1], rv[1] * rv[2], rv[2] * rv[2]};
+#pragma unroll
+				for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+					for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+						for (int tk = 0; tk < 2; ++tk) {
+							int t = ti * 4 + tj * 2 + tk;
+							double corr = mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk] / g.volume;
+							double f = pc * corr;
+							ar[t] += f;
+#pragma unroll
+							for (int r = 0; r < 6; ++r) ap[t][r] += f * T[r];
+						}
+			}
+		}
+	}
+#pragma unroll
+	for (int t = 0; t < 8; ++t) {
+		rho_out[(long long)t * g.ncells + b] = ar[t];
+#pragma unroll
+		for (int r = 0; r < 6; ++r) pi_out[((long long)t * 6 + r) * g.ncells + b] = ap[t][r];
+	}
+}
+
+// rhohat per cell from the corner sums of the 8 dual cells around it.
+__global__ void k_cell_assemble(Grid g, const double* __restrict__ rs, double* __restrict__ rho) {
+	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.ncells) return;
+	int k = (int)(c % g.N[2]);
+	int j = (int)((c / g.N[2]) % g.N[1]);
+	int i = (int)(c / ((long long)g.N[2] * g.N[1]));
+	double s = 0.0;
+	for (int a = 0; a < 2; ++a)
+		for (int b = 0; b < 2; ++b)
+			for (int cc = 0; cc < 2; ++cc) {
+				int bi = i - a, bj = j - b, bk = k - cc;
+				if (bi < 0 || bj < 0 || bk < 0) continue;
+				s += rs[(long long)(a * 4 + b * 2 + cc) * g.ncells + cell_id(g, bi, bj, bk)];
+			}
+	rho[c] = s;
+}
+
+// Pressure tensor at [i][j][k], j, k in {0, 1}: sum over every dual cell b with
+// b.x + ti == i of its (ti, j, k) partial. One block per (i, j, k); fixed-order tree.
+__global__ void __launch_bounds__(256) k_pressure_reduce(Grid g, const double* __restrict__ ps, double* __restrict__ Pi) {
+	int i = blockIdx.x / 4;
+	int tj = (blockIdx.x / 2) % 2;
+	int tk = blockIdx.x % 2;
+	__shared__ double red[6][256];
+	double acc[6] = {0, 0, 0, 0, 0, 0};
+	long long plane = (long long)g.N[1] * g.N[2];
+	for (int ti = 0; ti < 2; ++ti) {
+		int bi = i - ti;
+		if (bi < 0 || bi >= g.N[0]) continue;
+		int t = ti * 4 + tj * 2 + tk;
+		for (long long r = threadIdx.x; r < plane; r += blockDim.x) {
+			long long b = (long long)bi * plane + r;
+#pragma unroll
+			for (int q = 0; q < 6; ++q) acc[q] += ps[((long long)t * 6 + q) * g.ncells + b];
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = acc[q];
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w)
+#pragma unroll
+			for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		long long c = cell_id(g, i, tj, tk);
+		const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+		for (int q = 0; q < 9; ++q) Pi[q * g.ncells + c] = red[sym[q]][0];
+	}
+}
+
+// ------------------------------------------------------------------ folds and ghosts
+// One-rank periodic folds, one axis per launch (momentum_exchange.cpp:172-195, :446-460,
+// :807-811, :1657-1660). Node arrays: pairs (N - i, 4 - i), i = 0..4. Cell arrays: pairs
+// (N-1-i, lo-i), i < pairs. Every thread owns one (line, pair, component).
+__global__ void k_fold(Grid g, double* __restrict__ f, int ncomp, int axis, int nodes, int pairs, int hi0, int lo0) {
+	int ext[3];
+	for (int a = 0; a < 3; ++a) ext[a] = g.N[a] + (nodes ? 1 : 0);
+	int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
+	long long lines = (long long)ext[o1] * ext[o2];
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	long long total = lines * pairs * ncomp;
+	if (t >= total) return;
+	int comp = (int)(t % ncomp);
+	int pr = (int)((t / ncomp) % pairs);
+	long long line = t / ((long long)ncomp * pairs);
+	int a = (int)(line / ext[o2]), bb = (int)(line % ext[o2]);
+	int hi[3], lo[3];
+	hi[axis] = hi0 - pr;
+	lo[axis] = lo0 - pr;
+	hi[o1] = lo[o1] = a;
+	hi[o2] = lo[o2] = bb;
+	long long len = nodes ? g.nnodes : g.ncells;
+	long long ih = nodes ? node_id(g, hi[0], hi[1], hi[2]) : cell_id(g, hi[0], hi[1], hi[2]);
+	long long il = nodes ? node_id(g, lo[0], lo[1], lo[2]) : cell_id(g, lo[0], lo[1], lo[2]);
+	double v = f[comp * len + ih] + f[comp * len + il];
+	f[comp * len + ih] = v;
+	f[comp * len + il] = v;
+}
+
+// Periodic ghost fill of node (exchangeGeneralEfield, fields_exchange.cpp:158-170) or cell
+// (exchangeGeneralBfield, :629-637) arrays: every ghost takes its owned periodic image.
+__global__ void k_ghost_fill(Grid g, double* __restrict__ f, int ncomp, int nodes) {
+	long long len = nodes ? g.nnodes : g.ncells;
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= len) return;
+	int e1 = g.N[1] + (nodes ? 1 : 0), e2 = g.N[2] + (nodes ? 1 : 0);
+	int k = (int)(t % e2);
+	int j = (int)((t / e2) % e1);
+	int i = (int)(t / ((long long)e1 * e2));
+	bool ghost = i < 2 || j < 2 || k < 2 || i > g.n[0] + 1 || j > g.n[1] + 1 || k > g.n[2] + 1;
+	if (!ghost) return;
+	int si = wrap_owned(i, g.n[0]) + 2, sj = wrap_owned(j, g.n[1]) + 2, sk = wrap_owned(k, g.n[2]) + 2;
+	long long src = nodes ? node_id(g, si, sj, sk) : cell_id(g, si, sj, sk);
+	for (int c = 0; c < ncomp; ++c) f[c * len + t] = f[c * len + src];
+}
+
+// ------------------------------------------------------------------ moment post-processing
+// evaluateDivPressureTensor (fields.cpp:3158-3223) and J -= eta dt divPi (momentum_collecting.cpp:623-636).
+__global__ void k_div_pressure_update_j(Grid g, const double* __restrict__ Pi, double* __restrict__ J, double* __restrict__ divPi,
+                                        double eta, double dt) {
+	long long nid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (nid >= g.nnodes) return;
+	int k = (int)(nid % (g.N[2] + 1));
+	int j = (int)((nid / (g.N[2] + 1)) % (g.N[1] + 1));
+	int i = (int)(nid / ((long long)(g.N[2] + 1) * (g.N[1] + 1)));
+	double out[3] = {0.0, 0.0, 0.0};
+	if (!(i == 0 || i == g.N[0] || j == 0 || j == g.N[1] || k == 0 || k == g.N[2])) {
+		auto Pv = [&](int a, int b, int c, int l, int m) { return Pi[(l * 3 + m) * g.ncells + cell_id(g, a, b, c)]; };
+		for (int col = 0; col < 3; ++col) {
+			double rx = (Pv(i, j, k, 0, col) + Pv(i, j - 1, k, 0, col) + Pv(i, j, k - 1, 0, col) + Pv(i, j - 1, k - 1, 0, col)) * 0.25;
+			double lx = (Pv(i - 1, j, k, 0, col) + Pv(i - 1, j - 1, k, 0, col) + Pv(i - 1, j, k - 1, 0, col) +
+			             Pv(i - 1, j - 1, k - 1, 0, col)) * 0.25;
+			double by = (Pv(i, j, k, 1, col) + Pv(i - 1, j, k, 1, col) + Pv(i, j, k - 1, 1, col) + Pv(i - 1, j, k - 1, 1, col)) * 0.25;
+			double fy = (Pv(i, j - 1, k, 1, col) + Pv(i - 1, j - 1, k, 1, col) + Pv(i, j - 1, k - 1, 1, col) +
+			             Pv(i - 1, j - 1, k - 1, 1, col)) * 0.25;
+			double tz = (Pv(i, j, k, 2, col) + Pv(i, j - 1, k, 2, col) + Pv(i - 1, j, k, 2, col) + Pv(i - 1, j - 1, k, 2, col)) * 0.25;
+			double bz = (Pv(i, j, k - 1, 2, col) + Pv(i, j - 1, k - 1, 2, col) + Pv(i - 1, j, k - 1, 2, col) +
+			             Pv(i - 1, j - 1, k - 1, 2, col)) * 0.25;
+			out[col] = ((rx - lx) / g.d[0]) + ((by - fy) / g.d[1]) + ((tz - bz) / g.d[2]);
+		}
+	}
+	for (int c = 0; c < 3; ++c) {
+		divPi[c * g.nnodes + nid] = out[c];
+		J[c * g.nnodes + nid] = J[c * g.nnodes + nid] - out[c] * eta * dt;
+	}
+}
+
+// evaluateDivFlux (fields.cpp:3060-3156) and rhohat -= dt theta divJ (momentum_collecting.cpp:651-665).
+__global__ void k_div_flux_update_rho(Grid g, const double* __restrict__ J, double* __restrict__ rho, double dt, double theta) {
+	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.ncells) return;
+	int k = (int)(c % g.N[2]);
+	int j = (int)((c / g.N[2]) % g.N[1]);
+	int i = (int)(c / ((long long)g.N[2] * g.N[1]));
+	auto F = [&](int a, int b, int cc, int l) { return J[l * g.nnodes + node_id(g, a, b, cc)]; };
+	double rX = (F(i + 1, j, k, 0) + F(i + 1, j + 1, k, 0) + F(i + 1, j, k + 1, 0) + F(i + 1, j + 1, k + 1, 0)) * 0.25;
+	double lX = (F(i, j, k, 0) + F(i, j + 1, k, 0) + F(i, j, k + 1, 0) + F(i, j + 1, k + 1, 0)) * 0.25;
+	double rY = (F(i, j + 1, k, 1) + F(i + 1, j + 1, k, 1) + F(i, j + 1, k + 1, 1) + F(i + 1, j + 1, k + 1, 1)) * 0.25;
+	double lY = (F(i, j, k, 1) + F(i + 1, j, k, 1) + F(i, j, k + 1, 1) + F(i + 1, j, k + 1, 1)) * 0.25;
+	double rZ = (F(i + 1, j, k + 1, 2) + F(i, j, k + 1, 2) + F(i, j + 1, k + 1, 2) + F(i + 1, j + 1, k + 1, 2)) * 0.25;
+	double lZ = (F(i + 1, j, k, 2) + F(i, j, k, 2) + F(i, j + 1, k, 2) + F(i + 1, j + 1, k, 2)) * 0.25;
+	double divJ = ((rX - lX) / g.d[0]) + ((rY - lY) / g.d[1]) + ((rZ - lZ) / g.d[2]);
+	rho[c] -= dt * theta * divJ;
+}
+
+// ------------------------------------------------------------------ Maxwell system
+// Right part of createInternalEquation (fields.cpp:2635-2715) with evaluateRotB (:2799-2860)
+// and evaluateGradDensity (:3229-3259), for owned nodes, into the compact GMRES layout.
+__global__ void k_rhs(Grid g, const double* __restrict__ E, const double* __restrict__ B, const double* __restrict__ J,
+                      const double* __restrict__ rho, double theta, double dt, double* __restrict__ rhs) {
+	long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= g.nown) return;
+	int k = (int)(o % g.n[2]) + 2;
+	int j = (int)((o / g.n[2]) % g.n[1]) + 2;
+	int i = (int)(o / ((long long)g.n[2] * g.n[1])) + 2;
+	auto b = [&](int a, int bb, int c, int l) { return B[l * g.ncells + cell_id(g, a, bb, c)]; };
+	double rX[3], lX[3], rY[3], lY[3], rZ[3], lZ[3];
+	for (int l = 0; l < 3; ++l) {
+		rX[l] = (b(i, j, k, l) + b(i, j - 1, k, l) + b(i, j, k - 1, l) + b(i, j - 1, k - 1, l)) / 4.0;
+		lX[l] = (b(i - 1, j, k, l) + b(i - 1, j - 1, k, l) + b(i - 1, j, k - 1, l) + b(i - 1, j - 1, k - 1, l)) / 4.0;
+		rY[l] = (b(i, j, k, l) + b(i - 1, j, k, l) + b(i, j, k - 1, l) + b(i - 1, j, k - 1, l)) / 4.0;
+		lY[l] = (b(i, j - 1, k, l) + b(i - 1, j - 1, k, l) + b(i, j - 1, k - 1, l) + b(i - 1, j - 1, k - 1, l)) / 4.0;
+		rZ[l] = (b(i, j, k, l) + b(i - 1, j, k, l) + b(i, j - 1, k, l) + b(i - 1, j - 1, k, l)) / 4.0;
+		lZ[l] = (b(i, j, k - 1, l) + b(i - 1, j, k - 1, l) + b(i, j - 1, k - 1, l) + b(i - 1, j - 1, k - 1, l)) / 4.0;
+	}
+	double rb[3];
+	rb[0] = (rY[2] - lY[2]) / g.d[1] - (rZ[1] - lZ[1]) / g.d[2];
+	rb[1] = (rZ[0] - lZ[0]) / g.d[2] - (rX[2] - lX[2]) / g.d[0];
+	rb[2] = (rX[1] - lX[1]) / g.d[0] - (rY[0] - lY[0]) / g.d[1];
+	auto r = [&](int a, int bb, int c) { return rho[cell_id(g, a, bb, c)]; };
+	double dRX = (r(i, j, k) + r(i, j - 1, k) + r(i, j, k - 1) + r(i, j - 1, k - 1)) * 0.25;
+	double dLX = (r(i - 1, j, k) + r(i - 1, j - 1, k) + r(i - 1, j, k - 1) + r(i - 1, j - 1, k - 1)) * 0.25;
+	double dRY = (r(i, j, k) + r(i - 1, j, k) + r(i, j, k - 1) + r(i - 1, j, k - 1)) * 0.25;
+	double dLY = (r(i, j - 1, k) + r(i - 1, j - 1, k) + r(i, j - 1, k - 1) + r(i - 1, j - 1, k - 1)) * 0.25;
+	double dRZ = (r(i, j, k) + r(i, j - 1, k) + r(i - 1, j, k) + r(i - 1, j - 1, k)) * 0.25;
+	double dLZ = (r(i, j, k - 1) + r(i, j - 1, k - 1) + r(i - 1, j, k - 1) + r(i - 1, j - 1, k - 1)) * 0.25;
+	double gr[3] = {(dRX - dLX) / g.d[0], (dRY - dLY) / g.d[1], (dRZ - dLZ) / g.d[2]};
+	long long nid = node_id(g, i, j, k);
+	for (int a = 0; a < 3; ++a) {
+		double t1 = (rb[a] * kCprime - (J[a * g.nnodes + nid] * 4 * kPi)) * (theta * dt);
+		double t2 = gr[a] * kCprime2 * theta * theta * dt * dt * 4 * kPi;
+		rhs[a * g.nown + o] = (E[a * g.nnodes + nid] + t1) - t2;
+	}
+}
+
+// Owned-node copy of the folded dielectric tensor (9 planes, compact layout).
+__global__ void k_pack_mu(Grid g, const double* __restrict__ mu, double* __restrict__ muc) {
+	long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= g.nown) return;
+	int k = (int)(o % g.n[2]) + 2;
+	int j = (int)((o / g.n[2]) % g.n[1]) + 2;
+	int i = (int)(o / ((long long)g.n[2] * g.n[1])) + 2;
+	long long nid = node_id(g, i, j, k);
+	for (int q = 0; q < 9; ++q) muc[q * g.nown + o] = mu[q * g.nnodes + nid];
+}
+
+// The implicit Maxwell operator applied matrix-free on the compact periodic layout:
+// the 37 entries of createInternalEquation{X,Y,Z} (fields.cpp:1396-1509, :1961-2072,
+// :2521-2632), in push_back order, with mu read at the neighbouring nodes.
+struct Neigh {
+	int i[3], j[3], k[3];  // owned index of offset -1, 0, +1 per axis
+};
+
+__device__ inline long long oidx(const Grid& g, const Neigh& nb, int di, int dj, int dk) {
+	return ((long long)nb.i[di + 1] * g.n[1] + nb.j[dj + 1]) * g.n[2] + nb.k[dk + 1];
+}
+
+__global__ void k_apply_operator(Grid g, const double* __restrict__ muc, const double* __restrict__ v, double C,
+                                 double* __restrict__ out) {
+	long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= g.nown) return;
+	int k = (int)(o % g.n[2]);
+	int j = (int)((o / g.n[2]) % g.n[1]);
+	int i = (int)(o / ((long long)g.n[2] * g.n[1]));
+	Neigh nb;
+	nb.i[0] = i == 0 ? g.n[0] - 1 : i - 1;
+	nb.i[1] = i;
+	nb.i[2] = i == g.n[0] - 1 ? 0 : i + 1;
+	nb.j[0] = j == 0 ? g.n[1] - 1 : j - 1;
+	nb.j[1] = j;
+	nb.j[2] = j == g.n[1] - 1 ? 0 : j + 1;
+	nb.k[0] = k == 0 ? g.n[2] - 1 : k - 1;
+	nb.k[1] = k;
+	nb.k[2] = k == g.n[2] - 1 ? 0 : k + 1;
+	const long long L = g.nown;
+	auto M = [&](long long id, int l, int m) { return muc[(l * 3 + m) * L + id]; };
+	auto V = [&](long long id, int l) { return v[l * L + id]; };
+	double D2[3] = {g.d[0] * g.d[0], g.d[1] * g.d[1], g.d[2] * g.d[2]};
+	const int others[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+	const int face_order[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}};
+	const int trans_order[3][2] = {{1, 2}, {0, 2}, {1, 0}};
+	const int plane_order[3][2] = {{1, 2}, {0, 2}, {1, 0}};
+	const int sgn[4][2] = {{1, 1}, {-1, -1}, {-1, 1}, {1, -1}};
+	for (int c = 0; c < 3; ++c) {
+		double r = 0.0;
+		// diagonal (same node, same component)
+		double e = 1.0 - M(o, c, c);
+		for (int ax = 0; ax < 3; ++ax) {
+			if (ax == c) {
+				if (c == 0)
+					e += C * ((2.0 - 2 * M(o, c, c)) / D2[ax]);
+				else
+					e += C * (2.0 - 2 * M(o, c, c)) / D2[ax];
+			} else {
+				if (c == 0 || (c == 1 && ax == 2) || (c == 2 && ax == 1))
+					e += C * 2.0 / D2[ax];
+				else
+					e += C * (2.0 / D2[ax]);
+			}
+		}
+		r += e * V(o, c);
+		for (int t = 0; t < 2; ++t) {
+			int m = others[c][t];
+			double val = -M(o, c, m);
+			val -= C * 2 * M(o, c, m) / D2[c];
+			r += val * V(o, m);
+		}
+		for (int s = 1; s >= -1; s -= 2) {
+			int off[3] = {0, 0, 0};
+			off[c] = s;
+			long long q = oidx(g, nb, off[0], off[1], off[2]);
+			for (int t = 0; t < 3; ++t) {
+				int m = face_order[c][t];
+				double val = (m == c) ? C * (-1.0 + M(q, c, c)) / D2[c] : C * M(q, c, m) / D2[c];
+				r += val * V(q, m);
+			}
+		}
+		for (int t = 0; t < 2; ++t) {
+			int ax = trans_order[c][t];
+			double val = -C / D2[ax];
+			int off[3] = {0, 0, 0};
+			off[ax] = 1;
+			r += val * V(oidx(g, nb, off[0], off[1], off[2]), c);
+			off[ax] = -1;
+			r += val * V(oidx(g, nb, off[0], off[1], off[2]), c);
+		}
+		for (int t = 0; t < 2; ++t) {
+			int ax = plane_order[c][t];
+			double den;
+			if (c == 2 && ax == 1)
+				den = 4 * g.d[2] * g.d[1];
+			else if (c == 1 && ax == 2)
+				den = 4 * g.d[1] * g.d[2];
+			else
+				den = 4 * g.d[0] * g.d[ax == 0 ? c : ax];
+			int axA = c, axB = ax;
+			if ((c == 1 && ax == 0) || (c == 2 && ax == 0)) {
+				axA = 0;
+				axB = c;
+			}
+			if (c == 2 && ax == 1) {
+				axA = 1;
+				axB = 2;
+			}
+			for (int cr = 0; cr < 4; ++cr) {
+				int off[3] = {0, 0, 0};
+				off[axA] = sgn[cr][0];
+				off[axB] = sgn[cr][1];
+				long long q = oidx(g, nb, off[0], off[1], off[2]);
+				bool pos = sgn[cr][0] * sgn[cr][1] > 0;
+				for (int m = 0; m < 3; ++m) {
+					double val = pos ? C * M(q, ax, m) / den : -C * M(q, ax, m) / den;
+					r += val * V(q, m);
+				}
+			}
+		}
+		out[c * L + o] = r;
+	}
+}
+
+// ------------------------------------------------------------------ Krylov vector kernels
+// Deterministic dot product: fixed grid, fixed per-thread order, fixed tree.
+__global__ void __launch_bounds__(256) k_dot_partial(const double* __restrict__ a, const double* __restrict__ b, long long len,
+                                                     double* __restrict__ partial) {
+	__shared__ double red[256];
+	double s = 0.0;
+	for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < len; t += (long long)gridDim.x * blockDim.x)
+		s += a[t] * b[t];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Final stage: out = sum(partial); if do_sqrt, out = sqrt(out).
+__global__ void __launch_bounds__(256) k_dot_final(const double* __restrict__ partial, int n, double* __restrict__ out, int do_sqrt) {
+	__shared__ double red[256];
+	double s = 0.0;
+	for (int t = threadIdx.x; t < n; t += blockDim.x) s += partial[t];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) *out = do_sqrt ? sqrt(red[0]) : red[0];
+}
+
+// w = w - h * v, h read from device memory
+__global__ void k_axpy_dev(double* __restrict__ w, const double* __restrict__ v, const double* __restrict__ h, long long len) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= len) return;
+	w[t] -= (*h) * v[t];
+}
+
+// w = w / h if h > 0, else 0 (arnoldiIterations, specialmath.cpp:338-362)
+__global__ void k_normalize_dev(double* __restrict__ w, const double* __restrict__ h, long long len) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= len) return;
+	double hv = *h;
+	w[t] = hv > 0 ? w[t] / hv : 0.0;
+}
+
+// b = b / norm (norm on device); copy into v0
+__global__ void k_div_copy(double* __restrict__ b, double* __restrict__ v0, const double* __restrict__ norm, long long len) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= len) return;
+	double x = b[t] / *norm;
+	b[t] = x;
+	v0[t] = x;
+}
+
+// x = sum_m (v_m * y_m) * norm (specialmath.cpp:625-637)
+__global__ void k_combine(double* __restrict__ x, const double* const* __restrict__ V, const double* __restrict__ y, int m,
+                          const double* __restrict__ norm, long long len) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= len) return;
+	double s = 0.0;
+	double nm = *norm;
+	for (int q = 0; q < m; ++q) s += V[q][t] * y[q] * nm;
+	x[t] = s;
+}
+
+// E^{n+theta} on every ghosted node from the compact solution, and
+// newE = (E^{n+theta} - (1 - theta) E^n) / theta (fields.cpp:256-339).
+__global__ void k_unpack_e(Grid g, const double* __restrict__ x, const double* __restrict__ E, double theta,
+                           double* __restrict__ Etheta, double* __restrict__ Enew) {
+	long long nid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (nid >= g.nnodes) return;
+	int k = (int)(nid % (g.N[2] + 1));
+	int j = (int)((nid / (g.N[2] + 1)) % (g.N[1] + 1));
+	int i = (int)(nid / ((long long)(g.N[2] + 1) * (g.N[1] + 1)));
+	long long o = own_id(g, wrap_owned(i, g.n[0]), wrap_owned(j, g.n[1]), wrap_owned(k, g.n[2]));
+	for (int c = 0; c < 3; ++c) {
+		double et = x[c * g.nown + o];
+		Etheta[c * g.nnodes + nid] = et;
+		Enew[c * g.nnodes + nid] = (et - E[c * g.nnodes + nid] * (1 - theta)) / theta;
+	}
+}
+
+// evaluateMagneticField with evaluateRotEgeneral (fields.cpp:2717-2773, :2862-2953), all cells.
+__global__ void k_update_b(Grid g, const double* __restrict__ Et, const double* __restrict__ B, double dt,
+                           double* __restrict__ Bnew) {
+	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.ncells) return;
+	int k = (int)(c % g.N[2]);
+	int j = (int)((c / g.N[2]) % g.N[1]);
+	int i = (int)(c / ((long long)g.N[2] * g.N[1]));
+	auto e = [&](int a, int b, int cc, int l) { return Et[l * g.nnodes + node_id(g, a, b, cc)]; };
+	double rX[3], lX[3], rY[3], lY[3], rZ[3], lZ[3];
+	for (int l = 0; l < 3; ++l) {
+		rX[l] = (e(i + 1, j, k, l) + e(i + 1, j + 1, k, l) + e(i + 1, j, k + 1, l) + e(i + 1, j + 1, k + 1, l)) / 4.0;
+		lX[l] = (e(i, j, k, l) + e(i, j + 1, k, l) + e(i, j, k + 1, l) + e(i, j + 1, k + 1, l)) / 4.0;
+		rY[l] = (e(i, j + 1, k, l) + e(i + 1, j + 1, k, l) + e(i, j + 1, k + 1, l) + e(i + 1, j + 1, k + 1, l)) / 4.0;
+		lY[l] = (e(i, j, k, l) + e(i + 1, j, k, l) + e(i, j, k + 1, l) + e(i + 1, j, k + 1, l)) / 4.0;
+		rZ[l] = (e(i, j, k + 1, l) + e(i + 1, j, k + 1, l) + e(i, j + 1, k + 1, l) + e(i + 1, j + 1, k + 1, l)) / 4.0;
+		lZ[l] = (e(i, j, k, l) + e(i + 1, j, k, l) + e(i, j + 1, k, l) + e(i + 1, j + 1, k, l)) / 4.0;
+	}
+	double r[3];
+	r[0] = (rY[2] - lY[2]) / g.d[1] - (rZ[1] - lZ[1]) / g.d[2];
+	r[1] = (rZ[0] - lZ[0]) / g.d[2] - (rX[2] - lX[2]) / g.d[0];
+	r[2] = (rX[1] - lX[1]) / g.d[0] - (rY[0] - lY[0]) / g.d[1];
+	for (int l = 0; l < 3; ++l) Bnew[l * g.ncells + c] = B[l * g.ncells + c] - r[l] * (kCprime * dt);
+}
+
+// ------------------------------------------------------------------ mover
+// Particle::evaluateVelocity with its nonrelativistic branch (particle.cpp:366-379).
+__device__ inline void evaluate_velocity(const double p[3], double m, double v[3]) {
+	double p2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+	if (p2 < 0.0001 * m * m) {
+		for (int a = 0; a < 3; ++a) v[a] = p[a] / m;
+	} else {
+		double g = sqrt(p2 + m * m) / m;
+		for (int a = 0; a < 3; ++a) v[a] = p[a] / (m * g);
+	}
+}
+
+__device__ inline void cross(const double a[3], const double b[3], double out[3]) {
+	out[0] = a[1] * b[2] - a[2] * b[1];
+	out[1] = -a[0] * b[2] + a[2] * b[0];
+	out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ inline double dnorm(const double a[3], const double b[3]) {
+	double x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+	return sqrt(x * x + y * y + z * z);
+}
+
+// moveParticle (PIC++/mover.cpp:194-491) with the one-rank periodic wrap of exchangeParticles
+// (mover.cpp:1007-1021), followed by the new sort key. alpha is recomputed from E^n, B^n at x^n
+// exactly as evaluateParticlesRotationTensor computed it. One thread per particle.
+__global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start, const double* __restrict__ E,
+                       const double* __restrict__ Etheta, const double* __restrict__ B, double dt, double eta,
+                       uint32_t* __restrict__ keys, int* __restrict__ status) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P.count) return;
+	int s = 0;
+	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+	double m = sp.mass[s], q = sp.charge[s];
+	double x[3], p[3];
+	load_particle(P, i, x, p);
+	AxisMap mp[3];
+	if (!maps3(g, x, mp)) atomicOr(status, 8);
+	double En[3], Bf[3], v[3], gm, a[9];
+	gather_node(g, E, mp, En);
+	gather_cell(g, B, mp, Bf);
+	velocity_gamma(p, m, v, gm);
+	double beta = 0.5 * q * dt / m;
+	alpha_tensor(beta, v, gm, En, Bf, a);
+	double Ef[3];
+	gather_node(g, Etheta, mp, Ef);
+	double oldP[3] = {p[0], p[1], p[2]};
+	double newP[3] = {p[0], p[1], p[2]};
+	double oldX[3] = {x[0], x[1], x[2]};
+	double etaDt = eta * dt;
+	double vxb[3];
+	cross(v, Bf, vxb);
+	for (int c = 0; c < 3; ++c) newP[c] += (Ef[c] + vxb[c]) * q * dt;
+	double newV[3];
+	evaluate_velocity(newP, m, newV);
+	double mid[3];
+	for (int c = 0; c < 3; ++c) mid[c] = v[c] * (1 - eta) + newV[c] * eta;
+	for (int c = 0; c < 3; ++c) x[c] += mid[c] * etaDt;
+	double vhat[3];
+	for (int c = 0; c < 3; ++c) vhat[c] = (a[c * 3 + 0] * gm) * v[0] + (a[c * 3 + 1] * gm) * v[1] + (a[c * 3 + 2] * gm) * v[2];
+	double prevP[3] = {p[0], p[1], p[2]};
+	double pnorm = sqrt(oldP[0] * oldP[0] + oldP[1] * oldP[1] + oldP[2] * oldP[2]);
+	double err = dnorm(prevP, newP);
+	double prevX[3] = {x[0], x[1], x[2]};
+	double newX[3] = {x[0], x[1], x[2]};
+	double cerr = g.d[0];
+	int it = 0;
+	while (err > 1e-14 * pnorm && cerr > 1e-8 * g.d[0] && it < 3) {
+		++it;
+		for (int c = 0; c < 3; ++c) {
+			prevP[c] = newP[c];
+			prevX[c] = newX[c];
+		}
+		double rE[3];
+		for (int c = 0; c < 3; ++c) rE[c] = a[c * 3 + 0] * Ef[0] + a[c * 3 + 1] * Ef[1] + a[c * 3 + 2] * Ef[2];
+		for (int c = 0; c < 3; ++c) mid[c] = vhat[c] + rE[c] * beta;
+		for (int c = 0; c < 3; ++c) x[c] = oldX[c] + (mid[c] * etaDt);
+		for (int c = 0; c < 3; ++c) newX[c] = x[c];
+		if (!maps3(g, x, mp)) atomicOr(status, 8);
+		gather_node(g, Etheta, mp, Ef);
+		gather_cell(g, B, mp, Bf);
+		double mxb[3];
+		cross(mid, Bf, mxb);
+		for (int c = 0; c < 3; ++c) newP[c] = oldP[c] + (Ef[c] + mxb[c]) * (q * dt);
+		err = dnorm(prevP, newP);
+		cerr = dnorm(prevX, newX);
+	}
+	for (int c = 0; c < 3; ++c) {
+		p[c] = newP[c];
+		x[c] = oldX[c] + mid[c] * dt;
+	}
+	// one-rank periodic wrap (mover.cpp:435-490 classification, :1007-1021 shift)
+	for (int c = 0; c < 3; ++c) {
+		double lo = g.x0[c] + 2 * g.d[c];
+		double hi = g.x0[c] + (g.N[c] - 1 - 1) * g.d[c];
+		if (x[c] < lo)
+			x[c] += g.L[c];
+		else if (x[c] > hi)
+			x[c] -= g.L[c];
+		if (!(x[c] >= lo && x[c] <= hi)) atomicOr(status, 1);  // moved more than a box length
+	}
+	for (int c = 0; c < 3; ++c) {
+		P.x[c][i] = x[c];
+		P.p[c][i] = p[c];
+	}
+	if (!maps3(g, x, mp)) atomicOr(status, 8);
+	int bits[3];
+	for (int c = 0; c < 3; ++c) {
+		bits[c] = mp[c].cbase - mp[c].nbase + 1;
+		if (bits[c] < 0 || bits[c] > 1) atomicOr(status, 2);
+	}
+	keys[i] = particle_key(g, s, mp);
+}
+
+// Sort key of every particle at its current position (updateParticleCorrelationMaps).
+__global__ void k_keys(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start, uint32_t* __restrict__ keys,
+                       int* __restrict__ status) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P.count) return;
+	int s = 0;
+	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+	double x[3];
+	for (int c = 0; c < 3; ++c) {
+		x[c] = P.x[c][i];
+		double lo = g.x0[c] + 2 * g.d[c];
+		double hi = g.x0[c] + (g.N[c] - 2) * g.d[c];
+		if (!(x[c] >= lo && x[c] <= hi)) atomicOr(status, 4);
+	}
+	AxisMap mp[3];
+	if (!maps3(g, x, mp)) atomicOr(status, 8);
+	for (int c = 0; c < 3; ++c)
+		if (mp[c].cbase - mp[c].nbase + 1 < 0 || mp[c].cbase - mp[c].nbase + 1 > 1) atomicOr(status, 2);
+	keys[i] = particle_key(g, s, mp);
+}
+
+// Apply the sort permutation to the particle arrays.
+__global__ void k_permute(ParticleArrays src, ParticleArrays dst, const uint32_t* __restrict__ perm) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= src.count) return;
+	long long s = perm[i];
+	for (int c = 0; c < 3; ++c) {
+		dst.x[c][i] = src.x[c][s];
+		dst.p[c][i] = src.p[c][s];
+	}
+	dst.id[i] = src.id[s];
+}
+
+// start[key] = first sorted index with that key; start[nkeys] = count.
+__global__ void k_segments(const uint32_t* __restrict__ keys, long long count, long long nkeys, int* __restrict__ start) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i > count) return;
+	long long prev = (i == 0) ? -1 : (long long)keys[i - 1];
+	long long cur = (i == count) ? nkeys : (long long)keys[i];
+	for (long long k = prev + 1; k <= cur; ++k) start[k] = (int)i;
+}
+
+// ------------------------------------------------------------------ post-push density
+// updateDensityParameters (PIC++/momentum_collecting.cpp:760-1020): per dual cell and
+// species, the corner sums of concentration, energy (w gamma m), bulk momentum and charge.
+__global__ void __launch_bounds__(128) k_density(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ start,
+                                                 double* __restrict__ out) {
+	long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= g.ncells) return;
+	int bk = (int)(b % g.N[2]);
+	int bj = (int)((b / g.N[2]) % g.N[1]);
+	int bi = (int)(b / ((long long)g.N[2] * g.N[1]));
+	double rho[8];
+	for (int t = 0; t < 8; ++t) rho[t] = 0.0;
+	for (int s = 0; s < sp.count; ++s) {
+		double m = sp.mass[s], q = sp.charge[s], w = sp.weight[s];
+		double pc = q * w;
+		double conc[8], en[8], bulk[8][3];
+		for (int t = 0; t < 8; ++t) {
+			conc[t] = en[t] = 0.0;
+			bulk[t][0] = bulk[t][1] = bulk[t][2] = 0.0;
+		}
+		for (int seg = 0; seg < 8; ++seg) {
+			int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
+			int mi = bi + ax, mj = bj + ay, mk = bk + az;
+			if (mi >= g.N[0] || mj >= g.N[1] || mk >= g.N[2]) continue;
+			long long key = (((long long)s * g.ncells + cell_id(g, mi, mj, mk)) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) | (1 - az);
+			int lo = start[key], hi = start[key + 1];
+			for (int i = lo; i < hi; ++i) {
+				double x[3], p[3];
+				load_particle(P, i, x, p);
+				AxisMap mp[3];
+				maps3(g, x, mp);
+				double v[3], gm;
+				velocity_gamma(p, m, v, gm);
+				double full = gm * m;
+#pragma unroll
+				for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+					for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+						for (int tk = 0; tk < 2; ++tk) {
+							int t = ti * 4 + tj * 2 + tk;
+							double corr = mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk] / g.volume;
+							for (int c = 0; c < 3; ++c) bulk[t][c] += p[c] * w * corr;
+							rho[t] += pc * corr;
+							conc[t] += w * corr;
+							en[t] += w * full * corr;
+						}
+			}
+		}
+		long long base = (long long)s * 5 * 8;
+		for (int t = 0; t < 8; ++t) {
+			out[((base + 0 * 8 + t)) * g.ncells + b] = conc[t];
+			out[((base + 1 * 8 + t)) * g.ncells + b] = en[t];
+			for (int c = 0; c < 3; ++c) out[((base + (2 + c) * 8 + t)) * g.ncells + b] = bulk[t][c];
+		}
+	}
+	long long base = (long long)sp.count * 5 * 8;
+	for (int t = 0; t < 8; ++t) out[(base + t) * g.ncells + b] = rho[t];
+}
+
+// Cell values of the density quantities: nq planes of 8 corner sums -> nq cell planes.
+__global__ void k_density_assemble(Grid g, const double* __restrict__ cs, int nq, double* __restrict__ out) {
+	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.ncells) return;
+	int k = (int)(c % g.N[2]);
+	int j = (int)((c / g.N[2]) % g.N[1]);
+	int i = (int)(c / ((long long)g.N[2] * g.N[1]));
+	for (int q = 0; q < nq; ++q) {
+		double s = 0.0;
+		for (int a = 0; a < 2; ++a)
+			for (int b = 0; b < 2; ++b)
+				for (int cc = 0; cc < 2; ++cc) {
+					int bi = i - a, bj = j - b, bk = k - cc;
+					if (bi < 0 || bj < 0 || bk < 0) continue;
+					s += cs[((long long)q * 8 + a * 4 + b * 2 + cc) * g.ncells + cell_id(g, bi, bj, bk)];
+				}
+		out[(long long)q * g.ncells + c] = s;
+	}
+}
+
+// bulk velocity = bulk momentum / energy, or 0 (momentum_collecting.cpp:991-1008)
+__global__ void k_bulk_divide(Grid g, double* __restrict__ dens, int nspecies) {
+	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.ncells) return;
+	for (int s = 0; s < nspecies; ++s) {
+		double en = dens[((long long)s * 5 + 1) * g.ncells + c];
+		for (int a = 0; a < 3; ++a) {
+			double* bv = &dens[((long long)s * 5 + 2 + a) * g.ncells + c];
+			*bv = en > 0 ? *bv / en : 0.0;
+		}
+	}
+}
+
+// ------------------------------------------------------------------ energy (updateEnergy, simulation.cpp:1506-1640)
+// partial sums per block: [0] E energy, [1] B energy, [2..4] field momentum
+__global__ void __launch_bounds__(256) k_field_energy(Grid g, const double* __restrict__ E, const double* __restrict__ B,
+                                                      double* __restrict__ partial) {
+	__shared__ double red[5][256];
+	double acc[5] = {0, 0, 0, 0, 0};
+	for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < g.nown; o += (long long)gridDim.x * blockDim.x) {
+		int k = (int)(o % g.n[2]) + 2;
+		int j = (int)((o / g.n[2]) % g.n[1]) + 2;
+		int i = (int)(o / ((long long)g.n[2] * g.n[1])) + 2;
+		long long nid = node_id(g, i, j, k), cid = cell_id(g, i, j, k);
+		double e[3], b[3], ea[3];
+		for (int c = 0; c < 3; ++c) {
+			e[c] = E[c * g.nnodes + nid];
+			b[c] = B[c * g.ncells + cid];
+			ea[c] = (E[c * g.nnodes + node_id(g, i, j, k)] + E[c * g.nnodes + node_id(g, i, j, k + 1)] +
+			         E[c * g.nnodes + node_id(g, i, j + 1, k)] + E[c * g.nnodes + node_id(g, i, j + 1, k + 1)] +
+			         E[c * g.nnodes + node_id(g, i + 1, j, k)] + E[c * g.nnodes + node_id(g, i + 1, j, k + 1)] +
+			         E[c * g.nnodes + node_id(g, i + 1, j + 1, k)] + E[c * g.nnodes + node_id(g, i + 1, j + 1, k + 1)]) / 8;
+		}
+		acc[0] += (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * g.volume / (8 * kPi);
+		acc[1] += (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]) * g.volume / (8 * kPi);
+		double cr[3];
+		cross(ea, b, cr);
+		for (int c = 0; c < 3; ++c) acc[2 + c] += (cr[c] / (4 * kPi)) * g.volume;
+	}
+	for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = acc[q];
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w)
+			for (int q = 0; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		for (int q = 0; q < 5; ++q) partial[q * gridDim.x + blockIdx.x] = red[q][0];
+}
+
+// partial sums per block: [0] kinetic energy sum w (gamma - 1) m, [1..3] particle momentum
+__global__ void __launch_bounds__(256) k_particle_energy(SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start,
+                                                         double* __restrict__ partial) {
+	__shared__ double red[4][256];
+	double acc[4] = {0, 0, 0, 0};
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P.count; i += (long long)gridDim.x * blockDim.x) {
+		int s = 0;
+		while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+		double m = sp.mass[s], w = sp.weight[s];
+		double p[3] = {P.p[0][i], P.p[1][i], P.p[2][i]};
+		double p2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+		double gm = sqrt(p2 / (m * m) + 1.0);
+		acc[0] += ((gm - 1) * m) * w;
+		for (int c = 0; c < 3; ++c) acc[1 + c] += p[c] * w;
+	}
+	for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = acc[q];
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w)
+			for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		for (int q = 0; q < 4; ++q) partial[q * gridDim.x + blockIdx.x] = red[q][0];
+}
+
+// generic deterministic final reduce of `rows` rows of `n` partials each
+__global__ void __launch_bounds__(256) k_rows_final(const double* __restrict__ partial, int n, int rows, double* __restrict__ out) {
+	__shared__ double red[256];
+	int r = blockIdx.x;
+	if (r >= rows) return;
+	double s = 0.0;
+	for (int t = threadIdx.x; t < n; t += blockDim.x) s += partial[(long long)r * n + t];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) out[r] = red[0];
+}
+
+}  // namespace picgpu
+
+// ------------------------------------------------------------------ integer helpers
+namespace picgpu {
+
+__global__ void k_count_keys(const uint32_t* __restrict__ keys, long long count, int* __restrict__ counts) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= count) return;
+	atomicAdd(&counts[keys[i]], 1);
+}
+
+__global__ void k_iota(uint32_t* __restrict__ v, long long count) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= count) return;
+	v[i] = (uint32_t)i;
+}
+
+__global__ void k_add(double* __restrict__ x, const double* __restrict__ d, long long len) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < len) x[t] += d[t];
+}
+
+__global__ void k_sub(double* __restrict__ r, const double* __restrict__ b, const double* __restrict__ y, long long len) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < len) r[t] = b[t] - y[t];
+}
+
+// ------------------------------------------------------------------ launch wrappers
+namespace {
+inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+}  // namespace
+
+void launch_node_moments(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* start, const double* E,
+                         const double* B, double dt, double theta, double* scratch, hipStream_t s) {
+	hipLaunchKernelGGL(k_node_moments, dim3(blocks(g.ncells, 128)), dim3(128), 0, s, g, sp, P, start, E, B, dt, theta, scratch);
+}
+void launch_node_assemble(const Grid& g, const double* scratch, double* J, double* mu, hipStream_t s) {
+	hipLaunchKernelGGL(k_node_assemble, dim3(blocks(g.nnodes, 256)), dim3(256), 0, s, g, scratch, J, mu);
+}
+void launch_cell_moments(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* start, const double* E,
+                         const double* B, double dt, double* rho_scratch, double* pi_scratch, hipStream_t s) {
+	hipLaunchKernelGGL(k_cell_moments, dim3(blocks(g.ncells, 128)), dim3(128), 0, s, g, sp, P, start, E, B, dt, rho_scratch,
+	                   pi_scratch);
+}
+void launch_cell_assemble(const Grid& g, const double* rho_scratch, double* rho, hipStream_t s) {
+	hipLaunchKernelGGL(k_cell_assemble, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, rho_scratch, rho);
+}
+void launch_pressure_reduce(const Grid& g, const double* pi_scratch, double* Pi, hipStream_t s) {
+	hipLaunchKernelGGL(k_pressure_reduce, dim3(g.N[0] * 4), dim3(256), 0, s, g, pi_scratch, Pi);
+}
+void launch_fold_nodes(const Grid& g, double* f, int ncomp, hipStream_t s) {
+	for (int axis = 0; axis < 3; ++axis) {
+		int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
+		long long total = (long long)(g.N[o1] + 1) * (g.N[o2] + 1) * 5 * ncomp;
+		hipLaunchKernelGGL(k_fold, dim3(blocks(total, 256)), dim3(256), 0, s, g, f, ncomp, axis, 1, 5, g.N[axis], 4);
+	}
+}
+void launch_fold_cells(const Grid& g, double* f, int ncomp, int pairs, int lo0, hipStream_t s) {
+	for (int axis = 0; axis < 3; ++axis) {
+		int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
+		long long total = (long long)g.N[o1] * g.N[o2] * pairs * ncomp;
+		hipLaunchKernelGGL(k_fold, dim3(blocks(total, 256)), dim3(256), 0, s, g, f, ncomp, axis, 0, pairs, g.N[axis] - 1, lo0);
+	}
+}
+void launch_ghost_fill(const Grid& g, double* f, int ncomp, bool nodes, hipStream_t s) {
+	hipLaunchKernelGGL(k_ghost_fill, dim3(blocks(nodes ? g.nnodes : g.ncells, 256)), dim3(256), 0, s, g, f, ncomp, nodes ? 1 : 0);
+}
+void launch_div_pressure_update_j(const Grid& g, const double* Pi, double* J, double* divPi, double eta, double dt, hipStream_t s) {
+	hipLaunchKernelGGL(k_div_pressure_update_j, dim3(blocks(g.nnodes, 256)), dim3(256), 0, s, g, Pi, J, divPi, eta, dt);
+}
+void launch_div_flux_update_rho(const Grid& g, const double* J, double* rho, double dt, double theta, hipStream_t s) {
+	hipLaunchKernelGGL(k_div_flux_update_rho, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, J, rho, dt, theta);
+}
+void launch_rhs(const Grid& g, const double* E, const double* B, const double* J, const double* rho, double theta, double dt,
+                double* rhs, hipStream_t s) {
+	hipLaunchKernelGGL(k_rhs, dim3(blocks(g.nown, 256)), dim3(256), 0, s, g, E, B, J, rho, theta, dt, rhs);
+}
+void launch_pack_mu(const Grid& g, const double* mu, double* muc, hipStream_t s) {
+	hipLaunchKernelGGL(k_pack_mu, dim3(blocks(g.nown, 256)), dim3(256), 0, s, g, mu, muc);
+}
+void launch_apply_operator(const Grid& g, const double* muc, const double* v, double C, double* out, hipStream_t s) {
+	hipLaunchKernelGGL(k_apply_operator, dim3(blocks(g.nown, 256)), dim3(256), 0, s, g, muc, v, C, out);
+}
+void launch_dot(const double* a, const double* b, long long len, double* partial, double* out, bool do_sqrt, hipStream_t s) {
+	hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(256), 0, s, a, b, len, partial);
+	hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s, partial, kDotBlocks, out, do_sqrt ? 1 : 0);
+}
+void launch_axpy_dev(double* w, const double* v, const double* h, long long len, hipStream_t s) {
+	hipLaunchKernelGGL(k_axpy_dev, dim3(blocks(len, 256)), dim3(256), 0, s, w, v, h, len);
+}
+void launch_normalize_dev(double* w, const double* h, long long len, hipStream_t s) {
+	hipLaunchKernelGGL(k_normalize_dev, dim3(blocks(len, 256)), dim3(256), 0, s, w, h, len);
+}
+void launch_div_copy(double* b, double* v0, const double* norm, long long len, hipStream_t s) {
+	hipLaunchKernelGGL(k_div_copy, dim3(blocks(len, 256)), dim3(256), 0, s, b, v0, norm, len);
+}
+void launch_combine(double* x, const double* const* V, const double* y, int m, const double* norm, long long len, hipStream_t s) {
+	hipLaunchKernelGGL(k_combine, dim3(blocks(len, 256)), dim3(256), 0, s, x, V, y, m, norm, len);
+}
+void launch_add(double* x, const double* d, long long len, hipStream_t s) {
+	hipLaunchKernelGGL(k_add, dim3(blocks(len, 256)), dim3(256), 0, s, x, d, len);
+}
+void launch_sub(double* r, const double* b, const double* y, long long len, hipStream_t s) {
+	hipLaunchKernelGGL(k_sub, dim3(blocks(len, 256)), dim3(256), 0, s, r, b, y, len);
+}
+void launch_unpack_e(const Grid& g, const double* x, const double* E, double theta, double* Etheta, double* Enew, hipStream_t s) {
+	hipLaunchKernelGGL(k_unpack_e, dim3(blocks(g.nnodes, 256)), dim3(256), 0, s, g, x, E, theta, Etheta, Enew);
+}
+void launch_update_b(const Grid& g, const double* Etheta, const double* B, double dt, double* Bnew, hipStream_t s) {
+	hipLaunchKernelGGL(k_update_b, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, Etheta, B, dt, Bnew);
+}
+void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const double* E,
+                 const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, hipStream_t s) {
+	if (P.count == 0) return;
+	hipLaunchKernelGGL(k_push, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, E, Etheta, B, dt, eta, keys,
+	                   status);
+}
+void launch_keys(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, uint32_t* keys,
+                 int* status, hipStream_t s) {
+	if (P.count == 0) return;
+	hipLaunchKernelGGL(k_keys, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, keys, status);
+}
+void launch_permute(const ParticleArrays& src, const ParticleArrays& dst, const uint32_t* perm, hipStream_t s) {
+	if (src.count == 0) return;
+	hipLaunchKernelGGL(k_permute, dim3(blocks(src.count, 256)), dim3(256), 0, s, src, dst, perm);
+}
+void launch_count_keys(const uint32_t* keys, long long count, int* counts, hipStream_t s) {
+	if (count == 0) return;
+	hipLaunchKernelGGL(k_count_keys, dim3(blocks(count, 256)), dim3(256), 0, s, keys, count, counts);
+}
+void launch_iota(uint32_t* v, long long count, hipStream_t s) {
+	if (count == 0) return;
+	hipLaunchKernelGGL(k_iota, dim3(blocks(count, 256)), dim3(256), 0, s, v, count);
+}
+void launch_density(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* start, double* scratch,
+                    hipStream_t s) {
+	hipLaunchKernelGGL(k_density, dim3(blocks(g.ncells, 128)), dim3(128), 0, s, g, sp, P, start, scratch);
+}
+void launch_density_assemble(const Grid& g, const double* scratch, int nq, double* out, hipStream_t s) {
+	hipLaunchKernelGGL(k_density_assemble, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, scratch, nq, out);
+}
+void launch_bulk_divide(const Grid& g, double* dens, int nspecies, hipStream_t s) {
+	hipLaunchKernelGGL(k_bulk_divide, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, dens, nspecies);
+}
+void launch_field_energy(const Grid& g, const double* E, const double* B, double* partial, hipStream_t s) {
+	hipLaunchKernelGGL(k_field_energy, dim3(kEnergyBlocks), dim3(256), 0, s, g, E, B, partial);
+}
+void launch_particle_energy(const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, double* partial,
+                            hipStream_t s) {
+	hipLaunchKernelGGL(k_particle_energy, dim3(kEnergyBlocks), dim3(256), 0, s, sp, P, species_start, partial);
+}
+void launch_rows_final(const double* partial, int n, int rows, double* out, hipStream_t s) {
+	hipLaunchKernelGGL(k_rows_final, dim3(rows), dim3(256), 0, s, partial, n, rows, out);
+}
+
+}  // namespace picgpu

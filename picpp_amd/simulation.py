@@ -1,0 +1,196 @@
+"""Simulation-shaped host driver over the C ABI.
+
+Mirrors the reference's `Simulation` call surface for the implicit hot path
+(PIC++/simulation.cpp:205-390): the method names below are the reference's, and
+they run in the reference's order inside `step()`. State lives on the GPU; host
+copies are made only by the explicit accessors.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _lib
+from .workloads import InitialState, Workload, species_arrays
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+PRECISIONS = {"fp64": _lib.PRECISION_FP64, "fp32": _lib.PRECISION_FP32}
+STOP_RULES = {"reference": _lib.STOP_REFERENCE, "true_residual": _lib.STOP_TRUE_RESIDUAL}
+ORTHOS = {"mgs": _lib.ORTHO_MGS, "cgs2": _lib.ORTHO_CGS2}
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+class Simulation:
+    def __init__(self, workload: Workload, precision: str = "fp64", stop_rule: str = "reference", ortho: str = "mgs",
+                 capacity: Optional[int] = None, gmres_tolerance: Optional[float] = None,
+                 gmres_max_iterations: int = 50, stream: Optional[int] = None):
+        self.lib = _lib.load()
+        self.w = workload
+        cfg = _lib.PicConfig()
+        cfg.nx, cfg.ny, cfg.nz = workload.n
+        cfg.dx = cfg.dy = cfg.dz = workload.dx
+        cfg.dt = workload.dt
+        m, q, wt, _ = species_arrays(workload)
+        cfg.nspecies = len(m)
+        for s in range(len(m)):
+            cfg.mass[s], cfg.charge[s], cfg.weight[s] = m[s], q[s], wt[s]
+        cfg.capacity = int(capacity if capacity is not None else workload.particles_total)
+        cfg.precision = PRECISIONS[precision]
+        cfg.stop_rule = STOP_RULES[stop_rule]
+        cfg.ortho = ORTHOS[ortho]
+        cfg.gmres_tolerance = workload.gmres_tolerance if gmres_tolerance is None else gmres_tolerance
+        cfg.gmres_max_iterations = gmres_max_iterations
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.pic_create(ctypes.byref(cfg), ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)),
+                   None, "pic_create")
+        self.h = h
+        self.N = tuple(v + 4 for v in workload.n)
+        self.last_solve: Dict[str, float] = {}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.pic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        return _lib.check(rc, self.h, what)
+
+    # ------------------------------------------------------------ state transfer
+    def load_state(self, st: InitialState):
+        E = np.ascontiguousarray(np.moveaxis(st.E, -1, 0), np.float64)
+        B = np.ascontiguousarray(np.moveaxis(st.B, -1, 0), np.float64)
+        self._chk(self.lib.pic_upload_fields(self.h, _dp(E), _dp(B)), "pic_upload_fields")
+        sp = np.ascontiguousarray(st.species, np.int32)
+        ids = np.ascontiguousarray(st.ids, np.int32)
+        x = np.ascontiguousarray(st.x.T, np.float64)
+        p = np.ascontiguousarray(st.p.T, np.float64)
+        self._chk(self.lib.pic_upload_particles(self.h, len(sp), _ip(sp), _ip(ids), _dp(x), _dp(p)),
+                  "pic_upload_particles")
+
+    def num_particles(self) -> int:
+        return int(self.lib.pic_num_particles(self.h))
+
+    def particles(self) -> Dict[str, np.ndarray]:
+        n = self.num_particles()
+        sp = np.zeros(n, np.int32)
+        ids = np.zeros(n, np.int32)
+        x = np.zeros((3, n))
+        p = np.zeros((3, n))
+        self._chk(self.lib.pic_download_particles(self.h, _ip(sp), _ip(ids), _dp(x), _dp(p)), "pic_download_particles")
+        return dict(species=sp, ids=ids, x=x.T.copy(), p=p.T.copy())
+
+    def array(self, name: str) -> np.ndarray:
+        which = _lib.ARRAYS[name]
+        n = self.lib.pic_download_array(self.h, which, None)
+        self._chk(n, "pic_download_array")
+        out = np.zeros(n)
+        self._chk(self.lib.pic_download_array(self.h, which, _dp(out)), "pic_download_array")
+        N = self.N
+        nodes = (N[0] + 1) * (N[1] + 1) * (N[2] + 1)
+        cells = N[0] * N[1] * N[2]
+        own = int(np.prod(self.w.n))
+        if name in ("rhs", "solution"):
+            return np.moveaxis(out.reshape(3, *self.w.n), 0, -1)
+        if n % nodes == 0 and name in ("E", "Etheta", "Enew", "J", "divPi", "mu"):
+            c = n // nodes
+            a = np.moveaxis(out.reshape(c, N[0] + 1, N[1] + 1, N[2] + 1), 0, -1)
+            return a.reshape(a.shape[:3] + (3, 3)) if c == 9 else a
+        c = n // cells
+        a = np.moveaxis(out.reshape(c, *N), 0, -1)
+        if name == "Pi":
+            return a.reshape(a.shape[:3] + (3, 3))
+        if c == 1:
+            return a[..., 0]
+        return a
+
+    def device_array(self, name: str):
+        ptr = _D()
+        cnt = ctypes.c_longlong()
+        self._chk(self.lib.pic_device_array(self.h, _lib.ARRAYS[name], ctypes.byref(ptr), ctypes.byref(cnt)),
+                  "pic_device_array")
+        return ctypes.cast(ptr, ctypes.c_void_p).value, cnt.value
+
+    # ------------------------------------------------------------ reference phases
+    def initialize(self):
+        """Pre-loop sequence (PIC++/simulation.cpp:86-123)."""
+        self._chk(self.lib.pic_initialize(self.h), "pic_initialize")
+
+    def updateElectroMagneticParameters(self):
+        """evaluateParticlesRotationTensor + updateElectroMagneticParameters
+        (PIC++/mover.cpp:810-841, PIC++/momentum_collecting.cpp:365-758)."""
+        self._chk(self.lib.pic_rotation_and_moments(self.h), "pic_rotation_and_moments")
+
+    evaluateParticlesRotationTensor = updateElectroMagneticParameters
+
+    def evaluateMaxwellEquationMatrix(self):
+        """Right part of the Maxwell system (PIC++/fields.cpp:441-600); the operator is matrix-free."""
+        self._chk(self.lib.pic_field_rhs(self.h), "pic_field_rhs")
+
+    def evaluateElectricField(self) -> Dict[str, float]:
+        """GMRES solve for E^{n+theta} and newE (PIC++/fields.cpp:152-376)."""
+        info = _lib.PicSolveInfo()
+        self._chk(self.lib.pic_maxwell_solve(self.h, ctypes.byref(info)), "pic_maxwell_solve")
+        self.last_solve = dict(iterations=info.iterations, restarts=info.restarts,
+                               estimated_residual=info.estimated_residual, true_residual=info.true_residual,
+                               seconds=info.seconds)
+        return self.last_solve
+
+    def evaluateMagneticField(self):
+        """newB = B - c' dt curl E^{n+theta} (PIC++/fields.cpp:2717-2773)."""
+        self._chk(self.lib.pic_update_b(self.h), "pic_update_b")
+
+    def moveParticles(self):
+        """Mover + periodic exchange + updateDensityParameters (PIC++/mover.cpp:21-55, :1002-1210,
+        PIC++/momentum_collecting.cpp:760-1020)."""
+        self._chk(self.lib.pic_push(self.h), "pic_push")
+
+    def updateFields(self):
+        """E <- newE, B <- newB (PIC++/fields.cpp:422-439)."""
+        self._chk(self.lib.pic_commit_fields(self.h), "pic_commit_fields")
+
+    def updateEnergy(self) -> np.ndarray:
+        """[particle, E, B, total energy, momentum x, y, z] (PIC++/simulation.cpp:1506-1640)."""
+        out = np.zeros(7)
+        self._chk(self.lib.pic_energy(self.h, _dp(out)), "pic_energy")
+        return out
+
+    def step(self) -> Dict[str, float]:
+        info = _lib.PicSolveInfo()
+        self._chk(self.lib.pic_step(self.h, ctypes.byref(info)), "pic_step")
+        self.last_solve = dict(iterations=info.iterations, restarts=info.restarts,
+                               estimated_residual=info.estimated_residual, true_residual=info.true_residual,
+                               seconds=info.seconds)
+        return self.last_solve
+
+    def synchronize(self):
+        self._chk(self.lib.pic_synchronize(self.h), "pic_synchronize")
+
+    # ------------------------------------------------------------ solver boundary
+    def apply_operator(self, v_ptr: int, y_ptr: int):
+        self._chk(self.lib.pic_apply_operator(self.h, ctypes.cast(v_ptr, _D), ctypes.cast(y_ptr, _D)),
+                  "pic_apply_operator")
+
+    def gmres_solve(self, rhs_ptr: int, x_ptr: int) -> Dict[str, float]:
+        info = _lib.PicSolveInfo()
+        self._chk(self.lib.pic_gmres_solve(self.h, ctypes.cast(rhs_ptr, _D), ctypes.cast(x_ptr, _D), ctypes.byref(info)),
+                  "pic_gmres_solve")
+        return dict(iterations=info.iterations, restarts=info.restarts, estimated_residual=info.estimated_residual,
+                    true_residual=info.true_residual, seconds=info.seconds)

@@ -1,0 +1,144 @@
+"""FP64 HIP path against the FP64 oracle (which is bit-exact against the compiled
+reference, see test_oracle_golden.py), phase by phase, through the C ABI.
+
+Bar: normwise relative error <= 1e-12 on particle state, J, rho, mu and every field
+(the GPU sums deposits per cell and reduces dot products in a tree, so only the
+summation order differs from the reference's sequential loops).
+"""
+import numpy as np
+import pytest
+
+from conftest import GoldenState, relerr
+from oracle.oracle import OracleSim
+from picpp_amd import workloads as wl
+from picpp_amd.simulation import Simulation
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def fields_match(sim, orc, names, tol=TOL):
+    for name in names:
+        g = sim.array(name)
+        o = orc.array(name)
+        assert g.shape == o.shape, (name, g.shape, o.shape)
+        e = relerr(g, o)
+        assert e <= tol, (name, e)
+
+
+def particles_match(sim, orc, tol=TOL):
+    g = sim.particles()
+    o = orc.particles()
+    gi = np.argsort(g["ids"])
+    oi = np.argsort(o["ids"])
+    assert np.array_equal(g["ids"][gi], o["ids"][oi])
+    assert np.array_equal(g["species"][gi], o["species"][oi])
+    assert relerr(g["x"][gi], o["x"][oi]) <= tol
+    assert relerr(g["p"][gi], o["p"][oi]) <= tol
+
+
+def density_match(sim, orc, nspecies, tol=TOL):
+    d = sim.array("density")
+    for s in range(nspecies):
+        assert relerr(d[..., 5 * s + 0], orc.array("conc%d" % s)) <= tol
+        assert relerr(d[..., 5 * s + 1], orc.array("energy%d" % s)) <= tol
+        assert relerr(d[..., 5 * s + 2:5 * s + 5], orc.array("bulkv%d" % s)) <= tol
+    assert relerr(d[..., 5 * nspecies], orc.array("rho")) <= tol
+
+
+def run_phases(w, st, steps):
+    print("case", w.name, "particles", len(st.species), flush=True)
+    sim = Simulation(w)
+    sim.load_state(st)
+    orc = OracleSim(w, st)
+    sim.initialize()
+    orc.initial_sequence()
+    density_match(sim, orc, len(w.species))
+    assert relerr(sim.updateEnergy(), orc.energy()) <= TOL
+    for s in range(steps):
+        sim.updateElectroMagneticParameters()
+        orc.evaluate_rotation_tensor()
+        orc.update_em_parameters()
+        fields_match(sim, orc, ("J", "mu", "rhohat", "Pi", "divPi"))
+        sim.evaluateMaxwellEquationMatrix()
+        orc.build_rhs()
+        assert relerr(sim.array("rhs"), orc.array("rhs")[2:-2, 2:-2, 2:-2]) <= TOL
+        info = sim.evaluateElectricField()
+        orc.evaluate_electric_field()
+        assert info["iterations"] == orc.gmres_iterations()
+        orc.exchange_fields()
+        fields_match(sim, orc, ("Etheta", "Enew"))
+        sim.evaluateMagneticField()
+        orc.evaluate_magnetic_field()
+        orc.exchange_fields()
+        fields_match(sim, orc, ("Bnew",))
+        sim.moveParticles()
+        orc.move_particles()
+        orc.update_density_parameters()
+        particles_match(sim, orc)
+        density_match(sim, orc, len(w.species))
+        sim.updateFields()
+        orc.exchange_fields()
+        orc.update_fields()
+        orc.exchange_fields()
+        fields_match(sim, orc, ("E", "B"))
+        orc.update_energy()
+        assert relerr(sim.updateEnergy(), orc.energy()) <= TOL
+    sim.close()
+
+
+def test_mini_golden_state_phase_by_phase(mini_golden):
+    run_phases(wl.mini(), GoldenState(mini_golden), 5)
+
+
+def test_larger_box_nonuniform_fields():
+    w = wl.mini(n=(12, 10, 8), ppc=8, seed=11)
+    run_phases(w, wl.make_initial_state(w, perturb=0.05), 3)
+
+
+def test_full_steps_track_oracle_config1_shape():
+    """32x32x8 with 4 ppc per species, 5 full pic_step calls, final state vs oracle."""
+    w = wl.config1(ppc=4, steps=5)
+    st = wl.make_initial_state(w, perturb=0.02)
+    print("case", w.name, flush=True)
+    sim = Simulation(w)
+    sim.load_state(st)
+    orc = OracleSim(w, st)
+    sim.initialize()
+    orc.initial_sequence()
+    for _ in range(w.steps):
+        sim.step()
+        orc.step(1)
+    particles_match(sim, orc)
+    fields_match(sim, orc, ("E", "B"))
+    assert relerr(sim.updateEnergy(), orc.energy()) <= TOL
+
+
+def test_deterministic_bitwise():
+    """Two runs from the same state give bitwise identical fields and particles."""
+    w = wl.mini(n=(10, 8, 6), ppc=6, seed=3)
+    st = wl.make_initial_state(w, perturb=0.05)
+    outs = []
+    for _ in range(2):
+        sim = Simulation(w)
+        sim.load_state(st)
+        sim.initialize()
+        for _ in range(3):
+            sim.step()
+        outs.append((sim.array("E"), sim.array("B"), sim.array("J"), sim.array("mu"), sim.particles()["x"]))
+        sim.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_true_residual_stop_rule():
+    w = wl.mini(n=(12, 10, 8), ppc=8, seed=5)
+    st = wl.make_initial_state(w, perturb=0.05)
+    sim = Simulation(w, stop_rule="true_residual")
+    sim.load_state(st)
+    sim.initialize()
+    sim.updateElectroMagneticParameters()
+    info = sim.evaluateElectricField()
+    assert 0 <= info["true_residual"] <= 1e-8
+    sim.close()
