@@ -46,6 +46,15 @@ enum {
 	PIC_STOP_TRUE_RESIDUAL = 1
 };
 
+/* arithmetic of the Maxwell solve */
+enum {
+	PIC_SOLVER_FP64 = 0,  /* FP64 GMRES on the reference's 37-entry rows */
+	/* FP32 inner GMRES (FP32 basis and operator, FP64 dot accumulation and Hessenberg least
+	   squares) inside FP64 iterative refinement on the true residual; implies
+	   PIC_STOP_TRUE_RESIDUAL */
+	PIC_SOLVER_MIXED = 1
+};
+
 /* orthogonalisation inside the Arnoldi process */
 enum {
 	PIC_ORTHO_MGS = 0, /* modified Gram-Schmidt, one dot + axpy per basis vector (PIC++/specialmath.cpp:295-332) */
@@ -66,6 +75,7 @@ typedef struct {
 	int ortho;            /* PIC_ORTHO_* */
 	double gmres_tolerance; /* maxErrorLevel (PIC++/constants.h:31), 1e-8 */
 	int gmres_max_iterations; /* maxGMRESIterations (PIC++/constants.h:55), 50 */
+	int solver_precision;   /* PIC_SOLVER_* */
 } pic_config;
 
 typedef struct {

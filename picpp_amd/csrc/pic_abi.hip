@@ -13,6 +13,8 @@
 #include "../../include/picpp_amd.h"
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
+#include "pic_kernels_fp32.hpp"
+#include "pic_solver.hpp"
 
 using namespace picgpu;
 
@@ -55,8 +57,12 @@ struct pic_ctx {
 	double *J = nullptr, *mu = nullptr, *divPi = nullptr, *rhohat = nullptr, *Pi = nullptr, *dens = nullptr;
 	// scratch of the deterministic deposits
 	double *node_scratch = nullptr, *rho_scratch = nullptr, *pi_scratch = nullptr, *dens_scratch = nullptr;
-	// particles
+	// particles (FP64 form, or the FP32 cell + offset form when fp32)
+	bool fp32 = false;
 	ParticleArrays P{}, Palt{};
+	ParticleArrays32 Q{}, Qalt{};
+	float *E32 = nullptr, *Et32 = nullptr, *B32 = nullptr;
+	float *nscr32 = nullptr, *rscr32 = nullptr, *pscr32 = nullptr, *dscr32 = nullptr;
 	long long capacity = 0;
 	uint32_t *keys = nullptr, *keys_sorted = nullptr, *perm = nullptr, *perm_sorted = nullptr;
 	int *start = nullptr, *counts = nullptr;
@@ -72,6 +78,11 @@ struct pic_ctx {
 	std::vector<double*> V;
 	double** Vdev = nullptr;
 	int max_basis = 0;
+	// mixed-precision solve: FP32 basis, operator and scratch; FP64 operator scratch
+	std::vector<float*> V32;
+	float** V32dev = nullptr;
+	float *mu32 = nullptr, *w32 = nullptr;
+	double* w64 = nullptr;
 	// energies
 	double* epartial = nullptr;
 	double* eout = nullptr;
@@ -92,6 +103,7 @@ struct pic_ctx {
 	std::map<std::string, Stat> prof;
 
 	long long nown3() const { return 3 * g.nown; }
+	long long np() const { return fp32 ? Q.count : P.count; }
 };
 
 namespace {
@@ -154,7 +166,7 @@ void check_status(pic_ctx* c) {
 
 // Sort particles by key (stable radix sort: deterministic), rebuild start[].
 void sort_particles(pic_ctx* c) {
-	long long n = c->P.count;
+	long long n = c->np();
 	hipStream_t s = c->stream;
 	if (n > 0) {
 		{
@@ -164,11 +176,15 @@ void sort_particles(pic_ctx* c) {
 			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_temp, bytes, c->keys, c->keys_sorted, c->perm, c->perm_sorted, (int)n,
 			                                             0, c->end_bit, s));
 		}
-		{
+		if (c->fp32) {
+			Timed t(c, "permute", (double)n * (2 * 32 + 4));
+			launch_permute32(c->Q, c->Qalt, c->perm_sorted, s);
+			std::swap(c->Q, c->Qalt);
+		} else {
 			Timed t(c, "permute", (double)n * (2 * 52 + 4));
 			launch_permute(c->P, c->Palt, c->perm_sorted, s);
+			std::swap(c->P, c->Palt);
 		}
-		std::swap(c->P, c->Palt);
 	}
 	Timed t(c, "segments", (double)n * 4 + (double)(c->nkeys + 1) * 12);
 	HIP_CHECK(hipMemsetAsync(c->counts, 0, (c->nkeys + 1) * sizeof(int), s));
@@ -180,12 +196,19 @@ void sort_particles(pic_ctx* c) {
 void density(pic_ctx* c) {
 	hipStream_t s = c->stream;
 	int nq = c->sp.count * 5 + 1;
-	double np = (double)c->P.count;
-	{
-		Timed t(c, "density", np * 48 + (double)c->g.ncells * nq * 8);
-		launch_density(c->g, c->sp, c->P, c->start, c->dens_scratch, s);
-	}
-	{
+	double np = (double)c->np();
+	if (c->fp32) {
+		{
+			Timed t(c, "density", np * 28 + (double)c->g.ncells * nq * 8);
+			launch_density32(c->g, c->sp, c->Q, c->start, c->dscr32, s);
+		}
+		Timed t(c, "density_assemble", (double)c->g.ncells * nq * (8 * 4 + 8));
+		launch_cell_assemble32(c->g, c->dscr32, nq, c->dens, s);
+	} else {
+		{
+			Timed t(c, "density", np * 48 + (double)c->g.ncells * nq * 8);
+			launch_density(c->g, c->sp, c->P, c->start, c->dens_scratch, s);
+		}
 		Timed t(c, "density_assemble", (double)c->g.ncells * nq * 8 * 9);
 		launch_density_assemble(c->g, c->dens_scratch, nq, c->dens, s);
 	}
@@ -199,27 +222,56 @@ void density(pic_ctx* c) {
 void moments(pic_ctx* c) {
 	hipStream_t s = c->stream;
 	const Grid& g = c->g;
-	double np = (double)c->P.count;
-	{
-		Timed t(c, "node_moments", np * 48 + (double)g.nnodes * 12 * 8);
-		launch_node_moments(g, c->sp, c->P, c->start, c->E, c->B, c->cfg.dt, c->theta, c->node_scratch, s);
-	}
-	{
-		Timed t(c, "node_assemble", (double)g.ncells * 96 * 8 + (double)g.nnodes * 12 * 8);
-		launch_node_assemble(g, c->node_scratch, c->J, c->mu, s);
-	}
-	{
-		Timed t(c, "cell_moments", np * 48 + (double)g.ncells * 10 * 8);
-		launch_cell_moments(g, c->sp, c->P, c->start, c->E, c->B, c->cfg.dt, c->rho_scratch, c->pi_scratch, s);
-	}
-	{
-		Timed t(c, "cell_assemble", (double)g.ncells * 9 * 8);
-		launch_cell_assemble(g, c->rho_scratch, c->rhohat, s);
-	}
-	{
-		Timed t(c, "pressure_reduce", (double)g.ncells * 48 * 8);
-		HIP_CHECK(hipMemsetAsync(c->Pi, 0, 9 * g.ncells * sizeof(double), s));
-		launch_pressure_reduce(g, c->pi_scratch, c->Pi, s);
+	double np = (double)c->np();
+	if (c->fp32) {
+		{
+			Timed t(c, "fields_to_fp32", (double)(g.nnodes + g.ncells) * 3 * 12);
+			launch_f64_to_f32(c->E, c->E32, 3 * g.nnodes, s);
+			launch_f64_to_f32(c->B, c->B32, 3 * g.ncells, s);
+		}
+		{
+			Timed t(c, "node_moments", np * 28 + (double)g.nnodes * 12 * 4);
+			launch_node_moments32(g, c->sp, c->Q, c->start, c->E32, c->B32, (float)c->cfg.dt, (float)c->theta, c->nscr32, s);
+		}
+		{
+			Timed t(c, "node_assemble", (double)g.ncells * 96 * 4 + (double)g.nnodes * 12 * 8);
+			launch_node_assemble32(g, c->nscr32, c->J, c->mu, s);
+		}
+		{
+			Timed t(c, "cell_moments", np * 28 + (double)g.ncells * 10 * 4);
+			launch_cell_moments32(g, c->sp, c->Q, c->start, c->E32, c->B32, (float)c->cfg.dt, c->rscr32, c->pscr32, s);
+		}
+		{
+			Timed t(c, "cell_assemble", (double)g.ncells * (8 * 4 + 8));
+			launch_cell_assemble32(g, c->rscr32, 1, c->rhohat, s);
+		}
+		{
+			Timed t(c, "pressure_reduce", (double)g.ncells * 48 * 4);
+			HIP_CHECK(hipMemsetAsync(c->Pi, 0, 9 * g.ncells * sizeof(double), s));
+			launch_pressure_reduce32(g, c->pscr32, c->Pi, s);
+		}
+	} else {
+		{
+			Timed t(c, "node_moments", np * 48 + (double)g.nnodes * 12 * 8);
+			launch_node_moments(g, c->sp, c->P, c->start, c->E, c->B, c->cfg.dt, c->theta, c->node_scratch, s);
+		}
+		{
+			Timed t(c, "node_assemble", (double)g.ncells * 96 * 8 + (double)g.nnodes * 12 * 8);
+			launch_node_assemble(g, c->node_scratch, c->J, c->mu, s);
+		}
+		{
+			Timed t(c, "cell_moments", np * 48 + (double)g.ncells * 10 * 8);
+			launch_cell_moments(g, c->sp, c->P, c->start, c->E, c->B, c->cfg.dt, c->rho_scratch, c->pi_scratch, s);
+		}
+		{
+			Timed t(c, "cell_assemble", (double)g.ncells * 9 * 8);
+			launch_cell_assemble(g, c->rho_scratch, c->rhohat, s);
+		}
+		{
+			Timed t(c, "pressure_reduce", (double)g.ncells * 48 * 8);
+			HIP_CHECK(hipMemsetAsync(c->Pi, 0, 9 * g.ncells * sizeof(double), s));
+			launch_pressure_reduce(g, c->pi_scratch, c->Pi, s);
+		}
 	}
 	Timed tf(c, "folds", 0);
 	launch_fold_cells(g, c->rhohat, 1, 4, 3, s);  // sumChargeDensityHat{X,Y,Z}
@@ -379,7 +431,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	if (cfg->nx < 5 || cfg->ny < 5 || cfg->nz < 5 || cfg->nspecies < 1 || cfg->nspecies > kMaxSpecies || cfg->dt <= 0 ||
 	    cfg->dx <= 0 || cfg->dy <= 0 || cfg->dz <= 0 || cfg->capacity < 0)
 		return PIC_ERR_ARGUMENT;
-	if (cfg->precision != PIC_PRECISION_FP64) return PIC_ERR_ARGUMENT;
+	if (cfg->precision != PIC_PRECISION_FP64 && cfg->precision != PIC_PRECISION_FP32) return PIC_ERR_ARGUMENT;
 	pic_ctx* c = new pic_ctx();
 	try {
 		c->cfg = *cfg;
@@ -426,10 +478,31 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->rhohat = dalloc<double>(g.ncells);
 		c->Pi = dalloc<double>(9 * g.ncells);
 		c->dens = dalloc<double>((5 * cfg->nspecies + 1) * g.ncells);
-		c->node_scratch = dalloc<double>(96 * g.ncells);
-		c->rho_scratch = dalloc<double>(8 * g.ncells);
-		c->pi_scratch = dalloc<double>(48 * g.ncells);
-		c->dens_scratch = dalloc<double>((5 * cfg->nspecies + 1) * 8 * g.ncells);
+		c->fp32 = cfg->precision == PIC_PRECISION_FP32;
+		long long pcap = c->fp32 ? 0 : cfg->capacity;  // FP64 particle arrays
+		if (c->fp32) {
+			c->nscr32 = dalloc<float>(96 * g.ncells);
+			c->rscr32 = dalloc<float>(8 * g.ncells);
+			c->pscr32 = dalloc<float>(48 * g.ncells);
+			c->dscr32 = dalloc<float>((5 * cfg->nspecies + 1) * 8 * g.ncells);
+			c->E32 = dalloc<float>(3 * g.nnodes);
+			c->Et32 = dalloc<float>(3 * g.nnodes);
+			c->B32 = dalloc<float>(3 * g.ncells);
+			for (ParticleArrays32* Q : {&c->Q, &c->Qalt}) {
+				Q->cell = dalloc<int>(cfg->capacity);
+				for (int a = 0; a < 3; ++a) {
+					Q->off[a] = dalloc<float>(cfg->capacity);
+					Q->u[a] = dalloc<float>(cfg->capacity);
+				}
+				Q->id = dalloc<int>(cfg->capacity);
+				Q->count = 0;
+			}
+		} else {
+			c->node_scratch = dalloc<double>(96 * g.ncells);
+			c->rho_scratch = dalloc<double>(8 * g.ncells);
+			c->pi_scratch = dalloc<double>(48 * g.ncells);
+			c->dens_scratch = dalloc<double>((5 * cfg->nspecies + 1) * 8 * g.ncells);
+		}
 		for (double* p : {c->E, c->Etheta, c->Enew, c->J, c->divPi}) HIP_CHECK(hipMemset(p, 0, 3 * g.nnodes * sizeof(double)));
 		HIP_CHECK(hipMemset(c->mu, 0, 9 * g.nnodes * sizeof(double)));
 		for (double* p : {c->B, c->Bnew}) HIP_CHECK(hipMemset(p, 0, 3 * g.ncells * sizeof(double)));
@@ -439,10 +512,10 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->capacity = cfg->capacity;
 		for (ParticleArrays* P : {&c->P, &c->Palt}) {
 			for (int a = 0; a < 3; ++a) {
-				P->x[a] = dalloc<double>(c->capacity);
-				P->p[a] = dalloc<double>(c->capacity);
+				P->x[a] = dalloc<double>(pcap);
+				P->p[a] = dalloc<double>(pcap);
 			}
-			P->id = dalloc<int>(c->capacity);
+			P->id = dalloc<int>(pcap);
 			P->count = 0;
 		}
 		c->keys = dalloc<uint32_t>(c->capacity);
@@ -476,6 +549,15 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		for (auto& v : c->V) v = dalloc<double>(len);
 		c->Vdev = dalloc<double*>(c->V.size());
 		HIP_CHECK(hipMemcpy(c->Vdev, c->V.data(), c->V.size() * sizeof(double*), hipMemcpyHostToDevice));
+		c->w64 = dalloc<double>(len);
+		if (c->cfg.solver_precision == PIC_SOLVER_MIXED) {
+			c->mu32 = dalloc<float>(9 * g.nown);
+			c->w32 = dalloc<float>(len);
+			c->V32.resize(c->max_basis + 1);
+			for (auto& v : c->V32) v = dalloc<float>(len);
+			c->V32dev = dalloc<float*>(c->V32.size());
+			HIP_CHECK(hipMemcpy(c->V32dev, c->V32.data(), c->V32.size() * sizeof(float*), hipMemcpyHostToDevice));
+		}
 		c->partial = dalloc<double>((size_t)kDotBlocks * 128);
 		c->dscal = dalloc<double>(256);
 		c->ydev = dalloc<double>(128);
@@ -513,7 +595,20 @@ void pic_destroy(pic_ctx* c) {
 		}
 		f(P->id);
 	}
+	for (ParticleArrays32* Q : {&c->Q, &c->Qalt}) {
+		f(Q->cell);
+		for (int a = 0; a < 3; ++a) {
+			f(Q->off[a]);
+			f(Q->u[a]);
+		}
+		f(Q->id);
+	}
+	for (void* p : {(void*)c->E32, (void*)c->Et32, (void*)c->B32, (void*)c->nscr32, (void*)c->rscr32, (void*)c->pscr32,
+	                (void*)c->dscr32})
+		f(p);
 	for (double* v : c->V) f(v);
+	for (float* v : c->V32) f(v);
+	for (void* p : {(void*)c->V32dev, (void*)c->mu32, (void*)c->w32, (void*)c->w64}) f(p);
 	for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
 	if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
 	delete c;
@@ -535,11 +630,58 @@ int pic_upload_fields(pic_ctx* c, const double* E, const double* B) {
 	return PIC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Device-resident FP64 particles (grouped by species, cnt = species prefix counts) into
+// the context's particle form, then sorted by key.
+void load_particles(pic_ctx* c, const std::vector<int>& cnt, const double* x, const double* p, const int* ids) {
+	long long np = cnt[c->sp.count];
+	if (np > c->capacity) throw DeviceError{PIC_ERR_ARGUMENT, "particle capacity exceeded"};
+	hipStream_t s = c->stream;
+	c->species_start_h.assign(kMaxSpecies + 1, (int)np);
+	for (int q = 0; q <= c->sp.count; ++q) c->species_start_h[q] = cnt[q];
+	HIP_CHECK(hipMemcpyAsync(c->species_start, c->species_start_h.data(), (kMaxSpecies + 1) * sizeof(int),
+	                         hipMemcpyHostToDevice, s));
+	int* idst = c->fp32 ? c->Q.id : c->P.id;
+	if (ids) {
+		HIP_CHECK(hipMemcpyAsync(idst, ids, np * sizeof(int), hipMemcpyDeviceToDevice, s));
+	} else {
+		launch_iota(reinterpret_cast<uint32_t*>(idst), np, s);
+	}
+	if (c->fp32) {
+		c->Q.count = np;
+		c->Qalt.count = np;
+		launch_from_fp64(c->g, c->sp, x, p, np, c->species_start, c->Q, c->status, s);
+		check_status(c);
+		launch_keys32(c->g, c->sp, c->Q, c->species_start, c->keys, s);
+	} else {
+		for (int a = 0; a < 3; ++a) {
+			HIP_CHECK(hipMemcpyAsync(c->P.x[a], x + a * np, np * sizeof(double), hipMemcpyDeviceToDevice, s));
+			HIP_CHECK(hipMemcpyAsync(c->P.p[a], p + a * np, np * sizeof(double), hipMemcpyDeviceToDevice, s));
+		}
+		c->P.count = np;
+		c->Palt.count = np;
+		launch_keys(c->g, c->sp, c->P, c->species_start, c->keys, c->status, s);
+		check_status(c);
+	}
+	sort_particles(c);
+	HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" {
+
 int pic_upload_particles(pic_ctx* c, long long np, const int* species, const int* ids, const double* x, const double* p) {
 	if (!c || np < 0 || np > c->capacity || (np > 0 && (!species || !x || !p))) {
 		if (c) c->error = "pic_upload_particles: bad arguments or capacity exceeded";
 		return PIC_ERR_ARGUMENT;
 	}
+	double* dx = nullptr;
+	double* dp = nullptr;
+	int* di = nullptr;
 	try {
 		// group by species on the host so species ranges are contiguous before the key sort
 		std::vector<long long> order(np);
@@ -551,26 +693,28 @@ int pic_upload_particles(pic_ctx* c, long long np, const int* species, const int
 		for (int s = 0; s < c->sp.count; ++s) cnt[s + 1] += cnt[s];
 		std::vector<int> pos(cnt.begin(), cnt.end() - 1);
 		for (long long i = 0; i < np; ++i) order[pos[species[i]]++] = i;
-		std::vector<double> buf(np);
+		std::vector<double> hx(3 * np), hp(3 * np);
 		std::vector<int> ib(np);
-		for (int a = 0; a < 3; ++a) {
-			for (long long i = 0; i < np; ++i) buf[i] = x[a * np + order[i]];
-			HIP_CHECK(hipMemcpy(c->P.x[a], buf.data(), np * sizeof(double), hipMemcpyHostToDevice));
-			for (long long i = 0; i < np; ++i) buf[i] = p[a * np + order[i]];
-			HIP_CHECK(hipMemcpy(c->P.p[a], buf.data(), np * sizeof(double), hipMemcpyHostToDevice));
-		}
+		for (int a = 0; a < 3; ++a)
+			for (long long i = 0; i < np; ++i) {
+				hx[a * np + i] = x[a * np + order[i]];
+				hp[a * np + i] = p[a * np + order[i]];
+			}
 		for (long long i = 0; i < np; ++i) ib[i] = ids ? ids[order[i]] : (int)order[i];
-		HIP_CHECK(hipMemcpy(c->P.id, ib.data(), np * sizeof(int), hipMemcpyHostToDevice));
-		c->P.count = np;
-		c->Palt.count = np;
-		c->species_start_h.assign(kMaxSpecies + 1, (int)np);
-		for (int s = 0; s <= c->sp.count; ++s) c->species_start_h[s] = cnt[s];
-		HIP_CHECK(hipMemcpy(c->species_start, c->species_start_h.data(), (kMaxSpecies + 1) * sizeof(int), hipMemcpyHostToDevice));
-		launch_keys(c->g, c->sp, c->P, c->species_start, c->keys, c->status, c->stream);
-		check_status(c);
-		sort_particles(c);
-		HIP_CHECK(hipStreamSynchronize(c->stream));
+		dx = dalloc<double>(3 * np);
+		dp = dalloc<double>(3 * np);
+		di = dalloc<int>(np);
+		HIP_CHECK(hipMemcpy(dx, hx.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemcpy(dp, hp.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemcpy(di, ib.data(), np * sizeof(int), hipMemcpyHostToDevice));
+		load_particles(c, cnt, dx, dp, di);
+		hipFree(dx);
+		hipFree(dp);
+		hipFree(di);
 	} catch (const DeviceError& e) {
+		hipStreamSynchronize(c->stream);
+		for (void* q : {(void*)dx, (void*)dp, (void*)di})
+			if (q) hipFree(q);
 		return fail(c, e);
 	}
 	return PIC_OK;
@@ -579,56 +723,60 @@ int pic_upload_particles(pic_ctx* c, long long np, const int* species, const int
 int pic_upload_particles_device(pic_ctx* c, const long long* counts, const double* x, const double* p, const int* ids) {
 	if (!c || !counts || !x || !p) return PIC_ERR_ARGUMENT;
 	try {
-		long long np = 0;
 		std::vector<int> cnt(c->sp.count + 1, 0);
+		long long np = 0;
 		for (int s = 0; s < c->sp.count; ++s) {
 			if (counts[s] < 0) throw DeviceError{PIC_ERR_ARGUMENT, "negative particle count"};
 			np += counts[s];
+			if (np > c->capacity) throw DeviceError{PIC_ERR_ARGUMENT, "particle capacity exceeded"};
 			cnt[s + 1] = (int)np;
 		}
-		if (np > c->capacity) throw DeviceError{PIC_ERR_ARGUMENT, "particle capacity exceeded"};
-		hipStream_t s = c->stream;
-		for (int a = 0; a < 3; ++a) {
-			HIP_CHECK(hipMemcpyAsync(c->P.x[a], x + a * np, np * sizeof(double), hipMemcpyDeviceToDevice, s));
-			HIP_CHECK(hipMemcpyAsync(c->P.p[a], p + a * np, np * sizeof(double), hipMemcpyDeviceToDevice, s));
-		}
-		if (ids) {
-			HIP_CHECK(hipMemcpyAsync(c->P.id, ids, np * sizeof(int), hipMemcpyDeviceToDevice, s));
-		} else {
-			launch_iota(reinterpret_cast<uint32_t*>(c->P.id), np, s);
-		}
-		c->P.count = np;
-		c->Palt.count = np;
-		c->species_start_h.assign(kMaxSpecies + 1, (int)np);
-		for (int q = 0; q <= c->sp.count; ++q) c->species_start_h[q] = cnt[q];
-		HIP_CHECK(hipMemcpyAsync(c->species_start, c->species_start_h.data(), (kMaxSpecies + 1) * sizeof(int),
-		                         hipMemcpyHostToDevice, s));
-		launch_keys(c->g, c->sp, c->P, c->species_start, c->keys, c->status, s);
-		check_status(c);
-		sort_particles(c);
-		HIP_CHECK(hipStreamSynchronize(s));
+		load_particles(c, cnt, x, p, ids);
 	} catch (const DeviceError& e) {
 		return fail(c, e);
 	}
 	return PIC_OK;
 }
 
-long long pic_num_particles(const pic_ctx* c) { return c ? c->P.count : -1; }
+long long pic_num_particles(const pic_ctx* c) { return c ? c->np() : -1; }
 
 int pic_download_particles(pic_ctx* c, int* species, int* ids, double* x, double* p) {
 	if (!c) return PIC_ERR_ARGUMENT;
+	double* dx = nullptr;
+	double* dp = nullptr;
 	try {
-		long long n = c->P.count;
+		long long n = c->np();
 		HIP_CHECK(hipStreamSynchronize(c->stream));
-		for (int a = 0; a < 3; ++a) {
-			if (x) HIP_CHECK(hipMemcpy(x + a * n, c->P.x[a], n * sizeof(double), hipMemcpyDeviceToHost));
-			if (p) HIP_CHECK(hipMemcpy(p + a * n, c->P.p[a], n * sizeof(double), hipMemcpyDeviceToHost));
+		const double* xs[3];
+		const double* ps[3];
+		if (c->fp32) {
+			dx = dalloc<double>(3 * n);
+			dp = dalloc<double>(3 * n);
+			launch_to_fp64(c->g, c->sp, c->Q, c->species_start, dx, dp, c->stream);
+			HIP_CHECK(hipStreamSynchronize(c->stream));
+			for (int a = 0; a < 3; ++a) {
+				xs[a] = dx + a * n;
+				ps[a] = dp + a * n;
+			}
+		} else {
+			for (int a = 0; a < 3; ++a) {
+				xs[a] = c->P.x[a];
+				ps[a] = c->P.p[a];
+			}
 		}
-		if (ids) HIP_CHECK(hipMemcpy(ids, c->P.id, n * sizeof(int), hipMemcpyDeviceToHost));
+		for (int a = 0; a < 3; ++a) {
+			if (x) HIP_CHECK(hipMemcpy(x + a * n, xs[a], n * sizeof(double), hipMemcpyDeviceToHost));
+			if (p) HIP_CHECK(hipMemcpy(p + a * n, ps[a], n * sizeof(double), hipMemcpyDeviceToHost));
+		}
+		if (ids) HIP_CHECK(hipMemcpy(ids, c->fp32 ? c->Q.id : c->P.id, n * sizeof(int), hipMemcpyDeviceToHost));
 		if (species)
 			for (int s = 0; s < c->sp.count; ++s)
 				for (long long i = c->species_start_h[s]; i < c->species_start_h[s + 1]; ++i) species[i] = s;
+		if (dx) hipFree(dx);
+		if (dp) hipFree(dp);
 	} catch (const DeviceError& e) {
+		if (dx) hipFree(dx);
+		if (dp) hipFree(dp);
 		return fail(c, e);
 	}
 	return PIC_OK;
@@ -713,9 +861,90 @@ int pic_field_rhs(pic_ctx* c) {
 	return PIC_OK;
 }
 
+// GMRES with FP64 iterative refinement: each outer step solves A d = r approximately with an
+// FP32 basis and FP32 operator (dots accumulated and the Hessenberg least squares kept in
+// FP64), then x += d and r = b - A x in FP64, until ||r|| / ||b|| <= tol.
+static void mixed_solve(pic_ctx* c, const double* b, double* x, pic_solve_info& inf) {
+	hipStream_t s = c->stream;
+	const Grid& g = c->g;
+	long long len = c->nown3();
+	const double C = operator_C(c);
+	double* d_bn = c->dscal + 4;
+	double* d_rn = c->dscal + 5;
+	double* d_h = c->dscal + 8;
+	launch_to_f32(c->muc, c->mu32, 9 * g.nown, s);
+	HIP_CHECK(hipMemsetAsync(x, 0, len * sizeof(double), s));
+	HIP_CHECK(hipMemcpyAsync(c->work, b, len * sizeof(double), hipMemcpyDeviceToDevice, s));  // r = b
+	launch_dotT<double>(b, b, len, c->partial, d_bn, true, s);
+	HIP_CHECK(hipMemcpyAsync(d_rn, d_bn, sizeof(double), hipMemcpyDeviceToDevice, s));
+	double bnorm = 0;
+	HIP_CHECK(hipMemcpyAsync(&bnorm, d_bn, sizeof(double), hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	double rnorm = bnorm, rel = 0;
+	const double inner_floor = 1e-5;  // what an FP32 basis reliably delivers per cycle
+	int limit = std::min(c->cfg.gmres_max_iterations, c->max_basis + 1);
+	while (bnorm > 0 && rnorm / bnorm > c->cfg.gmres_tolerance && inf.restarts < 30) {
+		double target = std::max(inner_floor, 0.5 * c->cfg.gmres_tolerance * bnorm / rnorm);
+		launch_scaled_copy<float>(c->work, c->V32[0], d_rn, len, s);
+		Givens gv;
+		int nn = 2;
+		rel = 1.0;
+		std::vector<double> hcol;
+		while (rel > target && nn < limit) {
+			int j = nn - 2;
+			float* w = c->V32[j + 1];
+			{
+				Timed t(c, "apply_operator32", (double)g.nown * 15 * 4);
+				launch_apply_factored<float>(g, c->mu32, c->V32[j], c->w32, C, w, s);
+			}
+			{
+				Timed t(c, "orthogonalize32", (double)len * 4 * (5.0 * (j + 1) + 3));
+				for (int m = 0; m <= j; ++m) {
+					launch_dotT<float>(c->V32[m], w, len, c->partial, d_h + m, false, s);
+					launch_axpyT<float>(w, c->V32[m], d_h + m, len, s);
+				}
+				launch_dotT<float>(w, w, len, c->partial, d_h + j + 1, true, s);
+				launch_normalizeT<float>(w, d_h + j + 1, len, s);
+			}
+			hcol.assign(nn, 0.0);
+			HIP_CHECK(hipMemcpyAsync(hcol.data(), d_h, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+			gv.update(nn, hcol);
+			rel = std::fabs(gv.Q[nn - 1][0]);
+			nn++;
+		}
+		inf.iterations += nn - 2;
+		inf.restarts += 1;
+		HIP_CHECK(hipMemcpyAsync(c->ydev, gv.y.data(), gv.y.size() * sizeof(double), hipMemcpyHostToDevice, s));
+		{
+			Timed t(c, "refinement", (double)len * 8 * (gv.y.size() / 2.0 + 2 + 15.0 / 3 + 3));
+			launch_update_x<float>(x, c->V32dev, c->ydev, (int)gv.y.size(), d_rn, len, s);  // x += ||r|| sum y_m v_m
+			launch_apply_factored<double>(g, c->muc, x, c->w64, C, c->work2, s);          // A x in FP64
+			launch_residual(b, c->work2, c->work, len, s);                                 // r = b - A x
+			launch_dotT<double>(c->work, c->work, len, c->partial, d_rn, true, s);
+		}
+		HIP_CHECK(hipMemcpyAsync(&rnorm, d_rn, sizeof(double), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		if (nn == 2) break;
+	}
+	inf.true_residual = bnorm > 0 ? rnorm / bnorm : 0.0;
+	inf.estimated_residual = rel;
+	if (inf.true_residual > c->cfg.gmres_tolerance)
+		throw DeviceError{PIC_ERR_SOLVER, "mixed-precision GMRES did not reach the requested true residual"};
+}
+
 static int solve_impl(pic_ctx* c, double* b, double* x, pic_solve_info* info) {
 	hipStream_t s = c->stream;
+	HIP_CHECK(hipStreamSynchronize(s));  // the solve time below excludes work queued before it
 	auto t0 = std::chrono::steady_clock::now();
+	if (c->cfg.solver_precision == PIC_SOLVER_MIXED) {
+		pic_solve_info inf{0, 0, 0, -1, 0};
+		mixed_solve(c, b, x, inf);
+		HIP_CHECK(hipStreamSynchronize(s));
+		inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+		if (info) *info = inf;
+		return PIC_OK;
+	}
 	int dim = 3 * c->g.n[0] * c->g.n[1] * c->g.n[2];
 	double rel = 0;
 	pic_solve_info inf{0, 0, 0, -1, 0};
@@ -811,7 +1040,18 @@ int pic_update_b(pic_ctx* c) {
 int pic_push(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	try {
-		{
+		if (c->fp32) {
+			const Grid& g = c->g;
+			{
+				Timed t(c, "fields_to_fp32", (double)(2 * g.nnodes + g.ncells) * 3 * 12);
+				launch_f64_to_f32(c->E, c->E32, 3 * g.nnodes, c->stream);
+				launch_f64_to_f32(c->Etheta, c->Et32, 3 * g.nnodes, c->stream);
+				launch_f64_to_f32(c->B, c->B32, 3 * g.ncells, c->stream);
+			}
+			Timed t(c, "push", (double)c->np() * 60 + ((double)g.nnodes * 6 + (double)g.ncells * 3) * 4);
+			launch_push32(g, c->sp, c->Q, c->species_start, c->E32, c->Et32, c->B32, (float)c->cfg.dt, (float)c->eta, c->keys,
+			              c->status, c->stream);
+		} else {
 			Timed t(c, "push", (double)c->P.count * 100 + ((double)c->g.nnodes * 6 + (double)c->g.ncells * 3) * 8);
 			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->status,
 			            c->stream);
@@ -842,7 +1082,10 @@ int pic_energy(pic_ctx* c, double* out7) {
 	try {
 		hipStream_t s = c->stream;
 		launch_field_energy(c->g, c->E, c->B, c->epartial, s);
-		launch_particle_energy(c->sp, c->P, c->species_start, c->epartial + 5 * kEnergyBlocks, s);
+		if (c->fp32)
+			launch_particle_energy32(c->sp, c->Q, c->species_start, c->epartial + 5 * kEnergyBlocks, s);
+		else
+			launch_particle_energy(c->sp, c->P, c->species_start, c->epartial + 5 * kEnergyBlocks, s);
 		launch_rows_final(c->epartial, kEnergyBlocks, 9, c->eout, s);
 		double h[9];
 		HIP_CHECK(hipMemcpyAsync(h, c->eout, 9 * sizeof(double), hipMemcpyDeviceToHost, s));

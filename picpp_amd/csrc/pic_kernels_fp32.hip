@@ -1,0 +1,705 @@
+// FP32 particle kernels (the chosen form): positions kept as an integer cell plus an FP32
+// offset in [0, 1) per axis, momentum as u = p / (m c) = gamma v in FP32.
+//
+// The algorithm is the reference's (same predictor, Picard iterations, alpha tensor and
+// CIC maps, PIC++/mover.cpp:194-491), restated on cell offsets: the node-map weights of a
+// particle at offset f are (1 - f, f) from its cell's lower node, and the cell-map weights
+// are (1/2 - f, 1/2 + f) from the cell below when f < 1/2, else (3/2 - f, f - 1/2) from its
+// own cell — the closed forms of correlationBspline (PIC++/correlation.cpp:824-835).
+//
+// Deposits are deterministic without atomics. Particles are sorted by key (species, cell,
+// octant); a group of G = 8 * species lanes serves one cell (node moments) or one dual
+// cell (cell moments), one lane per (species, octant) segment. Each lane sums its segment
+// in order, the group then reduces with a fixed xor-shuffle tree and writes 8 corner sums.
+#include "pic_device.hpp"
+#include "pic_kernels.hpp"
+#include "pic_kernels_fp32.hpp"
+
+namespace picgpu {
+
+constexpr float kPiF = 3.14159265358979323846f;
+
+struct Map32 {
+	int nb, cb;    // node-map and cell-map base index (ghosted)
+	float nw[2];   // node weights
+	float cw[2];   // cell weights
+};
+
+__device__ inline Map32 map32(int c, float f) {
+	Map32 m;
+	m.nb = c;
+	m.nw[0] = 1.0f - f;
+	m.nw[1] = f;
+	if (f < 0.5f) {
+		m.cb = c - 1;
+		m.cw[0] = 0.5f - f;
+		m.cw[1] = 0.5f + f;
+	} else {
+		m.cb = c;
+		m.cw[0] = 1.5f - f;
+		m.cw[1] = f - 0.5f;
+	}
+	return m;
+}
+
+__device__ inline void cell_ijk(const Grid& g, int cell, int c[3]) {
+	c[2] = cell % g.N[2];
+	c[1] = (cell / g.N[2]) % g.N[1];
+	c[0] = cell / (g.N[2] * g.N[1]);
+}
+
+// Maps at a point given relative to cell c (offsets may leave [0, 1) during the Picard
+// iterations); bases are clamped into the ghosted arrays, returning false if a clamp was needed.
+__device__ inline bool maps32(const Grid& g, const int c[3], const float f[3], Map32 m[3]) {
+	bool ok = true;
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		float fl = floorf(f[a]);
+		int cc = c[a] + (int)fl;
+		m[a] = map32(cc, f[a] - fl);
+		if (m[a].nb < 0 || m[a].nb > g.N[a] - 1) {
+			m[a].nb = m[a].nb < 0 ? 0 : g.N[a] - 1;
+			ok = false;
+		}
+		if (m[a].cb < 0 || m[a].cb > g.N[a] - 2) {
+			m[a].cb = m[a].cb < 0 ? 0 : g.N[a] - 2;
+			ok = false;
+		}
+	}
+	return ok;
+}
+
+__device__ inline void gather_node32(const Grid& g, const float* __restrict__ f, const Map32* m, float out[3]) {
+	out[0] = out[1] = out[2] = 0.0f;
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 2; ++j)
+#pragma unroll
+			for (int k = 0; k < 2; ++k) {
+				long long id = node_id(g, m[0].nb + i, m[1].nb + j, m[2].nb + k);
+				float c = m[0].nw[i] * m[1].nw[j] * m[2].nw[k];
+				out[0] += f[id] * c;
+				out[1] += f[g.nnodes + id] * c;
+				out[2] += f[2 * g.nnodes + id] * c;
+			}
+}
+
+__device__ inline void gather_cell32(const Grid& g, const float* __restrict__ f, const Map32* m, float out[3]) {
+	out[0] = out[1] = out[2] = 0.0f;
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 2; ++j)
+#pragma unroll
+			for (int k = 0; k < 2; ++k) {
+				long long id = cell_id(g, m[0].cb + i, m[1].cb + j, m[2].cb + k);
+				float c = m[0].cw[i] * m[1].cw[j] * m[2].cw[k];
+				out[0] += f[id] * c;
+				out[1] += f[g.ncells + id] * c;
+				out[2] += f[2 * g.ncells + id] * c;
+			}
+}
+
+// alpha tensor (PIC++/mover.cpp:851-908) in FP32, with v and gamma from u = gamma v.
+__device__ inline void alpha32(float beta, const float v[3], float gamma, const float E[3], const float B[3], float a[9]) {
+	float G = beta * (E[0] * v[0] + E[1] * v[1] + E[2] * v[2]) + gamma;
+	float bs = beta / G;
+	float b2 = bs * bs;
+	float inv = 1.0f / (G * (1.0f + b2 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2])));
+	a[0] = (1.0f + b2 * B[0] * B[0]) * inv;
+	a[1] = (b2 * B[0] * B[1] + bs * B[2]) * inv;
+	a[2] = (b2 * B[0] * B[2] - bs * B[1]) * inv;
+	a[3] = (b2 * B[1] * B[0] - bs * B[2]) * inv;
+	a[4] = (1.0f + b2 * B[1] * B[1]) * inv;
+	a[5] = (b2 * B[1] * B[2] + bs * B[0]) * inv;
+	a[6] = (b2 * B[2] * B[0] + bs * B[1]) * inv;
+	a[7] = (b2 * B[2] * B[1] - bs * B[0]) * inv;
+	a[8] = (1.0f + b2 * B[2] * B[2]) * inv;
+}
+
+__device__ inline float group_sum(float v, int G) {
+	for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	return v;
+}
+
+__device__ inline void load32(const ParticleArrays32& P, int i, float f[3], float u[3]) {
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		f[a] = P.off[a][i];
+		u[a] = P.u[a][i];
+	}
+}
+
+// ------------------------------------------------------------------ moments at x^n
+// Node moments (J, mu): one group of G lanes per cell, lane = (species, octant).
+__global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
+                                                        const float* __restrict__ E, const float* __restrict__ B, float dt,
+                                                        float theta, int G, float* __restrict__ out) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	long long c = t / G;
+	int lane = (int)(t % G);
+	int s = lane >> 3, oct = lane & 7;
+	bool active = c < g.ncells && s < sp.count;
+	float acc[8][12];
+#pragma unroll
+	for (int a = 0; a < 8; ++a)
+#pragma unroll
+		for (int q = 0; q < 12; ++q) acc[a][q] = 0.0f;
+	if (active) {
+		long long key = (((long long)s * g.ncells + c) << 3) | oct;
+		int lo = start[key], hi = start[key + 1];
+		float m = (float)sp.mass[s], q = (float)sp.charge[s], w = (float)sp.weight[s];
+		float beta = 0.5f * q * dt / m;
+		float inv_v = (float)(1.0 / g.volume);
+		float omega = w * theta * dt * dt * 2.0f * kPiF * q * q / m * inv_v;
+		float pc = q * w * inv_v;
+		int cc[3];
+		cell_ijk(g, (int)c, cc);
+		for (int i = lo; i < hi; ++i) {
+			float f[3], u[3];
+			load32(P, i, f, u);
+			Map32 mp[3];
+			maps32(g, cc, f, mp);
+			float Ef[3], Bf[3], a[9];
+			gather_node32(g, E, mp, Ef);
+			gather_cell32(g, B, mp, Bf);
+			float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+			float ig = 1.0f / gm;
+			float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
+			alpha32(beta, v, gm, Ef, Bf, a);
+			float rv[3];
+#pragma unroll
+			for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
+#pragma unroll
+			for (int ci = 0; ci < 2; ++ci)
+#pragma unroll
+				for (int cj = 0; cj < 2; ++cj)
+#pragma unroll
+					for (int ck = 0; ck < 2; ++ck) {
+						int corner = ci * 4 + cj * 2 + ck;
+						float corr = mp[0].nw[ci] * mp[1].nw[cj] * mp[2].nw[ck];
+						float fj = pc * corr;
+#pragma unroll
+						for (int r = 0; r < 3; ++r) acc[corner][r] += rv[r] * fj;
+						float o = omega * corr;
+#pragma unroll
+						for (int r = 0; r < 9; ++r) acc[corner][3 + r] -= a[r] * o;
+					}
+		}
+	}
+#pragma unroll
+	for (int a = 0; a < 8; ++a)
+#pragma unroll
+		for (int q = 0; q < 12; ++q) acc[a][q] = group_sum(acc[a][q], G);
+	if (lane == 0 && c < g.ncells) {
+#pragma unroll
+		for (int a = 0; a < 8; ++a)
+#pragma unroll
+			for (int q = 0; q < 12; ++q) out[((long long)q * 8 + a) * g.ncells + c] = acc[a][q];
+	}
+}
+
+// Cell moments (rhohat corners and the reference's pressure-tensor partials) per dual cell b:
+// lane = (species, source octant); source cell b + a, octant bits 1 - a.
+__global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
+                                                        const float* __restrict__ E, const float* __restrict__ B, float dt,
+                                                        int G, float* __restrict__ rho_out, float* __restrict__ pi_out) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	long long b = t / G;
+	int lane = (int)(t % G);
+	int s = lane >> 3, seg = lane & 7;
+	float ar[8], ap[8][6];
+#pragma unroll
+	for (int q = 0; q < 8; ++q) {
+		ar[q] = 0.0f;
+#pragma unroll
+		for (int r = 0; r < 6; ++r) ap[q][r] = 0.0f;
+	}
+	if (b < g.ncells && s < sp.count) {
+		int bc[3];
+		cell_ijk(g, (int)b, bc);
+		int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
+		int mc[3] = {bc[0] + ax, bc[1] + ay, bc[2] + az};
+		if (mc[0] < g.N[0] && mc[1] < g.N[1] && mc[2] < g.N[2]) {
+			long long key = (((long long)s * g.ncells + cell_id(g, mc[0], mc[1], mc[2])) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) |
+			                (1 - az);
+			int lo = start[key], hi = start[key + 1];
+			float m = (float)sp.mass[s], q = (float)sp.charge[s], w = (float)sp.weight[s];
+			float beta = 0.5f * q * dt / m;
+			float pc = q * w * (float)(1.0 / g.volume);
+			for (int i = lo; i < hi; ++i) {
+				float f[3], u[3];
+				load32(P, i, f, u);
+				Map32 mp[3];
+				maps32(g, mc, f, mp);
+				float Ef[3], Bf[3], a[9];
+				gather_node32(g, E, mp, Ef);
+				gather_cell32(g, B, mp, Bf);
+				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+				float ig = 1.0f / gm;
+				float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
+				alpha32(beta, v, gm, Ef, Bf, a);
+				float rv[3];
+#pragma unroll
+				for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
+				float T[6] = {rv[0] * rv[0], rv[0] * rv[1], rv[0] * rv[2], rv[1] * rv[1], rv[1] * rv[2], rv[2] * rv[2]};
+#pragma unroll
+				for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+					for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+						for (int tk = 0; tk < 2; ++tk) {
+							int tt = ti * 4 + tj * 2 + tk;
+							float fr = pc * mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk];
+							ar[tt] += fr;
+#pragma unroll
+							for (int r = 0; r < 6; ++r) ap[tt][r] += fr * T[r];
+						}
+			}
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < 8; ++q) {
+		ar[q] = group_sum(ar[q], G);
+#pragma unroll
+		for (int r = 0; r < 6; ++r) ap[q][r] = group_sum(ap[q][r], G);
+	}
+	if (lane == 0 && b < g.ncells) {
+#pragma unroll
+		for (int q = 0; q < 8; ++q) {
+			rho_out[(long long)q * g.ncells + b] = ar[q];
+#pragma unroll
+			for (int r = 0; r < 6; ++r) pi_out[((long long)q * 6 + r) * g.ncells + b] = ap[q][r];
+		}
+	}
+}
+
+// Post-push density (updateDensityParameters) per dual cell: lane = (species, source octant).
+// Output planes (float): per species [conc, energy, bulk x3] x 8 corners, then rho x 8.
+__global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start, int G,
+                                                   float* __restrict__ out) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	long long b = t / G;
+	int lane = (int)(t % G);
+	int s = lane >> 3, seg = lane & 7;
+	float conc[8], en[8], bulk[8][3], rho[8];
+#pragma unroll
+	for (int q = 0; q < 8; ++q) {
+		conc[q] = en[q] = rho[q] = 0.0f;
+		bulk[q][0] = bulk[q][1] = bulk[q][2] = 0.0f;
+	}
+	if (b < g.ncells && s < sp.count) {
+		int bc[3];
+		cell_ijk(g, (int)b, bc);
+		int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
+		int mc[3] = {bc[0] + ax, bc[1] + ay, bc[2] + az};
+		if (mc[0] < g.N[0] && mc[1] < g.N[1] && mc[2] < g.N[2]) {
+			long long key = (((long long)s * g.ncells + cell_id(g, mc[0], mc[1], mc[2])) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) |
+			                (1 - az);
+			int lo = start[key], hi = start[key + 1];
+			float m = (float)sp.mass[s], w = (float)sp.weight[s];
+			float wv = w * (float)(1.0 / g.volume);
+			float pc = (float)sp.charge[s] * wv;
+			for (int i = lo; i < hi; ++i) {
+				float f[3], u[3];
+				load32(P, i, f, u);
+				Map32 mp[3];
+				maps32(g, mc, f, mp);
+				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+#pragma unroll
+				for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+					for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+						for (int tk = 0; tk < 2; ++tk) {
+							int tt = ti * 4 + tj * 2 + tk;
+							float corr = mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk];
+							float cw = wv * corr;
+							conc[tt] += cw;
+							en[tt] += cw * gm * m;
+							rho[tt] += pc * corr;
+#pragma unroll
+							for (int r = 0; r < 3; ++r) bulk[tt][r] += cw * u[r] * m;
+						}
+			}
+		}
+	}
+	// per-species sums over the 8 octant lanes, charge density over all lanes
+#pragma unroll
+	for (int q = 0; q < 8; ++q) {
+		conc[q] = group_sum(conc[q], 8);
+		en[q] = group_sum(en[q], 8);
+#pragma unroll
+		for (int r = 0; r < 3; ++r) bulk[q][r] = group_sum(bulk[q][r], 8);
+		rho[q] = group_sum(rho[q], G);
+	}
+	if (b < g.ncells && s < sp.count && seg == 0) {
+		long long base = (long long)s * 5 * 8;
+#pragma unroll
+		for (int q = 0; q < 8; ++q) {
+			out[(base + 0 * 8 + q) * g.ncells + b] = conc[q];
+			out[(base + 1 * 8 + q) * g.ncells + b] = en[q];
+#pragma unroll
+			for (int r = 0; r < 3; ++r) out[(base + (2 + r) * 8 + q) * g.ncells + b] = bulk[q][r];
+		}
+	}
+	if (b < g.ncells && lane == 0) {
+		long long base = (long long)sp.count * 5 * 8;
+#pragma unroll
+		for (int q = 0; q < 8; ++q) out[(base + q) * g.ncells + b] = rho[q];
+	}
+}
+
+// ------------------------------------------------------------------ mover
+// moveParticle (PIC++/mover.cpp:194-491) on cell offsets, one thread per particle; alpha is
+// recomputed from E^n, B^n at x^n. Momentum-space constants: du = (q dt / m) (E + v x B).
+__global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start,
+                                                const float* __restrict__ E, const float* __restrict__ Et,
+                                                const float* __restrict__ B, float dt, float eta, uint32_t* __restrict__ keys,
+                                                int* __restrict__ status) {
+	long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= P.count) return;
+	int i = (int)idx;
+	int s = 0;
+	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+	float m = (float)sp.mass[s], q = (float)sp.charge[s];
+	float qdm = q * dt / m;
+	float beta = 0.5f * q * dt / m;
+	float inv_d[3] = {(float)(1.0 / g.d[0]), (float)(1.0 / g.d[1]), (float)(1.0 / g.d[2])};
+	int cc[3];
+	cell_ijk(g, P.cell[i], cc);
+	float f[3], u[3];
+	load32(P, i, f, u);
+	Map32 mp[3];
+	bool ok = maps32(g, cc, f, mp);
+	float En[3], Bf[3], a[9], Ef[3];
+	gather_node32(g, E, mp, En);
+	gather_cell32(g, B, mp, Bf);
+	gather_node32(g, Et, mp, Ef);
+	float u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+	float gm = sqrtf(1.0f + u2);
+	float ig = 1.0f / gm;
+	float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
+	alpha32(beta, v, gm, En, Bf, a);
+	// predictor (mover.cpp:234-245), with Particle::evaluateVelocity's nonrelativistic branch
+	float nu[3];
+	nu[0] = u[0] + (Ef[0] + (v[1] * Bf[2] - v[2] * Bf[1])) * qdm;
+	nu[1] = u[1] + (Ef[1] + (-v[0] * Bf[2] + v[2] * Bf[0])) * qdm;
+	nu[2] = u[2] + (Ef[2] + (v[0] * Bf[1] - v[1] * Bf[0])) * qdm;
+	float nu2 = nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2];
+	float nig = nu2 < 0.0001f ? 1.0f : 1.0f / sqrtf(1.0f + nu2);
+	float mid[3], xh[3];
+#pragma unroll
+	for (int c = 0; c < 3; ++c) {
+		mid[c] = v[c] * (1.0f - eta) + nu[c] * nig * eta;
+		xh[c] = f[c] + mid[c] * (eta * dt) * inv_d[c];
+	}
+	float vhat[3];
+#pragma unroll
+	for (int r = 0; r < 3; ++r) vhat[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
+	float unorm = sqrtf(u2);
+	float du[3] = {nu[0] - u[0], nu[1] - u[1], nu[2] - u[2]};
+	float err = sqrtf(du[0] * du[0] + du[1] * du[1] + du[2] * du[2]);
+	float prevX[3] = {xh[0], xh[1], xh[2]};
+	float cerr = 1.0f;  // in cell units (coordinatesError = deltaX)
+	for (int it = 0; it < 3 && err > 1e-14f * unorm && cerr > 1e-8f; ++it) {
+		float prevU[3] = {nu[0], nu[1], nu[2]};
+		float rE[3];
+#pragma unroll
+		for (int r = 0; r < 3; ++r) rE[r] = a[r * 3 + 0] * Ef[0] + a[r * 3 + 1] * Ef[1] + a[r * 3 + 2] * Ef[2];
+		float nx[3];
+#pragma unroll
+		for (int c = 0; c < 3; ++c) {
+			mid[c] = vhat[c] + rE[c] * beta;
+			nx[c] = f[c] + mid[c] * (eta * dt) * inv_d[c];
+		}
+		ok &= maps32(g, cc, nx, mp);
+		gather_node32(g, Et, mp, Ef);
+		gather_cell32(g, B, mp, Bf);
+		nu[0] = u[0] + (Ef[0] + (mid[1] * Bf[2] - mid[2] * Bf[1])) * qdm;
+		nu[1] = u[1] + (Ef[1] + (-mid[0] * Bf[2] + mid[2] * Bf[0])) * qdm;
+		nu[2] = u[2] + (Ef[2] + (mid[0] * Bf[1] - mid[1] * Bf[0])) * qdm;
+		float d0 = prevU[0] - nu[0], d1 = prevU[1] - nu[1], d2 = prevU[2] - nu[2];
+		err = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+		float e0 = (prevX[0] - nx[0]) * (float)g.d[0], e1 = (prevX[1] - nx[1]) * (float)g.d[1], e2 = (prevX[2] - nx[2]) * (float)g.d[2];
+		cerr = sqrtf(e0 * e0 + e1 * e1 + e2 * e2) * inv_d[0];
+#pragma unroll
+		for (int c = 0; c < 3; ++c) prevX[c] = nx[c];
+	}
+	// x^{n+1} = x^n + mid dt, carried into the integer cell; one-rank periodic wrap
+	int nc[3];
+#pragma unroll
+	for (int c = 0; c < 3; ++c) {
+		float x = f[c] + mid[c] * dt * inv_d[c];
+		float fl = floorf(x);
+		x -= fl;
+		if (x >= 1.0f) {  // rounding of x - floor(x) up to 1
+			x = 0.0f;
+			fl += 1.0f;
+		}
+		int ci = cc[c] + (int)fl;
+		if (ci < 2) ci += g.n[c];
+		if (ci > g.n[c] + 1) ci -= g.n[c];
+		if (ci < 2 || ci > g.n[c] + 1) ok = false;
+		nc[c] = ci;
+		f[c] = x;
+	}
+	if (!ok) atomicOr(status, 8);
+	int cell = (int)cell_id(g, nc[0], nc[1], nc[2]);
+	P.cell[i] = cell;
+#pragma unroll
+	for (int c = 0; c < 3; ++c) {
+		P.off[c][i] = f[c];
+		P.u[c][i] = nu[c];
+	}
+	keys[i] = (uint32_t)((((long long)s * g.ncells + cell) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) | (f[2] >= 0.5f));
+}
+
+__global__ void k_keys32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start, uint32_t* __restrict__ keys) {
+	long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= P.count) return;
+	int i = (int)idx;
+	int s = 0;
+	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+	keys[i] = (uint32_t)((((long long)s * g.ncells + P.cell[i]) << 3) | ((P.off[0][i] >= 0.5f) << 2) |
+	                     ((P.off[1][i] >= 0.5f) << 1) | (P.off[2][i] >= 0.5f));
+}
+
+__global__ void k_permute32(ParticleArrays32 src, ParticleArrays32 dst, const uint32_t* __restrict__ perm) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= src.count) return;
+	long long s = perm[i];
+	dst.cell[i] = src.cell[s];
+#pragma unroll
+	for (int c = 0; c < 3; ++c) {
+		dst.off[c][i] = src.off[c][s];
+		dst.u[c][i] = src.u[c][s];
+	}
+	dst.id[i] = src.id[s];
+}
+
+// FP64 absolute positions and momenta -> cell + offset, u = p / m (owned cells 2..n+1).
+__global__ void k_from_fp64(Grid g, SpeciesTable sp, const double* __restrict__ x, const double* __restrict__ p, long long np,
+                            const int* __restrict__ species_start, ParticleArrays32 P, int* __restrict__ status) {
+	long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= np) return;
+	int i = (int)idx;
+	int s = 0;
+	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+	int cc[3];
+	for (int a = 0; a < 3; ++a) {
+		double r = (x[a * np + i] - g.x0[a]) / g.d[a];
+		double fl = floor(r);
+		int ci = (int)fl;
+		float f = (float)(r - fl);
+		if (f >= 1.0f) {
+			f = 0.0f;
+			ci += 1;
+		}
+		if (ci == g.n[a] + 2) ci = 2;  // x = 2L is the periodic image of x = L
+		if (ci < 2 || ci > g.n[a] + 1) atomicOr(status, 4);
+		cc[a] = ci < 0 ? 0 : (ci > g.N[a] - 1 ? g.N[a] - 1 : ci);
+		P.off[a][i] = f;
+		P.u[a][i] = (float)(p[a * np + i] / sp.mass[s]);
+	}
+	P.cell[i] = (int)cell_id(g, cc[0], cc[1], cc[2]);
+}
+
+// cell + offset, u -> FP64 absolute positions and momenta
+__global__ void k_to_fp64(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start, double* __restrict__ x,
+                          double* __restrict__ p) {
+	long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= P.count) return;
+	int i = (int)idx;
+	long long np = P.count;
+	int s = 0;
+	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+	int cc[3];
+	cell_ijk(g, P.cell[i], cc);
+	for (int a = 0; a < 3; ++a) {
+		x[a * np + i] = g.x0[a] + ((double)cc[a] + (double)P.off[a][i]) * g.d[a];
+		p[a * np + i] = (double)P.u[a][i] * sp.mass[s];
+	}
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) b[i] = (float)a[i];
+}
+
+// Node / cell values from float corner sums, accumulated in FP64.
+__global__ void k_node_assemble32(Grid g, const float* __restrict__ cs, double* __restrict__ J, double* __restrict__ mu) {
+	long long nid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (nid >= g.nnodes) return;
+	int k = (int)(nid % (g.N[2] + 1));
+	int j = (int)((nid / (g.N[2] + 1)) % (g.N[1] + 1));
+	int i = (int)(nid / ((long long)(g.N[2] + 1) * (g.N[1] + 1)));
+	double s[12];
+#pragma unroll
+	for (int q = 0; q < 12; ++q) s[q] = 0.0;
+	for (int a = 0; a < 2; ++a)
+		for (int b = 0; b < 2; ++b)
+			for (int cc = 0; cc < 2; ++cc) {
+				int ci = i - a, cj = j - b, ck = k - cc;
+				if (ci < 0 || cj < 0 || ck < 0 || ci >= g.N[0] || cj >= g.N[1] || ck >= g.N[2]) continue;
+				long long cid = cell_id(g, ci, cj, ck);
+				int corner = a * 4 + b * 2 + cc;
+#pragma unroll
+				for (int q = 0; q < 12; ++q) s[q] += (double)cs[((long long)q * 8 + corner) * g.ncells + cid];
+			}
+#pragma unroll
+	for (int q = 0; q < 3; ++q) J[q * g.nnodes + nid] = s[q];
+#pragma unroll
+	for (int q = 0; q < 9; ++q) mu[q * g.nnodes + nid] = s[3 + q];
+}
+
+__global__ void k_cell_assemble32(Grid g, const float* __restrict__ cs, int nq, double* __restrict__ out) {
+	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.ncells) return;
+	int k = (int)(c % g.N[2]);
+	int j = (int)((c / g.N[2]) % g.N[1]);
+	int i = (int)(c / ((long long)g.N[2] * g.N[1]));
+	for (int q = 0; q < nq; ++q) {
+		double s = 0.0;
+		for (int a = 0; a < 2; ++a)
+			for (int b = 0; b < 2; ++b)
+				for (int cc = 0; cc < 2; ++cc) {
+					int bi = i - a, bj = j - b, bk = k - cc;
+					if (bi < 0 || bj < 0 || bk < 0) continue;
+					s += (double)cs[((long long)q * 8 + a * 4 + b * 2 + cc) * g.ncells + cell_id(g, bi, bj, bk)];
+				}
+		out[(long long)q * g.ncells + c] = s;
+	}
+}
+
+__global__ void __launch_bounds__(256) k_pressure_reduce32(Grid g, const float* __restrict__ ps, double* __restrict__ Pi) {
+	int i = blockIdx.x / 4;
+	int tj = (blockIdx.x / 2) % 2;
+	int tk = blockIdx.x % 2;
+	__shared__ double red[6][256];
+	double acc[6] = {0, 0, 0, 0, 0, 0};
+	long long plane = (long long)g.N[1] * g.N[2];
+	for (int ti = 0; ti < 2; ++ti) {
+		int bi = i - ti;
+		if (bi < 0 || bi >= g.N[0]) continue;
+		int t = ti * 4 + tj * 2 + tk;
+		for (long long r = threadIdx.x; r < plane; r += blockDim.x) {
+			long long b = (long long)bi * plane + r;
+#pragma unroll
+			for (int q = 0; q < 6; ++q) acc[q] += (double)ps[((long long)t * 6 + q) * g.ncells + b];
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = acc[q];
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w)
+#pragma unroll
+			for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		long long c = cell_id(g, i, tj, tk);
+		const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+		for (int q = 0; q < 9; ++q) Pi[q * g.ncells + c] = red[sym[q]][0];
+	}
+}
+
+// kinetic energy with the compensated gamma - 1 = u^2 / (gamma + 1), and momentum
+__global__ void __launch_bounds__(256) k_particle_energy32(SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start,
+                                                           double* __restrict__ partial) {
+	__shared__ double red[4][256];
+	double acc[4] = {0, 0, 0, 0};
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P.count; i += (long long)gridDim.x * blockDim.x) {
+		int s = 0;
+		while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+		double m = sp.mass[s], w = sp.weight[s];
+		float u[3] = {P.u[0][i], P.u[1][i], P.u[2][i]};
+		float u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+		float gm1 = u2 / (sqrtf(1.0f + u2) + 1.0f);
+		acc[0] += (double)gm1 * m * w;
+		for (int c = 0; c < 3; ++c) acc[1 + c] += (double)u[c] * m * w;
+	}
+	for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = acc[q];
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w)
+			for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		for (int q = 0; q < 4; ++q) partial[q * gridDim.x + blockIdx.x] = red[q][0];
+}
+
+// ------------------------------------------------------------------ launch wrappers
+namespace {
+inline unsigned blocks32(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+}  // namespace
+
+int group_lanes(int nspecies) {
+	int G = 8;
+	while (G < 8 * nspecies) G *= 2;
+	return G;
+}
+
+void launch_node_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
+                           const float* B, float dt, float theta, float* scratch, hipStream_t s) {
+	int G = group_lanes(sp.count);
+	hipLaunchKernelGGL(k_node_moments32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start, E, B, dt, theta, G,
+	                   scratch);
+}
+void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
+                           const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s) {
+	int G = group_lanes(sp.count);
+	hipLaunchKernelGGL(k_cell_moments32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start, E, B, dt, G,
+	                   rho_scratch, pi_scratch);
+}
+void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
+                      hipStream_t s) {
+	int G = group_lanes(sp.count);
+	hipLaunchKernelGGL(k_density32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start, G, scratch);
+}
+void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
+                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* status, hipStream_t s) {
+	if (P.count == 0) return;
+	hipLaunchKernelGGL(k_push32, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, E, Et, B, dt, eta, keys,
+	                   status);
+}
+void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
+                   hipStream_t s) {
+	if (P.count == 0) return;
+	hipLaunchKernelGGL(k_keys32, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, keys);
+}
+void launch_permute32(const ParticleArrays32& src, const ParticleArrays32& dst, const uint32_t* perm, hipStream_t s) {
+	if (src.count == 0) return;
+	hipLaunchKernelGGL(k_permute32, dim3(blocks32(src.count, 256)), dim3(256), 0, s, src, dst, perm);
+}
+void launch_from_fp64(const Grid& g, const SpeciesTable& sp, const double* x, const double* p, long long np,
+                      const int* species_start, const ParticleArrays32& P, int* status, hipStream_t s) {
+	if (np == 0) return;
+	hipLaunchKernelGGL(k_from_fp64, dim3(blocks32(np, 256)), dim3(256), 0, s, g, sp, x, p, np, species_start, P, status);
+}
+void launch_to_fp64(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, double* x,
+                    double* p, hipStream_t s) {
+	if (P.count == 0) return;
+	hipLaunchKernelGGL(k_to_fp64, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, x, p);
+}
+void launch_f64_to_f32(const double* a, float* b, long long n, hipStream_t s) {
+	hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks32(n, 256)), dim3(256), 0, s, a, b, n);
+}
+void launch_node_assemble32(const Grid& g, const float* scratch, double* J, double* mu, hipStream_t s) {
+	hipLaunchKernelGGL(k_node_assemble32, dim3(blocks32(g.nnodes, 256)), dim3(256), 0, s, g, scratch, J, mu);
+}
+void launch_cell_assemble32(const Grid& g, const float* scratch, int nq, double* out, hipStream_t s) {
+	hipLaunchKernelGGL(k_cell_assemble32, dim3(blocks32(g.ncells, 256)), dim3(256), 0, s, g, scratch, nq, out);
+}
+void launch_pressure_reduce32(const Grid& g, const float* pi_scratch, double* Pi, hipStream_t s) {
+	hipLaunchKernelGGL(k_pressure_reduce32, dim3(g.N[0] * 4), dim3(256), 0, s, g, pi_scratch, Pi);
+}
+void launch_particle_energy32(const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, double* partial,
+                              hipStream_t s) {
+	hipLaunchKernelGGL(k_particle_energy32, dim3(kEnergyBlocks), dim3(256), 0, s, sp, P, species_start, partial);
+}
+
+}  // namespace picgpu

@@ -1,0 +1,41 @@
+// FP32 particle kernels: integer cell + FP32 offset positions, u = gamma v momenta.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pic_device.hpp"
+
+namespace picgpu {
+
+struct ParticleArrays32 {
+	int* cell;      // ghosted linear cell id
+	float* off[3];  // offset within the cell, [0, 1)
+	float* u[3];    // p / (m c)
+	int* id;
+	long long count;
+};
+
+int group_lanes(int nspecies);
+void launch_node_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
+                           const float* B, float dt, float theta, float* scratch, hipStream_t s);
+void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
+                           const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s);
+void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
+                      hipStream_t s);
+void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
+                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* status, hipStream_t s);
+void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
+                   hipStream_t s);
+void launch_permute32(const ParticleArrays32& src, const ParticleArrays32& dst, const uint32_t* perm, hipStream_t s);
+void launch_from_fp64(const Grid& g, const SpeciesTable& sp, const double* x, const double* p, long long np,
+                      const int* species_start, const ParticleArrays32& P, int* status, hipStream_t s);
+void launch_to_fp64(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, double* x,
+                    double* p, hipStream_t s);
+void launch_f64_to_f32(const double* a, float* b, long long n, hipStream_t s);
+void launch_node_assemble32(const Grid& g, const float* scratch, double* J, double* mu, hipStream_t s);
+void launch_cell_assemble32(const Grid& g, const float* scratch, int nq, double* out, hipStream_t s);
+void launch_pressure_reduce32(const Grid& g, const float* pi_scratch, double* Pi, hipStream_t s);
+void launch_particle_energy32(const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, double* partial,
+                              hipStream_t s);
+
+}  // namespace picgpu

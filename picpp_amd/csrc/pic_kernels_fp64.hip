@@ -957,10 +957,15 @@ __global__ void __launch_bounds__(256) k_rows_final(const double* __restrict__ p
 // ------------------------------------------------------------------ integer helpers
 namespace picgpu {
 
+// Run lengths of the sorted keys: the first index of every run writes its length.
 __global__ void k_count_keys(const uint32_t* __restrict__ keys, long long count, int* __restrict__ counts) {
 	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= count) return;
-	atomicAdd(&counts[keys[i]], 1);
+	uint32_t k = keys[i];
+	if (i > 0 && keys[i - 1] == k) return;
+	long long j = i + 1;
+	while (j < count && keys[j] == k) ++j;
+	counts[k] = (int)(j - i);
 }
 
 __global__ void k_iota(uint32_t* __restrict__ v, long long count) {

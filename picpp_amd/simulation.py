@@ -21,6 +21,7 @@ _I = ctypes.POINTER(ctypes.c_int)
 PRECISIONS = {"fp64": _lib.PRECISION_FP64, "fp32": _lib.PRECISION_FP32}
 STOP_RULES = {"reference": _lib.STOP_REFERENCE, "true_residual": _lib.STOP_TRUE_RESIDUAL}
 ORTHOS = {"mgs": _lib.ORTHO_MGS, "cgs2": _lib.ORTHO_CGS2}
+SOLVERS = {"fp64": _lib.SOLVER_FP64, "mixed": _lib.SOLVER_MIXED}
 
 
 def _dp(a):
@@ -34,7 +35,7 @@ def _ip(a):
 class Simulation:
     def __init__(self, workload: Workload, precision: str = "fp64", stop_rule: str = "reference", ortho: str = "mgs",
                  capacity: Optional[int] = None, gmres_tolerance: Optional[float] = None,
-                 gmres_max_iterations: int = 50, stream: Optional[int] = None):
+                 gmres_max_iterations: int = 50, stream: Optional[int] = None, solver: str = "fp64"):
         self.lib = _lib.load()
         self.w = workload
         cfg = _lib.PicConfig()
@@ -51,6 +52,7 @@ class Simulation:
         cfg.ortho = ORTHOS[ortho]
         cfg.gmres_tolerance = workload.gmres_tolerance if gmres_tolerance is None else gmres_tolerance
         cfg.gmres_max_iterations = gmres_max_iterations
+        cfg.solver_precision = SOLVERS[solver]
         self.cfg = cfg
         h = ctypes.c_void_p()
         _lib.check(self.lib.pic_create(ctypes.byref(cfg), ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)),

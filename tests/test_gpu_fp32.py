@@ -1,0 +1,102 @@
+"""FP32 chosen form (integer cell + FP32 offset positions, u = gamma v, FP32 deposits)
+against the FP64 oracle. The FP32 form is the same algorithm at lower precision, so
+the bar is a stated error, not bit parity: every quantity is checked to <= 2e-5
+normwise after the first step and the measured errors are printed.
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr
+from oracle.oracle import OracleSim
+from picpp_amd import workloads as wl
+from picpp_amd.simulation import Simulation
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 2e-5
+
+
+def test_fp32_first_step_errors_against_fp64_oracle():
+    w = wl.mini(n=(12, 10, 8), ppc=8, seed=21)
+    st = wl.make_initial_state(w, perturb=0.05)
+    print("case", w.name, flush=True)
+    sim = Simulation(w, precision="fp32")
+    sim.load_state(st)
+    orc = OracleSim(w, st)
+    sim.initialize()
+    orc.initial_sequence()
+    errs = {}
+    g, o = sim.particles(), orc.particles()
+    gi, oi = np.argsort(g["ids"]), np.argsort(o["ids"])
+    errs["x0"] = relerr(g["x"][gi], o["x"][oi])
+    sim.updateElectroMagneticParameters()
+    orc.evaluate_rotation_tensor()
+    orc.update_em_parameters()
+    for name in ("J", "mu", "rhohat", "divPi"):
+        errs[name] = relerr(sim.array(name), orc.array(name))
+    sim.evaluateElectricField()
+    orc.evaluate_electric_field()
+    orc.exchange_fields()
+    errs["Etheta"] = relerr(sim.array("Etheta"), orc.array("Etheta"))
+    sim.evaluateMagneticField()
+    orc.evaluate_magnetic_field()
+    orc.exchange_fields()
+    errs["Bnew"] = relerr(sim.array("Bnew"), orc.array("Bnew"))
+    sim.moveParticles()
+    orc.move_particles()
+    orc.update_density_parameters()
+    g, o = sim.particles(), orc.particles()
+    gi, oi = np.argsort(g["ids"]), np.argsort(o["ids"])
+    assert np.array_equal(g["ids"][gi], o["ids"][oi])
+    errs["x"] = relerr(g["x"][gi], o["x"][oi])
+    errs["p"] = relerr(g["p"][gi], o["p"][oi])
+    d = sim.array("density")
+    errs["rho"] = relerr(d[..., 10], orc.array("rho"))
+    errs["conc0"] = relerr(d[..., 0], orc.array("conc0"))
+    sim.updateFields()
+    orc.exchange_fields()
+    orc.update_fields()
+    orc.exchange_fields()
+    orc.update_energy()
+    errs["energy"] = relerr(sim.updateEnergy()[:4], orc.energy()[:4])
+    print("fp32 vs fp64 oracle:", {k: "%.2e" % v for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v <= TOL32, (k, v)
+    sim.close()
+
+
+def test_fp32_deterministic_bitwise():
+    w = wl.mini(n=(10, 8, 6), ppc=6, seed=3)
+    st = wl.make_initial_state(w, perturb=0.05)
+    outs = []
+    for _ in range(2):
+        sim = Simulation(w, precision="fp32")
+        sim.load_state(st)
+        sim.initialize()
+        for _ in range(3):
+            sim.step()
+        outs.append((sim.array("E"), sim.array("J"), sim.array("mu"), sim.particles()["x"]))
+        sim.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_fp32_energy_tracks_fp64_over_steps():
+    """Total energy of the FP32 form stays within 1e-6 (relative) of the FP64 GPU path
+    over 20 steps of the reference's CPU case at 8 ppc."""
+    w = wl.config1(ppc=8, steps=20)
+    st = wl.make_initial_state(w)
+    e = {}
+    for prec in ("fp64", "fp32"):
+        sim = Simulation(w, precision=prec)
+        sim.load_state(st)
+        sim.initialize()
+        series = []
+        for _ in range(w.steps):
+            sim.step()
+            series.append(sim.updateEnergy()[3])
+        e[prec] = np.array(series)
+        sim.close()
+    drift = np.max(np.abs(e["fp32"] - e["fp64"]) / np.abs(e["fp64"]))
+    print("max relative total-energy difference fp32 vs fp64 over %d steps: %.2e" % (w.steps, drift))
+    assert drift <= 1e-6
