@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--stop-rule", default="reference", choices=["reference", "true_residual"])
     ap.add_argument("--ortho", default="mgs", choices=["mgs", "cgs2"])
+    ap.add_argument("--solver", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     return ap.parse_args()
@@ -122,7 +123,7 @@ def main():
     w = wl.config2(n=n, ppc=args.ppc, steps=args.steps)
     stream = torch.cuda.current_stream(dev)
     sim = Simulation(w, precision=args.precision, stop_rule=args.stop_rule, ortho=args.ortho,
-                     capacity=w.particles_total, stream=stream.cuda_stream)
+                     capacity=w.particles_total, stream=stream.cuda_stream, solver=args.solver)
     # fields: E = 0, B = B0 x (uniform); ghosted component planes
     N = w.N
     E = np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3))
@@ -192,16 +193,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic",
-        "config": {"workload": "implicit e-/p+ step, %dx%dx%d cells, %d ppc per species (%d particles), dt 0.1, "
-                               "theta 0.5, %s mover/deposit, GMRES %s stop, %s" % (
-                                   n[0], n[1], n[2], args.ppc, npart, 0, args.precision, args.stop_rule,
-                                   args.ortho) if False else
-                   "implicit e-/p+ step, %dx%dx%d cells, %d ppc per species (%d particles), %s mover/deposit, "
-                   "GMRES %s stop rule, %s orthogonalisation" % (n[0], n[1], n[2], args.ppc, npart, args.precision,
-                                                                   args.stop_rule, args.ortho),
-                   "cells": list(n), "particles": npart, "precision": args.precision},
+        "config": {"workload": "implicit e-/p+ step, %dx%dx%d cells, %d ppc per species (%d particles), %s mover/deposit, "
+                               "%s GMRES, %s stop rule, %s orthogonalisation" % (
+                                   n[0], n[1], n[2], args.ppc, npart, args.precision, args.solver, args.stop_rule,
+                                   args.ortho),
+                   "cells": list(n), "particles": npart, "precision": args.precision, "solver": args.solver},
         "gmres": {"ms_per_solve": gm_ms, "iterations_per_solve": gm_it, "stop_rule": args.stop_rule,
-                  "learned_preconditioner": False},
+                  "solver": args.solver, "learned_preconditioner": False,
+                  "true_residual": float(np.mean([s["true_residual"] for s in solves]))},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": per_launch},

@@ -226,8 +226,8 @@ void moments(pic_ctx* c) {
 	if (c->fp32) {
 		{
 			Timed t(c, "fields_to_fp32", (double)(g.nnodes + g.ncells) * 3 * 12);
-			launch_f64_to_f32(c->E, c->E32, 3 * g.nnodes, s);
-			launch_f64_to_f32(c->B, c->B32, 3 * g.ncells, s);
+			launch_f64_to_f32(c->E, c->E32, g.nnodes, s);
+			launch_f64_to_f32(c->B, c->B32, g.ncells, s);
 		}
 		{
 			Timed t(c, "node_moments", np * 28 + (double)g.nnodes * 12 * 4);
@@ -485,9 +485,9 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 			c->rscr32 = dalloc<float>(8 * g.ncells);
 			c->pscr32 = dalloc<float>(48 * g.ncells);
 			c->dscr32 = dalloc<float>((5 * cfg->nspecies + 1) * 8 * g.ncells);
-			c->E32 = dalloc<float>(3 * g.nnodes);
-			c->Et32 = dalloc<float>(3 * g.nnodes);
-			c->B32 = dalloc<float>(3 * g.ncells);
+			c->E32 = dalloc<float>(4 * g.nnodes);  // node-major float4
+			c->Et32 = dalloc<float>(4 * g.nnodes);
+			c->B32 = dalloc<float>(4 * g.ncells);
 			for (ParticleArrays32* Q : {&c->Q, &c->Qalt}) {
 				Q->cell = dalloc<int>(cfg->capacity);
 				for (int a = 0; a < 3; ++a) {
@@ -1044,9 +1044,9 @@ int pic_push(pic_ctx* c) {
 			const Grid& g = c->g;
 			{
 				Timed t(c, "fields_to_fp32", (double)(2 * g.nnodes + g.ncells) * 3 * 12);
-				launch_f64_to_f32(c->E, c->E32, 3 * g.nnodes, c->stream);
-				launch_f64_to_f32(c->Etheta, c->Et32, 3 * g.nnodes, c->stream);
-				launch_f64_to_f32(c->B, c->B32, 3 * g.ncells, c->stream);
+				launch_f64_to_f32(c->E, c->E32, g.nnodes, c->stream);
+				launch_f64_to_f32(c->Etheta, c->Et32, g.nnodes, c->stream);
+				launch_f64_to_f32(c->B, c->B32, g.ncells, c->stream);
 			}
 			Timed t(c, "push", (double)c->np() * 60 + ((double)g.nnodes * 6 + (double)g.ncells * 3) * 4);
 			launch_push32(g, c->sp, c->Q, c->species_start, c->E32, c->Et32, c->B32, (float)c->cfg.dt, (float)c->eta, c->keys,

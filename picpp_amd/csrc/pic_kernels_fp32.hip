@@ -69,36 +69,68 @@ __device__ inline bool maps32(const Grid& g, const int c[3], const float f[3], M
 	return ok;
 }
 
-__device__ inline void gather_node32(const Grid& g, const float* __restrict__ f, const Map32* m, float out[3]) {
+// FP32 field copies are node-major float4 (x, y, z, 0): one 16-byte load per stencil point.
+__device__ inline void gather_node32(const Grid& g, const float4* __restrict__ f, const Map32* m, float out[3]) {
 	out[0] = out[1] = out[2] = 0.0f;
+	const int sj = g.N[2] + 1, si = (g.N[1] + 1) * sj;
+	const int base = m[0].nb * si + m[1].nb * sj + m[2].nb;
 #pragma unroll
 	for (int i = 0; i < 2; ++i)
 #pragma unroll
 		for (int j = 0; j < 2; ++j)
 #pragma unroll
 			for (int k = 0; k < 2; ++k) {
-				long long id = node_id(g, m[0].nb + i, m[1].nb + j, m[2].nb + k);
+				float4 e = f[base + i * si + j * sj + k];
 				float c = m[0].nw[i] * m[1].nw[j] * m[2].nw[k];
-				out[0] += f[id] * c;
-				out[1] += f[g.nnodes + id] * c;
-				out[2] += f[2 * g.nnodes + id] * c;
+				out[0] += e.x * c;
+				out[1] += e.y * c;
+				out[2] += e.z * c;
 			}
 }
 
-__device__ inline void gather_cell32(const Grid& g, const float* __restrict__ f, const Map32* m, float out[3]) {
+__device__ inline void gather_cell32(const Grid& g, const float4* __restrict__ f, const Map32* m, float out[3]) {
 	out[0] = out[1] = out[2] = 0.0f;
+	const int sj = g.N[2], si = g.N[1] * sj;
+	const int base = m[0].cb * si + m[1].cb * sj + m[2].cb;
 #pragma unroll
 	for (int i = 0; i < 2; ++i)
 #pragma unroll
 		for (int j = 0; j < 2; ++j)
 #pragma unroll
 			for (int k = 0; k < 2; ++k) {
-				long long id = cell_id(g, m[0].cb + i, m[1].cb + j, m[2].cb + k);
+				float4 b = f[base + i * si + j * sj + k];
 				float c = m[0].cw[i] * m[1].cw[j] * m[2].cw[k];
-				out[0] += f[id] * c;
-				out[1] += f[g.ncells + id] * c;
-				out[2] += f[2 * g.ncells + id] * c;
+				out[0] += b.x * c;
+				out[1] += b.y * c;
+				out[2] += b.z * c;
 			}
+}
+
+// The 8 stencil values of a fixed stencil, loaded once and reused for every particle of
+// a segment (all particles of one key share both stencil bases).
+struct Stencil8 {
+	float v[8][3];
+};
+
+__device__ inline void load_stencil(const float4* __restrict__ f, int base, int si, int sj, Stencil8& s) {
+#pragma unroll
+	for (int t = 0; t < 8; ++t) {
+		float4 e = f[base + ((t >> 2) & 1) * si + ((t >> 1) & 1) * sj + (t & 1)];
+		s.v[t][0] = e.x;
+		s.v[t][1] = e.y;
+		s.v[t][2] = e.z;
+	}
+}
+
+__device__ inline void interp8(const Stencil8& s, const float w0[2], const float w1[2], const float w2[2], float out[3]) {
+	out[0] = out[1] = out[2] = 0.0f;
+#pragma unroll
+	for (int t = 0; t < 8; ++t) {
+		float c = w0[(t >> 2) & 1] * w1[(t >> 1) & 1] * w2[t & 1];
+		out[0] += s.v[t][0] * c;
+		out[1] += s.v[t][1] * c;
+		out[2] += s.v[t][2] * c;
+	}
 }
 
 // alpha tensor (PIC++/mover.cpp:851-908) in FP32, with v and gamma from u = gamma v.
@@ -134,7 +166,7 @@ __device__ inline void load32(const ParticleArrays32& P, int i, float f[3], floa
 // ------------------------------------------------------------------ moments at x^n
 // Node moments (J, mu): one group of G lanes per cell, lane = (species, octant).
 __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
-                                                        const float* __restrict__ E, const float* __restrict__ B, float dt,
+                                                        const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
                                                         float theta, int G, float* __restrict__ out) {
 	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	long long c = t / G;
@@ -156,14 +188,29 @@ __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp,
 		float pc = q * w * inv_v;
 		int cc[3];
 		cell_ijk(g, (int)c, cc);
+		int bit[3] = {(oct >> 2) & 1, (oct >> 1) & 1, oct & 1};
+		Stencil8 SE, SB;
+		if (hi > lo) {
+			const int nj = g.N[2] + 1, ni = (g.N[1] + 1) * nj;
+			load_stencil(E4, cc[0] * ni + cc[1] * nj + cc[2], ni, nj, SE);
+			const int cj = g.N[2], ci = g.N[1] * cj;
+			load_stencil(B4, (cc[0] - 1 + bit[0]) * ci + (cc[1] - 1 + bit[1]) * cj + (cc[2] - 1 + bit[2]), ci, cj, SB);
+		}
 		for (int i = lo; i < hi; ++i) {
 			float f[3], u[3];
 			load32(P, i, f, u);
 			Map32 mp[3];
-			maps32(g, cc, f, mp);
+			float cw[3][2];
+#pragma unroll
+			for (int a3 = 0; a3 < 3; ++a3) {
+				mp[a3].nw[0] = 1.0f - f[a3];
+				mp[a3].nw[1] = f[a3];
+				cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
+				cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
+			}
 			float Ef[3], Bf[3], a[9];
-			gather_node32(g, E, mp, Ef);
-			gather_cell32(g, B, mp, Bf);
+			interp8(SE, mp[0].nw, mp[1].nw, mp[2].nw, Ef);
+			interp8(SB, cw[0], cw[1], cw[2], Bf);
 			float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
 			float ig = 1.0f / gm;
 			float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
@@ -203,7 +250,7 @@ __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp,
 // Cell moments (rhohat corners and the reference's pressure-tensor partials) per dual cell b:
 // lane = (species, source octant); source cell b + a, octant bits 1 - a.
 __global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
-                                                        const float* __restrict__ E, const float* __restrict__ B, float dt,
+                                                        const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
                                                         int G, float* __restrict__ rho_out, float* __restrict__ pi_out) {
 	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	long long b = t / G;
@@ -228,14 +275,28 @@ __global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp,
 			float m = (float)sp.mass[s], q = (float)sp.charge[s], w = (float)sp.weight[s];
 			float beta = 0.5f * q * dt / m;
 			float pc = q * w * (float)(1.0 / g.volume);
+			int bit[3] = {1 - ax, 1 - ay, 1 - az};
+			Stencil8 SE, SB;
+			if (hi > lo) {
+				const int nj = g.N[2] + 1, ni = (g.N[1] + 1) * nj;
+				load_stencil(E4, mc[0] * ni + mc[1] * nj + mc[2], ni, nj, SE);
+				const int cj = g.N[2], ci = g.N[1] * cj;
+				load_stencil(B4, bc[0] * ci + bc[1] * cj + bc[2], ci, cj, SB);
+			}
 			for (int i = lo; i < hi; ++i) {
 				float f[3], u[3];
 				load32(P, i, f, u);
 				Map32 mp[3];
-				maps32(g, mc, f, mp);
+#pragma unroll
+				for (int a3 = 0; a3 < 3; ++a3) {
+					mp[a3].nw[0] = 1.0f - f[a3];
+					mp[a3].nw[1] = f[a3];
+					mp[a3].cw[0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
+					mp[a3].cw[1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
+				}
 				float Ef[3], Bf[3], a[9];
-				gather_node32(g, E, mp, Ef);
-				gather_cell32(g, B, mp, Bf);
+				interp8(SE, mp[0].nw, mp[1].nw, mp[2].nw, Ef);
+				interp8(SB, mp[0].cw, mp[1].cw, mp[2].cw, Bf);
 				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
 				float ig = 1.0f / gm;
 				float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
@@ -301,11 +362,16 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 			float m = (float)sp.mass[s], w = (float)sp.weight[s];
 			float wv = w * (float)(1.0 / g.volume);
 			float pc = (float)sp.charge[s] * wv;
+			int bit[3] = {1 - ax, 1 - ay, 1 - az};
 			for (int i = lo; i < hi; ++i) {
 				float f[3], u[3];
 				load32(P, i, f, u);
 				Map32 mp[3];
-				maps32(g, mc, f, mp);
+#pragma unroll
+				for (int a3 = 0; a3 < 3; ++a3) {
+					mp[a3].cw[0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
+					mp[a3].cw[1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
+				}
 				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
 #pragma unroll
 				for (int ti = 0; ti < 2; ++ti)
@@ -355,8 +421,8 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 // moveParticle (PIC++/mover.cpp:194-491) on cell offsets, one thread per particle; alpha is
 // recomputed from E^n, B^n at x^n. Momentum-space constants: du = (q dt / m) (E + v x B).
 __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start,
-                                                const float* __restrict__ E, const float* __restrict__ Et,
-                                                const float* __restrict__ B, float dt, float eta, uint32_t* __restrict__ keys,
+                                                const float4* __restrict__ E, const float4* __restrict__ Et,
+                                                const float4* __restrict__ B, float dt, float eta, uint32_t* __restrict__ keys,
                                                 int* __restrict__ status) {
 	long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (idx >= P.count) return;
@@ -523,9 +589,10 @@ __global__ void k_to_fp64(Grid g, SpeciesTable sp, ParticleArrays32 P, const int
 	}
 }
 
-__global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
+// FP64 component planes [3][n] -> node-major float4 (x, y, z, 0)
+__global__ void k_f64_to_f32(const double* __restrict__ a, float4* __restrict__ b, long long n) {
 	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) b[i] = (float)a[i];
+	if (i < n) b[i] = make_float4((float)a[i], (float)a[n + i], (float)a[2 * n + i], 0.0f);
 }
 
 // Node / cell values from float corner sums, accumulated in FP64.
@@ -646,13 +713,15 @@ int group_lanes(int nspecies) {
 void launch_node_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
                            const float* B, float dt, float theta, float* scratch, hipStream_t s) {
 	int G = group_lanes(sp.count);
-	hipLaunchKernelGGL(k_node_moments32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start, E, B, dt, theta, G,
+	hipLaunchKernelGGL(k_node_moments32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start,
+	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, theta, G,
 	                   scratch);
 }
 void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
                            const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s) {
 	int G = group_lanes(sp.count);
-	hipLaunchKernelGGL(k_cell_moments32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start, E, B, dt, G,
+	hipLaunchKernelGGL(k_cell_moments32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start,
+	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, G,
 	                   rho_scratch, pi_scratch);
 }
 void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
@@ -663,8 +732,9 @@ void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArray
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* status, hipStream_t s) {
 	if (P.count == 0) return;
-	hipLaunchKernelGGL(k_push32, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, E, Et, B, dt, eta, keys,
-	                   status);
+	hipLaunchKernelGGL(k_push32, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start,
+	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(Et), reinterpret_cast<const float4*>(B), dt,
+	                   eta, keys, status);
 }
 void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
                    hipStream_t s) {
@@ -686,7 +756,7 @@ void launch_to_fp64(const Grid& g, const SpeciesTable& sp, const ParticleArrays3
 	hipLaunchKernelGGL(k_to_fp64, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, x, p);
 }
 void launch_f64_to_f32(const double* a, float* b, long long n, hipStream_t s) {
-	hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks32(n, 256)), dim3(256), 0, s, a, b, n);
+	hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks32(n, 256)), dim3(256), 0, s, a, reinterpret_cast<float4*>(b), n);
 }
 void launch_node_assemble32(const Grid& g, const float* scratch, double* J, double* mu, hipStream_t s) {
 	hipLaunchKernelGGL(k_node_assemble32, dim3(blocks32(g.nnodes, 256)), dim3(256), 0, s, g, scratch, J, mu);
