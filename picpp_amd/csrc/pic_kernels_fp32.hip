@@ -163,16 +163,209 @@ __device__ inline void load32(const ParticleArrays32& P, int i, float f[3], floa
 	}
 }
 
+// ------------------------------------------------------------------ LDS staging
+// A deposit workgroup first copies the particle state it will read (offset and u, 24 bytes
+// per particle) into LDS with coalesced loads; the state of one tile is a few contiguous
+// index runs of the sorted arrays. Lanes then walk their segments out of LDS in the same
+// order as from global memory, so the staged and direct paths give bitwise equal sums.
+// A tile whose particles exceed the LDS capacity reads global memory directly.
+constexpr int kStageCap = 2560;  // particles per tile at most (61,440 B of LDS)
+constexpr int kMaxTile = 16;     // cells per tile
+
+struct GlobalSrc {
+	const float* f0;
+	const float* f1;
+	const float* f2;
+	const float* u0;
+	const float* u1;
+	const float* u2;
+	__device__ inline void get(int i, float f[3], float u[3]) const {
+		f[0] = f0[i];
+		f[1] = f1[i];
+		f[2] = f2[i];
+		u[0] = u0[i];
+		u[1] = u1[i];
+		u[2] = u2[i];
+	}
+};
+
+__device__ inline GlobalSrc global_src(const ParticleArrays32& P) {
+	return GlobalSrc{P.off[0], P.off[1], P.off[2], P.u[0], P.u[1], P.u[2]};
+}
+
+struct LdsSrc {
+	const float* sm;  // planes [6][cap]
+	int cap;
+	int shift;  // LDS index = particle index + shift
+	__device__ inline void get(int i, float f[3], float u[3]) const {
+		int e = i + shift;
+#pragma unroll
+		for (int a = 0; a < 3; ++a) {
+			f[a] = sm[a * cap + e];
+			u[a] = sm[(3 + a) * cap + e];
+		}
+	}
+};
+
+// Dynamic LDS of a deposit tile: state planes [6][cap], then run starts and offsets.
+struct TileLds {
+	float* sm;
+	int* run_lo;
+	int* run_off;
+};
+
+__device__ inline TileLds tile_lds(int cap, int max_runs) {
+	extern __shared__ float lds_pool[];
+	TileLds t;
+	t.sm = lds_pool;
+	t.run_lo = reinterpret_cast<int*>(lds_pool + 6 * cap);
+	t.run_off = t.run_lo + max_runs;
+	return t;
+}
+
+// run_off[0..n-1] holds run lengths on entry and exclusive offsets on exit, run_off[n] the
+// total. Wave 0 scans; blockDim.x must be at least 64.
+__device__ inline void scan_runs(int* run_off, int n) {
+	if (threadIdx.x < 64) {
+		int lane = threadIdx.x;
+		int per = (n + 63) / 64;
+		int b = lane * per;
+		int loc = 0;
+		for (int q = 0; q < per; ++q)
+			if (b + q < n) loc += run_off[b + q];
+		int inc = loc;
+		for (int o = 1; o < 64; o <<= 1) {
+			int y = __shfl_up(inc, o, 64);
+			if (lane >= o) inc += y;
+		}
+		int ex = inc - loc;
+		for (int q = 0; q < per; ++q)
+			if (b + q < n) {
+				int l = run_off[b + q];
+				run_off[b + q] = ex;
+				ex += l;
+			}
+		if (lane == 63) run_off[n] = inc;
+	}
+	__syncthreads();
+}
+
+// Copies runs [run_lo[r], run_lo[r] + len_r) to LDS offsets run_off[r]; total <= cap.
+template <int K>
+__device__ inline void stage_runs(const ParticleArrays32& P, const int* run_lo, const int* run_off, int nruns, int total,
+                                  float* sm, int cap) {
+	for (int base = 0; base < total; base += K * (int)blockDim.x) {
+		int gi[K];
+		float v[K][6];
+#pragma unroll
+		for (int k = 0; k < K; ++k) {
+			int e = base + k * (int)blockDim.x + (int)threadIdx.x;
+			gi[k] = -1;
+			if (e < total) {
+				int lo = 0, hi = nruns - 1;  // last run whose offset is <= e
+				while (lo < hi) {
+					int mid = (lo + hi + 1) >> 1;
+					if (run_off[mid] <= e)
+						lo = mid;
+					else
+						hi = mid - 1;
+				}
+				gi[k] = run_lo[lo] + (e - run_off[lo]);
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < K; ++k)
+			if (gi[k] >= 0) {
+#pragma unroll
+				for (int a = 0; a < 3; ++a) {
+					v[k][a] = P.off[a][gi[k]];
+					v[k][3 + a] = P.u[a][gi[k]];
+				}
+			}
+#pragma unroll
+		for (int k = 0; k < K; ++k)
+			if (gi[k] >= 0) {
+				int e = base + k * (int)blockDim.x + (int)threadIdx.x;
+#pragma unroll
+				for (int q = 0; q < 6; ++q) sm[q * cap + e] = v[k][q];
+			}
+	}
+	__syncthreads();
+}
+
 // ------------------------------------------------------------------ moments at x^n
-// Node moments (J, mu): one group of G lanes per cell, lane = (species, octant).
+// Node moments (J, mu) of one (species, octant) segment of cell-sorted particles.
+template <class Src>
+__device__ __forceinline__ void node_segment(const Src& src, int lo, int hi, const Stencil8& SE, const Stencil8& SB, const int bit[3],
+                                             float beta, float omega, float pc, float acc[8][12]) {
+	for (int i = lo; i < hi; ++i) {
+		float f[3], u[3];
+		src.get(i, f, u);
+		Map32 mp[3];
+		float cw[3][2];
+#pragma unroll
+		for (int a3 = 0; a3 < 3; ++a3) {
+			mp[a3].nw[0] = 1.0f - f[a3];
+			mp[a3].nw[1] = f[a3];
+			cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
+			cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
+		}
+		float Ef[3], Bf[3], a[9];
+		interp8(SE, mp[0].nw, mp[1].nw, mp[2].nw, Ef);
+		interp8(SB, cw[0], cw[1], cw[2], Bf);
+		float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+		float ig = 1.0f / gm;
+		float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
+		alpha32(beta, v, gm, Ef, Bf, a);
+		float rv[3];
+#pragma unroll
+		for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
+#pragma unroll
+		for (int ci = 0; ci < 2; ++ci)
+#pragma unroll
+			for (int cj = 0; cj < 2; ++cj)
+#pragma unroll
+				for (int ck = 0; ck < 2; ++ck) {
+					int corner = ci * 4 + cj * 2 + ck;
+					float corr = mp[0].nw[ci] * mp[1].nw[cj] * mp[2].nw[ck];
+					float fj = pc * corr;
+#pragma unroll
+					for (int r = 0; r < 3; ++r) acc[corner][r] += rv[r] * fj;
+					float o = omega * corr;
+#pragma unroll
+					for (int r = 0; r < 9; ++r) acc[corner][3 + r] -= a[r] * o;
+				}
+	}
+}
+
+// One group of G lanes per cell, lane = (species, octant); a workgroup serves T consecutive
+// cells, whose particles are one index run per species.
 __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
                                                         const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
-                                                        float theta, int G, float* __restrict__ out) {
-	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	long long c = t / G;
-	int lane = (int)(t % G);
+                                                        float theta, int G, int T, int cap, float* __restrict__ out) {
+	const TileLds L = tile_lds(cap, kMaxSpecies);
+	float* sm = L.sm;
+	int* run_lo = L.run_lo;
+	int* run_off = L.run_off;
+	const long long c0 = (long long)blockIdx.x * T;
+	const int lt = threadIdx.x;
+	if (lt < sp.count) {
+		long long cend = c0 + T < g.ncells ? c0 + T : g.ncells;
+		int lo = start[((long long)lt * g.ncells + c0) << 3];
+		int hi = start[((long long)lt * g.ncells + cend) << 3];
+		run_lo[lt] = lo;
+		run_off[lt] = hi - lo;
+	}
+	__syncthreads();
+	scan_runs(run_off, sp.count);
+	const int total = run_off[sp.count];
+	const bool staged = total <= cap;
+	if (staged) stage_runs<4>(P, run_lo, run_off, sp.count, total, sm, cap);
+
+	long long c = c0 + lt / G;
+	int lane = lt % G;
 	int s = lane >> 3, oct = lane & 7;
-	bool active = c < g.ncells && s < sp.count;
+	bool active = lt < T * G && c < g.ncells && s < sp.count;
 	float acc[8][12];
 #pragma unroll
 	for (int a = 0; a < 8; ++a)
@@ -196,50 +389,16 @@ __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp,
 			const int cj = g.N[2], ci = g.N[1] * cj;
 			load_stencil(B4, (cc[0] - 1 + bit[0]) * ci + (cc[1] - 1 + bit[1]) * cj + (cc[2] - 1 + bit[2]), ci, cj, SB);
 		}
-		for (int i = lo; i < hi; ++i) {
-			float f[3], u[3];
-			load32(P, i, f, u);
-			Map32 mp[3];
-			float cw[3][2];
-#pragma unroll
-			for (int a3 = 0; a3 < 3; ++a3) {
-				mp[a3].nw[0] = 1.0f - f[a3];
-				mp[a3].nw[1] = f[a3];
-				cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
-				cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
-			}
-			float Ef[3], Bf[3], a[9];
-			interp8(SE, mp[0].nw, mp[1].nw, mp[2].nw, Ef);
-			interp8(SB, cw[0], cw[1], cw[2], Bf);
-			float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-			float ig = 1.0f / gm;
-			float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
-			alpha32(beta, v, gm, Ef, Bf, a);
-			float rv[3];
-#pragma unroll
-			for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
-#pragma unroll
-			for (int ci = 0; ci < 2; ++ci)
-#pragma unroll
-				for (int cj = 0; cj < 2; ++cj)
-#pragma unroll
-					for (int ck = 0; ck < 2; ++ck) {
-						int corner = ci * 4 + cj * 2 + ck;
-						float corr = mp[0].nw[ci] * mp[1].nw[cj] * mp[2].nw[ck];
-						float fj = pc * corr;
-#pragma unroll
-						for (int r = 0; r < 3; ++r) acc[corner][r] += rv[r] * fj;
-						float o = omega * corr;
-#pragma unroll
-						for (int r = 0; r < 9; ++r) acc[corner][3 + r] -= a[r] * o;
-					}
-		}
+		if (staged)
+			node_segment(LdsSrc{sm, cap, run_off[s] - run_lo[s]}, lo, hi, SE, SB, bit, beta, omega, pc, acc);
+		else
+			node_segment(global_src(P), lo, hi, SE, SB, bit, beta, omega, pc, acc);
 	}
 #pragma unroll
 	for (int a = 0; a < 8; ++a)
 #pragma unroll
 		for (int q = 0; q < 12; ++q) acc[a][q] = group_sum(acc[a][q], G);
-	if (lane == 0 && c < g.ncells) {
+	if (lane == 0 && lt < T * G && c < g.ncells) {
 #pragma unroll
 		for (int a = 0; a < 8; ++a)
 #pragma unroll
@@ -247,15 +406,108 @@ __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp,
 	}
 }
 
-// Cell moments (rhohat corners and the reference's pressure-tensor partials) per dual cell b:
-// lane = (species, source octant); source cell b + a, octant bits 1 - a.
+// Dual-cell tiles: a workgroup serves T base cells b = (i, j, k0 .. k0+T-1). Base b collects
+// the segments (species, cell b + a, octant 1 - a), a in {0,1}^3. For each species and
+// (ax, ay) the needed segments are, per source cell (i+ax, j+ay, k), the two octants with
+// x bit 1-ax and y bit 1-ay: one index run per (species, ax, ay, k), k = k0 .. k0+T.
+struct DualTile {
+	int i, j, k0;
+};
+
+__device__ inline DualTile dual_tile(const Grid& g, int T) {
+	int nt = (g.N[2] + T - 1) / T;
+	int b = blockIdx.x;
+	DualTile t;
+	t.k0 = (b % nt) * T;
+	b /= nt;
+	t.j = b % g.N[1];
+	t.i = b / g.N[1];
+	return t;
+}
+
+__device__ inline int dual_runs(const Grid& g, const SpeciesTable& sp, const int* __restrict__ start, const DualTile& t, int T,
+                                int* run_lo, int* run_off) {
+	int nruns = sp.count * 4 * (T + 1);
+	for (int r = threadIdx.x; r < nruns; r += blockDim.x) {
+		int kk = r % (T + 1);
+		int pr = (r / (T + 1)) % 4;
+		int s = r / (4 * (T + 1));
+		int ax = pr >> 1, ay = pr & 1;
+		int ci = t.i + ax, cj = t.j + ay, ck = t.k0 + kk;
+		int lo = 0, hi = 0;
+		if (ci < g.N[0] && cj < g.N[1] && ck < g.N[2]) {
+			long long key = (((long long)s * g.ncells + cell_id(g, ci, cj, ck)) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1);
+			lo = start[key];
+			hi = start[key + 2];
+		}
+		run_lo[r] = lo;
+		run_off[r] = hi - lo;
+	}
+	__syncthreads();
+	scan_runs(run_off, nruns);
+	return nruns;
+}
+
+// Cell moments (rhohat corners and the reference's pressure-tensor partials) of one segment.
+template <class Src>
+__device__ __forceinline__ void cell_segment(const Src& src, int lo, int hi, const Stencil8& SE, const Stencil8& SB, const int bit[3],
+                                             float beta, float pc, float ar[8], float ap[8][6]) {
+	for (int i = lo; i < hi; ++i) {
+		float f[3], u[3];
+		src.get(i, f, u);
+		Map32 mp[3];
+#pragma unroll
+		for (int a3 = 0; a3 < 3; ++a3) {
+			mp[a3].nw[0] = 1.0f - f[a3];
+			mp[a3].nw[1] = f[a3];
+			mp[a3].cw[0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
+			mp[a3].cw[1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
+		}
+		float Ef[3], Bf[3], a[9];
+		interp8(SE, mp[0].nw, mp[1].nw, mp[2].nw, Ef);
+		interp8(SB, mp[0].cw, mp[1].cw, mp[2].cw, Bf);
+		float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+		float ig = 1.0f / gm;
+		float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
+		alpha32(beta, v, gm, Ef, Bf, a);
+		float rv[3];
+#pragma unroll
+		for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
+		float Tt[6] = {rv[0] * rv[0], rv[0] * rv[1], rv[0] * rv[2], rv[1] * rv[1], rv[1] * rv[2], rv[2] * rv[2]};
+#pragma unroll
+		for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+			for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+				for (int tk = 0; tk < 2; ++tk) {
+					int tt = ti * 4 + tj * 2 + tk;
+					float fr = pc * mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk];
+					ar[tt] += fr;
+#pragma unroll
+					for (int r = 0; r < 6; ++r) ap[tt][r] += fr * Tt[r];
+				}
+	}
+}
+
+// Cell moments per dual cell b: lane = (species, source octant); source cell b + a, octant bits 1 - a.
 __global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
                                                         const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
-                                                        int G, float* __restrict__ rho_out, float* __restrict__ pi_out) {
-	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	long long b = t / G;
-	int lane = (int)(t % G);
+                                                        int G, int T, int cap, float* __restrict__ rho_out,
+                                                        float* __restrict__ pi_out) {
+	const TileLds L = tile_lds(cap, sp.count * 4 * (T + 1));
+	float* sm = L.sm;
+	const DualTile tile = dual_tile(g, T);
+	const int nruns = dual_runs(g, sp, start, tile, T, L.run_lo, L.run_off);
+	const int total = L.run_off[nruns];
+	const bool staged = total <= cap;
+	if (staged) stage_runs<4>(P, L.run_lo, L.run_off, nruns, total, sm, cap);
+
+	const int lt = threadIdx.x;
+	const int kb = lt / G;
+	int lane = lt % G;
 	int s = lane >> 3, seg = lane & 7;
+	const int bk = tile.k0 + kb;
+	const bool base_ok = kb < T && bk < g.N[2];
 	float ar[8], ap[8][6];
 #pragma unroll
 	for (int q = 0; q < 8; ++q) {
@@ -263,11 +515,9 @@ __global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp,
 #pragma unroll
 		for (int r = 0; r < 6; ++r) ap[q][r] = 0.0f;
 	}
-	if (b < g.ncells && s < sp.count) {
-		int bc[3];
-		cell_ijk(g, (int)b, bc);
+	if (base_ok && s < sp.count) {
 		int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
-		int mc[3] = {bc[0] + ax, bc[1] + ay, bc[2] + az};
+		int mc[3] = {tile.i + ax, tile.j + ay, bk + az};
 		if (mc[0] < g.N[0] && mc[1] < g.N[1] && mc[2] < g.N[2]) {
 			long long key = (((long long)s * g.ncells + cell_id(g, mc[0], mc[1], mc[2])) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) |
 			                (1 - az);
@@ -281,42 +531,13 @@ __global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp,
 				const int nj = g.N[2] + 1, ni = (g.N[1] + 1) * nj;
 				load_stencil(E4, mc[0] * ni + mc[1] * nj + mc[2], ni, nj, SE);
 				const int cj = g.N[2], ci = g.N[1] * cj;
-				load_stencil(B4, bc[0] * ci + bc[1] * cj + bc[2], ci, cj, SB);
+				load_stencil(B4, tile.i * ci + tile.j * cj + bk, ci, cj, SB);
 			}
-			for (int i = lo; i < hi; ++i) {
-				float f[3], u[3];
-				load32(P, i, f, u);
-				Map32 mp[3];
-#pragma unroll
-				for (int a3 = 0; a3 < 3; ++a3) {
-					mp[a3].nw[0] = 1.0f - f[a3];
-					mp[a3].nw[1] = f[a3];
-					mp[a3].cw[0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
-					mp[a3].cw[1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
-				}
-				float Ef[3], Bf[3], a[9];
-				interp8(SE, mp[0].nw, mp[1].nw, mp[2].nw, Ef);
-				interp8(SB, mp[0].cw, mp[1].cw, mp[2].cw, Bf);
-				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-				float ig = 1.0f / gm;
-				float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
-				alpha32(beta, v, gm, Ef, Bf, a);
-				float rv[3];
-#pragma unroll
-				for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
-				float T[6] = {rv[0] * rv[0], rv[0] * rv[1], rv[0] * rv[2], rv[1] * rv[1], rv[1] * rv[2], rv[2] * rv[2]};
-#pragma unroll
-				for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-					for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-						for (int tk = 0; tk < 2; ++tk) {
-							int tt = ti * 4 + tj * 2 + tk;
-							float fr = pc * mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk];
-							ar[tt] += fr;
-#pragma unroll
-							for (int r = 0; r < 6; ++r) ap[tt][r] += fr * T[r];
-						}
+			if (staged) {
+				int r = (s * 4 + ax * 2 + ay) * (T + 1) + kb + az;
+				cell_segment(LdsSrc{sm, cap, L.run_off[r] - L.run_lo[r]}, lo, hi, SE, SB, bit, beta, pc, ar, ap);
+			} else {
+				cell_segment(global_src(P), lo, hi, SE, SB, bit, beta, pc, ar, ap);
 			}
 		}
 	}
@@ -326,7 +547,8 @@ __global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp,
 #pragma unroll
 		for (int r = 0; r < 6; ++r) ap[q][r] = group_sum(ap[q][r], G);
 	}
-	if (lane == 0 && b < g.ncells) {
+	if (lane == 0 && base_ok) {
+		long long b = cell_id(g, tile.i, tile.j, bk);
 #pragma unroll
 		for (int q = 0; q < 8; ++q) {
 			rho_out[(long long)q * g.ncells + b] = ar[q];
@@ -336,25 +558,65 @@ __global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp,
 	}
 }
 
-// Post-push density (updateDensityParameters) per dual cell: lane = (species, source octant).
+// Post-push density (updateDensityParameters) of one segment.
+template <class Src>
+__device__ __forceinline__ void density_segment(const Src& src, int lo, int hi, const int bit[3], float m, float wv, float pc,
+                                                float conc[8], float en[8], float bulk[8][3], float rho[8]) {
+	for (int i = lo; i < hi; ++i) {
+		float f[3], u[3];
+		src.get(i, f, u);
+		float cw[3][2];
+#pragma unroll
+		for (int a3 = 0; a3 < 3; ++a3) {
+			cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
+			cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
+		}
+		float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+#pragma unroll
+		for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+			for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+				for (int tk = 0; tk < 2; ++tk) {
+					int tt = ti * 4 + tj * 2 + tk;
+					float corr = cw[0][ti] * cw[1][tj] * cw[2][tk];
+					float c = wv * corr;
+					conc[tt] += c;
+					en[tt] += c * gm * m;
+					rho[tt] += pc * corr;
+#pragma unroll
+					for (int r = 0; r < 3; ++r) bulk[tt][r] += c * u[r] * m;
+				}
+	}
+}
+
+// Post-push density per dual cell: lane = (species, source octant).
 // Output planes (float): per species [conc, energy, bulk x3] x 8 corners, then rho x 8.
 __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start, int G,
-                                                   float* __restrict__ out) {
-	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	long long b = t / G;
-	int lane = (int)(t % G);
+                                                   int T, int cap, float* __restrict__ out) {
+	const TileLds L = tile_lds(cap, sp.count * 4 * (T + 1));
+	float* sm = L.sm;
+	const DualTile tile = dual_tile(g, T);
+	const int nruns = dual_runs(g, sp, start, tile, T, L.run_lo, L.run_off);
+	const int total = L.run_off[nruns];
+	const bool staged = total <= cap;
+	if (staged) stage_runs<4>(P, L.run_lo, L.run_off, nruns, total, sm, cap);
+
+	const int lt = threadIdx.x;
+	const int kb = lt / G;
+	int lane = lt % G;
 	int s = lane >> 3, seg = lane & 7;
+	const int bk = tile.k0 + kb;
+	const bool base_ok = kb < T && bk < g.N[2];
 	float conc[8], en[8], bulk[8][3], rho[8];
 #pragma unroll
 	for (int q = 0; q < 8; ++q) {
 		conc[q] = en[q] = rho[q] = 0.0f;
 		bulk[q][0] = bulk[q][1] = bulk[q][2] = 0.0f;
 	}
-	if (b < g.ncells && s < sp.count) {
-		int bc[3];
-		cell_ijk(g, (int)b, bc);
+	if (base_ok && s < sp.count) {
 		int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
-		int mc[3] = {bc[0] + ax, bc[1] + ay, bc[2] + az};
+		int mc[3] = {tile.i + ax, tile.j + ay, bk + az};
 		if (mc[0] < g.N[0] && mc[1] < g.N[1] && mc[2] < g.N[2]) {
 			long long key = (((long long)s * g.ncells + cell_id(g, mc[0], mc[1], mc[2])) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) |
 			                (1 - az);
@@ -363,31 +625,11 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 			float wv = w * (float)(1.0 / g.volume);
 			float pc = (float)sp.charge[s] * wv;
 			int bit[3] = {1 - ax, 1 - ay, 1 - az};
-			for (int i = lo; i < hi; ++i) {
-				float f[3], u[3];
-				load32(P, i, f, u);
-				Map32 mp[3];
-#pragma unroll
-				for (int a3 = 0; a3 < 3; ++a3) {
-					mp[a3].cw[0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
-					mp[a3].cw[1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
-				}
-				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-#pragma unroll
-				for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-					for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-						for (int tk = 0; tk < 2; ++tk) {
-							int tt = ti * 4 + tj * 2 + tk;
-							float corr = mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk];
-							float cw = wv * corr;
-							conc[tt] += cw;
-							en[tt] += cw * gm * m;
-							rho[tt] += pc * corr;
-#pragma unroll
-							for (int r = 0; r < 3; ++r) bulk[tt][r] += cw * u[r] * m;
-						}
+			if (staged) {
+				int r = (s * 4 + ax * 2 + ay) * (T + 1) + kb + az;
+				density_segment(LdsSrc{sm, cap, L.run_off[r] - L.run_lo[r]}, lo, hi, bit, m, wv, pc, conc, en, bulk, rho);
+			} else {
+				density_segment(global_src(P), lo, hi, bit, m, wv, pc, conc, en, bulk, rho);
 			}
 		}
 	}
@@ -400,20 +642,23 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 		for (int r = 0; r < 3; ++r) bulk[q][r] = group_sum(bulk[q][r], 8);
 		rho[q] = group_sum(rho[q], G);
 	}
-	if (b < g.ncells && s < sp.count && seg == 0) {
-		long long base = (long long)s * 5 * 8;
+	if (base_ok) {
+		long long b = cell_id(g, tile.i, tile.j, bk);
+		if (s < sp.count && seg == 0) {
+			long long base = (long long)s * 5 * 8;
 #pragma unroll
-		for (int q = 0; q < 8; ++q) {
-			out[(base + 0 * 8 + q) * g.ncells + b] = conc[q];
-			out[(base + 1 * 8 + q) * g.ncells + b] = en[q];
+			for (int q = 0; q < 8; ++q) {
+				out[(base + 0 * 8 + q) * g.ncells + b] = conc[q];
+				out[(base + 1 * 8 + q) * g.ncells + b] = en[q];
 #pragma unroll
-			for (int r = 0; r < 3; ++r) out[(base + (2 + r) * 8 + q) * g.ncells + b] = bulk[q][r];
+				for (int r = 0; r < 3; ++r) out[(base + (2 + r) * 8 + q) * g.ncells + b] = bulk[q][r];
+			}
 		}
-	}
-	if (b < g.ncells && lane == 0) {
-		long long base = (long long)sp.count * 5 * 8;
+		if (lane == 0) {
+			long long base = (long long)sp.count * 5 * 8;
 #pragma unroll
-		for (int q = 0; q < 8; ++q) out[(base + q) * g.ncells + b] = rho[q];
+			for (int q = 0; q < 8; ++q) out[(base + q) * g.ncells + b] = rho[q];
+		}
 	}
 }
 
@@ -710,24 +955,48 @@ int group_lanes(int nspecies) {
 	return G;
 }
 
+// Deposit tile: T cells per workgroup (as many as fit 256 lanes, at most kMaxTile), halved
+// while the expected particles of a tile (mean per owned cell x T x 1.3) exceed kStageCap;
+// the LDS capacity is that expectation rounded up to 64. At least 64 lanes per workgroup.
+struct DepositTile {
+	int T, cap;
+	size_t lds_bytes(int max_runs) const { return (size_t)6 * cap * sizeof(float) + (size_t)(2 * max_runs + 1) * sizeof(int); }
+};
+
+DepositTile deposit_tile(const Grid& g, const ParticleArrays32& P, int G) {
+	const double margin = 1.3;
+	int T = 256 / G < kMaxTile ? 256 / G : kMaxTile;
+	double mean = (double)P.count / (double)(g.n[0] * (long long)g.n[1] * g.n[2]);
+	while (T * G > 64 && mean * T * margin > kStageCap) T /= 2;
+	double want = mean * T * margin;
+	int cap = want >= kStageCap ? kStageCap : (((int)want + 63) / 64) * 64;
+	if (cap < 64) cap = 64;
+	return DepositTile{T, cap};
+}
+
 void launch_node_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
                            const float* B, float dt, float theta, float* scratch, hipStream_t s) {
 	int G = group_lanes(sp.count);
-	hipLaunchKernelGGL(k_node_moments32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start,
-	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, theta, G,
-	                   scratch);
+	DepositTile t = deposit_tile(g, P, G);
+	hipLaunchKernelGGL(k_node_moments32, dim3(blocks32(g.ncells, t.T)), dim3(t.T * G), t.lds_bytes(kMaxSpecies), s, g, sp, P, start,
+	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, theta, G, t.T, t.cap, scratch);
 }
 void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
                            const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s) {
 	int G = group_lanes(sp.count);
-	hipLaunchKernelGGL(k_cell_moments32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start,
-	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, G,
-	                   rho_scratch, pi_scratch);
+	DepositTile t = deposit_tile(g, P, G);
+	unsigned blocks = (unsigned)(g.N[0] * g.N[1] * ((g.N[2] + t.T - 1) / t.T));
+	hipLaunchKernelGGL(k_cell_moments32, dim3(blocks), dim3(t.T * G), t.lds_bytes(sp.count * 4 * (t.T + 1)), s, g, sp, P, start,
+	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, G, t.T, t.cap, rho_scratch,
+	                   pi_scratch);
 }
 void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
                       hipStream_t s) {
 	int G = group_lanes(sp.count);
-	hipLaunchKernelGGL(k_density32, dim3(blocks32(g.ncells * G, 256)), dim3(256), 0, s, g, sp, P, start, G, scratch);
+	DepositTile t = deposit_tile(g, P, G);
+	unsigned blocks = (unsigned)(g.N[0] * g.N[1] * ((g.N[2] + t.T - 1) / t.T));
+	hipLaunchKernelGGL(k_density32, dim3(blocks), dim3(t.T * G), t.lds_bytes(sp.count * 4 * (t.T + 1)), s, g, sp, P, start, G, t.T,
+	                   t.cap, scratch);
 }
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* status, hipStream_t s) {

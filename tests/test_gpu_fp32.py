@@ -16,10 +16,8 @@ pytestmark = pytest.mark.gpu
 TOL32 = 2e-5
 
 
-def test_fp32_first_step_errors_against_fp64_oracle():
-    w = wl.mini(n=(12, 10, 8), ppc=8, seed=21)
-    st = wl.make_initial_state(w, perturb=0.05)
-    print("case", w.name, flush=True)
+def first_step_errors(w, st):
+    """Normwise relative error of every FP32 quantity after one step, against the oracle."""
     sim = Simulation(w, precision="fp32")
     sim.load_state(st)
     orc = OracleSim(w, st)
@@ -59,10 +57,35 @@ def test_fp32_first_step_errors_against_fp64_oracle():
     orc.exchange_fields()
     orc.update_energy()
     errs["energy"] = relerr(sim.updateEnergy()[:4], orc.energy()[:4])
+    sim.close()
+    return errs
+
+
+def test_fp32_first_step_errors_against_fp64_oracle():
+    w = wl.mini(n=(12, 10, 8), ppc=8, seed=21)
+    st = wl.make_initial_state(w, perturb=0.05)
+    print("case", w.name, flush=True)
+    errs = first_step_errors(w, st)
     print("fp32 vs fp64 oracle:", {k: "%.2e" % v for k, v in errs.items()})
     for k, v in errs.items():
         assert v <= TOL32, (k, v)
-    sim.close()
+
+
+def test_fp32_nonuniform_density_uses_both_deposit_paths():
+    """Particles piled up toward the low-x side (density ~ 1/sqrt): tiles in the dense
+    region hold more particles than the LDS staging capacity sized from the mean, so the
+    deposits take their direct-from-HBM path there and the staged path elsewhere. Both must
+    agree with the oracle to the FP32 bar."""
+    w = wl.mini(n=(16, 8, 8), ppc=12, seed=5)
+    st = wl.make_initial_state(w, perturb=0.05)
+    L = w.n[0] * w.dx
+    r = (st.x[:, 0] - L) / L
+    st.x[:, 0] = L + L * r * r
+    print("case", w.name, "compressed along x", flush=True)
+    errs = first_step_errors(w, st)
+    print("fp32 vs fp64 oracle:", {k: "%.2e" % v for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v <= TOL32, (k, v)
 
 
 def test_fp32_deterministic_bitwise():
