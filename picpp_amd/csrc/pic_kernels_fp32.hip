@@ -8,9 +8,9 @@
 // own cell — the closed forms of correlationBspline (PIC++/correlation.cpp:824-835).
 //
 // Deposits are deterministic without atomics. Particles are sorted by key (species, cell,
-// octant); a group of G = 8 * species lanes serves one cell (node moments) or one dual
-// cell (cell moments), one lane per (species, octant) segment. Each lane sums its segment
-// in order, the group then reduces with a fixed xor-shuffle tree and writes 8 corner sums.
+// octant); a group of G lanes serves one cell (node moments) or one dual cell (cell moments,
+// density) and takes its particles round-robin. Each lane sums its particles in that fixed
+// order, the group then reduces with a fixed xor-shuffle tree and writes 8 corner sums.
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
 #include "pic_kernels_fp32.hpp"
@@ -163,253 +163,14 @@ __device__ inline void load32(const ParticleArrays32& P, int i, float f[3], floa
 	}
 }
 
-// ------------------------------------------------------------------ LDS staging
-// A deposit workgroup first copies the particle state it will read (offset and u, 24 bytes
-// per particle) into LDS with coalesced loads; the state of one tile is a few contiguous
-// index runs of the sorted arrays. Lanes then walk their segments out of LDS in the same
-// order as from global memory, so the staged and direct paths give bitwise equal sums.
-// A tile whose particles exceed the LDS capacity reads global memory directly.
-constexpr int kStageCap = 2560;  // particles per tile at most (61,440 B of LDS)
-constexpr int kMaxTile = 16;     // cells per tile
-
-struct GlobalSrc {
-	const float* f0;
-	const float* f1;
-	const float* f2;
-	const float* u0;
-	const float* u1;
-	const float* u2;
-	__device__ inline void get(int i, float f[3], float u[3]) const {
-		f[0] = f0[i];
-		f[1] = f1[i];
-		f[2] = f2[i];
-		u[0] = u0[i];
-		u[1] = u1[i];
-		u[2] = u2[i];
-	}
-};
-
-__device__ inline GlobalSrc global_src(const ParticleArrays32& P) {
-	return GlobalSrc{P.off[0], P.off[1], P.off[2], P.u[0], P.u[1], P.u[2]};
-}
-
-struct LdsSrc {
-	const float* sm;  // planes [6][cap]
-	int cap;
-	int shift;  // LDS index = particle index + shift
-	__device__ inline void get(int i, float f[3], float u[3]) const {
-		int e = i + shift;
-#pragma unroll
-		for (int a = 0; a < 3; ++a) {
-			f[a] = sm[a * cap + e];
-			u[a] = sm[(3 + a) * cap + e];
-		}
-	}
-};
-
-// Dynamic LDS of a deposit tile: state planes [6][cap], then run starts and offsets.
-struct TileLds {
-	float* sm;
-	int* run_lo;
-	int* run_off;
-};
-
-__device__ inline TileLds tile_lds(int cap, int max_runs) {
-	extern __shared__ float lds_pool[];
-	TileLds t;
-	t.sm = lds_pool;
-	t.run_lo = reinterpret_cast<int*>(lds_pool + 6 * cap);
-	t.run_off = t.run_lo + max_runs;
-	return t;
-}
-
-// run_off[0..n-1] holds run lengths on entry and exclusive offsets on exit, run_off[n] the
-// total. Wave 0 scans; blockDim.x must be at least 64.
-__device__ inline void scan_runs(int* run_off, int n) {
-	if (threadIdx.x < 64) {
-		int lane = threadIdx.x;
-		int per = (n + 63) / 64;
-		int b = lane * per;
-		int loc = 0;
-		for (int q = 0; q < per; ++q)
-			if (b + q < n) loc += run_off[b + q];
-		int inc = loc;
-		for (int o = 1; o < 64; o <<= 1) {
-			int y = __shfl_up(inc, o, 64);
-			if (lane >= o) inc += y;
-		}
-		int ex = inc - loc;
-		for (int q = 0; q < per; ++q)
-			if (b + q < n) {
-				int l = run_off[b + q];
-				run_off[b + q] = ex;
-				ex += l;
-			}
-		if (lane == 63) run_off[n] = inc;
-	}
-	__syncthreads();
-}
-
-// Copies runs [run_lo[r], run_lo[r] + len_r) to LDS offsets run_off[r]; total <= cap.
-template <int K>
-__device__ inline void stage_runs(const ParticleArrays32& P, const int* run_lo, const int* run_off, int nruns, int total,
-                                  float* sm, int cap) {
-	for (int base = 0; base < total; base += K * (int)blockDim.x) {
-		int gi[K];
-		float v[K][6];
-#pragma unroll
-		for (int k = 0; k < K; ++k) {
-			int e = base + k * (int)blockDim.x + (int)threadIdx.x;
-			gi[k] = -1;
-			if (e < total) {
-				int lo = 0, hi = nruns - 1;  // last run whose offset is <= e
-				while (lo < hi) {
-					int mid = (lo + hi + 1) >> 1;
-					if (run_off[mid] <= e)
-						lo = mid;
-					else
-						hi = mid - 1;
-				}
-				gi[k] = run_lo[lo] + (e - run_off[lo]);
-			}
-		}
-#pragma unroll
-		for (int k = 0; k < K; ++k)
-			if (gi[k] >= 0) {
-#pragma unroll
-				for (int a = 0; a < 3; ++a) {
-					v[k][a] = P.off[a][gi[k]];
-					v[k][3 + a] = P.u[a][gi[k]];
-				}
-			}
-#pragma unroll
-		for (int k = 0; k < K; ++k)
-			if (gi[k] >= 0) {
-				int e = base + k * (int)blockDim.x + (int)threadIdx.x;
-#pragma unroll
-				for (int q = 0; q < 6; ++q) sm[q * cap + e] = v[k][q];
-			}
-	}
-	__syncthreads();
-}
+constexpr int kMaxTile = 16;  // cells per deposit workgroup
 
 // ------------------------------------------------------------------ moments at x^n
-// Node moments (J, mu) of one (species, octant) segment of cell-sorted particles.
-template <class Src>
-__device__ __forceinline__ void node_segment(const Src& src, int lo, int hi, const Stencil8& SE, const Stencil8& SB, const int bit[3],
-                                             float beta, float omega, float pc, float acc[8][12]) {
-	for (int i = lo; i < hi; ++i) {
-		float f[3], u[3];
-		src.get(i, f, u);
-		Map32 mp[3];
-		float cw[3][2];
-#pragma unroll
-		for (int a3 = 0; a3 < 3; ++a3) {
-			mp[a3].nw[0] = 1.0f - f[a3];
-			mp[a3].nw[1] = f[a3];
-			cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
-			cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
-		}
-		float Ef[3], Bf[3], a[9];
-		interp8(SE, mp[0].nw, mp[1].nw, mp[2].nw, Ef);
-		interp8(SB, cw[0], cw[1], cw[2], Bf);
-		float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-		float ig = 1.0f / gm;
-		float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
-		alpha32(beta, v, gm, Ef, Bf, a);
-		float rv[3];
-#pragma unroll
-		for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
-#pragma unroll
-		for (int ci = 0; ci < 2; ++ci)
-#pragma unroll
-			for (int cj = 0; cj < 2; ++cj)
-#pragma unroll
-				for (int ck = 0; ck < 2; ++ck) {
-					int corner = ci * 4 + cj * 2 + ck;
-					float corr = mp[0].nw[ci] * mp[1].nw[cj] * mp[2].nw[ck];
-					float fj = pc * corr;
-#pragma unroll
-					for (int r = 0; r < 3; ++r) acc[corner][r] += rv[r] * fj;
-					float o = omega * corr;
-#pragma unroll
-					for (int r = 0; r < 9; ++r) acc[corner][3 + r] -= a[r] * o;
-				}
-	}
-}
-
-// One group of G lanes per cell, lane = (species, octant); a workgroup serves T consecutive
-// cells, whose particles are one index run per species.
-__global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
-                                                        const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
-                                                        float theta, int G, int T, int cap, float* __restrict__ out) {
-	const TileLds L = tile_lds(cap, kMaxSpecies);
-	float* sm = L.sm;
-	int* run_lo = L.run_lo;
-	int* run_off = L.run_off;
-	const long long c0 = (long long)blockIdx.x * T;
-	const int lt = threadIdx.x;
-	if (lt < sp.count) {
-		long long cend = c0 + T < g.ncells ? c0 + T : g.ncells;
-		int lo = start[((long long)lt * g.ncells + c0) << 3];
-		int hi = start[((long long)lt * g.ncells + cend) << 3];
-		run_lo[lt] = lo;
-		run_off[lt] = hi - lo;
-	}
-	__syncthreads();
-	scan_runs(run_off, sp.count);
-	const int total = run_off[sp.count];
-	const bool staged = total <= cap;
-	if (staged) stage_runs<4>(P, run_lo, run_off, sp.count, total, sm, cap);
-
-	long long c = c0 + lt / G;
-	int lane = lt % G;
-	int s = lane >> 3, oct = lane & 7;
-	bool active = lt < T * G && c < g.ncells && s < sp.count;
-	float acc[8][12];
-#pragma unroll
-	for (int a = 0; a < 8; ++a)
-#pragma unroll
-		for (int q = 0; q < 12; ++q) acc[a][q] = 0.0f;
-	if (active) {
-		long long key = (((long long)s * g.ncells + c) << 3) | oct;
-		int lo = start[key], hi = start[key + 1];
-		float m = (float)sp.mass[s], q = (float)sp.charge[s], w = (float)sp.weight[s];
-		float beta = 0.5f * q * dt / m;
-		float inv_v = (float)(1.0 / g.volume);
-		float omega = w * theta * dt * dt * 2.0f * kPiF * q * q / m * inv_v;
-		float pc = q * w * inv_v;
-		int cc[3];
-		cell_ijk(g, (int)c, cc);
-		int bit[3] = {(oct >> 2) & 1, (oct >> 1) & 1, oct & 1};
-		Stencil8 SE, SB;
-		if (hi > lo) {
-			const int nj = g.N[2] + 1, ni = (g.N[1] + 1) * nj;
-			load_stencil(E4, cc[0] * ni + cc[1] * nj + cc[2], ni, nj, SE);
-			const int cj = g.N[2], ci = g.N[1] * cj;
-			load_stencil(B4, (cc[0] - 1 + bit[0]) * ci + (cc[1] - 1 + bit[1]) * cj + (cc[2] - 1 + bit[2]), ci, cj, SB);
-		}
-		if (staged)
-			node_segment(LdsSrc{sm, cap, run_off[s] - run_lo[s]}, lo, hi, SE, SB, bit, beta, omega, pc, acc);
-		else
-			node_segment(global_src(P), lo, hi, SE, SB, bit, beta, omega, pc, acc);
-	}
-#pragma unroll
-	for (int a = 0; a < 8; ++a)
-#pragma unroll
-		for (int q = 0; q < 12; ++q) acc[a][q] = group_sum(acc[a][q], G);
-	if (lane == 0 && lt < T * G && c < g.ncells) {
-#pragma unroll
-		for (int a = 0; a < 8; ++a)
-#pragma unroll
-			for (int q = 0; q < 12; ++q) out[((long long)q * 8 + a) * g.ncells + c] = acc[a][q];
-	}
-}
-
-// Dual-cell tiles: a workgroup serves T base cells b = (i, j, k0 .. k0+T-1). Base b collects
-// the segments (species, cell b + a, octant 1 - a), a in {0,1}^3. For each species and
-// (ax, ay) the needed segments are, per source cell (i+ax, j+ay, k), the two octants with
-// x bit 1-ax and y bit 1-ay: one index run per (species, ax, ay, k), k = k0 .. k0+T.
+// Deposit tiles: a workgroup serves T cells b = (i, j, k0 .. k0+T-1) of one z row, with a
+// group of G lanes per cell. Lanes of a group take the cell's particles of each species
+// round-robin (lane l: l, l+G, l+2G, ...), so they stay evenly loaded whatever the octant
+// counts. Each lane sums its particles in that fixed order and the group reduces with a
+// fixed xor-shuffle tree: the sums are deterministic.
 struct DualTile {
 	int i, j, k0;
 };
@@ -425,87 +186,159 @@ __device__ inline DualTile dual_tile(const Grid& g, int T) {
 	return t;
 }
 
-__device__ inline int dual_runs(const Grid& g, const SpeciesTable& sp, const int* __restrict__ start, const DualTile& t, int T,
-                                int* run_lo, int* run_off) {
-	int nruns = sp.count * 4 * (T + 1);
-	for (int r = threadIdx.x; r < nruns; r += blockDim.x) {
-		int kk = r % (T + 1);
-		int pr = (r / (T + 1)) % 4;
-		int s = r / (4 * (T + 1));
-		int ax = pr >> 1, ay = pr & 1;
-		int ci = t.i + ax, cj = t.j + ay, ck = t.k0 + kk;
-		int lo = 0, hi = 0;
-		if (ci < g.N[0] && cj < g.N[1] && ck < g.N[2]) {
-			long long key = (((long long)s * g.ncells + cell_id(g, ci, cj, ck)) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1);
-			lo = start[key];
-			hi = start[key + 2];
-		}
-		run_lo[r] = lo;
-		run_off[r] = hi - lo;
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+
+// Node moments (J, mu) per cell. The E stencil (the cell's 8 nodes) is shared by all its
+// particles and lives in registers; the B stencil depends on the octant and is read from
+// the tile's 3 x 3 x (T+2) box of cells in LDS.
+__global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
+                                                        const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
+                                                        float theta, int G, int T, float* __restrict__ out) {
+	extern __shared__ float4 Bn[];  // 3 x 3 x (T+2) cells around the tile
+	const DualTile tile = dual_tile(g, T);
+	const int lt = threadIdx.x;
+	const int nk = T + 2;
+	const long long c_first = cell_id(g, tile.i, tile.j, tile.k0);
+	for (int q = lt; q < 9 * nk; q += blockDim.x) {
+		int dk = q % nk, dj = (q / nk) % 3, di = q / (3 * nk);
+		int ci = clampi(tile.i - 1 + di, 0, g.N[0] - 1), cj = clampi(tile.j - 1 + dj, 0, g.N[1] - 1);
+		int ck = clampi(tile.k0 - 1 + dk, 0, g.N[2] - 1);
+		Bn[q] = B4[cell_id(g, ci, cj, ck)];
 	}
 	__syncthreads();
-	scan_runs(run_off, nruns);
-	return nruns;
-}
 
-// Cell moments (rhohat corners and the reference's pressure-tensor partials) of one segment.
-template <class Src>
-__device__ __forceinline__ void cell_segment(const Src& src, int lo, int hi, const Stencil8& SE, const Stencil8& SB, const int bit[3],
-                                             float beta, float pc, float ar[8], float ap[8][6]) {
-	for (int i = lo; i < hi; ++i) {
-		float f[3], u[3];
-		src.get(i, f, u);
-		Map32 mp[3];
+	const int kb = lt / G, lane = lt % G;
+	const int bk = tile.k0 + kb;
+	const bool cell_ok = kb < T && bk < g.N[2];
+	float acc[8][12];
 #pragma unroll
-		for (int a3 = 0; a3 < 3; ++a3) {
-			mp[a3].nw[0] = 1.0f - f[a3];
-			mp[a3].nw[1] = f[a3];
-			mp[a3].cw[0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
-			mp[a3].cw[1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
+	for (int a = 0; a < 8; ++a)
+#pragma unroll
+		for (int q = 0; q < 12; ++q) acc[a][q] = 0.0f;
+	if (cell_ok) {
+		const long long c = c_first + kb;
+		const float inv_v = (float)(1.0 / g.volume);
+		Stencil8 SE;
+		{
+			const int nj = g.N[2] + 1, ni = (g.N[1] + 1) * nj;
+			load_stencil(E4, tile.i * ni + tile.j * nj + bk, ni, nj, SE);
 		}
-		float Ef[3], Bf[3], a[9];
-		interp8(SE, mp[0].nw, mp[1].nw, mp[2].nw, Ef);
-		interp8(SB, mp[0].cw, mp[1].cw, mp[2].cw, Bf);
-		float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-		float ig = 1.0f / gm;
-		float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
-		alpha32(beta, v, gm, Ef, Bf, a);
-		float rv[3];
+		for (int s = 0; s < sp.count; ++s) {
+			const int lo = start[((long long)s * g.ncells + c) << 3], hi = start[((long long)s * g.ncells + c + 1) << 3];
+			const float m = (float)sp.mass[s], q = (float)sp.charge[s], w = (float)sp.weight[s];
+			const float beta = 0.5f * q * dt / m;
+			const float omega = w * theta * dt * dt * 2.0f * kPiF * q * q / m * inv_v;
+			const float pc = q * w * inv_v;
+			for (int i = lo + lane; i < hi; i += G) {
+				float f[3], u[3];
+				load32(P, i, f, u);
+				int bit[3];
+				float nw[3][2], cw[3][2];
 #pragma unroll
-		for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
-		float Tt[6] = {rv[0] * rv[0], rv[0] * rv[1], rv[0] * rv[2], rv[1] * rv[1], rv[1] * rv[2], rv[2] * rv[2]};
-#pragma unroll
-		for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-			for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-				for (int tk = 0; tk < 2; ++tk) {
-					int tt = ti * 4 + tj * 2 + tk;
-					float fr = pc * mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk];
-					ar[tt] += fr;
-#pragma unroll
-					for (int r = 0; r < 6; ++r) ap[tt][r] += fr * Tt[r];
+				for (int a3 = 0; a3 < 3; ++a3) {
+					bit[a3] = f[a3] >= 0.5f;
+					nw[a3][0] = 1.0f - f[a3];
+					nw[a3][1] = f[a3];
+					cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
+					cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
 				}
+				float Ef[3], Bf[3] = {0.0f, 0.0f, 0.0f}, a[9];
+				interp8(SE, nw[0], nw[1], nw[2], Ef);
+				const int bb = (bit[0] * 3 + bit[1]) * nk + kb + bit[2];
+#pragma unroll
+				for (int t = 0; t < 8; ++t) {
+					const int tx = (t >> 2) & 1, ty = (t >> 1) & 1, tz = t & 1;
+					float4 bv = Bn[bb + (tx * 3 + ty) * nk + tz];
+					float wt = cw[0][tx] * cw[1][ty] * cw[2][tz];
+					Bf[0] += bv.x * wt;
+					Bf[1] += bv.y * wt;
+					Bf[2] += bv.z * wt;
+				}
+				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+				float ig = 1.0f / gm;
+				float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
+				alpha32(beta, v, gm, Ef, Bf, a);
+				float rv[3];
+#pragma unroll
+				for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
+#pragma unroll
+				for (int ci = 0; ci < 2; ++ci)
+#pragma unroll
+					for (int cj = 0; cj < 2; ++cj)
+#pragma unroll
+						for (int ck = 0; ck < 2; ++ck) {
+							int corner = ci * 4 + cj * 2 + ck;
+							float corr = nw[0][ci] * nw[1][cj] * nw[2][ck];
+							float fj = pc * corr;
+#pragma unroll
+							for (int r = 0; r < 3; ++r) acc[corner][r] += rv[r] * fj;
+							float o = omega * corr;
+#pragma unroll
+							for (int r = 0; r < 9; ++r) acc[corner][3 + r] -= a[r] * o;
+						}
+			}
+		}
+	}
+#pragma unroll
+	for (int a = 0; a < 8; ++a)
+#pragma unroll
+		for (int q = 0; q < 12; ++q) acc[a][q] = group_sum(acc[a][q], G);
+	if (lane == 0 && cell_ok) {
+		const long long c = c_first + kb;
+#pragma unroll
+		for (int a = 0; a < 8; ++a)
+#pragma unroll
+			for (int q = 0; q < 12; ++q) out[((long long)q * 8 + a) * g.ncells + c] = acc[a][q];
 	}
 }
 
-// Cell moments per dual cell b: lane = (species, source octant); source cell b + a, octant bits 1 - a.
-__global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
-                                                        const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
-                                                        int G, int T, int cap, float* __restrict__ rho_out,
-                                                        float* __restrict__ pi_out) {
-	const TileLds L = tile_lds(cap, sp.count * 4 * (T + 1));
-	float* sm = L.sm;
-	const DualTile tile = dual_tile(g, T);
-	const int nruns = dual_runs(g, sp, start, tile, T, L.run_lo, L.run_off);
-	const int total = L.run_off[nruns];
-	const bool staged = total <= cap;
-	if (staged) stage_runs<4>(P, L.run_lo, L.run_off, nruns, total, sm, cap);
+// Dual cells: base b collects the segments (species, cell b + a, octant 1 - a), a in {0,1}^3.
+// The 8 segments of one (dual cell, species): combined index v in [0, p[8]) lies in segment
+// a = max{q : p[q] <= v}; its particle index is v + off[a].
+struct DualSegments {
+	int p[9];
+	int off[8];
+};
 
+__device__ inline void dual_segments(const Grid& g, const int* __restrict__ start, const DualTile& tile, int kb, int s,
+                                     DualSegments& d) {
+	const int bk = tile.k0 + kb;
+	d.p[0] = 0;
+#pragma unroll
+	for (int a = 0; a < 8; ++a) {
+		const int ax = (a >> 2) & 1, ay = (a >> 1) & 1, az = a & 1;
+		const int ci = tile.i + ax, cj = tile.j + ay, ck = bk + az;
+		int lo = 0, len = 0;
+		if (ci < g.N[0] && cj < g.N[1] && ck < g.N[2]) {
+			long long key = (((long long)s * g.ncells + cell_id(g, ci, cj, ck)) << 3) | (7 - a);
+			lo = start[key];
+			len = start[key + 1] - lo;
+		}
+		d.p[a + 1] = d.p[a] + len;
+		d.off[a] = lo - d.p[a];
+	}
+}
+
+// Cell moments (rhohat corners and the reference's pressure-tensor partials) per dual cell.
+// The B stencil (cells b + t) is shared by the dual cell's particles and lives in registers;
+// the E stencil depends on the source cell and is read from the tile's 3 x 3 x (T+2) box of
+// nodes in LDS.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_cell_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
+                                                        const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
+                                                        int G, int T, float* __restrict__ rho_out, float* __restrict__ pi_out) {
+	extern __shared__ float4 En[];  // 3 x 3 x (T+2) nodes from the tile's base corner
+	const DualTile tile = dual_tile(g, T);
 	const int lt = threadIdx.x;
-	const int kb = lt / G;
-	int lane = lt % G;
-	int s = lane >> 3, seg = lane & 7;
+	const int nk = T + 2;
+	for (int q = lt; q < 9 * nk; q += blockDim.x) {
+		int dk = q % nk, dj = (q / nk) % 3, di = q / (3 * nk);
+		int ni = clampi(tile.i + di, 0, g.N[0]), nj = clampi(tile.j + dj, 0, g.N[1]), nkk = clampi(tile.k0 + dk, 0, g.N[2]);
+		En[q] = E4[node_id(g, ni, nj, nkk)];
+	}
+	__syncthreads();
+
+	const int kb = lt / G, lane = lt % G;
 	const int bk = tile.k0 + kb;
 	const bool base_ok = kb < T && bk < g.N[2];
 	float ar[8], ap[8][6];
@@ -515,29 +348,74 @@ __global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp,
 #pragma unroll
 		for (int r = 0; r < 6; ++r) ap[q][r] = 0.0f;
 	}
-	if (base_ok && s < sp.count) {
-		int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
-		int mc[3] = {tile.i + ax, tile.j + ay, bk + az};
-		if (mc[0] < g.N[0] && mc[1] < g.N[1] && mc[2] < g.N[2]) {
-			long long key = (((long long)s * g.ncells + cell_id(g, mc[0], mc[1], mc[2])) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) |
-			                (1 - az);
-			int lo = start[key], hi = start[key + 1];
-			float m = (float)sp.mass[s], q = (float)sp.charge[s], w = (float)sp.weight[s];
-			float beta = 0.5f * q * dt / m;
-			float pc = q * w * (float)(1.0 / g.volume);
-			int bit[3] = {1 - ax, 1 - ay, 1 - az};
-			Stencil8 SE, SB;
-			if (hi > lo) {
-				const int nj = g.N[2] + 1, ni = (g.N[1] + 1) * nj;
-				load_stencil(E4, mc[0] * ni + mc[1] * nj + mc[2], ni, nj, SE);
+	if (base_ok) {
+		const float inv_v = (float)(1.0 / g.volume);
+		Stencil8 SB;
+		bool have_sb = false;
+		for (int s = 0; s < sp.count; ++s) {
+			DualSegments d;
+			dual_segments(g, start, tile, kb, s, d);
+			if (d.p[8] == 0) continue;
+			if (!have_sb) {  // cells b .. b+1 exist whenever b has particles
 				const int cj = g.N[2], ci = g.N[1] * cj;
 				load_stencil(B4, tile.i * ci + tile.j * cj + bk, ci, cj, SB);
+				have_sb = true;
 			}
-			if (staged) {
-				int r = (s * 4 + ax * 2 + ay) * (T + 1) + kb + az;
-				cell_segment(LdsSrc{sm, cap, L.run_off[r] - L.run_lo[r]}, lo, hi, SE, SB, bit, beta, pc, ar, ap);
-			} else {
-				cell_segment(global_src(P), lo, hi, SE, SB, bit, beta, pc, ar, ap);
+			const float m = (float)sp.mass[s], q = (float)sp.charge[s], w = (float)sp.weight[s];
+			const float beta = 0.5f * q * dt / m;
+			const float pc = q * w * inv_v;
+			for (int v = lane; v < d.p[8]; v += G) {
+				int a = 0, off = d.off[0];
+#pragma unroll
+				for (int qq = 1; qq < 8; ++qq)
+					if (v >= d.p[qq]) {
+						a = qq;
+						off = d.off[qq];
+					}
+				float f[3], u[3];
+				load32(P, v + off, f, u);
+				const int ax = (a >> 2) & 1, ay = (a >> 1) & 1, az = a & 1;
+				const int bit[3] = {1 - ax, 1 - ay, 1 - az};
+				float nw[3][2], cw[3][2];
+#pragma unroll
+				for (int a3 = 0; a3 < 3; ++a3) {
+					nw[a3][0] = 1.0f - f[a3];
+					nw[a3][1] = f[a3];
+					cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
+					cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
+				}
+				float Ef[3] = {0.0f, 0.0f, 0.0f}, Bf[3], al[9];
+				const int eb = (ax * 3 + ay) * nk + kb + az;
+#pragma unroll
+				for (int t = 0; t < 8; ++t) {
+					const int tx = (t >> 2) & 1, ty = (t >> 1) & 1, tz = t & 1;
+					float4 ev = En[eb + (tx * 3 + ty) * nk + tz];
+					float wt = nw[0][tx] * nw[1][ty] * nw[2][tz];
+					Ef[0] += ev.x * wt;
+					Ef[1] += ev.y * wt;
+					Ef[2] += ev.z * wt;
+				}
+				interp8(SB, cw[0], cw[1], cw[2], Bf);
+				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+				float ig = 1.0f / gm;
+				float vv[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
+				alpha32(beta, vv, gm, Ef, Bf, al);
+				float rv[3];
+#pragma unroll
+				for (int r = 0; r < 3; ++r) rv[r] = al[r * 3 + 0] * u[0] + al[r * 3 + 1] * u[1] + al[r * 3 + 2] * u[2];
+				float Tt[6] = {rv[0] * rv[0], rv[0] * rv[1], rv[0] * rv[2], rv[1] * rv[1], rv[1] * rv[2], rv[2] * rv[2]};
+#pragma unroll
+				for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+					for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+						for (int tk = 0; tk < 2; ++tk) {
+							int tt = ti * 4 + tj * 2 + tk;
+							float fr = pc * cw[0][ti] * cw[1][tj] * cw[2][tk];
+							ar[tt] += fr;
+#pragma unroll
+							for (int r = 0; r < 6; ++r) ap[tt][r] += fr * Tt[r];
+						}
 			}
 		}
 	}
@@ -558,93 +436,78 @@ __global__ void __launch_bounds__(256) k_cell_moments32(Grid g, SpeciesTable sp,
 	}
 }
 
-// Post-push density (updateDensityParameters) of one segment.
-template <class Src>
-__device__ __forceinline__ void density_segment(const Src& src, int lo, int hi, const int bit[3], float m, float wv, float pc,
-                                                float conc[8], float en[8], float bulk[8][3], float rho[8]) {
-	for (int i = lo; i < hi; ++i) {
-		float f[3], u[3];
-		src.get(i, f, u);
-		float cw[3][2];
-#pragma unroll
-		for (int a3 = 0; a3 < 3; ++a3) {
-			cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
-			cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
-		}
-		float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-#pragma unroll
-		for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-			for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-				for (int tk = 0; tk < 2; ++tk) {
-					int tt = ti * 4 + tj * 2 + tk;
-					float corr = cw[0][ti] * cw[1][tj] * cw[2][tk];
-					float c = wv * corr;
-					conc[tt] += c;
-					en[tt] += c * gm * m;
-					rho[tt] += pc * corr;
-#pragma unroll
-					for (int r = 0; r < 3; ++r) bulk[tt][r] += c * u[r] * m;
-				}
-	}
-}
-
-// Post-push density per dual cell: lane = (species, source octant).
+// Post-push density (updateDensityParameters) per dual cell; lanes of a group share each
+// species' particles round-robin, so the per-species sums reduce over the whole group.
 // Output planes (float): per species [conc, energy, bulk x3] x 8 corners, then rho x 8.
 __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start, int G,
-                                                   int T, int cap, float* __restrict__ out) {
-	const TileLds L = tile_lds(cap, sp.count * 4 * (T + 1));
-	float* sm = L.sm;
+                                                   int T, float* __restrict__ out) {
 	const DualTile tile = dual_tile(g, T);
-	const int nruns = dual_runs(g, sp, start, tile, T, L.run_lo, L.run_off);
-	const int total = L.run_off[nruns];
-	const bool staged = total <= cap;
-	if (staged) stage_runs<4>(P, L.run_lo, L.run_off, nruns, total, sm, cap);
-
 	const int lt = threadIdx.x;
-	const int kb = lt / G;
-	int lane = lt % G;
-	int s = lane >> 3, seg = lane & 7;
+
+	const int kb = lt / G, lane = lt % G;
 	const int bk = tile.k0 + kb;
 	const bool base_ok = kb < T && bk < g.N[2];
-	float conc[8], en[8], bulk[8][3], rho[8];
+	const long long b = base_ok ? cell_id(g, tile.i, tile.j, bk) : 0;
+	const float wvol = (float)(1.0 / g.volume);
+	float rho[8];
 #pragma unroll
-	for (int q = 0; q < 8; ++q) {
-		conc[q] = en[q] = rho[q] = 0.0f;
-		bulk[q][0] = bulk[q][1] = bulk[q][2] = 0.0f;
-	}
-	if (base_ok && s < sp.count) {
-		int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
-		int mc[3] = {tile.i + ax, tile.j + ay, bk + az};
-		if (mc[0] < g.N[0] && mc[1] < g.N[1] && mc[2] < g.N[2]) {
-			long long key = (((long long)s * g.ncells + cell_id(g, mc[0], mc[1], mc[2])) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) |
-			                (1 - az);
-			int lo = start[key], hi = start[key + 1];
-			float m = (float)sp.mass[s], w = (float)sp.weight[s];
-			float wv = w * (float)(1.0 / g.volume);
-			float pc = (float)sp.charge[s] * wv;
-			int bit[3] = {1 - ax, 1 - ay, 1 - az};
-			if (staged) {
-				int r = (s * 4 + ax * 2 + ay) * (T + 1) + kb + az;
-				density_segment(LdsSrc{sm, cap, L.run_off[r] - L.run_lo[r]}, lo, hi, bit, m, wv, pc, conc, en, bulk, rho);
-			} else {
-				density_segment(global_src(P), lo, hi, bit, m, wv, pc, conc, en, bulk, rho);
+	for (int q = 0; q < 8; ++q) rho[q] = 0.0f;
+	for (int s = 0; s < sp.count; ++s) {
+		float conc[8], en[8], bulk[8][3];
+#pragma unroll
+		for (int q = 0; q < 8; ++q) {
+			conc[q] = en[q] = 0.0f;
+			bulk[q][0] = bulk[q][1] = bulk[q][2] = 0.0f;
+		}
+		if (base_ok) {
+			DualSegments d;
+			dual_segments(g, start, tile, kb, s, d);
+			const float m = (float)sp.mass[s];
+			const float wv = (float)sp.weight[s] * wvol;
+			const float pc = (float)sp.charge[s] * wv;
+			for (int v = lane; v < d.p[8]; v += G) {
+				int a = 0, off = d.off[0];
+#pragma unroll
+				for (int qq = 1; qq < 8; ++qq)
+					if (v >= d.p[qq]) {
+						a = qq;
+						off = d.off[qq];
+					}
+				float f[3], u[3];
+				load32(P, v + off, f, u);
+				const int bit[3] = {1 - ((a >> 2) & 1), 1 - ((a >> 1) & 1), 1 - (a & 1)};
+				float cw[3][2];
+#pragma unroll
+				for (int a3 = 0; a3 < 3; ++a3) {
+					cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
+					cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
+				}
+				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+#pragma unroll
+				for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+					for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+						for (int tk = 0; tk < 2; ++tk) {
+							int tt = ti * 4 + tj * 2 + tk;
+							float corr = cw[0][ti] * cw[1][tj] * cw[2][tk];
+							float c = wv * corr;
+							conc[tt] += c;
+							en[tt] += c * gm * m;
+							rho[tt] += pc * corr;
+#pragma unroll
+							for (int r = 0; r < 3; ++r) bulk[tt][r] += c * u[r] * m;
+						}
 			}
 		}
-	}
-	// per-species sums over the 8 octant lanes, charge density over all lanes
 #pragma unroll
-	for (int q = 0; q < 8; ++q) {
-		conc[q] = group_sum(conc[q], 8);
-		en[q] = group_sum(en[q], 8);
+		for (int q = 0; q < 8; ++q) {
+			conc[q] = group_sum(conc[q], G);
+			en[q] = group_sum(en[q], G);
 #pragma unroll
-		for (int r = 0; r < 3; ++r) bulk[q][r] = group_sum(bulk[q][r], 8);
-		rho[q] = group_sum(rho[q], G);
-	}
-	if (base_ok) {
-		long long b = cell_id(g, tile.i, tile.j, bk);
-		if (s < sp.count && seg == 0) {
+			for (int r = 0; r < 3; ++r) bulk[q][r] = group_sum(bulk[q][r], G);
+		}
+		if (lane == 0 && base_ok) {
 			long long base = (long long)s * 5 * 8;
 #pragma unroll
 			for (int q = 0; q < 8; ++q) {
@@ -654,11 +517,13 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 				for (int r = 0; r < 3; ++r) out[(base + (2 + r) * 8 + q) * g.ncells + b] = bulk[q][r];
 			}
 		}
-		if (lane == 0) {
-			long long base = (long long)sp.count * 5 * 8;
+	}
 #pragma unroll
-			for (int q = 0; q < 8; ++q) out[(base + q) * g.ncells + b] = rho[q];
-		}
+	for (int q = 0; q < 8; ++q) rho[q] = group_sum(rho[q], G);
+	if (lane == 0 && base_ok) {
+		long long base = (long long)sp.count * 5 * 8;
+#pragma unroll
+		for (int q = 0; q < 8; ++q) out[(base + q) * g.ncells + b] = rho[q];
 	}
 }
 
@@ -955,48 +820,27 @@ int group_lanes(int nspecies) {
 	return G;
 }
 
-// Deposit tile: T cells per workgroup (as many as fit 256 lanes, at most kMaxTile), halved
-// while the expected particles of a tile (mean per owned cell x T x 1.3) exceed kStageCap;
-// the LDS capacity is that expectation rounded up to 64. At least 64 lanes per workgroup.
-struct DepositTile {
-	int T, cap;
-	size_t lds_bytes(int max_runs) const { return (size_t)6 * cap * sizeof(float) + (size_t)(2 * max_runs + 1) * sizeof(int); }
-};
-
-DepositTile deposit_tile(const Grid& g, const ParticleArrays32& P, int G) {
-	const double margin = 1.3;
-	int T = 256 / G < kMaxTile ? 256 / G : kMaxTile;
-	double mean = (double)P.count / (double)(g.n[0] * (long long)g.n[1] * g.n[2]);
-	while (T * G > 64 && mean * T * margin > kStageCap) T /= 2;
-	double want = mean * T * margin;
-	int cap = want >= kStageCap ? kStageCap : (((int)want + 63) / 64) * 64;
-	if (cap < 64) cap = 64;
-	return DepositTile{T, cap};
-}
+// Deposit workgroups: T cells of one z row (as many as fit 256 lanes, at most kMaxTile).
+int deposit_tile(int G) { return 256 / G < kMaxTile ? 256 / G : kMaxTile; }
+unsigned deposit_blocks(const Grid& g, int T) { return (unsigned)(g.N[0] * g.N[1] * ((g.N[2] + T - 1) / T)); }
+size_t box_bytes(int T) { return (size_t)9 * (T + 2) * sizeof(float4); }
 
 void launch_node_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
                            const float* B, float dt, float theta, float* scratch, hipStream_t s) {
-	int G = group_lanes(sp.count);
-	DepositTile t = deposit_tile(g, P, G);
-	hipLaunchKernelGGL(k_node_moments32, dim3(blocks32(g.ncells, t.T)), dim3(t.T * G), t.lds_bytes(kMaxSpecies), s, g, sp, P, start,
-	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, theta, G, t.T, t.cap, scratch);
+	int G = group_lanes(sp.count), T = deposit_tile(G);
+	hipLaunchKernelGGL(k_node_moments32, dim3(deposit_blocks(g, T)), dim3(T * G), box_bytes(T), s, g, sp, P, start,
+	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, theta, G, T, scratch);
 }
 void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
                            const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s) {
-	int G = group_lanes(sp.count);
-	DepositTile t = deposit_tile(g, P, G);
-	unsigned blocks = (unsigned)(g.N[0] * g.N[1] * ((g.N[2] + t.T - 1) / t.T));
-	hipLaunchKernelGGL(k_cell_moments32, dim3(blocks), dim3(t.T * G), t.lds_bytes(sp.count * 4 * (t.T + 1)), s, g, sp, P, start,
-	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, G, t.T, t.cap, rho_scratch,
-	                   pi_scratch);
+	int G = group_lanes(sp.count), T = deposit_tile(G);
+	hipLaunchKernelGGL(k_cell_moments32, dim3(deposit_blocks(g, T)), dim3(T * G), box_bytes(T), s, g, sp, P, start,
+	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, G, T, rho_scratch, pi_scratch);
 }
 void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
                       hipStream_t s) {
-	int G = group_lanes(sp.count);
-	DepositTile t = deposit_tile(g, P, G);
-	unsigned blocks = (unsigned)(g.N[0] * g.N[1] * ((g.N[2] + t.T - 1) / t.T));
-	hipLaunchKernelGGL(k_density32, dim3(blocks), dim3(t.T * G), t.lds_bytes(sp.count * 4 * (t.T + 1)), s, g, sp, P, start, G, t.T,
-	                   t.cap, scratch);
+	int G = group_lanes(sp.count), T = deposit_tile(G);
+	hipLaunchKernelGGL(k_density32, dim3(deposit_blocks(g, T)), dim3(T * G), 0, s, g, sp, P, start, G, T, scratch);
 }
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* status, hipStream_t s) {

@@ -71,11 +71,10 @@ def test_fp32_first_step_errors_against_fp64_oracle():
         assert v <= TOL32, (k, v)
 
 
-def test_fp32_nonuniform_density_uses_both_deposit_paths():
-    """Particles piled up toward the low-x side (density ~ 1/sqrt): tiles in the dense
-    region hold more particles than the LDS staging capacity sized from the mean, so the
-    deposits take their direct-from-HBM path there and the staged path elsewhere. Both must
-    agree with the oracle to the FP32 bar."""
+def test_fp32_nonuniform_density():
+    """Particles piled up toward the low-x side (density ~ 1/sqrt, up to 4x the mean per
+    cell, as behind a shock): per-cell and per-octant counts vary widely across the lanes of
+    a deposit group. Every quantity must still agree with the oracle to the FP32 bar."""
     w = wl.mini(n=(16, 8, 8), ppc=12, seed=5)
     st = wl.make_initial_state(w, perturb=0.05)
     L = w.n[0] * w.dx
