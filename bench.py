@@ -2,12 +2,19 @@
 
 Metric: particle-steps/s including every deposit (one particle advanced one full
 implicit step: moments, field solve, push, re-sort and post-push density), plus the
-GMRES time per Maxwell solve. Workload: the one-rank production box, 128^3 cells,
-64 particles per cell per species (electron-proton, m_p/m_e = 100, kT/mc^2 = 0.01,
-omega_ce/omega_pe = 0.5, dt = 0.1/omega_pe), synthetic seeded initial state generated
-on the GPU.
+GMRES time per Maxwell solve to the 1e-8 true residual, learned preconditioner against
+plain GMRES. Workload: the one-rank production box, 128^3 cells, 64 particles per cell
+per species (electron-proton, m_p/m_e = 100, kT/mc^2 = 0.01, omega_ce/omega_pe = 0.5,
+dt = 0.1/omega_pe), synthetic seeded initial state generated on the GPU.
+
+Default run: the chosen form (FP32 mover/deposit, mixed-precision GMRES) is timed for
+`--steps` steps and gives `value`; the learned preconditioner is trained on a 32^3 box of
+the same physics and compared with plain GMRES on a held-out 128^3 system; the plain FP64
+step (FP64 GMRES to the same residual) is timed beside it; the CPU baseline is the FP64
+oracle on the reference's CPU case.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 128] [--ppc 64]
+                       [--precision fp32|fp64] [--solver mixed|fp64] [--learned off|compare|use]
 """
 from __future__ import annotations
 
@@ -34,10 +41,21 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=64)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--stop-rule", default="reference", choices=["reference", "true_residual"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
+    ap.add_argument("--stop-rule", default="true_residual", choices=["reference", "true_residual"])
     ap.add_argument("--ortho", default="mgs", choices=["mgs", "cgs2"])
-    ap.add_argument("--solver", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--solver", default="mixed", choices=["fp64", "mixed"],
+                    help="arithmetic of the Maxwell solve in the timed steps")
+    ap.add_argument("--learned", default="compare", choices=["off", "compare", "use"],
+                    help="learned preconditioner: off; trained and compared with plain GMRES on held-out systems "
+                         "(compare); or also used in the timed steps (use)")
+    ap.add_argument("--train-cells", type=int, default=32, help="cells per axis of the box the network is trained on")
+    ap.add_argument("--train-pairs", type=int, default=2000)
+    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--no-fp64-compare", action="store_true", help="skip timing the plain FP64 step beside the chosen form")
+    ap.add_argument("--drift-steps", type=int, default=0,
+                    help="also run the chosen form and the plain FP64 step this many steps from the same state and "
+                         "report their total-energy difference")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     return ap.parse_args()
@@ -107,24 +125,14 @@ def cpu_baseline(seconds: float):
             "sample": "oracle FP64 step, 32x32x8 cells x 40 ppc (%d particles), %d steps in %.1f s" % (npart, steps, el)}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
-    import torch
-
+def open_box(w, torch, dev, **kw):
+    """A context on the current torch stream holding w's synthetic initial state (E = 0, B = B0
+    uniform, particles generated on the device), after the pre-loop sequence."""
     from picpp_amd import workloads as wl
     from picpp_amd.simulation import Simulation
 
-    dev = torch.device("cuda:0")
-    n = (args.cells,) * 3
-    w = wl.config2(n=n, ppc=args.ppc, steps=args.steps)
     stream = torch.cuda.current_stream(dev)
-    sim = Simulation(w, precision=args.precision, stop_rule=args.stop_rule, ortho=args.ortho,
-                     capacity=w.particles_total, stream=stream.cuda_stream, solver=args.solver)
-    # fields: E = 0, B = B0 x (uniform); ghosted component planes
+    sim = Simulation(w, capacity=w.particles_total, stream=stream.cuda_stream, **kw)
     N = w.N
     E = np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3))
     B = np.zeros(N + (3,))
@@ -135,17 +143,74 @@ def main():
     torch.cuda.synchronize()
     cnt = (ctypes.c_longlong * len(counts))(*counts)
     _D = ctypes.POINTER(ctypes.c_double)
-    rc = sim.lib.pic_upload_particles_device(sim.h, cnt, ctypes.cast(x.data_ptr(), _D), ctypes.cast(p.data_ptr(), _D),
-                                             None)
+    rc = sim.lib.pic_upload_particles_device(sim.h, cnt, ctypes.cast(x.data_ptr(), _D), ctypes.cast(p.data_ptr(), _D), None)
     sim._chk(rc, "pic_upload_particles_device")
     del x, p
     torch.cuda.empty_cache()
-    npart = sim.num_particles()
-    print("bench: %s, %d particles, precision %s" % (w.name, npart, args.precision), file=sys.stderr, flush=True)
     sim.initialize()
+    return sim
+
+
+def train_network(args, torch, dev):
+    """Trains the learned preconditioner on pairs logged by the mixed-precision solves of a
+    train_cells^3 box with the same physics (16 ppc per species). The network is convolutional
+    and periodic, so its parameters carry over to the production grid."""
+    from picpp_amd import workloads as wl
+
+    w = wl.config2(n=(args.train_cells,) * 3, ppc=16, steps=0)
+    sim = open_box(w, torch, dev, precision="fp32", stop_rule="true_residual", solver="mixed")
+    sim.precond_create()
+    sim.precond_log(args.train_pairs)
+    t0 = time.perf_counter()
+    steps = 0
+    while sim.precond_log(-1) < args.train_pairs and steps < 4000:
+        sim.step()
+        steps += 1
+    t_log = time.perf_counter() - t0
+    npairs = sim.precond_log(-1)
+    _, identity = sim.precond_train(0, 8, 2e-3, 0.1)
+    t0 = time.perf_counter()
+    tl, hl = sim.precond_train(args.train_epochs, 8, 2e-3, 0.1)
+    t_train = time.perf_counter() - t0
+    params, mu_scale = sim.precond_params()
+    sim.close()
+    print("network: %d pairs from %d steps of a %d^3 box in %.1f s; %d epochs in %.1f s; held-out loss %.3e (identity %.3e)"
+          % (npairs, steps, args.train_cells, t_log, args.train_epochs, t_train, hl, identity), file=sys.stderr, flush=True)
+    return params, mu_scale, {"box": [args.train_cells] * 3, "pairs": npairs, "solves": steps, "log_seconds": t_log,
+                              "epochs": args.train_epochs, "train_seconds": t_train, "train_loss": tl,
+                              "holdout_loss": hl, "identity_holdout_loss": identity}
+
+
+def compare_solvers(sim, torch, learned, reps=3):
+    """Solves the current step's Maxwell system (held out from training) with plain mixed
+    GMRES and, if available, with the learned preconditioner; each to the 1e-8 true residual."""
+    sim.updateElectroMagneticParameters()
+    sim.evaluateMaxwellEquationMatrix()
+    rhs_ptr, n = sim.device_array("rhs")
+    out = {}
+    for name, use in (("plain", False), ("learned", True)):
+        if use and not learned:
+            continue
+        if learned:
+            sim.precond_use(use)
+        runs = []
+        for _ in range(reps):
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            runs.append(sim.gmres_solve(rhs_ptr, x.data_ptr()))
+        out[name] = {"ms": 1e3 * float(np.mean([r["seconds"] for r in runs])),
+                     "iterations": float(np.mean([r["iterations"] for r in runs])),
+                     "true_residual": float(max(r["true_residual"] for r in runs)),
+                     "fallbacks": int(sum(r["fallbacks"] for r in runs))}
+    if learned:
+        sim.precond_use(False)
+    return out
+
+
+def time_steps(sim, args):
     for i in range(args.warmup):
         sim.step()
         print("warmup step %d done" % i, file=sys.stderr, flush=True)
+    import torch
     sim._chk(sim.lib.pic_profile_enable(sim.h, 1), "pic_profile_enable")
     solves = []
     torch.cuda.synchronize()
@@ -153,10 +218,11 @@ def main():
     for i in range(args.steps):
         solves.append(sim.step())
     torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    for i, s in enumerate(solves):
-        print("step %d: gmres %d iterations, %.3f ms" % (i, s["iterations"], 1e3 * s["seconds"]), file=sys.stderr)
-    # per-kernel statistics over the timed region (HIP events on the bench stream)
+    return time.perf_counter() - t0, solves
+
+
+def read_profile(sim, steps):
+    _D = ctypes.POINTER(ctypes.c_double)
     names = ctypes.create_string_buffer(64 * 32)
     ms = np.zeros(64)
     cn = np.zeros(64, np.int64)
@@ -168,18 +234,76 @@ def main():
     for i in range(k):
         nm = names.raw[32 * i:32 * i + 32].split(b"\0")[0].decode()
         kernels[nm] = {"ms_total": ms[i], "launches": int(cn[i]), "bytes_total": by[i]}
+    for nm, kv in sorted(kernels.items(), key=lambda q: -q[1]["ms_total"]):
+        bw = kv["bytes_total"] / (kv["ms_total"] * 1e-3) / 1e9 if kv["ms_total"] > 0 else 0
+        print("kernel %-18s %8.3f ms/step  %5d launches  %7.1f GB/s algorithmic" %
+              (nm, kv["ms_total"] / steps, kv["launches"], bw), file=sys.stderr)
+    return kernels
+
+
+def energy_drift(args, w, torch, dev):
+    """Total energy of the chosen form and of the plain FP64 step over drift_steps steps from
+    the same initial state: the largest relative difference between the two series, and each
+    series' own largest relative change from its initial value."""
+    series = {}
+    for name, kw in (("chosen", dict(precision=args.precision, stop_rule=args.stop_rule, solver=args.solver)),
+                     ("fp64", dict(precision="fp64", stop_rule="true_residual", solver="fp64"))):
+        sim = open_box(w, torch, dev, **kw)
+        e = [sim.updateEnergy()[3]]
+        for i in range(args.drift_steps):
+            sim.step()
+            e.append(sim.updateEnergy()[3])
+            if (i + 1) % 50 == 0:
+                print("drift %s: step %d" % (name, i + 1), file=sys.stderr, flush=True)
+        sim.close()
+        del sim
+        torch.cuda.empty_cache()
+        series[name] = np.array(e)
+    a, b = series["chosen"], series["fp64"]
+    return {"steps": args.drift_steps, "max_rel_difference": float(np.max(np.abs(a - b) / np.abs(b))),
+            "chosen_rel_change": float(np.max(np.abs(a - a[0]) / abs(a[0]))),
+            "fp64_rel_change": float(np.max(np.abs(b - b[0]) / abs(b[0])))}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from picpp_amd import workloads as wl
+
+    dev = torch.device("cuda:0")
+    learned = args.learned != "off" and args.solver == "mixed"
+    training = None
+    if learned:
+        params, mu_scale, training = train_network(args, torch, dev)
+    n = (args.cells,) * 3
+    w = wl.config2(n=n, ppc=args.ppc, steps=args.steps)
+    sim = open_box(w, torch, dev, precision=args.precision, stop_rule=args.stop_rule, ortho=args.ortho, solver=args.solver)
+    npart = sim.num_particles()
+    print("bench: %s, %d particles, precision %s, solver %s" % (w.name, npart, args.precision, args.solver), file=sys.stderr,
+          flush=True)
+    if learned:
+        sim.precond_create()
+        sim.set_precond_params(params, mu_scale)
+        sim.precond_use(args.learned == "use")
+    el, solves = time_steps(sim, args)
+    for i, s in enumerate(solves):
+        print("step %d: gmres %d iterations, %.3f ms" % (i, s["iterations"], 1e3 * s["seconds"]), file=sys.stderr)
+    kernels = read_profile(sim, args.steps)
     dom = max(kernels, key=lambda q: kernels[q]["ms_total"])
     kd = kernels[dom]
     avg_ms = kd["ms_total"] / kd["launches"]
     per_launch = kd["bytes_total"] / kd["launches"]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
-    for nm, kv in sorted(kernels.items(), key=lambda q: -q[1]["ms_total"]):
-        bw = kv["bytes_total"] / (kv["ms_total"] * 1e-3) / 1e9 if kv["ms_total"] > 0 else 0
-        print("kernel %-18s %8.3f ms/step  %5d launches  %7.1f GB/s algorithmic" %
-              (nm, kv["ms_total"] / args.steps, kv["launches"], bw), file=sys.stderr)
-    gm_ms = 1e3 * float(np.mean([s["seconds"] for s in solves]))
-    gm_it = float(np.mean([s["iterations"] for s in solves]))
     energy = sim.updateEnergy()
+    held_out = compare_solvers(sim, torch, learned) if args.solver == "mixed" else None
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
     out = {
         "metric": "particle-steps/s incl. deposit; GMRES ms to 1e-8 residual (learned vs plain)",
         "value": npart * args.steps / el,
@@ -194,20 +318,34 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic",
         "config": {"workload": "implicit e-/p+ step, %dx%dx%d cells, %d ppc per species (%d particles), %s mover/deposit, "
-                               "%s GMRES, %s stop rule, %s orthogonalisation" % (
-                                   n[0], n[1], n[2], args.ppc, npart, args.precision, args.solver, args.stop_rule,
-                                   args.ortho),
-                   "cells": list(n), "particles": npart, "precision": args.precision, "solver": args.solver},
-        "gmres": {"ms_per_solve": gm_ms, "iterations_per_solve": gm_it, "stop_rule": args.stop_rule,
-                  "solver": args.solver, "learned_preconditioner": False,
-                  "true_residual": float(np.mean([s["true_residual"] for s in solves]))},
+                               "%s GMRES%s, %s stop rule" % (
+                                   n[0], n[1], n[2], args.ppc, npart, args.precision, args.solver,
+                                   " + learned preconditioner" if args.learned == "use" and learned else "", args.stop_rule),
+                   "cells": list(n), "particles": npart, "precision": args.precision, "solver": args.solver,
+                   "learned_preconditioner_in_timed_steps": bool(learned and args.learned == "use")},
+        "gmres": {"ms_per_solve": 1e3 * float(np.mean([s["seconds"] for s in solves])),
+                  "iterations_per_solve": float(np.mean([s["iterations"] for s in solves])),
+                  "true_residual": float(max(s["true_residual"] for s in solves)), "stop_rule": args.stop_rule,
+                  "held_out_system": held_out, "training": training},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": per_launch},
         "kernels_ms_per_step": {nm: kv["ms_total"] / args.steps for nm, kv in kernels.items()},
         "energy": {"particles": energy[0], "E": energy[1], "B": energy[2], "total": energy[3]},
     }
-    sim.close()
+    if not args.no_fp64_compare and args.precision == "fp32":
+        sim = open_box(w, torch, dev, precision="fp64", stop_rule="true_residual", solver="fp64")
+        el64, solves64 = time_steps(sim, args)
+        sim.close()
+        del sim
+        torch.cuda.empty_cache()
+        out["fp64_step"] = {"ms_per_step": 1e3 * el64 / args.steps, "value": npart * args.steps / el64,
+                            "gmres_ms_per_solve": 1e3 * float(np.mean([s["seconds"] for s in solves64])),
+                            "stop_rule": "true_residual", "speedup_of_chosen_form": el64 / el}
+        print("fp64 step: %.2f ms (chosen form %.2f ms, %.2fx)" % (1e3 * el64 / args.steps, 1e3 * el / args.steps, el64 / el),
+              file=sys.stderr)
+    if args.drift_steps > 0:
+        out["energy_drift"] = energy_drift(args, w, torch, dev)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -84,6 +84,8 @@ typedef struct {
 	double estimated_residual; /* Givens estimate relative to ||b|| */
 	double true_residual;      /* ||b - A x|| / ||b|| when computed, else -1 */
 	double seconds;            /* wall time of the solve on the stream */
+	int preconditioned;        /* Arnoldi steps that applied the learned preconditioner */
+	int fallbacks;             /* switches from the learned preconditioner to plain GMRES */
 } pic_solve_info;
 
 /* which arrays pic_download_array / pic_device_array address */
@@ -173,6 +175,42 @@ int pic_upload_particles_device(pic_ctx* ctx, const long long* counts, const dou
    bytes of those launches; it returns the number of rows. */
 int pic_profile_enable(pic_ctx* ctx, int on);
 int pic_profile_read(pic_ctx* ctx, int max_entries, char* names, double* total_ms, long long* counts, double* bytes);
+
+/* ---- Learned preconditioner ----
+   No counterpart in the reference, whose GMRES is unpreconditioned
+   (PIC++/specialmath.cpp:367-671). A 3-D convolution U-Net (3 levels, 16/32/64 channels,
+   3x3x3 kernels, periodic, bfloat16 weights and activations with float32 accumulation on
+   MFMA) maps a solver vector r and the dielectric tensor mu to an approximate correction
+   z ~ A^-1 r. With pic_precond_use on, the mixed-precision solve (PIC_SOLVER_MIXED) runs
+   flexible GMRES with z_j = M(v_j) and falls back to plain GMRES when the network does not
+   reduce the residual; the FP64 true residual <= tolerance judges it either way.
+   Vectors are device float32: r, z, e [3][nx ny nz], mu [9][nx ny nz] (owned nodes, the
+   layout of pic_gmres_solve). Every physical dimension must be a multiple of 4. */
+/* Creates the network for the context's grid (He-initialised; the untrained net is z = r).
+   Requires solver_precision = PIC_SOLVER_MIXED. */
+int pic_precond_create(pic_ctx* ctx);
+/* Number of float32 parameters: per convolution W[27][cout][cin] then b[cout], in the order
+   enc1 16->16, enc2 16->32, enc3 32->64, dec3 64->32, dec2 32->16, dec1 16->16; then the
+   1x1x1 head W[3][16], b[3]. */
+long long pic_precond_param_count(void);
+/* Host parameter transfer; mu_scale divides mu at the network input. */
+int pic_precond_set_params(pic_ctx* ctx, const float* params, double mu_scale);
+int pic_precond_get_params(pic_ctx* ctx, float* params, double* mu_scale);
+/* z = M(r, mu) on device vectors. */
+int pic_precond_apply(pic_ctx* ctx, const float* r, const float* mu, float* z);
+/* Loss ||M(r, mu) - e||^2 / ||e||^2 and its gradient with respect to the parameters
+   (host array of pic_precond_param_count floats). */
+int pic_precond_gradient(pic_ctx* ctx, const float* r, const float* mu, const float* e, double* loss, float* grad);
+/* Starts logging training pairs (r, e) with A e = r from the following mixed-precision
+   solves: every Arnoldi step gives (A v_j, v_j) and every solve gives (b, x). max_pairs = 0
+   stops logging and frees the log. Returns the number of pairs held with max_pairs < 0. */
+int pic_precond_log(pic_ctx* ctx, int max_pairs);
+/* Adam on the logged pairs: `epochs` passes over the first (1 - holdout) of them in a seeded
+   shuffled order, `batch` pairs per step. Reports the mean training loss of the last epoch
+   and the mean loss on the held-out pairs. */
+int pic_precond_train(pic_ctx* ctx, int epochs, int batch, double lr, double holdout, double* train_loss, double* holdout_loss);
+/* Turns the learned preconditioner on (1) or off (0) for the following solves. */
+int pic_precond_use(pic_ctx* ctx, int on);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ ARRAYS = {"E": 0, "Etheta": 1, "Enew": 2, "B": 3, "Bnew": 4, "J": 5, "mu": 6, "d
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 _V = ctypes.c_void_p
+_F = ctypes.POINTER(ctypes.c_float)
 
 
 class PicConfig(ctypes.Structure):
@@ -40,7 +41,8 @@ class PicConfig(ctypes.Structure):
 
 class PicSolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("restarts", ctypes.c_int), ("estimated_residual", ctypes.c_double),
-                ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double), ("preconditioned", ctypes.c_int),
+                ("fallbacks", ctypes.c_int)]
 
 
 SIGNATURES = {
@@ -68,6 +70,15 @@ SIGNATURES = {
     "pic_upload_particles_device": (ctypes.c_int, [_V, ctypes.POINTER(ctypes.c_longlong), _D, _D, _I]),
     "pic_profile_enable": (ctypes.c_int, [_V, ctypes.c_int]),
     "pic_profile_read": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_char_p, _D, ctypes.POINTER(ctypes.c_longlong), _D]),
+    "pic_precond_create": (ctypes.c_int, [_V]),
+    "pic_precond_param_count": (ctypes.c_longlong, []),
+    "pic_precond_set_params": (ctypes.c_int, [_V, _F, ctypes.c_double]),
+    "pic_precond_get_params": (ctypes.c_int, [_V, _F, _D]),
+    "pic_precond_apply": (ctypes.c_int, [_V, _V, _V, _V]),
+    "pic_precond_gradient": (ctypes.c_int, [_V, _V, _V, _V, _D, _F]),
+    "pic_precond_log": (ctypes.c_int, [_V, ctypes.c_int]),
+    "pic_precond_train": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D]),
+    "pic_precond_use": (ctypes.c_int, [_V, ctypes.c_int]),
 }
 
 _lib = None

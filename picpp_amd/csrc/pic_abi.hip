@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -14,6 +16,7 @@
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
 #include "pic_kernels_fp32.hpp"
+#include "pic_precond.hpp"
 #include "pic_solver.hpp"
 
 using namespace picgpu;
@@ -83,6 +86,19 @@ struct pic_ctx {
 	float** V32dev = nullptr;
 	float *mu32 = nullptr, *w32 = nullptr;
 	double* w64 = nullptr;
+	// learned preconditioner: FGMRES basis Z = M(V) inside the mixed solver
+	std::unique_ptr<UNet> net;
+	double mu_scale = 0.0;
+	bool use_net = false;
+	std::vector<float*> Z32;
+	float** Z32dev = nullptr;
+	// training pairs (residual r, correction e with A e = r) logged from solves
+	bool logging = false;
+	int log_capacity = 0, log_count = 0;
+	float *log_r = nullptr, *log_e = nullptr;  // [capacity][3 nown]
+	std::vector<float*> log_mu;                 // dielectric tensor of each logged solve [9 nown]
+	std::vector<int> log_mu_of;                 // pair -> its solve
+	float* tmp32 = nullptr;
 	// energies
 	double* epartial = nullptr;
 	double* eout = nullptr;
@@ -557,6 +573,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 			for (auto& v : c->V32) v = dalloc<float>(len);
 			c->V32dev = dalloc<float*>(c->V32.size());
 			HIP_CHECK(hipMemcpy(c->V32dev, c->V32.data(), c->V32.size() * sizeof(float*), hipMemcpyHostToDevice));
+			c->tmp32 = dalloc<float>(2 * len);
 		}
 		c->partial = dalloc<double>((size_t)kDotBlocks * 128);
 		c->dscal = dalloc<double>(256);
@@ -608,7 +625,12 @@ void pic_destroy(pic_ctx* c) {
 		f(p);
 	for (double* v : c->V) f(v);
 	for (float* v : c->V32) f(v);
-	for (void* p : {(void*)c->V32dev, (void*)c->mu32, (void*)c->w32, (void*)c->w64}) f(p);
+	for (void* p : {(void*)c->V32dev, (void*)c->mu32, (void*)c->w32, (void*)c->w64, (void*)c->tmp32, (void*)c->Z32dev,
+	                (void*)c->log_r, (void*)c->log_e})
+		f(p);
+	for (float* v : c->Z32) f(v);
+	for (float* v : c->log_mu) f(v);
+	c->net.reset();
 	for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
 	if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
 	delete c;
@@ -861,9 +883,24 @@ int pic_field_rhs(pic_ctx* c) {
 	return PIC_OK;
 }
 
+// Appends a training pair (r, e) with A e = r to the log (float32 device vectors).
+static void log_pair(pic_ctx* c, const float* r, const float* e) {
+	if (!c->logging || c->log_count >= c->log_capacity || c->log_mu.empty()) return;
+	long long len = c->nown3();
+	HIP_CHECK(hipMemcpyAsync(c->log_r + (long long)c->log_count * len, r, len * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+	HIP_CHECK(hipMemcpyAsync(c->log_e + (long long)c->log_count * len, e, len * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+	c->log_mu_of.push_back((int)c->log_mu.size() - 1);
+	++c->log_count;
+}
+
 // GMRES with FP64 iterative refinement: each outer step solves A d = r approximately with an
 // FP32 basis and FP32 operator (dots accumulated and the Hessenberg least squares kept in
 // FP64), then x += d and r = b - A x in FP64, until ||r|| / ||b|| <= tol.
+// With the learned preconditioner M the inner solve is flexible GMRES (right preconditioned,
+// z_j = M v_j, x = Z y). The rest of the solve switches to plain GMRES (a fallback) when a
+// preconditioned cycle converges slowly (Givens estimate above 1e-2 after 8 steps: worse than
+// about 0.56 per step; the cycle stops there and its partial update is kept), leaves the true
+// residual above half its previous value, or produces a non-finite least-squares solution.
 static void mixed_solve(pic_ctx* c, const double* b, double* x, pic_solve_info& inf) {
 	hipStream_t s = c->stream;
 	const Grid& g = c->g;
@@ -873,6 +910,19 @@ static void mixed_solve(pic_ctx* c, const double* b, double* x, pic_solve_info& 
 	double* d_rn = c->dscal + 5;
 	double* d_h = c->dscal + 8;
 	launch_to_f32(c->muc, c->mu32, 9 * g.nown, s);
+	if (c->logging && c->log_count < c->log_capacity) {
+		if (c->mu_scale <= 0) {  // the network's input scale of mu: its rms over the first logged solve
+			launch_dotT<float>(c->mu32, c->mu32, 9 * g.nown, c->partial, d_h, false, s);
+			double ss = 0;
+			HIP_CHECK(hipMemcpyAsync(&ss, d_h, sizeof(double), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+			c->mu_scale = ss > 0 ? std::sqrt(ss / (9.0 * g.nown)) : 1.0;
+		}
+		float* m = dalloc<float>(9 * g.nown);
+		HIP_CHECK(hipMemcpyAsync(m, c->mu32, 9 * g.nown * sizeof(float), hipMemcpyDeviceToDevice, s));
+		c->log_mu.push_back(m);
+	}
+	bool pre = c->use_net && c->net != nullptr;
 	HIP_CHECK(hipMemsetAsync(x, 0, len * sizeof(double), s));
 	HIP_CHECK(hipMemcpyAsync(c->work, b, len * sizeof(double), hipMemcpyDeviceToDevice, s));  // r = b
 	launch_dotT<double>(b, b, len, c->partial, d_bn, true, s);
@@ -890,13 +940,22 @@ static void mixed_solve(pic_ctx* c, const double* b, double* x, pic_solve_info& 
 		int nn = 2;
 		rel = 1.0;
 		std::vector<double> hcol;
+		const bool cycle_pre = pre;
+		bool slow = false;
 		while (rel > target && nn < limit) {
 			int j = nn - 2;
 			float* w = c->V32[j + 1];
+			const float* dir = c->V32[j];
+			if (cycle_pre) {
+				Timed t(c, "precond_net", 0);
+				c->net->forward(c->V32[j], c->mu32, (float)c->mu_scale, c->Z32[j], s);
+				dir = c->Z32[j];
+			}
 			{
 				Timed t(c, "apply_operator32", (double)g.nown * 15 * 4);
-				launch_apply_factored<float>(g, c->mu32, c->V32[j], c->w32, C, w, s);
+				launch_apply_factored<float>(g, c->mu32, dir, c->w32, C, w, s);
 			}
+			log_pair(c, w, dir);
 			{
 				Timed t(c, "orthogonalize32", (double)len * 4 * (5.0 * (j + 1) + 3));
 				for (int m = 0; m <= j; ++m) {
@@ -912,23 +971,46 @@ static void mixed_solve(pic_ctx* c, const double* b, double* x, pic_solve_info& 
 			gv.update(nn, hcol);
 			rel = std::fabs(gv.Q[nn - 1][0]);
 			nn++;
+			if (cycle_pre) inf.preconditioned += 1;
+			if (!std::isfinite(rel)) break;
+			if (cycle_pre && nn - 2 >= 8 && rel > 1e-2) {
+				slow = true;
+				break;
+			}
 		}
 		inf.iterations += nn - 2;
 		inf.restarts += 1;
+		bool finite = std::isfinite(rel);
+		for (double yv : gv.y) finite = finite && std::isfinite(yv);
+		if (!finite) {  // only a preconditioned cycle can get here: drop it and go on without the network
+			pre = false;
+			inf.fallbacks += 1;
+			continue;
+		}
 		HIP_CHECK(hipMemcpyAsync(c->ydev, gv.y.data(), gv.y.size() * sizeof(double), hipMemcpyHostToDevice, s));
+		double rprev = rnorm;
 		{
 			Timed t(c, "refinement", (double)len * 8 * (gv.y.size() / 2.0 + 2 + 15.0 / 3 + 3));
-			launch_update_x<float>(x, c->V32dev, c->ydev, (int)gv.y.size(), d_rn, len, s);  // x += ||r|| sum y_m v_m
-			launch_apply_factored<double>(g, c->muc, x, c->w64, C, c->work2, s);          // A x in FP64
-			launch_residual(b, c->work2, c->work, len, s);                                 // r = b - A x
+			launch_update_x<float>(x, cycle_pre ? c->Z32dev : c->V32dev, c->ydev, (int)gv.y.size(), d_rn, len, s);  // x += ||r|| Z y
+			launch_apply_factored<double>(g, c->muc, x, c->w64, C, c->work2, s);  // A x in FP64
+			launch_residual(b, c->work2, c->work, len, s);                         // r = b - A x
 			launch_dotT<double>(c->work, c->work, len, c->partial, d_rn, true, s);
 		}
 		HIP_CHECK(hipMemcpyAsync(&rnorm, d_rn, sizeof(double), hipMemcpyDeviceToHost, s));
 		HIP_CHECK(hipStreamSynchronize(s));
+		if (cycle_pre && (slow || !(rnorm <= 0.5 * rprev))) {
+			pre = false;
+			inf.fallbacks += 1;
+		}
 		if (nn == 2) break;
 	}
 	inf.true_residual = bnorm > 0 ? rnorm / bnorm : 0.0;
 	inf.estimated_residual = rel;
+	if (c->logging && c->log_count < c->log_capacity && bnorm > 0) {  // the solve itself: A x = b
+		launch_to_f32(b, c->tmp32, len, s);
+		launch_to_f32(x, c->tmp32 + len, len, s);
+		log_pair(c, c->tmp32, c->tmp32 + len);
+	}
 	if (inf.true_residual > c->cfg.gmres_tolerance)
 		throw DeviceError{PIC_ERR_SOLVER, "mixed-precision GMRES did not reach the requested true residual"};
 }
@@ -938,7 +1020,7 @@ static int solve_impl(pic_ctx* c, double* b, double* x, pic_solve_info* info) {
 	HIP_CHECK(hipStreamSynchronize(s));  // the solve time below excludes work queued before it
 	auto t0 = std::chrono::steady_clock::now();
 	if (c->cfg.solver_precision == PIC_SOLVER_MIXED) {
-		pic_solve_info inf{0, 0, 0, -1, 0};
+		pic_solve_info inf{0, 0, 0, -1, 0, 0, 0};
 		mixed_solve(c, b, x, inf);
 		HIP_CHECK(hipStreamSynchronize(s));
 		inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -947,7 +1029,7 @@ static int solve_impl(pic_ctx* c, double* b, double* x, pic_solve_info* info) {
 	}
 	int dim = 3 * c->g.n[0] * c->g.n[1] * c->g.n[2];
 	double rel = 0;
-	pic_solve_info inf{0, 0, 0, -1, 0};
+	pic_solve_info inf{0, 0, 0, -1, 0, 0, 0};
 	long long len = c->nown3();
 	if (c->cfg.stop_rule == PIC_STOP_REFERENCE) {
 		// generalizedMinimalResidualMethod: b is normalised in place there; a copy keeps rhs intact here
@@ -1160,4 +1242,184 @@ int pic_synchronize(pic_ctx* c) {
 	return PIC_OK;
 }
 
+// ------------------------------------------------------------------ learned preconditioner
+static void need_net(pic_ctx* c) {
+	if (!c->net) throw DeviceError{PIC_ERR_ARGUMENT, "no preconditioner network: call pic_precond_create first"};
+}
+
+int pic_precond_create(pic_ctx* c) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	try {
+		if (c->cfg.solver_precision != PIC_SOLVER_MIXED)
+			throw DeviceError{PIC_ERR_ARGUMENT, "the learned preconditioner runs inside the mixed-precision solver"};
+		try {
+			c->net.reset(new UNet(c->g.n));
+		} catch (const std::exception& e) {
+			throw DeviceError{PIC_ERR_ARGUMENT, e.what()};
+		}
+		if (c->Z32.empty()) {
+			c->Z32.resize(c->max_basis + 1);
+			for (auto& v : c->Z32) v = dalloc<float>(c->nown3());
+			c->Z32dev = dalloc<float*>(c->Z32.size());
+			HIP_CHECK(hipMemcpy(c->Z32dev, c->Z32.data(), c->Z32.size() * sizeof(float*), hipMemcpyHostToDevice));
+		}
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+long long pic_precond_param_count(void) { return UNet::param_count(); }
+
+int pic_precond_set_params(pic_ctx* c, const float* params, double mu_scale) {
+	if (!c || !params || !(mu_scale > 0)) return PIC_ERR_ARGUMENT;
+	try {
+		need_net(c);
+		c->net->set_params(params, c->stream);
+		c->mu_scale = mu_scale;
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	} catch (const std::exception& e) {
+		return fail(c, DeviceError{PIC_ERR_DEVICE, e.what()});
+	}
+	return PIC_OK;
+}
+
+int pic_precond_get_params(pic_ctx* c, float* params, double* mu_scale) {
+	if (!c || !params) return PIC_ERR_ARGUMENT;
+	try {
+		need_net(c);
+		c->net->get_params(params, c->stream);
+		if (mu_scale) *mu_scale = c->mu_scale;
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	} catch (const std::exception& e) {
+		return fail(c, DeviceError{PIC_ERR_DEVICE, e.what()});
+	}
+	return PIC_OK;
+}
+
+int pic_precond_apply(pic_ctx* c, const float* r, const float* mu, float* z) {
+	if (!c || !r || !mu || !z) return PIC_ERR_ARGUMENT;
+	try {
+		need_net(c);
+		if (!(c->mu_scale > 0)) throw DeviceError{PIC_ERR_ARGUMENT, "mu_scale is not set (set_params or a logged solve sets it)"};
+		c->net->forward(r, mu, (float)c->mu_scale, z, c->stream);
+		HIP_CHECK(hipGetLastError());
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_precond_gradient(pic_ctx* c, const float* r, const float* mu, const float* e, double* loss, float* grad) {
+	if (!c || !r || !mu || !e) return PIC_ERR_ARGUMENT;
+	try {
+		need_net(c);
+		if (!(c->mu_scale > 0)) throw DeviceError{PIC_ERR_ARGUMENT, "mu_scale is not set"};
+		c->net->zero_grad(c->stream);
+		c->net->accumulate_gradient(r, mu, (float)c->mu_scale, e, c->stream);
+		HIP_CHECK(hipGetLastError());
+		double l = 0;
+		HIP_CHECK(hipMemcpyAsync(&l, c->net->loss_ptr(), sizeof(double), hipMemcpyDeviceToHost, c->stream));
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+		if (loss) *loss = l;
+		if (grad) c->net->get_grads(grad, c->stream);
+	} catch (const DeviceError& ex) {
+		return fail(c, ex);
+	} catch (const std::exception& ex) {
+		return fail(c, DeviceError{PIC_ERR_DEVICE, ex.what()});
+	}
+	return PIC_OK;
+}
+
+int pic_precond_log(pic_ctx* c, int max_pairs) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	if (max_pairs < 0) return c->log_count;
+	try {
+		if (c->cfg.solver_precision != PIC_SOLVER_MIXED)
+			throw DeviceError{PIC_ERR_ARGUMENT, "training pairs are logged by the mixed-precision solver"};
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+		for (void* p : {(void*)c->log_r, (void*)c->log_e}) if (p) HIP_CHECK(hipFree(p));
+		for (float* v : c->log_mu) HIP_CHECK(hipFree(v));
+		c->log_r = c->log_e = nullptr;
+		c->log_mu.clear();
+		c->log_mu_of.clear();
+		c->log_count = 0;
+		c->log_capacity = max_pairs;
+		c->logging = max_pairs > 0;
+		if (c->logging) {
+			c->log_r = dalloc<float>((long long)max_pairs * c->nown3());
+			c->log_e = dalloc<float>((long long)max_pairs * c->nown3());
+		}
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_precond_train(pic_ctx* c, int epochs, int batch, double lr, double holdout, double* train_loss, double* holdout_loss) {
+	if (!c || epochs < 0 || batch < 1 || !(lr > 0) || holdout < 0 || holdout >= 1) return PIC_ERR_ARGUMENT;
+	try {
+		need_net(c);
+		const int n = c->log_count;
+		const int ntrain = n - (int)std::floor(holdout * n);
+		if (ntrain < 1) throw DeviceError{PIC_ERR_ARGUMENT, "no logged training pairs"};
+		const long long len = c->nown3();
+		hipStream_t s = c->stream;
+		const float ms = (float)c->mu_scale;
+		auto sample = [&](int i) {
+			c->net->accumulate_gradient(c->log_r + (long long)i * len, c->log_mu[c->log_mu_of[i]], ms, c->log_e + (long long)i * len, s);
+		};
+		auto read_loss = [&]() {
+			double l = 0;
+			HIP_CHECK(hipMemcpyAsync(&l, c->net->loss_ptr(), sizeof(double), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+			return l;
+		};
+		std::vector<int> order(ntrain);
+		for (int i = 0; i < ntrain; ++i) order[i] = i;
+		std::mt19937_64 rng(2024);
+		double last = 0;
+		for (int ep = 0; ep < epochs; ++ep) {
+			std::shuffle(order.begin(), order.end(), rng);
+			double sum = 0;
+			for (int b0 = 0; b0 < ntrain; b0 += batch) {
+				int nb = std::min(batch, ntrain - b0);
+				c->net->zero_grad(s);
+				for (int q = 0; q < nb; ++q) sample(order[b0 + q]);
+				c->net->adam_step((float)lr, nb, s);
+				sum += read_loss();
+			}
+			last = sum / ntrain;
+		}
+		double hl = 0;
+		if (ntrain < n) {
+			c->net->zero_grad(s);
+			for (int i = ntrain; i < n; ++i) sample(i);
+			hl = read_loss() / (n - ntrain);
+			c->net->zero_grad(s);
+		}
+		HIP_CHECK(hipGetLastError());
+		if (train_loss) *train_loss = last;
+		if (holdout_loss) *holdout_loss = hl;
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	} catch (const std::exception& e) {
+		return fail(c, DeviceError{PIC_ERR_DEVICE, e.what()});
+	}
+	return PIC_OK;
+}
+
+int pic_precond_use(pic_ctx* c, int on) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	if (on && (!c->net || !(c->mu_scale > 0))) {
+		c->error = "the learned preconditioner needs a network and its mu_scale";
+		return PIC_ERR_ARGUMENT;
+	}
+	c->use_net = on != 0;
+	return PIC_OK;
+}
+
 }  // extern "C"
+

@@ -1,0 +1,671 @@
+// Learned preconditioner: a 3-D convolution U-Net (3 levels, 16/32/64 channels, 3x3x3
+// kernels, periodic) in bfloat16 with float32 accumulation on MFMA.
+//
+// Layout: activations are channels-last bf16 [voxel][channel], voxel = (x*d1 + y)*d2 + z,
+// the same order as the solver's owned-node vectors [3][V]. A 3x3x3 convolution is an
+// implicit GEMM: rows are voxels, columns output channels, and the reduction runs over
+// (tap, input channel) in chunks of 32 fed to v_mfma_f32_16x16x32_bf16. A lane's 8 A
+// values are 8 consecutive channels of one neighbour voxel (one 16-byte load), its 8 B
+// values 8 consecutive input channels of one output channel (weights stored W[t][co][ci]).
+// The input gradient of a convolution is the same kernel on the adjoint weights
+// W'[t][ci][co] = W[26 - t][co][ci] (tap 26 - t is offset -o_t). Weight gradients are
+// float32 outer-product sums over voxel chunks, reduced in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+
+#include "pic_precond.hpp"
+#include "pic_solver.hpp"
+
+namespace picgpu {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kHeadIn = 16, kHeadOut = 3;
+constexpr int kRedBlocks = 512;
+
+#define PC_CHECK(expr)                                                                                    \
+	do {                                                                                                  \
+		hipError_t e_ = (expr);                                                                           \
+		if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+	} while (0)
+
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// round to nearest even
+__device__ inline uint16_t f2bf(float f) {
+	uint32_t u = __float_as_uint(f);
+	u += 0x7FFFu + ((u >> 16) & 1u);
+	return (uint16_t)(u >> 16);
+}
+
+__device__ inline int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+__device__ inline bf16x8 load8(const uint16_t* p) {
+	uint4 q = *reinterpret_cast<const uint4*>(p);
+	return __builtin_bit_cast(bf16x8, q);
+}
+
+// ------------------------------------------------------------------ convolution (MFMA)
+// Y[v][co] = sum_t sum_ci X[v + o_t][ci] W[t][co][ci] (+ b[co], relu); 32 voxels per wave.
+template <int CIN, int COUT, class OutT>
+__global__ void __launch_bounds__(256) k_conv3(const uint16_t* __restrict__ X, const uint16_t* __restrict__ W,
+                                               const float* __restrict__ bias, int relu, OutT* __restrict__ Y, int d0, int d1,
+                                               int d2) {
+	constexpr int NT = COUT / 16;
+	constexpr int NCH = (27 * CIN + 31) / 32;
+	const int V = d0 * d1 * d2;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int vbase = (blockIdx.x * 4 + wave) * 32;
+	const int r16 = lane & 15, kg = lane >> 4;
+	int vx[2], vy[2], vz[2];
+#pragma unroll
+	for (int h = 0; h < 2; ++h) {
+		int v = vbase + 16 * h + r16;
+		v = v < V ? v : 0;
+		vz[h] = v % d2;
+		vy[h] = (v / d2) % d1;
+		vx[h] = v / (d1 * d2);
+	}
+	f32x4 acc[2][NT];
+#pragma unroll
+	for (int h = 0; h < 2; ++h)
+#pragma unroll
+		for (int n = 0; n < NT; ++n) acc[h][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+	for (int c = 0; c < NCH; ++c) {
+		const int kf = c * 32 + kg * 8;
+		const int t = kf / CIN, ci = kf % CIN;
+		bf16x8 a[2], b[NT];
+		if (t < 27) {
+			const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+#pragma unroll
+			for (int h = 0; h < 2; ++h) {
+				int idx = (wrap(vx[h] + dx, d0) * d1 + wrap(vy[h] + dy, d1)) * d2 + wrap(vz[h] + dz, d2);
+				a[h] = load8(X + (long long)idx * CIN + ci);
+			}
+#pragma unroll
+			for (int n = 0; n < NT; ++n) b[n] = load8(W + ((long long)t * COUT + n * 16 + r16) * CIN + ci);
+		} else {
+#pragma unroll
+			for (int h = 0; h < 2; ++h) a[h] = bf16x8{};
+#pragma unroll
+			for (int n = 0; n < NT; ++n) b[n] = bf16x8{};
+		}
+#pragma unroll
+		for (int h = 0; h < 2; ++h)
+#pragma unroll
+			for (int n = 0; n < NT; ++n) acc[h][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h], b[n], acc[h][n], 0, 0, 0);
+	}
+	// D layout: lane (kg, r16) holds rows 4 kg + i, column r16
+#pragma unroll
+	for (int h = 0; h < 2; ++h)
+#pragma unroll
+		for (int n = 0; n < NT; ++n)
+#pragma unroll
+			for (int i = 0; i < 4; ++i) {
+				const int v = vbase + 16 * h + 4 * kg + i;
+				const int co = n * 16 + r16;
+				if (v >= V) continue;
+				float val = acc[h][n][i] + (bias ? bias[co] : 0.0f);
+				if (relu) val = val > 0.0f ? val : 0.0f;
+				if constexpr (sizeof(OutT) == 2)
+					Y[(long long)v * COUT + co] = f2bf(val);
+				else
+					Y[(long long)v * COUT + co] = val;
+			}
+}
+
+template <int CIN, int COUT, class OutT>
+void conv3(const uint16_t* X, const uint16_t* W, const float* b, bool relu, OutT* Y, const int d[3], hipStream_t s) {
+	long long V = (long long)d[0] * d[1] * d[2];
+	unsigned blocks = (unsigned)((V + 127) / 128);
+	hipLaunchKernelGGL((k_conv3<CIN, COUT, OutT>), dim3(blocks), dim3(256), 0, s, X, W, b, relu ? 1 : 0, Y, d[0], d[1], d[2]);
+}
+
+// dispatch on the channel pair
+template <class OutT>
+void conv3_any(int cin, int cout, const uint16_t* X, const uint16_t* W, const float* b, bool relu, OutT* Y, const int d[3],
+               hipStream_t s) {
+	if (cin == 16 && cout == 16) return conv3<16, 16, OutT>(X, W, b, relu, Y, d, s);
+	if (cin == 16 && cout == 32) return conv3<16, 32, OutT>(X, W, b, relu, Y, d, s);
+	if (cin == 32 && cout == 64) return conv3<32, 64, OutT>(X, W, b, relu, Y, d, s);
+	if (cin == 64 && cout == 32) return conv3<64, 32, OutT>(X, W, b, relu, Y, d, s);
+	if (cin == 32 && cout == 16) return conv3<32, 16, OutT>(X, W, b, relu, Y, d, s);
+	throw std::runtime_error("conv3: unsupported channel pair");
+}
+
+// ------------------------------------------------------------------ weight gradient
+// partial[chunk][t][co][ci] = sum over the chunk's voxels of dZ[v][co] X[v + o_t][ci]; the
+// blocks with t = 27 write the bias sums partial[chunk][27 COUT CIN + co].
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_wgrad(const uint16_t* __restrict__ dZ, const uint16_t* __restrict__ X, int d0, int d1,
+                                               int d2, int chunk, float* __restrict__ partial) {
+	constexpr int NP = COUT * CIN;
+	constexpr int PER = (NP + 255) / 256;
+	constexpr int STRIDE = 27 * NP + COUT;
+	__shared__ float sz[64][COUT];
+	__shared__ float sx[64][CIN];
+	const int V = d0 * d1 * d2;
+	const int t = blockIdx.x;
+	const int v0 = blockIdx.y * chunk;
+	const int v1 = v0 + chunk < V ? v0 + chunk : V;
+	const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+	float acc[PER];
+#pragma unroll
+	for (int q = 0; q < PER; ++q) acc[q] = 0.0f;
+	for (int vb = v0; vb < v1; vb += 64) {
+		for (int e = threadIdx.x; e < 64 * COUT; e += 256) {
+			int v = vb + e / COUT;
+			sz[e / COUT][e % COUT] = v < v1 ? bf2f(dZ[(long long)v * COUT + e % COUT]) : 0.0f;
+		}
+		if (t < 27)
+			for (int e = threadIdx.x; e < 64 * CIN; e += 256) {
+				int v = vb + e / CIN;
+				float val = 0.0f;
+				if (v < v1) {
+					int z = v % d2, y = (v / d2) % d1, x = v / (d1 * d2);
+					int idx = (wrap(x + dx, d0) * d1 + wrap(y + dy, d1)) * d2 + wrap(z + dz, d2);
+					val = bf2f(X[(long long)idx * CIN + e % CIN]);
+				}
+				sx[e / CIN][e % CIN] = val;
+			}
+		__syncthreads();
+		if (t < 27) {
+			for (int j = 0; j < 64; ++j)
+#pragma unroll
+				for (int q = 0; q < PER; ++q) {
+					int p = threadIdx.x + 256 * q;
+					if (p < NP) acc[q] += sz[j][p / CIN] * sx[j][p % CIN];
+				}
+		} else if (threadIdx.x < COUT) {
+			for (int j = 0; j < 64; ++j) acc[0] += sz[j][threadIdx.x];
+		}
+		__syncthreads();
+	}
+	float* out = partial + (long long)blockIdx.y * STRIDE;
+	if (t < 27) {
+#pragma unroll
+		for (int q = 0; q < PER; ++q) {
+			int p = threadIdx.x + 256 * q;
+			if (p < NP) out[(long long)t * NP + p] = acc[q];
+		}
+	} else if (threadIdx.x < COUT) {
+		out[27 * NP + threadIdx.x] = acc[0];
+	}
+}
+
+// out[p] += sum_c partial[c][p], c in order
+__global__ void k_reduce_chunks(const float* __restrict__ partial, int nchunk, long long stride, float* __restrict__ out) {
+	long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= stride) return;
+	float s = 0.0f;
+	for (int c = 0; c < nchunk; ++c) s += partial[(long long)c * stride + p];
+	out[p] += s;
+}
+
+int wgrad_chunk(long long V) {
+	long long c = V / 32;
+	c = (c + 63) / 64 * 64;
+	return (int)(c < 64 ? 64 : c);
+}
+
+template <int CIN, int COUT>
+void wgrad(const uint16_t* dZ, const uint16_t* X, const int d[3], float* partial, float* grad_w, hipStream_t s) {
+	long long V = (long long)d[0] * d[1] * d[2];
+	int chunk = wgrad_chunk(V);
+	int nchunk = (int)((V + chunk - 1) / chunk);
+	hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(28, nchunk), dim3(256), 0, s, dZ, X, d[0], d[1], d[2], chunk, partial);
+	long long stride = 27LL * COUT * CIN + COUT;
+	hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0, s, partial, nchunk, stride, grad_w);
+}
+
+void wgrad_any(int cin, int cout, const uint16_t* dZ, const uint16_t* X, const int d[3], float* partial, float* grad_w,
+               hipStream_t s) {
+	if (cin == 16 && cout == 16) return wgrad<16, 16>(dZ, X, d, partial, grad_w, s);
+	if (cin == 16 && cout == 32) return wgrad<16, 32>(dZ, X, d, partial, grad_w, s);
+	if (cin == 32 && cout == 64) return wgrad<32, 64>(dZ, X, d, partial, grad_w, s);
+	if (cin == 64 && cout == 32) return wgrad<64, 32>(dZ, X, d, partial, grad_w, s);
+	if (cin == 32 && cout == 16) return wgrad<32, 16>(dZ, X, d, partial, grad_w, s);
+	throw std::runtime_error("wgrad: unsupported channel pair");
+}
+
+// ------------------------------------------------------------------ elementwise pieces
+__global__ void k_sumsq(const float* __restrict__ a, long long n, double* __restrict__ partial) {
+	__shared__ double red[256];
+	double s = 0.0;
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+		s += (double)a[i] * a[i];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_sumsq_diff(const float* __restrict__ a, const float* __restrict__ b, long long n, double* __restrict__ partial) {
+	__shared__ double red[256];
+	double s = 0.0;
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+		double d = (double)a[i] - b[i];
+		s += d * d;
+	}
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// out[k] = sum of partial[]; mode 0: plain, 1: s = sqrt(sum / n3) into out[1] too, 2: loss = out[3] / out[2] into out[4]
+__global__ void k_reduce_scalar(const double* __restrict__ partial, int n, double* __restrict__ out, int k, int mode, double n3) {
+	__shared__ double red[256];
+	double s = 0.0;
+	for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		out[k] = red[0];
+		if (mode == 1) out[1] = sqrt(red[0] / n3);
+		if (mode == 2) out[4] += out[2] > 0 ? red[0] / out[2] : 0.0;
+	}
+}
+
+// x0[v][0..2] = r / s, [3..11] = mu / mu_scale, [12..15] = 0
+__global__ void k_pack_input(const float* __restrict__ r, const float* __restrict__ mu, const double* __restrict__ sc,
+                             float inv_mu_scale, long long V, uint16_t* __restrict__ x0) {
+	long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= V) return;
+	double s = sc[1];
+	float is = s > 0 ? (float)(1.0 / s) : 0.0f;
+	uint16_t out[16];
+#pragma unroll
+	for (int c = 0; c < 3; ++c) out[c] = f2bf(r[c * V + v] * is);
+#pragma unroll
+	for (int c = 0; c < 9; ++c) out[3 + c] = f2bf(mu[c * V + v] * inv_mu_scale);
+#pragma unroll
+	for (int c = 12; c < 16; ++c) out[c] = 0;
+	uint4* dst = reinterpret_cast<uint4*>(x0 + v * 16);
+	dst[0] = make_uint4(out[0] | (uint32_t)out[1] << 16, out[2] | (uint32_t)out[3] << 16, out[4] | (uint32_t)out[5] << 16,
+	                    out[6] | (uint32_t)out[7] << 16);
+	dst[1] = make_uint4(out[8] | (uint32_t)out[9] << 16, out[10] | (uint32_t)out[11] << 16, out[12] | (uint32_t)out[13] << 16,
+	                    out[14] | (uint32_t)out[15] << 16);
+}
+
+// mean over the 8 children (bf16 in/out)
+__global__ void k_pool(const uint16_t* __restrict__ x, int c, int d0, int d1, int d2, uint16_t* __restrict__ y) {
+	const int h0 = d0 / 2, h1 = d1 / 2, h2 = d2 / 2;
+	long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	long long n = (long long)h0 * h1 * h2 * c;
+	if (e >= n) return;
+	int ch = (int)(e % c);
+	long long v = e / c;
+	int z = (int)(v % h2), yy = (int)((v / h2) % h1), xx = (int)(v / ((long long)h1 * h2));
+	float s = 0.0f;
+	for (int a = 0; a < 2; ++a)
+		for (int b = 0; b < 2; ++b)
+			for (int q = 0; q < 2; ++q) s += bf2f(x[(((long long)(2 * xx + a) * d1 + 2 * yy + b) * d2 + 2 * z + q) * c + ch]);
+	y[e] = f2bf(s * 0.125f);
+}
+
+// u = up(coarse) + skip (bf16); fine dims d
+__global__ void k_up_add(const uint16_t* __restrict__ coarse, const uint16_t* __restrict__ skip, int c, int d0, int d1, int d2,
+                         uint16_t* __restrict__ u) {
+	long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	long long n = (long long)d0 * d1 * d2 * c;
+	if (e >= n) return;
+	int ch = (int)(e % c);
+	long long v = e / c;
+	int z = (int)(v % d2), yy = (int)((v / d2) % d1), xx = (int)(v / ((long long)d1 * d2));
+	long long pv = ((long long)(xx / 2) * (d1 / 2) + yy / 2) * (d2 / 2) + z / 2;
+	u[e] = f2bf(bf2f(coarse[pv * c + ch]) + bf2f(skip[e]));
+}
+
+// z[c][v] = r[c][v] + s (sum_ci d1[v][ci] Wh[c][ci] + bh[c])
+__global__ void k_head(const uint16_t* __restrict__ d1, const float* __restrict__ Wh, const float* __restrict__ bh,
+                       const float* __restrict__ r, const double* __restrict__ sc, long long V, float* __restrict__ z) {
+	long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= V) return;
+	float s = (float)sc[1];
+	float x[kHeadIn];
+#pragma unroll
+	for (int i = 0; i < kHeadIn; ++i) x[i] = bf2f(d1[v * kHeadIn + i]);
+#pragma unroll
+	for (int c = 0; c < kHeadOut; ++c) {
+		float y = bh[c];
+#pragma unroll
+		for (int i = 0; i < kHeadIn; ++i) y += x[i] * Wh[c * kHeadIn + i];
+		z[c * V + v] = r[c * V + v] + s * y;
+	}
+}
+
+// dy = s 2 (z - e) / ||e||^2; dd1[v][i] = sum_c dy[c] Wh[c][i]; head parameter partial sums
+__global__ void __launch_bounds__(256) k_head_backward(const uint16_t* __restrict__ d1, const float* __restrict__ Wh,
+                                                       const float* __restrict__ z, const float* __restrict__ e,
+                                                       const double* __restrict__ sc, long long V, float* __restrict__ dd1,
+                                                       float* __restrict__ partial) {
+	constexpr int NQ = kHeadOut * kHeadIn + kHeadOut;
+	__shared__ float red[NQ][64];
+	float acc[NQ];
+#pragma unroll
+	for (int q = 0; q < NQ; ++q) acc[q] = 0.0f;
+	const float f = sc[2] > 0 ? (float)(sc[1] * 2.0 / sc[2]) : 0.0f;
+	for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long long)gridDim.x * blockDim.x) {
+		float dy[kHeadOut];
+#pragma unroll
+		for (int c = 0; c < kHeadOut; ++c) dy[c] = f * (z[c * V + v] - e[c * V + v]);
+#pragma unroll
+		for (int i = 0; i < kHeadIn; ++i) {
+			float x = bf2f(d1[v * kHeadIn + i]);
+			float g = 0.0f;
+#pragma unroll
+			for (int c = 0; c < kHeadOut; ++c) {
+				g += dy[c] * Wh[c * kHeadIn + i];
+				acc[c * kHeadIn + i] += dy[c] * x;
+			}
+			dd1[v * kHeadIn + i] = g;
+		}
+#pragma unroll
+		for (int c = 0; c < kHeadOut; ++c) acc[kHeadOut * kHeadIn + c] += dy[c];
+	}
+	// block reduction: lanes of each wave, then the 4 waves in order
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+	for (int q = 0; q < NQ; ++q) {
+		float v = acc[q];
+		for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+		if (lane == 0) red[q][wave] = v;
+	}
+	__syncthreads();
+	if (threadIdx.x < NQ) partial[(long long)blockIdx.x * NQ + threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] +
+	                                                                          red[threadIdx.x][2] + red[threadIdx.x][3];
+}
+
+// dZ = bf16(dout * (act > 0))
+__global__ void k_relu_mask(const float* __restrict__ dout, const uint16_t* __restrict__ act, long long n, uint16_t* __restrict__ dz) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) dz[i] = f2bf(bf2f(act[i]) > 0.0f ? dout[i] : 0.0f);
+}
+
+// coarse[p][c] = sum of the 8 children of fine[.][c]
+__global__ void k_pool_sum(const float* __restrict__ fine, int c, int d0, int d1, int d2, float* __restrict__ coarse) {
+	const int h0 = d0 / 2, h1 = d1 / 2, h2 = d2 / 2;
+	long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	long long n = (long long)h0 * h1 * h2 * c;
+	if (e >= n) return;
+	int ch = (int)(e % c);
+	long long v = e / c;
+	int z = (int)(v % h2), yy = (int)((v / h2) % h1), xx = (int)(v / ((long long)h1 * h2));
+	float s = 0.0f;
+	for (int a = 0; a < 2; ++a)
+		for (int b = 0; b < 2; ++b)
+			for (int q = 0; q < 2; ++q) s += fine[(((long long)(2 * xx + a) * d1 + 2 * yy + b) * d2 + 2 * z + q) * c + ch];
+	coarse[e] = s;
+}
+
+// out = up(coarse) / 8 + skip (float32); fine dims d
+__global__ void k_unpool_add(const float* __restrict__ coarse, const float* __restrict__ skip, int c, int d0, int d1, int d2,
+                             float* __restrict__ out) {
+	long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	long long n = (long long)d0 * d1 * d2 * c;
+	if (e >= n) return;
+	int ch = (int)(e % c);
+	long long v = e / c;
+	int z = (int)(v % d2), yy = (int)((v / d2) % d1), xx = (int)(v / ((long long)d1 * d2));
+	long long pv = ((long long)(xx / 2) * (d1 / 2) + yy / 2) * (d2 / 2) + z / 2;
+	out[e] = coarse[pv * c + ch] * 0.125f + skip[e];
+}
+
+__global__ void k_reduce_head(const float* __restrict__ partial, int nblk, float* __restrict__ grad) {
+	constexpr int NQ = kHeadOut * kHeadIn + kHeadOut;
+	int q = threadIdx.x;
+	if (q >= NQ) return;
+	float s = 0.0f;
+	for (int b = 0; b < nblk; ++b) s += partial[b * NQ + q];
+	grad[q] += s;
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v, long long n,
+                       float lr, float inv_batch, float c1, float c2) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+	float gi = g[i] * inv_batch;
+	float mi = b1 * m[i] + (1.0f - b1) * gi;
+	float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+	m[i] = mi;
+	v[i] = vi;
+	p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 forward weights W[t][co][ci] and adjoint weights W'[t][ci][co] = W[26 - t][co][ci]
+__global__ void k_weights_bf16(const float* __restrict__ w, int cin, int cout, uint16_t* __restrict__ wf, uint16_t* __restrict__ wb) {
+	int e = blockIdx.x * blockDim.x + threadIdx.x;
+	int n = 27 * cin * cout;
+	if (e >= n) return;
+	int ci = e % cin, co = (e / cin) % cout, t = e / (cin * cout);
+	uint16_t h = f2bf(w[e]);
+	wf[e] = h;
+	wb[((26 - t) * cin + ci) * cout + co] = h;
+}
+
+inline unsigned nb(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <class T>
+T* dmalloc(long long n) {
+	void* p = nullptr;
+	PC_CHECK(hipMalloc(&p, (size_t)(n > 0 ? n : 1) * sizeof(T)));
+	return static_cast<T*>(p);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ UNet
+UNet::UNet(const int d[3]) {
+	for (int a = 0; a < 3; ++a) {
+		if (d[a] < 4 || d[a] % 4) throw std::runtime_error("UNet: every grid dimension must be a multiple of 4");
+		d_[a] = d[a];
+	}
+	V_ = (long long)d[0] * d[1] * d[2];
+	const int spec[6][3] = {{16, 16, 0}, {16, 32, 1}, {32, 64, 2}, {64, 32, 2}, {32, 16, 1}, {16, 16, 0}};
+	long long off = 0;
+	for (auto& sp : spec) {
+		ConvSpec c{sp[0], sp[1], sp[2], off, off + 27LL * sp[0] * sp[1]};
+		off = c.b_off + sp[1];
+		convs_.push_back(c);
+	}
+	long long n = param_count();
+	params_ = dmalloc<float>(n);
+	grads_ = dmalloc<float>(n);
+	m_ = dmalloc<float>(n);
+	v_ = dmalloc<float>(n);
+	PC_CHECK(hipMemset(grads_, 0, n * sizeof(float)));
+	PC_CHECK(hipMemset(m_, 0, n * sizeof(float)));
+	PC_CHECK(hipMemset(v_, 0, n * sizeof(float)));
+	long long nw = 0;
+	for (auto& c : convs_) nw += 27LL * c.cin * c.cout;
+	wf_ = dmalloc<uint16_t>(nw);
+	wb_ = dmalloc<uint16_t>(nw);
+	const long long V = V_, V1 = V / 8, V2 = V / 64;
+	x0_ = dmalloc<uint16_t>(16 * V);
+	e1_ = dmalloc<uint16_t>(16 * V);
+	p1_ = dmalloc<uint16_t>(16 * V1);
+	e2_ = dmalloc<uint16_t>(32 * V1);
+	p2_ = dmalloc<uint16_t>(32 * V2);
+	e3_ = dmalloc<uint16_t>(64 * V2);
+	d3_ = dmalloc<uint16_t>(32 * V2);
+	u2_ = dmalloc<uint16_t>(32 * V1);
+	d2_ = dmalloc<uint16_t>(16 * V1);
+	u1_ = dmalloc<uint16_t>(16 * V);
+	d1_ = dmalloc<uint16_t>(16 * V);
+	dz_ = dmalloc<uint16_t>(16 * V);
+	g0_ = dmalloc<float>(16 * V);
+	g0b_ = dmalloc<float>(16 * V);
+	g1_ = dmalloc<float>(32 * V1);
+	g1b_ = dmalloc<float>(32 * V1);
+	g2_ = dmalloc<float>(64 * V2);
+	z_ = dmalloc<float>(3 * V);
+	long long worst = 0;
+	for (auto& c : convs_) {
+		long long Vl = V >> (3 * c.level);
+		long long chunk = wgrad_chunk(Vl);
+		long long nchunk = (Vl + chunk - 1) / chunk;
+		long long need = nchunk * (27LL * c.cin * c.cout + c.cout);
+		if (need > worst) worst = need;
+	}
+	long long head = (long long)kRedBlocks * (kHeadOut * kHeadIn + kHeadOut);
+	partial_len_ = worst > head ? worst : head;
+	partial_ = dmalloc<float>(partial_len_);
+	dscal_ = dmalloc<double>(8);
+	dpart_ = dmalloc<double>(kRedBlocks);
+	PC_CHECK(hipMemset(dscal_, 0, 8 * sizeof(double)));
+	// He-normal convolution weights, zero biases and a zero head: the untrained network is z = r
+	std::vector<float> h(n, 0.0f);
+	std::mt19937_64 rng(12345);
+	for (auto& c : convs_) {
+		std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (27.0f * c.cin)));
+		for (long long i = 0; i < 27LL * c.cin * c.cout; ++i) h[c.w_off + i] = nd(rng);
+	}
+	set_params(h.data(), nullptr);
+	PC_CHECK(hipDeviceSynchronize());
+}
+
+UNet::~UNet() {
+	for (void* p : {(void*)params_, (void*)grads_, (void*)m_, (void*)v_, (void*)wf_, (void*)wb_, (void*)x0_, (void*)e1_, (void*)p1_,
+	                (void*)e2_, (void*)p2_, (void*)e3_, (void*)d3_, (void*)u2_, (void*)d2_, (void*)u1_, (void*)d1_, (void*)dz_,
+	                (void*)g0_, (void*)g0b_, (void*)g1_, (void*)g1b_, (void*)g2_, (void*)z_, (void*)partial_, (void*)dscal_,
+	                (void*)dpart_})
+		if (p) (void)hipFree(p);
+}
+
+long long UNet::param_count() {
+	const int spec[6][2] = {{16, 16}, {16, 32}, {32, 64}, {64, 32}, {32, 16}, {16, 16}};
+	long long n = 0;
+	for (auto& sp : spec) n += 27LL * sp[0] * sp[1] + sp[1];
+	return n + kHeadOut * kHeadIn + kHeadOut;
+}
+
+void UNet::refresh_weights(hipStream_t s) {
+	long long off = 0;
+	for (auto& c : convs_) {
+		int n = 27 * c.cin * c.cout;
+		hipLaunchKernelGGL(k_weights_bf16, dim3(nb(n, 256)), dim3(256), 0, s, params_ + c.w_off, c.cin, c.cout, wf_ + off, wb_ + off);
+		off += n;
+	}
+}
+
+void UNet::set_params(const float* host, hipStream_t s) {
+	PC_CHECK(hipMemcpyAsync(params_, host, param_count() * sizeof(float), hipMemcpyHostToDevice, s));
+	refresh_weights(s);
+	PC_CHECK(hipStreamSynchronize(s));
+}
+
+void UNet::get_params(float* host, hipStream_t s) const {
+	PC_CHECK(hipMemcpyAsync(host, params_, param_count() * sizeof(float), hipMemcpyDeviceToHost, s));
+	PC_CHECK(hipStreamSynchronize(s));
+}
+
+void UNet::get_grads(float* host, hipStream_t s) const {
+	PC_CHECK(hipMemcpyAsync(host, grads_, param_count() * sizeof(float), hipMemcpyDeviceToHost, s));
+	PC_CHECK(hipStreamSynchronize(s));
+}
+
+void UNet::zero_grad(hipStream_t s) {
+	PC_CHECK(hipMemsetAsync(grads_, 0, param_count() * sizeof(float), s));
+	PC_CHECK(hipMemsetAsync(dscal_ + 4, 0, sizeof(double), s));
+}
+
+static long long weight_offset(const std::vector<ConvSpec>& convs, size_t k) {
+	long long off = 0;
+	for (size_t i = 0; i < k; ++i) off += 27LL * convs[i].cin * convs[i].cout;
+	return off;
+}
+
+void UNet::conv_forward(const ConvSpec& c, const uint16_t* x, uint16_t* y, int lvl, hipStream_t s) {
+	const int dl[3] = {d_[0] >> lvl, d_[1] >> lvl, d_[2] >> lvl};
+	size_t k = &c - convs_.data();
+	conv3_any<uint16_t>(c.cin, c.cout, x, wf_ + weight_offset(convs_, k), params_ + c.b_off, true, y, dl, s);
+}
+
+// dout: gradient of the layer's output (float32, [V][cout]); act: its output; input: its
+// input; din (optional): gradient of the input (float32 [V][cin]).
+void UNet::conv_backward(const ConvSpec& c, const float* dout, const uint16_t* act, const uint16_t* input, float* din, int lvl,
+                         hipStream_t s) {
+	const int dl[3] = {d_[0] >> lvl, d_[1] >> lvl, d_[2] >> lvl};
+	long long Vl = level_voxels(lvl);
+	hipLaunchKernelGGL(k_relu_mask, dim3(nb(Vl * c.cout, 256)), dim3(256), 0, s, dout, act, Vl * c.cout, dz_);
+	wgrad_any(c.cin, c.cout, dz_, input, dl, partial_, grads_ + c.w_off, s);
+	if (din) {
+		size_t k = &c - convs_.data();
+		conv3_any<float>(c.cout, c.cin, dz_, wb_ + weight_offset(convs_, k), nullptr, false, din, dl, s);
+	}
+}
+
+void UNet::forward(const float* r, const float* mu, float mu_scale, float* z, hipStream_t s) {
+	const long long V = V_;
+	hipLaunchKernelGGL(k_sumsq, dim3(kRedBlocks), dim3(256), 0, s, r, 3 * V, dpart_);
+	hipLaunchKernelGGL(k_reduce_scalar, dim3(1), dim3(256), 0, s, dpart_, kRedBlocks, dscal_, 0, 1, (double)(3 * V));
+	hipLaunchKernelGGL(k_pack_input, dim3(nb(V, 256)), dim3(256), 0, s, r, mu, dscal_, mu_scale > 0 ? 1.0f / mu_scale : 0.0f, V, x0_);
+	const int* d = d_;
+	conv_forward(convs_[0], x0_, e1_, 0, s);
+	hipLaunchKernelGGL(k_pool, dim3(nb(16 * V / 8, 256)), dim3(256), 0, s, e1_, 16, d[0], d[1], d[2], p1_);
+	conv_forward(convs_[1], p1_, e2_, 1, s);
+	hipLaunchKernelGGL(k_pool, dim3(nb(32 * V / 64, 256)), dim3(256), 0, s, e2_, 32, d[0] / 2, d[1] / 2, d[2] / 2, p2_);
+	conv_forward(convs_[2], p2_, e3_, 2, s);
+	conv_forward(convs_[3], e3_, d3_, 2, s);
+	hipLaunchKernelGGL(k_up_add, dim3(nb(32 * V / 8, 256)), dim3(256), 0, s, d3_, e2_, 32, d[0] / 2, d[1] / 2, d[2] / 2, u2_);
+	conv_forward(convs_[4], u2_, d2_, 1, s);
+	hipLaunchKernelGGL(k_up_add, dim3(nb(16 * V, 256)), dim3(256), 0, s, d2_, e1_, 16, d[0], d[1], d[2], u1_);
+	conv_forward(convs_[5], u1_, d1_, 0, s);
+	const long long hw = convs_.back().b_off + convs_.back().cout;
+	hipLaunchKernelGGL(k_head, dim3(nb(V, 256)), dim3(256), 0, s, d1_, params_ + hw, params_ + hw + kHeadOut * kHeadIn, r, dscal_, V,
+	                   z);
+}
+
+void UNet::accumulate_gradient(const float* r, const float* mu, float mu_scale, const float* e, hipStream_t s) {
+	const long long V = V_;
+	const int* d = d_;
+	forward(r, mu, mu_scale, z_, s);
+	hipLaunchKernelGGL(k_sumsq, dim3(kRedBlocks), dim3(256), 0, s, e, 3 * V, dpart_);
+	hipLaunchKernelGGL(k_reduce_scalar, dim3(1), dim3(256), 0, s, dpart_, kRedBlocks, dscal_, 2, 0, 0.0);
+	hipLaunchKernelGGL(k_sumsq_diff, dim3(kRedBlocks), dim3(256), 0, s, z_, e, 3 * V, dpart_);
+	hipLaunchKernelGGL(k_reduce_scalar, dim3(1), dim3(256), 0, s, dpart_, kRedBlocks, dscal_, 3, 2, 0.0);
+	const long long hw = convs_.back().b_off + convs_.back().cout;
+	hipLaunchKernelGGL(k_head_backward, dim3(kRedBlocks), dim3(256), 0, s, d1_, params_ + hw, z_, e, dscal_, V, g0_, partial_);
+	hipLaunchKernelGGL(k_reduce_head, dim3(1), dim3(64), 0, s, partial_, kRedBlocks, grads_ + hw);
+	const ConvSpec &enc1 = convs_[0], &enc2 = convs_[1], &enc3 = convs_[2], &dec3 = convs_[3], &dec2 = convs_[4], &dec1 = convs_[5];
+	conv_backward(dec1, g0_, d1_, u1_, g0b_, 0, s);  // du1 -> g0b (also the skip gradient of e1)
+	hipLaunchKernelGGL(k_pool_sum, dim3(nb(16 * V / 8, 256)), dim3(256), 0, s, g0b_, 16, d[0], d[1], d[2], g1_);
+	conv_backward(dec2, g1_, d2_, u2_, g1b_, 1, s);  // du2 -> g1b (skip gradient of e2)
+	hipLaunchKernelGGL(k_pool_sum, dim3(nb(32 * V / 64, 256)), dim3(256), 0, s, g1b_, 32, d[0] / 2, d[1] / 2, d[2] / 2, g2_);
+	conv_backward(dec3, g2_, d3_, e3_, g2_, 2, s);   // de3 -> g2 (dz was taken from g2 first)
+	conv_backward(enc3, g2_, e3_, p2_, g2_, 2, s);   // dp2 -> g2
+	hipLaunchKernelGGL(k_unpool_add, dim3(nb(32 * V / 8, 256)), dim3(256), 0, s, g2_, g1b_, 32, d[0] / 2, d[1] / 2, d[2] / 2, g1_);
+	conv_backward(enc2, g1_, e2_, p1_, g1_, 1, s);   // dp1 -> g1
+	hipLaunchKernelGGL(k_unpool_add, dim3(nb(16 * V, 256)), dim3(256), 0, s, g1_, g0b_, 16, d[0], d[1], d[2], g0_);
+	conv_backward(enc1, g0_, e1_, x0_, nullptr, 0, s);
+}
+
+void UNet::adam_step(float lr, int batch, hipStream_t s) {
+	++adam_t_;
+	float c1 = 1.0f - std::pow(0.9f, (float)adam_t_), c2 = 1.0f - std::pow(0.999f, (float)adam_t_);
+	long long n = param_count();
+	hipLaunchKernelGGL(k_adam, dim3(nb(n, 256)), dim3(256), 0, s, params_, grads_, m_, v_, n, lr, 1.0f / (float)(batch > 0 ? batch : 1),
+	                   c1, c2);
+	refresh_weights(s);
+}
+
+}  // namespace picgpu

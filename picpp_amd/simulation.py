@@ -17,6 +17,7 @@ from .workloads import InitialState, Workload, species_arrays
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
+_F = ctypes.POINTER(ctypes.c_float)
 
 PRECISIONS = {"fp64": _lib.PRECISION_FP64, "fp32": _lib.PRECISION_FP32}
 STOP_RULES = {"reference": _lib.STOP_REFERENCE, "true_residual": _lib.STOP_TRUE_RESIDUAL}
@@ -150,9 +151,7 @@ class Simulation:
         """GMRES solve for E^{n+theta} and newE (PIC++/fields.cpp:152-376)."""
         info = _lib.PicSolveInfo()
         self._chk(self.lib.pic_maxwell_solve(self.h, ctypes.byref(info)), "pic_maxwell_solve")
-        self.last_solve = dict(iterations=info.iterations, restarts=info.restarts,
-                               estimated_residual=info.estimated_residual, true_residual=info.true_residual,
-                               seconds=info.seconds)
+        self.last_solve = _info_dict(info)
         return self.last_solve
 
     def evaluateMagneticField(self):
@@ -177,9 +176,7 @@ class Simulation:
     def step(self) -> Dict[str, float]:
         info = _lib.PicSolveInfo()
         self._chk(self.lib.pic_step(self.h, ctypes.byref(info)), "pic_step")
-        self.last_solve = dict(iterations=info.iterations, restarts=info.restarts,
-                               estimated_residual=info.estimated_residual, true_residual=info.true_residual,
-                               seconds=info.seconds)
+        self.last_solve = _info_dict(info)
         return self.last_solve
 
     def synchronize(self):
@@ -194,5 +191,52 @@ class Simulation:
         info = _lib.PicSolveInfo()
         self._chk(self.lib.pic_gmres_solve(self.h, ctypes.cast(rhs_ptr, _D), ctypes.cast(x_ptr, _D), ctypes.byref(info)),
                   "pic_gmres_solve")
-        return dict(iterations=info.iterations, restarts=info.restarts, estimated_residual=info.estimated_residual,
-                    true_residual=info.true_residual, seconds=info.seconds)
+        return _info_dict(info)
+
+    # ------------------------------------------------------------ learned preconditioner
+    def precond_create(self):
+        self._chk(self.lib.pic_precond_create(self.h), "pic_precond_create")
+
+    def precond_params(self):
+        n = int(self.lib.pic_precond_param_count())
+        p = np.zeros(n, np.float32)
+        ms = ctypes.c_double()
+        self._chk(self.lib.pic_precond_get_params(self.h, p.ctypes.data_as(_F), ctypes.byref(ms)), "pic_precond_get_params")
+        return p, ms.value
+
+    def set_precond_params(self, params, mu_scale: float):
+        p = np.ascontiguousarray(params, np.float32)
+        self._chk(self.lib.pic_precond_set_params(self.h, p.ctypes.data_as(_F), float(mu_scale)), "pic_precond_set_params")
+
+    def precond_apply(self, r_ptr: int, mu_ptr: int, z_ptr: int):
+        self._chk(self.lib.pic_precond_apply(self.h, r_ptr, mu_ptr, z_ptr), "pic_precond_apply")
+
+    def precond_gradient(self, r_ptr: int, mu_ptr: int, e_ptr: int):
+        n = int(self.lib.pic_precond_param_count())
+        g = np.zeros(n, np.float32)
+        loss = ctypes.c_double()
+        self._chk(self.lib.pic_precond_gradient(self.h, r_ptr, mu_ptr, e_ptr, ctypes.byref(loss), g.ctypes.data_as(_F)),
+                  "pic_precond_gradient")
+        return loss.value, g
+
+    def precond_log(self, max_pairs: int) -> int:
+        rc = self.lib.pic_precond_log(self.h, int(max_pairs))
+        if max_pairs < 0:
+            return int(rc)
+        self._chk(rc, "pic_precond_log")
+        return 0
+
+    def precond_train(self, epochs: int, batch: int = 8, lr: float = 1e-3, holdout: float = 0.1):
+        tl, hl = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.pic_precond_train(self.h, int(epochs), int(batch), float(lr), float(holdout), ctypes.byref(tl),
+                                             ctypes.byref(hl)), "pic_precond_train")
+        return tl.value, hl.value
+
+    def precond_use(self, on: bool):
+        self._chk(self.lib.pic_precond_use(self.h, 1 if on else 0), "pic_precond_use")
+
+
+def _info_dict(info) -> Dict[str, float]:
+    return dict(iterations=info.iterations, restarts=info.restarts, estimated_residual=info.estimated_residual,
+                true_residual=info.true_residual, seconds=info.seconds, preconditioned=info.preconditioned,
+                fallbacks=info.fallbacks)
