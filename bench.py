@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--drift-steps", type=int, default=0,
                     help="also run the chosen form and the plain FP64 step this many steps from the same state and "
                          "report their total-energy difference")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB between two device buffers three times (the reference kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     return ap.parse_args()
@@ -265,6 +268,17 @@ def energy_drift(args, w, torch, dev):
             "fp64_rel_change": float(np.max(np.abs(b - b[0]) / abs(b[0])))}
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE counter passes
+    (scaled by a plain copy kernel's counters), when they were taken on this workload."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if args.precision != "fp32" or args.cells != 128 or args.ppc != 64 or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh)["kernels"].get(kernel)
+    return rec["traffic_bytes"] if rec else None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -276,6 +290,14 @@ def main():
     from picpp_amd import workloads as wl
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("copied 1 GiB three times", file=sys.stderr)
+        return
     learned = args.learned != "off" and args.solver == "mixed"
     training = None
     if learned:
@@ -299,6 +321,7 @@ def main():
     avg_ms = kd["ms_total"] / kd["launches"]
     per_launch = kd["bytes_total"] / kd["launches"]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom, args)
     energy = sim.updateEnergy()
     held_out = compare_solvers(sim, torch, learned) if args.solver == "mixed" else None
     sim.close()
@@ -328,7 +351,7 @@ def main():
                   "true_residual": float(max(s["true_residual"] for s in solves)), "stop_rule": args.stop_rule,
                   "held_out_system": held_out, "training": training},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": per_launch},
         "kernels_ms_per_step": {nm: kv["ms_total"] / args.steps for nm, kv in kernels.items()},
         "energy": {"particles": energy[0], "E": energy[1], "B": energy[2], "total": energy[3]},

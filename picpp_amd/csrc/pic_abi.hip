@@ -70,6 +70,11 @@ struct pic_ctx {
 	uint32_t *keys = nullptr, *keys_sorted = nullptr, *perm = nullptr, *perm_sorted = nullptr;
 	int *start = nullptr, *counts = nullptr;
 	long long nkeys = 0;
+	// rebin of the FP32 particles after a push: mover flags, their prefix count, the movers
+	// (key, old index) and per-key stay / arrival counts
+	int *moved = nullptr, *Fscan = nullptr, *start_new = nullptr, *stay = nullptr, *arrive = nullptr, *first = nullptr;
+	uint32_t *mk = nullptr, *mi = nullptr, *mk_s = nullptr, *mi_s = nullptr;
+	long long mover_capacity = 0;
 	int end_bit = 0;
 	void* cub_temp = nullptr;
 	size_t cub_temp_bytes = 0;
@@ -207,6 +212,53 @@ void sort_particles(pic_ctx* c) {
 	launch_count_keys(c->keys_sorted, n, c->counts, s);
 	size_t bytes = c->cub_temp_bytes;
 	HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start, (int)(c->nkeys + 1), s));
+}
+
+// Restores the cell-sorted order of the FP32 particles after a push without a full sort
+// (pic_kernels_fp32.hip, rebin kernels); more than a quarter of the particles changing
+// segment takes the full sort instead. Same result as a stable sort by key within each new
+// segment's stayers and arrivals.
+void rebin_particles32(pic_ctx* c) {
+	const long long n = c->np();
+	hipStream_t s = c->stream;
+	int M = 0;
+	if (n > 0) {
+		Timed t(c, "rebin_scan", (double)n * 8);
+		HIP_CHECK(hipMemsetAsync(c->Fscan, 0, sizeof(int), s));
+		size_t bytes = c->cub_temp_bytes;
+		HIP_CHECK(hipcub::DeviceScan::InclusiveSum(c->cub_temp, bytes, c->moved, c->Fscan + 1, (int)n, s));
+		HIP_CHECK(hipMemcpyAsync(&M, c->Fscan + n, sizeof(int), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+	}
+	if (n == 0 || M > c->mover_capacity) {
+		sort_particles(c);
+		return;
+	}
+	{
+		Timed t(c, "rebin_movers", (double)n * 12 + (double)M * 32 + (double)(c->nkeys + 1) * 4);
+		launch_gather_movers(c->moved, c->Fscan, c->keys, n, c->mk, c->mi, s);
+		if (M > 0) {
+			size_t bytes = c->cub_temp_bytes;
+			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_temp, bytes, c->mk, c->mk_s, c->mi, c->mi_s, M, 0, c->end_bit, s));
+		}
+		HIP_CHECK(hipMemsetAsync(c->arrive, 0, (c->nkeys + 1) * sizeof(int), s));
+		launch_count_keys(c->mk_s, M, c->arrive, s);
+		launch_first_of_run(c->mk_s, M, c->first, s);
+	}
+	{
+		Timed t(c, "rebin_segments", (double)(c->nkeys + 1) * 4 * 6);
+		launch_rebin_counts(c->start, c->Fscan, c->arrive, c->nkeys, c->stay, c->counts, s);
+		size_t bytes = c->cub_temp_bytes;
+		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start_new, (int)(c->nkeys + 1), s));
+	}
+	{
+		Timed t(c, "rebin_scatter", (double)n * (2 * 32 + 12));
+		launch_rebin_scatter(c->Q, c->Qalt, c->moved, c->Fscan, c->keys, c->start, c->start_new, c->mk_s, c->mi_s, M, c->stay,
+		                     c->first, s);
+	}
+	c->Qalt.count = c->Q.count;
+	std::swap(c->Q, c->Qalt);
+	std::swap(c->start, c->start_new);
 }
 
 void density(pic_ctx* c) {
@@ -541,11 +593,23 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->start = dalloc<int>(c->nkeys + 1);
 		c->counts = dalloc<int>(c->nkeys + 1);
 		HIP_CHECK(hipMemset(c->start, 0, (c->nkeys + 1) * sizeof(int)));
-		size_t b1 = 0, b2 = 0;
+		size_t b1 = 0, b2 = 0, b3 = 0;
 		HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c->keys, c->keys_sorted, c->perm, c->perm_sorted,
 		                                             (int)std::max<long long>(c->capacity, 1), 0, c->end_bit, c->stream));
 		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->counts, c->start, (int)(c->nkeys + 1), c->stream));
-		c->cub_temp_bytes = std::max(b1, b2);
+		if (c->fp32) {
+			c->moved = dalloc<int>(c->capacity);
+			c->Fscan = dalloc<int>(c->capacity + 1);
+			c->start_new = dalloc<int>(c->nkeys + 1);
+			c->stay = dalloc<int>(c->nkeys + 1);
+			c->arrive = dalloc<int>(c->nkeys + 1);
+			c->first = dalloc<int>(c->nkeys + 1);
+			c->mover_capacity = c->capacity / 4 + 1;
+			for (uint32_t** p : {&c->mk, &c->mi, &c->mk_s, &c->mi_s}) *p = dalloc<uint32_t>(c->mover_capacity);
+			HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, b3, c->moved, c->Fscan + 1, (int)std::max<long long>(c->capacity, 1),
+			                                           c->stream));
+		}
+		c->cub_temp_bytes = std::max(std::max(b1, b2), b3);
 		c->cub_temp = dalloc<char>(c->cub_temp_bytes);
 		c->species_start = dalloc<int>(kMaxSpecies + 1);
 		c->species_start_h.assign(kMaxSpecies + 1, 0);
@@ -626,7 +690,8 @@ void pic_destroy(pic_ctx* c) {
 	for (double* v : c->V) f(v);
 	for (float* v : c->V32) f(v);
 	for (void* p : {(void*)c->V32dev, (void*)c->mu32, (void*)c->w32, (void*)c->w64, (void*)c->tmp32, (void*)c->Z32dev,
-	                (void*)c->log_r, (void*)c->log_e})
+	                (void*)c->log_r, (void*)c->log_e, (void*)c->moved, (void*)c->Fscan, (void*)c->start_new, (void*)c->stay,
+	                (void*)c->arrive, (void*)c->first, (void*)c->mk, (void*)c->mi, (void*)c->mk_s, (void*)c->mi_s})
 		f(p);
 	for (float* v : c->Z32) f(v);
 	for (float* v : c->log_mu) f(v);
@@ -1132,14 +1197,17 @@ int pic_push(pic_ctx* c) {
 			}
 			Timed t(c, "push", (double)c->np() * 60 + ((double)g.nnodes * 6 + (double)g.ncells * 3) * 4);
 			launch_push32(g, c->sp, c->Q, c->species_start, c->E32, c->Et32, c->B32, (float)c->cfg.dt, (float)c->eta, c->keys,
-			              c->status, c->stream);
+			              c->moved, c->status, c->stream);
 		} else {
 			Timed t(c, "push", (double)c->P.count * 100 + ((double)c->g.nnodes * 6 + (double)c->g.ncells * 3) * 8);
 			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->status,
 			            c->stream);
 		}
 		check_status(c);
-		sort_particles(c);
+		if (c->fp32)
+			rebin_particles32(c);
+		else
+			sort_particles(c);
 		density(c);
 	} catch (const DeviceError& e) {
 		return fail(c, e);

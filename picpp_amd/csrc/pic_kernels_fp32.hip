@@ -294,6 +294,32 @@ __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp,
 }
 
 // Dual cells: base b collects the segments (species, cell b + a, octant 1 - a), a in {0,1}^3.
+// A dual-cell workgroup serves Tj x Tk bases (i, j0 .. j0+Tj-1, k0 .. k0+Tk-1) of one x plane.
+// The octants of a source cell with one x bit are contiguous (octant = x<<2 | y<<1 | z), and
+// each x half feeds exactly one base plane, so a plane tile reads half cells once; only the
+// tile's j and k edges are shared with its neighbours. Workgroups are renumbered so that each
+// XCD (blocks are dealt to the 8 XCDs in turn) walks a contiguous run of tiles and finds the
+// shared edges in its own L2.
+struct PlaneTile {
+	int i, j0, k0;
+	bool ok;
+};
+
+__device__ inline PlaneTile plane_tile(const Grid& g, int Tj, int Tk) {
+	const int nj = (g.N[1] + Tj - 1) / Tj, nk = (g.N[2] + Tk - 1) / Tk;
+	const long long n = (long long)g.N[0] * nj * nk;
+	const long long per = (n + 7) / 8;
+	long long L = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+	PlaneTile t;
+	t.ok = L < n;
+	if (!t.ok) L = 0;
+	t.k0 = (int)(L % nk) * Tk;
+	L /= nk;
+	t.j0 = (int)(L % nj) * Tj;
+	t.i = (int)(L / nj);
+	return t;
+}
+
 // The 8 segments of one (dual cell, species): combined index v in [0, p[8]) lies in segment
 // a = max{q : p[q] <= v}; its particle index is v + off[a].
 struct DualSegments {
@@ -301,14 +327,12 @@ struct DualSegments {
 	int off[8];
 };
 
-__device__ inline void dual_segments(const Grid& g, const int* __restrict__ start, const DualTile& tile, int kb, int s,
-                                     DualSegments& d) {
-	const int bk = tile.k0 + kb;
+__device__ inline void dual_segments(const Grid& g, const int* __restrict__ start, int bi, int bj, int bk, int s, DualSegments& d) {
 	d.p[0] = 0;
 #pragma unroll
 	for (int a = 0; a < 8; ++a) {
 		const int ax = (a >> 2) & 1, ay = (a >> 1) & 1, az = a & 1;
-		const int ci = tile.i + ax, cj = tile.j + ay, ck = bk + az;
+		const int ci = bi + ax, cj = bj + ay, ck = bk + az;
 		int lo = 0, len = 0;
 		if (ci < g.N[0] && cj < g.N[1] && ck < g.N[2]) {
 			long long key = (((long long)s * g.ncells + cell_id(g, ci, cj, ck)) << 3) | (7 - a);
@@ -322,25 +346,27 @@ __device__ inline void dual_segments(const Grid& g, const int* __restrict__ star
 
 // Cell moments (rhohat corners and the reference's pressure-tensor partials) per dual cell.
 // The B stencil (cells b + t) is shared by the dual cell's particles and lives in registers;
-// the E stencil depends on the source cell and is read from the tile's 3 x 3 x (T+2) box of
-// nodes in LDS.
+// the E stencil depends on the source cell and is read from the tile's 3 x (Tj+2) x (Tk+2)
+// box of nodes in LDS.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_cell_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
                                                         const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
-                                                        int G, int T, float* __restrict__ rho_out, float* __restrict__ pi_out) {
-	extern __shared__ float4 En[];  // 3 x 3 x (T+2) nodes from the tile's base corner
-	const DualTile tile = dual_tile(g, T);
+                                                        int G, int Tj, int Tk, float* __restrict__ rho_out,
+                                                        float* __restrict__ pi_out) {
+	extern __shared__ float4 En[];  // 3 x (Tj+2) x (Tk+2) nodes from the tile's base corner
+	const PlaneTile tile = plane_tile(g, Tj, Tk);
 	const int lt = threadIdx.x;
-	const int nk = T + 2;
-	for (int q = lt; q < 9 * nk; q += blockDim.x) {
-		int dk = q % nk, dj = (q / nk) % 3, di = q / (3 * nk);
-		int ni = clampi(tile.i + di, 0, g.N[0]), nj = clampi(tile.j + dj, 0, g.N[1]), nkk = clampi(tile.k0 + dk, 0, g.N[2]);
+	const int T = Tj * Tk, mj = Tj + 2, mk = Tk + 2;
+	for (int q = lt; q < 3 * mj * mk; q += blockDim.x) {
+		int dk = q % mk, dj = (q / mk) % mj, di = q / (mj * mk);
+		int ni = clampi(tile.i + di, 0, g.N[0]), nj = clampi(tile.j0 + dj, 0, g.N[1]), nkk = clampi(tile.k0 + dk, 0, g.N[2]);
 		En[q] = E4[node_id(g, ni, nj, nkk)];
 	}
 	__syncthreads();
 
 	const int kb = lt / G, lane = lt % G;
-	const int bk = tile.k0 + kb;
-	const bool base_ok = kb < T && bk < g.N[2];
+	const int jb = kb / Tk, kk = kb % Tk;
+	const int bj = tile.j0 + jb, bk = tile.k0 + kk;
+	const bool base_ok = tile.ok && kb < T && bj < g.N[1] && bk < g.N[2];
 	float ar[8], ap[8][6];
 #pragma unroll
 	for (int q = 0; q < 8; ++q) {
@@ -354,11 +380,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 		bool have_sb = false;
 		for (int s = 0; s < sp.count; ++s) {
 			DualSegments d;
-			dual_segments(g, start, tile, kb, s, d);
+			dual_segments(g, start, tile.i, bj, bk, s, d);
 			if (d.p[8] == 0) continue;
 			if (!have_sb) {  // cells b .. b+1 exist whenever b has particles
 				const int cj = g.N[2], ci = g.N[1] * cj;
-				load_stencil(B4, tile.i * ci + tile.j * cj + bk, ci, cj, SB);
+				load_stencil(B4, tile.i * ci + bj * cj + bk, ci, cj, SB);
 				have_sb = true;
 			}
 			const float m = (float)sp.mass[s], q = (float)sp.charge[s], w = (float)sp.weight[s];
@@ -385,11 +411,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 					cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
 				}
 				float Ef[3] = {0.0f, 0.0f, 0.0f}, Bf[3], al[9];
-				const int eb = (ax * 3 + ay) * nk + kb + az;
+				const int eb = (ax * mj + jb + ay) * mk + kk + az;
 #pragma unroll
 				for (int t = 0; t < 8; ++t) {
 					const int tx = (t >> 2) & 1, ty = (t >> 1) & 1, tz = t & 1;
-					float4 ev = En[eb + (tx * 3 + ty) * nk + tz];
+					float4 ev = En[eb + (tx * mj + ty) * mk + tz];
 					float wt = nw[0][tx] * nw[1][ty] * nw[2][tz];
 					Ef[0] += ev.x * wt;
 					Ef[1] += ev.y * wt;
@@ -426,7 +452,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 		for (int r = 0; r < 6; ++r) ap[q][r] = group_sum(ap[q][r], G);
 	}
 	if (lane == 0 && base_ok) {
-		long long b = cell_id(g, tile.i, tile.j, bk);
+		long long b = cell_id(g, tile.i, bj, bk);
 #pragma unroll
 		for (int q = 0; q < 8; ++q) {
 			rho_out[(long long)q * g.ncells + b] = ar[q];
@@ -440,14 +466,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 // species' particles round-robin, so the per-species sums reduce over the whole group.
 // Output planes (float): per species [conc, energy, bulk x3] x 8 corners, then rho x 8.
 __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start, int G,
-                                                   int T, float* __restrict__ out) {
-	const DualTile tile = dual_tile(g, T);
+                                                   int Tj, int Tk, float* __restrict__ out) {
+	const PlaneTile tile = plane_tile(g, Tj, Tk);
 	const int lt = threadIdx.x;
-
+	const int T = Tj * Tk;
 	const int kb = lt / G, lane = lt % G;
-	const int bk = tile.k0 + kb;
-	const bool base_ok = kb < T && bk < g.N[2];
-	const long long b = base_ok ? cell_id(g, tile.i, tile.j, bk) : 0;
+	const int bj = tile.j0 + kb / Tk, bk = tile.k0 + kb % Tk;
+	const bool base_ok = tile.ok && kb < T && bj < g.N[1] && bk < g.N[2];
+	const long long b = base_ok ? cell_id(g, tile.i, bj, bk) : 0;
 	const float wvol = (float)(1.0 / g.volume);
 	float rho[8];
 #pragma unroll
@@ -461,7 +487,7 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 		}
 		if (base_ok) {
 			DualSegments d;
-			dual_segments(g, start, tile, kb, s, d);
+			dual_segments(g, start, tile.i, bj, bk, s, d);
 			const float m = (float)sp.mass[s];
 			const float wv = (float)sp.weight[s] * wvol;
 			const float pc = (float)sp.charge[s] * wv;
@@ -533,7 +559,7 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start,
                                                 const float4* __restrict__ E, const float4* __restrict__ Et,
                                                 const float4* __restrict__ B, float dt, float eta, uint32_t* __restrict__ keys,
-                                                int* __restrict__ status) {
+                                                int* __restrict__ moved, int* __restrict__ status) {
 	long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (idx >= P.count) return;
 	int i = (int)idx;
@@ -543,10 +569,13 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 	float qdm = q * dt / m;
 	float beta = 0.5f * q * dt / m;
 	float inv_d[3] = {(float)(1.0 / g.d[0]), (float)(1.0 / g.d[1]), (float)(1.0 / g.d[2])};
+	const int cell0 = P.cell[i];
 	int cc[3];
-	cell_ijk(g, P.cell[i], cc);
+	cell_ijk(g, cell0, cc);
 	float f[3], u[3];
 	load32(P, i, f, u);
+	const uint32_t key0 = (uint32_t)((((long long)s * g.ncells + cell0) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) |
+	                                 (f[2] >= 0.5f));
 	Map32 mp[3];
 	bool ok = maps32(g, cc, f, mp);
 	float En[3], Bf[3], a[9], Ef[3];
@@ -629,7 +658,77 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		P.off[c][i] = f[c];
 		P.u[c][i] = nu[c];
 	}
-	keys[i] = (uint32_t)((((long long)s * g.ncells + cell) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) | (f[2] >= 0.5f));
+	const uint32_t key = (uint32_t)((((long long)s * g.ncells + cell) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) |
+	                                (f[2] >= 0.5f));
+	keys[i] = key;
+	moved[i] = key != key0;
+}
+
+// ------------------------------------------------------------------ rebin after a push
+// Few particles change segment in one step, so the cell-sorted order is restored without a
+// full sort. New segment k holds its stayers in their old order, then its arrivals in the
+// order of their old indices: deterministic, and the same order a stable sort by key gives
+// inside each group. F is the inclusive prefix count of movers shifted by one (F[0] = 0).
+
+// movers in old-index order: mk[F[i]] = key, mi[F[i]] = i
+__global__ void k_gather_movers(const int* __restrict__ moved, const int* __restrict__ F, const uint32_t* __restrict__ keys, long long n,
+                                uint32_t* __restrict__ mk, uint32_t* __restrict__ mi) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n || !moved[i]) return;
+	int m = F[i];
+	mk[m] = keys[i];
+	mi[m] = (uint32_t)i;
+}
+
+// first[k] = index of the first sorted mover with key k
+__global__ void k_first_of_run(const uint32_t* __restrict__ mk, long long M, int* __restrict__ first) {
+	long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (m >= M) return;
+	if (m == 0 || mk[m - 1] != mk[m]) first[mk[m]] = (int)m;
+}
+
+// stay[k] = old segment length minus its leavers; counts[k] = stay[k] + arrivals[k]
+__global__ void k_rebin_counts(const int* __restrict__ start_old, const int* __restrict__ F, const int* __restrict__ arrive, long long nkeys,
+                               int* __restrict__ stay, int* __restrict__ counts) {
+	long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k > nkeys) return;
+	if (k == nkeys) {
+		counts[k] = 0;
+		return;
+	}
+	int lo = start_old[k], hi = start_old[k + 1];
+	int st = (hi - lo) - (F[hi] - F[lo]);
+	stay[k] = st;
+	counts[k] = st + arrive[k];
+}
+
+__device__ inline void copy_particle(const ParticleArrays32& src, const ParticleArrays32& dst, long long from, long long to) {
+	dst.cell[to] = src.cell[from];
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		dst.off[a][to] = src.off[a][from];
+		dst.u[a][to] = src.u[a][from];
+	}
+	dst.id[to] = src.id[from];
+}
+
+__global__ void k_rebin_stayers(ParticleArrays32 src, ParticleArrays32 dst, const int* __restrict__ moved, const int* __restrict__ F,
+                                const uint32_t* __restrict__ keys, const int* __restrict__ start_old, const int* __restrict__ start_new) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= src.count || moved[i]) return;
+	uint32_t k = keys[i];
+	int so = start_old[k];
+	long long to = start_new[k] + (i - so) - (F[i] - F[so]);
+	copy_particle(src, dst, i, to);
+}
+
+__global__ void k_rebin_movers(ParticleArrays32 src, ParticleArrays32 dst, const uint32_t* __restrict__ mk, const uint32_t* __restrict__ mi,
+                               long long M, const int* __restrict__ start_new, const int* __restrict__ stay, const int* __restrict__ first) {
+	long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (m >= M) return;
+	uint32_t k = mk[m];
+	long long to = (long long)start_new[k] + stay[k] + (m - first[k]);
+	copy_particle(src, dst, mi[m], to);
 }
 
 __global__ void k_keys32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start, uint32_t* __restrict__ keys) {
@@ -831,23 +930,61 @@ void launch_node_moments32(const Grid& g, const SpeciesTable& sp, const Particle
 	hipLaunchKernelGGL(k_node_moments32, dim3(deposit_blocks(g, T)), dim3(T * G), box_bytes(T), s, g, sp, P, start,
 	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, theta, G, T, scratch);
 }
+// Dual-cell plane tiles: Tk = 4 bases along z (fewer when T < 4), Tj = T / Tk along y; the
+// grid is a multiple of 8 blocks (one run of tiles per XCD).
+struct PlaneTiling {
+	int Tj, Tk;
+	unsigned blocks;
+};
+PlaneTiling plane_tiling(const Grid& g, int T) {
+	PlaneTiling p;
+	p.Tk = T >= 4 ? 4 : T;
+	p.Tj = T / p.Tk;
+	long long n = (long long)g.N[0] * ((g.N[1] + p.Tj - 1) / p.Tj) * ((g.N[2] + p.Tk - 1) / p.Tk);
+	p.blocks = (unsigned)(8 * ((n + 7) / 8));
+	return p;
+}
+
 void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
                            const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s) {
 	int G = group_lanes(sp.count), T = deposit_tile(G);
-	hipLaunchKernelGGL(k_cell_moments32, dim3(deposit_blocks(g, T)), dim3(T * G), box_bytes(T), s, g, sp, P, start,
-	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, G, T, rho_scratch, pi_scratch);
+	PlaneTiling pt = plane_tiling(g, T);
+	size_t box = (size_t)3 * (pt.Tj + 2) * (pt.Tk + 2) * sizeof(float4);
+	hipLaunchKernelGGL(k_cell_moments32, dim3(pt.blocks), dim3(T * G), box, s, g, sp, P, start, reinterpret_cast<const float4*>(E),
+	                   reinterpret_cast<const float4*>(B), dt, G, pt.Tj, pt.Tk, rho_scratch, pi_scratch);
 }
 void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
                       hipStream_t s) {
 	int G = group_lanes(sp.count), T = deposit_tile(G);
-	hipLaunchKernelGGL(k_density32, dim3(deposit_blocks(g, T)), dim3(T * G), 0, s, g, sp, P, start, G, T, scratch);
+	PlaneTiling pt = plane_tiling(g, T);
+	hipLaunchKernelGGL(k_density32, dim3(pt.blocks), dim3(T * G), 0, s, g, sp, P, start, G, pt.Tj, pt.Tk, scratch);
 }
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
-                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* status, hipStream_t s) {
+                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, hipStream_t s) {
 	if (P.count == 0) return;
 	hipLaunchKernelGGL(k_push32, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start,
 	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(Et), reinterpret_cast<const float4*>(B), dt,
-	                   eta, keys, status);
+	                   eta, keys, moved, status);
+}
+void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, uint32_t* mk, uint32_t* mi, hipStream_t s) {
+	if (n == 0) return;
+	hipLaunchKernelGGL(k_gather_movers, dim3(blocks32(n, 256)), dim3(256), 0, s, moved, F, keys, n, mk, mi);
+}
+void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_t s) {
+	if (M == 0) return;
+	hipLaunchKernelGGL(k_first_of_run, dim3(blocks32(M, 256)), dim3(256), 0, s, mk, M, first);
+}
+void launch_rebin_counts(const int* start_old, const int* F, const int* arrive, long long nkeys, int* stay, int* counts, hipStream_t s) {
+	hipLaunchKernelGGL(k_rebin_counts, dim3(blocks32(nkeys + 1, 256)), dim3(256), 0, s, start_old, F, arrive, nkeys, stay, counts);
+}
+void launch_rebin_scatter(const ParticleArrays32& src, const ParticleArrays32& dst, const int* moved, const int* F, const uint32_t* keys,
+                          const int* start_old, const int* start_new, const uint32_t* mk, const uint32_t* mi, long long M,
+                          const int* stay, const int* first, hipStream_t s) {
+	if (src.count > 0)
+		hipLaunchKernelGGL(k_rebin_stayers, dim3(blocks32(src.count, 256)), dim3(256), 0, s, src, dst, moved, F, keys, start_old,
+		                   start_new);
+	if (M > 0)
+		hipLaunchKernelGGL(k_rebin_movers, dim3(blocks32(M, 256)), dim3(256), 0, s, src, dst, mk, mi, M, start_new, stay, first);
 }
 void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
                    hipStream_t s) {

@@ -23,7 +23,13 @@ void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const Particle
 void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
                       hipStream_t s);
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
-                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* status, hipStream_t s);
+                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, hipStream_t s);
+void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, uint32_t* mk, uint32_t* mi, hipStream_t s);
+void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_t s);
+void launch_rebin_counts(const int* start_old, const int* F, const int* arrive, long long nkeys, int* stay, int* counts, hipStream_t s);
+void launch_rebin_scatter(const ParticleArrays32& src, const ParticleArrays32& dst, const int* moved, const int* F, const uint32_t* keys,
+                          const int* start_old, const int* start_new, const uint32_t* mk, const uint32_t* mi, long long M,
+                          const int* stay, const int* first, hipStream_t s);
 void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
                    hipStream_t s);
 void launch_permute32(const ParticleArrays32& src, const ParticleArrays32& dst, const uint32_t* perm, hipStream_t s);

@@ -122,3 +122,42 @@ def test_fp32_energy_tracks_fp64_over_steps():
     drift = np.max(np.abs(e["fp32"] - e["fp64"]) / np.abs(e["fp64"]))
     print("max relative total-energy difference fp32 vs fp64 over %d steps: %.2e" % (w.steps, drift))
     assert drift <= 1e-6
+
+
+def test_fp32_rebin_keeps_particles_cell_sorted():
+    """After each push the FP32 path restores the cell-sorted order by rebinning only the
+    particles that changed segment. Over several steps, at a time step large enough that many
+    particles change octant, the particles must stay sorted by (species, cell, octant), each
+    id must appear once, and the state must match a run whose order is rebuilt by a full sort
+    (the FP64 path, within the FP32 bar)."""
+    w = wl.mini(n=(12, 10, 8), ppc=8, seed=17)
+    w = type(w)(**{**w.__dict__, "dt": 0.5})
+    st = wl.make_initial_state(w, perturb=0.05)
+    print("case", w.name, "dt 0.5, 6 steps", flush=True)
+    sim = Simulation(w, precision="fp32")
+    sim.load_state(st)
+    sim.initialize()
+    ref = Simulation(w, precision="fp64")
+    ref.load_state(st)
+    ref.initialize()
+    x0 = np.array([v * w.dx - 2 * w.dx for v in w.n])
+    N = np.array(w.N)
+    for step in range(6):
+        sim.step()
+        ref.step()
+        p = sim.particles()
+        r = (p["x"] - x0) / w.dx
+        c = np.floor(r).astype(np.int64)
+        f = r - c
+        cell = (c[:, 0] * N[1] + c[:, 1]) * N[2] + c[:, 2]
+        octant = ((f[:, 0] >= 0.5).astype(np.int64) << 2) | ((f[:, 1] >= 0.5).astype(np.int64) << 1) | (f[:, 2] >= 0.5)
+        key = ((p["species"].astype(np.int64) * int(np.prod(N)) + cell) << 3) | octant
+        assert np.all(np.diff(key) >= 0), step
+        assert np.array_equal(np.sort(p["ids"]), np.sort(st.ids))
+    q = ref.particles()
+    gi, oi = np.argsort(p["ids"]), np.argsort(q["ids"])
+    ex, ep = relerr(p["x"][gi], q["x"][oi]), relerr(p["p"][gi], q["p"][oi])
+    print("after 6 steps fp32 vs fp64: x %.2e, p %.2e" % (ex, ep))
+    assert ex <= TOL32 and ep <= 1e-4
+    sim.close()
+    ref.close()
