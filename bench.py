@@ -56,6 +56,11 @@ def parse():
     ap.add_argument("--drift-steps", type=int, default=0,
                     help="also run the chosen form and the plain FP64 step this many steps from the same state and "
                          "report their total-energy difference")
+    ap.add_argument("--weibel", type=int, default=0, metavar="STEPS",
+                    help="only run the counterstreaming Weibel case (e-/e+ beams along +/-z, drift gamma 5) for STEPS steps "
+                         "in the chosen form and in FP64, and compare magnetic-energy growth and total energy")
+    ap.add_argument("--weibel-cells", type=int, default=192)
+    ap.add_argument("--weibel-ppc", type=int, default=32, help="particles per cell per species per beam")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB between two device buffers three times (the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -93,8 +98,18 @@ def make_state_on_device(w, torch, device):
         cost = 2 * torch.rand(cnt, generator=gen, device=device, dtype=torch.float64) - 1
         phi = 2 * math.pi * torch.rand(cnt, generator=gen, device=device, dtype=torch.float64)
         sint = torch.sqrt((1 - cost * cost).clamp_min(0))
-        p = torch.stack([mag * sint * torch.cos(phi), mag * sint * torch.sin(phi), mag * cost]) * s.mass
+        u = torch.stack([mag * sint * torch.cos(phi), mag * sint * torch.sin(phi), mag * cost])
         del cost, phi, sint, mag
+        if s.drift_gamma > 1.0:  # boost along the drift axis (volume-transform flipping, as workloads.boost)
+            beta = s.drift_sign * math.sqrt(1.0 - 1.0 / (s.drift_gamma * s.drift_gamma))
+            g = torch.sqrt(1.0 + (u * u).sum(0))
+            ua = u[s.drift_axis]
+            flip = (-beta * ua / g) > torch.rand(cnt, generator=gen, device=device, dtype=torch.float64)
+            ua = torch.where(flip, -ua, ua)
+            u[s.drift_axis] = s.drift_gamma * (ua + beta * g)
+            del g, ua, flip
+        p = u * s.mass
+        del u
         xs.append(x)
         ps.append(p)
         counts.append(cnt)
@@ -279,6 +294,49 @@ def measured_traffic(kernel, args):
     return rec["traffic_bytes"] if rec else None
 
 
+def weibel_check(args, torch, dev):
+    """Counterstreaming Weibel filamentation: the chosen form and the plain FP64 step from the
+    same state. The magnetic energy's growth rate is the steepest least-squares slope of
+    ln W_B over 20-step windows; total energy is compared step by step."""
+    from picpp_amd import workloads as wl
+
+    w = wl.weibel(n=(args.weibel_cells,) * 3, ppc=args.weibel_ppc, steps=args.weibel)
+    res = {}
+    for name, kw in (("chosen", dict(precision=args.precision, stop_rule=args.stop_rule, solver=args.solver)),
+                     ("fp64", dict(precision="fp64", stop_rule="true_residual", solver="fp64"))):
+        sim = open_box(w, torch, dev, **kw)
+        npart = sim.num_particles()
+        e = [sim.updateEnergy()[:4]]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.weibel):
+            sim.step()
+            e.append(sim.updateEnergy()[:4])
+            if (i + 1) % 25 == 0:
+                print("weibel %s: step %d" % (name, i + 1), file=sys.stderr, flush=True)
+        el = time.perf_counter() - t0
+        sim.close()
+        del sim
+        torch.cuda.empty_cache()
+        res[name] = (np.array(e), el, npart)
+    t = np.arange(args.weibel + 1) * w.dt
+    out = {"cells": [args.weibel_cells] * 3, "particles": res["chosen"][2], "steps": args.weibel, "dt": w.dt}
+    for name, (e, el, npart) in res.items():
+        lb = np.log(np.maximum(e[:, 2], 1e-300))
+        win = 20
+        # from step 1: B starts at zero and step 1 carries the particle noise
+        slopes = [np.polyfit(t[k:k + win], lb[k:k + win], 1)[0] for k in range(1, len(t) - win)]
+        k = 1 + int(np.argmax(slopes))
+        out[name] = {"ms_per_step": 1e3 * el / args.weibel, "particle_steps_per_s": npart * args.weibel / el,
+                     "magnetic_energy_growth_rate": float(slopes[k - 1]), "growth_window_steps": [k, k + win],
+                     "magnetic_energy_final": float(e[-1, 2]), "total_energy_rel_change": float(np.max(np.abs(e[:, 3] - e[0, 3]) / e[0, 3]))}
+    a, b = res["chosen"][0], res["fp64"][0]
+    out["growth_rate_rel_difference"] = abs(out["chosen"]["magnetic_energy_growth_rate"] / out["fp64"]["magnetic_energy_growth_rate"] - 1)
+    out["total_energy_max_rel_difference"] = float(np.max(np.abs(a[:, 3] - b[:, 3]) / np.abs(b[:, 3])))
+    out["magnetic_energy_max_rel_difference"] = float(np.max(np.abs(a[:, 2] - b[:, 2]) / np.maximum(np.abs(b[:, 2]), 1e-300)))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -297,6 +355,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print("copied 1 GiB three times", file=sys.stderr)
+        return
+    if args.weibel > 0:
+        print(json.dumps({"weibel": weibel_check(args, torch, dev)}))
         return
     learned = args.learned != "off" and args.solver == "mixed"
     training = None

@@ -252,9 +252,14 @@ void rebin_particles32(pic_ctx* c) {
 		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start_new, (int)(c->nkeys + 1), s));
 	}
 	{
-		Timed t(c, "rebin_scatter", (double)n * (2 * 32 + 12));
-		launch_rebin_scatter(c->Q, c->Qalt, c->moved, c->Fscan, c->keys, c->start, c->start_new, c->mk_s, c->mi_s, M, c->stay,
-		                     c->first, s);
+		Timed t(c, "rebin_perm", (double)n * (3 * 4 + 4) + (double)c->nkeys * 12);
+		// the arrival counts are folded into start_new by now: their array takes the per-key shifts
+		launch_rebin_perm(n, c->Fscan, c->keys, c->start, c->start_new, c->nkeys, c->arrive, c->mk_s, c->mi_s, M, c->stay, c->first,
+		                  c->perm_sorted, s);
+	}
+	{
+		Timed t(c, "permute", (double)n * (2 * 32 + 4));
+		launch_permute32(c->Q, c->Qalt, c->perm_sorted, s);
 	}
 	c->Qalt.count = c->Q.count;
 	std::swap(c->Q, c->Qalt);

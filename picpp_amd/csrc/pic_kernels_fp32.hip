@@ -702,33 +702,34 @@ __global__ void k_rebin_counts(const int* __restrict__ start_old, const int* __r
 	counts[k] = st + arrive[k];
 }
 
-__device__ inline void copy_particle(const ParticleArrays32& src, const ParticleArrays32& dst, long long from, long long to) {
-	dst.cell[to] = src.cell[from];
-#pragma unroll
-	for (int a = 0; a < 3; ++a) {
-		dst.off[a][to] = src.off[a][from];
-		dst.u[a][to] = src.u[a][from];
-	}
-	dst.id[to] = src.id[from];
-}
-
-__global__ void k_rebin_stayers(ParticleArrays32 src, ParticleArrays32 dst, const int* __restrict__ moved, const int* __restrict__ F,
-                                const uint32_t* __restrict__ keys, const int* __restrict__ start_old, const int* __restrict__ start_new) {
-	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= src.count || moved[i]) return;
-	uint32_t k = keys[i];
+// shift[k] = start_new[k] - start_old[k] + F[start_old[k]]: a stayer i of segment k lands at
+// i - F[i] + shift[k] (its old offset in the segment minus the leavers before it)
+__global__ void k_rebin_shift(const int* __restrict__ start_old, const int* __restrict__ start_new, const int* __restrict__ F,
+                              long long nkeys, int* __restrict__ shift) {
+	long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= nkeys) return;
 	int so = start_old[k];
-	long long to = start_new[k] + (i - so) - (F[i] - F[so]);
-	copy_particle(src, dst, i, to);
+	shift[k] = start_new[k] - so + F[so];
 }
 
-__global__ void k_rebin_movers(ParticleArrays32 src, ParticleArrays32 dst, const uint32_t* __restrict__ mk, const uint32_t* __restrict__ mi,
-                               long long M, const int* __restrict__ start_new, const int* __restrict__ stay, const int* __restrict__ first) {
+// The destinations are written as a source index per new position (perm[to] = i), and the
+// particles then move with one coalesced gather (k_permute32): scattering the 32-byte
+// particles directly writes partial cache lines at every segment boundary.
+__global__ void k_rebin_stayers(long long n, const int* __restrict__ F, const uint32_t* __restrict__ keys, const int* __restrict__ shift,
+                                uint32_t* __restrict__ perm) {
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int fi = F[i];
+	if (F[i + 1] != fi) return;  // a mover
+	perm[i - fi + shift[keys[i]]] = (uint32_t)i;
+}
+
+__global__ void k_rebin_movers(const uint32_t* __restrict__ mk, const uint32_t* __restrict__ mi, long long M, const int* __restrict__ start_new,
+                               const int* __restrict__ stay, const int* __restrict__ first, uint32_t* __restrict__ perm) {
 	long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (m >= M) return;
 	uint32_t k = mk[m];
-	long long to = (long long)start_new[k] + stay[k] + (m - first[k]);
-	copy_particle(src, dst, mi[m], to);
+	perm[(long long)start_new[k] + stay[k] + (m - first[k])] = mi[m];
 }
 
 __global__ void k_keys32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start, uint32_t* __restrict__ keys) {
@@ -977,14 +978,12 @@ void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_
 void launch_rebin_counts(const int* start_old, const int* F, const int* arrive, long long nkeys, int* stay, int* counts, hipStream_t s) {
 	hipLaunchKernelGGL(k_rebin_counts, dim3(blocks32(nkeys + 1, 256)), dim3(256), 0, s, start_old, F, arrive, nkeys, stay, counts);
 }
-void launch_rebin_scatter(const ParticleArrays32& src, const ParticleArrays32& dst, const int* moved, const int* F, const uint32_t* keys,
-                          const int* start_old, const int* start_new, const uint32_t* mk, const uint32_t* mi, long long M,
-                          const int* stay, const int* first, hipStream_t s) {
-	if (src.count > 0)
-		hipLaunchKernelGGL(k_rebin_stayers, dim3(blocks32(src.count, 256)), dim3(256), 0, s, src, dst, moved, F, keys, start_old,
-		                   start_new);
-	if (M > 0)
-		hipLaunchKernelGGL(k_rebin_movers, dim3(blocks32(M, 256)), dim3(256), 0, s, src, dst, mk, mi, M, start_new, stay, first);
+void launch_rebin_perm(long long n, const int* F, const uint32_t* keys, const int* start_old, const int* start_new, long long nkeys,
+                       int* shift, const uint32_t* mk, const uint32_t* mi, long long M, const int* stay, const int* first, uint32_t* perm,
+                       hipStream_t s) {
+	hipLaunchKernelGGL(k_rebin_shift, dim3(blocks32(nkeys, 256)), dim3(256), 0, s, start_old, start_new, F, nkeys, shift);
+	if (n > 0) hipLaunchKernelGGL(k_rebin_stayers, dim3(blocks32(n, 256)), dim3(256), 0, s, n, F, keys, shift, perm);
+	if (M > 0) hipLaunchKernelGGL(k_rebin_movers, dim3(blocks32(M, 256)), dim3(256), 0, s, mk, mi, M, start_new, stay, first, perm);
 }
 void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
                    hipStream_t s) {

@@ -27,9 +27,10 @@ void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32
 void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, uint32_t* mk, uint32_t* mi, hipStream_t s);
 void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_t s);
 void launch_rebin_counts(const int* start_old, const int* F, const int* arrive, long long nkeys, int* stay, int* counts, hipStream_t s);
-void launch_rebin_scatter(const ParticleArrays32& src, const ParticleArrays32& dst, const int* moved, const int* F, const uint32_t* keys,
-                          const int* start_old, const int* start_new, const uint32_t* mk, const uint32_t* mi, long long M,
-                          const int* stay, const int* first, hipStream_t s);
+// perm[new position] = old index; shift: scratch of nkeys ints (overwritten)
+void launch_rebin_perm(long long n, const int* F, const uint32_t* keys, const int* start_old, const int* start_new, long long nkeys,
+                       int* shift, const uint32_t* mk, const uint32_t* mi, long long M, const int* stay, const int* first, uint32_t* perm,
+                       hipStream_t s);
 void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
                    hipStream_t s);
 void launch_permute32(const ParticleArrays32& src, const ParticleArrays32& dst, const uint32_t* perm, hipStream_t s);
