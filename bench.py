@@ -324,11 +324,14 @@ def weibel_check(args, torch, dev):
     for name, (e, el, npart) in res.items():
         lb = np.log(np.maximum(e[:, 2], 1e-300))
         win = 20
-        # from step 1: B starts at zero and step 1 carries the particle noise
-        slopes = [np.polyfit(t[k:k + win], lb[k:k + win], 1)[0] for k in range(1, len(t) - win)]
-        k = 1 + int(np.argmax(slopes))
+        # B starts at zero and the first steps only bring up the particle noise level: the
+        # search for the exponential phase starts after the first 10 % of the steps
+        k0 = max(1, len(t) // 10)
+        slopes = [np.polyfit(t[k:k + win], lb[k:k + win], 1)[0] for k in range(k0, len(t) - win)]
+        k = k0 + int(np.argmax(slopes))
         out[name] = {"ms_per_step": 1e3 * el / args.weibel, "particle_steps_per_s": npart * args.weibel / el,
-                     "magnetic_energy_growth_rate": float(slopes[k - 1]), "growth_window_steps": [k, k + win],
+                     "magnetic_energy_growth_rate": float(slopes[k - k0]), "growth_window_steps": [k, k + win],
+                     "magnetic_energy_every_10_steps": [float(v) for v in e[::10, 2]],
                      "magnetic_energy_final": float(e[-1, 2]), "total_energy_rel_change": float(np.max(np.abs(e[:, 3] - e[0, 3]) / e[0, 3]))}
     a, b = res["chosen"][0], res["fp64"][0]
     out["growth_rate_rel_difference"] = abs(out["chosen"]["magnetic_energy_growth_rate"] / out["fp64"]["magnetic_energy_growth_rate"] - 1)
