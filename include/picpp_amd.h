@@ -55,6 +55,17 @@ enum {
 	PIC_SOLVER_MIXED = 1
 };
 
+/* x boundary of the subdomain (y and z are periodic) */
+enum {
+	PIC_BOUNDARY_PERIODIC = 0,
+	/* a perfectly conducting reflecting wall at the left end (SUPER_CONDUCTOR_LEFT,
+	   PIC++/mover.cpp:248-272, fields.cpp:690-760) and a free right end held at constant
+	   fields e_right, b_right (ConstantBoundaryFieldEvaluator, boundaryFieldEvaluator.cpp:30-41);
+	   particles leaving through the right end are removed. The Maxwell unknowns then span
+	   nodes 2 .. nx+3 along x: vectors of pic_gmres_solve hold 3 x (nx+2) ny nz values. */
+	PIC_BOUNDARY_WALL = 1
+};
+
 /* orthogonalisation inside the Arnoldi process */
 enum {
 	PIC_ORTHO_MGS = 0, /* modified Gram-Schmidt, one dot + axpy per basis vector (PIC++/specialmath.cpp:295-332) */
@@ -76,6 +87,9 @@ typedef struct {
 	double gmres_tolerance; /* maxErrorLevel (PIC++/constants.h:31), 1e-8 */
 	int gmres_max_iterations; /* maxGMRESIterations (PIC++/constants.h:55), 50 */
 	int solver_precision;   /* PIC_SOLVER_* */
+	int boundary_x;         /* PIC_BOUNDARY_* */
+	double e_right[3];      /* E held at the free right boundary (PIC_BOUNDARY_WALL) */
+	double b_right[3];      /* B held at the free right boundary */
 } pic_config;
 
 typedef struct {

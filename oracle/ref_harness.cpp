@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "simulation.h"
+#include "boundaryFieldEvaluator.h"
 #include "particle.h"
 #include "vector3d.h"
 #include "matrix3d.h"
@@ -236,6 +237,21 @@ int main(int argc, char** argv) {
 		                           (ParticleTypes)sp_type[sp], x, y, z, px, py, pz, s.deltaX / 2, s.deltaY / 2,
 		                           s.deltaZ / 2, dt);
 		s.particles.push_back(q);
+	}
+	// optional trailer: x boundary (0 periodic, 1 super-conducting left wall + constant-field
+	// free right boundary, as initializeFluxFromRight sets up, PIC++/initialization.cpp:5201-5210)
+	int32_t boundary = 0;
+	if (fread(&boundary, sizeof(int32_t), 1, in) != 1) boundary = 0;
+	Vector3d e_right, b_right;
+	if (boundary == 1) {
+		e_right.x = rd<double>(in);
+		e_right.y = rd<double>(in);
+		e_right.z = rd<double>(in);
+		b_right.x = rd<double>(in);
+		b_right.y = rd<double>(in);
+		b_right.z = rd<double>(in);
+		s.boundaryConditionTypeX = SUPER_CONDUCTOR_LEFT;
+		s.rightBoundaryFieldEvaluator = new ConstantBoundaryFieldEvaluator(e_right, b_right);
 	}
 	fclose(in);
 	s.particlesNumber = (int)np;

@@ -18,6 +18,7 @@ PRECISION_FP64, PRECISION_FP32 = 0, 1
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 ORTHO_MGS, ORTHO_CGS2 = 0, 1
 SOLVER_FP64, SOLVER_MIXED = 0, 1
+BOUNDARY_PERIODIC, BOUNDARY_WALL = 0, 1
 
 ARRAYS = {"E": 0, "Etheta": 1, "Enew": 2, "B": 3, "Bnew": 4, "J": 5, "mu": 6, "divPi": 7, "rhohat": 8, "Pi": 9,
           "density": 10, "rhs": 11, "solution": 12}
@@ -36,7 +37,8 @@ class PicConfig(ctypes.Structure):
                 ("mass", ctypes.c_double * 8), ("charge", ctypes.c_double * 8), ("weight", ctypes.c_double * 8),
                 ("capacity", ctypes.c_longlong), ("precision", ctypes.c_int), ("stop_rule", ctypes.c_int),
                 ("ortho", ctypes.c_int), ("gmres_tolerance", ctypes.c_double), ("gmres_max_iterations", ctypes.c_int),
-                ("solver_precision", ctypes.c_int)]
+                ("solver_precision", ctypes.c_int), ("boundary_x", ctypes.c_int), ("e_right", ctypes.c_double * 3),
+                ("b_right", ctypes.c_double * 3)]
 
 
 class PicSolveInfo(ctypes.Structure):

@@ -81,6 +81,10 @@ struct pic_ctx {
 	std::vector<int> species_start_h;
 	int* species_start = nullptr;
 	int* status = nullptr;
+	// wall boundary: particles removed at the free right end, and the plane masks of the density
+	// array (5 per species + charge) for the mirror (negated x momentum) and the right-end fold
+	int* dead = nullptr;
+	int *dens_flip = nullptr, *dens_zero = nullptr;
 	// Krylov solver
 	double *rhs = nullptr, *muc = nullptr, *xsol = nullptr, *work = nullptr, *work2 = nullptr, *partial = nullptr, *dscal = nullptr, *ydev = nullptr;
 	std::vector<double*> V;
@@ -185,10 +189,18 @@ void check_status(pic_ctx* c) {
 	}
 }
 
-// Sort particles by key (stable radix sort: deterministic), rebuild start[].
+// Sort particles by key (stable radix sort: deterministic), rebuild start[]. Particles the push
+// removed (wall boundary: key 0xFFFFFFFF) sort to the end and are dropped; the species ranges
+// are then read back from start[].
 void sort_particles(pic_ctx* c) {
 	long long n = c->np();
 	hipStream_t s = c->stream;
+	int removed = 0;
+	if (c->g.wall && n > 0) {
+		HIP_CHECK(hipMemcpyAsync(&removed, c->dead, sizeof(int), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		HIP_CHECK(hipMemsetAsync(c->dead, 0, sizeof(int), s));
+	}
 	if (n > 0) {
 		{
 			Timed t(c, "sort_radix", (double)n * 16);
@@ -207,11 +219,26 @@ void sort_particles(pic_ctx* c) {
 			std::swap(c->P, c->Palt);
 		}
 	}
+	if (removed > 0) {
+		n -= removed;
+		c->P.count = c->Palt.count = c->fp32 ? 0 : n;
+		c->Q.count = c->Qalt.count = c->fp32 ? n : 0;
+	}
 	Timed t(c, "segments", (double)n * 4 + (double)(c->nkeys + 1) * 12);
 	HIP_CHECK(hipMemsetAsync(c->counts, 0, (c->nkeys + 1) * sizeof(int), s));
 	launch_count_keys(c->keys_sorted, n, c->counts, s);
 	size_t bytes = c->cub_temp_bytes;
 	HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start, (int)(c->nkeys + 1), s));
+	if (removed > 0) {  // species ranges after the removal: start[] at each species' first key
+		for (int q = 0; q < c->sp.count; ++q)
+			HIP_CHECK(hipMemcpyAsync(&c->species_start_h[q], c->start + (((long long)q * c->g.ncells) << 3), sizeof(int),
+			                         hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		for (int q = c->sp.count; q <= kMaxSpecies; ++q) c->species_start_h[q] = (int)n;
+		HIP_CHECK(hipMemcpyAsync(c->species_start, c->species_start_h.data(), (kMaxSpecies + 1) * sizeof(int),
+		                         hipMemcpyHostToDevice, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+	}
 }
 
 // Restores the cell-sorted order of the FP32 particles after a push without a full sort
@@ -287,6 +314,10 @@ void density(pic_ctx* c) {
 	}
 	{
 		Timed t(c, "folds", 0);
+		if (c->g.wall) {
+			launch_wall_mirror_cells(c->g, c->dens, nq, c->dens_flip, s);
+			launch_wall_right_fold(c->g, c->dens, nq, c->dens_zero, s);
+		}
 		launch_fold_cells(c->g, c->dens, nq, 4, 3, s);  // sumCellParameters{X,Y,Z}, sumCellVectorParameters{X,Y,Z}
 		launch_bulk_divide(c->g, c->dens, c->sp.count, s);
 	}
@@ -347,12 +378,19 @@ void moments(pic_ctx* c) {
 		}
 	}
 	Timed tf(c, "folds", 0);
+	if (g.wall) {  // cell-map deposits behind the wall go to their mirror cells
+		launch_wall_mirror_cells(g, c->rhohat, 1, nullptr, s);
+		launch_wall_mirror_cells(g, c->Pi, 9, nullptr, s);
+	}
 	launch_fold_cells(g, c->rhohat, 1, 4, 3, s);  // sumChargeDensityHat{X,Y,Z}
 	launch_fold_cells(g, c->Pi, 9, 3, 2, s);      // sumCellMatrixParameters{X,Y,Z}
 	launch_fold_nodes(g, c->J, 3, s);             // sumNodeVectorParameters{X,Y,Z}
 	launch_div_pressure_update_j(g, c->Pi, c->J, c->divPi, c->eta, c->cfg.dt, s);
+	// the wall node and the ghosts behind it carry no current (momentum_collecting.cpp:497-506, :638-646)
+	if (g.wall) launch_zero_node_planes(g, c->J, 0, 3, 0, 2, s);
 	launch_div_flux_update_rho(g, c->J, c->rhohat, c->cfg.dt, c->theta, s);
 	launch_fold_nodes(g, c->mu, 9, s);            // sumNodeMatrixParameters{X,Y,Z}
+	if (g.wall) launch_zero_node_planes(g, c->mu, 0, 9, 2, 2, s);
 }
 
 double operator_C(const pic_ctx* c) { return (c->theta * c->cfg.dt) * (c->theta * c->cfg.dt) * kCprime2; }
@@ -505,6 +543,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	    cfg->dx <= 0 || cfg->dy <= 0 || cfg->dz <= 0 || cfg->capacity < 0)
 		return PIC_ERR_ARGUMENT;
 	if (cfg->precision != PIC_PRECISION_FP64 && cfg->precision != PIC_PRECISION_FP32) return PIC_ERR_ARGUMENT;
+	if (cfg->boundary_x != PIC_BOUNDARY_PERIODIC && cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
 	pic_ctx* c = new pic_ctx();
 	try {
 		c->cfg = *cfg;
@@ -523,7 +562,13 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		g.volume = d[0] * d[1] * d[2];
 		g.nnodes = (long long)(g.N[0] + 1) * (g.N[1] + 1) * (g.N[2] + 1);
 		g.ncells = (long long)g.N[0] * g.N[1] * g.N[2];
-		g.nown = (long long)n[0] * n[1] * n[2];
+		g.wall = cfg->boundary_x == PIC_BOUNDARY_WALL ? 1 : 0;
+		g.nu0 = n[0] + 2 * g.wall;
+		g.nown = (long long)g.nu0 * n[1] * n[2];
+		for (int a = 0; a < 3; ++a) {
+			g.e_right[a] = cfg->e_right[a];
+			g.b_right[a] = cfg->b_right[a];
+		}
 		c->sp.count = cfg->nspecies;
 		for (int s = 0; s < kMaxSpecies; ++s) {
 			c->sp.mass[s] = s < cfg->nspecies ? cfg->mass[s] : 1.0;
@@ -620,6 +665,20 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->species_start_h.assign(kMaxSpecies + 1, 0);
 		HIP_CHECK(hipMemset(c->species_start, 0, (kMaxSpecies + 1) * sizeof(int)));
 		c->status = dalloc<int>(1);
+		c->dead = dalloc<int>(1);
+		HIP_CHECK(hipMemset(c->dead, 0, sizeof(int)));
+		{
+			int nq = 5 * cfg->nspecies + 1;
+			std::vector<int> flip(nq, 0), zero(nq, 1);
+			for (int q = 0; q < 5 * cfg->nspecies; ++q) {
+				flip[q] = q % 5 == 2;  // x momentum sum of each species
+				zero[q] = q % 5 <= 1;  // concentration, energy (charge is the last plane)
+			}
+			c->dens_flip = dalloc<int>(nq);
+			c->dens_zero = dalloc<int>(nq);
+			HIP_CHECK(hipMemcpy(c->dens_flip, flip.data(), nq * sizeof(int), hipMemcpyHostToDevice));
+			HIP_CHECK(hipMemcpy(c->dens_zero, zero.data(), nq * sizeof(int), hipMemcpyHostToDevice));
+		}
 		HIP_CHECK(hipMemset(c->status, 0, sizeof(int)));
 		// Krylov storage: up to max_iterations basis vectors
 		c->max_basis = std::min(120, std::max(2, c->cfg.gmres_max_iterations));
@@ -671,7 +730,7 @@ void pic_destroy(pic_ctx* c) {
 	                (void*)c->divPi, (void*)c->rhohat, (void*)c->Pi, (void*)c->dens, (void*)c->node_scratch,
 	                (void*)c->rho_scratch, (void*)c->pi_scratch, (void*)c->dens_scratch, (void*)c->keys, (void*)c->keys_sorted,
 	                (void*)c->perm, (void*)c->perm_sorted, (void*)c->start, (void*)c->counts, c->cub_temp,
-	                (void*)c->species_start, (void*)c->status, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
+	                (void*)c->species_start, (void*)c->status, (void*)c->dead, (void*)c->dens_flip, (void*)c->dens_zero, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
 	                (void*)c->Vdev, (void*)c->partial, (void*)c->dscal, (void*)c->ydev, (void*)c->epartial, (void*)c->eout})
 		f(p);
 	for (ParticleArrays* P : {&c->P, &c->Palt}) {
@@ -1202,14 +1261,14 @@ int pic_push(pic_ctx* c) {
 			}
 			Timed t(c, "push", (double)c->np() * 60 + ((double)g.nnodes * 6 + (double)g.ncells * 3) * 4);
 			launch_push32(g, c->sp, c->Q, c->species_start, c->E32, c->Et32, c->B32, (float)c->cfg.dt, (float)c->eta, c->keys,
-			              c->moved, c->status, c->stream);
+			              c->moved, c->status, c->dead, c->stream);
 		} else {
 			Timed t(c, "push", (double)c->P.count * 100 + ((double)c->g.nnodes * 6 + (double)c->g.ncells * 3) * 8);
 			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->status,
-			            c->stream);
+			            c->dead, c->stream);
 		}
 		check_status(c);
-		if (c->fp32)
+		if (c->fp32 && !c->g.wall)  // the rebin keeps every particle; removals take the full sort
 			rebin_particles32(c);
 		else
 			sort_particles(c);
@@ -1326,7 +1385,8 @@ int pic_precond_create(pic_ctx* c) {
 		if (c->cfg.solver_precision != PIC_SOLVER_MIXED)
 			throw DeviceError{PIC_ERR_ARGUMENT, "the learned preconditioner runs inside the mixed-precision solver"};
 		try {
-			c->net.reset(new UNet(c->g.n));
+			const int dims[3] = {c->g.nu0, c->g.n[1], c->g.n[2]};  // the solver vectors' grid
+			c->net.reset(new UNet(dims));
 		} catch (const std::exception& e) {
 			throw DeviceError{PIC_ERR_ARGUMENT, e.what()};
 		}

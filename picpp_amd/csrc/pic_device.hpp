@@ -22,7 +22,12 @@ struct Grid {
 	double x0[3];
 	double L[3];
 	double volume;
+	// nown: unknowns of the Maxwell solve per component, nu0 * n1 * n2 owned-node columns
 	long long nnodes, ncells, nown;
+	// x boundary: 0 periodic (nu0 = n0), 1 reflecting wall left + constant-field free right end
+	// (nu0 = n0 + 2: nodes 2 .. n0+3, the wall node 2 and the boundary nodes n0+2, n0+3)
+	int wall, nu0;
+	double e_right[3], b_right[3];
 };
 
 struct SpeciesTable {

@@ -45,7 +45,14 @@ void launch_update_b(const Grid& g, const double* Etheta, const double* B, doubl
 
 // particles
 void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const double* E,
-                 const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, hipStream_t s);
+                 const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, int* dead,
+                 hipStream_t s);
+// wall boundary helpers: cell planes 0, 1 folded into their mirrors 3, 2 (flip[c] negates
+// component c), the right-end fold of cells N-2, N-1 into N-3 (zero_mask[c] zeroes them), and
+// zeroing of node planes i0..i1 for components [c0, c1)
+void launch_wall_mirror_cells(const Grid& g, double* f, int ncomp, const int* flip, hipStream_t s);
+void launch_wall_right_fold(const Grid& g, double* f, int ncomp, const int* zero_mask, hipStream_t s);
+void launch_zero_node_planes(const Grid& g, double* f, int c0, int c1, int i0, int i1, hipStream_t s);
 void launch_keys(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, uint32_t* keys,
                  int* status, hipStream_t s);
 void launch_permute(const ParticleArrays& src, const ParticleArrays& dst, const uint32_t* perm, hipStream_t s);

@@ -559,7 +559,7 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start,
                                                 const float4* __restrict__ E, const float4* __restrict__ Et,
                                                 const float4* __restrict__ B, float dt, float eta, uint32_t* __restrict__ keys,
-                                                int* __restrict__ moved, int* __restrict__ status) {
+                                                int* __restrict__ moved, int* __restrict__ status, int* __restrict__ dead) {
 	long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (idx >= P.count) return;
 	int i = (int)idx;
@@ -595,11 +595,31 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 	float nu2 = nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2];
 	float nig = nu2 < 0.0001f ? 1.0f : 1.0f / sqrtf(1.0f + nu2);
 	float mid[3], xh[3];
+	const float vpred[3] = {nu[0] * nig, nu[1] * nig, nu[2] * nig};  // the predictor velocity
 #pragma unroll
 	for (int c = 0; c < 3; ++c) {
 		mid[c] = v[c] * (1.0f - eta) + nu[c] * nig * eta;
 		xh[c] = f[c] + mid[c] * (eta * dt) * inv_d[c];
 	}
+	// Wall boundary (x node 2): a predictor or Picard position behind the wall ends the push
+	// there, reflected with |v_x| of the predictor velocity and the rest of the step
+	// (PIC++/mover.cpp:248-258, :315-331, Particle::setMomentumByV). Positions are in cell units
+	// relative to the start cell, so the wall sits at 2 - cc[0].
+	const float rest = (1.0f - eta) * dt;
+	float pos[3], unew[3];
+	bool reflected = false;
+	auto reflect_at_wall = [&](const float xc[3]) {
+		pos[0] = (float)(4 - 2 * cc[0]) - xc[0] + fabsf(mid[0] * rest) * inv_d[0];
+		pos[1] = xc[1] + mid[1] * rest * inv_d[1];
+		pos[2] = xc[2] + mid[2] * rest * inv_d[2];
+		float vr[3] = {fabsf(vpred[0]), vpred[1], vpred[2]};
+		float v2 = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2];
+		float gr = v2 < 0.0001f ? 1.0f : 1.0f / sqrtf(1.0f - v2);
+#pragma unroll
+		for (int c = 0; c < 3; ++c) unew[c] = vr[c] * gr;
+		reflected = true;
+	};
+	if (g.wall && xh[0] < (float)(2 - cc[0])) reflect_at_wall(xh);
 	float vhat[3];
 #pragma unroll
 	for (int r = 0; r < 3; ++r) vhat[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
@@ -608,7 +628,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 	float err = sqrtf(du[0] * du[0] + du[1] * du[1] + du[2] * du[2]);
 	float prevX[3] = {xh[0], xh[1], xh[2]};
 	float cerr = 1.0f;  // in cell units (coordinatesError = deltaX)
-	for (int it = 0; it < 3 && err > 1e-14f * unorm && cerr > 1e-8f; ++it) {
+	for (int it = 0; !reflected && it < 3 && err > 1e-14f * unorm && cerr > 1e-8f; ++it) {
 		float prevU[3] = {nu[0], nu[1], nu[2]};
 		float rE[3];
 #pragma unroll
@@ -618,6 +638,10 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		for (int c = 0; c < 3; ++c) {
 			mid[c] = vhat[c] + rE[c] * beta;
 			nx[c] = f[c] + mid[c] * (eta * dt) * inv_d[c];
+		}
+		if (g.wall && nx[0] < (float)(2 - cc[0])) {
+			reflect_at_wall(nx);
+			break;
 		}
 		ok &= maps32(g, cc, nx, mp);
 		gather_node32(g, Et, mp, Ef);
@@ -632,11 +656,28 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 #pragma unroll
 		for (int c = 0; c < 3; ++c) prevX[c] = nx[c];
 	}
-	// x^{n+1} = x^n + mid dt, carried into the integer cell; one-rank periodic wrap
+	if (!reflected) {
+#pragma unroll
+		for (int c = 0; c < 3; ++c) {
+			pos[c] = f[c] + mid[c] * dt * inv_d[c];
+			unew[c] = nu[c];
+		}
+		if (g.wall && pos[0] < (float)(2 - cc[0])) {  // final position behind the wall (mover.cpp:435-441)
+			pos[0] = (float)(4 - 2 * cc[0]) - pos[0];
+			unew[0] = -unew[0];
+		}
+	}
+	if (g.wall && pos[0] > (float)(g.n[0] + 2 - cc[0])) {  // out through the free right end: removed
+		keys[i] = 0xFFFFFFFFu;
+		moved[i] = 1;
+		atomicAdd(dead, 1);
+		return;
+	}
+	// x^{n+1} = x^n + mid dt, carried into the integer cell; one-rank periodic wrap (x too unless walled)
 	int nc[3];
 #pragma unroll
 	for (int c = 0; c < 3; ++c) {
-		float x = f[c] + mid[c] * dt * inv_d[c];
+		float x = pos[c];
 		float fl = floorf(x);
 		x -= fl;
 		if (x >= 1.0f) {  // rounding of x - floor(x) up to 1
@@ -644,9 +685,10 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 			fl += 1.0f;
 		}
 		int ci = cc[c] + (int)fl;
-		if (ci < 2) ci += g.n[c];
-		if (ci > g.n[c] + 1) ci -= g.n[c];
-		if (ci < 2 || ci > g.n[c] + 1) ok = false;
+		const bool walled = g.wall && c == 0;
+		if (!walled && ci < 2) ci += g.n[c];
+		if (!walled && ci > g.n[c] + 1) ci -= g.n[c];
+		if (ci < 2 || ci > g.n[c] + 1 + (walled ? 1 : 0)) ok = false;
 		nc[c] = ci;
 		f[c] = x;
 	}
@@ -656,7 +698,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 #pragma unroll
 	for (int c = 0; c < 3; ++c) {
 		P.off[c][i] = f[c];
-		P.u[c][i] = nu[c];
+		P.u[c][i] = unew[c];
 	}
 	const uint32_t key = (uint32_t)((((long long)s * g.ncells + cell) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) |
 	                                (f[2] >= 0.5f));
@@ -961,11 +1003,12 @@ void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArray
 	hipLaunchKernelGGL(k_density32, dim3(pt.blocks), dim3(T * G), 0, s, g, sp, P, start, G, pt.Tj, pt.Tk, scratch);
 }
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
-                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, hipStream_t s) {
+                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, int* dead,
+                   hipStream_t s) {
 	if (P.count == 0) return;
 	hipLaunchKernelGGL(k_push32, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start,
 	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(Et), reinterpret_cast<const float4*>(B), dt,
-	                   eta, keys, moved, status);
+	                   eta, keys, moved, status, dead);
 }
 void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, uint32_t* mk, uint32_t* mi, hipStream_t s) {
 	if (n == 0) return;

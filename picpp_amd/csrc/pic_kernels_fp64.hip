@@ -287,6 +287,8 @@ __global__ void k_fold(Grid g, double* __restrict__ f, int ncomp, int axis, int 
 
 // Periodic ghost fill of node (exchangeGeneralEfield, fields_exchange.cpp:158-170) or cell
 // (exchangeGeneralBfield, :629-637) arrays: every ghost takes its owned periodic image.
+// With the wall boundary the x ghosts behind the wall copy index 2 (fields_exchange.cpp:171-183,
+// :639-651) and the ghosts beyond the free right end keep their values (y and z stay periodic).
 __global__ void k_ghost_fill(Grid g, double* __restrict__ f, int ncomp, int nodes) {
 	long long len = nodes ? g.nnodes : g.ncells;
 	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,9 +299,59 @@ __global__ void k_ghost_fill(Grid g, double* __restrict__ f, int ncomp, int node
 	int i = (int)(t / ((long long)e1 * e2));
 	bool ghost = i < 2 || j < 2 || k < 2 || i > g.n[0] + 1 || j > g.n[1] + 1 || k > g.n[2] + 1;
 	if (!ghost) return;
-	int si = wrap_owned(i, g.n[0]) + 2, sj = wrap_owned(j, g.n[1]) + 2, sk = wrap_owned(k, g.n[2]) + 2;
+	int si = g.wall ? (i < 2 ? 2 : i) : wrap_owned(i, g.n[0]) + 2;
+	int sj = wrap_owned(j, g.n[1]) + 2, sk = wrap_owned(k, g.n[2]) + 2;
+	if (si == i && sj == j && sk == k) return;
 	long long src = nodes ? node_id(g, si, sj, sk) : cell_id(g, si, sj, sk);
 	for (int c = 0; c < ncomp; ++c) f[c * len + t] = f[c * len + src];
+}
+
+// ------------------------------------------------------------------ wall boundary (PIC_BOUNDARY_WALL)
+// Cell-map deposits that land behind the wall go to their mirror cells: cell c <= 1 -> 3 - c
+// (momentum_collecting.cpp:557-567 for rhohat, :570-574 for the pressure tensor, :885-898
+// for the post-push density). One thread per (mirror pair, y-z column, component); `flip`
+// marks components that change sign under the reflection (the x momentum sums).
+__global__ void k_wall_mirror_cells(Grid g, double* __restrict__ f, int ncomp, const int* __restrict__ flip) {
+	long long cols = (long long)g.N[1] * g.N[2];
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= cols * ncomp) return;
+	int comp = (int)(t / cols);
+	long long col = t % cols;
+	long long plane = cols;
+	double sgn = (flip && flip[comp]) ? -1.0 : 1.0;
+	double* fc = f + (long long)comp * g.ncells;
+	for (int c = 1; c >= 0; --c) {
+		fc[(3 - c) * plane + col] += sgn * fc[c * plane + col];
+		fc[c * plane + col] = 0.0;
+	}
+}
+
+// Right end of the post-push density (sumCellParametersX / sumCellVectorParametersX, non-periodic
+// branch, momentum_exchange.cpp:960-985, :1340-1350): cells N-2, N-1 add into N-3; the zeroed
+// planes are those with zero_mask set (charge, concentration, energy; not the momentum sums).
+__global__ void k_wall_right_fold(Grid g, double* __restrict__ f, int ncomp, const int* __restrict__ zero_mask) {
+	long long cols = (long long)g.N[1] * g.N[2];
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= cols * ncomp) return;
+	int comp = (int)(t / cols);
+	long long col = t % cols;
+	double* fc = f + (long long)comp * g.ncells;
+	const int N0 = g.N[0];
+	for (int i = 0; i <= 1; ++i) {
+		fc[(long long)(N0 - 3) * cols + col] += fc[(long long)(N0 - 2 + i) * cols + col];
+		if (zero_mask[comp]) fc[(long long)(N0 - 2 + i) * cols + col] = 0.0;
+	}
+}
+
+// Zeroes components [c0, c1) of a node array on the x planes i0 .. i1 (all y, z).
+__global__ void k_zero_node_planes(Grid g, double* __restrict__ f, int c0, int c1, int i0, int i1) {
+	long long plane = (long long)(g.N[1] + 1) * (g.N[2] + 1);
+	long long n = plane * (i1 - i0 + 1) * (c1 - c0);
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	int comp = c0 + (int)(t / (plane * (i1 - i0 + 1)));
+	long long r = t % (plane * (i1 - i0 + 1));
+	f[(long long)comp * g.nnodes + (long long)i0 * plane + r] = 0.0;
 }
 
 // ------------------------------------------------------------------ moment post-processing
@@ -354,6 +406,10 @@ __global__ void k_div_flux_update_rho(Grid g, const double* __restrict__ J, doub
 // ------------------------------------------------------------------ Maxwell system
 // Right part of createInternalEquation (fields.cpp:2635-2715) with evaluateRotB (:2799-2860)
 // and evaluateGradDensity (:3229-3259), for owned nodes, into the compact GMRES layout.
+// With the wall boundary the unknowns run over x nodes 2 .. n0+3: node 2 carries the
+// super-conducting wall rows (createSuperConductorLeftEquation, fields.cpp:690-767: the x row
+// E_x(2) - E_x(3) = -pi (sum of the 4 adjacent rhohat), tangential rows E = 0), nodes n0+2 and
+// n0+3 the free right rows (createFreeRightEquation, :769-781: E = e_right).
 __global__ void k_rhs(Grid g, const double* __restrict__ E, const double* __restrict__ B, const double* __restrict__ J,
                       const double* __restrict__ rho, double theta, double dt, double* __restrict__ rhs) {
 	long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -361,6 +417,17 @@ __global__ void k_rhs(Grid g, const double* __restrict__ E, const double* __rest
 	int k = (int)(o % g.n[2]) + 2;
 	int j = (int)((o / g.n[2]) % g.n[1]) + 2;
 	int i = (int)(o / ((long long)g.n[2] * g.n[1])) + 2;
+	if (g.wall && i == 2) {
+		auto r = [&](int a, int bb, int c) { return rho[cell_id(g, a, bb, c)]; };
+		rhs[o] = -kPi * (r(i, j, k) + r(i, j - 1, k) + r(i, j, k - 1) + r(i, j - 1, k - 1));
+		rhs[g.nown + o] = 0.0;
+		rhs[2 * g.nown + o] = 0.0;
+		return;
+	}
+	if (g.wall && i >= g.n[0] + 2) {
+		for (int a = 0; a < 3; ++a) rhs[a * g.nown + o] = g.e_right[a];
+		return;
+	}
 	auto b = [&](int a, int bb, int c, int l) { return B[l * g.ncells + cell_id(g, a, bb, c)]; };
 	double rX[3], lX[3], rY[3], lY[3], rZ[3], lZ[3];
 	for (int l = 0; l < 3; ++l) {
@@ -391,7 +458,7 @@ __global__ void k_rhs(Grid g, const double* __restrict__ E, const double* __rest
 	}
 }
 
-// Owned-node copy of the folded dielectric tensor (9 planes, compact layout).
+// Copy of the folded dielectric tensor at the unknowns' nodes (9 planes, compact layout).
 __global__ void k_pack_mu(Grid g, const double* __restrict__ mu, double* __restrict__ muc) {
 	long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (o >= g.nown) return;
@@ -420,10 +487,18 @@ __global__ void k_apply_operator(Grid g, const double* __restrict__ muc, const d
 	int k = (int)(o % g.n[2]);
 	int j = (int)((o / g.n[2]) % g.n[1]);
 	int i = (int)(o / ((long long)g.n[2] * g.n[1]));
+	if (g.wall && (i == 0 || i >= g.n[0])) {
+		// wall rows (x: E_x(2) - E_x(3), tangential: identity) and free right rows (identity)
+		const long long L = g.nown;
+		out[o] = i == 0 ? v[o] + -1.0 * v[o + (long long)g.n[1] * g.n[2]] : v[o];
+		out[L + o] = v[L + o];
+		out[2 * L + o] = v[2 * L + o];
+		return;
+	}
 	Neigh nb;
-	nb.i[0] = i == 0 ? g.n[0] - 1 : i - 1;
+	nb.i[0] = i == 0 ? g.nu0 - 1 : i - 1;
 	nb.i[1] = i;
-	nb.i[2] = i == g.n[0] - 1 ? 0 : i + 1;
+	nb.i[2] = i == g.nu0 - 1 ? 0 : i + 1;
 	nb.j[0] = j == 0 ? g.n[1] - 1 : j - 1;
 	nb.j[1] = j;
 	nb.j[2] = j == g.n[1] - 1 ? 0 : j + 1;
@@ -591,6 +666,18 @@ __global__ void k_unpack_e(Grid g, const double* __restrict__ x, const double* _
 	int k = (int)(nid % (g.N[2] + 1));
 	int j = (int)((nid / (g.N[2] + 1)) % (g.N[1] + 1));
 	int i = (int)(nid / ((long long)(g.N[2] + 1) * (g.N[1] + 1)));
+	if (g.wall) {
+		// evaluateElectricField with the wall (fields.cpp:256-339): E^{n+theta} from the solution on
+		// nodes 2 .. N-1, the ghosts behind the wall copy node 2 (exchangeEfield), node N holds e_right
+		long long o = ((long long)((i < 2 ? 2 : i) - 2) * g.n[1] + wrap_owned(j, g.n[1])) * g.n[2] + wrap_owned(k, g.n[2]);
+		long long src = i < 2 ? node_id(g, 2, j, k) : nid;
+		for (int c = 0; c < 3; ++c) {
+			double et = i == g.N[0] ? g.e_right[c] : x[c * g.nown + o];
+			Etheta[c * g.nnodes + nid] = et;
+			Enew[c * g.nnodes + nid] = i == g.N[0] ? g.e_right[c] : (et - E[c * g.nnodes + src] * (1 - theta)) / theta;
+		}
+		return;
+	}
 	long long o = own_id(g, wrap_owned(i, g.n[0]), wrap_owned(j, g.n[1]), wrap_owned(k, g.n[2]));
 	for (int c = 0; c < 3; ++c) {
 		double et = x[c * g.nown + o];
@@ -607,6 +694,11 @@ __global__ void k_update_b(Grid g, const double* __restrict__ Et, const double* 
 	int k = (int)(c % g.N[2]);
 	int j = (int)((c / g.N[2]) % g.N[1]);
 	int i = (int)(c / ((long long)g.N[2] * g.N[1]));
+	if (g.wall && i >= g.N[0] - 2) {  // the free right end holds b_right (fields.cpp:2739-2743)
+		for (int l = 0; l < 3; ++l) Bnew[l * g.ncells + c] = g.b_right[l];
+		return;
+	}
+	if (g.wall && i < 2) i = 2;  // behind the wall: a copy of cell 2 (fields.cpp:2747-2757)
 	auto e = [&](int a, int b, int cc, int l) { return Et[l * g.nnodes + node_id(g, a, b, cc)]; };
 	double rX[3], lX[3], rY[3], lY[3], rZ[3], lZ[3];
 	for (int l = 0; l < 3; ++l) {
@@ -621,7 +713,8 @@ __global__ void k_update_b(Grid g, const double* __restrict__ Et, const double* 
 	r[0] = (rY[2] - lY[2]) / g.d[1] - (rZ[1] - lZ[1]) / g.d[2];
 	r[1] = (rZ[0] - lZ[0]) / g.d[2] - (rX[2] - lX[2]) / g.d[0];
 	r[2] = (rX[1] - lX[1]) / g.d[0] - (rY[0] - lY[0]) / g.d[1];
-	for (int l = 0; l < 3; ++l) Bnew[l * g.ncells + c] = B[l * g.ncells + c] - r[l] * (kCprime * dt);
+	const long long self = cell_id(g, i, j, k);
+	for (int l = 0; l < 3; ++l) Bnew[l * g.ncells + c] = B[l * g.ncells + self] - r[l] * (kCprime * dt);
 }
 
 // ------------------------------------------------------------------ mover
@@ -652,9 +745,12 @@ __device__ inline double dnorm(const double a[3], const double b[3]) {
 // exactly as evaluateParticlesRotationTensor computed it. One thread per particle.
 __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start, const double* __restrict__ E,
                        const double* __restrict__ Etheta, const double* __restrict__ B, double dt, double eta,
-                       uint32_t* __restrict__ keys, int* __restrict__ status) {
+                       uint32_t* __restrict__ keys, int* __restrict__ status, int* __restrict__ dead) {
 	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= P.count) return;
+	const double xw = g.x0[0] + 2 * g.d[0];  // the wall, xgrid[2]
+	const double rest = (1.0 - eta) * dt;
+	bool reflected = false;
 	int s = 0;
 	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
 	double m = sp.mass[s], q = sp.charge[s];
@@ -682,6 +778,24 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 	double mid[3];
 	for (int c = 0; c < 3; ++c) mid[c] = v[c] * (1 - eta) + newV[c] * eta;
 	for (int c = 0; c < 3; ++c) x[c] += mid[c] * etaDt;
+	// A predictor or Picard position behind the wall ends the push there: the particle is
+	// reflected with |v_x| of the predictor velocity and the rest of the step (mover.cpp:248-258,
+	// :315-331; Particle::setMomentumByV, particle.cpp:337-364).
+	auto reflect_at_wall = [&]() {
+		x[0] = 2 * xw - x[0] + fabs(mid[0] * rest);
+		x[1] = x[1] + mid[1] * rest;
+		x[2] = x[2] + mid[2] * rest;
+		double vr[3] = {fabs(newV[0]), newV[1], newV[2]};
+		double v2 = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2];
+		if (v2 < 0.0001) {
+			for (int c = 0; c < 3; ++c) p[c] = vr[c] * m;
+		} else {
+			double gr = 1 / sqrt(1 - v2);
+			for (int c = 0; c < 3; ++c) p[c] = vr[c] * (m * gr);
+		}
+		reflected = true;
+	};
+	if (g.wall && x[0] < xw) reflect_at_wall();
 	double vhat[3];
 	for (int c = 0; c < 3; ++c) vhat[c] = (a[c * 3 + 0] * gm) * v[0] + (a[c * 3 + 1] * gm) * v[1] + (a[c * 3 + 2] * gm) * v[2];
 	double prevP[3] = {p[0], p[1], p[2]};
@@ -691,7 +805,7 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 	double newX[3] = {x[0], x[1], x[2]};
 	double cerr = g.d[0];
 	int it = 0;
-	while (err > 1e-14 * pnorm && cerr > 1e-8 * g.d[0] && it < 3) {
+	while (!reflected && err > 1e-14 * pnorm && cerr > 1e-8 * g.d[0] && it < 3) {
 		++it;
 		for (int c = 0; c < 3; ++c) {
 			prevP[c] = newP[c];
@@ -702,6 +816,10 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 		for (int c = 0; c < 3; ++c) mid[c] = vhat[c] + rE[c] * beta;
 		for (int c = 0; c < 3; ++c) x[c] = oldX[c] + (mid[c] * etaDt);
 		for (int c = 0; c < 3; ++c) newX[c] = x[c];
+		if (g.wall && x[0] < xw) {
+			reflect_at_wall();
+			break;
+		}
 		if (!maps3(g, x, mp)) atomicOr(status, 8);
 		gather_node(g, Etheta, mp, Ef);
 		gather_cell(g, B, mp, Bf);
@@ -711,12 +829,32 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 		err = dnorm(prevP, newP);
 		cerr = dnorm(prevX, newX);
 	}
-	for (int c = 0; c < 3; ++c) {
-		p[c] = newP[c];
-		x[c] = oldX[c] + mid[c] * dt;
+	if (!reflected) {
+		for (int c = 0; c < 3; ++c) {
+			p[c] = newP[c];
+			x[c] = oldX[c] + mid[c] * dt;
+		}
+		if (g.wall && x[0] < xw) {  // final position behind the wall (mover.cpp:435-441)
+			x[0] = 2 * xw - x[0];
+			p[0] = -p[0];
+		}
+	}
+	if (g.wall) {
+		// leaving through the free right end removes the particle (mover.cpp:447-458, exchangeParticles
+		// keeps no right escapes without periodicity, :1007-1021)
+		const double xr = g.x0[0] + (g.N[0] - 2) * g.d[0];
+		if (x[0] > xr) {
+			for (int c = 0; c < 3; ++c) {
+				P.x[c][i] = x[c];
+				P.p[c][i] = p[c];
+			}
+			keys[i] = 0xFFFFFFFFu;
+			atomicAdd(dead, 1);
+			return;
+		}
 	}
 	// one-rank periodic wrap (mover.cpp:435-490 classification, :1007-1021 shift)
-	for (int c = 0; c < 3; ++c) {
+	for (int c = g.wall ? 1 : 0; c < 3; ++c) {
 		double lo = g.x0[c] + 2 * g.d[c];
 		double hi = g.x0[c] + (g.N[c] - 1 - 1) * g.d[c];
 		if (x[c] < lo)
@@ -879,10 +1017,14 @@ __global__ void __launch_bounds__(256) k_field_energy(Grid g, const double* __re
                                                       double* __restrict__ partial) {
 	__shared__ double red[5][256];
 	double acc[5] = {0, 0, 0, 0, 0};
-	for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < g.nown; o += (long long)gridDim.x * blockDim.x) {
+	// x range of updateEnergy (simulation.cpp:1522-1530): owned nodes and cells 2 .. n0+1, or
+	// 0 .. N-2 with the wall boundary (its ghost layers included)
+	const int nx = g.wall ? g.N[0] - 1 : g.n[0], x_lo = g.wall ? 0 : 2;
+	const long long cols = (long long)nx * g.n[1] * g.n[2];
+	for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < cols; o += (long long)gridDim.x * blockDim.x) {
 		int k = (int)(o % g.n[2]) + 2;
 		int j = (int)((o / g.n[2]) % g.n[1]) + 2;
-		int i = (int)(o / ((long long)g.n[2] * g.n[1])) + 2;
+		int i = (int)(o / ((long long)g.n[2] * g.n[1])) + x_lo;
 		long long nid = node_id(g, i, j, k), cid = cell_id(g, i, j, k);
 		double e[3], b[3], ea[3];
 		for (int c = 0; c < 3; ++c) {
@@ -1008,14 +1150,14 @@ void launch_pressure_reduce(const Grid& g, const double* pi_scratch, double* Pi,
 	hipLaunchKernelGGL(k_pressure_reduce, dim3(g.N[0] * 4), dim3(256), 0, s, g, pi_scratch, Pi);
 }
 void launch_fold_nodes(const Grid& g, double* f, int ncomp, hipStream_t s) {
-	for (int axis = 0; axis < 3; ++axis) {
+	for (int axis = g.wall ? 1 : 0; axis < 3; ++axis) {  // no x fold behind a wall (momentum_exchange.cpp:173)
 		int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
 		long long total = (long long)(g.N[o1] + 1) * (g.N[o2] + 1) * 5 * ncomp;
 		hipLaunchKernelGGL(k_fold, dim3(blocks(total, 256)), dim3(256), 0, s, g, f, ncomp, axis, 1, 5, g.N[axis], 4);
 	}
 }
 void launch_fold_cells(const Grid& g, double* f, int ncomp, int pairs, int lo0, hipStream_t s) {
-	for (int axis = 0; axis < 3; ++axis) {
+	for (int axis = g.wall ? 1 : 0; axis < 3; ++axis) {  // no x fold behind a wall (momentum_exchange.cpp:173)
 		int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
 		long long total = (long long)g.N[o1] * g.N[o2] * pairs * ncomp;
 		hipLaunchKernelGGL(k_fold, dim3(blocks(total, 256)), dim3(256), 0, s, g, f, ncomp, axis, 0, pairs, g.N[axis] - 1, lo0);
@@ -1069,10 +1211,23 @@ void launch_update_b(const Grid& g, const double* Etheta, const double* B, doubl
 	hipLaunchKernelGGL(k_update_b, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, Etheta, B, dt, Bnew);
 }
 void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const double* E,
-                 const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, hipStream_t s) {
+                 const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, int* dead,
+                 hipStream_t s) {
 	if (P.count == 0) return;
 	hipLaunchKernelGGL(k_push, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, E, Etheta, B, dt, eta, keys,
-	                   status);
+	                   status, dead);
+}
+void launch_wall_mirror_cells(const Grid& g, double* f, int ncomp, const int* flip, hipStream_t s) {
+	long long n = (long long)g.N[1] * g.N[2] * ncomp;
+	hipLaunchKernelGGL(k_wall_mirror_cells, dim3(blocks(n, 256)), dim3(256), 0, s, g, f, ncomp, flip);
+}
+void launch_wall_right_fold(const Grid& g, double* f, int ncomp, const int* zero_mask, hipStream_t s) {
+	long long n = (long long)g.N[1] * g.N[2] * ncomp;
+	hipLaunchKernelGGL(k_wall_right_fold, dim3(blocks(n, 256)), dim3(256), 0, s, g, f, ncomp, zero_mask);
+}
+void launch_zero_node_planes(const Grid& g, double* f, int c0, int c1, int i0, int i1, hipStream_t s) {
+	long long n = (long long)(g.N[1] + 1) * (g.N[2] + 1) * (i1 - i0 + 1) * (c1 - c0);
+	hipLaunchKernelGGL(k_zero_node_planes, dim3(blocks(n, 256)), dim3(256), 0, s, g, f, c0, c1, i0, i1);
 }
 void launch_keys(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, uint32_t* keys,
                  int* status, hipStream_t s) {

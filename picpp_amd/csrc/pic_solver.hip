@@ -8,6 +8,8 @@
 //   (A E)_c = E_c - w_c - C lap_h E_c + C d2_c w_c
 //             + C / (4 dc dd) sum_{d != c} [w_d(+c+d) + w_d(-c-d) - w_d(-c+d) - w_d(+c-d)],
 // C = (c' theta dt)^2, lap_h the 7-point Laplacian, d2_c the second difference along c.
+// With the wall boundary the x range of the unknowns is nodes 2 .. n0+3 and the first and the
+// last two x planes carry the boundary rows; the interior rows then never wrap along x.
 #include "pic_device.hpp"
 #include "pic_solver.hpp"
 
@@ -27,10 +29,19 @@ template <class T>
 __global__ void __launch_bounds__(256) k_stencil(Grid g, const T* __restrict__ v, const T* __restrict__ w, T C, T* __restrict__ out) {
 	long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (o >= g.nown) return;
-	const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+	const int n0 = g.nu0, n1 = g.n[1], n2 = g.n[2];
 	int k = (int)(o % n2);
 	int j = (int)((o / n2) % n1);
 	int i = (int)(o / ((long long)n2 * n1));
+	if (g.wall && (i == 0 || i >= g.n[0])) {
+		// wall rows (x: E_x(2) - E_x(3), tangential: identity) and free right rows (identity),
+		// PIC++/fields.cpp:690-781
+		const long long L = g.nown;
+		out[o] = i == 0 ? v[o] + (T)(-1.0) * v[o + (long long)n1 * n2] : v[o];
+		out[L + o] = v[L + o];
+		out[2 * L + o] = v[2 * L + o];
+		return;
+	}
 	int I[3] = {i == 0 ? n0 - 1 : i - 1, i, i == n0 - 1 ? 0 : i + 1};
 	int Jj[3] = {j == 0 ? n1 - 1 : j - 1, j, j == n1 - 1 ? 0 : j + 1};
 	int K[3] = {k == 0 ? n2 - 1 : k - 1, k, k == n2 - 1 ? 0 : k + 1};

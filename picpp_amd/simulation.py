@@ -54,6 +54,10 @@ class Simulation:
         cfg.gmres_tolerance = workload.gmres_tolerance if gmres_tolerance is None else gmres_tolerance
         cfg.gmres_max_iterations = gmres_max_iterations
         cfg.solver_precision = SOLVERS[solver]
+        cfg.boundary_x = _lib.BOUNDARY_WALL if workload.boundary == "wall" else _lib.BOUNDARY_PERIODIC
+        for a in range(3):
+            cfg.e_right[a] = workload.e_right[a]
+            cfg.b_right[a] = workload.b_right[a]
         self.cfg = cfg
         h = ctypes.c_void_p()
         _lib.check(self.lib.pic_create(ctypes.byref(cfg), ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)),
@@ -111,7 +115,8 @@ class Simulation:
         cells = N[0] * N[1] * N[2]
         own = int(np.prod(self.w.n))
         if name in ("rhs", "solution"):
-            return np.moveaxis(out.reshape(3, *self.w.n), 0, -1)
+            nu = (self.w.n[0] + (2 if self.w.boundary == "wall" else 0), self.w.n[1], self.w.n[2])
+            return np.moveaxis(out.reshape(3, *nu), 0, -1)
         if n % nodes == 0 and name in ("E", "Etheta", "Enew", "J", "divPi", "mu"):
             c = n // nodes
             a = np.moveaxis(out.reshape(c, N[0] + 1, N[1] + 1, N[2] + 1), 0, -1)

@@ -61,6 +61,11 @@ class Workload:
     e0: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     seed: int = 42
     gmres_tolerance: float = 1e-8
+    # x boundary: "periodic", or "wall" (a perfectly conducting reflecting wall at the left
+    # end and a constant-field free boundary at the right end, PIC++/initialization.cpp:5201-5210)
+    boundary: str = "periodic"
+    e_right: Tuple[float, float, float] = (0.0, 0.0, 0.0)  # fields held at the free right boundary
+    b_right: Tuple[float, float, float] = (0.0, 0.0, 0.0)
 
     @property
     def N(self) -> Tuple[int, int, int]:
@@ -115,6 +120,22 @@ def mini(n=(6, 5, 5), ppc=4, steps=5, seed=7) -> Workload:
     """Small case for golden vectors and fast parity tests."""
     return Workload("mini-%dx%dx%d-ppc%d" % (n[0], n[1], n[2], ppc), tuple(n), 1.0, 0.1, steps,
                     electron_proton(ppc, 0.01, 100.0), b0=(b_for_gyro(0.5), 0.0, 0.0), seed=seed)
+
+
+def shock(n=(1024, 64, 64), ppc=16, gamma0=10.0, sigma=0.1, theta=1e-4, mass_ratio=64.0, dt=0.1, steps=500, seed=11,
+          name="shock") -> Workload:
+    """Relativistic shock setup: a cold e-/p+ flow with bulk Lorentz factor gamma0 moving along
+    -x onto the reflecting wall at the left end, transverse B0 along z with magnetisation
+    sigma = B0^2 / (4 pi gamma0 n (m_e + m_p) c^2), and the motional field E0 = -V0 x B0
+    (PIC++/initialization.cpp:5201-5210). The right boundary holds E0, B0."""
+    v = math.sqrt(1.0 - 1.0 / (gamma0 * gamma0))
+    b = math.sqrt(4.0 * math.pi * sigma * gamma0 * (1.0 + mass_ratio))
+    e0 = (0.0, -v * b, 0.0)  # -V0 x B0 with V0 = (-v, 0, 0), B0 = (0, 0, b)
+    sp = electron_proton(ppc, theta, mass_ratio)
+    for s in sp:
+        s.drift_gamma, s.drift_axis, s.drift_sign = gamma0, 0, -1
+    return Workload("%s-%dx%dx%d-ppc%d" % (name, n[0], n[1], n[2], ppc), tuple(n), 1.0, dt, steps, sp, b0=(0.0, 0.0, b),
+                    e0=e0, seed=seed, boundary="wall", e_right=e0, b_right=(0.0, 0.0, b))
 
 
 def weibel(n=(192, 192, 192), ppc=32, steps=300, seed=7) -> Workload:
@@ -228,7 +249,12 @@ def make_initial_state(w: Workload, perturb: float = 0.0) -> InitialState:
             Eo[..., c] += perturb * b0 * np.sin(ph + 0.3 * c)
             ph = 2 * math.pi * (grid_c[0] * 1 + grid_c[1] * (c % 2 + 1) + grid_c[2] * 1)
             Bo[..., c] += perturb * b0 * np.cos(ph + 0.7 * c)
-    return InitialState(sp, np.arange(len(sp), dtype=np.int32), x, p, periodic_fill_nodes(Eo), periodic_fill_cells(Bo))
+    E = periodic_fill_nodes(Eo)
+    if w.boundary == "wall":
+        # tangential E vanishes on the conducting wall and in the ghost layers behind it
+        # (PIC++/initialization.cpp:5270-5283)
+        E[:3, :, :, 1:] = 0.0
+    return InitialState(sp, np.arange(len(sp), dtype=np.int32), x, p, E, periodic_fill_cells(Bo))
 
 
 def species_arrays(w: Workload):

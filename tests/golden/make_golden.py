@@ -8,6 +8,9 @@ fixtures next to this script:
   mini_ref.npz     6x5x5 cells, 4 ppc per species, 5 steps, perturbed fields
   config1_ref.npz  32x32x8 cells, 20 ppc per species, 50 steps (energies, norms
                    and a sample of particles only)
+  wall_ref.npz     12x5x5 cells, 4 ppc per species, 5 steps of a gamma 2 flow onto the
+                   reflecting left wall (super-conducting wall, constant-field free right
+                   boundary, no injection), dt = 0.5, perturbed fields
 
 Usage: python tests/golden/make_golden.py [mini|config1]...
 """
@@ -47,6 +50,11 @@ def write_input(path, w, st, steps):
         rec["x"] = st.x
         rec["p"] = st.p
         f.write(rec.tobytes())
+        if w.boundary == "wall":
+            f.write(struct.pack("<i", 1))
+            f.write(struct.pack("<6d", *w.e_right, *w.b_right))
+        else:
+            f.write(struct.pack("<i", 0))
 
 
 def read_output(path):
@@ -97,6 +105,17 @@ def make_mini():
     np.savez_compressed(os.path.join(HERE, "mini_ref.npz"), **inputs(st), **out)
 
 
+def wall_case():
+    w = wl.shock(n=(12, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=5, seed=19, name="wall")
+    return w, wl.make_initial_state(w, perturb=0.05)
+
+
+def make_wall():
+    w, st = wall_case()
+    out = run(w, st, w.steps)
+    np.savez_compressed(os.path.join(HERE, "wall_ref.npz"), **inputs(st), **out)
+
+
 def make_config1():
     w = wl.config1()
     st = wl.make_initial_state(w)
@@ -122,5 +141,5 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref", "-j8"], check=True)
     for name in which:
-        {"mini": make_mini, "config1": make_config1}[name]()
+        {"mini": make_mini, "config1": make_config1, "wall": make_wall}[name]()
         print("wrote", name)
