@@ -594,8 +594,8 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 	nu[2] = u[2] + (Ef[2] + (v[0] * Bf[1] - v[1] * Bf[0])) * qdm;
 	float nu2 = nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2];
 	float nig = nu2 < 0.0001f ? 1.0f : 1.0f / sqrtf(1.0f + nu2);
+	const float nup[3] = {nu[0], nu[1], nu[2]};  // the predictor's u (the Picard loop overwrites nu)
 	float mid[3], xh[3];
-	const float vpred[3] = {nu[0] * nig, nu[1] * nig, nu[2] * nig};  // the predictor velocity
 #pragma unroll
 	for (int c = 0; c < 3; ++c) {
 		mid[c] = v[c] * (1.0f - eta) + nu[c] * nig * eta;
@@ -612,11 +612,14 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		pos[0] = (float)(4 - 2 * cc[0]) - xc[0] + fabsf(mid[0] * rest) * inv_d[0];
 		pos[1] = xc[1] + mid[1] * rest * inv_d[1];
 		pos[2] = xc[2] + mid[2] * rest * inv_d[2];
-		float vr[3] = {fabsf(vpred[0]), vpred[1], vpred[2]};
-		float v2 = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2];
-		float gr = v2 < 0.0001f ? 1.0f : 1.0f / sqrtf(1.0f - v2);
-#pragma unroll
-		for (int c = 0; c < 3; ++c) unew[c] = vr[c] * gr;
+		// u = v / sqrt(1 - v^2) of the reflected predictor velocity v = nup nig. For v^2 >= 1e-4
+		// that is nup itself (|v| is unchanged by the reflection and 1 - v^2 = nig^2), which
+		// avoids forming 1 - v^2 in FP32 at high gamma; below, the nonrelativistic branch u = v.
+		const float v2 = nu2 * nig * nig;
+		const float sc = v2 < 0.0001f ? nig : 1.0f;
+		unew[0] = fabsf(nup[0]) * sc;
+		unew[1] = nup[1] * sc;
+		unew[2] = nup[2] * sc;
 		reflected = true;
 	};
 	if (g.wall && xh[0] < (float)(2 - cc[0])) reflect_at_wall(xh);
@@ -667,7 +670,10 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 			unew[0] = -unew[0];
 		}
 	}
-	if (g.wall && pos[0] > (float)(g.n[0] + 2 - cc[0])) {  // out through the free right end: removed
+	// Out through the free right end: removed (mover.cpp:447-458). The reference removes x > node
+	// n+2; a position exactly on that node is removed here too, so that every kept particle has
+	// an owned cell.
+	if (g.wall && pos[0] >= (float)(g.n[0] + 2 - cc[0])) {
 		keys[i] = 0xFFFFFFFFu;
 		moved[i] = 1;
 		atomicAdd(dead, 1);
@@ -688,7 +694,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		const bool walled = g.wall && c == 0;
 		if (!walled && ci < 2) ci += g.n[c];
 		if (!walled && ci > g.n[c] + 1) ci -= g.n[c];
-		if (ci < 2 || ci > g.n[c] + 1 + (walled ? 1 : 0)) ok = false;
+		if (ci < 2 || ci > g.n[c] + 1) ok = false;
 		nc[c] = ci;
 		f[c] = x;
 	}

@@ -11,6 +11,8 @@ fixtures next to this script:
   wall_ref.npz     12x5x5 cells, 4 ppc per species, 5 steps of a gamma 2 flow onto the
                    reflecting left wall (super-conducting wall, constant-field free right
                    boundary, no injection), dt = 0.5, perturbed fields
+  wall_out_ref.npz the wall case with the particles of the right half turned to move along
+                   +x, so that they leave through the free right end
 
 Usage: python tests/golden/make_golden.py [mini|config1]...
 """
@@ -116,6 +118,20 @@ def make_wall():
     np.savez_compressed(os.path.join(HERE, "wall_ref.npz"), **inputs(st), **out)
 
 
+def wall_out_case():
+    w, st = wall_case()
+    L = w.n[0] * w.dx
+    right = st.x[:, 0] > 0.5 * L
+    st.p[right, 0] = np.abs(st.p[right, 0])
+    return w, st
+
+
+def make_wall_out():
+    w, st = wall_out_case()
+    out = run(w, st, w.steps)
+    np.savez_compressed(os.path.join(HERE, "wall_out_ref.npz"), **inputs(st), **out)
+
+
 def make_config1():
     w = wl.config1()
     st = wl.make_initial_state(w)
@@ -141,5 +157,5 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref", "-j8"], check=True)
     for name in which:
-        {"mini": make_mini, "config1": make_config1, "wall": make_wall}[name]()
+        {"mini": make_mini, "config1": make_config1, "wall": make_wall, "wall_out": make_wall_out}[name]()
         print("wrote", name)
