@@ -90,6 +90,21 @@ typedef struct {
 	int boundary_x;         /* PIC_BOUNDARY_* */
 	double e_right[3];      /* E held at the free right boundary (PIC_BOUNDARY_WALL) */
 	double b_right[3];      /* B held at the free right boundary */
+	/* Injection (PIC_BOUNDARY_WALL only), the set-up of initializeFluxFromRight
+	   (PIC++/initialization.cpp:5201-5210): with inject = 1 every pic_push, after the push,
+	   adds the flow at the right end (Simulation::injectNewParticles, PIC++/mover.cpp:910-1000,
+	   with injectionLength advanced by |V0_x| dt, simulation.cpp:281-283) and replaces every
+	   particle that left through the right end by a new one at the boundary
+	   (addToPreserveChargeGlobal, PIC++/initialization.cpp:8281-8338). New particles get ids
+	   from one past the largest id uploaded. Random numbers come from Philox4x64-10 keyed by
+	   seed (the reference uses rand()), so injected momenta and in-cell offsets follow the
+	   reference's distributions, not its samples. */
+	int inject;
+	double flow[3];          /* V0, the flow velocity (|V0| < 1) */
+	int ppc[8];              /* particlesPerBin of each species; 0 = not injected */
+	double temperature[8];   /* kT / (m c^2) of each injected species in the flow frame, < 0.1
+	                            (createParticle's Maxwellian branch, initialization.cpp:8507-8514) */
+	unsigned long long seed;
 } pic_config;
 
 typedef struct {
@@ -172,6 +187,12 @@ int pic_step(pic_ctx* ctx, pic_solve_info* info);
 int pic_gmres_solve(pic_ctx* ctx, const double* rhs, double* x, pic_solve_info* info);
 /* y = A v on device vectors [3][n0*n1*n2] (multiplySpecialMatrixVector, specialmath.cpp:202-252) */
 int pic_apply_operator(pic_ctx* ctx, const double* v, double* y);
+
+/* Philox4x64-10, the generator of the injection: out (device, 4 n words) receives the
+   outputs for counters (c0, c1 + i, c2, c3), i < n, under key (k0, k1). For tests of the
+   generator against an independent implementation. */
+int pic_philox(pic_ctx* ctx, unsigned long long c0, unsigned long long c1, unsigned long long c2, unsigned long long c3,
+               unsigned long long k0, unsigned long long k1, long long n, unsigned long long* out);
 
 /* Blocks until the context's stream is idle. */
 int pic_synchronize(pic_ctx* ctx);

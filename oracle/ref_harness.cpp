@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstdint>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -253,6 +254,40 @@ int main(int argc, char** argv) {
 		s.boundaryConditionTypeX = SUPER_CONDUCTOR_LEFT;
 		s.rightBoundaryFieldEvaluator = new ConstantBoundaryFieldEvaluator(e_right, b_right);
 	}
+	// optional trailer: injection at the right end (config of initializeFluxFromRight): flow
+	// velocity V0, then per species the particles per bin and kT / (m c^2). Each step then adds
+	// columns of new particles (Simulation::injectNewParticles, PIC++/mover.cpp:910-1000) and
+	// replaces right escapes (addToPreserveChargeGlobal, PIC++/initialization.cpp:8281-8338),
+	// as PIC++/simulation.cpp:281-289 does.
+	int32_t inject = 0;
+	if (boundary == 1 && fread(&inject, sizeof(int32_t), 1, in) != 1) inject = 0;
+	if (inject == 1) {
+		s.V0.x = rd<double>(in);
+		s.V0.y = rd<double>(in);
+		s.V0.z = rd<double>(in);
+		for (int t = 0; t < s.typesNumber; ++t) s.types[t].particlesPerBin = 0;
+		for (int q = 0; q < nspecies; ++q) {
+			int ppc = rd<int32_t>(in);
+			double theta = rd<double>(in);
+			int t = 0;
+			while (t < s.typesNumber && s.types[t].type != (ParticleTypes)sp_type[q]) ++t;
+			if (t == s.typesNumber) {
+				fprintf(stderr, "species type %d not found\n", sp_type[q]);
+				return 2;
+			}
+			ParticleTypeContainer& tc = s.types[t];
+			tc.mass = sp_mass[q];  // the species as loaded (createParticle reads the type's mass and charge)
+			tc.charge = sp_charge[q];
+			tc.chargeCount = sp_charge[q] > 0 ? 1 : -1;
+			tc.particlesPerBin = ppc;
+			tc.particesDeltaX = s.deltaX / ppc;
+			tc.injectionLength = tc.particesDeltaX - 0.0001 * s.deltaX;  // PIC++/simulation.cpp:125-127
+			double T = theta * sp_mass[q] / s.kBoltzman_normalized;
+			tc.temperatureX = tc.temperatureY = tc.temperatureZ = T;
+			tc.concentration = sp_weight[q] * ppc / s.volumeB();  // weight = concentration / ppc * volumeB
+		}
+		s.preserveChargeGlobal = true;
+	}
 	fclose(in);
 	s.particlesNumber = (int)np;
 
@@ -308,6 +343,11 @@ int main(int argc, char** argv) {
 		s.moveParticles();
 		s.eraseEscapedPaticles();
 		s.exchangeParticles();
+		if (inject == 1) {  // PIC++/simulation.cpp:281-289
+			for (int t = 0; t < s.typesNumber; ++t) s.types[t].injectionLength += fabs(s.V0.x * s.deltaT);
+			s.injectNewParticles();
+			s.addToPreserveChargeGlobal();
+		}
 		s.updateParticleCorrelationMaps();
 		s.updateDensityParameters();
 		s.exchangeGeneralEfield(s.newEfield);
@@ -320,6 +360,11 @@ int main(int argc, char** argv) {
 		s.currentIteration++;
 		s.updateEnergy();
 		dump_particles(s, tag("particles", step));
+		if (inject == 1) {  // reference type of every particle, in the same order
+			std::vector<double> ty;
+			for (size_t p = 0; p < s.particles.size(); ++p) ty.push_back((double)s.particles[p]->type);
+			dump(tag("ptype", step).c_str(), ty, {(uint64_t)ty.size()});
+		}
 		dump_node_vec(s, s.Efield, tag("E", step));
 		dump_cell_vec(s, s.Bfield, tag("B", step));
 		dump_cell_scalar(s, s.chargeDensity, tag("rho", step));

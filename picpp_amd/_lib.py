@@ -38,7 +38,8 @@ class PicConfig(ctypes.Structure):
                 ("capacity", ctypes.c_longlong), ("precision", ctypes.c_int), ("stop_rule", ctypes.c_int),
                 ("ortho", ctypes.c_int), ("gmres_tolerance", ctypes.c_double), ("gmres_max_iterations", ctypes.c_int),
                 ("solver_precision", ctypes.c_int), ("boundary_x", ctypes.c_int), ("e_right", ctypes.c_double * 3),
-                ("b_right", ctypes.c_double * 3)]
+                ("b_right", ctypes.c_double * 3), ("inject", ctypes.c_int), ("flow", ctypes.c_double * 3),
+                ("ppc", ctypes.c_int * 8), ("temperature", ctypes.c_double * 8), ("seed", ctypes.c_ulonglong)]
 
 
 class PicSolveInfo(ctypes.Structure):
@@ -69,6 +70,7 @@ SIGNATURES = {
     "pic_gmres_solve": (ctypes.c_int, [_V, _D, _D, ctypes.POINTER(PicSolveInfo)]),
     "pic_apply_operator": (ctypes.c_int, [_V, _D, _D]),
     "pic_synchronize": (ctypes.c_int, [_V]),
+    "pic_philox": (ctypes.c_int, [_V] + [ctypes.c_ulonglong] * 6 + [ctypes.c_longlong, _V]),
     "pic_upload_particles_device": (ctypes.c_int, [_V, ctypes.POINTER(ctypes.c_longlong), _D, _D, _I]),
     "pic_profile_enable": (ctypes.c_int, [_V, ctypes.c_int]),
     "pic_profile_read": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_char_p, _D, ctypes.POINTER(ctypes.c_longlong), _D]),

@@ -16,6 +16,7 @@
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
 #include "pic_kernels_fp32.hpp"
+#include "pic_inject.hpp"
 #include "pic_precond.hpp"
 #include "pic_solver.hpp"
 
@@ -85,6 +86,15 @@ struct pic_ctx {
 	// array (5 per species + charge) for the mirror (negated x momentum) and the right-end fold
 	int* dead = nullptr;
 	int *dens_flip = nullptr, *dens_zero = nullptr;
+	// indices of the particles that left through the right end (push order, then ascending)
+	uint32_t *esc = nullptr, *esc_sorted = nullptr;
+	int esc_bits = 1;
+	// injection at the right end: injectionLength per species, step and slab numbers of the
+	// random streams, next particle id
+	double inj_len[kMaxSpecies] = {};
+	unsigned long long inj_step = 0;
+	unsigned inj_slabs = 0;
+	long long next_id = 0;
 	// Krylov solver
 	double *rhs = nullptr, *muc = nullptr, *xsol = nullptr, *work = nullptr, *work2 = nullptr, *partial = nullptr, *dscal = nullptr, *ydev = nullptr;
 	std::vector<double*> V;
@@ -189,18 +199,13 @@ void check_status(pic_ctx* c) {
 	}
 }
 
-// Sort particles by key (stable radix sort: deterministic), rebuild start[]. Particles the push
-// removed (wall boundary: key 0xFFFFFFFF) sort to the end and are dropped; the species ranges
-// are then read back from start[].
-void sort_particles(pic_ctx* c) {
+// Sort particles by key (stable radix sort: deterministic), rebuild start[]. The `removed`
+// particles the push took out (wall boundary: key 0xFFFFFFFF) sort to the end and are
+// dropped; after a removal or an injection (`resized`) the species ranges are read back from
+// start[].
+void sort_particles(pic_ctx* c, int removed = 0, bool resized = false) {
 	long long n = c->np();
 	hipStream_t s = c->stream;
-	int removed = 0;
-	if (c->g.wall && n > 0) {
-		HIP_CHECK(hipMemcpyAsync(&removed, c->dead, sizeof(int), hipMemcpyDeviceToHost, s));
-		HIP_CHECK(hipStreamSynchronize(s));
-		HIP_CHECK(hipMemsetAsync(c->dead, 0, sizeof(int), s));
-	}
 	if (n > 0) {
 		{
 			Timed t(c, "sort_radix", (double)n * 16);
@@ -229,7 +234,7 @@ void sort_particles(pic_ctx* c) {
 	launch_count_keys(c->keys_sorted, n, c->counts, s);
 	size_t bytes = c->cub_temp_bytes;
 	HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start, (int)(c->nkeys + 1), s));
-	if (removed > 0) {  // species ranges after the removal: start[] at each species' first key
+	if (removed > 0 || resized) {  // species ranges after the removal: start[] at each species' first key
 		for (int q = 0; q < c->sp.count; ++q)
 			HIP_CHECK(hipMemcpyAsync(&c->species_start_h[q], c->start + (((long long)q * c->g.ncells) << 3), sizeof(int),
 			                         hipMemcpyDeviceToHost, s));
@@ -239,6 +244,71 @@ void sort_particles(pic_ctx* c) {
 		                         hipMemcpyHostToDevice, s));
 		HIP_CHECK(hipStreamSynchronize(s));
 	}
+}
+
+void set_count(pic_ctx* c, long long n) {
+	c->P.count = c->Palt.count = c->fp32 ? 0 : n;
+	c->Q.count = c->Qalt.count = c->fp32 ? n : 0;
+}
+
+InjectPhysics inject_physics(const pic_ctx* c) {
+	InjectPhysics ph{};
+	for (int a = 0; a < 3; ++a) ph.flow[a] = c->cfg.flow[a];
+	for (int s = 0; s < kMaxSpecies; ++s) ph.theta[s] = s < c->sp.count ? c->cfg.temperature[s] : 0.0;
+	ph.seed = c->cfg.seed;
+	ph.step = c->inj_step;
+	return ph;
+}
+
+// After a push with the wall boundary (PIC++/simulation.cpp:258-289 order: erase and exchange,
+// injection, preserve-charge replacement): particles that left through the right end are
+// dropped, or with injection on replaced in place; the injected slabs are appended; one full
+// sort restores the cell order and the species ranges.
+void wall_particles(pic_ctx* c) {
+	hipStream_t s = c->stream;
+	int removed = 0;
+	HIP_CHECK(hipMemcpyAsync(&removed, c->dead, sizeof(int), hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	HIP_CHECK(hipMemsetAsync(c->dead, 0, sizeof(int), s));
+	bool resized = removed > 0;
+	if (c->cfg.inject) {
+		long long n = c->np();
+		InjectPhysics ph = inject_physics(c);
+		{
+			Timed t(c, "inject", 0);
+			std::vector<InjectSlabs> plan =
+			    plan_injection(c->g, c->sp.count, c->cfg.ppc, c->cfg.flow[0], c->cfg.dt, c->inj_len, c->inj_slabs);
+			const long long ncols = (long long)c->g.n[1] * c->g.n[2];
+			long long ninj = 0;
+			for (const InjectSlabs& b : plan) ninj += b.count * ncols;
+			if (n + ninj > c->capacity)
+				throw DeviceError{PIC_ERR_MEMORY, "particle capacity exceeded by injection (" + std::to_string(n + ninj) + " > " +
+				                                      std::to_string(c->capacity) + ")"};
+			if (c->next_id + ninj + removed > 0x7FFFFFFFLL) throw DeviceError{PIC_ERR_MEMORY, "particle ids exhausted"};
+			long long at = n;
+			for (const InjectSlabs& b : plan) {
+				ph.id_base = (int)c->next_id;
+				launch_inject(c->g, c->sp, b, ph, c->P, c->Q, c->fp32, at, c->keys, s);
+				at += b.count * ncols;
+				c->next_id += b.count * ncols;
+			}
+			if (ninj > 0) {
+				set_count(c, n + ninj);
+				resized = true;
+			}
+		}
+		if (removed > 0) {
+			Timed t(c, "replace", 0);
+			size_t bytes = c->cub_temp_bytes;
+			HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(c->cub_temp, bytes, c->esc, c->esc_sorted, removed, 0, c->esc_bits, s));
+			ph.id_base = (int)c->next_id;
+			launch_replace(c->g, c->sp, ph, c->species_start, c->esc_sorted, removed, c->P, c->Q, c->fp32, c->keys, s);
+			c->next_id += removed;
+			removed = 0;
+		}
+		c->inj_step++;
+	}
+	sort_particles(c, removed, resized);
 }
 
 // Restores the cell-sorted order of the FP32 particles after a push without a full sort
@@ -544,6 +614,14 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		return PIC_ERR_ARGUMENT;
 	if (cfg->precision != PIC_PRECISION_FP64 && cfg->precision != PIC_PRECISION_FP32) return PIC_ERR_ARGUMENT;
 	if (cfg->boundary_x != PIC_BOUNDARY_PERIODIC && cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
+	if (cfg->inject) {  // injection needs the wall set-up and createParticle's Maxwellian branch
+		if (cfg->inject != 1 || cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
+		double v2 = cfg->flow[0] * cfg->flow[0] + cfg->flow[1] * cfg->flow[1] + cfg->flow[2] * cfg->flow[2];
+		if (!(v2 < 1.0)) return PIC_ERR_ARGUMENT;
+		for (int s = 0; s < cfg->nspecies; ++s)
+			if (cfg->ppc[s] < 0 || (cfg->ppc[s] > 0 && !(cfg->temperature[s] >= 0.0 && cfg->temperature[s] < 0.1)))
+				return PIC_ERR_ARGUMENT;
+	}
 	pic_ctx* c = new pic_ctx();
 	try {
 		c->cfg = *cfg;
@@ -667,6 +745,21 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->status = dalloc<int>(1);
 		c->dead = dalloc<int>(1);
 		HIP_CHECK(hipMemset(c->dead, 0, sizeof(int)));
+		if (g.wall) {
+			c->esc = dalloc<uint32_t>(c->capacity);
+			c->esc_sorted = dalloc<uint32_t>(c->capacity);
+			while ((1LL << c->esc_bits) < c->capacity) ++c->esc_bits;
+			size_t b4 = 0;
+			HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, b4, c->esc, c->esc_sorted, (int)std::max<long long>(c->capacity, 1), 0,
+			                                            c->esc_bits, c->stream));
+			if (b4 > c->cub_temp_bytes) {
+				hipFree(c->cub_temp);
+				c->cub_temp_bytes = b4;
+				c->cub_temp = dalloc<char>(c->cub_temp_bytes);
+			}
+			for (int s = 0; s < cfg->nspecies; ++s)  // injectionLength at the loop start (PIC++/simulation.cpp:125-127)
+				if (cfg->inject && cfg->ppc[s] > 0) c->inj_len[s] = cfg->dx / cfg->ppc[s] - 0.0001 * cfg->dx;
+		}
 		{
 			int nq = 5 * cfg->nspecies + 1;
 			std::vector<int> flip(nq, 0), zero(nq, 1);
@@ -730,7 +823,7 @@ void pic_destroy(pic_ctx* c) {
 	                (void*)c->divPi, (void*)c->rhohat, (void*)c->Pi, (void*)c->dens, (void*)c->node_scratch,
 	                (void*)c->rho_scratch, (void*)c->pi_scratch, (void*)c->dens_scratch, (void*)c->keys, (void*)c->keys_sorted,
 	                (void*)c->perm, (void*)c->perm_sorted, (void*)c->start, (void*)c->counts, c->cub_temp,
-	                (void*)c->species_start, (void*)c->status, (void*)c->dead, (void*)c->dens_flip, (void*)c->dens_zero, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
+	                (void*)c->species_start, (void*)c->status, (void*)c->dead, (void*)c->dens_flip, (void*)c->dens_zero, (void*)c->esc, (void*)c->esc_sorted, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
 	                (void*)c->Vdev, (void*)c->partial, (void*)c->dscal, (void*)c->ydev, (void*)c->epartial, (void*)c->eout})
 		f(p);
 	for (ParticleArrays* P : {&c->P, &c->Palt}) {
@@ -798,8 +891,24 @@ void load_particles(pic_ctx* c, const std::vector<int>& cnt, const double* x, co
 	int* idst = c->fp32 ? c->Q.id : c->P.id;
 	if (ids) {
 		HIP_CHECK(hipMemcpyAsync(idst, ids, np * sizeof(int), hipMemcpyDeviceToDevice, s));
+		// new particles are numbered from one past the largest id (the reference's particlesNumber)
+		c->next_id = 0;
+		if (np > 0) {
+			int* dmax = dalloc<int>(1);
+			size_t bytes = 0;
+			HIP_CHECK(hipcub::DeviceReduce::Max(nullptr, bytes, ids, dmax, (int)np, s));
+			char* tmp = dalloc<char>(bytes);
+			HIP_CHECK(hipcub::DeviceReduce::Max(tmp, bytes, ids, dmax, (int)np, s));
+			int mx = 0;
+			HIP_CHECK(hipMemcpyAsync(&mx, dmax, sizeof(int), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+			hipFree(tmp);
+			hipFree(dmax);
+			c->next_id = (long long)mx + 1;
+		}
 	} else {
 		launch_iota(reinterpret_cast<uint32_t*>(idst), np, s);
+		c->next_id = np;
 	}
 	if (c->fp32) {
 		c->Q.count = np;
@@ -1261,14 +1370,16 @@ int pic_push(pic_ctx* c) {
 			}
 			Timed t(c, "push", (double)c->np() * 60 + ((double)g.nnodes * 6 + (double)g.ncells * 3) * 4);
 			launch_push32(g, c->sp, c->Q, c->species_start, c->E32, c->Et32, c->B32, (float)c->cfg.dt, (float)c->eta, c->keys,
-			              c->moved, c->status, c->dead, c->stream);
+			              c->moved, c->status, c->dead, c->esc, c->stream);
 		} else {
 			Timed t(c, "push", (double)c->P.count * 100 + ((double)c->g.nnodes * 6 + (double)c->g.ncells * 3) * 8);
 			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->status,
-			            c->dead, c->stream);
+			            c->dead, c->esc, c->stream);
 		}
 		check_status(c);
-		if (c->fp32 && !c->g.wall)  // the rebin keeps every particle; removals take the full sort
+		if (c->g.wall)  // removals, replacements and injection, then the full sort
+			wall_particles(c);
+		else if (c->fp32)
 			rebin_particles32(c);
 		else
 			sort_particles(c);
@@ -1370,6 +1481,18 @@ int pic_synchronize(pic_ctx* c) {
 	if (e != hipSuccess) {
 		c->error = hipGetErrorString(e);
 		return PIC_ERR_DEVICE;
+	}
+	return PIC_OK;
+}
+
+int pic_philox(pic_ctx* c, unsigned long long c0, unsigned long long c1, unsigned long long c2, unsigned long long c3,
+               unsigned long long k0, unsigned long long k1, long long n, unsigned long long* out) {
+	if (!c || n < 0 || (n > 0 && !out)) return PIC_ERR_ARGUMENT;
+	try {
+		launch_philox(c0, c1, c2, c3, k0, k1, n, out, c->stream);
+		HIP_CHECK(hipGetLastError());
+	} catch (const DeviceError& e) {
+		return fail(c, e);
 	}
 	return PIC_OK;
 }

@@ -46,7 +46,7 @@ void launch_update_b(const Grid& g, const double* Etheta, const double* B, doubl
 // particles
 void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const double* E,
                  const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, int* dead,
-                 hipStream_t s);
+                 uint32_t* esc, hipStream_t s);
 // wall boundary helpers: cell planes 0, 1 folded into their mirrors 3, 2 (flip[c] negates
 // component c), the right-end fold of cells N-2, N-1 into N-3 (zero_mask[c] zeroes them), and
 // zeroing of node planes i0..i1 for components [c0, c1)

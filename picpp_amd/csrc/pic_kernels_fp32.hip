@@ -559,7 +559,8 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start,
                                                 const float4* __restrict__ E, const float4* __restrict__ Et,
                                                 const float4* __restrict__ B, float dt, float eta, uint32_t* __restrict__ keys,
-                                                int* __restrict__ moved, int* __restrict__ status, int* __restrict__ dead) {
+                                                int* __restrict__ moved, int* __restrict__ status, int* __restrict__ dead,
+                                                uint32_t* __restrict__ esc) {
 	long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (idx >= P.count) return;
 	int i = (int)idx;
@@ -676,7 +677,10 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 	if (g.wall && pos[0] >= (float)(g.n[0] + 2 - cc[0])) {
 		keys[i] = 0xFFFFFFFFu;
 		moved[i] = 1;
-		atomicAdd(dead, 1);
+		// listed for replacement, with its absolute y, z in cell units
+		P.off[1][i] = (float)cc[1] + pos[1];
+		P.off[2][i] = (float)cc[2] + pos[2];
+		esc[atomicAdd(dead, 1)] = (uint32_t)i;
 		return;
 	}
 	// x^{n+1} = x^n + mid dt, carried into the integer cell; one-rank periodic wrap (x too unless walled)
@@ -1010,11 +1014,11 @@ void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArray
 }
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, int* dead,
-                   hipStream_t s) {
+                   uint32_t* esc, hipStream_t s) {
 	if (P.count == 0) return;
 	hipLaunchKernelGGL(k_push32, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start,
 	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(Et), reinterpret_cast<const float4*>(B), dt,
-	                   eta, keys, moved, status, dead);
+	                   eta, keys, moved, status, dead, esc);
 }
 void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, uint32_t* mk, uint32_t* mi, hipStream_t s) {
 	if (n == 0) return;

@@ -745,7 +745,8 @@ __device__ inline double dnorm(const double a[3], const double b[3]) {
 // exactly as evaluateParticlesRotationTensor computed it. One thread per particle.
 __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start, const double* __restrict__ E,
                        const double* __restrict__ Etheta, const double* __restrict__ B, double dt, double eta,
-                       uint32_t* __restrict__ keys, int* __restrict__ status, int* __restrict__ dead) {
+                       uint32_t* __restrict__ keys, int* __restrict__ status, int* __restrict__ dead,
+                       uint32_t* __restrict__ esc) {
 	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= P.count) return;
 	const double xw = g.x0[0] + 2 * g.d[0];  // the wall, xgrid[2]
@@ -849,7 +850,7 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 				P.p[c][i] = p[c];
 			}
 			keys[i] = 0xFFFFFFFFu;
-			atomicAdd(dead, 1);
+			esc[atomicAdd(dead, 1)] = (uint32_t)i;  // listed for replacement (y, z left unwrapped)
 			return;
 		}
 	}
@@ -1212,10 +1213,10 @@ void launch_update_b(const Grid& g, const double* Etheta, const double* B, doubl
 }
 void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const double* E,
                  const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, int* dead,
-                 hipStream_t s) {
+                 uint32_t* esc, hipStream_t s) {
 	if (P.count == 0) return;
 	hipLaunchKernelGGL(k_push, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, E, Etheta, B, dt, eta, keys,
-	                   status, dead);
+	                   status, dead, esc);
 }
 void launch_wall_mirror_cells(const Grid& g, double* f, int ncomp, const int* flip, hipStream_t s) {
 	long long n = (long long)g.N[1] * g.N[2] * ncomp;

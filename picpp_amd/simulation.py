@@ -47,7 +47,7 @@ class Simulation:
         cfg.nspecies = len(m)
         for s in range(len(m)):
             cfg.mass[s], cfg.charge[s], cfg.weight[s] = m[s], q[s], wt[s]
-        cfg.capacity = int(capacity if capacity is not None else workload.particles_total)
+        cfg.capacity = int(capacity if capacity is not None else workload.capacity_needed())
         cfg.precision = PRECISIONS[precision]
         cfg.stop_rule = STOP_RULES[stop_rule]
         cfg.ortho = ORTHOS[ortho]
@@ -58,6 +58,14 @@ class Simulation:
         for a in range(3):
             cfg.e_right[a] = workload.e_right[a]
             cfg.b_right[a] = workload.b_right[a]
+        if workload.inject:
+            cfg.inject = 1
+            for a in range(3):
+                cfg.flow[a] = workload.flow[a]
+            for s, sp in enumerate(workload.species):
+                cfg.ppc[s] = sp.ppc
+                cfg.temperature[s] = sp.temperature
+            cfg.seed = workload.seed
         self.cfg = cfg
         h = ctypes.c_void_p()
         _lib.check(self.lib.pic_create(ctypes.byref(cfg), ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)),
@@ -186,6 +194,14 @@ class Simulation:
 
     def synchronize(self):
         self._chk(self.lib.pic_synchronize(self.h), "pic_synchronize")
+
+    def philox(self, counter, key, n: int, out_ptr: int):
+        """Philox4x64-10 outputs of counters (c0, c1 + i, c2, c3), i < n, into the device
+        buffer out_ptr (4 n 64-bit words): the generator of the injection."""
+        c = [int(v) for v in counter]
+        k = [int(v) for v in key]
+        self._chk(self.lib.pic_philox(self.h, c[0], c[1], c[2], c[3], k[0], k[1], int(n), ctypes.c_void_p(out_ptr)),
+                  "pic_philox")
 
     # ------------------------------------------------------------ solver boundary
     def apply_operator(self, v_ptr: int, y_ptr: int):

@@ -66,6 +66,9 @@ class Workload:
     boundary: str = "periodic"
     e_right: Tuple[float, float, float] = (0.0, 0.0, 0.0)  # fields held at the free right boundary
     b_right: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    # wall boundary only: inject the flow at the right end each step and replace particles that
+    # leave there (PIC++/mover.cpp:910-1000, initialization.cpp:8281-8338)
+    inject: bool = False
 
     @property
     def N(self) -> Tuple[int, int, int]:
@@ -79,8 +82,29 @@ class Workload:
     def particles_total(self) -> int:
         return sum(s.ppc for s in self.species) * int(np.prod(self.n))
 
+    def capacity_needed(self, steps: Optional[int] = None) -> int:
+        """Particle capacity for `steps` steps (default: the workload's): the initial particles
+        plus, with injection, the slabs injected at the right end (|V0_x| dt / dx ppc of them
+        per step and species, one particle per (y, z) column each)."""
+        total = self.particles_total
+        if self.inject:
+            steps = self.steps if steps is None else steps
+            cols = self.n[1] * self.n[2]
+            slabs = abs(self.flow[0]) * self.dt * steps / self.dx + 1
+            total += sum(int(math.ceil(slabs * s.ppc)) * cols for s in self.species)
+        return total
+
     def weight(self, s: Species) -> float:
         return s.density * self.cell_volume / s.ppc
+
+    @property
+    def flow(self) -> Tuple[float, float, float]:
+        """Flow velocity V0 of the injected plasma: the drift of the first species."""
+        s = self.species[0]
+        v = [0.0, 0.0, 0.0]
+        if s.drift_gamma > 1.0:
+            v[s.drift_axis] = s.drift_sign * math.sqrt(1.0 - 1.0 / (s.drift_gamma * s.drift_gamma))
+        return tuple(v)
 
     def origin(self) -> Tuple[float, float, float]:
         """Coordinate of node 0 per axis: L - 2 dx (PIC++/initialization.cpp:194)."""
@@ -123,11 +147,12 @@ def mini(n=(6, 5, 5), ppc=4, steps=5, seed=7) -> Workload:
 
 
 def shock(n=(1024, 64, 64), ppc=16, gamma0=10.0, sigma=0.1, theta=1e-4, mass_ratio=64.0, dt=0.1, steps=500, seed=11,
-          name="shock") -> Workload:
+          name="shock", inject=True) -> Workload:
     """Relativistic shock setup: a cold e-/p+ flow with bulk Lorentz factor gamma0 moving along
     -x onto the reflecting wall at the left end, transverse B0 along z with magnetisation
     sigma = B0^2 / (4 pi gamma0 n (m_e + m_p) c^2), and the motional field E0 = -V0 x B0
-    (PIC++/initialization.cpp:5201-5210). The right boundary holds E0, B0."""
+    (PIC++/initialization.cpp:5201-5210). The right boundary holds E0, B0; with `inject` the
+    flow is injected there every step (PIC++/simulation.cpp:281-289)."""
     v = math.sqrt(1.0 - 1.0 / (gamma0 * gamma0))
     b = math.sqrt(4.0 * math.pi * sigma * gamma0 * (1.0 + mass_ratio))
     e0 = (0.0, -v * b, 0.0)  # -V0 x B0 with V0 = (-v, 0, 0), B0 = (0, 0, b)
@@ -135,7 +160,7 @@ def shock(n=(1024, 64, 64), ppc=16, gamma0=10.0, sigma=0.1, theta=1e-4, mass_rat
     for s in sp:
         s.drift_gamma, s.drift_axis, s.drift_sign = gamma0, 0, -1
     return Workload("%s-%dx%dx%d-ppc%d" % (name, n[0], n[1], n[2], ppc), tuple(n), 1.0, dt, steps, sp, b0=(0.0, 0.0, b),
-                    e0=e0, seed=seed, boundary="wall", e_right=e0, b_right=(0.0, 0.0, b))
+                    e0=e0, seed=seed, boundary="wall", e_right=e0, b_right=(0.0, 0.0, b), inject=inject)
 
 
 def weibel(n=(192, 192, 192), ppc=32, steps=300, seed=7) -> Workload:

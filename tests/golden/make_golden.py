@@ -13,11 +13,14 @@ fixtures next to this script:
                    boundary, no injection), dt = 0.5, perturbed fields
   wall_out_ref.npz the wall case with the particles of the right half turned to move along
                    +x, so that they leave through the free right end
+  inject_ref.npz   the escape case with the flow injected at the right end every step and
+                   every particle leaving there replaced (the reference's own injection)
 
 Usage: python tests/golden/make_golden.py [mini|config1]...
 """
 from __future__ import annotations
 
+import dataclasses
 import os
 import struct
 import subprocess
@@ -55,6 +58,11 @@ def write_input(path, w, st, steps):
         if w.boundary == "wall":
             f.write(struct.pack("<i", 1))
             f.write(struct.pack("<6d", *w.e_right, *w.b_right))
+            f.write(struct.pack("<i", 1 if w.inject else 0))
+            if w.inject:
+                f.write(struct.pack("<3d", *w.flow))
+                for sp in w.species:
+                    f.write(struct.pack("<id", sp.ppc, sp.temperature))
         else:
             f.write(struct.pack("<i", 0))
 
@@ -108,7 +116,8 @@ def make_mini():
 
 
 def wall_case():
-    w = wl.shock(n=(12, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=5, seed=19, name="wall")
+    w = wl.shock(n=(12, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=5, seed=19, name="wall",
+                 inject=False)
     return w, wl.make_initial_state(w, perturb=0.05)
 
 
@@ -130,6 +139,19 @@ def make_wall_out():
     w, st = wall_out_case()
     out = run(w, st, w.steps)
     np.savez_compressed(os.path.join(HERE, "wall_out_ref.npz"), **inputs(st), **out)
+
+
+def inject_case():
+    """The escape case with injection at the right end and replacement of the particles that
+    leave there."""
+    w, st = wall_out_case()
+    return dataclasses.replace(w, inject=True, name="inject"), st
+
+
+def make_inject():
+    w, st = inject_case()
+    out = run(w, st, w.steps)
+    np.savez_compressed(os.path.join(HERE, "inject_ref.npz"), **inputs(st), **out)
 
 
 def make_config1():
@@ -157,5 +179,5 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref", "-j8"], check=True)
     for name in which:
-        {"mini": make_mini, "config1": make_config1, "wall": make_wall, "wall_out": make_wall_out}[name]()
+        {"mini": make_mini, "config1": make_config1, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject}[name]()
         print("wrote", name)

@@ -31,7 +31,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 def wall_workload():
     from picpp_amd import workloads as wl
 
-    return wl.shock(n=(12, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=5, seed=19, name="wall")
+    return wl.shock(n=(12, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=5, seed=19, name="wall",
+                    inject=False)
 
 
 def compare_particles(sim, ref):

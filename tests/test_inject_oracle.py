@@ -1,0 +1,162 @@
+"""Pins the CPU restatement of the injection (oracle/inject_oracle.py), which the GPU
+injection tests compare against:
+- Philox4x64-10 against the Random123 known-answer vectors and against numpy.random.Philox,
+  an independent implementation of the same generator;
+- the per-step slabs (count and x of every injected particle) and the placement of the
+  injected particles against the compiled reference's own injection
+  (tests/golden/inject_ref.npz: injectNewParticles and addToPreserveChargeGlobal run inside
+  the reference's step);
+- the flow momentum against closed forms (a Lorentz boost of the rest-frame four-velocity,
+  the nonrelativistic branch, the cold limit) and the reference's injected momenta against
+  the oracle's distribution (means and spreads within sampling error).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import inject_oracle as io
+from picpp_amd import workloads as wl
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MASK = (1 << 64) - 1
+
+
+def test_philox_known_answers():
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x16554d9eca36314c, 0xdb20fe9d672d0fdc, 0xd7e772cee186176b, 0x7e68b68aec7ba23b)),
+        ((MASK,) * 4, (MASK, MASK), (0x87b092c3013fe90b, 0x438c3c67be8d0224, 0x9cc7d7c69cd777b6, 0xa09caebf594f0ba0)),
+        ((0x243f6a8885a308d3, 0x13198a2e03707344, 0xa4093822299f31d0, 0x082efa98ec4e6c89),
+         (0x452821e638d01377, 0xbe5466cf34e90c6c),
+         (0xa528f45403e61d95, 0x38c72dbd566e9788, 0xa5a1610e72fd18b5, 0x57bd43b5e52b7fe6)),
+    ]
+    for ctr, key, want in kat:
+        assert tuple(io.philox4x64_10(ctr, key)) == want
+
+
+def test_philox_matches_numpy():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        ctr = [int(v) for v in rng.integers(0, 2 ** 63, 4)]
+        key = [int(v) for v in rng.integers(0, 2 ** 63, 2)]
+        # numpy's Philox advances its counter before each block of 4 outputs
+        bg = np.random.Philox(counter=np.array(ctr, np.uint64), key=np.array(key, np.uint64))
+        got = [int(v) for v in bg.random_raw(4)]
+        nxt = list(ctr)
+        nxt[0] = (nxt[0] + 1) & MASK
+        assert got == io.philox4x64_10(nxt, key)
+
+
+def inject_workload():
+    import dataclasses
+    w = wl.shock(n=(12, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=5, seed=19, name="inject")
+    return w
+
+
+def golden_new_particles(d, steps):
+    """Per step: ids, reference type and state of the particles that are new in that step."""
+    prev = set(d["in/ids"].astype(int).tolist())
+    out = []
+    for s in range(1, steps + 1):
+        p = d["s%d/particles" % s]
+        t = d["s%d/ptype" % s].astype(int)
+        ids = p[:, 0].astype(int)
+        new = np.array([i not in prev for i in ids.tolist()])
+        vanished = len(prev - set(ids.tolist()))
+        order = np.argsort(ids[new])
+        out.append((ids[new][order], t[new][order], p[new][order], vanished))
+        prev = set(ids.tolist())
+    return out
+
+
+def test_slabs_and_placement_match_reference():
+    d = np.load(os.path.join(HERE, "golden", "inject_ref.npz"))
+    w = inject_workload()
+    inj = io.Injector(w, next_id=int(d["in/ids"].max()) + 1)
+    n1, n2 = w.n[1], w.n[2]
+    xr = inj.right_edge()
+    x0 = w.origin()
+    for step, (ids, types, parts, vanished) in enumerate(golden_new_particles(d, w.steps), start=1):
+        plan = inj.plan()
+        ninj = len(plan) * n1 * n2
+        assert len(ids) == ninj + vanished, step  # injected + one replacement per escape
+        # ids follow the reference's particlesNumber: injected first (species, slab, j, k)
+        assert np.array_equal(ids, np.arange(ids[0], ids[0] + len(ids)))
+        injected, repl = parts[:ninj], parts[ninj:]
+        for b, (s, x, _) in enumerate(plan):
+            blk = injected[b * n1 * n2:(b + 1) * n1 * n2]
+            assert np.all(types[b * n1 * n2:(b + 1) * n1 * n2] == w.species[s].ptype)
+            assert np.all(blk[:, 1] == x), (step, b)  # exact: the same double arithmetic
+            cols = np.arange(n1 * n2)
+            gy = x0[1] + (2 + cols // n2) * w.dx
+            gz = x0[2] + (2 + cols % n2) * w.dx
+            dy, dz = blk[:, 2] - gy, blk[:, 3] - gz
+            assert np.all((dy >= 0) & (dy < w.dx)) and np.all((dz >= 0) & (dz < w.dx))
+            assert np.ptp(dy) < 1e-12 and np.ptp(dz) < 1e-12  # one offset per slab
+        assert np.all(repl[:, 1] == xr)
+        lo, hi = x0[1] + 2 * w.dx, x0[1] + (w.n[1] + 2) * w.dx
+        assert np.all((repl[:, 2] >= lo) & (repl[:, 2] <= hi))
+        inj.step += 1
+
+
+def test_flow_momentum_closed_forms():
+    rng = np.random.default_rng(3)
+    flows = [(-0.866, 0.0, 0.0), (0.3, -0.4, 0.5), (0.0, 0.0, -0.995)]
+    for flow in flows:
+        V2 = sum(v * v for v in flow)
+        V = math.sqrt(V2)
+        e = np.array(flow) / V
+        gV = 1.0 / math.sqrt(1.0 - V2)
+        for theta in (0.01, 1e-4, 0.05):
+            for _ in range(20):
+                w = [int(v) for v in rng.integers(0, 2 ** 63, 4)]
+                p = io.flow_momentum(flow, 3.0, theta, w)
+                # rest-frame four-velocity and its Lorentz boost along e
+                n0, n1 = io.box_muller(w[0], w[1])
+                n2, _ = io.box_muller(w[2], w[3])
+                u1 = math.sqrt(theta) * np.array([n0, n1, n2])
+                g1 = math.sqrt(1.0 + u1 @ u1)
+                upar = u1 @ e
+                u = u1 + (gV * (upar + V * g1) - upar) * e
+                assert np.allclose(p / 3.0, u, rtol=1e-11, atol=1e-13), (flow, theta)
+    # cold flow: every particle at gamma_V m V0
+    p = io.flow_momentum((-0.866, 0.0, 0.0), 2.0, 0.0, [1, 2, 3, 4])
+    assert np.allclose(p, [-0.866 * 2.0 / math.sqrt(1 - 0.866 ** 2), 0, 0], rtol=1e-14)
+    # no flow: setMomentumByV's nonrelativistic branch p = m v below |v|^2 = 1e-4, m gamma v above
+    for theta in (1e-6, 0.04):
+        w = [int(v) for v in rng.integers(0, 2 ** 63, 4)]
+        p = io.flow_momentum((0.0, 0.0, 0.0), 1.0, theta, w)
+        n0, n1 = io.box_muller(w[0], w[1])
+        n2, _ = io.box_muller(w[2], w[3])
+        u1 = math.sqrt(theta) * np.array([n0, n1, n2])
+        v = u1 / math.sqrt(1 + u1 @ u1)
+        want = v if v @ v < 1e-4 else u1
+        assert np.allclose(p, want, rtol=1e-12)
+
+
+def test_reference_injected_momenta_follow_the_oracle_distribution():
+    """The reference's injected momenta (rand()-driven) and the oracle's (Philox-driven) come
+    from the same distribution: per species, mean and spread of u = p / m along and across
+    the flow agree within 5 standard errors."""
+    d = np.load(os.path.join(HERE, "golden", "inject_ref.npz"))
+    w = inject_workload()
+    ref = {sp.ptype: [] for sp in w.species}
+    for ids, types, parts, _ in golden_new_particles(d, w.steps):
+        for t, q in zip(types, parts):
+            ref[int(t)].append(q[4:7])
+    inj = io.Injector(w, next_id=0)
+    mine = {sp.ptype: [] for sp in w.species}
+    for _ in range(40):
+        out = inj.step_particles([])
+        for s, p in zip(out["species"], out["p"]):
+            mine[w.species[s].ptype].append(p)
+    for sp in w.species:
+        a = np.array(ref[sp.ptype]) / sp.mass
+        b = np.array(mine[sp.ptype]) / sp.mass
+        assert len(a) > 200
+        for c in range(3):
+            se = math.sqrt(a[:, c].var() / len(a) + b[:, c].var() / len(b))
+            assert abs(a[:, c].mean() - b[:, c].mean()) < 5 * se + 1e-12, (sp.name, c)
+            sa, sb = a[:, c].std(), b[:, c].std()
+            assert abs(sa - sb) < 5 * sb * math.sqrt(1.0 / (2 * len(a)) + 1.0 / (2 * len(b))) + 1e-12, (sp.name, c, sa, sb)
