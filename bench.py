@@ -61,6 +61,15 @@ def parse():
                          "in the chosen form and in FP64, and compare magnetic-energy growth and total energy")
     ap.add_argument("--weibel-cells", type=int, default=192)
     ap.add_argument("--weibel-ppc", type=int, default=32, help="particles per cell per species per beam")
+    ap.add_argument("--shock", type=int, default=0, metavar="STEPS",
+                    help="run the relativistic shock set-up (reflecting wall, flow injected at the right end) for STEPS "
+                         "steps and report its throughput and density pile-up instead of the default benchmark")
+    ap.add_argument("--shock-cells", type=int, default=1024, help="cells along x (64 x 64 across)")
+    ap.add_argument("--shock-ppc", type=int, default=16, help="particles per cell per species at injection")
+    ap.add_argument("--pressure", default=None, choices=["reference", "cell"],
+                    help="pressure tensor deposit: the reference's (default, except --shock) or cell-local (default for "
+                         "--shock: the reference's deposit gathers each x plane's pressure in the corner columns, which "
+                         "drives the fields there without bound in the shock set-up)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB between two device buffers three times (the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -150,9 +159,12 @@ def open_box(w, torch, dev, **kw):
     from picpp_amd.simulation import Simulation
 
     stream = torch.cuda.current_stream(dev)
-    sim = Simulation(w, capacity=w.particles_total, stream=stream.cuda_stream, **kw)
+    sim = Simulation(w, capacity=w.capacity_needed(), stream=stream.cuda_stream, **kw)
     N = w.N
     E = np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3))
+    E[...] = np.array(w.e0)
+    if w.boundary == "wall":  # no tangential E at and behind the wall (PIC++/initialization.cpp:5282-5287)
+        E[:3, :, :, 1:] = 0.0
     B = np.zeros(N + (3,))
     B[...] = np.array(w.b0)
     st = wl.InitialState(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros((0, 3)), np.zeros((0, 3)), E, B)
@@ -266,7 +278,7 @@ def energy_drift(args, w, torch, dev):
     series = {}
     for name, kw in (("chosen", dict(precision=args.precision, stop_rule=args.stop_rule, solver=args.solver)),
                      ("fp64", dict(precision="fp64", stop_rule="true_residual", solver="fp64"))):
-        sim = open_box(w, torch, dev, **kw)
+        sim = open_box(w, torch, dev, pressure=args.pressure or "reference", **kw)
         e = [sim.updateEnergy()[3]]
         for i in range(args.drift_steps):
             sim.step()
@@ -304,7 +316,7 @@ def weibel_check(args, torch, dev):
     res = {}
     for name, kw in (("chosen", dict(precision=args.precision, stop_rule=args.stop_rule, solver=args.solver)),
                      ("fp64", dict(precision="fp64", stop_rule="true_residual", solver="fp64"))):
-        sim = open_box(w, torch, dev, **kw)
+        sim = open_box(w, torch, dev, pressure=args.pressure or "reference", **kw)
         npart = sim.num_particles()
         e = [sim.updateEnergy()[:4]]
         torch.cuda.synchronize()
@@ -320,7 +332,8 @@ def weibel_check(args, torch, dev):
         torch.cuda.empty_cache()
         res[name] = (np.array(e), el, npart)
     t = np.arange(args.weibel + 1) * w.dt
-    out = {"cells": [args.weibel_cells] * 3, "particles": res["chosen"][2], "steps": args.weibel, "dt": w.dt}
+    out = {"cells": [args.weibel_cells] * 3, "particles": res["chosen"][2], "steps": args.weibel, "dt": w.dt,
+           "pressure": args.pressure or "reference"}
     for name, (e, el, npart) in res.items():
         lb = np.log(np.maximum(e[:, 2], 1e-300))
         win = 20
@@ -338,6 +351,55 @@ def weibel_check(args, torch, dev):
     out["total_energy_max_rel_difference"] = float(np.max(np.abs(a[:, 3] - b[:, 3]) / np.abs(b[:, 3])))
     out["magnetic_energy_max_rel_difference"] = float(np.max(np.abs(a[:, 2] - b[:, 2]) / np.maximum(np.abs(b[:, 2]), 1e-300)))
     return out
+
+
+def shock_run(args, torch, dev):
+    """The relativistic shock set-up: a cold e-/p+ flow (gamma0 10, mass ratio 64,
+    kT/mc^2 = 1e-4, sigma 0.1) onto the reflecting wall, injected at the right end every step.
+    Reports the step time, the particle count and the electron density profile along x
+    (averaged over y, z), whose pile-up behind the wall makes per-cell counts strongly
+    nonuniform for the sorted deposits."""
+    from picpp_amd import workloads as wl
+
+    w = wl.shock(n=(args.shock_cells, 64, 64), ppc=args.shock_ppc, steps=args.shock)
+    pressure = args.pressure or "cell"
+    sim = open_box(w, torch, dev, precision=args.precision, stop_rule=args.stop_rule, solver=args.solver, pressure=pressure)
+    n0 = sim.num_particles()
+    print("shock: %s, %d particles, precision %s, solver %s" % (w.name, n0, args.precision, args.solver), file=sys.stderr,
+          flush=True)
+    iters, secs, counts, energies = [], [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    paused = 0.0
+    for i in range(args.shock):
+        s = sim.step()
+        iters.append(s["iterations"])
+        secs.append(s["seconds"])
+        if (i + 1) % 25 == 0:  # bookkeeping outside the timed work
+            t1 = time.perf_counter()
+            counts.append(sim.num_particles())
+            energies.append([float(v) for v in sim.updateEnergy()[:4]])
+            paused += time.perf_counter() - t1
+            print("shock: step %d, %d particles, energy (particles, E, B) %.4g %.4g %.4g, gmres %d, %.1f ms per step so far" % (
+                i + 1, counts[-1], energies[-1][0], energies[-1][1], energies[-1][2], s["iterations"],
+                1e3 * (time.perf_counter() - t0 - paused) / (i + 1)), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0 - paused
+    n1 = sim.num_particles()
+    dens = sim.array("density")
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+    ne = dens[2:w.n[0] + 2, 2:w.n[1] + 2, 2:w.n[2] + 2, 0].mean(axis=(1, 2))  # electron concentration per x cell
+    upstream = float(np.median(ne[len(ne) // 2:]))
+    return {"cells": list(w.n), "ppc_per_species": args.shock_ppc, "gamma0": 10.0, "steps": args.shock, "dt": w.dt,
+            "precision": args.precision, "solver": args.solver, "pressure": pressure, "particles_initial": n0,
+            "particles_final": n1,
+            "particle_count_every_25_steps": counts, "energy_every_25_steps": energies, "ms_per_step": 1e3 * el / args.shock,
+            "particle_steps_per_s": 0.5 * (n0 + n1) * args.shock / el,
+            "gmres_iterations_per_solve": float(np.mean(iters)), "gmres_ms_per_solve": 1e3 * float(np.mean(secs)),
+            "electron_density_upstream": upstream, "electron_density_max_over_upstream": float(ne.max() / upstream),
+            "electron_density_profile_every_8_cells": [float(v) for v in ne[::8]]}
 
 
 def main():
@@ -362,13 +424,17 @@ def main():
     if args.weibel > 0:
         print(json.dumps({"weibel": weibel_check(args, torch, dev)}))
         return
+    if args.shock > 0:
+        print(json.dumps({"shock": shock_run(args, torch, dev)}))
+        return
     learned = args.learned != "off" and args.solver == "mixed"
     training = None
     if learned:
         params, mu_scale, training = train_network(args, torch, dev)
     n = (args.cells,) * 3
     w = wl.config2(n=n, ppc=args.ppc, steps=args.steps)
-    sim = open_box(w, torch, dev, precision=args.precision, stop_rule=args.stop_rule, ortho=args.ortho, solver=args.solver)
+    sim = open_box(w, torch, dev, precision=args.precision, stop_rule=args.stop_rule, ortho=args.ortho, solver=args.solver,
+                   pressure=args.pressure or "reference")
     npart = sim.num_particles()
     print("bench: %s, %d particles, precision %s, solver %s" % (w.name, npart, args.precision, args.solver), file=sys.stderr,
           flush=True)
@@ -421,7 +487,8 @@ def main():
         "energy": {"particles": energy[0], "E": energy[1], "B": energy[2], "total": energy[3]},
     }
     if not args.no_fp64_compare and args.precision == "fp32":
-        sim = open_box(w, torch, dev, precision="fp64", stop_rule="true_residual", solver="fp64")
+        sim = open_box(w, torch, dev, precision="fp64", stop_rule="true_residual", solver="fp64",
+                       pressure=args.pressure or "reference")
         el64, solves64 = time_steps(sim, args)
         sim.close()
         del sim

@@ -66,6 +66,20 @@ enum {
 	PIC_BOUNDARY_WALL = 1
 };
 
+/* pressure tensor Pi of the implicit current (J -= eta dt div Pi) */
+enum {
+	/* the reference's deposit: Pi[curI][j][k] with the stencil slot indices j, k in {0, 1}
+	   (PIC++/momentum_collecting.cpp:565, :577), so each x plane's pressure lands in the
+	   cells j, k <= 1, then the periodic folds of period n + 1 (momentum_exchange.cpp:1646-1660).
+	   Needed for parity with the reference. */
+	PIC_PRESSURE_REFERENCE = 0,
+	/* Pi[curI][curJ][curK]: each particle's pressure in the cells of its own stencil, folded as
+	   chargeDensityHat is. The reference's deposit concentrates the pressure of a whole x plane
+	   in the corner columns, whose spurious div Pi grows with ny nz; with a relativistic flow
+	   (the shock set-up at 64 x 64 cells across) it drives the fields there without bound. */
+	PIC_PRESSURE_CELL = 1
+};
+
 /* orthogonalisation inside the Arnoldi process */
 enum {
 	PIC_ORTHO_MGS = 0, /* modified Gram-Schmidt, one dot + axpy per basis vector (PIC++/specialmath.cpp:295-332) */
@@ -105,6 +119,7 @@ typedef struct {
 	double temperature[8];   /* kT / (m c^2) of each injected species in the flow frame, < 0.1
 	                            (createParticle's Maxwellian branch, initialization.cpp:8507-8514) */
 	unsigned long long seed;
+	int pressure;            /* PIC_PRESSURE_* */
 } pic_config;
 
 typedef struct {

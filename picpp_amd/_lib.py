@@ -19,6 +19,7 @@ STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 ORTHO_MGS, ORTHO_CGS2 = 0, 1
 SOLVER_FP64, SOLVER_MIXED = 0, 1
 BOUNDARY_PERIODIC, BOUNDARY_WALL = 0, 1
+PRESSURE_REFERENCE, PRESSURE_CELL = 0, 1
 
 ARRAYS = {"E": 0, "Etheta": 1, "Enew": 2, "B": 3, "Bnew": 4, "J": 5, "mu": 6, "divPi": 7, "rhohat": 8, "Pi": 9,
           "density": 10, "rhs": 11, "solution": 12}
@@ -39,7 +40,8 @@ class PicConfig(ctypes.Structure):
                 ("ortho", ctypes.c_int), ("gmres_tolerance", ctypes.c_double), ("gmres_max_iterations", ctypes.c_int),
                 ("solver_precision", ctypes.c_int), ("boundary_x", ctypes.c_int), ("e_right", ctypes.c_double * 3),
                 ("b_right", ctypes.c_double * 3), ("inject", ctypes.c_int), ("flow", ctypes.c_double * 3),
-                ("ppc", ctypes.c_int * 8), ("temperature", ctypes.c_double * 8), ("seed", ctypes.c_ulonglong)]
+                ("ppc", ctypes.c_int * 8), ("temperature", ctypes.c_double * 8), ("seed", ctypes.c_ulonglong),
+                ("pressure", ctypes.c_int)]
 
 
 class PicSolveInfo(ctypes.Structure):

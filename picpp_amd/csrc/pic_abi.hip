@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -143,6 +144,9 @@ struct pic_ctx {
 
 namespace {
 
+// status words: [0] particle check bits, [1] record claimed, [2..25] the FP32 mover's record
+constexpr int kStatusWords = 32;
+
 int fail(pic_ctx* c, const DeviceError& e) {
 	if (c) c->error = e.what;
 	return e.code;
@@ -185,16 +189,28 @@ struct Timed {
 };
 
 void check_status(pic_ctx* c) {
-	int st = 0;
-	HIP_CHECK(hipMemcpyAsync(&st, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+	int st[kStatusWords] = {};
+	HIP_CHECK(hipMemcpyAsync(st, c->status, sizeof(st), hipMemcpyDeviceToHost, c->stream));
 	HIP_CHECK(hipStreamSynchronize(c->stream));
-	if (st) {
-		HIP_CHECK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+	if (st[0]) {
+		HIP_CHECK(hipMemsetAsync(c->status, 0, sizeof(st), c->stream));
 		std::string why;
-		if (st & 1) why += " particle moved more than one box length;";
-		if (st & 2) why += " particle maps inconsistent (moved more than half a cell?);";
-		if (st & 4) why += " particle outside the owned region;";
-		if (st & 8) why += " particle stencil outside the ghost layers;";
+		if (st[0] & 1) why += " particle moved more than one box length;";
+		if (st[0] & 2) why += " particle maps inconsistent (moved more than half a cell?);";
+		if (st[0] & 4) why += " particle outside the owned region;";
+		if (st[0] & 8) why += " particle stencil outside the ghost layers;";
+		if (st[1]) {  // the FP32 mover's record of the first failing particle
+			float d[24];
+			std::memcpy(d, st + 2, sizeof(d));
+			char buf[640];
+			snprintf(buf, sizeof(buf),
+			         " first: checks %d, index %d, species %d, reflected %d, cell (%d %d %d), offset (%.9g %.9g %.9g), "
+			         "u (%.9g %.9g %.9g), new position (%.9g %.9g %.9g) cells from the cell, new u (%.9g %.9g %.9g), "
+			         "mid v (%.9g %.9g %.9g), E_x %.6g, B_z %.6g",
+			         (int)d[0], (int)d[1], (int)d[2], (int)d[3], (int)d[4], (int)d[5], (int)d[6], d[7], d[8], d[9], d[10], d[11],
+			         d[12], d[13], d[14], d[15], d[16], d[17], d[18], d[19], d[20], d[21], d[22], d[23]);
+			why += buf;
+		}
 		throw DeviceError{PIC_ERR_PARTICLE, "particle check failed:" + why};
 	}
 }
@@ -419,7 +435,10 @@ void moments(pic_ctx* c) {
 			Timed t(c, "cell_assemble", (double)g.ncells * (8 * 4 + 8));
 			launch_cell_assemble32(g, c->rscr32, 1, c->rhohat, s);
 		}
-		{
+		if (c->cfg.pressure == PIC_PRESSURE_CELL) {
+			Timed t(c, "pressure_cell", (double)g.ncells * (48 * 4 + 72));
+			launch_pressure_cell32(g, c->pscr32, c->Pi, s);
+		} else {
 			Timed t(c, "pressure_reduce", (double)g.ncells * 48 * 4);
 			HIP_CHECK(hipMemsetAsync(c->Pi, 0, 9 * g.ncells * sizeof(double), s));
 			launch_pressure_reduce32(g, c->pscr32, c->Pi, s);
@@ -441,7 +460,10 @@ void moments(pic_ctx* c) {
 			Timed t(c, "cell_assemble", (double)g.ncells * 9 * 8);
 			launch_cell_assemble(g, c->rho_scratch, c->rhohat, s);
 		}
-		{
+		if (c->cfg.pressure == PIC_PRESSURE_CELL) {
+			Timed t(c, "pressure_cell", (double)g.ncells * (48 * 8 + 72));
+			launch_pressure_cell(g, c->pi_scratch, c->Pi, s);
+		} else {
 			Timed t(c, "pressure_reduce", (double)g.ncells * 48 * 8);
 			HIP_CHECK(hipMemsetAsync(c->Pi, 0, 9 * g.ncells * sizeof(double), s));
 			launch_pressure_reduce(g, c->pi_scratch, c->Pi, s);
@@ -453,7 +475,10 @@ void moments(pic_ctx* c) {
 		launch_wall_mirror_cells(g, c->Pi, 9, nullptr, s);
 	}
 	launch_fold_cells(g, c->rhohat, 1, 4, 3, s);  // sumChargeDensityHat{X,Y,Z}
-	launch_fold_cells(g, c->Pi, 9, 3, 2, s);      // sumCellMatrixParameters{X,Y,Z}
+	if (c->cfg.pressure == PIC_PRESSURE_CELL)
+		launch_fold_cells(g, c->Pi, 9, 4, 3, s);  // the periodic images of chargeDensityHat's folds
+	else
+		launch_fold_cells(g, c->Pi, 9, 3, 2, s);  // sumCellMatrixParameters{X,Y,Z}
 	launch_fold_nodes(g, c->J, 3, s);             // sumNodeVectorParameters{X,Y,Z}
 	launch_div_pressure_update_j(g, c->Pi, c->J, c->divPi, c->eta, c->cfg.dt, s);
 	// the wall node and the ghosts behind it carry no current (momentum_collecting.cpp:497-506, :638-646)
@@ -613,6 +638,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	    cfg->dx <= 0 || cfg->dy <= 0 || cfg->dz <= 0 || cfg->capacity < 0)
 		return PIC_ERR_ARGUMENT;
 	if (cfg->precision != PIC_PRECISION_FP64 && cfg->precision != PIC_PRECISION_FP32) return PIC_ERR_ARGUMENT;
+	if (cfg->pressure != PIC_PRESSURE_REFERENCE && cfg->pressure != PIC_PRESSURE_CELL) return PIC_ERR_ARGUMENT;
 	if (cfg->boundary_x != PIC_BOUNDARY_PERIODIC && cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
 	if (cfg->inject) {  // injection needs the wall set-up and createParticle's Maxwellian branch
 		if (cfg->inject != 1 || cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
@@ -742,7 +768,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->species_start = dalloc<int>(kMaxSpecies + 1);
 		c->species_start_h.assign(kMaxSpecies + 1, 0);
 		HIP_CHECK(hipMemset(c->species_start, 0, (kMaxSpecies + 1) * sizeof(int)));
-		c->status = dalloc<int>(1);
+		c->status = dalloc<int>(kStatusWords);
 		c->dead = dalloc<int>(1);
 		HIP_CHECK(hipMemset(c->dead, 0, sizeof(int)));
 		if (g.wall) {
@@ -772,7 +798,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 			HIP_CHECK(hipMemcpy(c->dens_flip, flip.data(), nq * sizeof(int), hipMemcpyHostToDevice));
 			HIP_CHECK(hipMemcpy(c->dens_zero, zero.data(), nq * sizeof(int), hipMemcpyHostToDevice));
 		}
-		HIP_CHECK(hipMemset(c->status, 0, sizeof(int)));
+		HIP_CHECK(hipMemset(c->status, 0, kStatusWords * sizeof(int)));
 		// Krylov storage: up to max_iterations basis vectors
 		c->max_basis = std::min(120, std::max(2, c->cfg.gmres_max_iterations));
 		long long len = 3 * g.nown;

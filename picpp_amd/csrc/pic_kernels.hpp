@@ -22,6 +22,9 @@ void launch_cell_moments(const Grid& g, const SpeciesTable& sp, const ParticleAr
                          const double* B, double dt, double* rho_scratch, double* pi_scratch, hipStream_t s);
 void launch_cell_assemble(const Grid& g, const double* rho_scratch, double* rho, hipStream_t s);
 void launch_pressure_reduce(const Grid& g, const double* pi_scratch, double* Pi, hipStream_t s);
+// cell-local pressure tensor from the dual-cell partials (FP64 or FP32 partials)
+void launch_pressure_cell(const Grid& g, const double* pi_scratch, double* Pi, hipStream_t s);
+void launch_pressure_cell32(const Grid& g, const float* pi_scratch, double* Pi, hipStream_t s);
 void launch_fold_nodes(const Grid& g, double* f, int ncomp, hipStream_t s);
 void launch_fold_cells(const Grid& g, double* f, int ncomp, int pairs, int lo0, hipStream_t s);
 void launch_ghost_fill(const Grid& g, double* f, int ncomp, bool nodes, hipStream_t s);

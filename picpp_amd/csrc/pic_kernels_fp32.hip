@@ -578,7 +578,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 	const uint32_t key0 = (uint32_t)((((long long)s * g.ncells + cell0) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) |
 	                                 (f[2] >= 0.5f));
 	Map32 mp[3];
-	bool ok = maps32(g, cc, f, mp);
+	int why = maps32(g, cc, f, mp) ? 0 : 1;  // failed checks: 1 start stencil, 2 Picard stencil, 4 << c new cell
 	float En[3], Bf[3], a[9], Ef[3];
 	gather_node32(g, E, mp, En);
 	gather_cell32(g, B, mp, Bf);
@@ -647,7 +647,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 			reflect_at_wall(nx);
 			break;
 		}
-		ok &= maps32(g, cc, nx, mp);
+		if (!maps32(g, cc, nx, mp)) why |= 2;
 		gather_node32(g, Et, mp, Ef);
 		gather_cell32(g, B, mp, Bf);
 		nu[0] = u[0] + (Ef[0] + (mid[1] * Bf[2] - mid[2] * Bf[1])) * qdm;
@@ -684,6 +684,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		return;
 	}
 	// x^{n+1} = x^n + mid dt, carried into the integer cell; one-rank periodic wrap (x too unless walled)
+	const float f0[3] = {f[0], f[1], f[2]};
 	int nc[3];
 #pragma unroll
 	for (int c = 0; c < 3; ++c) {
@@ -698,11 +699,22 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		const bool walled = g.wall && c == 0;
 		if (!walled && ci < 2) ci += g.n[c];
 		if (!walled && ci > g.n[c] + 1) ci -= g.n[c];
-		if (ci < 2 || ci > g.n[c] + 1) ok = false;
+		if (ci < 2 || ci > g.n[c] + 1) why |= 4 << c;
 		nc[c] = ci;
 		f[c] = x;
 	}
-	if (!ok) atomicOr(status, 8);
+	if (why) {
+		atomicOr(status, 8);
+		// the first failing particle's state for the host's error message: status[1] claims the
+		// record, status[2..] holds it
+		if (atomicCAS(status + 1, 0, 1) == 0) {
+			float* d = reinterpret_cast<float*>(status + 2);
+			const float rec[24] = {(float)why, (float)i, (float)s, (float)reflected, (float)cc[0], (float)cc[1], (float)cc[2],
+			                       f0[0], f0[1], f0[2], u[0], u[1], u[2], pos[0], pos[1], pos[2], unew[0], unew[1], unew[2],
+			                       mid[0], mid[1], mid[2], Ef[0], Bf[2]};
+			for (int q = 0; q < 24; ++q) d[q] = rec[q];
+		}
+	}
 	int cell = (int)cell_id(g, nc[0], nc[1], nc[2]);
 	P.cell[i] = cell;
 #pragma unroll

@@ -256,6 +256,32 @@ __global__ void __launch_bounds__(256) k_pressure_reduce(Grid g, const double* _
 	}
 }
 
+// Cell-local pressure tensor (PIC_PRESSURE_CELL): Pi at cell c from the partials of the 8
+// dual cells around it, the deposit the reference's pressure loop intends
+// (PIC++/momentum_collecting.cpp:565, :577 with [curI][curJ][curK]), assembled as rhohat is.
+template <typename T>
+__global__ void k_pressure_cell(Grid g, const T* __restrict__ ps, double* __restrict__ Pi) {
+	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.ncells) return;
+	int k = (int)(c % g.N[2]);
+	int j = (int)((c / g.N[2]) % g.N[1]);
+	int i = (int)(c / ((long long)g.N[2] * g.N[1]));
+	double acc[6] = {0, 0, 0, 0, 0, 0};
+	for (int a = 0; a < 2; ++a)
+		for (int b = 0; b < 2; ++b)
+			for (int cc = 0; cc < 2; ++cc) {
+				int bi = i - a, bj = j - b, bk = k - cc;
+				if (bi < 0 || bj < 0 || bk < 0) continue;
+				const long long src = cell_id(g, bi, bj, bk);
+				const int t = a * 4 + b * 2 + cc;
+#pragma unroll
+				for (int q = 0; q < 6; ++q) acc[q] += (double)ps[((long long)t * 6 + q) * g.ncells + src];
+			}
+	const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+#pragma unroll
+	for (int q = 0; q < 9; ++q) Pi[q * g.ncells + c] = acc[sym[q]];
+}
+
 // ------------------------------------------------------------------ folds and ghosts
 // One-rank periodic folds, one axis per launch (momentum_exchange.cpp:172-195, :446-460,
 // :807-811, :1657-1660). Node arrays: pairs (N - i, 4 - i), i = 0..4. Cell arrays: pairs
@@ -1146,6 +1172,12 @@ void launch_cell_moments(const Grid& g, const SpeciesTable& sp, const ParticleAr
 }
 void launch_cell_assemble(const Grid& g, const double* rho_scratch, double* rho, hipStream_t s) {
 	hipLaunchKernelGGL(k_cell_assemble, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, rho_scratch, rho);
+}
+void launch_pressure_cell(const Grid& g, const double* pi_scratch, double* Pi, hipStream_t s) {
+	hipLaunchKernelGGL(k_pressure_cell<double>, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, pi_scratch, Pi);
+}
+void launch_pressure_cell32(const Grid& g, const float* pi_scratch, double* Pi, hipStream_t s) {
+	hipLaunchKernelGGL(k_pressure_cell<float>, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, pi_scratch, Pi);
 }
 void launch_pressure_reduce(const Grid& g, const double* pi_scratch, double* Pi, hipStream_t s) {
 	hipLaunchKernelGGL(k_pressure_reduce, dim3(g.N[0] * 4), dim3(256), 0, s, g, pi_scratch, Pi);

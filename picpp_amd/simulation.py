@@ -36,7 +36,8 @@ def _ip(a):
 class Simulation:
     def __init__(self, workload: Workload, precision: str = "fp64", stop_rule: str = "reference", ortho: str = "mgs",
                  capacity: Optional[int] = None, gmres_tolerance: Optional[float] = None,
-                 gmres_max_iterations: int = 50, stream: Optional[int] = None, solver: str = "fp64"):
+                 gmres_max_iterations: int = 50, stream: Optional[int] = None, solver: str = "fp64",
+                 pressure: str = "reference"):
         self.lib = _lib.load()
         self.w = workload
         cfg = _lib.PicConfig()
@@ -55,6 +56,7 @@ class Simulation:
         cfg.gmres_max_iterations = gmres_max_iterations
         cfg.solver_precision = SOLVERS[solver]
         cfg.boundary_x = _lib.BOUNDARY_WALL if workload.boundary == "wall" else _lib.BOUNDARY_PERIODIC
+        cfg.pressure = {"reference": _lib.PRESSURE_REFERENCE, "cell": _lib.PRESSURE_CELL}[pressure]
         for a in range(3):
             cfg.e_right[a] = workload.e_right[a]
             cfg.b_right[a] = workload.b_right[a]
