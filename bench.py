@@ -371,7 +371,10 @@ def shock_run(args, torch, dev):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     paused = 0.0
+    prof_steps = min(20, args.shock)
     for i in range(args.shock):
+        if i == args.shock - prof_steps:  # per-kernel event timing over the last steps
+            sim._chk(sim.lib.pic_profile_enable(sim.h, 1), "pic_profile_enable")
         s = sim.step()
         iters.append(s["iterations"])
         secs.append(s["seconds"])
@@ -386,6 +389,7 @@ def shock_run(args, torch, dev):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0 - paused
     n1 = sim.num_particles()
+    kernels = read_profile(sim, prof_steps)
     dens = sim.array("density")
     sim.close()
     del sim
@@ -399,7 +403,8 @@ def shock_run(args, torch, dev):
             "particle_steps_per_s": 0.5 * (n0 + n1) * args.shock / el,
             "gmres_iterations_per_solve": float(np.mean(iters)), "gmres_ms_per_solve": 1e3 * float(np.mean(secs)),
             "electron_density_upstream": upstream, "electron_density_max_over_upstream": float(ne.max() / upstream),
-            "electron_density_profile_every_8_cells": [float(v) for v in ne[::8]]}
+            "electron_density_profile_every_8_cells": [float(v) for v in ne[::8]],
+            "kernels_ms_per_step_last_%d_steps" % prof_steps: {nm: kv["ms_total"] / prof_steps for nm, kv in kernels.items()}}
 
 
 def main():

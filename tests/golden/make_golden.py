@@ -15,6 +15,9 @@ fixtures next to this script:
                    +x, so that they leave through the free right end
   inject_ref.npz   the escape case with the flow injected at the right end every step and
                    every particle leaving there replaced (the reference's own injection)
+  wall_long_ref.npz the shock set-up (gamma 10 e-/p+ flow onto the wall, mass ratio 64,
+                   sigma 0.1) at 16x8x8 cells, 8 ppc per species, 150 steps of dt = 0.1, no
+                   injection: energies and E, B, rho norms every step, a particle sample at the end
 
 Usage: python tests/golden/make_golden.py [mini|config1]...
 """
@@ -154,6 +157,27 @@ def make_inject():
     np.savez_compressed(os.path.join(HERE, "inject_ref.npz"), **inputs(st), **out)
 
 
+def wall_long_case():
+    w = wl.shock(n=(16, 8, 8), ppc=8, steps=150, inject=False, seed=5, name="wall-long")
+    return w, wl.make_initial_state(w)
+
+
+def make_wall_long():
+    w, st = wall_long_case()
+    out = run(w, st, w.steps)
+    keep = {"meta": out["meta"]}
+    keep.update(inputs(st))
+    for s in range(w.steps + 1):
+        keep["s%d/energy" % s] = out["s%d/energy" % s]
+        for f in ("E", "B", "rho"):
+            a = out["s%d/%s" % (s, f)]
+            keep["s%d/%s_norm" % (s, f)] = np.array([np.sqrt(np.sum(a * a)), np.sum(a)])
+    parts = out["s%d/particles" % w.steps]
+    rng = np.random.default_rng(1)
+    keep["final_particles_sample"] = parts[np.argsort(parts[:, 0])][np.sort(rng.choice(len(parts), 512, replace=False))]
+    np.savez_compressed(os.path.join(HERE, "wall_long_ref.npz"), **keep)
+
+
 def make_config1():
     w = wl.config1()
     st = wl.make_initial_state(w)
@@ -179,5 +203,5 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref", "-j8"], check=True)
     for name in which:
-        {"mini": make_mini, "config1": make_config1, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject}[name]()
+        {"mini": make_mini, "config1": make_config1, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject, "wall_long": make_wall_long}[name]()
         print("wrote", name)

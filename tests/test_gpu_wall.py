@@ -129,3 +129,43 @@ def test_fp32_wall_tracks_fp64(fixture):
         assert v <= TOL32, (k, v)
     for sim in sims.values():
         sim.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
+def test_long_shock_run_tracks_reference(precision, tol):
+    """150 steps of the shock set-up (gamma 10 flow onto the wall, 16x8x8 cells, no injection)
+    against the compiled reference's energies and E, B, rho norms at every step and a particle
+    sample at the end (tests/golden/wall_long_ref.npz). FP64 keeps to 1e-10 (measured about
+    1e-12 after 150 steps); the FP32 form to 1e-5."""
+    from picpp_amd import workloads as wl
+
+    d = np.load(os.path.join(HERE, "golden", "wall_long_ref.npz"))
+    w = wl.shock(n=(16, 8, 8), ppc=8, steps=150, inject=False, seed=5, name="wall-long")
+    print("case", w.name, precision, flush=True)
+    sim = Simulation(w, precision=precision)
+    sim.load_state(GoldenState(d))
+    sim.initialize()
+    worst = {}
+
+    def track(name, got, want):
+        e = relerr(got, want)
+        worst[name] = max(worst.get(name, 0.0), e)
+
+    for step in range(1, w.steps + 1):
+        sim.step()
+        track("energy", sim.updateEnergy()[:4], d["s%d/energy" % step][:4])
+        for f in ("E", "B"):
+            a = sim.array(f)
+            track(f, [np.sqrt(np.sum(a * a)), np.sum(a)], d["s%d/%s_norm" % (step, f)])
+        rho = sim.array("density")[..., 5 * len(w.species)]
+        track("rho", np.sqrt(np.sum(rho * rho)), d["s%d/rho_norm" % step][0])
+    g = sim.particles()
+    ref = d["final_particles_sample"]
+    idx = {int(i): k for k, i in enumerate(g["ids"])}
+    sel = np.array([idx[int(i)] for i in ref[:, 0]])
+    track("x", g["x"][sel], ref[:, 1:4])
+    track("p", g["p"][sel], ref[:, 4:7])
+    print("%s over 150 steps vs reference (worst):" % precision, {k: "%.1e" % v for k, v in worst.items()})
+    for k, v in worst.items():
+        assert v <= tol, (k, v)
+    sim.close()
