@@ -120,6 +120,12 @@ typedef struct {
 	                            (createParticle's Maxwellian branch, initialization.cpp:8507-8514) */
 	unsigned long long seed;
 	int pressure;            /* PIC_PRESSURE_* */
+	/* Smoothing after the moments (PIC++/simulation.cpp:213-217): smoothing_count passes over
+	   chargeDensityHat, electricFlux and the dielectric tensor of the 27-point kernel with
+	   weights (s/2, 1 - s, s/2) per axis, s = smoothing_parameter (PIC++/smoothing.cpp). The
+	   shipped input uses 8 passes of s = 0.5; 0 passes is the default. */
+	int smoothing_count;
+	double smoothing_parameter;
 } pic_config;
 
 typedef struct {

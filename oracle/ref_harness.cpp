@@ -288,6 +288,18 @@ int main(int argc, char** argv) {
 		}
 		s.preserveChargeGlobal = true;
 	}
+	// optional trailer: smoothing passes and parameter (smoothingCount, smoothingParameter;
+	// the step loop smooths chargeDensityHat, electricFlux and dielectricTensor,
+	// PIC++/simulation.cpp:213-217)
+	int32_t smooth_count = 0;
+	double smooth_param = 0.0;
+	if (fread(&smooth_count, sizeof(int32_t), 1, in) == 1) {
+		smooth_param = rd<double>(in);
+		s.smoothingCount = smooth_count;
+		s.smoothingParameter = smooth_param;
+	} else {
+		s.smoothingCount = 0;
+	}
 	fclose(in);
 	s.particlesNumber = (int)np;
 
@@ -320,6 +332,11 @@ int main(int argc, char** argv) {
 		s.evaluateParticlesRotationTensor();
 		if (full) dump_alpha(s, tag("alpha", step));
 		s.updateElectroMagneticParameters();
+		for (int n = 0; n < s.smoothingCount; ++n) {  // PIC++/simulation.cpp:213-217
+			s.smoothChargeDensityHat();
+			s.smoothFlux();
+			s.smoothMatrixNodeParameter(s.dielectricTensor);
+		}
 		if (full) {
 			dump_node_vec(s, s.electricFlux, tag("J", step));
 			dump_node_mat(s, s.dielectricTensor, tag("mu", step));

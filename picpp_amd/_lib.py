@@ -41,7 +41,7 @@ class PicConfig(ctypes.Structure):
                 ("solver_precision", ctypes.c_int), ("boundary_x", ctypes.c_int), ("e_right", ctypes.c_double * 3),
                 ("b_right", ctypes.c_double * 3), ("inject", ctypes.c_int), ("flow", ctypes.c_double * 3),
                 ("ppc", ctypes.c_int * 8), ("temperature", ctypes.c_double * 8), ("seed", ctypes.c_ulonglong),
-                ("pressure", ctypes.c_int)]
+                ("pressure", ctypes.c_int), ("smoothing_count", ctypes.c_int), ("smoothing_parameter", ctypes.c_double)]
 
 
 class PicSolveInfo(ctypes.Structure):
@@ -97,6 +97,7 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libpicpp_amd.so is not built (run python -m picpp_amd.build): %s" % LIB_PATH)
+    _share_torch_hip_runtime()
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
@@ -104,6 +105,18 @@ def load():
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def _share_torch_hip_runtime():
+    """A process holds one HIP runtime (libamdhip64.so.7): the library's dependency binds to the
+    copy already loaded. When torch is installed its bundled runtime is brought up first, so
+    that torch and the library share it; the other order leaves torch without a usable device.
+    Device buffers from torch are the library's inputs in tests and benchmarks."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
 
 
 class PicError(RuntimeError):

@@ -96,7 +96,11 @@ struct pic_ctx {
 	unsigned long long inj_step = 0;
 	unsigned inj_slabs = 0;
 	long long next_id = 0;
+	// smoothing scratch: 9 node planes
+	double* smooth_tmp = nullptr;
 	// Krylov solver
+	double* muh = nullptr;    // mu on the halo layout of the operators
+	float* mu32h = nullptr;
 	double *rhs = nullptr, *muc = nullptr, *xsol = nullptr, *work = nullptr, *work2 = nullptr, *partial = nullptr, *dscal = nullptr, *ydev = nullptr;
 	std::vector<double*> V;
 	double** Vdev = nullptr;
@@ -488,6 +492,24 @@ void moments(pic_ctx* c) {
 	if (g.wall) launch_zero_node_planes(g, c->mu, 0, 9, 2, 2, s);
 }
 
+// The smoothing loop of the step (PIC++/simulation.cpp:213-217): smoothChargeDensityHat with its
+// scalar cell exchange, smoothFlux with exchangeGeneralEfield, smoothMatrixNodeParameter on the
+// dielectric tensor, whose ghosts the reference does not refresh between passes.
+void smooth(pic_ctx* c) {
+	if (c->cfg.smoothing_count <= 0) return;
+	const Grid& g = c->g;
+	hipStream_t s = c->stream;
+	Timed t(c, "smoothing", (double)c->cfg.smoothing_count * ((double)g.ncells * 2 + (double)g.nnodes * 12 * 2) * 8 * 2);
+	const Smooth27 m = smoothing_matrix(c->cfg.smoothing_parameter);
+	for (int n = 0; n < c->cfg.smoothing_count; ++n) {
+		launch_smooth_cells(g, c->rhohat, c->smooth_tmp, m, s);
+		launch_ghost_fill(g, c->rhohat, 1, false, s, /*wall_zero=*/true);
+		launch_smooth_nodes(g, c->J, 3, c->smooth_tmp, m, s);
+		launch_ghost_fill(g, c->J, 3, true, s);
+		launch_smooth_nodes(g, c->mu, 9, c->smooth_tmp, m, s);
+	}
+}
+
 double operator_C(const pic_ctx* c) { return (c->theta * c->cfg.dt) * (c->theta * c->cfg.dt) * kCprime2; }
 
 // Givens-updated least squares of the reference GMRES (PIC++/specialmath.cpp:487-600).
@@ -580,7 +602,7 @@ int gmres_cycle(pic_ctx* c, double* b, double* x, double target, int max_iterati
 		double* w = c->V[j + 1];
 		{
 			Timed t(c, "apply_operator", (double)c->g.nown * 15 * 8);
-			launch_apply_operator(c->g, c->muc, c->V[j], C, w, s);
+			launch_apply_operator(c->g, c->muh, c->V[j], C, w, s);
 		}
 		{
 			// algorithmic bytes: MGS reads v_m and w twice and writes w once per basis vector
@@ -639,6 +661,8 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		return PIC_ERR_ARGUMENT;
 	if (cfg->precision != PIC_PRECISION_FP64 && cfg->precision != PIC_PRECISION_FP32) return PIC_ERR_ARGUMENT;
 	if (cfg->pressure != PIC_PRESSURE_REFERENCE && cfg->pressure != PIC_PRESSURE_CELL) return PIC_ERR_ARGUMENT;
+	if (cfg->smoothing_count < 0 || (cfg->smoothing_count > 0 && !(cfg->smoothing_parameter >= 0.0 && cfg->smoothing_parameter <= 1.0)))
+		return PIC_ERR_ARGUMENT;
 	if (cfg->boundary_x != PIC_BOUNDARY_PERIODIC && cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
 	if (cfg->inject) {  // injection needs the wall set-up and createParticle's Maxwellian branch
 		if (cfg->inject != 1 || cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
@@ -769,6 +793,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->species_start_h.assign(kMaxSpecies + 1, 0);
 		HIP_CHECK(hipMemset(c->species_start, 0, (kMaxSpecies + 1) * sizeof(int)));
 		c->status = dalloc<int>(kStatusWords);
+		if (cfg->smoothing_count > 0) c->smooth_tmp = dalloc<double>(9 * g.nnodes);
 		c->dead = dalloc<int>(1);
 		HIP_CHECK(hipMemset(c->dead, 0, sizeof(int)));
 		if (g.wall) {
@@ -804,6 +829,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		long long len = 3 * g.nown;
 		c->rhs = dalloc<double>(len);
 		c->muc = dalloc<double>(9 * g.nown);
+		c->muh = dalloc<double>(9 * halo_count(g));
 		c->xsol = dalloc<double>(len);
 		c->work = dalloc<double>(len);
 		c->work2 = dalloc<double>(len);
@@ -812,10 +838,11 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		for (auto& v : c->V) v = dalloc<double>(len);
 		c->Vdev = dalloc<double*>(c->V.size());
 		HIP_CHECK(hipMemcpy(c->Vdev, c->V.data(), c->V.size() * sizeof(double*), hipMemcpyHostToDevice));
-		c->w64 = dalloc<double>(len);
+		c->w64 = dalloc<double>(3 * halo_count(g));
 		if (c->cfg.solver_precision == PIC_SOLVER_MIXED) {
 			c->mu32 = dalloc<float>(9 * g.nown);
-			c->w32 = dalloc<float>(len);
+			c->mu32h = dalloc<float>(9 * halo_count(g));
+			c->w32 = dalloc<float>(3 * halo_count(g));
 			c->V32.resize(c->max_basis + 1);
 			for (auto& v : c->V32) v = dalloc<float>(len);
 			c->V32dev = dalloc<float*>(c->V32.size());
@@ -849,7 +876,7 @@ void pic_destroy(pic_ctx* c) {
 	                (void*)c->divPi, (void*)c->rhohat, (void*)c->Pi, (void*)c->dens, (void*)c->node_scratch,
 	                (void*)c->rho_scratch, (void*)c->pi_scratch, (void*)c->dens_scratch, (void*)c->keys, (void*)c->keys_sorted,
 	                (void*)c->perm, (void*)c->perm_sorted, (void*)c->start, (void*)c->counts, c->cub_temp,
-	                (void*)c->species_start, (void*)c->status, (void*)c->dead, (void*)c->dens_flip, (void*)c->dens_zero, (void*)c->esc, (void*)c->esc_sorted, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
+	                (void*)c->species_start, (void*)c->status, (void*)c->dead, (void*)c->dens_flip, (void*)c->dens_zero, (void*)c->esc, (void*)c->esc_sorted, (void*)c->smooth_tmp, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
 	                (void*)c->Vdev, (void*)c->partial, (void*)c->dscal, (void*)c->ydev, (void*)c->epartial, (void*)c->eout})
 		f(p);
 	for (ParticleArrays* P : {&c->P, &c->Palt}) {
@@ -872,7 +899,7 @@ void pic_destroy(pic_ctx* c) {
 		f(p);
 	for (double* v : c->V) f(v);
 	for (float* v : c->V32) f(v);
-	for (void* p : {(void*)c->V32dev, (void*)c->mu32, (void*)c->w32, (void*)c->w64, (void*)c->tmp32, (void*)c->Z32dev,
+	for (void* p : {(void*)c->V32dev, (void*)c->mu32, (void*)c->mu32h, (void*)c->muh, (void*)c->w32, (void*)c->w64, (void*)c->tmp32, (void*)c->Z32dev,
 	                (void*)c->log_r, (void*)c->log_e, (void*)c->moved, (void*)c->Fscan, (void*)c->start_new, (void*)c->stay,
 	                (void*)c->arrive, (void*)c->first, (void*)c->mk, (void*)c->mi, (void*)c->mk_s, (void*)c->mi_s})
 		f(p);
@@ -1129,6 +1156,7 @@ int pic_rotation_and_moments(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	try {
 		moments(c);
+		smooth(c);
 	} catch (const DeviceError& e) {
 		return fail(c, e);
 	}
@@ -1141,6 +1169,7 @@ int pic_field_rhs(pic_ctx* c) {
 		Timed t(c, "rhs", (double)c->g.nown * (3 + 3 + 3 + 1 + 3 + 9 + 9) * 8);
 		launch_rhs(c->g, c->E, c->B, c->J, c->rhohat, c->theta, c->cfg.dt, c->rhs, c->stream);
 		launch_pack_mu(c->g, c->mu, c->muc, c->stream);
+		launch_pack_mu_halo(c->g, c->mu, c->muh, c->stream);
 	} catch (const DeviceError& e) {
 		return fail(c, e);
 	}
@@ -1174,6 +1203,7 @@ static void mixed_solve(pic_ctx* c, const double* b, double* x, pic_solve_info& 
 	double* d_rn = c->dscal + 5;
 	double* d_h = c->dscal + 8;
 	launch_to_f32(c->muc, c->mu32, 9 * g.nown, s);
+	launch_to_f32(c->muh, c->mu32h, 9 * halo_count(g), s);
 	if (c->logging && c->log_count < c->log_capacity) {
 		if (c->mu_scale <= 0) {  // the network's input scale of mu: its rms over the first logged solve
 			launch_dotT<float>(c->mu32, c->mu32, 9 * g.nown, c->partial, d_h, false, s);
@@ -1217,7 +1247,7 @@ static void mixed_solve(pic_ctx* c, const double* b, double* x, pic_solve_info& 
 			}
 			{
 				Timed t(c, "apply_operator32", (double)g.nown * 15 * 4);
-				launch_apply_factored<float>(g, c->mu32, dir, c->w32, C, w, s);
+				launch_apply_factored<float>(g, c->mu32h, dir, c->w32, C, w, s);
 			}
 			log_pair(c, w, dir);
 			{
@@ -1256,7 +1286,7 @@ static void mixed_solve(pic_ctx* c, const double* b, double* x, pic_solve_info& 
 		{
 			Timed t(c, "refinement", (double)len * 8 * (gv.y.size() / 2.0 + 2 + 15.0 / 3 + 3));
 			launch_update_x<float>(x, cycle_pre ? c->Z32dev : c->V32dev, c->ydev, (int)gv.y.size(), d_rn, len, s);  // x += ||r|| Z y
-			launch_apply_factored<double>(g, c->muc, x, c->w64, C, c->work2, s);  // A x in FP64
+			launch_apply_factored<double>(g, c->muh, x, c->w64, C, c->work2, s);  // A x in FP64
 			launch_residual(b, c->work2, c->work, len, s);                         // r = b - A x
 			launch_dotT<double>(c->work, c->work, len, c->partial, d_rn, true, s);
 		}
@@ -1319,7 +1349,7 @@ static int solve_impl(pic_ctx* c, double* b, double* x, pic_solve_info* info) {
 			inf.iterations += it;
 			inf.restarts += 1;
 			launch_add(x, c->work2, len, s);
-			launch_apply_operator(c->g, c->muc, x, C, c->work2, s);
+			launch_apply_operator(c->g, c->muh, x, C, c->work2, s);
 			launch_sub(c->work, b, c->work2, len, s);  // r = b - A x
 			launch_dot(c->work, c->work, len, c->partial, c->dscal + 5, true, s);
 			HIP_CHECK(hipMemcpyAsync(&rnorm, c->dscal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1365,7 +1395,8 @@ int pic_apply_operator(pic_ctx* c, const double* v, double* y) {
 	if (!c || !v || !y) return PIC_ERR_ARGUMENT;
 	try {
 		launch_pack_mu(c->g, c->mu, c->muc, c->stream);
-		launch_apply_operator(c->g, c->muc, v, operator_C(c), y, c->stream);
+		launch_pack_mu_halo(c->g, c->mu, c->muh, c->stream);
+		launch_apply_operator(c->g, c->muh, v, operator_C(c), y, c->stream);
 	} catch (const DeviceError& e) {
 		return fail(c, e);
 	}

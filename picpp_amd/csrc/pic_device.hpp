@@ -46,6 +46,14 @@ __host__ __device__ inline long long cell_id(const Grid& g, int i, int j, int k)
 __host__ __device__ inline long long own_id(const Grid& g, int i, int j, int k) {
 	return ((long long)i * g.n[1] + j) * g.n[2] + k;
 }
+// The operators' halo layout of node data: the Maxwell unknowns (nu0 x n1 x n2 owned nodes)
+// with one node on each side, halo index h <-> ghosted node h + 1 per axis.
+__host__ __device__ inline long long halo_count(const Grid& g) {
+	return (long long)(g.nu0 + 2) * (g.n[1] + 2) * (g.n[2] + 2);
+}
+__host__ __device__ inline long long halo_id(const Grid& g, int hi, int hj, int hk) {
+	return ((long long)hi * (g.n[1] + 2) + hj) * (g.n[2] + 2) + hk;
+}
 // periodic owned index (0..n-1) of a ghosted index
 __host__ __device__ inline int wrap_owned(int i, int n) {
 	int r = (i - 2) % n;

@@ -27,7 +27,15 @@ void launch_pressure_cell(const Grid& g, const double* pi_scratch, double* Pi, h
 void launch_pressure_cell32(const Grid& g, const float* pi_scratch, double* Pi, hipStream_t s);
 void launch_fold_nodes(const Grid& g, double* f, int ncomp, hipStream_t s);
 void launch_fold_cells(const Grid& g, double* f, int ncomp, int pairs, int lo0, hipStream_t s);
-void launch_ghost_fill(const Grid& g, double* f, int ncomp, bool nodes, hipStream_t s);
+// periodic ghost fill; with the wall, x ghosts behind it copy index 2 (or are zeroed with wall_zero)
+void launch_ghost_fill(const Grid& g, double* f, int ncomp, bool nodes, hipStream_t s, bool wall_zero = false);
+// smoothing passes (PIC++/smoothing.cpp): tmp holds at least ncomp node planes or one cell plane
+struct Smooth27 {
+	double m[27];
+};
+Smooth27 smoothing_matrix(double smoothing_parameter);
+void launch_smooth_cells(const Grid& g, double* f, double* tmp, const Smooth27& m, hipStream_t s);
+void launch_smooth_nodes(const Grid& g, double* f, int ncomp, double* tmp, const Smooth27& m, hipStream_t s);
 void launch_div_pressure_update_j(const Grid& g, const double* Pi, double* J, double* divPi, double eta, double dt, hipStream_t s);
 void launch_div_flux_update_rho(const Grid& g, const double* J, double* rho, double dt, double theta, hipStream_t s);
 
@@ -35,7 +43,9 @@ void launch_div_flux_update_rho(const Grid& g, const double* J, double* rho, dou
 void launch_rhs(const Grid& g, const double* E, const double* B, const double* J, const double* rho, double theta, double dt,
                 double* rhs, hipStream_t s);
 void launch_pack_mu(const Grid& g, const double* mu, double* muc, hipStream_t s);
-void launch_apply_operator(const Grid& g, const double* muc, const double* v, double C, double* out, hipStream_t s);
+// mu on the unknowns and one node around them (halo_count(g) nodes x 9), for the operators
+void launch_pack_mu_halo(const Grid& g, const double* mu, double* muh, hipStream_t s);
+void launch_apply_operator(const Grid& g, const double* muh, const double* v, double C, double* out, hipStream_t s);
 void launch_dot(const double* a, const double* b, long long len, double* partial, double* out, bool do_sqrt, hipStream_t s);
 void launch_axpy_dev(double* w, const double* v, const double* h, long long len, hipStream_t s);
 void launch_normalize_dev(double* w, const double* h, long long len, hipStream_t s);

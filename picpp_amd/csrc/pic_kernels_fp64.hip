@@ -315,7 +315,9 @@ __global__ void k_fold(Grid g, double* __restrict__ f, int ncomp, int axis, int 
 // (exchangeGeneralBfield, :629-637) arrays: every ghost takes its owned periodic image.
 // With the wall boundary the x ghosts behind the wall copy index 2 (fields_exchange.cpp:171-183,
 // :639-651) and the ghosts beyond the free right end keep their values (y and z stay periodic).
-__global__ void k_ghost_fill(Grid g, double* __restrict__ f, int ncomp, int nodes) {
+// With wall_zero the ghosts behind the wall are zeroed instead (exchangeGeneralScalarCellFieldX,
+// one-rank SUPER_CONDUCTOR_LEFT branch, fields_exchange.cpp:1468-1478).
+__global__ void k_ghost_fill(Grid g, double* __restrict__ f, int ncomp, int nodes, int wall_zero) {
 	long long len = nodes ? g.nnodes : g.ncells;
 	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= len) return;
@@ -325,11 +327,77 @@ __global__ void k_ghost_fill(Grid g, double* __restrict__ f, int ncomp, int node
 	int i = (int)(t / ((long long)e1 * e2));
 	bool ghost = i < 2 || j < 2 || k < 2 || i > g.n[0] + 1 || j > g.n[1] + 1 || k > g.n[2] + 1;
 	if (!ghost) return;
+	if (g.wall && wall_zero && i < 2) {
+		for (int c = 0; c < ncomp; ++c) f[c * len + t] = 0.0;
+		return;
+	}
 	int si = g.wall ? (i < 2 ? 2 : i) : wrap_owned(i, g.n[0]) + 2;
 	int sj = wrap_owned(j, g.n[1]) + 2, sk = wrap_owned(k, g.n[2]) + 2;
 	if (si == i && sj == j && sk == k) return;
 	long long src = nodes ? node_id(g, si, sj, sk) : cell_id(g, si, sj, sk);
 	for (int c = 0; c < ncomp; ++c) f[c * len + t] = f[c * len + src];
+}
+
+// ------------------------------------------------------------------ smoothing (PIC++/smoothing.cpp)
+// One pass of the 27-point kernel m = f(a) f(b) f(c), f = (s/2, 1-s, s/2), into `out` over the
+// reference's loop ranges; the boundary planes it leaves unsmoothed are copied. A second launch
+// with `back` set copies the pass back into f over the same range.
+// Cells (smoothCellParameter, smoothing.cpp:20-93): i in [2, N0-2], j, k in [2, N-3]; with the
+// wall the planes i = 2 and i = N0-2 are copied.
+__global__ void k_smooth_cells(Grid g, double* __restrict__ in, double* __restrict__ out, Smooth27 sm, int back) {
+	const int ni = g.N[0] - 3, nj = g.n[1], nk = g.n[2];
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long long)ni * nj * nk) return;
+	int k = 2 + (int)(t % nk);
+	int j = 2 + (int)((t / nk) % nj);
+	int i = 2 + (int)(t / ((long long)nk * nj));
+	long long c = cell_id(g, i, j, k);
+	if (back) {
+		in[c] = out[c];
+		return;
+	}
+	double acc;
+	if (!g.wall || (i > 2 && i < g.N[0] - 2)) {
+		acc = 0.0;
+		for (int a = -1; a <= 1; ++a)
+			for (int b = -1; b <= 1; ++b)
+				for (int d = -1; d <= 1; ++d) acc += in[cell_id(g, i + a, j + b, k + d)] * sm.m[(a + 1) * 9 + (b + 1) * 3 + (d + 1)];
+	} else {
+		acc = in[c];
+	}
+	out[c] = acc;
+}
+
+// Nodes (smoothVectorNodeParameter, smoothMatrixNodeParameter, smoothing.cpp:139-364): i in
+// [2, N0-2] (from 3 behind a wall), j, k in [2, N-2]; with the wall the plane i = N0-2 is copied.
+// ncomp component planes.
+__global__ void k_smooth_nodes(Grid g, double* __restrict__ in, double* __restrict__ out, int ncomp, Smooth27 sm, int back) {
+	const int i0 = g.wall ? 3 : 2;
+	const int ni = g.N[0] - 1 - i0, nj = g.n[1] + 1, nk = g.n[2] + 1;
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long long)ni * nj * nk * ncomp) return;
+	const long long pts = (long long)ni * nj * nk;
+	int comp = (int)(t / pts);
+	long long r = t % pts;
+	int k = 2 + (int)(r % nk);
+	int j = 2 + (int)((r / nk) % nj);
+	int i = i0 + (int)(r / ((long long)nk * nj));
+	const double* f = in + (long long)comp * g.nnodes;
+	long long o = (long long)comp * g.nnodes + node_id(g, i, j, k);
+	if (back) {
+		in[o] = out[o];
+		return;
+	}
+	double acc;
+	if (!(g.wall && i == g.N[0] - 2)) {
+		acc = 0.0;
+		for (int a = -1; a <= 1; ++a)
+			for (int b = -1; b <= 1; ++b)
+				for (int d = -1; d <= 1; ++d) acc += f[node_id(g, i + a, j + b, k + d)] * sm.m[(a + 1) * 9 + (b + 1) * 3 + (d + 1)];
+	} else {
+		acc = f[node_id(g, i, j, k)];
+	}
+	out[o] = acc;
 }
 
 // ------------------------------------------------------------------ wall boundary (PIC_BOUNDARY_WALL)
@@ -495,9 +563,24 @@ __global__ void k_pack_mu(Grid g, const double* __restrict__ mu, double* __restr
 	for (int q = 0; q < 9; ++q) muc[q * g.nown + o] = mu[q * g.nnodes + nid];
 }
 
+// mu in the halo layout: the ghosted array's values on the unknowns and one node around them.
+// The rows read mu at the neighbouring nodes, ghost nodes included (the reference indexes
+// dielectricTensor[i +- 1][j][k] directly). Those hold the periodic images after the folds,
+// and keep their values through the smoothing passes, which refresh no mu ghosts.
+__global__ void k_pack_mu_halo(Grid g, const double* __restrict__ mu, double* __restrict__ muh) {
+	const long long H = halo_count(g);
+	long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (h >= H) return;
+	int hk = (int)(h % (g.n[2] + 2));
+	int hj = (int)((h / (g.n[2] + 2)) % (g.n[1] + 2));
+	int hi = (int)(h / ((long long)(g.n[2] + 2) * (g.n[1] + 2)));
+	long long nid = node_id(g, hi + 1, hj + 1, hk + 1);
+	for (int q = 0; q < 9; ++q) muh[q * H + h] = mu[q * g.nnodes + nid];
+}
+
 // The implicit Maxwell operator applied matrix-free on the compact periodic layout:
 // the 37 entries of createInternalEquation{X,Y,Z} (fields.cpp:1396-1509, :1961-2072,
-// :2521-2632), in push_back order, with mu read at the neighbouring nodes.
+// :2521-2632), in push_back order, with mu read at the neighbouring nodes (halo layout).
 struct Neigh {
 	int i[3], j[3], k[3];  // owned index of offset -1, 0, +1 per axis
 };
@@ -506,7 +589,7 @@ __device__ inline long long oidx(const Grid& g, const Neigh& nb, int di, int dj,
 	return ((long long)nb.i[di + 1] * g.n[1] + nb.j[dj + 1]) * g.n[2] + nb.k[dk + 1];
 }
 
-__global__ void k_apply_operator(Grid g, const double* __restrict__ muc, const double* __restrict__ v, double C,
+__global__ void k_apply_operator(Grid g, const double* __restrict__ muh, const double* __restrict__ v, double C,
                                  double* __restrict__ out) {
 	long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (o >= g.nown) return;
@@ -532,7 +615,9 @@ __global__ void k_apply_operator(Grid g, const double* __restrict__ muc, const d
 	nb.k[1] = k;
 	nb.k[2] = k == g.n[2] - 1 ? 0 : k + 1;
 	const long long L = g.nown;
-	auto M = [&](long long id, int l, int m) { return muc[(l * 3 + m) * L + id]; };
+	const long long H = halo_count(g);
+	auto M = [&](const int* off, int l, int m) { return muh[(l * 3 + m) * H + halo_id(g, i + 1 + off[0], j + 1 + off[1], k + 1 + off[2])]; };
+	const int here[3] = {0, 0, 0};
 	auto V = [&](long long id, int l) { return v[l * L + id]; };
 	double D2[3] = {g.d[0] * g.d[0], g.d[1] * g.d[1], g.d[2] * g.d[2]};
 	const int others[3][2] = {{1, 2}, {0, 2}, {0, 1}};
@@ -543,13 +628,13 @@ __global__ void k_apply_operator(Grid g, const double* __restrict__ muc, const d
 	for (int c = 0; c < 3; ++c) {
 		double r = 0.0;
 		// diagonal (same node, same component)
-		double e = 1.0 - M(o, c, c);
+		double e = 1.0 - M(here, c, c);
 		for (int ax = 0; ax < 3; ++ax) {
 			if (ax == c) {
 				if (c == 0)
-					e += C * ((2.0 - 2 * M(o, c, c)) / D2[ax]);
+					e += C * ((2.0 - 2 * M(here, c, c)) / D2[ax]);
 				else
-					e += C * (2.0 - 2 * M(o, c, c)) / D2[ax];
+					e += C * (2.0 - 2 * M(here, c, c)) / D2[ax];
 			} else {
 				if (c == 0 || (c == 1 && ax == 2) || (c == 2 && ax == 1))
 					e += C * 2.0 / D2[ax];
@@ -560,8 +645,8 @@ __global__ void k_apply_operator(Grid g, const double* __restrict__ muc, const d
 		r += e * V(o, c);
 		for (int t = 0; t < 2; ++t) {
 			int m = others[c][t];
-			double val = -M(o, c, m);
-			val -= C * 2 * M(o, c, m) / D2[c];
+			double val = -M(here, c, m);
+			val -= C * 2 * M(here, c, m) / D2[c];
 			r += val * V(o, m);
 		}
 		for (int s = 1; s >= -1; s -= 2) {
@@ -570,7 +655,7 @@ __global__ void k_apply_operator(Grid g, const double* __restrict__ muc, const d
 			long long q = oidx(g, nb, off[0], off[1], off[2]);
 			for (int t = 0; t < 3; ++t) {
 				int m = face_order[c][t];
-				double val = (m == c) ? C * (-1.0 + M(q, c, c)) / D2[c] : C * M(q, c, m) / D2[c];
+				double val = (m == c) ? C * (-1.0 + M(off, c, c)) / D2[c] : C * M(off, c, m) / D2[c];
 				r += val * V(q, m);
 			}
 		}
@@ -608,7 +693,7 @@ __global__ void k_apply_operator(Grid g, const double* __restrict__ muc, const d
 				long long q = oidx(g, nb, off[0], off[1], off[2]);
 				bool pos = sgn[cr][0] * sgn[cr][1] > 0;
 				for (int m = 0; m < 3; ++m) {
-					double val = pos ? C * M(q, ax, m) / den : -C * M(q, ax, m) / den;
+					double val = pos ? C * M(off, ax, m) / den : -C * M(off, ax, m) / den;
 					r += val * V(q, m);
 				}
 			}
@@ -1196,8 +1281,27 @@ void launch_fold_cells(const Grid& g, double* f, int ncomp, int pairs, int lo0, 
 		hipLaunchKernelGGL(k_fold, dim3(blocks(total, 256)), dim3(256), 0, s, g, f, ncomp, axis, 0, pairs, g.N[axis] - 1, lo0);
 	}
 }
-void launch_ghost_fill(const Grid& g, double* f, int ncomp, bool nodes, hipStream_t s) {
-	hipLaunchKernelGGL(k_ghost_fill, dim3(blocks(nodes ? g.nnodes : g.ncells, 256)), dim3(256), 0, s, g, f, ncomp, nodes ? 1 : 0);
+void launch_ghost_fill(const Grid& g, double* f, int ncomp, bool nodes, hipStream_t s, bool wall_zero) {
+	hipLaunchKernelGGL(k_ghost_fill, dim3(blocks(nodes ? g.nnodes : g.ncells, 256)), dim3(256), 0, s, g, f, ncomp, nodes ? 1 : 0,
+	                   wall_zero ? 1 : 0);
+}
+Smooth27 smoothing_matrix(double sp) {
+	Smooth27 m;
+	const double f[3] = {sp / 2, 1 - sp, sp / 2};  // createSmoothingMatrix, smoothing.cpp:105-137
+	for (int a = 0; a < 3; ++a)
+		for (int b = 0; b < 3; ++b)
+			for (int c = 0; c < 3; ++c) m.m[a * 9 + b * 3 + c] = f[a] * f[b] * f[c];
+	return m;
+}
+void launch_smooth_cells(const Grid& g, double* f, double* tmp, const Smooth27& m, hipStream_t s) {
+	long long n = (long long)(g.N[0] - 3) * g.n[1] * g.n[2];
+	hipLaunchKernelGGL(k_smooth_cells, dim3(blocks(n, 256)), dim3(256), 0, s, g, f, tmp, m, 0);
+	hipLaunchKernelGGL(k_smooth_cells, dim3(blocks(n, 256)), dim3(256), 0, s, g, f, tmp, m, 1);
+}
+void launch_smooth_nodes(const Grid& g, double* f, int ncomp, double* tmp, const Smooth27& m, hipStream_t s) {
+	long long n = (long long)(g.N[0] - 1 - (g.wall ? 3 : 2)) * (g.n[1] + 1) * (g.n[2] + 1) * ncomp;
+	hipLaunchKernelGGL(k_smooth_nodes, dim3(blocks(n, 256)), dim3(256), 0, s, g, f, tmp, ncomp, m, 0);
+	hipLaunchKernelGGL(k_smooth_nodes, dim3(blocks(n, 256)), dim3(256), 0, s, g, f, tmp, ncomp, m, 1);
 }
 void launch_div_pressure_update_j(const Grid& g, const double* Pi, double* J, double* divPi, double eta, double dt, hipStream_t s) {
 	hipLaunchKernelGGL(k_div_pressure_update_j, dim3(blocks(g.nnodes, 256)), dim3(256), 0, s, g, Pi, J, divPi, eta, dt);
@@ -1212,8 +1316,11 @@ void launch_rhs(const Grid& g, const double* E, const double* B, const double* J
 void launch_pack_mu(const Grid& g, const double* mu, double* muc, hipStream_t s) {
 	hipLaunchKernelGGL(k_pack_mu, dim3(blocks(g.nown, 256)), dim3(256), 0, s, g, mu, muc);
 }
-void launch_apply_operator(const Grid& g, const double* muc, const double* v, double C, double* out, hipStream_t s) {
-	hipLaunchKernelGGL(k_apply_operator, dim3(blocks(g.nown, 256)), dim3(256), 0, s, g, muc, v, C, out);
+void launch_apply_operator(const Grid& g, const double* muh, const double* v, double C, double* out, hipStream_t s) {
+	hipLaunchKernelGGL(k_apply_operator, dim3(blocks(g.nown, 256)), dim3(256), 0, s, g, muh, v, C, out);
+}
+void launch_pack_mu_halo(const Grid& g, const double* mu, double* muh, hipStream_t s) {
+	hipLaunchKernelGGL(k_pack_mu_halo, dim3(blocks(halo_count(g), 256)), dim3(256), 0, s, g, mu, muh);
 }
 void launch_dot(const double* a, const double* b, long long len, double* partial, double* out, bool do_sqrt, hipStream_t s) {
 	hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(256), 0, s, a, b, len, partial);

@@ -15,14 +15,25 @@
 
 namespace picgpu {
 
+// w = mu v on the halo layout: mu of the node (ghost nodes included), v of its periodic image
+// among the unknowns. The stencil then reads w at the neighbouring nodes without a wrap.
 template <class T>
 __global__ void k_mu_times(Grid g, const T* __restrict__ mu, const T* __restrict__ v, T* __restrict__ w) {
-	long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (o >= g.nown) return;
+	const long long H = halo_count(g);
+	long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (h >= H) return;
+	const int n0 = g.nu0, n1 = g.n[1], n2 = g.n[2];
+	int hk = (int)(h % (n2 + 2));
+	int hj = (int)((h / (n2 + 2)) % (n1 + 2));
+	int hi = (int)(h / ((long long)(n2 + 2) * (n1 + 2)));
+	int i = hi == 0 ? n0 - 1 : (hi == n0 + 1 ? 0 : hi - 1);
+	int j = hj == 0 ? n1 - 1 : (hj == n1 + 1 ? 0 : hj - 1);
+	int k = hk == 0 ? n2 - 1 : (hk == n2 + 1 ? 0 : hk - 1);
 	const long long L = g.nown;
+	const long long o = ((long long)i * n1 + j) * n2 + k;
 	T e0 = v[o], e1 = v[L + o], e2 = v[2 * L + o];
 #pragma unroll
-	for (int c = 0; c < 3; ++c) w[c * L + o] = mu[(c * 3 + 0) * L + o] * e0 + mu[(c * 3 + 1) * L + o] * e1 + mu[(c * 3 + 2) * L + o] * e2;
+	for (int c = 0; c < 3; ++c) w[c * H + h] = mu[(c * 3 + 0) * H + h] * e0 + mu[(c * 3 + 1) * H + h] * e1 + mu[(c * 3 + 2) * H + h] * e2;
 }
 
 template <class T>
@@ -47,25 +58,28 @@ __global__ void __launch_bounds__(256) k_stencil(Grid g, const T* __restrict__ v
 	int K[3] = {k == 0 ? n2 - 1 : k - 1, k, k == n2 - 1 ? 0 : k + 1};
 	auto id = [&](int a, int b, int c) { return ((long long)I[a + 1] * n1 + Jj[b + 1]) * n2 + K[c + 1]; };
 	const long long L = g.nown;
+	const long long H = halo_count(g);
+	auto hid = [&](int a, int b, int c) { return halo_id(g, i + 1 + a, j + 1 + b, k + 1 + c); };
 	T id2[3] = {(T)(1.0 / (g.d[0] * g.d[0])), (T)(1.0 / (g.d[1] * g.d[1])), (T)(1.0 / (g.d[2] * g.d[2]))};
 	T q4[3][3];
 	for (int a = 0; a < 3; ++a)
 		for (int b = 0; b < 3; ++b) q4[a][b] = (T)(1.0 / (4.0 * g.d[a] * g.d[b]));
 	long long nb[3][2] = {{id(-1, 0, 0), id(1, 0, 0)}, {id(0, -1, 0), id(0, 1, 0)}, {id(0, 0, -1), id(0, 0, 1)}};
+	long long hb[3][2] = {{hid(-1, 0, 0), hid(1, 0, 0)}, {hid(0, -1, 0), hid(0, 1, 0)}, {hid(0, 0, -1), hid(0, 0, 1)}};
 #pragma unroll
 	for (int c = 0; c < 3; ++c) {
 		const T* vc = v + c * L;
-		const T* wc = w + c * L;
-		T vo = vc[o], wo = wc[o];
+		const T* wc = w + c * H;
+		T vo = vc[o], wo = wc[hid(0, 0, 0)];
 		T lap = 0;
 #pragma unroll
 		for (int a = 0; a < 3; ++a) lap += (vc[nb[a][0]] - 2 * vo + vc[nb[a][1]]) * id2[a];
-		T d2w = (wc[nb[c][0]] - 2 * wo + wc[nb[c][1]]) * id2[c];
+		T d2w = (wc[hb[c][0]] - 2 * wo + wc[hb[c][1]]) * id2[c];
 		T cross = 0;
 #pragma unroll
 		for (int d = 0; d < 3; ++d) {
 			if (d == c) continue;
-			const T* wd = w + d * L;
+			const T* wd = w + d * H;
 			int pp[3] = {0, 0, 0}, mm[3] = {0, 0, 0}, mp[3] = {0, 0, 0}, pm[3] = {0, 0, 0};
 			pp[c] = 1;
 			pp[d] = 1;
@@ -75,8 +89,8 @@ __global__ void __launch_bounds__(256) k_stencil(Grid g, const T* __restrict__ v
 			mp[d] = 1;
 			pm[c] = 1;
 			pm[d] = -1;
-			cross += (wd[id(pp[0], pp[1], pp[2])] + wd[id(mm[0], mm[1], mm[2])] - wd[id(mp[0], mp[1], mp[2])] -
-			          wd[id(pm[0], pm[1], pm[2])]) * q4[c][d];
+			cross += (wd[hid(pp[0], pp[1], pp[2])] + wd[hid(mm[0], mm[1], mm[2])] - wd[hid(mp[0], mp[1], mp[2])] -
+			          wd[hid(pm[0], pm[1], pm[2])]) * q4[c][d];
 		}
 		out[c * L + o] = vo - wo - C * lap + C * (d2w + cross);
 	}
@@ -159,7 +173,7 @@ inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs)
 
 template <class T>
 void launch_apply_factored(const Grid& g, const T* mu, const T* v, T* w, double C, T* out, hipStream_t s) {
-	hipLaunchKernelGGL(k_mu_times<T>, dim3(nblk(g.nown, 256)), dim3(256), 0, s, g, mu, v, w);
+	hipLaunchKernelGGL(k_mu_times<T>, dim3(nblk(halo_count(g), 256)), dim3(256), 0, s, g, mu, v, w);
 	hipLaunchKernelGGL(k_stencil<T>, dim3(nblk(g.nown, 256)), dim3(256), 0, s, g, v, w, (T)C, out);
 }
 template <class T>

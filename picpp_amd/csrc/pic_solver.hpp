@@ -8,7 +8,8 @@ namespace picgpu {
 
 constexpr int kSolverDotBlocks = 1024;
 
-// out = A v in factored form: w = mu v (scratch), then the stencil on v and w.
+// out = A v in factored form: w = mu v on the halo layout (scratch of 3 halo_count(g) values,
+// mu in the halo layout of launch_pack_mu_halo), then the stencil on v and w.
 template <class T>
 void launch_apply_factored(const Grid& g, const T* mu, const T* v, T* w, double C, T* out, hipStream_t s);
 template <class T>

@@ -57,6 +57,8 @@ class Simulation:
         cfg.solver_precision = SOLVERS[solver]
         cfg.boundary_x = _lib.BOUNDARY_WALL if workload.boundary == "wall" else _lib.BOUNDARY_PERIODIC
         cfg.pressure = {"reference": _lib.PRESSURE_REFERENCE, "cell": _lib.PRESSURE_CELL}[pressure]
+        cfg.smoothing_count = workload.smoothing_count
+        cfg.smoothing_parameter = workload.smoothing_parameter
         for a in range(3):
             cfg.e_right[a] = workload.e_right[a]
             cfg.b_right[a] = workload.b_right[a]
@@ -153,7 +155,8 @@ class Simulation:
 
     def updateElectroMagneticParameters(self):
         """evaluateParticlesRotationTensor + updateElectroMagneticParameters
-        (PIC++/mover.cpp:810-841, PIC++/momentum_collecting.cpp:365-758)."""
+        (PIC++/mover.cpp:810-841, PIC++/momentum_collecting.cpp:365-758), then the workload's
+        smoothing passes (PIC++/simulation.cpp:213-217)."""
         self._chk(self.lib.pic_rotation_and_moments(self.h), "pic_rotation_and_moments")
 
     evaluateParticlesRotationTensor = updateElectroMagneticParameters

@@ -69,6 +69,10 @@ class Workload:
     # wall boundary only: inject the flow at the right end each step and replace particles that
     # leave there (PIC++/mover.cpp:910-1000, initialization.cpp:8281-8338)
     inject: bool = False
+    # smoothing of rhohat, J and the dielectric tensor after the moments (smoothingCount passes
+    # of the 27-point kernel with weights (s/2, 1-s, s/2) per axis, PIC++/smoothing.cpp)
+    smoothing_count: int = 0
+    smoothing_parameter: float = 0.5
 
     @property
     def N(self) -> Tuple[int, int, int]:

@@ -15,6 +15,8 @@ fixtures next to this script:
                    +x, so that they leave through the free right end
   inject_ref.npz   the escape case with the flow injected at the right end every step and
                    every particle leaving there replaced (the reference's own injection)
+  smooth_ref.npz, smooth_wall_ref.npz  the mini case and the escape case with 8 smoothing
+                   passes of s = 0.5 (the shipped input.dat's setting) every step
   wall_long_ref.npz the shock set-up (gamma 10 e-/p+ flow onto the wall, mass ratio 64,
                    sigma 0.1) at 16x8x8 cells, 8 ppc per species, 150 steps of dt = 0.1, no
                    injection: energies and E, B, rho norms every step, a particle sample at the end
@@ -68,6 +70,7 @@ def write_input(path, w, st, steps):
                     f.write(struct.pack("<id", sp.ppc, sp.temperature))
         else:
             f.write(struct.pack("<i", 0))
+        f.write(struct.pack("<id", w.smoothing_count, w.smoothing_parameter))
 
 
 def read_output(path):
@@ -178,6 +181,20 @@ def make_wall_long():
     np.savez_compressed(os.path.join(HERE, "wall_long_ref.npz"), **keep)
 
 
+def smooth_cases():
+    """The mini case and the wall case with the shipped input's smoothing (8 passes, s = 0.5)."""
+    w = dataclasses.replace(wl.mini(), smoothing_count=8, smoothing_parameter=0.5, name="mini-smooth")
+    ww, wst = wall_out_case()
+    ww = dataclasses.replace(ww, smoothing_count=8, smoothing_parameter=0.5, name="wall-smooth")
+    return (w, wl.make_initial_state(w, perturb=0.05)), (ww, wst)
+
+
+def make_smooth():
+    for (w, st), fname in zip(smooth_cases(), ("smooth_ref.npz", "smooth_wall_ref.npz")):
+        out = run(w, st, w.steps)
+        np.savez_compressed(os.path.join(HERE, fname), **inputs(st), **out)
+
+
 def make_config1():
     w = wl.config1()
     st = wl.make_initial_state(w)
@@ -203,5 +220,5 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref", "-j8"], check=True)
     for name in which:
-        {"mini": make_mini, "config1": make_config1, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject, "wall_long": make_wall_long}[name]()
+        {"mini": make_mini, "config1": make_config1, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject, "wall_long": make_wall_long, "smooth": make_smooth}[name]()
         print("wrote", name)
