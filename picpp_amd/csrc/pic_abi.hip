@@ -148,7 +148,8 @@ struct pic_ctx {
 
 namespace {
 
-// status words: [0] particle check bits, [1] record claimed, [2..25] the FP32 mover's record
+// status words: [0] particle check bits, [1] record claimed, [2..4] the FP32 mover's first failing
+// particle: failed checks, index, species
 constexpr int kStatusWords = 32;
 
 int fail(pic_ctx* c, const DeviceError& e) {
@@ -203,16 +204,24 @@ void check_status(pic_ctx* c) {
 		if (st[0] & 2) why += " particle maps inconsistent (moved more than half a cell?);";
 		if (st[0] & 4) why += " particle outside the owned region;";
 		if (st[0] & 8) why += " particle stencil outside the ghost layers;";
-		if (st[1]) {  // the FP32 mover's record of the first failing particle
-			float d[24];
-			std::memcpy(d, st + 2, sizeof(d));
-			char buf[640];
+		if (st[1] && c->fp32) {  // the FP32 mover's first failing particle, still in its pre-push state
+			const long long i = st[3];
+			int cell = 0;
+			float off[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+			if (i >= 0 && i < c->Q.count) {
+				HIP_CHECK(hipMemcpy(&cell, c->Q.cell + i, sizeof(int), hipMemcpyDeviceToHost));
+				for (int a = 0; a < 3; ++a) {
+					HIP_CHECK(hipMemcpy(&off[a], c->Q.off[a] + i, sizeof(float), hipMemcpyDeviceToHost));
+					HIP_CHECK(hipMemcpy(&u[a], c->Q.u[a] + i, sizeof(float), hipMemcpyDeviceToHost));
+				}
+			}
+			const Grid& g = c->g;
+			const int ck = cell % g.N[2], cj = (cell / g.N[2]) % g.N[1], ci = cell / (g.N[2] * g.N[1]);
+			char buf[512];
 			snprintf(buf, sizeof(buf),
-			         " first: checks %d, index %d, species %d, reflected %d, cell (%d %d %d), offset (%.9g %.9g %.9g), "
-			         "u (%.9g %.9g %.9g), new position (%.9g %.9g %.9g) cells from the cell, new u (%.9g %.9g %.9g), "
-			         "mid v (%.9g %.9g %.9g), E_x %.6g, B_z %.6g",
-			         (int)d[0], (int)d[1], (int)d[2], (int)d[3], (int)d[4], (int)d[5], (int)d[6], d[7], d[8], d[9], d[10], d[11],
-			         d[12], d[13], d[14], d[15], d[16], d[17], d[18], d[19], d[20], d[21], d[22], d[23]);
+			         " first: checks %d (1 start stencil, 2 Picard stencil, 4/8/16 new cell x/y/z), index %lld, species %d, "
+			         "cell (%d %d %d), offset (%.9g %.9g %.9g), u (%.9g %.9g %.9g)",
+			         st[2], i, st[4], ci, cj, ck, off[0], off[1], off[2], u[0], u[1], u[2]);
 			why += buf;
 		}
 		throw DeviceError{PIC_ERR_PARTICLE, "particle check failed:" + why};

@@ -556,6 +556,7 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 // ------------------------------------------------------------------ mover
 // moveParticle (PIC++/mover.cpp:194-491) on cell offsets, one thread per particle; alpha is
 // recomputed from E^n, B^n at x^n. Momentum-space constants: du = (q dt / m) (E + v x B).
+template <bool WALL>
 __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start,
                                                 const float4* __restrict__ E, const float4* __restrict__ Et,
                                                 const float4* __restrict__ B, float dt, float eta, uint32_t* __restrict__ keys,
@@ -623,7 +624,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		unew[2] = nup[2] * sc;
 		reflected = true;
 	};
-	if (g.wall && xh[0] < (float)(2 - cc[0])) reflect_at_wall(xh);
+	if (WALL && xh[0] < (float)(2 - cc[0])) reflect_at_wall(xh);
 	float vhat[3];
 #pragma unroll
 	for (int r = 0; r < 3; ++r) vhat[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
@@ -643,7 +644,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 			mid[c] = vhat[c] + rE[c] * beta;
 			nx[c] = f[c] + mid[c] * (eta * dt) * inv_d[c];
 		}
-		if (g.wall && nx[0] < (float)(2 - cc[0])) {
+		if (WALL && nx[0] < (float)(2 - cc[0])) {
 			reflect_at_wall(nx);
 			break;
 		}
@@ -666,7 +667,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 			pos[c] = f[c] + mid[c] * dt * inv_d[c];
 			unew[c] = nu[c];
 		}
-		if (g.wall && pos[0] < (float)(2 - cc[0])) {  // final position behind the wall (mover.cpp:435-441)
+		if (WALL && pos[0] < (float)(2 - cc[0])) {  // final position behind the wall (mover.cpp:435-441)
 			pos[0] = (float)(4 - 2 * cc[0]) - pos[0];
 			unew[0] = -unew[0];
 		}
@@ -674,7 +675,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 	// Out through the free right end: removed (mover.cpp:447-458). The reference removes x > node
 	// n+2; a position exactly on that node is removed here too, so that every kept particle has
 	// an owned cell.
-	if (g.wall && pos[0] >= (float)(g.n[0] + 2 - cc[0])) {
+	if (WALL && pos[0] >= (float)(g.n[0] + 2 - cc[0])) {
 		keys[i] = 0xFFFFFFFFu;
 		moved[i] = 1;
 		// listed for replacement, with its absolute y, z in cell units
@@ -684,7 +685,6 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		return;
 	}
 	// x^{n+1} = x^n + mid dt, carried into the integer cell; one-rank periodic wrap (x too unless walled)
-	const float f0[3] = {f[0], f[1], f[2]};
 	int nc[3];
 #pragma unroll
 	for (int c = 0; c < 3; ++c) {
@@ -696,7 +696,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 			fl += 1.0f;
 		}
 		int ci = cc[c] + (int)fl;
-		const bool walled = g.wall && c == 0;
+		const bool walled = WALL && c == 0;
 		if (!walled && ci < 2) ci += g.n[c];
 		if (!walled && ci > g.n[c] + 1) ci -= g.n[c];
 		if (ci < 2 || ci > g.n[c] + 1) why |= 4 << c;
@@ -704,16 +704,16 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		f[c] = x;
 	}
 	if (why) {
+		// A failed check ends the step with an error. The first failing particle keeps its state
+		// from before the push, and status[2..4] hold the failed checks, its index and species
+		// for the host's message.
 		atomicOr(status, 8);
-		// the first failing particle's state for the host's error message: status[1] claims the
-		// record, status[2..] holds it
 		if (atomicCAS(status + 1, 0, 1) == 0) {
-			float* d = reinterpret_cast<float*>(status + 2);
-			const float rec[24] = {(float)why, (float)i, (float)s, (float)reflected, (float)cc[0], (float)cc[1], (float)cc[2],
-			                       f0[0], f0[1], f0[2], u[0], u[1], u[2], pos[0], pos[1], pos[2], unew[0], unew[1], unew[2],
-			                       mid[0], mid[1], mid[2], Ef[0], Bf[2]};
-			for (int q = 0; q < 24; ++q) d[q] = rec[q];
+			status[2] = why;
+			status[3] = i;
+			status[4] = s;
 		}
+		return;
 	}
 	int cell = (int)cell_id(g, nc[0], nc[1], nc[2]);
 	P.cell[i] = cell;
@@ -1028,7 +1028,7 @@ void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, int* dead,
                    uint32_t* esc, hipStream_t s) {
 	if (P.count == 0) return;
-	hipLaunchKernelGGL(k_push32, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start,
+	hipLaunchKernelGGL(g.wall ? k_push32<true> : k_push32<false>, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start,
 	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(Et), reinterpret_cast<const float4*>(B), dt,
 	                   eta, keys, moved, status, dead, esc);
 }

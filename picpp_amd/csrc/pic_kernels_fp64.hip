@@ -854,6 +854,7 @@ __device__ inline double dnorm(const double a[3], const double b[3]) {
 // moveParticle (PIC++/mover.cpp:194-491) with the one-rank periodic wrap of exchangeParticles
 // (mover.cpp:1007-1021), followed by the new sort key. alpha is recomputed from E^n, B^n at x^n
 // exactly as evaluateParticlesRotationTensor computed it. One thread per particle.
+template <bool WALL>
 __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start, const double* __restrict__ E,
                        const double* __restrict__ Etheta, const double* __restrict__ B, double dt, double eta,
                        uint32_t* __restrict__ keys, int* __restrict__ status, int* __restrict__ dead,
@@ -907,7 +908,7 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 		}
 		reflected = true;
 	};
-	if (g.wall && x[0] < xw) reflect_at_wall();
+	if (WALL && x[0] < xw) reflect_at_wall();
 	double vhat[3];
 	for (int c = 0; c < 3; ++c) vhat[c] = (a[c * 3 + 0] * gm) * v[0] + (a[c * 3 + 1] * gm) * v[1] + (a[c * 3 + 2] * gm) * v[2];
 	double prevP[3] = {p[0], p[1], p[2]};
@@ -928,7 +929,7 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 		for (int c = 0; c < 3; ++c) mid[c] = vhat[c] + rE[c] * beta;
 		for (int c = 0; c < 3; ++c) x[c] = oldX[c] + (mid[c] * etaDt);
 		for (int c = 0; c < 3; ++c) newX[c] = x[c];
-		if (g.wall && x[0] < xw) {
+		if (WALL && x[0] < xw) {
 			reflect_at_wall();
 			break;
 		}
@@ -946,12 +947,12 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 			p[c] = newP[c];
 			x[c] = oldX[c] + mid[c] * dt;
 		}
-		if (g.wall && x[0] < xw) {  // final position behind the wall (mover.cpp:435-441)
+		if (WALL && x[0] < xw) {  // final position behind the wall (mover.cpp:435-441)
 			x[0] = 2 * xw - x[0];
 			p[0] = -p[0];
 		}
 	}
-	if (g.wall) {
+	if (WALL) {
 		// leaving through the free right end removes the particle (mover.cpp:447-458, exchangeParticles
 		// keeps no right escapes without periodicity, :1007-1021)
 		const double xr = g.x0[0] + (g.N[0] - 2) * g.d[0];
@@ -966,7 +967,7 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 		}
 	}
 	// one-rank periodic wrap (mover.cpp:435-490 classification, :1007-1021 shift)
-	for (int c = g.wall ? 1 : 0; c < 3; ++c) {
+	for (int c = WALL ? 1 : 0; c < 3; ++c) {
 		double lo = g.x0[c] + 2 * g.d[c];
 		double hi = g.x0[c] + (g.N[c] - 1 - 1) * g.d[c];
 		if (x[c] < lo)
@@ -1354,7 +1355,8 @@ void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P,
                  const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, int* dead,
                  uint32_t* esc, hipStream_t s) {
 	if (P.count == 0) return;
-	hipLaunchKernelGGL(k_push, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, E, Etheta, B, dt, eta, keys,
+	hipLaunchKernelGGL(g.wall ? k_push<true> : k_push<false>, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, E,
+	                   Etheta, B, dt, eta, keys,
 	                   status, dead, esc);
 }
 void launch_wall_mirror_cells(const Grid& g, double* f, int ncomp, const int* flip, hipStream_t s) {
