@@ -150,9 +150,22 @@ __device__ inline void alpha32(float beta, const float v[3], float gamma, const 
 	a[8] = (1.0f + b2 * B[2] * B[2]) * inv;
 }
 
-__device__ inline float group_sum(float v, int G) {
-	for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-	return v;
+// Sums 8 per-lane values over each group of G consecutive lanes (8 <= G, G divides the
+// workgroup) through LDS: group lane a < 8 gets the sum of value a over lanes 0, 1, ..., G-1,
+// in that order, so the sums are deterministic. Every thread of the workgroup must call it;
+// red holds 9 floats per thread (row stride 9 spreads the column reads over the LDS banks).
+__device__ inline float group_sum8(const float v[8], float* red, int lt, int lane, int G) {
+	__syncthreads();  // the previous chunk's reads are done
+	float* row = red + lt * 9;
+#pragma unroll
+	for (int q = 0; q < 8; ++q) row[q] = v[q];
+	__syncthreads();
+	float s = 0.0f;
+	if (lane < 8) {
+		const float* col = red + (lt - lane) * 9 + lane;
+		for (int l = 0; l < G; ++l) s += col[l * 9];
+	}
+	return s;
 }
 
 __device__ inline void load32(const ParticleArrays32& P, int i, float f[3], float u[3]) {
@@ -196,6 +209,7 @@ __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp,
                                                         const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
                                                         float theta, int G, int T, float* __restrict__ out) {
 	extern __shared__ float4 Bn[];  // 3 x 3 x (T+2) cells around the tile
+	__shared__ float red[256 * 9];
 	const DualTile tile = dual_tile(g, T);
 	const int lt = threadIdx.x;
 	const int nk = T + 2;
@@ -280,16 +294,14 @@ __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp,
 			}
 		}
 	}
+	const long long c = c_first + kb;
 #pragma unroll
-	for (int a = 0; a < 8; ++a)
+	for (int q = 0; q < 12; ++q) {
+		float v8[8];
 #pragma unroll
-		for (int q = 0; q < 12; ++q) acc[a][q] = group_sum(acc[a][q], G);
-	if (lane == 0 && cell_ok) {
-		const long long c = c_first + kb;
-#pragma unroll
-		for (int a = 0; a < 8; ++a)
-#pragma unroll
-			for (int q = 0; q < 12; ++q) out[((long long)q * 8 + a) * g.ncells + c] = acc[a][q];
+		for (int a = 0; a < 8; ++a) v8[a] = acc[a][q];
+		const float sum = group_sum8(v8, red, lt, lane, G);
+		if (cell_ok && lane < 8) out[((long long)q * 8 + lane) * g.ncells + c] = sum;
 	}
 }
 
@@ -353,6 +365,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                                                         int G, int Tj, int Tk, float* __restrict__ rho_out,
                                                         float* __restrict__ pi_out) {
 	extern __shared__ float4 En[];  // 3 x (Tj+2) x (Tk+2) nodes from the tile's base corner
+	__shared__ float red[256 * 9];
 	const PlaneTile tile = plane_tile(g, Tj, Tk);
 	const int lt = threadIdx.x;
 	const int T = Tj * Tk, mj = Tj + 2, mk = Tk + 2;
@@ -445,20 +458,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 			}
 		}
 	}
-#pragma unroll
-	for (int q = 0; q < 8; ++q) {
-		ar[q] = group_sum(ar[q], G);
-#pragma unroll
-		for (int r = 0; r < 6; ++r) ap[q][r] = group_sum(ap[q][r], G);
+	const long long b = base_ok ? cell_id(g, tile.i, bj, bk) : 0;
+	{
+		const float sum = group_sum8(ar, red, lt, lane, G);
+		if (base_ok && lane < 8) rho_out[(long long)lane * g.ncells + b] = sum;
 	}
-	if (lane == 0 && base_ok) {
-		long long b = cell_id(g, tile.i, bj, bk);
 #pragma unroll
-		for (int q = 0; q < 8; ++q) {
-			rho_out[(long long)q * g.ncells + b] = ar[q];
+	for (int r = 0; r < 6; ++r) {
+		float v8[8];
 #pragma unroll
-			for (int r = 0; r < 6; ++r) pi_out[((long long)q * 6 + r) * g.ncells + b] = ap[q][r];
-		}
+		for (int q = 0; q < 8; ++q) v8[q] = ap[q][r];
+		const float sum = group_sum8(v8, red, lt, lane, G);
+		if (base_ok && lane < 8) pi_out[((long long)lane * 6 + r) * g.ncells + b] = sum;
 	}
 }
 
@@ -467,6 +478,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 // Output planes (float): per species [conc, energy, bulk x3] x 8 corners, then rho x 8.
 __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start, int G,
                                                    int Tj, int Tk, float* __restrict__ out) {
+	__shared__ float red[256 * 9];
 	const PlaneTile tile = plane_tile(g, Tj, Tk);
 	const int lt = threadIdx.x;
 	const int T = Tj * Tk;
@@ -526,30 +538,27 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 						}
 			}
 		}
-#pragma unroll
-		for (int q = 0; q < 8; ++q) {
-			conc[q] = group_sum(conc[q], G);
-			en[q] = group_sum(en[q], G);
-#pragma unroll
-			for (int r = 0; r < 3; ++r) bulk[q][r] = group_sum(bulk[q][r], G);
+		const long long base = (long long)s * 5 * 8;
+		{
+			const float sum = group_sum8(conc, red, lt, lane, G);
+			if (base_ok && lane < 8) out[(base + 0 * 8 + lane) * g.ncells + b] = sum;
 		}
-		if (lane == 0 && base_ok) {
-			long long base = (long long)s * 5 * 8;
+		{
+			const float sum = group_sum8(en, red, lt, lane, G);
+			if (base_ok && lane < 8) out[(base + 1 * 8 + lane) * g.ncells + b] = sum;
+		}
 #pragma unroll
-			for (int q = 0; q < 8; ++q) {
-				out[(base + 0 * 8 + q) * g.ncells + b] = conc[q];
-				out[(base + 1 * 8 + q) * g.ncells + b] = en[q];
+		for (int r = 0; r < 3; ++r) {
+			float v8[8];
 #pragma unroll
-				for (int r = 0; r < 3; ++r) out[(base + (2 + r) * 8 + q) * g.ncells + b] = bulk[q][r];
-			}
+			for (int q = 0; q < 8; ++q) v8[q] = bulk[q][r];
+			const float sum = group_sum8(v8, red, lt, lane, G);
+			if (base_ok && lane < 8) out[(base + (2 + r) * 8 + lane) * g.ncells + b] = sum;
 		}
 	}
-#pragma unroll
-	for (int q = 0; q < 8; ++q) rho[q] = group_sum(rho[q], G);
-	if (lane == 0 && base_ok) {
-		long long base = (long long)sp.count * 5 * 8;
-#pragma unroll
-		for (int q = 0; q < 8; ++q) out[(base + q) * g.ncells + b] = rho[q];
+	{
+		const float sum = group_sum8(rho, red, lt, lane, G);
+		if (base_ok && lane < 8) out[((long long)sp.count * 5 * 8 + lane) * g.ncells + b] = sum;
 	}
 }
 
