@@ -78,6 +78,7 @@ struct pic_ctx {
 	uint32_t *mk = nullptr, *mi = nullptr, *mk_s = nullptr, *mi_s = nullptr;
 	long long mover_capacity = 0;
 	int end_bit = 0;
+	int mover_bit = 0;  // radix bits of the mover keys, which include the removal key nkeys
 	void* cub_temp = nullptr;
 	size_t cub_temp_bytes = 0;
 	std::vector<int> species_start_h;
@@ -228,6 +229,18 @@ void check_status(pic_ctx* c) {
 	}
 }
 
+// Species ranges after a removal or an injection: species q starts at start[] of its first key.
+void species_ranges(pic_ctx* c, long long n) {
+	hipStream_t s = c->stream;
+	for (int q = 0; q < c->sp.count; ++q)
+		HIP_CHECK(hipMemcpyAsync(&c->species_start_h[q], c->start + (((long long)q * c->g.ncells) << 3), sizeof(int),
+		                         hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	for (int q = c->sp.count; q <= kMaxSpecies; ++q) c->species_start_h[q] = (int)n;
+	HIP_CHECK(hipMemcpyAsync(c->species_start, c->species_start_h.data(), (kMaxSpecies + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+}
+
 // Sort particles by key (stable radix sort: deterministic), rebuild start[]. The `removed`
 // particles the push took out (wall boundary: key 0xFFFFFFFF) sort to the end and are
 // dropped; after a removal or an injection (`resized`) the species ranges are read back from
@@ -263,16 +276,7 @@ void sort_particles(pic_ctx* c, int removed = 0, bool resized = false) {
 	launch_count_keys(c->keys_sorted, n, c->counts, s);
 	size_t bytes = c->cub_temp_bytes;
 	HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start, (int)(c->nkeys + 1), s));
-	if (removed > 0 || resized) {  // species ranges after the removal: start[] at each species' first key
-		for (int q = 0; q < c->sp.count; ++q)
-			HIP_CHECK(hipMemcpyAsync(&c->species_start_h[q], c->start + (((long long)q * c->g.ncells) << 3), sizeof(int),
-			                         hipMemcpyDeviceToHost, s));
-		HIP_CHECK(hipStreamSynchronize(s));
-		for (int q = c->sp.count; q <= kMaxSpecies; ++q) c->species_start_h[q] = (int)n;
-		HIP_CHECK(hipMemcpyAsync(c->species_start, c->species_start_h.data(), (kMaxSpecies + 1) * sizeof(int),
-		                         hipMemcpyHostToDevice, s));
-		HIP_CHECK(hipStreamSynchronize(s));
-	}
+	if (removed > 0 || resized) species_ranges(c, n);
 }
 
 void set_count(pic_ctx* c, long long n) {
@@ -289,10 +293,12 @@ InjectPhysics inject_physics(const pic_ctx* c) {
 	return ph;
 }
 
+void rebin_particles32(pic_ctx* c, int removed = 0, bool resized = false);
+
 // After a push with the wall boundary (PIC++/simulation.cpp:258-289 order: erase and exchange,
 // injection, preserve-charge replacement): particles that left through the right end are
-// dropped, or with injection on replaced in place; the injected slabs are appended; one full
-// sort restores the cell order and the species ranges.
+// dropped, or with injection on replaced in place; the injected slabs are appended; the FP32
+// rebin (or one full sort) restores the cell order and the species ranges.
 void wall_particles(pic_ctx* c) {
 	hipStream_t s = c->stream;
 	int removed = 0;
@@ -317,7 +323,7 @@ void wall_particles(pic_ctx* c) {
 			long long at = n;
 			for (const InjectSlabs& b : plan) {
 				ph.id_base = (int)c->next_id;
-				launch_inject(c->g, c->sp, b, ph, c->P, c->Q, c->fp32, at, c->keys, s);
+				launch_inject(c->g, c->sp, b, ph, c->P, c->Q, c->fp32, at, c->keys, c->fp32 ? c->moved : nullptr, s);
 				at += b.count * ncols;
 				c->next_id += b.count * ncols;
 			}
@@ -337,14 +343,19 @@ void wall_particles(pic_ctx* c) {
 		}
 		c->inj_step++;
 	}
-	sort_particles(c, removed, resized);
+	if (c->fp32)
+		rebin_particles32(c, removed, resized);
+	else
+		sort_particles(c, removed, resized);
 }
 
 // Restores the cell-sorted order of the FP32 particles after a push without a full sort
 // (pic_kernels_fp32.hip, rebin kernels); more than a quarter of the particles changing
 // segment takes the full sort instead. Same result as a stable sort by key within each new
-// segment's stayers and arrivals.
-void rebin_particles32(pic_ctx* c) {
+// segment's stayers and arrivals. With the wall boundary the `removed` particles (key
+// 0xFFFFFFFF, flagged as movers by the push) are dropped, and particles appended after the
+// push (injection, flagged as movers) arrive like any other mover.
+void rebin_particles32(pic_ctx* c, int removed, bool resized) {
 	const long long n = c->np();
 	hipStream_t s = c->stream;
 	int M = 0;
@@ -357,15 +368,15 @@ void rebin_particles32(pic_ctx* c) {
 		HIP_CHECK(hipStreamSynchronize(s));
 	}
 	if (n == 0 || M > c->mover_capacity) {
-		sort_particles(c);
+		sort_particles(c, removed, resized);
 		return;
 	}
 	{
 		Timed t(c, "rebin_movers", (double)n * 12 + (double)M * 32 + (double)(c->nkeys + 1) * 4);
-		launch_gather_movers(c->moved, c->Fscan, c->keys, n, c->mk, c->mi, s);
+		launch_gather_movers(c->moved, c->Fscan, c->keys, n, c->nkeys, c->mk, c->mi, s);
 		if (M > 0) {
 			size_t bytes = c->cub_temp_bytes;
-			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_temp, bytes, c->mk, c->mk_s, c->mi, c->mi_s, M, 0, c->end_bit, s));
+			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_temp, bytes, c->mk, c->mk_s, c->mi, c->mi_s, M, 0, c->mover_bit, s));
 		}
 		HIP_CHECK(hipMemsetAsync(c->arrive, 0, (c->nkeys + 1) * sizeof(int), s));
 		launch_count_keys(c->mk_s, M, c->arrive, s);
@@ -383,13 +394,15 @@ void rebin_particles32(pic_ctx* c) {
 		launch_rebin_perm(n, c->Fscan, c->keys, c->start, c->start_new, c->nkeys, c->arrive, c->mk_s, c->mi_s, M, c->stay, c->first,
 		                  c->perm_sorted, s);
 	}
+	const long long n_new = n - removed;
+	c->Q.count = c->Qalt.count = n_new;  // the permutation fills the first n_new places
 	{
-		Timed t(c, "permute", (double)n * (2 * 32 + 4));
+		Timed t(c, "permute", (double)n_new * (2 * 32 + 4));
 		launch_permute32(c->Q, c->Qalt, c->perm_sorted, s);
 	}
-	c->Qalt.count = c->Q.count;
 	std::swap(c->Q, c->Qalt);
 	std::swap(c->start, c->start_new);
+	if (removed > 0 || resized) species_ranges(c, n_new);
 }
 
 void density(pic_ctx* c) {
@@ -722,6 +735,8 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		if (c->nkeys >= (1LL << 32) - 1) throw DeviceError{PIC_ERR_ARGUMENT, "grid too large for 32-bit particle keys"};
 		c->end_bit = 1;
 		while ((1LL << c->end_bit) < c->nkeys) ++c->end_bit;
+		c->mover_bit = c->end_bit;
+		while ((1LL << c->mover_bit) <= c->nkeys) ++c->mover_bit;
 		c->E = dalloc<double>(3 * g.nnodes);
 		c->Etheta = dalloc<double>(3 * g.nnodes);
 		c->Enew = dalloc<double>(3 * g.nnodes);
@@ -780,7 +795,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->start = dalloc<int>(c->nkeys + 1);
 		c->counts = dalloc<int>(c->nkeys + 1);
 		HIP_CHECK(hipMemset(c->start, 0, (c->nkeys + 1) * sizeof(int)));
-		size_t b1 = 0, b2 = 0, b3 = 0;
+		size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
 		HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c->keys, c->keys_sorted, c->perm, c->perm_sorted,
 		                                             (int)std::max<long long>(c->capacity, 1), 0, c->end_bit, c->stream));
 		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->counts, c->start, (int)(c->nkeys + 1), c->stream));
@@ -795,8 +810,10 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 			for (uint32_t** p : {&c->mk, &c->mi, &c->mk_s, &c->mi_s}) *p = dalloc<uint32_t>(c->mover_capacity);
 			HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, b3, c->moved, c->Fscan + 1, (int)std::max<long long>(c->capacity, 1),
 			                                           c->stream));
+			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b4, c->mk, c->mk_s, c->mi, c->mi_s, (int)c->mover_capacity, 0,
+			                                             c->mover_bit, c->stream));
 		}
-		c->cub_temp_bytes = std::max(std::max(b1, b2), b3);
+		c->cub_temp_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
 		c->cub_temp = dalloc<char>(c->cub_temp_bytes);
 		c->species_start = dalloc<int>(kMaxSpecies + 1);
 		c->species_start_h.assign(kMaxSpecies + 1, 0);

@@ -743,13 +743,15 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 // order of their old indices: deterministic, and the same order a stable sort by key gives
 // inside each group. F is the inclusive prefix count of movers shifted by one (F[0] = 0).
 
-// movers in old-index order: mk[F[i]] = key, mi[F[i]] = i
+// movers in old-index order: mk[F[i]] = key, mi[F[i]] = i; removed particles (keys >= nkeys)
+// take the key nkeys, which has no segment
 __global__ void k_gather_movers(const int* __restrict__ moved, const int* __restrict__ F, const uint32_t* __restrict__ keys, long long n,
-                                uint32_t* __restrict__ mk, uint32_t* __restrict__ mi) {
+                                uint32_t nkeys, uint32_t* __restrict__ mk, uint32_t* __restrict__ mi) {
 	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n || !moved[i]) return;
 	int m = F[i];
-	mk[m] = keys[i];
+	const uint32_t k = keys[i];
+	mk[m] = k < nkeys ? k : nkeys;
 	mi[m] = (uint32_t)i;
 }
 
@@ -798,10 +800,11 @@ __global__ void k_rebin_stayers(long long n, const int* __restrict__ F, const ui
 }
 
 __global__ void k_rebin_movers(const uint32_t* __restrict__ mk, const uint32_t* __restrict__ mi, long long M, const int* __restrict__ start_new,
-                               const int* __restrict__ stay, const int* __restrict__ first, uint32_t* __restrict__ perm) {
+                               const int* __restrict__ stay, const int* __restrict__ first, uint32_t nkeys, uint32_t* __restrict__ perm) {
 	long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (m >= M) return;
 	uint32_t k = mk[m];
+	if (k >= nkeys) return;  // removed: no place in the new order
 	perm[(long long)start_new[k] + stay[k] + (m - first[k])] = mi[m];
 }
 
@@ -1049,9 +1052,10 @@ void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32
 	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(Et), reinterpret_cast<const float4*>(B), dt,
 	                   eta, keys, moved, status, dead, esc);
 }
-void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, uint32_t* mk, uint32_t* mi, hipStream_t s) {
+void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, long long nkeys, uint32_t* mk, uint32_t* mi,
+                          hipStream_t s) {
 	if (n == 0) return;
-	hipLaunchKernelGGL(k_gather_movers, dim3(blocks32(n, 256)), dim3(256), 0, s, moved, F, keys, n, mk, mi);
+	hipLaunchKernelGGL(k_gather_movers, dim3(blocks32(n, 256)), dim3(256), 0, s, moved, F, keys, n, (uint32_t)nkeys, mk, mi);
 }
 void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_t s) {
 	if (M == 0) return;
@@ -1065,7 +1069,8 @@ void launch_rebin_perm(long long n, const int* F, const uint32_t* keys, const in
                        hipStream_t s) {
 	hipLaunchKernelGGL(k_rebin_shift, dim3(blocks32(nkeys, 256)), dim3(256), 0, s, start_old, start_new, F, nkeys, shift);
 	if (n > 0) hipLaunchKernelGGL(k_rebin_stayers, dim3(blocks32(n, 256)), dim3(256), 0, s, n, F, keys, shift, perm);
-	if (M > 0) hipLaunchKernelGGL(k_rebin_movers, dim3(blocks32(M, 256)), dim3(256), 0, s, mk, mi, M, start_new, stay, first, perm);
+	if (M > 0)
+		hipLaunchKernelGGL(k_rebin_movers, dim3(blocks32(M, 256)), dim3(256), 0, s, mk, mi, M, start_new, stay, first, (uint32_t)nkeys, perm);
 }
 void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
                    hipStream_t s) {

@@ -25,7 +25,8 @@ void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArray
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, int* dead,
                    uint32_t* esc, hipStream_t s);
-void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, uint32_t* mk, uint32_t* mi, hipStream_t s);
+void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, long long nkeys, uint32_t* mk, uint32_t* mi,
+                          hipStream_t s);
 void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_t s);
 void launch_rebin_counts(const int* start_old, const int* F, const int* arrive, long long nkeys, int* stay, int* counts, hipStream_t s);
 // perm[new position] = old index; shift: scratch of nkeys ints (overwritten)
