@@ -106,6 +106,95 @@ __device__ inline void gather_cell32(const Grid& g, const float4* __restrict__ f
 			}
 }
 
+// The push's field boxes in LDS. A workgroup's 256 particles are consecutive in the sorted
+// order; when they all lie in one (i, j) row of cells, k0 <= k <= k0 + K - 1 with K <= kBoxK,
+// the workgroup stages the fields around that row: E^n on the row's own nodes (the start
+// position's node map), E^{n+theta} with one node and B with two cells more on every side
+// (the maps of Picard iterates up to one cell away). A gather whose 2 x 2 x 2 stencil lies
+// in its box reads LDS, any other reads HBM; both sum the same values in the same order.
+// Vector memory instructions (one per stencil point and field) bound the push otherwise.
+constexpr int kBoxK = 20;
+
+// LDS pointers carry their address space, so that the box and HBM paths of a gather stay
+// ds_read and global_load instructions (a generic pointer would make both flat loads).
+typedef const __attribute__((address_space(3))) float lds_float;
+
+// A box W x W x (K + W - 1) points, laid out with the strides of the largest K (kBoxK) so
+// that a stencil's 8 LDS addresses differ by constants.
+template <int W>
+struct Box32 {
+	static constexpr int sj = kBoxK + W - 1, si = W * sj, size = W * si;
+	int o[3];  // ghosted index of local (0, 0, 0)
+	int limk;  // largest local stencil base along k: K + W - 3
+};
+
+template <int W>
+__device__ inline Box32<W> make_box(int i0, int j0, int k0, int K) {
+	Box32<W> b;
+	constexpr int margin = W == 2 ? 0 : W - 3;
+	b.o[0] = i0 - margin;
+	b.o[1] = j0 - margin;
+	b.o[2] = k0 - margin;
+	b.limk = K + W - 3;
+	return b;
+}
+
+// local base of the stencil at ghosted base (a, c, d); false when the stencil leaves the box
+template <int W>
+__device__ inline bool in_box(const Box32<W>& b, int a, int c, int d, int& l) {
+	a -= b.o[0];
+	c -= b.o[1];
+	d -= b.o[2];
+	l = a * Box32<W>::si + c * Box32<W>::sj + d;
+	return (unsigned)a <= (unsigned)(W - 2) && (unsigned)c <= (unsigned)(W - 2) && (unsigned)d <= (unsigned)b.limk;
+}
+
+__device__ inline void sum8(const float e[8][3], const float w0[2], const float w1[2], const float w2[2], float out[3]) {
+	out[0] = out[1] = out[2] = 0.0f;
+#pragma unroll
+	for (int t = 0; t < 8; ++t) {
+		float c = w0[(t >> 2) & 1] * w1[(t >> 1) & 1] * w2[t & 1];
+		out[0] += e[t][0] * c;
+		out[1] += e[t][1] * c;
+		out[2] += e[t][2] * c;
+	}
+}
+
+// 8-point gather of a node (NODE) or cell field with the maps m, from the LDS box when the
+// workgroup has one (use) and the stencil lies in it, else from HBM. Same sums as
+// gather_node32 / gather_cell32.
+template <bool NODE, int W>
+__device__ inline void gather_boxed(const Grid& g, const float4* __restrict__ f, lds_float* box, const Box32<W>& b, bool use,
+                                   const Map32* m, float out[3]) {
+	const int a = NODE ? m[0].nb : m[0].cb, c = NODE ? m[1].nb : m[1].cb, d = NODE ? m[2].nb : m[2].cb;
+	const float* w0 = NODE ? m[0].nw : m[0].cw;
+	const float* w1 = NODE ? m[1].nw : m[1].cw;
+	const float* w2 = NODE ? m[2].nw : m[2].cw;
+	int l;
+	float e[8][3];
+	if (use && in_box(b, a, c, d, l)) {
+		lds_float* p0 = box + 4 * l;  // the 8 points at constant offsets from it
+#pragma unroll
+		for (int t = 0; t < 8; ++t) {
+			lds_float* p = p0 + 4 * (((t >> 2) & 1) * Box32<W>::si + ((t >> 1) & 1) * Box32<W>::sj + (t & 1));
+			e[t][0] = p[0];
+			e[t][1] = p[1];
+			e[t][2] = p[2];
+		}
+	} else {
+		const int sj = NODE ? g.N[2] + 1 : g.N[2], si = (NODE ? g.N[1] + 1 : g.N[1]) * sj;
+		const int base = a * si + c * sj + d;
+#pragma unroll
+		for (int t = 0; t < 8; ++t) {
+			const float4 v = f[base + ((t >> 2) & 1) * si + ((t >> 1) & 1) * sj + (t & 1)];
+			e[t][0] = v.x;
+			e[t][1] = v.y;
+			e[t][2] = v.z;
+		}
+	}
+	sum8(e, w0, w1, w2, out);
+}
+
 // The 8 stencil values of a fixed stencil, loaded once and reused for every particle of
 // a segment (all particles of one key share both stencil bases).
 struct Stencil8 {
@@ -133,12 +222,19 @@ __device__ inline void interp8(const Stencil8& s, const float w0[2], const float
 	}
 }
 
+// The FP32 kernels take reciprocals and inverse square roots from the hardware's
+// v_rcp_f32 / v_rsq_f32 (about 1 ulp) instead of correctly rounded divisions and square
+// roots, which cost about ten instructions each.
+__device__ inline float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float rsq32(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // alpha tensor (PIC++/mover.cpp:851-908) in FP32, with v and gamma from u = gamma v.
 __device__ inline void alpha32(float beta, const float v[3], float gamma, const float E[3], const float B[3], float a[9]) {
 	float G = beta * (E[0] * v[0] + E[1] * v[1] + E[2] * v[2]) + gamma;
-	float bs = beta / G;
+	float iG = rcp32(G);
+	float bs = beta * iG;
 	float b2 = bs * bs;
-	float inv = 1.0f / (G * (1.0f + b2 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2])));
+	float inv = iG * rcp32(1.0f + b2 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]));
 	a[0] = (1.0f + b2 * B[0] * B[0]) * inv;
 	a[1] = (b2 * B[0] * B[1] + bs * B[2]) * inv;
 	a[2] = (b2 * B[0] * B[2] - bs * B[1]) * inv;
@@ -269,8 +365,9 @@ __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp,
 					Bf[1] += bv.y * wt;
 					Bf[2] += bv.z * wt;
 				}
-				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-				float ig = 1.0f / gm;
+				const float g2 = 1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+				float ig = rsq32(g2);
+				float gm = g2 * ig;
 				float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
 				alpha32(beta, v, gm, Ef, Bf, a);
 				float rv[3];
@@ -435,8 +532,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 					Ef[2] += ev.z * wt;
 				}
 				interp8(SB, cw[0], cw[1], cw[2], Bf);
-				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-				float ig = 1.0f / gm;
+				const float g2 = 1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+				float ig = rsq32(g2);
+				float gm = g2 * ig;
 				float vv[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
 				alpha32(beta, vv, gm, Ef, Bf, al);
 				float rv[3];
@@ -520,7 +618,8 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 					cw[a3][0] = (bit[a3] ? 1.5f : 0.5f) - f[a3];
 					cw[a3][1] = f[a3] - (bit[a3] ? 0.5f : -0.5f);
 				}
-				float gm = sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+				const float g2 = 1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+				float gm = g2 * rsq32(g2);
 #pragma unroll
 				for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -563,39 +662,103 @@ __global__ void __launch_bounds__(256) k_density32(Grid g, SpeciesTable sp, Part
 }
 
 // ------------------------------------------------------------------ mover
+// Round-up multiplier division (Granlund and Montgomery, PLDI 1994, fig. 4.1): n / d for any
+// 32-bit n, with l = ceil(log2 d) >= 1 and m = floor(2^32 (2^l - d) / d) + 1.
+__device__ inline uint32_t div_magic(uint32_t n, uint32_t m, int l) {
+	const uint32_t t = __umulhi(n, m);
+	return (t + ((n - t) >> 1)) >> (l - 1);
+}
+
+__device__ inline void cell_ijk_magic(const Grid& g, const Push32Consts& K, int cell, int c[3]) {
+	const uint32_t q1 = div_magic((uint32_t)cell, K.m2, K.l2);  // cell / N[2]
+	const uint32_t q2 = div_magic(q1, K.m1, K.l1);              // cell / (N[2] N[1])
+	c[2] = cell - (int)q1 * g.N[2];
+	c[1] = (int)q1 - (int)q2 * g.N[1];
+	c[0] = (int)q2;
+}
+
 // moveParticle (PIC++/mover.cpp:194-491) on cell offsets, one thread per particle; alpha is
 // recomputed from E^n, B^n at x^n. Momentum-space constants: du = (q dt / m) (E + v x B).
 template <bool WALL>
-__global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start,
-                                                const float4* __restrict__ E, const float4* __restrict__ Et,
-                                                const float4* __restrict__ B, float dt, float eta, uint32_t* __restrict__ keys,
-                                                int* __restrict__ moved, int* __restrict__ status, int* __restrict__ dead,
-                                                uint32_t* __restrict__ esc) {
-	long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32Consts K, ParticleArrays32 P,
+                                                const int* __restrict__ species_start, const float4* __restrict__ E,
+                                                const float4* __restrict__ Et, const float4* __restrict__ B, float dt, float eta,
+                                                uint32_t* __restrict__ keys, int* __restrict__ moved, int* __restrict__ status,
+                                                int* __restrict__ dead, uint32_t* __restrict__ esc) {
+	constexpr int oEt = Box32<2>::size, oB = Box32<2>::size + Box32<4>::size;
+	__shared__ float4 boxes[Box32<2>::size + Box32<4>::size + Box32<5>::size];  // E^n, E^{n+theta}, B
+	__shared__ int row_ends[2];
+	const long long b0 = (long long)blockIdx.x * blockDim.x;
+	const long long idx = b0 + threadIdx.x;
+	const int last = (int)min((long long)blockDim.x - 1, P.count - 1 - b0);
+	if ((int)threadIdx.x == 0) row_ends[0] = P.cell[b0];
+	if ((int)threadIdx.x == last) row_ends[1] = P.cell[b0 + last];
+	__syncthreads();
+	int rf[3], rl[3];
+	cell_ijk_magic(g, K, row_ends[0], rf);
+	cell_ijk_magic(g, K, row_ends[1], rl);
+	const int KB = rl[2] - rf[2] + 1;
+	const bool use_box = rf[0] == rl[0] && rf[1] == rl[1] && KB >= 1 && KB <= kBoxK;
+	const Box32<2> bEn = make_box<2>(rf[0], rf[1], rf[2], KB);
+	const Box32<4> bEt = make_box<4>(rf[0], rf[1], rf[2], KB);
+	const Box32<5> bB = make_box<5>(rf[0], rf[1], rf[2], KB);
+	if (use_box) {
+		// Stage the boxes: 4, 16 and 25 rows along k of K + W - 1 points (W = 2, 4, 5), one
+		// point per thread and pass. Point e of a box is row r = e / len, point ld = e - r len
+		// (the quotient from a float reciprocal, exact for these sizes: the fraction of
+		// (e + 1/2) / len is at least 1 / (2 len) away from an integer); a row r of a box of
+		// width W is (la, lc) = (r / W, r % W), r / 5 = (13 r) >> 6 for r < 25. Ghosted indexes
+		// are clamped into the arrays.
+		const int len0 = KB + 1, len1 = KB + 3, len2 = KB + 4;
+		const int n0 = 4 * len0, n1 = n0 + 16 * len1, n2 = n1 + 25 * len2;
+		const float rl0 = rcp32((float)len0), rl1 = rcp32((float)len1), rl2 = rcp32((float)len2);
+		for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+			const int kind = e < n0 ? 0 : (e < n1 ? 1 : 2);  // E^n, E^{n+theta}, B
+			const int eb = e - (kind == 0 ? 0 : (kind == 1 ? n0 : n1));
+			const int len = kind == 0 ? len0 : (kind == 1 ? len1 : len2);
+			const float rl = kind == 0 ? rl0 : (kind == 1 ? rl1 : rl2);
+			const int r = (int)(((float)eb + 0.5f) * rl);
+			const int ld = eb - r * len;
+			const int w = kind == 0 ? 2 : (kind == 1 ? 4 : 5);
+			const int la = kind == 2 ? (r * 13) >> 6 : r >> (kind + 1);
+			const int lc = r - la * w;
+			const bool node = kind < 2;
+			const int lim = node ? 0 : 1;  // nodes 0..N, cells 0..N-1
+			const int a = clampi(rf[0] - kind + la, 0, g.N[0] - lim), c = clampi(rf[1] - kind + lc, 0, g.N[1] - lim),
+			          d = clampi(rf[2] - kind + ld, 0, g.N[2] - lim);
+			const int S1 = node ? g.N[1] + 1 : g.N[1], S2 = node ? g.N[2] + 1 : g.N[2];
+			const float4* src = kind == 0 ? E : (kind == 1 ? Et : B);
+			const int sj = kind == 0 ? Box32<2>::sj : (kind == 1 ? Box32<4>::sj : Box32<5>::sj);
+			const int lo = kind == 0 ? 0 : (kind == 1 ? oEt : oB);
+			boxes[lo + r * sj + ld] = src[(a * S1 + c) * S2 + d];
+		}
+	}
+	__syncthreads();
 	if (idx >= P.count) return;
 	int i = (int)idx;
 	int s = 0;
 	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
-	float m = (float)sp.mass[s], q = (float)sp.charge[s];
-	float qdm = q * dt / m;
-	float beta = 0.5f * q * dt / m;
-	float inv_d[3] = {(float)(1.0 / g.d[0]), (float)(1.0 / g.d[1]), (float)(1.0 / g.d[2])};
+	const float qdm = K.qdm[s], beta = K.beta[s];
+	const float inv_d[3] = {K.inv_d[0], K.inv_d[1], K.inv_d[2]};
 	const int cell0 = P.cell[i];
 	int cc[3];
-	cell_ijk(g, cell0, cc);
+	cell_ijk_magic(g, K, cell0, cc);
 	float f[3], u[3];
 	load32(P, i, f, u);
-	const uint32_t key0 = (uint32_t)((((long long)s * g.ncells + cell0) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) |
-	                                 (f[2] >= 0.5f));
+	const uint32_t key_s = (uint32_t)s * (uint32_t)g.ncells;  // keys fit 32 bits (checked on the host)
+	const uint32_t key0 = ((key_s + (uint32_t)cell0) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) | (f[2] >= 0.5f);
 	Map32 mp[3];
 	int why = maps32(g, cc, f, mp) ? 0 : 1;  // failed checks: 1 start stencil, 2 Picard stencil, 4 << c new cell
 	float En[3], Bf[3], a[9], Ef[3];
-	gather_node32(g, E, mp, En);
-	gather_cell32(g, B, mp, Bf);
-	gather_node32(g, Et, mp, Ef);
+	lds_float* lEn = (lds_float*)boxes;
+	lds_float* lEt = (lds_float*)(boxes + oEt);
+	lds_float* lB = (lds_float*)(boxes + oB);
+	gather_boxed<true, 2>(g, E, lEn, bEn, use_box, mp, En);
+	gather_boxed<false, 5>(g, B, lB, bB, use_box, mp, Bf);
+	gather_boxed<true, 4>(g, Et, lEt, bEt, use_box, mp, Ef);
 	float u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-	float gm = sqrtf(1.0f + u2);
-	float ig = 1.0f / gm;
+	float ig = rsq32(1.0f + u2);
+	float gm = (1.0f + u2) * ig;
 	float v[3] = {u[0] * ig, u[1] * ig, u[2] * ig};
 	alpha32(beta, v, gm, En, Bf, a);
 	// predictor (mover.cpp:234-245), with Particle::evaluateVelocity's nonrelativistic branch
@@ -604,7 +767,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 	nu[1] = u[1] + (Ef[1] + (-v[0] * Bf[2] + v[2] * Bf[0])) * qdm;
 	nu[2] = u[2] + (Ef[2] + (v[0] * Bf[1] - v[1] * Bf[0])) * qdm;
 	float nu2 = nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2];
-	float nig = nu2 < 0.0001f ? 1.0f : 1.0f / sqrtf(1.0f + nu2);
+	float nig = nu2 < 0.0001f ? 1.0f : rsq32(1.0f + nu2);
 	const float nup[3] = {nu[0], nu[1], nu[2]};  // the predictor's u (the Picard loop overwrites nu)
 	float mid[3], xh[3];
 #pragma unroll
@@ -637,12 +800,13 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 	float vhat[3];
 #pragma unroll
 	for (int r = 0; r < 3; ++r) vhat[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
-	float unorm = sqrtf(u2);
+	// the stopping rule on squares: |du| > 1e-14 |u| and |dx| > 1e-8 dx (mover.cpp:292)
 	float du[3] = {nu[0] - u[0], nu[1] - u[1], nu[2] - u[2]};
-	float err = sqrtf(du[0] * du[0] + du[1] * du[1] + du[2] * du[2]);
+	float err2 = du[0] * du[0] + du[1] * du[1] + du[2] * du[2];
 	float prevX[3] = {xh[0], xh[1], xh[2]};
-	float cerr = 1.0f;  // in cell units (coordinatesError = deltaX)
-	for (int it = 0; !reflected && it < 3 && err > 1e-14f * unorm && cerr > 1e-8f; ++it) {
+	float cerr2 = 1.0f;  // in cell units (coordinatesError = deltaX)
+	const float inv_d02 = inv_d[0] * inv_d[0];
+	for (int it = 0; !reflected && it < 3 && err2 > 1e-28f * u2 && cerr2 > 1e-16f; ++it) {
 		float prevU[3] = {nu[0], nu[1], nu[2]};
 		float rE[3];
 #pragma unroll
@@ -658,15 +822,15 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 			break;
 		}
 		if (!maps32(g, cc, nx, mp)) why |= 2;
-		gather_node32(g, Et, mp, Ef);
-		gather_cell32(g, B, mp, Bf);
+		gather_boxed<true, 4>(g, Et, lEt, bEt, use_box, mp, Ef);
+		gather_boxed<false, 5>(g, B, lB, bB, use_box, mp, Bf);
 		nu[0] = u[0] + (Ef[0] + (mid[1] * Bf[2] - mid[2] * Bf[1])) * qdm;
 		nu[1] = u[1] + (Ef[1] + (-mid[0] * Bf[2] + mid[2] * Bf[0])) * qdm;
 		nu[2] = u[2] + (Ef[2] + (mid[0] * Bf[1] - mid[1] * Bf[0])) * qdm;
 		float d0 = prevU[0] - nu[0], d1 = prevU[1] - nu[1], d2 = prevU[2] - nu[2];
-		err = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-		float e0 = (prevX[0] - nx[0]) * (float)g.d[0], e1 = (prevX[1] - nx[1]) * (float)g.d[1], e2 = (prevX[2] - nx[2]) * (float)g.d[2];
-		cerr = sqrtf(e0 * e0 + e1 * e1 + e2 * e2) * inv_d[0];
+		err2 = d0 * d0 + d1 * d1 + d2 * d2;
+		float e0 = (prevX[0] - nx[0]) * K.d[0], e1 = (prevX[1] - nx[1]) * K.d[1], e2 = (prevX[2] - nx[2]) * K.d[2];
+		cerr2 = (e0 * e0 + e1 * e1 + e2 * e2) * inv_d02;
 #pragma unroll
 		for (int c = 0; c < 3; ++c) prevX[c] = nx[c];
 	}
@@ -724,15 +888,14 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Particl
 		}
 		return;
 	}
-	int cell = (int)cell_id(g, nc[0], nc[1], nc[2]);
+	const int cell = (nc[0] * g.N[1] + nc[1]) * g.N[2] + nc[2];
 	P.cell[i] = cell;
 #pragma unroll
 	for (int c = 0; c < 3; ++c) {
 		P.off[c][i] = f[c];
 		P.u[c][i] = unew[c];
 	}
-	const uint32_t key = (uint32_t)((((long long)s * g.ncells + cell) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) |
-	                                (f[2] >= 0.5f));
+	const uint32_t key = ((key_s + (uint32_t)cell) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) | (f[2] >= 0.5f);
 	keys[i] = key;
 	moved[i] = key != key0;
 }
@@ -1044,11 +1207,29 @@ void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArray
 	PlaneTiling pt = plane_tiling(g, T);
 	hipLaunchKernelGGL(k_density32, dim3(pt.blocks), dim3(T * G), 0, s, g, sp, P, start, G, pt.Tj, pt.Tk, scratch);
 }
+static void magic_divisor(uint32_t d, uint32_t& m, int& l) {
+	l = 0;
+	while ((1ull << l) < d) ++l;
+	m = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+}
+
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, int* dead,
                    uint32_t* esc, hipStream_t s) {
 	if (P.count == 0) return;
-	hipLaunchKernelGGL(g.wall ? k_push32<true> : k_push32<false>, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start,
+	Push32Consts K{};
+	for (int q = 0; q < sp.count; ++q) {  // the same float arithmetic as q dt / m on the device
+		const float mf = (float)sp.mass[q], qf = (float)sp.charge[q];
+		K.qdm[q] = qf * dt / mf;
+		K.beta[q] = 0.5f * qf * dt / mf;
+	}
+	for (int a = 0; a < 3; ++a) {
+		K.inv_d[a] = (float)(1.0 / g.d[a]);
+		K.d[a] = (float)g.d[a];
+	}
+	magic_divisor((uint32_t)g.N[2], K.m2, K.l2);  // N >= 5: l >= 3
+	magic_divisor((uint32_t)g.N[1], K.m1, K.l1);
+	hipLaunchKernelGGL(g.wall ? k_push32<true> : k_push32<false>, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, K, P, species_start,
 	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(Et), reinterpret_cast<const float4*>(B), dt,
 	                   eta, keys, moved, status, dead, esc);
 }

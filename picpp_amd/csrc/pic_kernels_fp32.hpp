@@ -22,6 +22,17 @@ void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const Particle
                            const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s);
 void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
                       hipStream_t s);
+// Per-launch constants of the FP32 push, formed on the host: q dt / m and q dt / (2 m) per
+// species in float arithmetic, the cell sizes and their inverses, and the multipliers and
+// shifts that divide a cell index by N[2] and N[1].
+struct Push32Consts {
+	float qdm[kMaxSpecies];
+	float beta[kMaxSpecies];
+	float inv_d[3], d[3];
+	uint32_t m2, m1;
+	int l2, l1;
+};
+
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, int* dead,
                    uint32_t* esc, hipStream_t s);
