@@ -74,7 +74,9 @@ struct pic_ctx {
 	long long nkeys = 0;
 	// rebin of the FP32 particles after a push: mover flags, their prefix count, the movers
 	// (key, old index) and per-key stay / arrival counts
-	int *moved = nullptr, *Fscan = nullptr, *start_new = nullptr, *stay = nullptr, *arrive = nullptr, *first = nullptr;
+	uint64_t* mask = nullptr;  // FP32 push: mover bit per particle
+	int *wcount = nullptr, *wprefix = nullptr;  // mask word populations and their exclusive scan
+	int *start_new = nullptr, *stay = nullptr, *arrive = nullptr, *first = nullptr;
 	uint32_t *mk = nullptr, *mi = nullptr, *mk_s = nullptr, *mi_s = nullptr;
 	long long mover_capacity = 0;
 	int end_bit = 0;
@@ -323,10 +325,11 @@ void wall_particles(pic_ctx* c) {
 			long long at = n;
 			for (const InjectSlabs& b : plan) {
 				ph.id_base = (int)c->next_id;
-				launch_inject(c->g, c->sp, b, ph, c->P, c->Q, c->fp32, at, c->keys, c->fp32 ? c->moved : nullptr, s);
+				launch_inject(c->g, c->sp, b, ph, c->P, c->Q, c->fp32, at, c->keys, s);
 				at += b.count * ncols;
 				c->next_id += b.count * ncols;
 			}
+			if (c->fp32) launch_mark_movers(c->mask, n, n + ninj, s);  // the new particles arrive in the rebin
 			if (ninj > 0) {
 				set_count(c, n + ninj);
 				resized = true;
@@ -360,11 +363,12 @@ void rebin_particles32(pic_ctx* c, int removed, bool resized) {
 	hipStream_t s = c->stream;
 	int M = 0;
 	if (n > 0) {
-		Timed t(c, "rebin_scan", (double)n * 8);
-		HIP_CHECK(hipMemsetAsync(c->Fscan, 0, sizeof(int), s));
+		Timed t(c, "rebin_scan", (double)n / 64 * 16);
+		const long long nw = (n + 63) / 64;
+		launch_mask_counts(c->mask, nw, c->wcount, s);
 		size_t bytes = c->cub_temp_bytes;
-		HIP_CHECK(hipcub::DeviceScan::InclusiveSum(c->cub_temp, bytes, c->moved, c->Fscan + 1, (int)n, s));
-		HIP_CHECK(hipMemcpyAsync(&M, c->Fscan + n, sizeof(int), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->wcount, c->wprefix, (int)(nw + 1), s));
+		HIP_CHECK(hipMemcpyAsync(&M, c->wprefix + nw, sizeof(int), hipMemcpyDeviceToHost, s));
 		HIP_CHECK(hipStreamSynchronize(s));
 	}
 	if (n == 0 || M > c->mover_capacity) {
@@ -372,8 +376,8 @@ void rebin_particles32(pic_ctx* c, int removed, bool resized) {
 		return;
 	}
 	{
-		Timed t(c, "rebin_movers", (double)n * 12 + (double)M * 32 + (double)(c->nkeys + 1) * 4);
-		launch_gather_movers(c->moved, c->Fscan, c->keys, n, c->nkeys, c->mk, c->mi, s);
+		Timed t(c, "rebin_movers", (double)n / 64 * 12 + (double)M * 36 + (double)(c->nkeys + 1) * 4);
+		launch_gather_movers(c->mask, c->wprefix, c->keys, n, c->nkeys, c->mk, c->mi, s);
 		if (M > 0) {
 			size_t bytes = c->cub_temp_bytes;
 			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_temp, bytes, c->mk, c->mk_s, c->mi, c->mi_s, M, 0, c->mover_bit, s));
@@ -384,15 +388,15 @@ void rebin_particles32(pic_ctx* c, int removed, bool resized) {
 	}
 	{
 		Timed t(c, "rebin_segments", (double)(c->nkeys + 1) * 4 * 6);
-		launch_rebin_counts(c->start, c->Fscan, c->arrive, c->nkeys, c->stay, c->counts, s);
+		launch_rebin_counts(c->start, c->mask, c->wprefix, c->arrive, c->nkeys, c->stay, c->counts, s);
 		size_t bytes = c->cub_temp_bytes;
 		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start_new, (int)(c->nkeys + 1), s));
 	}
 	{
-		Timed t(c, "rebin_perm", (double)n * (3 * 4 + 4) + (double)c->nkeys * 12);
+		Timed t(c, "rebin_perm", (double)n * (4 + 4) + (double)c->nkeys * 12);
 		// the arrival counts are folded into start_new by now: their array takes the per-key shifts
-		launch_rebin_perm(n, c->Fscan, c->keys, c->start, c->start_new, c->nkeys, c->arrive, c->mk_s, c->mi_s, M, c->stay, c->first,
-		                  c->perm_sorted, s);
+		launch_rebin_perm(n, c->mask, c->wprefix, c->keys, c->start, c->start_new, c->nkeys, c->arrive, c->mk_s, c->mi_s, M, c->stay,
+		                  c->first, c->perm_sorted, s);
 	}
 	const long long n_new = n - removed;
 	c->Q.count = c->Qalt.count = n_new;  // the permutation fills the first n_new places
@@ -800,16 +804,17 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		                                             (int)std::max<long long>(c->capacity, 1), 0, c->end_bit, c->stream));
 		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->counts, c->start, (int)(c->nkeys + 1), c->stream));
 		if (c->fp32) {
-			c->moved = dalloc<int>(c->capacity);
-			c->Fscan = dalloc<int>(c->capacity + 1);
+			const long long words = c->capacity / 64 + 2;
+			c->mask = dalloc<uint64_t>(words);
+			c->wcount = dalloc<int>(words);
+			c->wprefix = dalloc<int>(words);
 			c->start_new = dalloc<int>(c->nkeys + 1);
 			c->stay = dalloc<int>(c->nkeys + 1);
 			c->arrive = dalloc<int>(c->nkeys + 1);
 			c->first = dalloc<int>(c->nkeys + 1);
 			c->mover_capacity = c->capacity / 4 + 1;
 			for (uint32_t** p : {&c->mk, &c->mi, &c->mk_s, &c->mi_s}) *p = dalloc<uint32_t>(c->mover_capacity);
-			HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, b3, c->moved, c->Fscan + 1, (int)std::max<long long>(c->capacity, 1),
-			                                           c->stream));
+			HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, c->wcount, c->wprefix, (int)words, c->stream));
 			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b4, c->mk, c->mk_s, c->mi, c->mi_s, (int)c->mover_capacity, 0,
 			                                             c->mover_bit, c->stream));
 		}
@@ -926,7 +931,7 @@ void pic_destroy(pic_ctx* c) {
 	for (double* v : c->V) f(v);
 	for (float* v : c->V32) f(v);
 	for (void* p : {(void*)c->V32dev, (void*)c->mu32, (void*)c->mu32h, (void*)c->muh, (void*)c->w32, (void*)c->w64, (void*)c->tmp32, (void*)c->Z32dev,
-	                (void*)c->log_r, (void*)c->log_e, (void*)c->moved, (void*)c->Fscan, (void*)c->start_new, (void*)c->stay,
+	                (void*)c->log_r, (void*)c->log_e, (void*)c->mask, (void*)c->wcount, (void*)c->wprefix, (void*)c->start_new, (void*)c->stay,
 	                (void*)c->arrive, (void*)c->first, (void*)c->mk, (void*)c->mi, (void*)c->mk_s, (void*)c->mi_s})
 		f(p);
 	for (float* v : c->Z32) f(v);
@@ -1453,7 +1458,7 @@ int pic_push(pic_ctx* c) {
 			}
 			Timed t(c, "push", (double)c->np() * 60 + ((double)g.nnodes * 6 + (double)g.ncells * 3) * 4);
 			launch_push32(g, c->sp, c->Q, c->species_start, c->E32, c->Et32, c->B32, (float)c->cfg.dt, (float)c->eta, c->keys,
-			              c->moved, c->status, c->dead, c->esc, c->stream);
+			              c->mask, c->status, c->dead, c->esc, c->stream);
 		} else {
 			Timed t(c, "push", (double)c->P.count * 100 + ((double)c->g.nnodes * 6 + (double)c->g.ncells * 3) * 8);
 			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->status,

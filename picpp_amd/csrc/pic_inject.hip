@@ -140,7 +140,7 @@ __device__ inline void store_particle(const Grid& g, int s, const SpeciesTable& 
 }
 
 __global__ void k_inject(Grid g, SpeciesTable sp, InjectSlabs S, InjectPhysics ph, ParticleArrays P, ParticleArrays32 Q,
-                         int fp32, long long first, uint32_t* __restrict__ keys, int* __restrict__ moved) {
+                         int fp32, long long first, uint32_t* __restrict__ keys) {
 	const int ncols = g.n[1] * g.n[2];
 	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= (long long)S.count * ncols) return;
@@ -159,7 +159,6 @@ __global__ void k_inject(Grid g, SpeciesTable sp, InjectSlabs S, InjectPhysics p
 	double p[3];
 	flow_momentum(ph, sp.mass[s], ph.theta[s], c, p);
 	store_particle(g, s, sp, x, p, ph.id_base + (int)t, P, Q, fp32 != 0, first + t, keys);
-	if (moved) moved[first + t] = 1;
 }
 
 __global__ void k_replace(Grid g, SpeciesTable sp, InjectPhysics ph, const int* __restrict__ species_start,
@@ -238,11 +237,11 @@ std::vector<InjectSlabs> plan_injection(const Grid& g, int nspecies, const int* 
 }
 
 void launch_inject(const Grid& g, const SpeciesTable& sp, const InjectSlabs& slabs, const InjectPhysics& ph,
-                   const ParticleArrays& P, const ParticleArrays32& Q, bool fp32, long long first, uint32_t* keys, int* moved,
+                   const ParticleArrays& P, const ParticleArrays32& Q, bool fp32, long long first, uint32_t* keys,
                    hipStream_t s) {
 	const long long total = (long long)slabs.count * g.n[1] * g.n[2];
 	if (total == 0) return;
-	hipLaunchKernelGGL(k_inject, dim3(nblocks(total, 256)), dim3(256), 0, s, g, sp, slabs, ph, P, Q, fp32 ? 1 : 0, first, keys, moved);
+	hipLaunchKernelGGL(k_inject, dim3(nblocks(total, 256)), dim3(256), 0, s, g, sp, slabs, ph, P, Q, fp32 ? 1 : 0, first, keys);
 }
 
 void launch_replace(const Grid& g, const SpeciesTable& sp, const InjectPhysics& ph, const int* species_start,

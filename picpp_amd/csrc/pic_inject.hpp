@@ -39,10 +39,9 @@ std::vector<InjectSlabs> plan_injection(const Grid& g, int nspecies, const int* 
                                         unsigned& slab_counter);
 
 // Writes count * n1 * n2 new particles at indices first.. (FP64 form into P, or FP32 form
-// into Q) with their sort keys; with `moved` given, flags each new particle as a mover for the
-// rebin after the push.
+// into Q) with their sort keys.
 void launch_inject(const Grid& g, const SpeciesTable& sp, const InjectSlabs& slabs, const InjectPhysics& ph,
-                   const ParticleArrays& P, const ParticleArrays32& Q, bool fp32, long long first, uint32_t* keys, int* moved,
+                   const ParticleArrays& P, const ParticleArrays32& Q, bool fp32, long long first, uint32_t* keys,
                    hipStream_t s);
 
 // Replaces the particles at indices esc[0..count) (ascending) in place: same species, at the

@@ -683,7 +683,7 @@ template <bool WALL>
 __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32Consts K, ParticleArrays32 P,
                                                 const int* __restrict__ species_start, const float4* __restrict__ E,
                                                 const float4* __restrict__ Et, const float4* __restrict__ B, float dt, float eta,
-                                                uint32_t* __restrict__ keys, int* __restrict__ moved, int* __restrict__ status,
+                                                uint32_t* __restrict__ keys, uint64_t* __restrict__ mask, int* __restrict__ status,
                                                 int* __restrict__ dead, uint32_t* __restrict__ esc) {
 	constexpr int oEt = Box32<2>::size, oB = Box32<2>::size + Box32<4>::size;
 	__shared__ float4 boxes[Box32<2>::size + Box32<4>::size + Box32<5>::size];  // E^n, E^{n+theta}, B
@@ -848,71 +848,107 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 	// Out through the free right end: removed (mover.cpp:447-458). The reference removes x > node
 	// n+2; a position exactly on that node is removed here too, so that every kept particle has
 	// an owned cell.
+	bool mv = false;  // the particle leaves its (species, cell, octant) segment
 	if (WALL && pos[0] >= (float)(g.n[0] + 2 - cc[0])) {
 		keys[i] = 0xFFFFFFFFu;
-		moved[i] = 1;
+		mv = true;
 		// listed for replacement, with its absolute y, z in cell units
 		P.off[1][i] = (float)cc[1] + pos[1];
 		P.off[2][i] = (float)cc[2] + pos[2];
 		esc[atomicAdd(dead, 1)] = (uint32_t)i;
-		return;
-	}
-	// x^{n+1} = x^n + mid dt, carried into the integer cell; one-rank periodic wrap (x too unless walled)
-	int nc[3];
+	} else {
+		// x^{n+1} = x^n + mid dt, carried into the integer cell; one-rank periodic wrap (x too
+		// unless walled)
+		int nc[3];
 #pragma unroll
-	for (int c = 0; c < 3; ++c) {
-		float x = pos[c];
-		float fl = floorf(x);
-		x -= fl;
-		if (x >= 1.0f) {  // rounding of x - floor(x) up to 1
-			x = 0.0f;
-			fl += 1.0f;
+		for (int c = 0; c < 3; ++c) {
+			float x = pos[c];
+			float fl = floorf(x);
+			x -= fl;
+			if (x >= 1.0f) {  // rounding of x - floor(x) up to 1
+				x = 0.0f;
+				fl += 1.0f;
+			}
+			int ci = cc[c] + (int)fl;
+			const bool walled = WALL && c == 0;
+			if (!walled && ci < 2) ci += g.n[c];
+			if (!walled && ci > g.n[c] + 1) ci -= g.n[c];
+			if (ci < 2 || ci > g.n[c] + 1) why |= 4 << c;
+			nc[c] = ci;
+			f[c] = x;
 		}
-		int ci = cc[c] + (int)fl;
-		const bool walled = WALL && c == 0;
-		if (!walled && ci < 2) ci += g.n[c];
-		if (!walled && ci > g.n[c] + 1) ci -= g.n[c];
-		if (ci < 2 || ci > g.n[c] + 1) why |= 4 << c;
-		nc[c] = ci;
-		f[c] = x;
-	}
-	if (why) {
-		// A failed check ends the step with an error. The first failing particle keeps its state
-		// from before the push, and status[2..4] hold the failed checks, its index and species
-		// for the host's message.
-		atomicOr(status, 8);
-		if (atomicCAS(status + 1, 0, 1) == 0) {
-			status[2] = why;
-			status[3] = i;
-			status[4] = s;
-		}
-		return;
-	}
-	const int cell = (nc[0] * g.N[1] + nc[1]) * g.N[2] + nc[2];
-	P.cell[i] = cell;
+		if (why) {
+			// A failed check ends the step with an error. The first failing particle keeps its
+			// state from before the push, and status[2..4] hold the failed checks, its index and
+			// species for the host's message.
+			atomicOr(status, 8);
+			if (atomicCAS(status + 1, 0, 1) == 0) {
+				status[2] = why;
+				status[3] = i;
+				status[4] = s;
+			}
+		} else {
+			const int cell = (nc[0] * g.N[1] + nc[1]) * g.N[2] + nc[2];
+			P.cell[i] = cell;
 #pragma unroll
-	for (int c = 0; c < 3; ++c) {
-		P.off[c][i] = f[c];
-		P.u[c][i] = unew[c];
+			for (int c = 0; c < 3; ++c) {
+				P.off[c][i] = f[c];
+				P.u[c][i] = unew[c];
+			}
+			const uint32_t key = ((key_s + (uint32_t)cell) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) | (f[2] >= 0.5f);
+			keys[i] = key;
+			mv = key != key0;
+		}
 	}
-	const uint32_t key = ((key_s + (uint32_t)cell) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) | (f[2] >= 0.5f);
-	keys[i] = key;
-	moved[i] = key != key0;
+	// one mover bit per particle: bit i & 63 of word i >> 6, written by the wave's first lane
+	// (waves are 64 consecutive particles; lanes past the end contribute zeros)
+	const unsigned long long bits = __ballot(mv);
+	if ((threadIdx.x & 63) == 0) mask[i >> 6] = bits;
 }
 
 // ------------------------------------------------------------------ rebin after a push
 // Few particles change segment in one step, so the cell-sorted order is restored without a
 // full sort. New segment k holds its stayers in their old order, then its arrivals in the
 // order of their old indices: deterministic, and the same order a stable sort by key gives
-// inside each group. F is the inclusive prefix count of movers shifted by one (F[0] = 0).
+// inside each group. The push leaves one mover bit per particle (mask word i >> 6, bit
+// i & 63); F(i), the number of movers before particle i, is the exclusive prefix count of the
+// words' populations plus the population of the word's bits below i.
 
-// movers in old-index order: mk[F[i]] = key, mi[F[i]] = i; removed particles (keys >= nkeys)
+__device__ inline bool is_mover(const uint64_t* __restrict__ mask, long long i) { return (mask[i >> 6] >> (i & 63)) & 1ull; }
+
+// F(i) for 0 <= i <= n; prefix has one entry per word plus the total
+__device__ inline int movers_before(const uint64_t* __restrict__ mask, const int* __restrict__ prefix, long long i) {
+	const long long w = i >> 6;
+	const int b = (int)(i & 63);
+	const int below = b ? __popcll(mask[w] & (~0ull >> (64 - b))) : 0;
+	return prefix[w] + below;
+}
+
+// word populations; counts[nwords] = 0 closes the scan
+__global__ void k_mask_counts(const uint64_t* __restrict__ mask, long long nwords, int* __restrict__ counts) {
+	long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (w > nwords) return;
+	counts[w] = w < nwords ? __popcll(mask[w]) : 0;
+}
+
+// sets the mover bits of particles lo .. hi-1 (appended after the push), keeping the bits
+// below lo and clearing those from hi on in the last word; one thread per word
+__global__ void k_mark_movers(uint64_t* __restrict__ mask, long long lo, long long hi) {
+	const long long w = (lo >> 6) + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (w > ((hi - 1) >> 6)) return;
+	const long long a = w * 64 > lo ? w * 64 : lo, b = (w + 1) * 64 < hi ? (w + 1) * 64 : hi;
+	const uint64_t set = (~0ull >> (64 - (int)(b - a))) << (int)(a - w * 64);
+	const uint64_t keep = w * 64 < lo ? mask[w] & (~0ull >> (64 - (int)(lo - w * 64))) : 0ull;
+	mask[w] = keep | set;
+}
+
+// movers in old-index order: mk[F(i)] = key, mi[F(i)] = i; removed particles (keys >= nkeys)
 // take the key nkeys, which has no segment
-__global__ void k_gather_movers(const int* __restrict__ moved, const int* __restrict__ F, const uint32_t* __restrict__ keys, long long n,
-                                uint32_t nkeys, uint32_t* __restrict__ mk, uint32_t* __restrict__ mi) {
+__global__ void k_gather_movers(const uint64_t* __restrict__ mask, const int* __restrict__ prefix, const uint32_t* __restrict__ keys,
+                                long long n, uint32_t nkeys, uint32_t* __restrict__ mk, uint32_t* __restrict__ mi) {
 	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n || !moved[i]) return;
-	int m = F[i];
+	if (i >= n || !is_mover(mask, i)) return;
+	int m = movers_before(mask, prefix, i);
 	const uint32_t k = keys[i];
 	mk[m] = k < nkeys ? k : nkeys;
 	mi[m] = (uint32_t)i;
@@ -926,8 +962,8 @@ __global__ void k_first_of_run(const uint32_t* __restrict__ mk, long long M, int
 }
 
 // stay[k] = old segment length minus its leavers; counts[k] = stay[k] + arrivals[k]
-__global__ void k_rebin_counts(const int* __restrict__ start_old, const int* __restrict__ F, const int* __restrict__ arrive, long long nkeys,
-                               int* __restrict__ stay, int* __restrict__ counts) {
+__global__ void k_rebin_counts(const int* __restrict__ start_old, const uint64_t* __restrict__ mask, const int* __restrict__ prefix,
+                               const int* __restrict__ arrive, long long nkeys, int* __restrict__ stay, int* __restrict__ counts) {
 	long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k > nkeys) return;
 	if (k == nkeys) {
@@ -935,31 +971,29 @@ __global__ void k_rebin_counts(const int* __restrict__ start_old, const int* __r
 		return;
 	}
 	int lo = start_old[k], hi = start_old[k + 1];
-	int st = (hi - lo) - (F[hi] - F[lo]);
+	int st = (hi - lo) - (movers_before(mask, prefix, hi) - movers_before(mask, prefix, lo));
 	stay[k] = st;
 	counts[k] = st + arrive[k];
 }
 
-// shift[k] = start_new[k] - start_old[k] + F[start_old[k]]: a stayer i of segment k lands at
-// i - F[i] + shift[k] (its old offset in the segment minus the leavers before it)
-__global__ void k_rebin_shift(const int* __restrict__ start_old, const int* __restrict__ start_new, const int* __restrict__ F,
-                              long long nkeys, int* __restrict__ shift) {
+// shift[k] = start_new[k] - start_old[k] + F(start_old[k]): a stayer i of segment k lands at
+// i - F(i) + shift[k] (its old offset in the segment minus the leavers before it)
+__global__ void k_rebin_shift(const int* __restrict__ start_old, const int* __restrict__ start_new, const uint64_t* __restrict__ mask,
+                              const int* __restrict__ prefix, long long nkeys, int* __restrict__ shift) {
 	long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= nkeys) return;
 	int so = start_old[k];
-	shift[k] = start_new[k] - so + F[so];
+	shift[k] = start_new[k] - so + movers_before(mask, prefix, so);
 }
 
 // The destinations are written as a source index per new position (perm[to] = i), and the
 // particles then move with one coalesced gather (k_permute32): scattering the 32-byte
 // particles directly writes partial cache lines at every segment boundary.
-__global__ void k_rebin_stayers(long long n, const int* __restrict__ F, const uint32_t* __restrict__ keys, const int* __restrict__ shift,
-                                uint32_t* __restrict__ perm) {
+__global__ void k_rebin_stayers(long long n, const uint64_t* __restrict__ mask, const int* __restrict__ prefix,
+                                const uint32_t* __restrict__ keys, const int* __restrict__ shift, uint32_t* __restrict__ perm) {
 	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const int fi = F[i];
-	if (F[i + 1] != fi) return;  // a mover
-	perm[i - fi + shift[keys[i]]] = (uint32_t)i;
+	if (i >= n || is_mover(mask, i)) return;
+	perm[i - movers_before(mask, prefix, i) + shift[keys[i]]] = (uint32_t)i;
 }
 
 __global__ void k_rebin_movers(const uint32_t* __restrict__ mk, const uint32_t* __restrict__ mi, long long M, const int* __restrict__ start_new,
@@ -1214,7 +1248,7 @@ static void magic_divisor(uint32_t d, uint32_t& m, int& l) {
 }
 
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
-                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, int* dead,
+                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, uint64_t* mask, int* status, int* dead,
                    uint32_t* esc, hipStream_t s) {
 	if (P.count == 0) return;
 	Push32Consts K{};
@@ -1231,25 +1265,35 @@ void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32
 	magic_divisor((uint32_t)g.N[1], K.m1, K.l1);
 	hipLaunchKernelGGL(g.wall ? k_push32<true> : k_push32<false>, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, K, P, species_start,
 	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(Et), reinterpret_cast<const float4*>(B), dt,
-	                   eta, keys, moved, status, dead, esc);
+	                   eta, keys, mask, status, dead, esc);
 }
-void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, long long nkeys, uint32_t* mk, uint32_t* mi,
-                          hipStream_t s) {
+void launch_mask_counts(const uint64_t* mask, long long nwords, int* counts, hipStream_t s) {
+	hipLaunchKernelGGL(k_mask_counts, dim3(blocks32(nwords + 1, 256)), dim3(256), 0, s, mask, nwords, counts);
+}
+void launch_mark_movers(uint64_t* mask, long long lo, long long hi, hipStream_t s) {
+	if (hi <= lo) return;
+	const long long words = ((hi - 1) >> 6) - (lo >> 6) + 1;
+	hipLaunchKernelGGL(k_mark_movers, dim3(blocks32(words, 256)), dim3(256), 0, s, mask, lo, hi);
+}
+void launch_gather_movers(const uint64_t* mask, const int* prefix, const uint32_t* keys, long long n, long long nkeys, uint32_t* mk,
+                          uint32_t* mi, hipStream_t s) {
 	if (n == 0) return;
-	hipLaunchKernelGGL(k_gather_movers, dim3(blocks32(n, 256)), dim3(256), 0, s, moved, F, keys, n, (uint32_t)nkeys, mk, mi);
+	hipLaunchKernelGGL(k_gather_movers, dim3(blocks32(n, 256)), dim3(256), 0, s, mask, prefix, keys, n, (uint32_t)nkeys, mk, mi);
 }
 void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_t s) {
 	if (M == 0) return;
 	hipLaunchKernelGGL(k_first_of_run, dim3(blocks32(M, 256)), dim3(256), 0, s, mk, M, first);
 }
-void launch_rebin_counts(const int* start_old, const int* F, const int* arrive, long long nkeys, int* stay, int* counts, hipStream_t s) {
-	hipLaunchKernelGGL(k_rebin_counts, dim3(blocks32(nkeys + 1, 256)), dim3(256), 0, s, start_old, F, arrive, nkeys, stay, counts);
+void launch_rebin_counts(const int* start_old, const uint64_t* mask, const int* prefix, const int* arrive, long long nkeys, int* stay,
+                         int* counts, hipStream_t s) {
+	hipLaunchKernelGGL(k_rebin_counts, dim3(blocks32(nkeys + 1, 256)), dim3(256), 0, s, start_old, mask, prefix, arrive, nkeys, stay,
+	                   counts);
 }
-void launch_rebin_perm(long long n, const int* F, const uint32_t* keys, const int* start_old, const int* start_new, long long nkeys,
-                       int* shift, const uint32_t* mk, const uint32_t* mi, long long M, const int* stay, const int* first, uint32_t* perm,
-                       hipStream_t s) {
-	hipLaunchKernelGGL(k_rebin_shift, dim3(blocks32(nkeys, 256)), dim3(256), 0, s, start_old, start_new, F, nkeys, shift);
-	if (n > 0) hipLaunchKernelGGL(k_rebin_stayers, dim3(blocks32(n, 256)), dim3(256), 0, s, n, F, keys, shift, perm);
+void launch_rebin_perm(long long n, const uint64_t* mask, const int* prefix, const uint32_t* keys, const int* start_old,
+                       const int* start_new, long long nkeys, int* shift, const uint32_t* mk, const uint32_t* mi, long long M,
+                       const int* stay, const int* first, uint32_t* perm, hipStream_t s) {
+	hipLaunchKernelGGL(k_rebin_shift, dim3(blocks32(nkeys, 256)), dim3(256), 0, s, start_old, start_new, mask, prefix, nkeys, shift);
+	if (n > 0) hipLaunchKernelGGL(k_rebin_stayers, dim3(blocks32(n, 256)), dim3(256), 0, s, n, mask, prefix, keys, shift, perm);
 	if (M > 0)
 		hipLaunchKernelGGL(k_rebin_movers, dim3(blocks32(M, 256)), dim3(256), 0, s, mk, mi, M, start_new, stay, first, (uint32_t)nkeys, perm);
 }

@@ -33,17 +33,25 @@ struct Push32Consts {
 	int l2, l1;
 };
 
+// The push leaves a mover bit per particle in mask (word i >> 6, bit i & 63), written by each
+// wave of 64 particles.
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
-                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, int* moved, int* status, int* dead,
+                   const float* Et, const float* B, float dt, float eta, uint32_t* keys, uint64_t* mask, int* status, int* dead,
                    uint32_t* esc, hipStream_t s);
-void launch_gather_movers(const int* moved, const int* F, const uint32_t* keys, long long n, long long nkeys, uint32_t* mk, uint32_t* mi,
-                          hipStream_t s);
+// counts[w] = population of mask word w < nwords, counts[nwords] = 0
+void launch_mask_counts(const uint64_t* mask, long long nwords, int* counts, hipStream_t s);
+// mover bits of the particles lo .. hi-1 appended after the push
+void launch_mark_movers(uint64_t* mask, long long lo, long long hi, hipStream_t s);
+// prefix: exclusive scan of the word populations (nwords + 1 entries)
+void launch_gather_movers(const uint64_t* mask, const int* prefix, const uint32_t* keys, long long n, long long nkeys, uint32_t* mk,
+                          uint32_t* mi, hipStream_t s);
 void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_t s);
-void launch_rebin_counts(const int* start_old, const int* F, const int* arrive, long long nkeys, int* stay, int* counts, hipStream_t s);
+void launch_rebin_counts(const int* start_old, const uint64_t* mask, const int* prefix, const int* arrive, long long nkeys, int* stay,
+                         int* counts, hipStream_t s);
 // perm[new position] = old index; shift: scratch of nkeys ints (overwritten)
-void launch_rebin_perm(long long n, const int* F, const uint32_t* keys, const int* start_old, const int* start_new, long long nkeys,
-                       int* shift, const uint32_t* mk, const uint32_t* mi, long long M, const int* stay, const int* first, uint32_t* perm,
-                       hipStream_t s);
+void launch_rebin_perm(long long n, const uint64_t* mask, const int* prefix, const uint32_t* keys, const int* start_old,
+                       const int* start_new, long long nkeys, int* shift, const uint32_t* mk, const uint32_t* mi, long long M,
+                       const int* stay, const int* first, uint32_t* perm, hipStream_t s);
 void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
                    hipStream_t s);
 void launch_permute32(const ParticleArrays32& src, const ParticleArrays32& dst, const uint32_t* perm, hipStream_t s);
