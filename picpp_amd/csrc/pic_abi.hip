@@ -81,6 +81,11 @@ struct pic_ctx {
 	long long mover_capacity = 0;
 	int end_bit = 0;
 	int mover_bit = 0;  // radix bits of the mover keys, which include the removal key nkeys
+	// FP32: the post-push density pass also formed the next step's cell moments (rscr32,
+	// pscr32) and FP32 fields (E32, B32) from Enew, Bnew. 1: after pic_push; 2: pic_commit_fields
+	// made E, B equal to them, and the next pic_rotation_and_moments uses them. Every other call
+	// that can change particles or fields resets it to 0.
+	int fused = 0;
 	void* cub_temp = nullptr;
 	size_t cub_temp_bytes = 0;
 	std::vector<int> species_start_h;
@@ -409,11 +414,27 @@ void rebin_particles32(pic_ctx* c, int removed, bool resized) {
 	if (removed > 0 || resized) species_ranges(c, n_new);
 }
 
-void density(pic_ctx* c) {
+// after_push (FP32): the density pass also forms the next step's cell moments, from the
+// FP32 copies of Enew and Bnew (c->fused = 1).
+void density(pic_ctx* c, bool after_push = false) {
 	hipStream_t s = c->stream;
 	int nq = c->sp.count * 5 + 1;
 	double np = (double)c->np();
-	if (c->fp32) {
+	if (c->fp32 && after_push) {
+		const Grid& g = c->g;
+		{
+			Timed t(c, "fields_to_fp32", (double)(g.nnodes + g.ncells) * 3 * 12);
+			launch_f64_to_f32(c->Enew, c->E32, g.nnodes, s);
+			launch_f64_to_f32(c->Bnew, c->B32, g.ncells, s);
+		}
+		{
+			Timed t(c, "density_cell_moments", np * 28 + (double)g.ncells * (10 + nq) * 8 * 4);
+			launch_cell_moments32(g, c->sp, c->Q, c->start, c->E32, c->B32, (float)c->cfg.dt, c->rscr32, c->pscr32, s, c->dscr32);
+		}
+		c->fused = 1;
+		Timed t(c, "density_assemble", (double)g.ncells * nq * (8 * 4 + 8));
+		launch_cell_assemble32(g, c->dscr32, nq, c->dens, s);
+	} else if (c->fp32) {
 		{
 			Timed t(c, "density", np * 28 + (double)c->g.ncells * nq * 8);
 			launch_density32(c->g, c->sp, c->Q, c->start, c->dscr32, s);
@@ -443,8 +464,10 @@ void moments(pic_ctx* c) {
 	hipStream_t s = c->stream;
 	const Grid& g = c->g;
 	double np = (double)c->np();
+	const bool cached = c->fused == 2;  // cell moments and FP32 fields from the last push's density pass
+	c->fused = 0;
 	if (c->fp32) {
-		{
+		if (!cached) {
 			Timed t(c, "fields_to_fp32", (double)(g.nnodes + g.ncells) * 3 * 12);
 			launch_f64_to_f32(c->E, c->E32, g.nnodes, s);
 			launch_f64_to_f32(c->B, c->B32, g.ncells, s);
@@ -457,7 +480,7 @@ void moments(pic_ctx* c) {
 			Timed t(c, "node_assemble", (double)g.ncells * 96 * 4 + (double)g.nnodes * 12 * 8);
 			launch_node_assemble32(g, c->nscr32, c->J, c->mu, s);
 		}
-		{
+		if (!cached) {
 			Timed t(c, "cell_moments", np * 28 + (double)g.ncells * 10 * 4);
 			launch_cell_moments32(g, c->sp, c->Q, c->start, c->E32, c->B32, (float)c->cfg.dt, c->rscr32, c->pscr32, s);
 		}
@@ -944,6 +967,7 @@ void pic_destroy(pic_ctx* c) {
 
 int pic_upload_fields(pic_ctx* c, const double* E, const double* B) {
 	if (!c || !E || !B) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
 	try {
 		const Grid& g = c->g;
 		HIP_CHECK(hipMemcpyAsync(c->E, E, 3 * g.nnodes * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1023,6 +1047,7 @@ int pic_upload_particles(pic_ctx* c, long long np, const int* species, const int
 		if (c) c->error = "pic_upload_particles: bad arguments or capacity exceeded";
 		return PIC_ERR_ARGUMENT;
 	}
+	c->fused = 0;
 	double* dx = nullptr;
 	double* dp = nullptr;
 	int* di = nullptr;
@@ -1066,6 +1091,7 @@ int pic_upload_particles(pic_ctx* c, long long np, const int* species, const int
 
 int pic_upload_particles_device(pic_ctx* c, const long long* counts, const double* x, const double* p, const int* ids) {
 	if (!c || !counts || !x || !p) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
 	try {
 		std::vector<int> cnt(c->sp.count + 1, 0);
 		long long np = 0;
@@ -1126,8 +1152,7 @@ int pic_download_particles(pic_ctx* c, int* species, int* ids, double* x, double
 	return PIC_OK;
 }
 
-int pic_device_array(pic_ctx* c, int which, double** ptr, long long* count) {
-	if (!c || !ptr || !count) return PIC_ERR_ARGUMENT;
+static int array_ptr(pic_ctx* c, int which, double** ptr, long long* count) {
 	const Grid& g = c->g;
 	switch (which) {
 	case PIC_ARRAY_E: *ptr = c->E; *count = 3 * g.nnodes; break;
@@ -1148,10 +1173,19 @@ int pic_device_array(pic_ctx* c, int which, double** ptr, long long* count) {
 	return PIC_OK;
 }
 
+// The caller may write through the pointer: the cell moments formed after the last push are
+// not used.
+int pic_device_array(pic_ctx* c, int which, double** ptr, long long* count) {
+	if (!c || !ptr || !count) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
+	return array_ptr(c, which, ptr, count);
+}
+
 long long pic_download_array(pic_ctx* c, int which, double* out) {
+	if (!c) return PIC_ERR_ARGUMENT;
 	double* ptr = nullptr;
 	long long n = 0;
-	int rc = pic_device_array(c, which, &ptr, &n);
+	int rc = array_ptr(c, which, &ptr, &n);
 	if (rc) return rc;
 	if (!out) return n;
 	try {
@@ -1165,6 +1199,7 @@ long long pic_download_array(pic_ctx* c, int which, double* out) {
 
 int pic_initialize(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
 	try {
 		const Grid& g = c->g;
 		hipStream_t s = c->stream;
@@ -1196,6 +1231,7 @@ int pic_rotation_and_moments(pic_ctx* c) {
 
 int pic_field_rhs(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
 	try {
 		Timed t(c, "rhs", (double)c->g.nown * (3 + 3 + 3 + 1 + 3 + 9 + 9) * 8);
 		launch_rhs(c->g, c->E, c->B, c->J, c->rhohat, c->theta, c->cfg.dt, c->rhs, c->stream);
@@ -1400,6 +1436,7 @@ static int solve_impl(pic_ctx* c, double* b, double* x, pic_solve_info* info) {
 
 int pic_maxwell_solve(pic_ctx* c, pic_solve_info* info) {
 	if (!c) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
 	try {
 		int rc = pic_field_rhs(c);
 		if (rc) return rc;
@@ -1415,6 +1452,7 @@ int pic_maxwell_solve(pic_ctx* c, pic_solve_info* info) {
 
 int pic_gmres_solve(pic_ctx* c, const double* rhs, double* x, pic_solve_info* info) {
 	if (!c || !rhs || !x) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
 	try {
 		return solve_impl(c, const_cast<double*>(rhs), x, info);
 	} catch (const DeviceError& e) {
@@ -1424,6 +1462,7 @@ int pic_gmres_solve(pic_ctx* c, const double* rhs, double* x, pic_solve_info* in
 
 int pic_apply_operator(pic_ctx* c, const double* v, double* y) {
 	if (!c || !v || !y) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
 	try {
 		launch_pack_mu(c->g, c->mu, c->muc, c->stream);
 		launch_pack_mu_halo(c->g, c->mu, c->muh, c->stream);
@@ -1436,6 +1475,7 @@ int pic_apply_operator(pic_ctx* c, const double* v, double* y) {
 
 int pic_update_b(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
 	try {
 		Timed t(c, "update_b", (double)c->g.ncells * 6 * 8 + (double)c->g.nnodes * 3 * 8);
 		launch_update_b(c->g, c->Etheta, c->B, c->cfg.dt, c->Bnew, c->stream);
@@ -1447,6 +1487,7 @@ int pic_update_b(pic_ctx* c) {
 
 int pic_push(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
 	try {
 		if (c->fp32) {
 			const Grid& g = c->g;
@@ -1471,7 +1512,7 @@ int pic_push(pic_ctx* c) {
 			rebin_particles32(c);
 		else
 			sort_particles(c);
-		density(c);
+		density(c, true);
 	} catch (const DeviceError& e) {
 		return fail(c, e);
 	}
@@ -1480,6 +1521,7 @@ int pic_push(pic_ctx* c) {
 
 int pic_commit_fields(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
+	c->fused = c->fused == 1 ? 2 : 0;  // E, B become the Enew, Bnew the push's density pass used
 	try {
 		const Grid& g = c->g;
 		HIP_CHECK(hipMemcpyAsync(c->E, c->Enew, 3 * g.nnodes * sizeof(double), hipMemcpyDeviceToDevice, c->stream));

@@ -457,10 +457,15 @@ __device__ inline void dual_segments(const Grid& g, const int* __restrict__ star
 // The B stencil (cells b + t) is shared by the dual cell's particles and lives in registers;
 // the E stencil depends on the source cell and is read from the tile's 3 x (Tj+2) x (Tk+2)
 // box of nodes in LDS.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_cell_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
+// DENS: the same pass also deposits the post-push density (updateDensityParameters, the
+// planes of k_density32) into dens_out. Right after a push the particles sit at x^{n+1} and
+// E^{n+1}, B^{n+1} exist, so one pass gives both this step's density and the next step's cell
+// moments; rho and rhohat's corner sums are the same sums.
+template <bool DENS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DENS ? 2 : 3))) k_cell_moments32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ start,
                                                         const float4* __restrict__ E4, const float4* __restrict__ B4, float dt,
                                                         int G, int Tj, int Tk, float* __restrict__ rho_out,
-                                                        float* __restrict__ pi_out) {
+                                                        float* __restrict__ pi_out, float* __restrict__ dens_out) {
 	extern __shared__ float4 En[];  // 3 x (Tj+2) x (Tk+2) nodes from the tile's base corner
 	__shared__ float red[256 * 9];
 	const PlaneTile tile = plane_tile(g, Tj, Tk);
@@ -484,14 +489,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
 		for (int r = 0; r < 6; ++r) ap[q][r] = 0.0f;
 	}
-	if (base_ok) {
-		const float inv_v = (float)(1.0 / g.volume);
-		Stencil8 SB;
-		bool have_sb = false;
-		for (int s = 0; s < sp.count; ++s) {
-			DualSegments d;
-			dual_segments(g, start, tile.i, bj, bk, s, d);
-			if (d.p[8] == 0) continue;
+	const long long b = base_ok ? cell_id(g, tile.i, bj, bk) : 0;
+	const float inv_v = (float)(1.0 / g.volume);
+	Stencil8 SB;
+	bool have_sb = false;
+	for (int s = 0; s < sp.count; ++s) {  // every thread runs every species: DENS reduces per species
+		float conc[8], en[8], bulk[8][3];
+		if (DENS) {
+#pragma unroll
+			for (int q = 0; q < 8; ++q) conc[q] = en[q] = bulk[q][0] = bulk[q][1] = bulk[q][2] = 0.0f;
+		}
+		DualSegments d;
+		if (base_ok) dual_segments(g, start, tile.i, bj, bk, s, d);
+		if (base_ok && d.p[8] > 0) {
 			if (!have_sb) {  // cells b .. b+1 exist whenever b has particles
 				const int cj = g.N[2], ci = g.N[1] * cj;
 				load_stencil(B4, tile.i * ci + bj * cj + bk, ci, cj, SB);
@@ -499,7 +509,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 			}
 			const float m = (float)sp.mass[s], q = (float)sp.charge[s], w = (float)sp.weight[s];
 			const float beta = 0.5f * q * dt / m;
-			const float pc = q * w * inv_v;
+			const float wv = w * inv_v;
+			const float pc = q * wv;
 			for (int v = lane; v < d.p[8]; v += G) {
 				int a = 0, off = d.off[0];
 #pragma unroll
@@ -548,18 +559,47 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
 						for (int tk = 0; tk < 2; ++tk) {
 							int tt = ti * 4 + tj * 2 + tk;
-							float fr = pc * cw[0][ti] * cw[1][tj] * cw[2][tk];
+							const float corr = cw[0][ti] * cw[1][tj] * cw[2][tk];
+							const float fr = pc * corr;
 							ar[tt] += fr;
 #pragma unroll
 							for (int r = 0; r < 6; ++r) ap[tt][r] += fr * Tt[r];
+							if (DENS) {
+								const float c = wv * corr;
+								conc[tt] += c;
+								en[tt] += c * gm * m;
+#pragma unroll
+								for (int r = 0; r < 3; ++r) bulk[tt][r] += c * u[r] * m;
+							}
 						}
 			}
 		}
+		if (DENS) {  // planes per species [conc, energy, bulk x3] x 8 corners, as k_density32
+			const long long base = (long long)s * 5 * 8;
+			{
+				const float sum = group_sum8(conc, red, lt, lane, G);
+				if (base_ok && lane < 8) dens_out[(base + 0 * 8 + lane) * g.ncells + b] = sum;
+			}
+			{
+				const float sum = group_sum8(en, red, lt, lane, G);
+				if (base_ok && lane < 8) dens_out[(base + 1 * 8 + lane) * g.ncells + b] = sum;
+			}
+#pragma unroll
+			for (int r = 0; r < 3; ++r) {
+				float v8[8];
+#pragma unroll
+				for (int q = 0; q < 8; ++q) v8[q] = bulk[q][r];
+				const float sum = group_sum8(v8, red, lt, lane, G);
+				if (base_ok && lane < 8) dens_out[(base + (2 + r) * 8 + lane) * g.ncells + b] = sum;
+			}
+		}
 	}
-	const long long b = base_ok ? cell_id(g, tile.i, bj, bk) : 0;
 	{
 		const float sum = group_sum8(ar, red, lt, lane, G);
-		if (base_ok && lane < 8) rho_out[(long long)lane * g.ncells + b] = sum;
+		if (base_ok && lane < 8) {
+			rho_out[(long long)lane * g.ncells + b] = sum;
+			if (DENS) dens_out[((long long)sp.count * 5 * 8 + lane) * g.ncells + b] = sum;
+		}
 	}
 #pragma unroll
 	for (int r = 0; r < 6; ++r) {
@@ -1224,12 +1264,13 @@ PlaneTiling plane_tiling(const Grid& g, int T) {
 }
 
 void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
-                           const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s) {
+                           const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s, float* dens_scratch) {
 	int G = kMomentLanes, T = deposit_tile(G);
 	PlaneTiling pt = plane_tiling(g, T);
 	size_t box = (size_t)3 * (pt.Tj + 2) * (pt.Tk + 2) * sizeof(float4);
-	hipLaunchKernelGGL(k_cell_moments32, dim3(pt.blocks), dim3(T * G), box, s, g, sp, P, start, reinterpret_cast<const float4*>(E),
-	                   reinterpret_cast<const float4*>(B), dt, G, pt.Tj, pt.Tk, rho_scratch, pi_scratch);
+	hipLaunchKernelGGL(dens_scratch ? k_cell_moments32<true> : k_cell_moments32<false>, dim3(pt.blocks), dim3(T * G), box, s, g, sp, P,
+	                   start, reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(B), dt, G, pt.Tj, pt.Tk, rho_scratch,
+	                   pi_scratch, dens_scratch);
 }
 void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
                       hipStream_t s) {

@@ -18,8 +18,11 @@ struct ParticleArrays32 {
 int group_lanes(int nspecies);
 void launch_node_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
                            const float* B, float dt, float theta, float* scratch, hipStream_t s);
+// With dens_scratch given, the same pass also writes the post-push density planes of
+// launch_density32 (the particles at x^{n+1}, E and B the fields of step n+1).
 void launch_cell_moments32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, const float* E,
-                           const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s);
+                           const float* B, float dt, float* rho_scratch, float* pi_scratch, hipStream_t s,
+                           float* dens_scratch = nullptr);
 void launch_density32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* start, float* scratch,
                       hipStream_t s);
 // Per-launch constants of the FP32 push, formed on the host: q dt / m and q dt / (2 m) per

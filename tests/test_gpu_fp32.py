@@ -161,3 +161,44 @@ def test_fp32_rebin_keeps_particles_cell_sorted():
     assert ex <= TOL32 and ep <= 1e-4
     sim.close()
     ref.close()
+
+
+@pytest.mark.parametrize("case", ["periodic", "wall_inject"])
+def test_fp32_cell_moments_formed_after_the_push(case):
+    """After a push the FP32 density pass also forms the next step's cell moments (rhohat
+    corners and pressure partials, with E^{n+1} and B^{n+1}), and pic_rotation_and_moments then
+    uses them. They must match a fresh computation to FP32 rounding (the fused kernel is the
+    same arithmetic, compiled with the density sums beside it, so the compiler may fuse a
+    multiply-add differently): rhohat, Pi, divPi and J (through J -= eta dt div Pi) within
+    1e-6 normwise, mu (node moments, untouched) bit for bit. The post-push density of that pass
+    must match the separate density kernel (FP32 sums in another lane order) to 1e-6."""
+    if case == "periodic":
+        w = wl.mini(n=(12, 10, 8), ppc=8, seed=23)
+        st = wl.make_initial_state(w, perturb=0.05)
+    else:
+        w = wl.shock(n=(12, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=3, seed=19, name="inject")
+        st = wl.make_initial_state(w)
+    print("case", case, w.name, flush=True)
+    sim = Simulation(w, precision="fp32", pressure="cell" if case == "wall_inject" else "reference")
+    sim.load_state(st)
+    sim.initialize()
+    for _ in range(2):
+        sim.step()
+    fused_density = sim.array("density")
+    sim.updateElectroMagneticParameters()  # the cell moments of the last push's density pass
+    names = ("rhohat", "Pi", "divPi", "J", "mu")
+    cached = {nm: sim.array(nm) for nm in names}
+    sim.device_array("E")  # a caller that may write the fields: the moments are formed again
+    sim.updateElectroMagneticParameters()
+    fresh = {nm: sim.array(nm) for nm in names}
+    errs = {nm: relerr(cached[nm], fresh[nm]) for nm in names}
+    print("  cached vs fresh moments:", {k: "%.2e" % v for k, v in errs.items()})
+    for nm in names[:-1]:
+        assert errs[nm] <= 1e-6, nm
+    assert np.array_equal(cached["mu"], fresh["mu"])
+    sim.initialize()  # recomputes the post-push density with the separate kernel
+    sep = sim.array("density")
+    err = relerr(fused_density, sep)
+    print("  post-push density, fused pass vs separate kernel: %.2e" % err)
+    assert err <= 1e-6
+    sim.close()
