@@ -181,13 +181,15 @@ def open_box(w, torch, dev, **kw):
     return sim
 
 
-def train_network(args, torch, dev):
+def train_network(args, torch, dev, w=None):
     """Trains the learned preconditioner on pairs logged by the mixed-precision solves of a
-    train_cells^3 box with the same physics (16 ppc per species). The network is convolutional
-    and periodic, so its parameters carry over to the production grid."""
+    train_cells^3 box with the same physics (by default the benchmark's, 16 ppc per species).
+    The network is convolutional and periodic, so its parameters carry over to the production
+    grid."""
     from picpp_amd import workloads as wl
 
-    w = wl.config2(n=(args.train_cells,) * 3, ppc=16, steps=0)
+    if w is None:
+        w = wl.config2(n=(args.train_cells,) * 3, ppc=16, steps=0)
     sim = open_box(w, torch, dev, precision="fp32", stop_rule="true_residual", solver="mixed")
     sim.precond_create()
     sim.precond_log(args.train_pairs)
@@ -307,22 +309,37 @@ def measured_traffic(kernel, args):
 
 
 def weibel_check(args, torch, dev):
-    """Counterstreaming Weibel filamentation: the chosen form and the plain FP64 step from the
-    same state. The magnetic energy's growth rate is the steepest least-squares slope of
-    ln W_B over 20-step windows; total energy is compared step by step."""
+    """Counterstreaming Weibel filamentation: the chosen form (FP32 mover/deposit, mixed GMRES
+    with the learned preconditioner unless --learned off, trained on a train_cells^3 box of the
+    same beams) and the plain FP64 step from the same state. The magnetic energy's growth rate
+    is the steepest least-squares slope of ln W_B over 20-step windows; total energy is compared
+    step by step."""
     from picpp_amd import workloads as wl
 
     w = wl.weibel(n=(args.weibel_cells,) * 3, ppc=args.weibel_ppc, steps=args.weibel)
+    learned = args.learned != "off" and args.solver == "mixed" and args.precision == "fp32"
+    training = None
+    if learned:
+        params, mu_scale, training = train_network(args, torch, dev, wl.weibel(n=(args.train_cells,) * 3, ppc=8, steps=0))
     res = {}
+    solver_stats = {}
     for name, kw in (("chosen", dict(precision=args.precision, stop_rule=args.stop_rule, solver=args.solver)),
                      ("fp64", dict(precision="fp64", stop_rule="true_residual", solver="fp64"))):
         sim = open_box(w, torch, dev, pressure=args.pressure or "reference", **kw)
+        if name == "chosen" and learned:
+            sim.precond_create()
+            sim.set_precond_params(params, mu_scale)
+            sim.precond_use(True)
         npart = sim.num_particles()
         e = [sim.updateEnergy()[:4]]
+        its, secs, falls = [], [], 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(args.weibel):
-            sim.step()
+            info = sim.step()
+            its.append(info["iterations"])
+            secs.append(info["seconds"])
+            falls += info["fallbacks"]
             e.append(sim.updateEnergy()[:4])
             if (i + 1) % 25 == 0:
                 print("weibel %s: step %d" % (name, i + 1), file=sys.stderr, flush=True)
@@ -331,9 +348,11 @@ def weibel_check(args, torch, dev):
         del sim
         torch.cuda.empty_cache()
         res[name] = (np.array(e), el, npart)
+        solver_stats[name] = {"gmres_iterations_per_solve": float(np.mean(its)), "gmres_ms_per_solve": 1e3 * float(np.mean(secs)),
+                              "fallbacks": int(falls)}
     t = np.arange(args.weibel + 1) * w.dt
     out = {"cells": [args.weibel_cells] * 3, "particles": res["chosen"][2], "steps": args.weibel, "dt": w.dt,
-           "pressure": args.pressure or "reference"}
+           "pressure": args.pressure or "reference", "learned_preconditioner": learned, "training": training}
     for name, (e, el, npart) in res.items():
         lb = np.log(np.maximum(e[:, 2], 1e-300))
         win = 20
@@ -342,7 +361,7 @@ def weibel_check(args, torch, dev):
         k0 = max(1, len(t) // 10)
         slopes = [np.polyfit(t[k:k + win], lb[k:k + win], 1)[0] for k in range(k0, len(t) - win)]
         k = k0 + int(np.argmax(slopes))
-        out[name] = {"ms_per_step": 1e3 * el / args.weibel, "particle_steps_per_s": npart * args.weibel / el,
+        out[name] = {**solver_stats[name], "ms_per_step": 1e3 * el / args.weibel, "particle_steps_per_s": npart * args.weibel / el,
                      "magnetic_energy_growth_rate": float(slopes[k - k0]), "growth_window_steps": [k, k + win],
                      "magnetic_energy_every_10_steps": [float(v) for v in e[::10, 2]],
                      "magnetic_energy_final": float(e[-1, 2]), "total_energy_rel_change": float(np.max(np.abs(e[:, 3] - e[0, 3]) / e[0, 3]))}
