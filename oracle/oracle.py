@@ -58,6 +58,16 @@ def _load():
     lib.oracle_bspline.restype = ctypes.c_double
     lib.oracle_bspline.argtypes = [ctypes.c_double] * 4
     lib.oracle_alpha.argtypes = [ctypes.c_double, _D, ctypes.c_double, _D, _D, _D]
+    for name in ("yee_init", "yee_exchange", "tristan_b_half", "tristan_move", "tristan_flux", "tristan_e",
+                 "erase_and_exchange"):
+        getattr(lib, "oracle_" + name).argtypes = [ctypes.c_void_p]
+    lib.oracle_yee_dims.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _I]
+    lib.oracle_yee_get.restype = ctypes.c_long
+    lib.oracle_yee_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _D]
+    lib.oracle_yee_set.restype = ctypes.c_long
+    lib.oracle_yee_set.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _D]
+    lib.oracle_explicit_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.oracle_interp_yee.argtypes = [ctypes.c_void_p, _D, _D, _D]
     _lib = lib
     return lib
 
@@ -172,6 +182,31 @@ class OracleSim:
     def set_mu(self, mu: np.ndarray):
         mu = np.ascontiguousarray(mu, np.float64)
         self.lib.oracle_set_mu(self.h, _dp(mu))
+
+    # ---- explicit (BUNEMAN / Tristan) path
+    def yee_dims(self, name: str):
+        d = np.zeros(3, np.int32)
+        self.lib.oracle_yee_dims(self.h, name.encode(), _ip(d))
+        return tuple(int(v) for v in d)
+
+    def yee(self, name: str) -> np.ndarray:
+        out = np.zeros(self.lib.oracle_yee_get(self.h, name.encode(), None))
+        self.lib.oracle_yee_get(self.h, name.encode(), _dp(out))
+        return out.reshape(self.yee_dims(name))
+
+    def set_yee(self, name: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.shape == self.yee_dims(name), (name, a.shape)
+        self.lib.oracle_yee_set(self.h, name.encode(), _dp(a))
+
+    def explicit_step(self, n=1):
+        self.lib.oracle_explicit_step(self.h, n)
+
+    def interp_yee(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        E, B = np.zeros(3), np.zeros(3)
+        self.lib.oracle_interp_yee(self.h, _dp(x), _dp(E), _dp(B))
+        return E, B
 
     def row_entries(self, i, j, k, c):
         val = np.zeros(64)

@@ -174,6 +174,57 @@ void oracle_set_mu(void* h, const double* mu) {
 
 double oracle_bspline(double x, double hw, double left, double right) { return Oracle::bspline(x, hw, left, right); }
 
+// ---- explicit (BUNEMAN / Tristan) path. Yee arrays by name: Ex Ey Ez Bx By Bz Jx Jy Jz, each
+// in the reference's own layout (dims from oracle_yee_dims).
+static std::vector<double>* yee_array(Oracle& o, const std::string& n) {
+	if (n.size() != 2) return nullptr;
+	int c = n[1] - 'x';
+	if (c < 0 || c > 2) return nullptr;
+	if (n[0] == 'E') return &o.yE[c];
+	if (n[0] == 'B') return &o.yB[c];
+	if (n[0] == 'J') return &o.yJ[c];
+	return nullptr;
+}
+void oracle_yee_init(void* h) { static_cast<Oracle*>(h)->yee_init(); }
+void oracle_yee_dims(void* h, const char* name, int* dims) {
+	Oracle& o = *static_cast<Oracle*>(h);
+	o.yee_dims(name[0] == 'B', name[1] - 'x', dims);
+}
+long oracle_yee_get(void* h, const char* name, double* out) {
+	Oracle& o = *static_cast<Oracle*>(h);
+	std::vector<double>* v = yee_array(o, name);
+	if (!v) return -1;
+	if (out) std::memcpy(out, v->data(), v->size() * 8);
+	return (long)v->size();
+}
+long oracle_yee_set(void* h, const char* name, const double* in) {
+	Oracle& o = *static_cast<Oracle*>(h);
+	std::vector<double>* v = yee_array(o, name);
+	if (!v) return -1;
+	std::memcpy(v->data(), in, v->size() * 8);
+	return (long)v->size();
+}
+// exchangeBunemanBfield + exchangeBunemanEfield (the pre-loop exchanges, PIC++/simulation.cpp:83-85)
+void oracle_yee_exchange(void* h) {
+	Oracle& o = *static_cast<Oracle*>(h);
+	for (int c = 0; c < 3; ++c) {
+		int d[3];
+		o.yee_dims(true, c, d);
+		o.exchange_yee(o.yB[c], d);
+		o.yee_dims(false, c, d);
+		o.exchange_yee(o.yE[c], d);
+	}
+}
+void oracle_explicit_step(void* h, int nsteps) {
+	for (int s = 0; s < nsteps; ++s) static_cast<Oracle*>(h)->explicit_step();
+}
+void oracle_tristan_b_half(void* h) { static_cast<Oracle*>(h)->tristan_b_half(); }
+void oracle_tristan_move(void* h) { static_cast<Oracle*>(h)->move_particles_tristan(); }
+void oracle_tristan_flux(void* h) { static_cast<Oracle*>(h)->tristan_flux(); }
+void oracle_tristan_e(void* h) { static_cast<Oracle*>(h)->tristan_e(); }
+void oracle_erase_and_exchange(void* h) { static_cast<Oracle*>(h)->erase_and_exchange_particles(); }
+void oracle_interp_yee(void* h, const double* x, double* E, double* B) { static_cast<Oracle*>(h)->interp_yee(x, E, B); }
+
 void oracle_alpha(double beta, const double* v, double gamma, const double* E, const double* B, double* out9) {
 	Oracle::alpha_tensor(beta, v, gamma, E, B, out9);
 }

@@ -95,6 +95,23 @@ public:
 	void initial_sequence();  // PIC++/simulation.cpp:86-123
 	void step();              // PIC++/simulation.cpp:205-390
 
+	// ---- the explicit (BUNEMAN / Tristan) path, explicit_oracle.cpp ----
+	// Yee ("Buneman") grid, ghosted, laid out as the reference allocates it
+	// (PIC++/initialization.cpp:7145-7175): E and J component c have N cells along axis c and
+	// N + 1 along the others; B component c has N + 1 along axis c and N along the others.
+	std::vector<double> yE[3], yB[3], yJ[3];
+	void yee_dims(bool b_field, int c, int dims[3]) const;
+	size_t yidx(const int dims[3], int i, int j, int k) const { return ((size_t)i * dims[1] + j) * dims[2] + k; }
+	void yee_init();                                          // allocate the Yee arrays (zero)
+	void exchange_yee(std::vector<double>& f, const int dims[3]) const;  // exchangeBuneman{E,B}field
+	void fold_yee(std::vector<double>& f, const int dims[3]) const;      // exchangeBunemanFlux
+	void tristan_b_half();                                    // tristanEvaluateBhalfStep
+	void tristan_e();                                         // tristanEvaluateE
+	void interp_yee(const double x[3], double Ef[3], double Bf[3]) const;  // correlationBunemanEBfieldsWithoutMaps
+	void move_particles_tristan();                            // moveParticles, BUNEMAN branch
+	void tristan_flux();                                      // tristanUpdateFlux
+	void explicit_step();                                     // PIC++/simulation.cpp:163-197, then erase/exchange
+
 	size_t lvec(int i, int j, int k) const { return ((size_t)i * N[1] + j) * N[2] + k; }  // GMRES vector cell index
 
 private:
@@ -109,6 +126,8 @@ private:
 	void grad_rhohat(int i, int j, int k, double out[3]) const;
 	void rot_e(const std::vector<double>& Ef, int i, int j, int k, double out[3]) const;
 	bool move_one(Particle& p);
+	bool move_tristan(Particle& p);
+	void zigzag(const Particle& p);
 };
 
 }  // namespace picoracle

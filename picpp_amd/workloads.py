@@ -286,6 +286,48 @@ def make_initial_state(w: Workload, perturb: float = 0.0) -> InitialState:
     return InitialState(sp, np.arange(len(sp), dtype=np.int32), x, p, E, periodic_fill_cells(Bo))
 
 
+YEE_NAMES = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
+
+
+def yee_dims(w: Workload, name: str):
+    """Shape of a ghosted Yee (Buneman) array of the explicit solver, as the reference
+    allocates it (PIC++/initialization.cpp:7146-7174): E and J component c have N points along
+    axis c and N + 1 along the others; B component c the opposite."""
+    N = [v + 4 for v in w.n]
+    c = "xyz".index(name[1])
+    b = name[0] == "B"
+    return tuple(N[a] if (a == c) != b else N[a] + 1 for a in range(3))
+
+
+def yee_initial_fields(w: Workload, perturb: float = 0.0):
+    """Yee fields of the explicit solver: the uniform E0, B0 plus an optional smooth periodic
+    perturbation of relative size `perturb`, on every point of the ghosted arrays (each point
+    at its staggered position: a component's own axis at cell centres for E, nodes for B)."""
+    out = {}
+    b0 = max(1.0, float(np.linalg.norm(w.b0)))
+    for name in YEE_NAMES:
+        dims = yee_dims(w, name)
+        c = "xyz".index(name[1])
+        is_b = name[0] == "B"
+        base = (w.b0 if is_b else w.e0)[c]
+        a = np.full(dims, float(base))
+        if perturb:
+            # coordinates in units of the box, ghost index 2 at 0; cell-centred axes shifted by 1/2
+            cc = []
+            for ax in range(3):
+                half = 0.5 if (ax == c) != is_b else 0.0
+                cc.append((np.arange(dims[ax]) - 2 + half) / w.n[ax])
+            g = np.meshgrid(*cc, indexing="ij")
+            if is_b:
+                ph = 2 * math.pi * (g[0] * 1 + g[1] * (c % 2 + 1) + g[2] * 1)
+                a += perturb * b0 * np.cos(ph + 0.7 * c)
+            else:
+                ph = 2 * math.pi * (g[0] * (c + 1) + g[1] * 1 + g[2] * (2 - c % 2))
+                a += perturb * b0 * np.sin(ph + 0.3 * c)
+        out[name] = a
+    return out
+
+
 def species_arrays(w: Workload):
     """(mass, charge, weight, ptype) arrays per species."""
     return (np.array([s.mass for s in w.species], np.float64),
