@@ -80,6 +80,15 @@ enum {
 	PIC_PRESSURE_CELL = 1
 };
 
+/* time-stepping scheme */
+enum {
+	PIC_SCHEME_IMPLICIT = 0, /* the theta-implicit step (the reference's IMPLICIT solver) */
+	/* The reference's BUNEMAN solver: explicit Yee fields, the Tristan mover and the zig-zag
+	   current (PIC++/simulation.cpp:163-197). FP64, periodic, no smoothing; fields are the Yee
+	   arrays (PIC_ARRAY_YEE_*), stepped by pic_explicit_step. */
+	PIC_SCHEME_EXPLICIT = 1
+};
+
 /* orthogonalisation inside the Arnoldi process */
 enum {
 	PIC_ORTHO_MGS = 0, /* modified Gram-Schmidt, one dot + axpy per basis vector (PIC++/specialmath.cpp:295-332) */
@@ -126,6 +135,7 @@ typedef struct {
 	   shipped input uses 8 passes of s = 0.5; 0 passes is the default. */
 	int smoothing_count;
 	double smoothing_parameter;
+	int scheme;              /* PIC_SCHEME_* */
 } pic_config;
 
 typedef struct {
@@ -152,7 +162,20 @@ enum {
 	PIC_ARRAY_PI = 9,      /* pressureTensor, cells x 9 */
 	PIC_ARRAY_DENSITY = 10,/* updateDensityParameters: per species [concentration, energy, bulk v x3], then rho; cells x (5S+1) */
 	PIC_ARRAY_RHS = 11,    /* Maxwell right part, owned nodes x 3 (compact n0*n1*n2) */
-	PIC_ARRAY_SOLUTION = 12/* last GMRES solution, owned nodes x 3 */
+	PIC_ARRAY_SOLUTION = 12,/* last GMRES solution, owned nodes x 3 */
+	/* PIC_SCHEME_EXPLICIT: the ghosted Yee arrays bunemanE{x,y,z}, bunemanB{x,y,z},
+	   bunemanJ{x,y,z} in the reference's layouts (PIC++/initialization.cpp:7146-7174): E and J
+	   component c have N = n + 4 points along axis c and N + 1 along the others, B component c
+	   N + 1 along axis c and N along the others; index ((i * d1) + j) * d2 + k */
+	PIC_ARRAY_YEE_EX = 13,
+	PIC_ARRAY_YEE_EY = 14,
+	PIC_ARRAY_YEE_EZ = 15,
+	PIC_ARRAY_YEE_BX = 16,
+	PIC_ARRAY_YEE_BY = 17,
+	PIC_ARRAY_YEE_BZ = 18,
+	PIC_ARRAY_YEE_JX = 19,
+	PIC_ARRAY_YEE_JY = 20,
+	PIC_ARRAY_YEE_JZ = 21
 };
 
 /* Context lifetime. `stream` may be NULL (the library creates one). */
@@ -162,6 +185,18 @@ const char* pic_last_error(const pic_ctx* ctx);
 
 /* Host <-> device transfer (synchronous). Field arrays are ghosted component planes. */
 int pic_upload_fields(pic_ctx* ctx, const double* E, const double* B);
+/* PIC_SCHEME_EXPLICIT: the six Yee arrays Ex, Ey, Ez, Bx, By, Bz (host, the layouts of
+   PIC_ARRAY_YEE_*), then the reference's pre-loop exchanges exchangeBunemanBfield and
+   exchangeBunemanEfield (PIC++/simulation.cpp:83-85). Replaces pic_upload_fields and
+   pic_initialize for that scheme. */
+int pic_upload_yee(pic_ctx* ctx, const double* const* fields6);
+/* PIC_SCHEME_EXPLICIT: one step of the BUNEMAN loop body (PIC++/simulation.cpp:163-197 with
+   smoothingCount = 0, then eraseEscapedPaticles / exchangeParticles): tristanEvaluateBhalfStep
+   and exchangeBunemanBfield, moveParticleTristan for every particle with its zig-zag current
+   (addParticleFluxZigzag, summed in 64-bit fixed point: the same sums in every run), the
+   second half step, exchangeBunemanFlux, tristanEvaluateE and exchangeBunemanEfield, the
+   one-rank periodic wrap. Replaces the implicit phases for that scheme. */
+int pic_explicit_step(pic_ctx* ctx);
 /* Particles: species[np], ids[np], x[3][np], p[3][np] (component planes, absolute
    coordinates in [L, 2L] per axis, momentum p = m gamma v). */
 int pic_upload_particles(pic_ctx* ctx, long long np, const int* species, const int* ids, const double* x, const double* p);

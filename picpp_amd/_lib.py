@@ -20,9 +20,11 @@ ORTHO_MGS, ORTHO_CGS2 = 0, 1
 SOLVER_FP64, SOLVER_MIXED = 0, 1
 BOUNDARY_PERIODIC, BOUNDARY_WALL = 0, 1
 PRESSURE_REFERENCE, PRESSURE_CELL = 0, 1
+SCHEME_IMPLICIT, SCHEME_EXPLICIT = 0, 1
 
 ARRAYS = {"E": 0, "Etheta": 1, "Enew": 2, "B": 3, "Bnew": 4, "J": 5, "mu": 6, "divPi": 7, "rhohat": 8, "Pi": 9,
-          "density": 10, "rhs": 11, "solution": 12}
+          "density": 10, "rhs": 11, "solution": 12,
+          "Ex": 13, "Ey": 14, "Ez": 15, "Bx": 16, "By": 17, "Bz": 18, "Jx": 19, "Jy": 20, "Jz": 21}
 
 # Every symbol the header declares, with (restype, argtypes).
 _D = ctypes.POINTER(ctypes.c_double)
@@ -41,7 +43,8 @@ class PicConfig(ctypes.Structure):
                 ("solver_precision", ctypes.c_int), ("boundary_x", ctypes.c_int), ("e_right", ctypes.c_double * 3),
                 ("b_right", ctypes.c_double * 3), ("inject", ctypes.c_int), ("flow", ctypes.c_double * 3),
                 ("ppc", ctypes.c_int * 8), ("temperature", ctypes.c_double * 8), ("seed", ctypes.c_ulonglong),
-                ("pressure", ctypes.c_int), ("smoothing_count", ctypes.c_int), ("smoothing_parameter", ctypes.c_double)]
+                ("pressure", ctypes.c_int), ("smoothing_count", ctypes.c_int), ("smoothing_parameter", ctypes.c_double),
+                ("scheme", ctypes.c_int)]
 
 
 class PicSolveInfo(ctypes.Structure):
@@ -85,6 +88,8 @@ SIGNATURES = {
     "pic_precond_log": (ctypes.c_int, [_V, ctypes.c_int]),
     "pic_precond_train": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D]),
     "pic_precond_use": (ctypes.c_int, [_V, ctypes.c_int]),
+    "pic_upload_yee": (ctypes.c_int, [_V, ctypes.POINTER(_D)]),
+    "pic_explicit_step": (ctypes.c_int, [_V]),
 }
 
 _lib = None

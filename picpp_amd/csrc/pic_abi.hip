@@ -16,6 +16,7 @@
 #include "../../include/picpp_amd.h"
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
+#include "pic_explicit.hpp"
 #include "pic_kernels_fp32.hpp"
 #include "pic_inject.hpp"
 #include "pic_precond.hpp"
@@ -86,6 +87,10 @@ struct pic_ctx {
 	// made E, B equal to them, and the next pic_rotation_and_moments uses them. Every other call
 	// that can change particles or fields resets it to 0.
 	int fused = 0;
+	// PIC_SCHEME_EXPLICIT: the Yee arrays and the largest particle count of a cell
+	bool explicit_scheme = false;
+	YeeGrid yee{};
+	int* maxcount = nullptr;
 	void* cub_temp = nullptr;
 	size_t cub_temp_bytes = 0;
 	std::vector<int> species_start_h;
@@ -713,6 +718,10 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	if (cfg->smoothing_count < 0 || (cfg->smoothing_count > 0 && !(cfg->smoothing_parameter >= 0.0 && cfg->smoothing_parameter <= 1.0)))
 		return PIC_ERR_ARGUMENT;
 	if (cfg->boundary_x != PIC_BOUNDARY_PERIODIC && cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
+	if (cfg->scheme != PIC_SCHEME_IMPLICIT && cfg->scheme != PIC_SCHEME_EXPLICIT) return PIC_ERR_ARGUMENT;
+	if (cfg->scheme == PIC_SCHEME_EXPLICIT &&  // the explicit step is built FP64, periodic, without smoothing or injection
+	    (cfg->precision != PIC_PRECISION_FP64 || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->smoothing_count != 0 || cfg->inject))
+		return PIC_ERR_ARGUMENT;
 	if (cfg->inject) {  // injection needs the wall set-up and createParticle's Maxwellian branch
 		if (cfg->inject != 1 || cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
 		double v2 = cfg->flow[0] * cfg->flow[0] + cfg->flow[1] * cfg->flow[1] + cfg->flow[2] * cfg->flow[2];
@@ -757,6 +766,21 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		} else {
 			HIP_CHECK(hipStreamCreate(&c->stream));
 			c->own_stream = true;
+		}
+		if (cfg->scheme == PIC_SCHEME_EXPLICIT) {
+			c->explicit_scheme = true;
+			for (int comp = 0; comp < 3; ++comp) {
+				yee_dims(g, false, comp, c->yee.dE[comp]);
+				yee_dims(g, true, comp, c->yee.dB[comp]);
+				c->yee.E[comp] = dalloc<double>(yee_count(c->yee.dE[comp]));
+				c->yee.J[comp] = dalloc<double>(yee_count(c->yee.dE[comp]));
+				c->yee.Jacc[comp] = dalloc<long long>(yee_count(c->yee.dE[comp]));
+				c->yee.B[comp] = dalloc<double>(yee_count(c->yee.dB[comp]));
+				HIP_CHECK(hipMemset(c->yee.E[comp], 0, yee_count(c->yee.dE[comp]) * sizeof(double)));
+				HIP_CHECK(hipMemset(c->yee.J[comp], 0, yee_count(c->yee.dE[comp]) * sizeof(double)));
+				HIP_CHECK(hipMemset(c->yee.B[comp], 0, yee_count(c->yee.dB[comp]) * sizeof(double)));
+			}
+			c->maxcount = dalloc<int>(1);
 		}
 		c->nkeys = (long long)cfg->nspecies * g.ncells * 8;
 		if (c->nkeys >= (1LL << 32) - 1) throw DeviceError{PIC_ERR_ARGUMENT, "grid too large for 32-bit particle keys"};
@@ -940,6 +964,13 @@ void pic_destroy(pic_ctx* c) {
 		}
 		f(P->id);
 	}
+	for (int comp = 0; comp < 3; ++comp) {
+		f(c->yee.E[comp]);
+		f(c->yee.B[comp]);
+		f(c->yee.J[comp]);
+		f(c->yee.Jacc[comp]);
+	}
+	f(c->maxcount);
 	for (ParticleArrays32* Q : {&c->Q, &c->Qalt}) {
 		f(Q->cell);
 		for (int a = 0; a < 3; ++a) {
@@ -1168,7 +1199,15 @@ static int array_ptr(pic_ctx* c, int which, double** ptr, long long* count) {
 	case PIC_ARRAY_DENSITY: *ptr = c->dens; *count = (5 * c->sp.count + 1) * g.ncells; break;
 	case PIC_ARRAY_RHS: *ptr = c->rhs; *count = 3 * g.nown; break;
 	case PIC_ARRAY_SOLUTION: *ptr = c->xsol; *count = 3 * g.nown; break;
-	default: c->error = "unknown array id"; return PIC_ERR_ARGUMENT;
+	default:
+		if (which >= PIC_ARRAY_YEE_EX && which <= PIC_ARRAY_YEE_JZ && c->explicit_scheme) {
+			const int k = which - PIC_ARRAY_YEE_EX, comp = k % 3;
+			*ptr = k < 3 ? c->yee.E[comp] : (k < 6 ? c->yee.B[comp] : c->yee.J[comp]);
+			*count = yee_count(k >= 3 && k < 6 ? c->yee.dB[comp] : c->yee.dE[comp]);
+			break;
+		}
+		c->error = "unknown array id";
+		return PIC_ERR_ARGUMENT;
 	}
 	return PIC_OK;
 }
@@ -1558,8 +1597,88 @@ int pic_energy(pic_ctx* c, double* out7) {
 	return PIC_OK;
 }
 
+int pic_upload_yee(pic_ctx* c, const double* const* fields6) {
+	if (!c || !fields6) return PIC_ERR_ARGUMENT;
+	if (!c->explicit_scheme) {
+		c->error = "pic_upload_yee: the context was not created with PIC_SCHEME_EXPLICIT";
+		return PIC_ERR_ARGUMENT;
+	}
+	try {
+		for (int k = 0; k < 6; ++k) {
+			const int comp = k % 3;
+			if (!fields6[k]) throw DeviceError{PIC_ERR_ARGUMENT, "pic_upload_yee: null field"};
+			double* dst = k < 3 ? c->yee.E[comp] : c->yee.B[comp];
+			const long long n = yee_count(k < 3 ? c->yee.dE[comp] : c->yee.dB[comp]);
+			HIP_CHECK(hipMemcpyAsync(dst, fields6[k], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+		}
+		launch_yee_exchange(c->g, c->yee, true, c->stream);  // PIC++/simulation.cpp:83-85
+		launch_yee_exchange(c->g, c->yee, false, c->stream);
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_explicit_step(pic_ctx* c) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	if (!c->explicit_scheme) {
+		c->error = "pic_explicit_step: the context was not created with PIC_SCHEME_EXPLICIT";
+		return PIC_ERR_ARGUMENT;
+	}
+	try {
+		const Grid& g = c->g;
+		hipStream_t s = c->stream;
+		const double dt = c->cfg.dt;
+		{
+			Timed t(c, "yee_b_half", (double)g.ncells * 3 * 8 * 3);
+			launch_tristan_b_half(g, c->yee, dt, s);
+			launch_yee_exchange(g, c->yee, true, s);
+		}
+		// fixed-point scale of the current: an entry takes at most 2 parts from each particle
+		// whose cell before or after the move touches it (4 + 4 cells; the particles of the 4
+		// cells after the move come from their 27 neighbours), each part at most 2 |q w / V|:
+		// below 512 m |q w / V| with m particles in the fullest cell
+		int maxcount = 0;
+		HIP_CHECK(hipMemsetAsync(c->maxcount, 0, sizeof(int), s));
+		launch_max_cell_count(g, c->sp.count, c->start, c->maxcount, s);
+		HIP_CHECK(hipMemcpyAsync(&maxcount, c->maxcount, sizeof(int), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		double fcd = 0;
+		for (int q = 0; q < c->sp.count; ++q) fcd = std::max(fcd, std::fabs(c->sp.charge[q] * c->sp.weight[q] / g.volume));
+		const double bound = 512.0 * std::max(maxcount, 1) * std::max(fcd, 1e-300);
+		const double scale = std::ldexp(1.0, (int)std::floor(62.0 - std::log2(bound)));
+		for (int comp = 0; comp < 3; ++comp)
+			HIP_CHECK(hipMemsetAsync(c->yee.Jacc[comp], 0, yee_count(c->yee.dE[comp]) * sizeof(long long), s));
+		{
+			Timed t(c, "tristan_push", (double)c->P.count * 100 + (double)g.ncells * 6 * 8 * 8);
+			launch_tristan_push(g, c->sp, c->P, c->species_start, c->yee, dt, scale, c->keys, c->status, s);
+		}
+		check_status(c);
+		{
+			Timed t(c, "yee_b_half", (double)g.ncells * 3 * 8 * 3);
+			launch_tristan_b_half(g, c->yee, dt, s);
+			launch_yee_exchange(g, c->yee, true, s);
+		}
+		{
+			Timed t(c, "yee_current", (double)g.ncells * 3 * 16);
+			launch_yee_current(g, c->yee, 1.0 / scale, s);
+		}
+		{
+			Timed t(c, "yee_e", (double)g.ncells * 3 * 8 * 4);
+			launch_tristan_e(g, c->yee, dt, s);
+			launch_yee_exchange(g, c->yee, false, s);
+		}
+		sort_particles(c);
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
 int pic_step(pic_ctx* c, pic_solve_info* info) {
 	int rc;
+	if (c && c->explicit_scheme) return pic_explicit_step(c);
 	if ((rc = pic_rotation_and_moments(c))) return rc;
 	if ((rc = pic_maxwell_solve(c, info))) return rc;
 	if ((rc = pic_update_b(c))) return rc;

@@ -1,0 +1,413 @@
+// The explicit (BUNEMAN / Tristan) step of PIC++ on the Yee grid, FP64:
+// - tristanEvaluateBhalfStep / tristanEvaluateE (PIC++/fields.cpp:21-150, curls at
+//   fields.cpp:3273-3338) as one thread per point of the owned box;
+// - exchangeBuneman{E,B}field and exchangeBunemanFlux, one-rank periodic branches
+//   (PIC++/fields_exchange.cpp:2523-4200, momentum_exchange.cpp:1806-2310), one thread per
+//   line along the exchanged axis;
+// - moveParticleTristan (PIC++/mover.cpp:718-808) with the field interpolation of
+//   correlationBunemanEBfieldsWithoutMaps (correlation.cpp:441-626), fused with the zig-zag
+//   current of the same move (addParticleFluxZigzag, momentum_collecting.cpp:85-249).
+//
+// The zig-zag current is summed without floating-point atomics: every contribution is
+// rounded to a 64-bit fixed-point value (a power-of-two scale chosen per step from the
+// largest number of particles in a cell, so that no entry can overflow) and added with
+// integer atomics. Integer addition is associative, so the sums, and everything after them,
+// are the same in every run whatever the order of the atomics. Each contribution is rounded
+// by at most half a unit of 2^-62 times the bound on an entry, far below FP64 rounding of the
+// entry itself at these counts.
+#include <math.h>
+
+#include "pic_explicit.hpp"
+
+namespace picgpu {
+
+namespace {
+
+constexpr double kCprimeX = 1.025;  // speed_of_light_correction (PIC++/constants.h:45)
+
+inline unsigned nb(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+__device__ inline long long yi(const int d[3], int i, int j, int k) { return ((long long)i * d[1] + j) * d[2] + k; }
+
+// tristanEvaluateBhalfStep over the owned box i, j, k in [2, N-2]: Bx on i <= N-2, j, k < N-2;
+// By on j <= N-2; Bz on k <= N-2 (fields.cpp:26-74).
+__global__ void k_b_half(Grid g, YeeGrid y, double cdtd2) {
+	const int ni = g.N[0] - 3, nj = g.N[1] - 3, nk = g.N[2] - 3;
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long long)ni * nj * nk) return;
+	const int k = 2 + (int)(t % nk), j = 2 + (int)((t / nk) % nj), i = 2 + (int)(t / ((long long)nk * nj));
+	const int hx = g.N[0] - 2, hy = g.N[1] - 2, hz = g.N[2] - 2;
+	const double* Ex = y.E[0];
+	const double* Ey = y.E[1];
+	const double* Ez = y.E[2];
+	if (j < hy && k < hz) {  // evaluateBunemanRotEx (fields.cpp:3273-3284)
+		double r = 0;
+		r += (Ez[yi(y.dE[2], i, j + 1, k)] - Ez[yi(y.dE[2], i, j, k)]) / g.d[1];
+		r -= (Ey[yi(y.dE[1], i, j, k + 1)] - Ey[yi(y.dE[1], i, j, k)]) / g.d[2];
+		double& b = y.B[0][yi(y.dB[0], i, j, k)];
+		b = b - r * cdtd2;
+	}
+	if (i < hx && k < hz) {  // evaluateBunemanRotEy (:3286-3294)
+		double r = -((Ez[yi(y.dE[2], i + 1, j, k)] - Ez[yi(y.dE[2], i, j, k)]) / g.d[0]);
+		r += (Ex[yi(y.dE[0], i, j, k + 1)] - Ex[yi(y.dE[0], i, j, k)]) / g.d[2];
+		double& b = y.B[1][yi(y.dB[1], i, j, k)];
+		b = b - r * cdtd2;
+	}
+	if (i < hx && j < hy) {  // evaluateBunemanRotEz (:3296-3304)
+		double r = (Ey[yi(y.dE[1], i + 1, j, k)] - Ey[yi(y.dE[1], i, j, k)]) / g.d[0];
+		r -= (Ex[yi(y.dE[0], i, j + 1, k)] - Ex[yi(y.dE[0], i, j, k)]) / g.d[1];
+		double& b = y.B[2][yi(y.dB[2], i, j, k)];
+		b = b - r * cdtd2;
+	}
+}
+
+// tristanEvaluateE over the owned box: Ex on i < N-2, Ey on j < N-2, Ez on k < N-2
+// (fields.cpp:81-150, periodic x).
+__global__ void k_e(Grid g, YeeGrid y, double dt, double four_pi) {
+	const int ni = g.N[0] - 3, nj = g.N[1] - 3, nk = g.N[2] - 3;
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long long)ni * nj * nk) return;
+	const int k = 2 + (int)(t % nk), j = 2 + (int)((t / nk) % nj), i = 2 + (int)(t / ((long long)nk * nj));
+	const int hx = g.N[0] - 2, hy = g.N[1] - 2, hz = g.N[2] - 2;
+	const double* Bx = y.B[0];
+	const double* By = y.B[1];
+	const double* Bz = y.B[2];
+	if (i < hx) {  // evaluateBunemanRotBx (fields.cpp:3306-3317)
+		double r = 0;
+		r += (Bz[yi(y.dB[2], i, j, k)] - Bz[yi(y.dB[2], i, j - 1, k)]) / g.d[1];
+		r -= (By[yi(y.dB[1], i, j, k)] - By[yi(y.dB[1], i, j, k - 1)]) / g.d[2];
+		const long long q = yi(y.dE[0], i, j, k);
+		y.E[0][q] = y.E[0][q] + (kCprimeX * r - four_pi * y.J[0][q]) * dt;
+	}
+	if (j < hy) {  // evaluateBunemanRotBy (:3319-3327)
+		double r = -((Bz[yi(y.dB[2], i, j, k)] - Bz[yi(y.dB[2], i - 1, j, k)]) / g.d[0]);
+		r += (Bx[yi(y.dB[0], i, j, k)] - Bx[yi(y.dB[0], i, j, k - 1)]) / g.d[2];
+		const long long q = yi(y.dE[1], i, j, k);
+		y.E[1][q] = y.E[1][q] + (kCprimeX * r - four_pi * y.J[1][q]) * dt;
+	}
+	if (k < hz) {  // evaluateBunemanRotBz (:3329-3338)
+		double r = (By[yi(y.dB[1], i, j, k)] - By[yi(y.dB[1], i - 1, j, k)]) / g.d[0];
+		r -= (Bx[yi(y.dB[0], i, j, k)] - Bx[yi(y.dB[0], i, j - 1, k)]) / g.d[1];
+		const long long q = yi(y.dE[2], i, j, k);
+		y.E[2][q] = y.E[2][q] + (kCprimeX * r - four_pi * y.J[2][q]) * dt;
+	}
+}
+
+struct Yee3 {
+	double* f[3];
+	int d[3][3];
+};
+
+// One line along axis ax of component c; lines of the three components one after another.
+__device__ inline bool line_of(const Yee3& y, int ax, long long t, int& c, int idx[3]) {
+	const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+	for (c = 0; c < 3; ++c) {
+		const long long lines = (long long)y.d[c][o1] * y.d[c][o2];
+		if (t < lines) {
+			idx[o1] = (int)(t / y.d[c][o2]);
+			idx[o2] = (int)(t % y.d[c][o2]);
+			return true;
+		}
+		t -= lines;
+	}
+	return false;
+}
+
+// Periodic ghost copy along ax: an axis of N points (cells) takes f[N-2+i] = f[2+i],
+// f[i] = f[N-4+i] (i = 0, 1); an axis of N + 1 points (nodes) f[N-1+i] = f[3+i],
+// f[i] = f[N-4+i], then f[N-2] = f[2].
+__global__ void k_yee_exchange(Yee3 y, int ax, int Nax) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	int c, idx[3];
+	if (!line_of(y, ax, t, c, idx)) return;
+	const int* d = y.d[c];
+	double* f = y.f[c];
+	const bool node = d[ax] == Nax + 1;
+	auto at = [&](int v) -> double& {
+		idx[ax] = v;
+		return f[yi(d, idx[0], idx[1], idx[2])];
+	};
+	for (int i = 0; i <= 1; ++i) {
+		if (node)
+			at(Nax - 1 + i) = at(3 + i);
+		else
+			at(Nax - 2 + i) = at(2 + i);
+		at(i) = at(Nax - 4 + i);
+	}
+	if (node) at(Nax - 2) = at(2);
+}
+
+// Periodic fold of the current along ax: N points f[N-1-i] += f[3-i], f[3-i] = f[N-1-i]
+// (i = 0..3); N + 1 points f[N-i] += f[4-i], f[4-i] = f[N-i] (i = 0..4).
+__global__ void k_yee_fold(Yee3 y, int ax, int Nax) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	int c, idx[3];
+	if (!line_of(y, ax, t, c, idx)) return;
+	const int* d = y.d[c];
+	double* f = y.f[c];
+	const bool node = d[ax] == Nax + 1;
+	auto at = [&](int v) -> double& {
+		idx[ax] = v;
+		return f[yi(d, idx[0], idx[1], idx[2])];
+	};
+	if (node) {
+		for (int i = 0; i <= 4; ++i) {
+			at(Nax - i) += at(4 - i);
+			at(4 - i) = at(Nax - i);
+		}
+	} else {
+		for (int i = 0; i <= 3; ++i) {
+			at(Nax - 1 - i) += at(3 - i);
+			at(3 - i) = at(Nax - 1 - i);
+		}
+	}
+}
+
+__global__ void k_fixed_to_double(const long long* __restrict__ acc, double* __restrict__ J, long long n, double inv_scale) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	J[t] = (double)acc[t] * inv_scale;
+}
+
+__global__ void k_max_cell_count(Grid g, int nspecies, const int* __restrict__ start, int* __restrict__ out) {
+	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.ncells) return;
+	int n = 0;
+	for (int s = 0; s < nspecies; ++s) {
+		const long long k0 = ((long long)s * g.ncells + c) << 3;
+		n += start[k0 + 8] - start[k0];
+	}
+	if (n > 0) atomicMax(out, n);
+}
+
+__device__ inline void add_fixed(long long* acc, long long q, double v, double scale) {
+	atomicAdd(reinterpret_cast<unsigned long long*>(acc + q), (unsigned long long)llrint(v * scale));
+}
+
+__global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start,
+                                                      YeeGrid y, double dt, double scale, uint32_t* __restrict__ keys,
+                                                      int* __restrict__ status) {
+	const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= P.count) return;
+	const int i = (int)idx;
+	int s = 0;
+	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+	const double m = sp.mass[s], q = sp.charge[s];
+	double x[3], p[3];
+	for (int a = 0; a < 3; ++a) {
+		x[a] = P.x[a][i];
+		p[a] = P.p[a][i];
+	}
+	// correlationBunemanEBfieldsWithoutMaps: cell and node maps per axis
+	int lc[3], rc[3], ln[3], rn[3];
+	double wlc[3], wrc[3], wln[3], wrn[3];
+	bool ok = true;
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		const double g0 = g.x0[a], d = g.d[a];
+		int cnt = (int)floor((x[a] - g0) / d);
+		const double midc = ((g0 + cnt * d) + (g0 + (cnt + 1) * d)) / 2;
+		if (x[a] < midc) {
+			lc[a] = cnt - 1;
+			rc[a] = cnt;
+			wrc[a] = 0.5 + (x[a] - (g0 + cnt * d)) / d;
+			wlc[a] = 1.0 - wrc[a];
+		} else {
+			lc[a] = cnt;
+			rc[a] = cnt + 1;
+			wlc[a] = 0.5 + ((g0 + (cnt + 1) * d) - x[a]) / d;
+			wrc[a] = 1.0 - wlc[a];
+		}
+		cnt = (int)floor(((x[a] - g0) / d) + 0.5);
+		if (x[a] < g0 + cnt * d) {
+			ln[a] = cnt - 1;
+			rn[a] = cnt;
+			wrn[a] = 0.5 + (x[a] - ((g0 + (cnt - 1) * d) + (g0 + cnt * d)) / 2) / d;
+			wln[a] = 1.0 - wrn[a];
+		} else {
+			ln[a] = cnt;
+			rn[a] = cnt + 1;
+			wln[a] = 0.5 + (((g0 + cnt * d) + (g0 + (cnt + 1) * d)) / 2 - x[a]) / d;
+			wrn[a] = 1.0 - wln[a];
+		}
+		// stencils inside the arrays: cells 0 .. N-1 need lc >= 0, rc <= N-1; nodes ln >= 0, rn <= N
+		if (lc[a] < 0 || rc[a] > g.N[a] - 1 || ln[a] < 0 || rn[a] > g.N[a]) ok = false;
+	}
+	if (!ok) {  // no access outside the arrays: report and leave the particle
+		atomicOr(status, 8);
+		return;
+	}
+	double Ef[3] = {0, 0, 0}, Bf[3] = {0, 0, 0};
+	for (int sx = 0; sx < 2; ++sx)
+		for (int sy = 0; sy < 2; ++sy)
+			for (int sz = 0; sz < 2; ++sz) {
+				const int sv[3] = {sx, sy, sz};
+#pragma unroll
+				for (int c = 0; c < 3; ++c) {
+					int ie[3], ib[3];
+					double we[3], wb[3];
+#pragma unroll
+					for (int a = 0; a < 3; ++a) {
+						const int ci = sv[a] ? rc[a] : lc[a], ni = sv[a] ? rn[a] : ln[a];
+						const double cw = sv[a] ? wrc[a] : wlc[a], nw = sv[a] ? wrn[a] : wln[a];
+						ie[a] = a == c ? ci : ni;
+						we[a] = a == c ? cw : nw;
+						ib[a] = a == c ? ni : ci;
+						wb[a] = a == c ? nw : cw;
+					}
+					Ef[c] += y.E[c][yi(y.dE[c], ie[0], ie[1], ie[2])] * we[0] * we[1] * we[2];
+					Bf[c] += y.B[c][yi(y.dB[c], ib[0], ib[1], ib[2])] * wb[0] * wb[1] * wb[2];
+				}
+			}
+	// moveParticleTristan: half kick, rotation, half kick
+	const double hs = q * dt * 0.5;
+	double dp[3];
+	for (int a = 0; a < 3; ++a) {
+		dp[a] = Ef[a] * hs;
+		p[a] = p[a] + dp[a];
+	}
+	const double gamma = sqrt((p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / (m * m) + 1.0);
+	const double beta = q * dt / (2.0 * m);
+	const double bs = beta / gamma;
+	const double f = 2.0 / sqrt(1.0 + bs * bs * (Bf[0] * Bf[0] + Bf[1] * Bf[1] + Bf[2] * Bf[2]));
+	double c1[3] = {p[1] * Bf[2] - p[2] * Bf[1], -p[0] * Bf[2] + p[2] * Bf[0], p[0] * Bf[1] - p[1] * Bf[0]};
+	double tm[3];
+	for (int a = 0; a < 3; ++a) tm[a] = p[a] + c1[a] * bs;
+	double c2[3] = {tm[1] * Bf[2] - tm[2] * Bf[1], -tm[0] * Bf[2] + tm[2] * Bf[0], tm[0] * Bf[1] - tm[1] * Bf[0]};
+	const double fb = f * bs;
+	for (int a = 0; a < 3; ++a) p[a] += c2[a] * fb + dp[a];
+	const double gn = sqrt((p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / (m * m) + 1.0);
+	double v[3];
+	for (int a = 0; a < 3; ++a) {
+		v[a] = p[a] / (m * gn);
+		x[a] += v[a] * dt;
+	}
+	// addParticleFluxZigzag at the unwrapped position
+	const double fcd = q * sp.weight[s] / g.volume;
+	double loc[3], prev[3], r[3];
+	int cur[3], pre[3];
+	for (int a = 0; a < 3; ++a) {
+		loc[a] = x[a] - g.x0[a];
+		cur[a] = (int)floor(loc[a] / g.d[a]);
+		prev[a] = loc[a] - v[a] * dt;
+		pre[a] = (int)floor(prev[a] / g.d[a]);
+		r[a] = (loc[a] + prev[a]) / 2;
+		if (cur[a] != pre[a]) r[a] = fmax(g.x0[a] + pre[a] * g.d[a], g.x0[a] + cur[a] * g.d[a]) - g.x0[a];
+		// J stencils of both parts inside J's arrays (cells 0 .. N-2 along the component's
+		// own axis, nodes 0 .. N-1 along the others)
+		if (cur[a] < 0 || pre[a] < 0 || cur[a] > g.N[a] - 2 || pre[a] > g.N[a] - 2) ok = false;
+	}
+	if (!ok) {
+		atomicOr(status, 8);
+		return;
+	}
+	double F1[3], F2[3], W1[3], W2[3];
+	for (int a = 0; a < 3; ++a) F1[a] = fcd * (r[a] - prev[a]) / dt;
+	F2[0] = fcd * (loc[0] - r[0]) / dt;
+	F2[1] = fcd * v[1] - F1[1];
+	F2[2] = fcd * v[2] - F1[2];
+	for (int a = 0; a < 3; ++a) {
+		W1[a] = (prev[a] + r[a]) / (2 * g.d[a]) - pre[a];
+		W2[a] = (loc[a] + r[a]) / (2 * g.d[a]) - cur[a];
+	}
+	for (int part = 0; part < 2; ++part) {
+		const int* cc = part ? cur : pre;
+		const double* F = part ? F2 : F1;
+		const double* W = part ? W2 : W1;
+		const double om[3] = {1.0 - W[0], 1.0 - W[1], 1.0 - W[2]};
+		const int a = cc[0], b = cc[1], c = cc[2];
+		long long* Jx = y.Jacc[0];
+		long long* Jy = y.Jacc[1];
+		long long* Jz = y.Jacc[2];
+		add_fixed(Jx, yi(y.dE[0], a, b, c), F[0] * om[1] * om[2], scale);
+		add_fixed(Jx, yi(y.dE[0], a, b + 1, c), F[0] * W[1] * om[2], scale);
+		add_fixed(Jx, yi(y.dE[0], a, b, c + 1), F[0] * om[1] * W[2], scale);
+		add_fixed(Jx, yi(y.dE[0], a, b + 1, c + 1), F[0] * W[1] * W[2], scale);
+		add_fixed(Jy, yi(y.dE[1], a, b, c), F[1] * om[0] * om[2], scale);
+		add_fixed(Jy, yi(y.dE[1], a + 1, b, c), F[1] * W[0] * om[2], scale);
+		add_fixed(Jy, yi(y.dE[1], a, b, c + 1), F[1] * om[0] * W[2], scale);
+		add_fixed(Jy, yi(y.dE[1], a + 1, b, c + 1), F[1] * W[0] * W[2], scale);
+		add_fixed(Jz, yi(y.dE[2], a, b, c), F[2] * om[0] * om[1], scale);
+		add_fixed(Jz, yi(y.dE[2], a, b + 1, c), F[2] * om[0] * W[1], scale);
+		add_fixed(Jz, yi(y.dE[2], a + 1, b, c), F[2] * W[0] * om[1], scale);
+		add_fixed(Jz, yi(y.dE[2], a + 1, b + 1, c), F[2] * W[0] * W[1], scale);
+	}
+	// one-rank periodic wrap (mover.cpp:770-806 classification, exchangeParticles' shift)
+	for (int a = 0; a < 3; ++a) {
+		const double lo = g.x0[a] + 2 * g.d[a], hi = g.x0[a] + (g.N[a] - 2) * g.d[a];
+		if (x[a] < lo)
+			x[a] += g.L[a];
+		else if (x[a] > hi)
+			x[a] -= g.L[a];
+		if (!(x[a] >= lo && x[a] <= hi)) atomicOr(status, 1);  // moved more than a box length
+	}
+	for (int a = 0; a < 3; ++a) {
+		P.x[a][i] = x[a];
+		P.p[a][i] = p[a];
+	}
+	AxisMap mp[3];
+	for (int a = 0; a < 3; ++a) mp[a] = axis_map(x[a], g.x0[a], g.d[a]);
+	keys[i] = particle_key(g, s, mp);
+}
+
+}  // namespace
+
+void yee_dims(const Grid& g, bool b_field, int c, int dims[3]) {
+	for (int a = 0; a < 3; ++a) dims[a] = ((a == c) != b_field) ? g.N[a] : g.N[a] + 1;
+}
+
+void launch_tristan_b_half(const Grid& g, const YeeGrid& y, double dt, hipStream_t s) {
+	const long long n = (long long)(g.N[0] - 3) * (g.N[1] - 3) * (g.N[2] - 3);
+	hipLaunchKernelGGL(k_b_half, dim3(nb(n, 256)), dim3(256), 0, s, g, y, kCprimeX * dt / 2.0);
+}
+
+void launch_tristan_e(const Grid& g, const YeeGrid& y, double dt, hipStream_t s) {
+	const long long n = (long long)(g.N[0] - 3) * (g.N[1] - 3) * (g.N[2] - 3);
+	const double pi = 4 * atan2(1.0, 1.0);  // PIC++/constants.h:28-29
+	hipLaunchKernelGGL(k_e, dim3(nb(n, 256)), dim3(256), 0, s, g, y, dt, 4 * pi);
+}
+
+static void yee_axes(const Grid& g, Yee3& q, bool fold, hipStream_t s) {
+	for (int ax = 0; ax < 3; ++ax) {
+		const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+		long long lines = 0;
+		for (int c = 0; c < 3; ++c) lines += (long long)q.d[c][o1] * q.d[c][o2];
+		if (fold)
+			hipLaunchKernelGGL(k_yee_fold, dim3(nb(lines, 256)), dim3(256), 0, s, q, ax, g.N[ax]);
+		else
+			hipLaunchKernelGGL(k_yee_exchange, dim3(nb(lines, 256)), dim3(256), 0, s, q, ax, g.N[ax]);
+	}
+}
+
+void launch_yee_exchange(const Grid& g, const YeeGrid& y, bool b_field, hipStream_t s) {
+	Yee3 q;
+	for (int c = 0; c < 3; ++c) {
+		q.f[c] = b_field ? y.B[c] : y.E[c];
+		for (int a = 0; a < 3; ++a) q.d[c][a] = b_field ? y.dB[c][a] : y.dE[c][a];
+	}
+	yee_axes(g, q, false, s);
+}
+
+void launch_yee_current(const Grid& g, const YeeGrid& y, double inv_scale, hipStream_t s) {
+	Yee3 q;
+	for (int c = 0; c < 3; ++c) {
+		const long long n = yee_count(y.dE[c]);
+		hipLaunchKernelGGL(k_fixed_to_double, dim3(nb(n, 256)), dim3(256), 0, s, y.Jacc[c], y.J[c], n, inv_scale);
+		q.f[c] = y.J[c];
+		for (int a = 0; a < 3; ++a) q.d[c][a] = y.dE[c][a];
+	}
+	yee_axes(g, q, true, s);
+}
+
+void launch_max_cell_count(const Grid& g, int nspecies, const int* start, int* out, hipStream_t s) {
+	hipLaunchKernelGGL(k_max_cell_count, dim3(nb(g.ncells, 256)), dim3(256), 0, s, g, nspecies, start, out);
+}
+
+void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
+                         const YeeGrid& y, double dt, double scale, uint32_t* keys, int* status, hipStream_t s) {
+	if (P.count == 0) return;
+	hipLaunchKernelGGL(k_tristan_push, dim3(nb(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, y, dt, scale, keys,
+	                   status);
+}
+
+}  // namespace picgpu

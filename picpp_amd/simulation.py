@@ -37,7 +37,7 @@ class Simulation:
     def __init__(self, workload: Workload, precision: str = "fp64", stop_rule: str = "reference", ortho: str = "mgs",
                  capacity: Optional[int] = None, gmres_tolerance: Optional[float] = None,
                  gmres_max_iterations: int = 50, stream: Optional[int] = None, solver: str = "fp64",
-                 pressure: str = "reference"):
+                 pressure: str = "reference", scheme: str = "implicit"):
         self.lib = _lib.load()
         self.w = workload
         cfg = _lib.PicConfig()
@@ -59,6 +59,7 @@ class Simulation:
         cfg.pressure = {"reference": _lib.PRESSURE_REFERENCE, "cell": _lib.PRESSURE_CELL}[pressure]
         cfg.smoothing_count = workload.smoothing_count
         cfg.smoothing_parameter = workload.smoothing_parameter
+        cfg.scheme = {"implicit": _lib.SCHEME_IMPLICIT, "explicit": _lib.SCHEME_EXPLICIT}[scheme]
         for a in range(3):
             cfg.e_right[a] = workload.e_right[a]
             cfg.b_right[a] = workload.b_right[a]
@@ -196,6 +197,34 @@ class Simulation:
         self._chk(self.lib.pic_step(self.h, ctypes.byref(info)), "pic_step")
         self.last_solve = _info_dict(info)
         return self.last_solve
+
+    # ------------------------------------------------------------ explicit (BUNEMAN) scheme
+    def load_yee(self, fields: Dict[str, np.ndarray]):
+        """Yee arrays Ex Ey Ez Bx By Bz (workloads.yee_dims layouts), then the reference's
+        pre-loop exchanges (PIC++/simulation.cpp:83-85)."""
+        from .workloads import YEE_NAMES, yee_dims
+        keep = []
+        for name in YEE_NAMES:
+            a = np.ascontiguousarray(fields[name], np.float64)
+            if a.shape != yee_dims(self.w, name):
+                raise ValueError("%s: shape %s, expected %s" % (name, a.shape, yee_dims(self.w, name)))
+            keep.append(a)
+        ptrs = (_lib._D * 6)(*[a.ctypes.data_as(_lib._D) for a in keep])
+        self._chk(self.lib.pic_upload_yee(self.h, ptrs), "pic_upload_yee")
+
+    def yee(self, name: str) -> np.ndarray:
+        """A Yee array (Ex .. Bz, Jx .. Jz) in its ghosted layout."""
+        from .workloads import yee_dims
+        which = _lib.ARRAYS[name]
+        n = self.lib.pic_download_array(self.h, which, None)
+        self._chk(n, "pic_download_array")
+        out = np.zeros(n)
+        self._chk(self.lib.pic_download_array(self.h, which, _dp(out)), "pic_download_array")
+        return out.reshape(yee_dims(self.w, name))
+
+    def explicit_step(self):
+        """One step of the reference's BUNEMAN loop body (PIC++/simulation.cpp:163-197)."""
+        self._chk(self.lib.pic_explicit_step(self.h), "pic_explicit_step")
 
     def synchronize(self):
         self._chk(self.lib.pic_synchronize(self.h), "pic_synchronize")

@@ -1,0 +1,83 @@
+"""The HIP explicit (BUNEMAN / Tristan) step (PIC_SCHEME_EXPLICIT, pic_explicit_step) against
+the compiled reference's own explicit step (tests/golden/explicit_ref.npz, made by
+oracle/ref_harness_explicit.cpp; the CPU restatement oracle/explicit_oracle.cpp matches it
+bit for bit, tests/test_explicit_oracle.py).
+
+The case: 6x5x5 cells, 4 ppc per species, dt = 0.4, fast electrons (u ~ N(0, 0.5) per axis)
+so that many zig-zag paths cross cells and some particles leave through every face, perturbed
+Yee fields, 6 steps. Particles (matched by id), the Yee E and B after every step and the Yee
+current of the first two steps must agree to 1e-12 normwise; the fixed-point current makes
+two runs bitwise identical.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+from picpp_amd import workloads as wl
+from picpp_amd.simulation import Simulation
+from test_explicit_oracle import explicit_workload
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def open_explicit(d, w):
+    N = tuple(v + 4 for v in w.n)
+    st = wl.InitialState(d["in/species"].astype(np.int32), d["in/ids"].astype(np.int32), d["in/x"], d["in/p"],
+                         np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3)), np.zeros(N + (3,)))
+    sim = Simulation(w, precision="fp64", scheme="explicit")
+    sim.load_state(st)
+    sim.load_yee({name: d["in/" + name] for name in wl.YEE_NAMES})
+    return sim
+
+
+def by_id(ids, x, p):
+    o = np.argsort(ids)
+    return ids[o], x[o], p[o]
+
+
+def test_explicit_step_matches_reference():
+    d = np.load(os.path.join(HERE, "golden", "explicit_ref.npz"))
+    w = explicit_workload()
+    print("case", w.name, "particles", len(d["in/ids"]), flush=True)
+    sim = open_explicit(d, w)
+    worst = {}
+    for name in wl.YEE_NAMES:
+        worst[name + "0"] = relerr(sim.yee(name), d["s0/" + name])
+    for step in range(1, w.steps + 1):
+        sim.explicit_step()
+        p = sim.particles()
+        ids, x, mom = by_id(p["ids"], p["x"], p["p"])
+        ref = d["s%d/particles" % step]
+        ref = ref[np.argsort(ref[:, 0])]
+        assert np.array_equal(ids.astype(np.float64), ref[:, 0]), step
+        worst["x"] = max(worst.get("x", 0.0), relerr(x, ref[:, 1:4]))
+        worst["p"] = max(worst.get("p", 0.0), relerr(mom, ref[:, 4:7]))
+        for name in wl.YEE_NAMES:
+            worst[name] = max(worst.get(name, 0.0), relerr(sim.yee(name), d["s%d/%s" % (step, name)]))
+        if step <= 2:
+            for name in ("Jx", "Jy", "Jz"):
+                worst[name] = max(worst.get(name, 0.0), relerr(sim.yee(name), d["s%d/%s" % (step, name)]))
+    print("explicit step vs reference:", {k: "%.1e" % v for k, v in worst.items()})
+    for k, v in worst.items():
+        assert v <= 1e-12, (k, v)
+    sim.close()
+
+
+def test_explicit_step_deterministic_bitwise():
+    d = np.load(os.path.join(HERE, "golden", "explicit_ref.npz"))
+    w = explicit_workload()
+    print("case", w.name, "two runs of 3 steps", flush=True)
+    out = []
+    for _ in range(2):
+        sim = open_explicit(d, w)
+        for _ in range(3):
+            sim.explicit_step()
+        p = sim.particles()
+        out.append([sim.yee(n) for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz")] + [p["x"], p["p"]])
+        sim.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
