@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--shock", type=int, default=0, metavar="STEPS",
                     help="run the relativistic shock set-up (reflecting wall, flow injected at the right end) for STEPS "
                          "steps and report its throughput and density pile-up instead of the default benchmark")
+    ap.add_argument("--explicit", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS steps of the explicit (BUNEMAN / Tristan) FP64 step on the benchmark box "
+                         "(--cells, --ppc) instead of the default benchmark")
     ap.add_argument("--shock-cells", type=int, default=1024, help="cells along x (64 x 64 across)")
     ap.add_argument("--shock-ppc", type=int, default=16, help="particles per cell per species at injection")
     ap.add_argument("--pressure", default=None, choices=["reference", "cell"],
@@ -372,6 +375,32 @@ def weibel_check(args, torch, dev):
     return out
 
 
+def explicit_run(args, torch, dev):
+    """The reference's BUNEMAN step (Yee fields, Tristan mover, zig-zag current) in FP64 on the
+    benchmark's plasma: particle-steps/s and the per-kernel times of the timed steps."""
+    from picpp_amd import workloads as wl
+
+    w = wl.config2(n=(args.cells,) * 3, ppc=args.ppc, steps=args.explicit)
+    sim = open_box(w, torch, dev, precision="fp64", scheme="explicit")
+    sim.load_yee(wl.yee_initial_fields(w))
+    npart = sim.num_particles()
+    print("explicit: %s, %d particles" % (w.name, npart), file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        sim.explicit_step()
+    sim._chk(sim.lib.pic_profile_enable(sim.h, 1), "pic_profile_enable")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.explicit):
+        sim.explicit_step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kernels = read_profile(sim, args.explicit)
+    sim.close()
+    return {"cells": [args.cells] * 3, "particles": npart, "steps": args.explicit, "precision": "fp64",
+            "ms_per_step": 1e3 * el / args.explicit, "particle_steps_per_s": npart * args.explicit / el,
+            "kernels_ms_per_step": {nm: kv["ms_total"] / args.explicit for nm, kv in kernels.items()}}
+
+
 def shock_run(args, torch, dev):
     """The relativistic shock set-up: a cold e-/p+ flow (gamma0 10, mass ratio 64,
     kT/mc^2 = 1e-4, sigma 0.1) onto the reflecting wall, injected at the right end every step.
@@ -450,6 +479,9 @@ def main():
         return
     if args.shock > 0:
         print(json.dumps({"shock": shock_run(args, torch, dev)}))
+        return
+    if args.explicit > 0:
+        print(json.dumps({"explicit": explicit_run(args, torch, dev)}))
         return
     learned = args.learned != "off" and args.solver == "mixed"
     training = None

@@ -180,16 +180,88 @@ __global__ void k_max_cell_count(Grid g, int nspecies, const int* __restrict__ s
 	if (n > 0) atomicMax(out, n);
 }
 
-__device__ inline void add_fixed(long long* acc, long long q, double v, double scale) {
-	atomicAdd(reinterpret_cast<unsigned long long*>(acc + q), (unsigned long long)llrint(v * scale));
+// The workgroup's current box in LDS. A workgroup's 256 particles are consecutive in the
+// sorted order; when they start in one (i, j) row of cells k0 .. k0 + K - 1 (K <= kJBoxK), both
+// parts of every zig-zag path deposit within one cell of that row: Jx on cells i0-1 .. i0+1,
+// nodes j0-1 .. j0+2, k0-1 .. k0+K+1 (Jy, Jz alike with their own staggering). The
+// workgroup sums into LDS (64-bit integer atomics) and adds the box to the global sums once;
+// any other contribution goes to the global sums directly. All sums are integers, so the
+// result does not depend on the path.
+constexpr int kJBoxK = 20, kJBoxL = kJBoxK + 3;  // points along k per box row
+constexpr int kJBoxX = 3 * 4 * kJBoxL, kJBoxY = 4 * 3 * kJBoxL, kJBoxZ = 4 * 4 * kJBoxL;
+
+struct JBox {
+	bool use;
+	int o[3];    // ghosted index of local (0, 0, 0): (i0 - 1, j0 - 1, k0 - 1)
+	int kext;    // K + 3 (points along k in use)
+};
+
+__device__ inline void jbox_extent(int comp, int e[2]) {  // extents along x, y
+	e[0] = comp == 0 ? 3 : 4;
+	e[1] = comp == 1 ? 3 : 4;
 }
+
+__device__ inline void add_fixed(long long* acc, long long* box, const JBox& jb, int comp, const int* d, int a, int b, int c, double v,
+                                 double scale) {
+	const long long q = llrint(v * scale);
+	if (jb.use) {
+		int e[2];
+		jbox_extent(comp, e);
+		const int la = a - jb.o[0], lb = b - jb.o[1], lc = c - jb.o[2];
+		if ((unsigned)la < (unsigned)e[0] && (unsigned)lb < (unsigned)e[1] && (unsigned)lc < (unsigned)jb.kext) {
+			const int base = comp == 0 ? 0 : (comp == 1 ? kJBoxX : kJBoxX + kJBoxY);
+			atomicAdd(reinterpret_cast<unsigned long long*>(box + base + (la * e[1] + lb) * kJBoxL + lc), (unsigned long long)q);
+			return;
+		}
+	}
+	atomicAdd(reinterpret_cast<unsigned long long*>(acc + yi(d, a, b, c)), (unsigned long long)q);
+}
+
+__device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* __restrict__ species_start,
+                                 const YeeGrid& y, double dt, double scale, uint32_t* __restrict__ keys, int* __restrict__ status,
+                                 int i, long long* box, const JBox& jb);
 
 __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start,
                                                       YeeGrid y, double dt, double scale, uint32_t* __restrict__ keys,
                                                       int* __restrict__ status) {
-	const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= P.count) return;
-	const int i = (int)idx;
+	__shared__ long long box[kJBoxX + kJBoxY + kJBoxZ];
+	__shared__ int row_ends[2][3];
+	const long long b0 = (long long)blockIdx.x * blockDim.x;
+	const long long idx = b0 + threadIdx.x;
+	const int last = (int)min((long long)blockDim.x - 1, P.count - 1 - b0);
+	if ((int)threadIdx.x == 0 || (int)threadIdx.x == last) {
+		const int w = (int)threadIdx.x == 0 ? 0 : 1;
+		for (int a = 0; a < 3; ++a) row_ends[w][a] = (int)floor((P.x[a][idx] - g.x0[a]) / g.d[a]);
+	}
+	for (int q = threadIdx.x; q < kJBoxX + kJBoxY + kJBoxZ; q += blockDim.x) box[q] = 0;
+	__syncthreads();
+	JBox jb;
+	{
+		const int K = row_ends[1][2] - row_ends[0][2] + 1;
+		jb.use = row_ends[0][0] == row_ends[1][0] && row_ends[0][1] == row_ends[1][1] && K >= 1 && K <= kJBoxK;
+		for (int a = 0; a < 3; ++a) jb.o[a] = row_ends[0][a] - 1;
+		jb.kext = K + 3;
+	}
+	if (idx < P.count) tristan_particle(g, sp, P, species_start, y, dt, scale, keys, status, (int)idx, box, jb);
+	__syncthreads();
+	if (!jb.use) return;
+	for (int q = threadIdx.x; q < kJBoxX + kJBoxY + kJBoxZ; q += blockDim.x) {
+		const long long v = box[q];
+		if (v == 0) continue;
+		const int comp = q < kJBoxX ? 0 : (q < kJBoxX + kJBoxY ? 1 : 2);
+		const int r = q - (comp == 0 ? 0 : (comp == 1 ? kJBoxX : kJBoxX + kJBoxY));
+		int e[2];
+		jbox_extent(comp, e);
+		const int lc = r % kJBoxL, lab = r / kJBoxL, lb = lab % e[1], la = lab / e[1];
+		atomicAdd(reinterpret_cast<unsigned long long*>(y.Jacc[comp] + yi(y.dE[comp], jb.o[0] + la, jb.o[1] + lb, jb.o[2] + lc)),
+		          (unsigned long long)v);
+	}
+}
+
+// One particle of k_tristan_push.
+__device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* __restrict__ species_start,
+                                 const YeeGrid& y, double dt, double scale, uint32_t* __restrict__ keys, int* __restrict__ status,
+                                 int i, long long* box, const JBox& jb) {
 	int s = 0;
 	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
 	const double m = sp.mass[s], q = sp.charge[s];
@@ -319,18 +391,18 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 		long long* Jx = y.Jacc[0];
 		long long* Jy = y.Jacc[1];
 		long long* Jz = y.Jacc[2];
-		add_fixed(Jx, yi(y.dE[0], a, b, c), F[0] * om[1] * om[2], scale);
-		add_fixed(Jx, yi(y.dE[0], a, b + 1, c), F[0] * W[1] * om[2], scale);
-		add_fixed(Jx, yi(y.dE[0], a, b, c + 1), F[0] * om[1] * W[2], scale);
-		add_fixed(Jx, yi(y.dE[0], a, b + 1, c + 1), F[0] * W[1] * W[2], scale);
-		add_fixed(Jy, yi(y.dE[1], a, b, c), F[1] * om[0] * om[2], scale);
-		add_fixed(Jy, yi(y.dE[1], a + 1, b, c), F[1] * W[0] * om[2], scale);
-		add_fixed(Jy, yi(y.dE[1], a, b, c + 1), F[1] * om[0] * W[2], scale);
-		add_fixed(Jy, yi(y.dE[1], a + 1, b, c + 1), F[1] * W[0] * W[2], scale);
-		add_fixed(Jz, yi(y.dE[2], a, b, c), F[2] * om[0] * om[1], scale);
-		add_fixed(Jz, yi(y.dE[2], a, b + 1, c), F[2] * om[0] * W[1], scale);
-		add_fixed(Jz, yi(y.dE[2], a + 1, b, c), F[2] * W[0] * om[1], scale);
-		add_fixed(Jz, yi(y.dE[2], a + 1, b + 1, c), F[2] * W[0] * W[1], scale);
+		add_fixed(Jx, box, jb, 0, y.dE[0], a, b, c, F[0] * om[1] * om[2], scale);
+		add_fixed(Jx, box, jb, 0, y.dE[0], a, b + 1, c, F[0] * W[1] * om[2], scale);
+		add_fixed(Jx, box, jb, 0, y.dE[0], a, b, c + 1, F[0] * om[1] * W[2], scale);
+		add_fixed(Jx, box, jb, 0, y.dE[0], a, b + 1, c + 1, F[0] * W[1] * W[2], scale);
+		add_fixed(Jy, box, jb, 1, y.dE[1], a, b, c, F[1] * om[0] * om[2], scale);
+		add_fixed(Jy, box, jb, 1, y.dE[1], a + 1, b, c, F[1] * W[0] * om[2], scale);
+		add_fixed(Jy, box, jb, 1, y.dE[1], a, b, c + 1, F[1] * om[0] * W[2], scale);
+		add_fixed(Jy, box, jb, 1, y.dE[1], a + 1, b, c + 1, F[1] * W[0] * W[2], scale);
+		add_fixed(Jz, box, jb, 2, y.dE[2], a, b, c, F[2] * om[0] * om[1], scale);
+		add_fixed(Jz, box, jb, 2, y.dE[2], a, b + 1, c, F[2] * om[0] * W[1], scale);
+		add_fixed(Jz, box, jb, 2, y.dE[2], a + 1, b, c, F[2] * W[0] * om[1], scale);
+		add_fixed(Jz, box, jb, 2, y.dE[2], a + 1, b + 1, c, F[2] * W[0] * W[1], scale);
 	}
 	// one-rank periodic wrap (mover.cpp:770-806 classification, exchangeParticles' shift)
 	for (int a = 0; a < 3; ++a) {
