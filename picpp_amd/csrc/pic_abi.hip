@@ -305,7 +305,7 @@ InjectPhysics inject_physics(const pic_ctx* c) {
 	return ph;
 }
 
-void rebin_particles32(pic_ctx* c, int removed = 0, bool resized = false);
+void rebin_particles(pic_ctx* c, int removed = 0, bool resized = false);
 
 // After a push with the wall boundary (PIC++/simulation.cpp:258-289 order: erase and exchange,
 // injection, preserve-charge replacement): particles that left through the right end are
@@ -357,18 +357,19 @@ void wall_particles(pic_ctx* c) {
 		c->inj_step++;
 	}
 	if (c->fp32)
-		rebin_particles32(c, removed, resized);
+		rebin_particles(c, removed, resized);
 	else
 		sort_particles(c, removed, resized);
 }
 
-// Restores the cell-sorted order of the FP32 particles after a push without a full sort
-// (pic_kernels_fp32.hip, rebin kernels); more than a quarter of the particles changing
-// segment takes the full sort instead. Same result as a stable sort by key within each new
-// segment's stayers and arrivals. With the wall boundary the `removed` particles (key
-// 0xFFFFFFFF, flagged as movers by the push) are dropped, and particles appended after the
-// push (injection, flagged as movers) arrive like any other mover.
-void rebin_particles32(pic_ctx* c, int removed, bool resized) {
+// Restores the cell-sorted order after a push that left one mover bit per particle in
+// c->mask (the FP32 push, the explicit push), without a full sort (pic_kernels_fp32.hip, rebin
+// kernels); more than a quarter of the particles changing segment takes the full sort
+// instead. Same result as a stable sort by key within each new segment's stayers and
+// arrivals. With the wall boundary the `removed` particles (key 0xFFFFFFFF, flagged as movers
+// by the push) are dropped, and particles appended after the push (injection, flagged as
+// movers) arrive like any other mover.
+void rebin_particles(pic_ctx* c, int removed, bool resized) {
 	const long long n = c->np();
 	hipStream_t s = c->stream;
 	int M = 0;
@@ -409,12 +410,16 @@ void rebin_particles32(pic_ctx* c, int removed, bool resized) {
 		                  c->first, c->perm_sorted, s);
 	}
 	const long long n_new = n - removed;
-	c->Q.count = c->Qalt.count = n_new;  // the permutation fills the first n_new places
-	{
+	set_count(c, n_new);  // the permutation fills the first n_new places
+	if (c->fp32) {
 		Timed t(c, "permute", (double)n_new * (2 * 32 + 4));
 		launch_permute32(c->Q, c->Qalt, c->perm_sorted, s);
+		std::swap(c->Q, c->Qalt);
+	} else {
+		Timed t(c, "permute", (double)n_new * (2 * 52 + 4));
+		launch_permute(c->P, c->Palt, c->perm_sorted, s);
+		std::swap(c->P, c->Palt);
 	}
-	std::swap(c->Q, c->Qalt);
 	std::swap(c->start, c->start_new);
 	if (removed > 0 || resized) species_ranges(c, n_new);
 }
@@ -769,6 +774,8 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		}
 		if (cfg->scheme == PIC_SCHEME_EXPLICIT) {
 			c->explicit_scheme = true;
+			if ((long long)(g.N[0] + 1) * (g.N[1] + 1) * (g.N[2] + 1) > 0x7FFFFFFFLL)
+				throw DeviceError{PIC_ERR_ARGUMENT, "explicit scheme: the Yee arrays need fewer than 2^31 points"};
 			for (int comp = 0; comp < 3; ++comp) {
 				yee_dims(g, false, comp, c->yee.dE[comp]);
 				yee_dims(g, true, comp, c->yee.dB[comp]);
@@ -850,7 +857,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c->keys, c->keys_sorted, c->perm, c->perm_sorted,
 		                                             (int)std::max<long long>(c->capacity, 1), 0, c->end_bit, c->stream));
 		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->counts, c->start, (int)(c->nkeys + 1), c->stream));
-		if (c->fp32) {
+		if (c->fp32 || c->explicit_scheme) {  // the rebin after a push with mover bits
 			const long long words = c->capacity / 64 + 2;
 			c->mask = dalloc<uint64_t>(words);
 			c->wcount = dalloc<int>(words);
@@ -1548,7 +1555,7 @@ int pic_push(pic_ctx* c) {
 		if (c->g.wall)  // removals, replacements and injection, then the full sort
 			wall_particles(c);
 		else if (c->fp32)
-			rebin_particles32(c);
+			rebin_particles(c);
 		else
 			sort_particles(c);
 		density(c, true);
@@ -1647,12 +1654,16 @@ int pic_explicit_step(pic_ctx* c) {
 		double fcd = 0;
 		for (int q = 0; q < c->sp.count; ++q) fcd = std::max(fcd, std::fabs(c->sp.charge[q] * c->sp.weight[q] / g.volume));
 		const double bound = 512.0 * std::max(maxcount, 1) * std::max(fcd, 1e-300);
-		const double scale = std::ldexp(1.0, (int)std::floor(62.0 - std::log2(bound)));
+		// and a single contribution (below 2 |q w / V| as |v| < 1) stays below 2^50, for the
+		// push's rounding to integers
+		const int e_sum = (int)std::floor(62.0 - std::log2(bound));
+		const int e_one = (int)std::floor(49.0 - std::log2(std::max(fcd, 1e-300)));
+		const double scale = std::ldexp(1.0, std::min(e_sum, e_one));
 		for (int comp = 0; comp < 3; ++comp)
 			HIP_CHECK(hipMemsetAsync(c->yee.Jacc[comp], 0, yee_count(c->yee.dE[comp]) * sizeof(long long), s));
 		{
 			Timed t(c, "tristan_push", (double)c->P.count * 100 + (double)g.ncells * 6 * 8 * 8);
-			launch_tristan_push(g, c->sp, c->P, c->species_start, c->yee, dt, scale, c->keys, c->status, s);
+			launch_tristan_push(g, c->sp, c->P, c->species_start, c->yee, dt, scale, c->keys, c->mask, c->status, s);
 		}
 		check_status(c);
 		{
@@ -1669,7 +1680,7 @@ int pic_explicit_step(pic_ctx* c) {
 			launch_tristan_e(g, c->yee, dt, s);
 			launch_yee_exchange(g, c->yee, false, s);
 		}
-		sort_particles(c);
+		rebin_particles(c);
 	} catch (const DeviceError& e) {
 		return fail(c, e);
 	}

@@ -100,6 +100,15 @@ __device__ inline AxisMap axis_map(double x, double x0, double d) {
 	return m;
 }
 
+// The node and cell bases of axis_map alone, with the same arithmetic (so the same key).
+__device__ inline void axis_bases(double x, double x0, double d, int& nbase, int& cbase) {
+	const int cc = (int)floor((x - x0) / d);
+	const double mid = ((x0 + cc * d) + (x0 + (cc + 1) * d)) / 2;
+	cbase = cc + ((x < mid) ? -1 : 0);
+	const int nc = (int)floor(((x - x0) / d) + 0.5);
+	nbase = nc + ((x < x0 + nc * d) ? -1 : 0);
+}
+
 // Sort key of a particle: (species, node-map cell, octant). The octant bit per axis is
 // 1 when the cell-map base equals the node-map base (upper half of the cell), so the
 // particles of one key share both stencils.
@@ -109,6 +118,13 @@ __device__ inline uint32_t particle_key(const Grid& g, int sp, const AxisMap* m)
 	int bz = m[2].cbase - m[2].nbase + 1;
 	long long cid = cell_id(g, m[0].nbase, m[1].nbase, m[2].nbase);
 	return (uint32_t)((((long long)sp * g.ncells + cid) << 3) | (bx << 2) | (by << 1) | bz);
+}
+
+// particle_key of the position x (axis_bases per axis)
+__device__ inline uint32_t position_key(const Grid& g, int sp, const double x[3]) {
+	AxisMap m[3];
+	for (int a = 0; a < 3; ++a) axis_bases(x[a], g.x0[a], g.d[a], m[a].nbase, m[a].cbase);
+	return particle_key(g, sp, m);
 }
 
 // correlationGeneralEfield / correlationBfield (PIC++/correlation.cpp:23-58, :103-164):

@@ -201,29 +201,136 @@ __device__ inline void jbox_extent(int comp, int e[2]) {  // extents along x, y
 	e[1] = comp == 1 ? 3 : 4;
 }
 
-__device__ inline void add_fixed(long long* acc, long long* box, const JBox& jb, int comp, const int* d, int a, int b, int c, double v,
-                                 double scale) {
-	const long long q = llrint(v * scale);
-	if (jb.use) {
-		int e[2];
-		jbox_extent(comp, e);
-		const int la = a - jb.o[0], lb = b - jb.o[1], lc = c - jb.o[2];
-		if ((unsigned)la < (unsigned)e[0] && (unsigned)lb < (unsigned)e[1] && (unsigned)lc < (unsigned)jb.kext) {
-			const int base = comp == 0 ? 0 : (comp == 1 ? kJBoxX : kJBoxX + kJBoxY);
-			atomicAdd(reinterpret_cast<unsigned long long*>(box + base + (la * e[1] + lb) * kJBoxL + lc), (unsigned long long)q);
-			return;
-		}
+// Per-launch constants of the Tristan push, formed once on the host: reciprocals of the cell
+// sizes and the species' factors.
+struct TristanConsts {
+	double inv_d[3], inv_2d[3];
+	double hs[kMaxSpecies];      // q dt / 2
+	double beta[kMaxSpecies];    // q dt / (2 m)
+	double inv_m2[kMaxSpecies];  // 1 / m^2
+	double fcd[kMaxSpecies];     // q w / V
+	double fcd_dt[kMaxSpecies];  // q w / (V dt)
+	double scale;                // fixed-point scale of the current, a power of two
+};
+
+// v * scale rounded to the nearest integer, ties to even (llrint). The scale is a power of two,
+// so the product is exact, and it keeps every contribution below 2^50 in magnitude: adding
+// 1.5 * 2^52 leaves the rounded integer in the low bits of the sum.
+__device__ inline long long to_fixed(double v, double scale) {
+	return __double_as_longlong(v * scale + 6755399441055744.0) - 0x4338000000000000LL;
+}
+
+// The 12 fixed-point values of one part of a zig-zag path whose stencil starts at cell
+// (a, b, c): Jx at (a, b..b+1, c..c+1), Jy at (a..a+1, b, c..c+1), Jz at (a..a+1, b..b+1, c)
+// with the bilinear weights of the part's midpoint (addParticleFluxZigzag,
+// momentum_collecting.cpp:176-245).
+__device__ inline void quantize_part(const double F[3], const double W[3], double scale, long long q[12]) {
+	const double om[3] = {1.0 - W[0], 1.0 - W[1], 1.0 - W[2]};
+	q[0] = to_fixed(F[0] * om[1] * om[2], scale);
+	q[1] = to_fixed(F[0] * W[1] * om[2], scale);
+	q[2] = to_fixed(F[0] * om[1] * W[2], scale);
+	q[3] = to_fixed(F[0] * W[1] * W[2], scale);
+	q[4] = to_fixed(F[1] * om[0] * om[2], scale);
+	q[5] = to_fixed(F[1] * W[0] * om[2], scale);
+	q[6] = to_fixed(F[1] * om[0] * W[2], scale);
+	q[7] = to_fixed(F[1] * W[0] * W[2], scale);
+	q[8] = to_fixed(F[2] * om[0] * om[1], scale);
+	q[9] = to_fixed(F[2] * om[0] * W[1], scale);
+	q[10] = to_fixed(F[2] * W[0] * om[1], scale);
+	q[11] = to_fixed(F[2] * W[0] * W[1], scale);
+}
+
+__device__ inline void add64(long long* p, long long q) { atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)q); }
+
+// Adds one part's 12 values: into the workgroup's box when the whole 2 x 2 stencil of every
+// component lies in it, else into the global sums.
+__device__ inline void add_part(const YeeGrid& y, long long* box, const JBox& jb, int a, int b, int c, const long long q[12]) {
+	const int la = a - jb.o[0], lb = b - jb.o[1], lc = c - jb.o[2];
+	if (jb.use && (unsigned)la <= 2u && (unsigned)lb <= 2u && lc >= 0 && lc + 1 < jb.kext) {
+		constexpr int L = kJBoxL;
+		long long* bx = box + (la * 4 + lb) * L + lc;                  // Jx: 3 x 4 x L
+		long long* by = box + kJBoxX + (la * 3 + lb) * L + lc;         // Jy: 4 x 3 x L
+		long long* bz = box + kJBoxX + kJBoxY + (la * 4 + lb) * L + lc;  // Jz: 4 x 4 x L
+		add64(bx, q[0]);
+		add64(bx + L, q[1]);
+		add64(bx + 1, q[2]);
+		add64(bx + L + 1, q[3]);
+		add64(by, q[4]);
+		add64(by + 3 * L, q[5]);
+		add64(by + 1, q[6]);
+		add64(by + 3 * L + 1, q[7]);
+		add64(bz, q[8]);
+		add64(bz + L, q[9]);
+		add64(bz + 4 * L, q[10]);
+		add64(bz + 5 * L, q[11]);
+		return;
 	}
-	atomicAdd(reinterpret_cast<unsigned long long*>(acc + yi(d, a, b, c)), (unsigned long long)q);
+	long long* Jx = y.Jacc[0];
+	long long* Jy = y.Jacc[1];
+	long long* Jz = y.Jacc[2];
+	add64(Jx + yi(y.dE[0], a, b, c), q[0]);
+	add64(Jx + yi(y.dE[0], a, b + 1, c), q[1]);
+	add64(Jx + yi(y.dE[0], a, b, c + 1), q[2]);
+	add64(Jx + yi(y.dE[0], a, b + 1, c + 1), q[3]);
+	add64(Jy + yi(y.dE[1], a, b, c), q[4]);
+	add64(Jy + yi(y.dE[1], a + 1, b, c), q[5]);
+	add64(Jy + yi(y.dE[1], a, b, c + 1), q[6]);
+	add64(Jy + yi(y.dE[1], a + 1, b, c + 1), q[7]);
+	add64(Jz + yi(y.dE[2], a, b, c), q[8]);
+	add64(Jz + yi(y.dE[2], a, b + 1, c), q[9]);
+	add64(Jz + yi(y.dE[2], a + 1, b, c), q[10]);
+	add64(Jz + yi(y.dE[2], a + 1, b + 1, c), q[11]);
+}
+
+// What a particle leaves for the wave's segmented sum: the first part of its path (with the
+// second part added when both parts share the cell), or cell = -1 for nothing.
+struct PartSum {
+	int cell;  // linear ghosted index of the stencil's cell
+	int a, b, c;
+	long long q[12];
+	bool moved;  // the particle's sort key changed
+};
+
+// A 64-bit DPP move: lanes whose source is outside the pattern or whose row is masked off
+// read 0.
+template <int CTRL, int ROWS>
+__device__ inline long long dpp64(long long v) {
+	const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(unsigned long long)v, CTRL, ROWS, 0xF, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)((unsigned long long)v >> 32), CTRL, ROWS, 0xF, false);
+	return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+template <int CTRL, int ROWS>
+__device__ inline void seg_step(long long q[12], bool take) {
+#pragma unroll
+	for (int v = 0; v < 12; ++v) {
+		const long long t = dpp64<CTRL, ROWS>(q[v]);
+		q[v] += take ? t : 0;
+	}
+}
+
+// Inclusive sums over runs of lanes with the same cell (the particles are sorted by cell, so
+// a wave holds a few runs): after it the last lane of every run holds the run's total.
+// Steps of 1, 2, 4, 8 lanes inside rows of 16 (row_shr), then the last lane of row 0 and 2
+// into rows 1 and 3 (row_bcast:15) and the last lane of row 1 into rows 2 and 3
+// (row_bcast:31), each taken only when the source lane lies in the same run. Integer sums, so
+// the result is the same as any other order. Needs all 64 lanes of the wave active.
+__device__ inline void wave_run_sums(long long q[12], int lane, int dist) {
+	seg_step<0x111, 0xF>(q, dist >= 1);
+	seg_step<0x112, 0xF>(q, dist >= 2);
+	seg_step<0x114, 0xF>(q, dist >= 4);
+	seg_step<0x118, 0xF>(q, dist >= 8);
+	seg_step<0x142, 0xA>(q, dist > (lane & 15));
+	seg_step<0x143, 0xC>(q, dist > lane - 32);
 }
 
 __device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* __restrict__ species_start,
-                                 const YeeGrid& y, double dt, double scale, uint32_t* __restrict__ keys, int* __restrict__ status,
-                                 int i, long long* box, const JBox& jb);
+                                 const YeeGrid& y, double dt, const TristanConsts& K, uint32_t* __restrict__ keys,
+                                 int* __restrict__ status, long long i, long long* box, const JBox& jb, PartSum& ps);
 
 __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start,
-                                                      YeeGrid y, double dt, double scale, uint32_t* __restrict__ keys,
-                                                      int* __restrict__ status) {
+                                                      YeeGrid y, double dt, TristanConsts K, uint32_t* __restrict__ keys,
+                                                      uint64_t* __restrict__ mask, int* __restrict__ status) {
 	__shared__ long long box[kJBoxX + kJBoxY + kJBoxZ];
 	__shared__ int row_ends[2][3];
 	const long long b0 = (long long)blockIdx.x * blockDim.x;
@@ -242,7 +349,24 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 		for (int a = 0; a < 3; ++a) jb.o[a] = row_ends[0][a] - 1;
 		jb.kext = K + 3;
 	}
-	if (idx < P.count) tristan_particle(g, sp, P, species_start, y, dt, scale, keys, status, (int)idx, box, jb);
+	PartSum ps;
+	ps.cell = -1;
+	ps.moved = false;
+	tristan_particle(g, sp, P, species_start, y, dt, K, keys, status, idx, box, jb, ps);
+	{
+		const int lane = (int)(threadIdx.x & 63);
+		// one mover bit per particle: bit i & 63 of word i >> 6 (waves are 64 consecutive
+		// particles; lanes past the end contribute zeros)
+		const unsigned long long bits = __ballot(ps.moved);
+		if (lane == 0 && idx < P.count) mask[idx >> 6] = bits;
+		const int prev = __shfl_up(ps.cell, 1);
+		const unsigned long long heads = __ballot(lane == 0 || prev != ps.cell);
+		const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1;
+		const int head = 63 - __clzll(heads & upto);
+		wave_run_sums(ps.q, lane, lane - head);
+		const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1);
+		if (tail && ps.cell >= 0) add_part(y, box, jb, ps.a, ps.b, ps.c, ps.q);
+	}
 	__syncthreads();
 	if (!jb.use) return;
 	for (int q = threadIdx.x; q < kJBoxX + kJBoxY + kJBoxZ; q += blockDim.x) {
@@ -258,89 +382,104 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 	}
 }
 
-// One particle of k_tristan_push.
-__device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* __restrict__ species_start,
-                                 const YeeGrid& y, double dt, double scale, uint32_t* __restrict__ keys, int* __restrict__ status,
-                                 int i, long long* box, const JBox& jb) {
+// One particle of k_tristan_push (none when i >= P.count): the second part of a path that
+// changes cell goes to the sums here; the first part, with the second when both share the
+// cell, is left in ps for the wave's run sums. Divisions by the cell sizes, by dt and by m^2 are
+// products with the host's reciprocals.
+__device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P,
+                                                 const int* __restrict__ species_start, const YeeGrid& y, double dt,
+                                                 const TristanConsts& K, uint32_t* __restrict__ keys, int* __restrict__ status,
+                                                 long long i, long long* box, const JBox& jb, PartSum& ps) {
+	for (int v = 0; v < 12; ++v) ps.q[v] = 0;
+	if (i >= P.count) return;
 	int s = 0;
 	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
-	const double m = sp.mass[s], q = sp.charge[s];
+	const double m = sp.mass[s];
 	double x[3], p[3];
 	for (int a = 0; a < 3; ++a) {
 		x[a] = P.x[a][i];
 		p[a] = P.p[a][i];
 	}
-	// correlationBunemanEBfieldsWithoutMaps: cell and node maps per axis
-	int lc[3], rc[3], ln[3], rn[3];
-	double wlc[3], wrc[3], wln[3], wrn[3];
+	const uint32_t key0 = position_key(g, s, x);
+	// correlationBunemanEBfieldsWithoutMaps: the lower cell and node of the stencil per axis,
+	// with the weights of the lower and the upper point
+	int lc[3], ln[3];
+	double wc[3][2], wn[3][2];
 	bool ok = true;
 #pragma unroll
 	for (int a = 0; a < 3; ++a) {
-		const double g0 = g.x0[a], d = g.d[a];
-		int cnt = (int)floor((x[a] - g0) / d);
+		const double g0 = g.x0[a], d = g.d[a], id = K.inv_d[a];
+		int cnt = (int)floor((x[a] - g0) * id);
 		const double midc = ((g0 + cnt * d) + (g0 + (cnt + 1) * d)) / 2;
 		if (x[a] < midc) {
 			lc[a] = cnt - 1;
-			rc[a] = cnt;
-			wrc[a] = 0.5 + (x[a] - (g0 + cnt * d)) / d;
-			wlc[a] = 1.0 - wrc[a];
+			wc[a][1] = 0.5 + (x[a] - (g0 + cnt * d)) * id;
+			wc[a][0] = 1.0 - wc[a][1];
 		} else {
 			lc[a] = cnt;
-			rc[a] = cnt + 1;
-			wlc[a] = 0.5 + ((g0 + (cnt + 1) * d) - x[a]) / d;
-			wrc[a] = 1.0 - wlc[a];
+			wc[a][0] = 0.5 + ((g0 + (cnt + 1) * d) - x[a]) * id;
+			wc[a][1] = 1.0 - wc[a][0];
 		}
-		cnt = (int)floor(((x[a] - g0) / d) + 0.5);
+		cnt = (int)floor(((x[a] - g0) * id) + 0.5);
 		if (x[a] < g0 + cnt * d) {
 			ln[a] = cnt - 1;
-			rn[a] = cnt;
-			wrn[a] = 0.5 + (x[a] - ((g0 + (cnt - 1) * d) + (g0 + cnt * d)) / 2) / d;
-			wln[a] = 1.0 - wrn[a];
+			wn[a][1] = 0.5 + (x[a] - ((g0 + (cnt - 1) * d) + (g0 + cnt * d)) / 2) * id;
+			wn[a][0] = 1.0 - wn[a][1];
 		} else {
 			ln[a] = cnt;
-			rn[a] = cnt + 1;
-			wln[a] = 0.5 + (((g0 + cnt * d) + (g0 + (cnt + 1) * d)) / 2 - x[a]) / d;
-			wrn[a] = 1.0 - wln[a];
+			wn[a][0] = 0.5 + (((g0 + cnt * d) + (g0 + (cnt + 1) * d)) / 2 - x[a]) * id;
+			wn[a][1] = 1.0 - wn[a][0];
 		}
-		// stencils inside the arrays: cells 0 .. N-1 need lc >= 0, rc <= N-1; nodes ln >= 0, rn <= N
-		if (lc[a] < 0 || rc[a] > g.N[a] - 1 || ln[a] < 0 || rn[a] > g.N[a]) ok = false;
+		// stencils inside the arrays: cells 0 .. N-1, nodes 0 .. N
+		if (lc[a] < 0 || lc[a] + 1 > g.N[a] - 1 || ln[a] < 0 || ln[a] + 1 > g.N[a]) ok = false;
 	}
 	if (!ok) {  // no access outside the arrays: report and leave the particle
 		atomicOr(status, 8);
 		return;
 	}
-	double Ef[3] = {0, 0, 0}, Bf[3] = {0, 0, 0};
-	for (int sx = 0; sx < 2; ++sx)
-		for (int sy = 0; sy < 2; ++sy)
-			for (int sz = 0; sz < 2; ++sz) {
-				const int sv[3] = {sx, sy, sz};
+	// E component c sits on cells along c and nodes along the others, B the opposite: each
+	// takes a 2 x 2 x 2 block from its lower corner (indices below 2^31, checked on the host)
+	double Ef[3], Bf[3];
 #pragma unroll
-				for (int c = 0; c < 3; ++c) {
-					int ie[3], ib[3];
-					double we[3], wb[3];
+	for (int c = 0; c < 3; ++c) {
+		int e0[3], b0[3];
+		double we[3][2], wb[3][2];
 #pragma unroll
-					for (int a = 0; a < 3; ++a) {
-						const int ci = sv[a] ? rc[a] : lc[a], ni = sv[a] ? rn[a] : ln[a];
-						const double cw = sv[a] ? wrc[a] : wlc[a], nw = sv[a] ? wrn[a] : wln[a];
-						ie[a] = a == c ? ci : ni;
-						we[a] = a == c ? cw : nw;
-						ib[a] = a == c ? ni : ci;
-						wb[a] = a == c ? nw : cw;
-					}
-					Ef[c] += y.E[c][yi(y.dE[c], ie[0], ie[1], ie[2])] * we[0] * we[1] * we[2];
-					Bf[c] += y.B[c][yi(y.dB[c], ib[0], ib[1], ib[2])] * wb[0] * wb[1] * wb[2];
-				}
+		for (int a = 0; a < 3; ++a) {
+			e0[a] = a == c ? lc[a] : ln[a];
+			b0[a] = a == c ? ln[a] : lc[a];
+			for (int t = 0; t < 2; ++t) {
+				we[a][t] = a == c ? wc[a][t] : wn[a][t];
+				wb[a][t] = a == c ? wn[a][t] : wc[a][t];
 			}
+		}
+		const int* dE = y.dE[c];
+		const int* dB = y.dB[c];
+		const double* pe = y.E[c] + ((e0[0] * dE[1] + e0[1]) * dE[2] + e0[2]);
+		const double* pb = y.B[c] + ((b0[0] * dB[1] + b0[1]) * dB[2] + b0[2]);
+		const int se = dE[1] * dE[2], sb = dB[1] * dB[2];
+		double e = 0, b = 0;
+#pragma unroll
+		for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+			for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+				for (int sz = 0; sz < 2; ++sz) {
+					e += pe[sx * se + sy * dE[2] + sz] * we[0][sx] * we[1][sy] * we[2][sz];
+					b += pb[sx * sb + sy * dB[2] + sz] * wb[0][sx] * wb[1][sy] * wb[2][sz];
+				}
+		Ef[c] = e;
+		Bf[c] = b;
+	}
 	// moveParticleTristan: half kick, rotation, half kick
-	const double hs = q * dt * 0.5;
+	const double hs = K.hs[s];
 	double dp[3];
 	for (int a = 0; a < 3; ++a) {
 		dp[a] = Ef[a] * hs;
 		p[a] = p[a] + dp[a];
 	}
-	const double gamma = sqrt((p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / (m * m) + 1.0);
-	const double beta = q * dt / (2.0 * m);
-	const double bs = beta / gamma;
+	const double gamma = sqrt((p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * K.inv_m2[s] + 1.0);
+	const double bs = K.beta[s] / gamma;
 	const double f = 2.0 / sqrt(1.0 + bs * bs * (Bf[0] * Bf[0] + Bf[1] * Bf[1] + Bf[2] * Bf[2]));
 	double c1[3] = {p[1] * Bf[2] - p[2] * Bf[1], -p[0] * Bf[2] + p[2] * Bf[0], p[0] * Bf[1] - p[1] * Bf[0]};
 	double tm[3];
@@ -348,21 +487,21 @@ __device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const Pa
 	double c2[3] = {tm[1] * Bf[2] - tm[2] * Bf[1], -tm[0] * Bf[2] + tm[2] * Bf[0], tm[0] * Bf[1] - tm[1] * Bf[0]};
 	const double fb = f * bs;
 	for (int a = 0; a < 3; ++a) p[a] += c2[a] * fb + dp[a];
-	const double gn = sqrt((p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / (m * m) + 1.0);
+	const double gn = sqrt((p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * K.inv_m2[s] + 1.0);
+	const double iv = 1.0 / (m * gn);
 	double v[3];
 	for (int a = 0; a < 3; ++a) {
-		v[a] = p[a] / (m * gn);
+		v[a] = p[a] * iv;
 		x[a] += v[a] * dt;
 	}
 	// addParticleFluxZigzag at the unwrapped position
-	const double fcd = q * sp.weight[s] / g.volume;
 	double loc[3], prev[3], r[3];
 	int cur[3], pre[3];
 	for (int a = 0; a < 3; ++a) {
 		loc[a] = x[a] - g.x0[a];
-		cur[a] = (int)floor(loc[a] / g.d[a]);
+		cur[a] = (int)floor(loc[a] * K.inv_d[a]);
 		prev[a] = loc[a] - v[a] * dt;
-		pre[a] = (int)floor(prev[a] / g.d[a]);
+		pre[a] = (int)floor(prev[a] * K.inv_d[a]);
 		r[a] = (loc[a] + prev[a]) / 2;
 		if (cur[a] != pre[a]) r[a] = fmax(g.x0[a] + pre[a] * g.d[a], g.x0[a] + cur[a] * g.d[a]) - g.x0[a];
 		// J stencils of both parts inside J's arrays (cells 0 .. N-2 along the component's
@@ -373,37 +512,30 @@ __device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const Pa
 		atomicOr(status, 8);
 		return;
 	}
+	const double fcd = K.fcd[s], fdt = K.fcd_dt[s];
 	double F1[3], F2[3], W1[3], W2[3];
-	for (int a = 0; a < 3; ++a) F1[a] = fcd * (r[a] - prev[a]) / dt;
-	F2[0] = fcd * (loc[0] - r[0]) / dt;
+	for (int a = 0; a < 3; ++a) F1[a] = fdt * (r[a] - prev[a]);
+	F2[0] = fdt * (loc[0] - r[0]);
 	F2[1] = fcd * v[1] - F1[1];
 	F2[2] = fcd * v[2] - F1[2];
 	for (int a = 0; a < 3; ++a) {
-		W1[a] = (prev[a] + r[a]) / (2 * g.d[a]) - pre[a];
-		W2[a] = (loc[a] + r[a]) / (2 * g.d[a]) - cur[a];
+		W1[a] = (prev[a] + r[a]) * K.inv_2d[a] - pre[a];
+		W2[a] = (loc[a] + r[a]) * K.inv_2d[a] - cur[a];
 	}
-	for (int part = 0; part < 2; ++part) {
-		const int* cc = part ? cur : pre;
-		const double* F = part ? F2 : F1;
-		const double* W = part ? W2 : W1;
-		const double om[3] = {1.0 - W[0], 1.0 - W[1], 1.0 - W[2]};
-		const int a = cc[0], b = cc[1], c = cc[2];
-		long long* Jx = y.Jacc[0];
-		long long* Jy = y.Jacc[1];
-		long long* Jz = y.Jacc[2];
-		add_fixed(Jx, box, jb, 0, y.dE[0], a, b, c, F[0] * om[1] * om[2], scale);
-		add_fixed(Jx, box, jb, 0, y.dE[0], a, b + 1, c, F[0] * W[1] * om[2], scale);
-		add_fixed(Jx, box, jb, 0, y.dE[0], a, b, c + 1, F[0] * om[1] * W[2], scale);
-		add_fixed(Jx, box, jb, 0, y.dE[0], a, b + 1, c + 1, F[0] * W[1] * W[2], scale);
-		add_fixed(Jy, box, jb, 1, y.dE[1], a, b, c, F[1] * om[0] * om[2], scale);
-		add_fixed(Jy, box, jb, 1, y.dE[1], a + 1, b, c, F[1] * W[0] * om[2], scale);
-		add_fixed(Jy, box, jb, 1, y.dE[1], a, b, c + 1, F[1] * om[0] * W[2], scale);
-		add_fixed(Jy, box, jb, 1, y.dE[1], a + 1, b, c + 1, F[1] * W[0] * W[2], scale);
-		add_fixed(Jz, box, jb, 2, y.dE[2], a, b, c, F[2] * om[0] * om[1], scale);
-		add_fixed(Jz, box, jb, 2, y.dE[2], a, b + 1, c, F[2] * om[0] * W[1], scale);
-		add_fixed(Jz, box, jb, 2, y.dE[2], a + 1, b, c, F[2] * W[0] * om[1], scale);
-		add_fixed(Jz, box, jb, 2, y.dE[2], a + 1, b + 1, c, F[2] * W[0] * W[1], scale);
+	quantize_part(F1, W1, K.scale, ps.q);
+	{
+		long long q2[12];
+		quantize_part(F2, W2, K.scale, q2);
+		if (cur[0] == pre[0] && cur[1] == pre[1] && cur[2] == pre[2]) {
+			for (int v = 0; v < 12; ++v) ps.q[v] += q2[v];
+		} else {
+			add_part(y, box, jb, cur[0], cur[1], cur[2], q2);
+		}
 	}
+	ps.a = pre[0];
+	ps.b = pre[1];
+	ps.c = pre[2];
+	ps.cell = (int)(((long long)pre[0] * g.N[1] + pre[1]) * g.N[2] + pre[2]);
 	// one-rank periodic wrap (mover.cpp:770-806 classification, exchangeParticles' shift)
 	for (int a = 0; a < 3; ++a) {
 		const double lo = g.x0[a] + 2 * g.d[a], hi = g.x0[a] + (g.N[a] - 2) * g.d[a];
@@ -417,9 +549,9 @@ __device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const Pa
 		P.x[a][i] = x[a];
 		P.p[a][i] = p[a];
 	}
-	AxisMap mp[3];
-	for (int a = 0; a < 3; ++a) mp[a] = axis_map(x[a], g.x0[a], g.d[a]);
-	keys[i] = particle_key(g, s, mp);
+	const uint32_t key = position_key(g, s, x);
+	keys[i] = key;
+	ps.moved = key != key0;
 }
 
 }  // namespace
@@ -476,9 +608,24 @@ void launch_max_cell_count(const Grid& g, int nspecies, const int* start, int* o
 }
 
 void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
-                         const YeeGrid& y, double dt, double scale, uint32_t* keys, int* status, hipStream_t s) {
+                         const YeeGrid& y, double dt, double scale, uint32_t* keys, uint64_t* mask, int* status,
+                         hipStream_t s) {
 	if (P.count == 0) return;
-	hipLaunchKernelGGL(k_tristan_push, dim3(nb(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, y, dt, scale, keys,
+	TristanConsts K{};
+	for (int a = 0; a < 3; ++a) {
+		K.inv_d[a] = 1.0 / g.d[a];
+		K.inv_2d[a] = 1.0 / (2 * g.d[a]);
+	}
+	for (int q = 0; q < sp.count; ++q) {
+		const double m = sp.mass[q], c = sp.charge[q];
+		K.hs[q] = c * dt * 0.5;
+		K.beta[q] = c * dt / (2.0 * m);
+		K.inv_m2[q] = 1.0 / (m * m);
+		K.fcd[q] = c * sp.weight[q] / g.volume;
+		K.fcd_dt[q] = K.fcd[q] / dt;
+	}
+	K.scale = scale;
+	hipLaunchKernelGGL(k_tristan_push, dim3(nb(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, y, dt, K, keys, mask,
 	                   status);
 }
 

@@ -33,8 +33,10 @@ void launch_yee_current(const Grid& g, const YeeGrid& y, double inv_scale, hipSt
 // largest number of particles of all species in one cell (start: segment starts per key)
 void launch_max_cell_count(const Grid& g, int nspecies, const int* start, int* out, hipStream_t s);
 // moveParticleTristan for every particle, the zig-zag current of the move into Jacc (fixed
-// point, scale a power of two), the one-rank periodic wrap and the new sort key
+// point, scale a power of two), the one-rank periodic wrap, the new sort key and one mover bit
+// per particle in mask (bit i & 63 of word i >> 6: the key changed)
 void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
-                         const YeeGrid& y, double dt, double scale, uint32_t* keys, int* status, hipStream_t s);
+                         const YeeGrid& y, double dt, double scale, uint32_t* keys, uint64_t* mask, int* status,
+                         hipStream_t s);
 
 }  // namespace picgpu
