@@ -8,6 +8,11 @@ so that many zig-zag paths cross cells and some particles leave through every fa
 Yee fields, 6 steps. Particles (matched by id), the Yee E and B after every step and the Yee
 current of the first two steps must agree to 1e-12 normwise; the fixed-point current makes
 two runs bitwise identical.
+
+A second case checks the paths that the small case cannot reach against the CPU restatement
+(oracle/explicit_oracle.cpp): at 8x8x24 cells with 32 ppc per species, a workgroup's 256
+particles start in one (i, j) row of about 8 cells, so the push sums the current of most
+workgroups in its LDS box, and long runs of lanes share a cell in the wave's run sums.
 """
 import os
 
@@ -17,6 +22,7 @@ import pytest
 from conftest import relerr
 from picpp_amd import workloads as wl
 from picpp_amd.simulation import Simulation
+from oracle.oracle import OracleSim
 from test_explicit_oracle import explicit_workload
 
 pytestmark = pytest.mark.gpu
@@ -81,3 +87,42 @@ def test_explicit_step_deterministic_bitwise():
         sim.close()
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+def box_case():
+    w = wl.Workload("explicit-box-8x8x24-ppc32", (8, 8, 24), 1.0, 0.4, 3, wl.electron_proton(32, 0.01, 100.0),
+                    b0=(wl.b_for_gyro(0.5), 0.0, 0.0), seed=5)
+    st = wl.make_initial_state(w)
+    rng = np.random.default_rng(31)
+    e = st.species == 0
+    st.p[e] = w.species[0].mass * rng.normal(0.0, 0.3, (int(e.sum()), 3))
+    return w, st, wl.yee_initial_fields(w, perturb=0.05)
+
+
+def test_explicit_step_matches_oracle_with_workgroup_boxes():
+    w, st, yee = box_case()
+    print("case", w.name, "particles", len(st.ids), flush=True)
+    sim = Simulation(w, precision="fp64", scheme="explicit")
+    sim.load_state(st)
+    sim.load_yee(yee)
+    o = OracleSim(w, st)
+    o.yee_init()
+    for name in wl.YEE_NAMES:
+        o.set_yee(name, yee[name])
+    o.yee_exchange()
+    worst = {}
+    for step in range(1, w.steps + 1):
+        sim.explicit_step()
+        o.explicit_step()
+        p = sim.particles()
+        ids, x, mom = by_id(p["ids"], p["x"], p["p"])
+        rids, rx, rp = by_id(*(o.particles()[k] for k in ("ids", "x", "p")))
+        assert np.array_equal(ids, rids), step
+        worst["x"] = max(worst.get("x", 0.0), relerr(x, rx))
+        worst["p"] = max(worst.get("p", 0.0), relerr(mom, rp))
+        for name in wl.YEE_NAMES + ("Jx", "Jy", "Jz"):
+            worst[name] = max(worst.get(name, 0.0), relerr(sim.yee(name), o.yee(name)))
+    print("explicit step vs oracle:", {k: "%.1e" % v for k, v in worst.items()})
+    for k, v in worst.items():
+        assert v <= 1e-12, (k, v)
+    sim.close()
