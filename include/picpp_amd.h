@@ -267,6 +267,48 @@ int pic_upload_particles_device(pic_ctx* ctx, const long long* counts, const dou
 int pic_profile_enable(pic_ctx* ctx, int on);
 int pic_profile_read(pic_ctx* ctx, int max_entries, char* names, double* total_ms, long long* counts, double* bytes);
 
+/* ---- Checkpoints and the reference's backup files ---- */
+
+/* Lossless checkpoint of the state at a step boundary, in this library's own binary format
+   (little endian; magic "PICAMDCK", version 1): the configuration it was written with, the
+   node and cell fields (E, B, their step copies, the density planes), the Yee arrays of the
+   explicit scheme, the particles in the context's own precision with their ids, and the
+   injection state. pic_checkpoint_load needs a context created with the same grid, time step,
+   species, precision, scheme and boundary (PIC_ERR_ARGUMENT otherwise) and restores the state
+   bit for bit; the following steps repeat an uninterrupted run's bits (FP64 and the explicit
+   scheme; FP32 forms the first step's cell moments afresh, equal to FP32 rounding). The
+   reference's backup (outputSimulationBackup, PIC++/output.cpp:3432-3602) is text with 10
+   significant digits for particles, and its reader (readBackup, PIC++/input.cpp:341-622) does
+   not read what the writer writes, so it cannot resume a run; this format replaces it. */
+int pic_checkpoint_save(pic_ctx* ctx, const char* path);
+int pic_checkpoint_load(pic_ctx* ctx, const char* path);
+
+/* What the reference's backup header holds beyond the context's own state (Simulation members
+   written by outputSimulationBackup, PIC++/output.cpp:3443-3540). */
+typedef struct pic_backup_info {
+	int input_type;                 /* 0 CGS, 1 Theoretical, 2 other */
+	int particles_per_bin[6];       /* types[0..5].particlesPerBin */
+	double concentration[6];        /* types[0..5].concentration */
+	double temperature, plasma_period, scale_factor;
+	double time, max_time;
+	int current_iteration, max_iteration;
+	double size[3];                 /* xsizeGeneral, ysizeGeneral, zsizeGeneral */
+	int debug_mode;
+	int maxwell_matrix_size;
+	double ext_j;
+	double V0[3], E0[3], B0[3];
+	int species_type[8];            /* per species: 0 electron, 1 proton, 2 positron, 3 alpha */
+	double particle_dx;             /* the particles' dx member */
+} pic_backup_info;
+
+/* The reference's backup files for the current state of an implicit-scheme context
+   (outputSimulationBackup with one rank, PIC++/output.cpp:3432-3602): the general file, E on
+   all ghosted nodes and B on all ghosted cells (outputFields, PIC++/output.cpp:1375-1520,
+   "%30.20g" triples, i then j then k), and one line per particle in ascending id order
+   (outputBackupParticle). Text identical to the reference's for the same state. */
+int pic_write_backup(pic_ctx* ctx, const pic_backup_info* info, const char* general_file, const char* e_file,
+                     const char* b_file, const char* particles_file);
+
 /* ---- Learned preconditioner ----
    No counterpart in the reference, whose GMRES is unpreconditioned
    (PIC++/specialmath.cpp:367-671). A 3-D convolution U-Net (3 levels, 16/32/64 channels,

@@ -6,6 +6,9 @@
 // (PIC++/simulation.cpp:94-123 pre-loop, :205-390 loop body, implicit branch,
 // smoothingCount = 0), dumping every per-phase array the hot path produces.
 //
+// With a third argument (a path prefix) it first writes the reference's backup files of the
+// loaded state (outputSimulationBackup): prefix + general.dat, E.dat, B.dat, particles.dat.
+//
 // Input  (binary, little endian): see read_input() below.
 // Output (binary): a sequence of named float64 arrays, read by
 //                  tests/golden/make_golden.py.
@@ -17,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "output.h"
 #include "simulation.h"
 #include "boundaryFieldEvaluator.h"
 #include "particle.h"
@@ -302,6 +306,13 @@ int main(int argc, char** argv) {
 	}
 	fclose(in);
 	s.particlesNumber = (int)np;
+	if (argc > 3) {
+		// the reference's backup files of the loaded state (outputSimulationBackup,
+		// PIC++/output.cpp:3432), named by the prefix argv[3]
+		const std::string pre = argv[3];
+		outputSimulationBackup((pre + "general.dat").c_str(), (pre + "E.dat").c_str(), (pre + "B.dat").c_str(),
+		                       (pre + "particles.dat").c_str(), &s);
+	}
 
 	g_out = fopen(argv[2], "wb");
 	std::vector<double> meta = {(double)NX, (double)NY, (double)NZ, s.deltaX, s.deltaT, s.xgrid[0], s.ygrid[0],

@@ -53,6 +53,16 @@ class PicSolveInfo(ctypes.Structure):
                 ("fallbacks", ctypes.c_int)]
 
 
+class PicBackupInfo(ctypes.Structure):
+    _fields_ = [("input_type", ctypes.c_int), ("particles_per_bin", ctypes.c_int * 6),
+                ("concentration", ctypes.c_double * 6), ("temperature", ctypes.c_double),
+                ("plasma_period", ctypes.c_double), ("scale_factor", ctypes.c_double), ("time", ctypes.c_double),
+                ("max_time", ctypes.c_double), ("current_iteration", ctypes.c_int), ("max_iteration", ctypes.c_int),
+                ("size", ctypes.c_double * 3), ("debug_mode", ctypes.c_int), ("maxwell_matrix_size", ctypes.c_int),
+                ("ext_j", ctypes.c_double), ("V0", ctypes.c_double * 3), ("E0", ctypes.c_double * 3),
+                ("B0", ctypes.c_double * 3), ("species_type", ctypes.c_int * 8), ("particle_dx", ctypes.c_double)]
+
+
 SIGNATURES = {
     "pic_create": (ctypes.c_int, [ctypes.POINTER(PicConfig), _V, ctypes.POINTER(_V)]),
     "pic_destroy": (None, [_V]),
@@ -90,6 +100,9 @@ SIGNATURES = {
     "pic_precond_use": (ctypes.c_int, [_V, ctypes.c_int]),
     "pic_upload_yee": (ctypes.c_int, [_V, ctypes.POINTER(_D)]),
     "pic_explicit_step": (ctypes.c_int, [_V]),
+    "pic_checkpoint_save": (ctypes.c_int, [_V, ctypes.c_char_p]),
+    "pic_checkpoint_load": (ctypes.c_int, [_V, ctypes.c_char_p]),
+    "pic_write_backup": (ctypes.c_int, [_V, ctypes.POINTER(PicBackupInfo)] + [ctypes.c_char_p] * 4),
 }
 
 _lib = None

@@ -1697,6 +1697,289 @@ int pic_step(pic_ctx* c, pic_solve_info* info) {
 	return pic_commit_fields(c);
 }
 
+// ---------------------------------------------------------------- checkpoints and backups
+
+}  // extern "C"
+
+namespace {
+
+constexpr char kCkptMagic[8] = {'P', 'I', 'C', 'A', 'M', 'D', 'C', 'K'};
+constexpr char kCkptEnd[8] = {'C', 'K', 'P', 'T', 'E', 'N', 'D', '\0'};
+constexpr uint32_t kCkptVersion = 1;
+constexpr size_t kStage = size_t(64) << 20;  // host staging per copy, bytes
+
+struct File {
+	FILE* f = nullptr;
+	std::string path;
+	File(const char* p, const char* mode) : f(std::fopen(p, mode)), path(p) {
+		if (!f) throw DeviceError{PIC_ERR_ARGUMENT, std::string("cannot open ") + p};
+	}
+	~File() {
+		if (f) std::fclose(f);
+	}
+	void put(const void* data, size_t n) {
+		if (n && std::fwrite(data, 1, n, f) != n) throw DeviceError{PIC_ERR_ARGUMENT, "short write to " + path};
+	}
+	void get(void* data, size_t n) {
+		if (n && std::fread(data, 1, n, f) != n) throw DeviceError{PIC_ERR_ARGUMENT, "short read from " + path};
+	}
+	template <class T>
+	void put(T v) {
+		put(&v, sizeof(T));
+	}
+	template <class T>
+	T get() {
+		T v;
+		get(&v, sizeof(T));
+		return v;
+	}
+	void close() {
+		if (std::fclose(f) != 0) {
+			f = nullptr;
+			throw DeviceError{PIC_ERR_ARGUMENT, "cannot close " + path};
+		}
+		f = nullptr;
+	}
+};
+
+// device bytes to the file (or back), through a bounded host buffer
+void put_device(File& out, const void* d, size_t bytes, std::vector<char>& stage) {
+	for (size_t o = 0; o < bytes; o += kStage) {
+		const size_t n = std::min(kStage, bytes - o);
+		HIP_CHECK(hipMemcpy(stage.data(), static_cast<const char*>(d) + o, n, hipMemcpyDeviceToHost));
+		out.put(stage.data(), n);
+	}
+}
+void get_device(File& in, void* d, size_t bytes, std::vector<char>& stage) {
+	for (size_t o = 0; o < bytes; o += kStage) {
+		const size_t n = std::min(kStage, bytes - o);
+		in.get(stage.data(), n);
+		HIP_CHECK(hipMemcpy(static_cast<char*>(d) + o, stage.data(), n, hipMemcpyHostToDevice));
+	}
+}
+
+// The arrays of a checkpoint, in file order, with their sizes in bytes for np particles.
+std::vector<std::pair<void*, size_t>> checkpoint_arrays(pic_ctx* c, long long np) {
+	const Grid& g = c->g;
+	std::vector<std::pair<void*, size_t>> a;
+	const size_t nn = 3 * g.nnodes * sizeof(double), nc = 3 * g.ncells * sizeof(double);
+	a.push_back({c->E, nn});
+	a.push_back({c->Etheta, nn});
+	a.push_back({c->Enew, nn});
+	a.push_back({c->B, nc});
+	a.push_back({c->Bnew, nc});
+	a.push_back({c->dens, (size_t)(5 * c->sp.count + 1) * g.ncells * sizeof(double)});
+	if (c->explicit_scheme)
+		for (int comp = 0; comp < 3; ++comp) {
+			a.push_back({c->yee.E[comp], yee_count(c->yee.dE[comp]) * sizeof(double)});
+			a.push_back({c->yee.B[comp], yee_count(c->yee.dB[comp]) * sizeof(double)});
+			a.push_back({c->yee.J[comp], yee_count(c->yee.dE[comp]) * sizeof(double)});
+		}
+	if (c->fp32) {
+		a.push_back({c->Q.cell, np * sizeof(int)});
+		for (int d = 0; d < 3; ++d) a.push_back({c->Q.off[d], np * sizeof(float)});
+		for (int d = 0; d < 3; ++d) a.push_back({c->Q.u[d], np * sizeof(float)});
+		a.push_back({c->Q.id, np * sizeof(int)});
+	} else {
+		for (int d = 0; d < 3; ++d) a.push_back({c->P.x[d], np * sizeof(double)});
+		for (int d = 0; d < 3; ++d) a.push_back({c->P.p[d], np * sizeof(double)});
+		a.push_back({c->P.id, np * sizeof(int)});
+	}
+	return a;
+}
+
+void checkpoint_config(pic_ctx* c, File& f, bool write) {
+	// what must match between the writing and the reading context
+	auto same = [&](auto mine, const char* what) {
+		using T = decltype(mine);
+		if (write) {
+			f.put<T>(mine);
+		} else if (f.get<T>() != mine) {
+			throw DeviceError{PIC_ERR_ARGUMENT, std::string("checkpoint written with another ") + what};
+		}
+	};
+	for (int a = 0; a < 3; ++a) same((int32_t)c->g.n[a], "grid");
+	for (int a = 0; a < 3; ++a) same(c->g.d[a], "cell size");
+	same(c->cfg.dt, "time step");
+	same((int32_t)c->sp.count, "species count");
+	for (int s = 0; s < c->sp.count; ++s) {
+		same(c->sp.mass[s], "species mass");
+		same(c->sp.charge[s], "species charge");
+		same(c->sp.weight[s], "species weight");
+	}
+	same((int32_t)c->fp32, "precision");
+	same((int32_t)c->explicit_scheme, "scheme");
+	same((int32_t)c->g.wall, "boundary");
+}
+
+}  // namespace
+
+extern "C" {
+
+int pic_checkpoint_save(pic_ctx* c, const char* path) {
+	if (!c || !path) return PIC_ERR_ARGUMENT;
+	try {
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+		File f(path, "wb");
+		f.put(kCkptMagic, 8);
+		f.put<uint32_t>(kCkptVersion);
+		checkpoint_config(c, f, true);
+		const long long np = c->np();
+		f.put<int64_t>(np);
+		f.put<int64_t>(c->next_id);
+		f.put<uint64_t>(c->inj_step);
+		f.put<uint32_t>(c->inj_slabs);
+		for (int s = 0; s < kMaxSpecies; ++s) f.put<double>(c->inj_len[s]);
+		for (int s = 0; s <= kMaxSpecies; ++s) f.put<int32_t>(c->species_start_h[s]);
+		std::vector<char> stage(kStage);
+		for (const auto& a : checkpoint_arrays(c, np)) put_device(f, a.first, a.second, stage);
+		f.put(kCkptEnd, 8);
+		f.close();
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_checkpoint_load(pic_ctx* c, const char* path) {
+	if (!c || !path) return PIC_ERR_ARGUMENT;
+	c->fused = 0;
+	try {
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+		File f(path, "rb");
+		char magic[8];
+		f.get(magic, 8);
+		if (std::memcmp(magic, kCkptMagic, 8) != 0) throw DeviceError{PIC_ERR_ARGUMENT, "not a checkpoint of this library"};
+		if (f.get<uint32_t>() != kCkptVersion) throw DeviceError{PIC_ERR_ARGUMENT, "unsupported checkpoint version"};
+		checkpoint_config(c, f, false);
+		const long long np = f.get<int64_t>();
+		if (np < 0 || np > c->capacity) throw DeviceError{PIC_ERR_ARGUMENT, "checkpoint particle count exceeds the capacity"};
+		const long long next_id = f.get<int64_t>();
+		const uint64_t inj_step = f.get<uint64_t>();
+		const uint32_t inj_slabs = f.get<uint32_t>();
+		double inj_len[kMaxSpecies];
+		for (int s = 0; s < kMaxSpecies; ++s) inj_len[s] = f.get<double>();
+		std::vector<int> start(kMaxSpecies + 1);
+		for (int s = 0; s <= kMaxSpecies; ++s) start[s] = f.get<int32_t>();
+		for (int s = 0; s < c->sp.count; ++s)
+			if (start[s] < 0 || start[s] > start[s + 1] || start[s + 1] > np) throw DeviceError{PIC_ERR_ARGUMENT, "bad species ranges"};
+		std::vector<char> stage(kStage);
+		for (const auto& a : checkpoint_arrays(c, np)) get_device(f, a.first, a.second, stage);
+		char end[8];
+		f.get(end, 8);
+		if (std::memcmp(end, kCkptEnd, 8) != 0) throw DeviceError{PIC_ERR_ARGUMENT, "checkpoint truncated or corrupt"};
+		set_count(c, np);
+		c->next_id = next_id;
+		c->inj_step = inj_step;
+		c->inj_slabs = inj_slabs;
+		for (int s = 0; s < kMaxSpecies; ++s) c->inj_len[s] = inj_len[s];
+		c->species_start_h = start;
+		hipStream_t s = c->stream;
+		HIP_CHECK(hipMemcpyAsync(c->species_start, c->species_start_h.data(), (kMaxSpecies + 1) * sizeof(int),
+		                         hipMemcpyHostToDevice, s));
+		// the particles were written in key order: the stable sort keeps that order and
+		// rebuilds the segment starts
+		if (c->fp32) {
+			launch_keys32(c->g, c->sp, c->Q, c->species_start, c->keys, s);
+		} else {
+			launch_keys(c->g, c->sp, c->P, c->species_start, c->keys, c->status, s);
+			check_status(c);
+		}
+		sort_particles(c);
+		HIP_CHECK(hipStreamSynchronize(s));
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_write_backup(pic_ctx* c, const pic_backup_info* info, const char* general_file, const char* e_file, const char* b_file,
+                     const char* particles_file) {
+	if (!c || !info || !general_file || !e_file || !b_file || !particles_file) return PIC_ERR_ARGUMENT;
+	if (c->explicit_scheme) {
+		c->error = "pic_write_backup: the implicit scheme's fields only (the explicit scheme keeps the Yee arrays)";
+		return PIC_ERR_ARGUMENT;
+	}
+	try {
+		for (int s = 0; s < c->sp.count; ++s)
+			if (info->species_type[s] < 0 || info->species_type[s] > 3)
+				throw DeviceError{PIC_ERR_ARGUMENT, "species_type must be 0 .. 3 (electron, proton, positron, alpha)"};
+		const Grid& g = c->g;
+		const long long np = c->np();
+		{  // outputSimulationBackup's general file (PIC++/output.cpp:3443-3540)
+			File f(general_file, "w");
+			FILE* o = f.f;
+			std::fprintf(o, "%d\n", info->input_type == 0 ? 0 : (info->input_type == 1 ? 1 : 2));
+			for (int a = 0; a < 3; ++a) std::fprintf(o, "%d\n", g.n[a]);
+			std::fprintf(o, "%d\n", (int)np);
+			for (int t = 0; t < 6; ++t) std::fprintf(o, "%d\n", info->particles_per_bin[t]);
+			for (int t = 0; t < 6; ++t) std::fprintf(o, "%lf\n", info->concentration[t]);
+			for (double v : {info->temperature, info->plasma_period, info->scale_factor, info->time, info->max_time})
+				std::fprintf(o, "%15.10g\n", v);
+			std::fprintf(o, "%d\n", info->current_iteration);
+			std::fprintf(o, "%d\n", info->max_iteration);
+			for (double v : {info->size[0], info->size[1], info->size[2], c->theta, c->eta}) std::fprintf(o, "%15.10g\n", v);
+			std::fprintf(o, "%d\n", info->debug_mode ? 1 : 0);
+			std::fprintf(o, "%d\n", c->explicit_scheme ? 0 : 1);  // solverType: IMPLICIT 1
+			std::fprintf(o, "%d\n", g.wall ? 0 : 1);             // boundaryConditionTypeX: PERIODIC 1
+			std::fprintf(o, "%d\n", info->maxwell_matrix_size);
+			std::fprintf(o, "%15.10g\n", info->ext_j);
+			for (const double* v : {info->V0, info->E0, info->B0})
+				for (int a = 0; a < 3; ++a) std::fprintf(o, "%15.10g\n", v[a]);
+			f.close();
+		}
+		HIP_CHECK(hipStreamSynchronize(c->stream));
+		{  // outputFields with scale 1 over all ghosted nodes (E) and cells (B)
+			std::vector<double> E(3 * g.nnodes), B(3 * g.ncells);
+			HIP_CHECK(hipMemcpy(E.data(), c->E, E.size() * sizeof(double), hipMemcpyDeviceToHost));
+			HIP_CHECK(hipMemcpy(B.data(), c->B, B.size() * sizeof(double), hipMemcpyDeviceToHost));
+			File fe(e_file, "w");
+			for (int i = 0; i <= g.N[0]; ++i)
+				for (int j = 0; j <= g.N[1]; ++j)
+					for (int k = 0; k <= g.N[2]; ++k) {
+						const long long q = node_id(g, i, j, k);
+						std::fprintf(fe.f, "%30.20g %30.20g %30.20g\n", E[q], E[g.nnodes + q], E[2 * g.nnodes + q]);
+					}
+			fe.close();
+			File fb(b_file, "w");
+			for (int i = 0; i < g.N[0]; ++i)
+				for (int j = 0; j < g.N[1]; ++j)
+					for (int k = 0; k < g.N[2]; ++k) {
+						const long long q = cell_id(g, i, j, k);
+						std::fprintf(fb.f, "%30.20g %30.20g %30.20g\n", B[q], B[g.ncells + q], B[2 * g.ncells + q]);
+					}
+			fb.close();
+		}
+		{  // outputBackupParticle per particle (PIC++/output.cpp:3576-3601), ascending id
+			std::vector<int> sp(np), ids(np);
+			std::vector<double> x(3 * np), p(3 * np);
+			if (np > 0) {
+				const int rc = pic_download_particles(c, sp.data(), ids.data(), x.data(), p.data());
+				if (rc != PIC_OK) return rc;
+			}
+			std::vector<long long> order(np);
+			for (long long i = 0; i < np; ++i) order[i] = i;
+			std::sort(order.begin(), order.end(), [&](long long a, long long b) { return ids[a] < ids[b]; });
+			File fp(particles_file, "w");
+			for (long long i : order) {
+				const int s = sp[i];
+				std::fprintf(fp.f, "%d ", ids[i]);
+				std::fprintf(fp.f, "%15.10g ", c->sp.mass[s]);
+				std::fprintf(fp.f, "%15.10g ", c->sp.charge[s]);
+				std::fprintf(fp.f, "%15.10g ", c->sp.weight[s]);
+				std::fprintf(fp.f, "%d ", info->species_type[s]);
+				for (int a = 0; a < 3; ++a) std::fprintf(fp.f, "%15.10g ", x[a * np + i]);
+				for (int a = 0; a < 3; ++a) std::fprintf(fp.f, "%15.10g ", p[a * np + i]);
+				std::fprintf(fp.f, "%15.10g\n", info->particle_dx);
+			}
+			fp.close();
+		}
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
 int pic_profile_enable(pic_ctx* c, int on) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	try {

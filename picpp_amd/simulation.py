@@ -8,6 +8,7 @@ copies are made only by the explicit accessors.
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import Dict, Optional
 
 import numpy as np
@@ -225,6 +226,31 @@ class Simulation:
     def explicit_step(self):
         """One step of the reference's BUNEMAN loop body (PIC++/simulation.cpp:163-197)."""
         self._chk(self.lib.pic_explicit_step(self.h), "pic_explicit_step")
+
+    def save_checkpoint(self, path: str):
+        """Lossless checkpoint of the state at a step boundary (the library's own format)."""
+        self._chk(self.lib.pic_checkpoint_save(self.h, os.fsencode(path)), "pic_checkpoint_save")
+
+    def load_checkpoint(self, path: str):
+        """Restores a checkpoint written by a context of the same configuration."""
+        self._chk(self.lib.pic_checkpoint_load(self.h, os.fsencode(path)), "pic_checkpoint_load")
+
+    def outputSimulationBackup(self, general_file: str, e_file: str, b_file: str, particles_file: str, **info):
+        """The reference's backup files (PIC++/output.cpp:3432-3602) of the current state. The
+        keyword arguments fill pic_backup_info (input_type, particles_per_bin, concentration,
+        temperature, plasma_period, scale_factor, time, max_time, current_iteration,
+        max_iteration, size, debug_mode, maxwell_matrix_size, ext_j, V0, E0, B0, species_type,
+        particle_dx); missing ones are 0."""
+        bi = _lib.PicBackupInfo()
+        for key, val in info.items():
+            cur = getattr(bi, key)
+            if isinstance(cur, ctypes.Array):
+                for i, v in enumerate(val):
+                    cur[i] = v
+            else:
+                setattr(bi, key, val)
+        paths = [os.fsencode(p) for p in (general_file, e_file, b_file, particles_file)]
+        self._chk(self.lib.pic_write_backup(self.h, ctypes.byref(bi), *paths), "pic_write_backup")
 
     def synchronize(self):
         self._chk(self.lib.pic_synchronize(self.h), "pic_synchronize")

@@ -125,6 +125,25 @@ def make_mini():
     np.savez_compressed(os.path.join(HERE, "mini_ref.npz"), **inputs(st), **out)
 
 
+def make_backup():
+    """The compiled reference's backup files (outputSimulationBackup) of the mini case's
+    initial state, with the harness's constructor values in the general file."""
+    w = wl.mini()
+    st = wl.make_initial_state(w, perturb=0.05)
+    with tempfile.TemporaryDirectory() as td:
+        for d in REF_DIRS:
+            os.makedirs(os.path.join(td, d), exist_ok=True)
+        fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_input(fi, w, st, 0)
+        pre = os.path.join(td, "backup_")
+        subprocess.run([HARNESS, fi, fo, pre], check=True, stdout=subprocess.DEVNULL, cwd=td)
+        files = {}
+        for name in ("general", "E", "B", "particles"):
+            with open(pre + name + ".dat", "rb") as f:
+                files["file/" + name] = np.frombuffer(f.read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "backup_ref.npz"), **inputs(st), **files)
+
+
 def wall_case():
     w = wl.shock(n=(12, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=5, seed=19, name="wall",
                  inject=False)
@@ -267,5 +286,6 @@ if __name__ == "__main__":
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref", "-j8"], check=True)
     for name in which:
         {"mini": make_mini, "config1": make_config1, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject,
-         "wall_long": make_wall_long, "smooth": make_smooth, "explicit": make_explicit}[name]()
+         "wall_long": make_wall_long, "smooth": make_smooth, "explicit": make_explicit,
+         "backup": make_backup}[name]()
         print("wrote", name)

@@ -1,0 +1,175 @@
+"""Checkpoints (pic_checkpoint_save / pic_checkpoint_load, the library's own lossless format)
+and the reference's backup files (pic_write_backup, outputSimulationBackup).
+
+- A run interrupted by a checkpoint and resumed in a fresh context repeats the uninterrupted
+  run's bits: FP64 implicit steps (mini case) and explicit steps (the explicit golden case).
+  FP32 agrees to FP32 rounding: its first resumed step forms the cell moments afresh instead
+  of taking them from the post-push pass.
+- A checkpoint does not load into a context of another configuration, and a truncated file is
+  refused.
+- The backup files of the mini case's initial state are byte for byte the compiled
+  reference's (tests/golden/backup_ref.npz, written by oracle/ref_harness.cpp when given a
+  path prefix).
+"""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+from picpp_amd import _lib
+from picpp_amd import workloads as wl
+from picpp_amd.simulation import Simulation
+from test_gpu_explicit import open_explicit
+from test_explicit_oracle import explicit_workload
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def state(sim, explicit=False):
+    p = sim.particles()
+    out = {"ids": p["ids"], "x": p["x"], "p": p["p"]}
+    names = wl.YEE_NAMES if explicit else ("E", "B")
+    for n in names:
+        out[n] = sim.yee(n) if explicit else sim.array(n)
+    return out
+
+
+def assert_bitwise(a, b):
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_checkpoint_resume_implicit(tmp_path, precision):
+    w = wl.mini()
+    st = wl.make_initial_state(w, perturb=0.05)
+    print("case", w.name, precision, flush=True)
+    a = Simulation(w, precision=precision)
+    a.load_state(st)
+    a.initialize()
+    for _ in range(2):
+        a.step()
+    path = str(tmp_path / "run.ckpt")
+    a.save_checkpoint(path)
+    saved = state(a)
+    for _ in range(3):
+        a.step()
+    b = Simulation(w, precision=precision)
+    b.load_checkpoint(path)
+    assert_bitwise(saved, state(b))
+    for _ in range(3):
+        b.step()
+    sa, sb = state(a), state(b)
+    if precision == "fp64":
+        assert_bitwise(sa, sb)
+    else:
+        assert np.array_equal(sa["ids"], sb["ids"])
+        errs = {k: relerr(sb[k], sa[k]) for k in ("x", "p", "E", "B")}
+        print("fp32 resumed vs uninterrupted:", {k: "%.1e" % v for k, v in errs.items()})
+        for k, v in errs.items():
+            assert v <= 1e-5, (k, v)
+    a.close()
+    b.close()
+
+
+def test_checkpoint_resume_explicit(tmp_path):
+    d = np.load(os.path.join(HERE, "golden", "explicit_ref.npz"))
+    w = explicit_workload()
+    print("case", w.name, flush=True)
+    a = open_explicit(d, w)
+    for _ in range(2):
+        a.explicit_step()
+    path = str(tmp_path / "run.ckpt")
+    a.save_checkpoint(path)
+    for _ in range(3):
+        a.explicit_step()
+    b = Simulation(w, precision="fp64", scheme="explicit")
+    b.load_checkpoint(path)
+    for _ in range(3):
+        b.explicit_step()
+    assert_bitwise(state(a, True), state(b, True))
+    a.close()
+    b.close()
+
+
+def test_checkpoint_refuses_other_configuration_and_truncation(tmp_path):
+    w = wl.mini()
+    st = wl.make_initial_state(w, perturb=0.05)
+    a = Simulation(w, precision="fp64")
+    a.load_state(st)
+    a.initialize()
+    path = str(tmp_path / "run.ckpt")
+    a.save_checkpoint(path)
+    for other in (dataclasses.replace(w, dt=0.2), dataclasses.replace(w, n=(6, 5, 6))):
+        b = Simulation(other, precision="fp64")
+        with pytest.raises(_lib.PicError, match="checkpoint written with another"):
+            b.load_checkpoint(path)
+        b.close()
+    c = Simulation(w, precision="fp32")
+    with pytest.raises(_lib.PicError, match="precision"):
+        c.load_checkpoint(path)
+    c.close()
+    cut = str(tmp_path / "cut.ckpt")
+    with open(path, "rb") as f:
+        data = f.read()
+    with open(cut, "wb") as f:
+        f.write(data[: len(data) - 100])
+    d = Simulation(w, precision="fp64")
+    with pytest.raises(_lib.PicError, match="short read|truncated"):
+        d.load_checkpoint(cut)
+    d.close()
+    a.close()
+
+
+def parse_general(text):
+    """The general file's values in outputSimulationBackup's order (PIC++/output.cpp:3443-3540)."""
+    v = text.split()
+    it = iter(v)
+    nxt = lambda: next(it)  # noqa: E731
+    info = {"input_type": int(nxt())}
+    for _ in range(4):
+        nxt()  # x, y, z cell counts and the particle count: the context's own
+    info["particles_per_bin"] = [int(nxt()) for _ in range(6)]
+    info["concentration"] = [float(nxt()) for _ in range(6)]
+    for key in ("temperature", "plasma_period", "scale_factor", "time", "max_time"):
+        info[key] = float(nxt())
+    info["current_iteration"] = int(nxt())
+    info["max_iteration"] = int(nxt())
+    info["size"] = [float(nxt()) for _ in range(3)]
+    nxt(), nxt()  # theta, eta: the context's own
+    info["debug_mode"] = int(nxt())
+    nxt(), nxt()  # solver type, boundary type: the context's own
+    info["maxwell_matrix_size"] = int(nxt())
+    info["ext_j"] = float(nxt())
+    for key in ("V0", "E0", "B0"):
+        info[key] = [float(nxt()) for _ in range(3)]
+    return info
+
+
+def test_backup_files_match_reference(tmp_path):
+    d = np.load(os.path.join(HERE, "golden", "backup_ref.npz"))
+    w = wl.mini()
+    print("case", w.name, "backup of the initial state", flush=True)
+    st = wl.InitialState(d["in/species"].astype(np.int32), d["in/ids"].astype(np.int32), d["in/x"], d["in/p"],
+                         d["in/E"], d["in/B"])
+    sim = Simulation(w, precision="fp64")
+    sim.load_state(st)
+    ref = {k[5:]: d[k].tobytes().decode() for k in d.files if k.startswith("file/")}
+    info = parse_general(ref["general"])
+    info["species_type"] = [{"electron": 0, "proton": 1}[s.name] for s in w.species]
+    info["particle_dx"] = w.dx / 2
+    names = {k: str(tmp_path / (k + ".dat")) for k in ("general", "E", "B", "particles")}
+    sim.outputSimulationBackup(names["general"], names["E"], names["B"], names["particles"], **info)
+    for k, path in names.items():
+        with open(path) as f:
+            mine = f.read()
+        if mine != ref[k]:
+            a, b = mine.splitlines(), ref[k].splitlines()
+            bad = next((i for i, (x, y) in enumerate(zip(a, b)) if x != y), min(len(a), len(b)))
+            pytest.fail("%s differs at line %d: %r vs %r" % (k, bad, a[bad] if bad < len(a) else None,
+                                                            b[bad] if bad < len(b) else None))
+    sim.close()
