@@ -89,8 +89,8 @@ def test_explicit_step_deterministic_bitwise():
         assert np.array_equal(a, b)
 
 
-def box_case():
-    w = wl.Workload("explicit-box-8x8x24-ppc32", (8, 8, 24), 1.0, 0.4, 3, wl.electron_proton(32, 0.01, 100.0),
+def box_case(n=(8, 8, 24), ppc=32, steps=3):
+    w = wl.Workload("explicit-%dx%dx%d-ppc%d" % (n + (ppc,)), n, 1.0, 0.4, steps, wl.electron_proton(ppc, 0.01, 100.0),
                     b0=(wl.b_for_gyro(0.5), 0.0, 0.0), seed=5)
     st = wl.make_initial_state(w)
     rng = np.random.default_rng(31)
@@ -99,8 +99,11 @@ def box_case():
     return w, st, wl.yee_initial_fields(w, perturb=0.05)
 
 
-def test_explicit_step_matches_oracle_with_workgroup_boxes():
-    w, st, yee = box_case()
+@pytest.mark.parametrize("n,ppc,steps", [((8, 8, 24), 32, 3), ((64, 64, 64), 4, 2)])
+def test_explicit_step_matches_oracle_with_workgroup_boxes(n, ppc, steps):
+    # the second case: 2.1 M particles on 68^3 ghosted cells, rows of 64 cells holding about
+    # 256 particles per species, so many workgroups straddle two rows
+    w, st, yee = box_case(n, ppc, steps)
     print("case", w.name, "particles", len(st.ids), flush=True)
     sim = Simulation(w, precision="fp64", scheme="explicit")
     sim.load_state(st)
