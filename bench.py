@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--explicit", type=int, default=0, metavar="STEPS",
                     help="only time STEPS steps of the explicit (BUNEMAN / Tristan) FP64 step on the benchmark box "
                          "(--cells, --ppc) instead of the default benchmark")
+    ap.add_argument("--explicit-steps", type=int, default=5,
+                    help="with the default benchmark, also time this many explicit (BUNEMAN / Tristan) FP64 steps on the "
+                         "same box and report them as explicit_step (0: skip)")
     ap.add_argument("--shock-cells", type=int, default=1024, help="cells along x (64 x 64 across)")
     ap.add_argument("--shock-ppc", type=int, default=16, help="particles per cell per species at injection")
     ap.add_argument("--pressure", default=None, choices=["reference", "cell"],
@@ -554,6 +557,13 @@ def main():
                             "stop_rule": "true_residual", "speedup_of_chosen_form": el64 / el}
         print("fp64 step: %.2f ms (chosen form %.2f ms, %.2fx)" % (1e3 * el64 / args.steps, 1e3 * el / args.steps, el64 / el),
               file=sys.stderr)
+    if args.explicit_steps > 0 and args.precision == "fp32":
+        # the reference's default (BUNEMAN) step on the same box, FP64, reported beside the headline
+        ex = argparse.Namespace(**vars(args))
+        ex.explicit, ex.warmup = args.explicit_steps, 1
+        out["explicit_step"] = explicit_run(ex, torch, dev)
+        torch.cuda.empty_cache()
+        print("explicit step: %.2f ms" % out["explicit_step"]["ms_per_step"], file=sys.stderr, flush=True)
     if args.drift_steps > 0:
         out["energy_drift"] = energy_drift(args, w, torch, dev)
     if not args.no_cpu_baseline:
