@@ -129,3 +129,28 @@ def test_explicit_step_matches_oracle_with_workgroup_boxes(n, ppc, steps):
     for k, v in worst.items():
         assert v <= 1e-12, (k, v)
     sim.close()
+
+
+def test_explicit_step_without_particles():
+    # no particles: the Yee leapfrog alone (a vacuum), against the oracle; the current stays 0
+    w, st, yee = box_case((6, 5, 7), 1, 3)
+    empty = wl.InitialState(st.species[:0], st.ids[:0], st.x[:0], st.p[:0], st.E, st.B)
+    print("case", w.name, "without particles", flush=True)
+    sim = Simulation(w, precision="fp64", scheme="explicit")
+    sim.load_state(empty)
+    sim.load_yee(yee)
+    o = OracleSim(w, empty)
+    o.yee_init()
+    for name in wl.YEE_NAMES:
+        o.set_yee(name, yee[name])
+    o.yee_exchange()
+    for step in range(w.steps):
+        sim.explicit_step()
+        o.explicit_step()
+        assert sim.num_particles() == 0
+        for name in wl.YEE_NAMES:
+            err = relerr(sim.yee(name), o.yee(name))
+            assert err <= 1e-13, (step, name, err)
+        for name in ("Jx", "Jy", "Jz"):
+            assert not np.any(sim.yee(name)), name
+    sim.close()
