@@ -33,6 +33,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+from picpp_amd.simulation import open_synthetic_box  # noqa: E402
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -84,55 +86,6 @@ def parse():
     return ap.parse_args()
 
 
-def make_state_on_device(w, torch, device):
-    """Seeded synthetic state on the GPU: uniform positions in each cell, Maxwell-Juettner
-    momenta (tabulated inverse CDF, as picpp_amd.workloads does on the host)."""
-    from picpp_amd import workloads as wl
-
-    gen = torch.Generator(device=device)
-    gen.manual_seed(w.seed)
-    n = w.n
-    ncell = n[0] * n[1] * n[2]
-    xs, ps, counts = [], [], []
-    for s in w.species:
-        cnt = ncell * s.ppc
-        cid = torch.arange(ncell, device=device, dtype=torch.int64).repeat_interleave(s.ppc)
-        ci = torch.stack([cid // (n[1] * n[2]), (cid // n[2]) % n[1], cid % n[2]]).to(torch.float64)
-        del cid
-        lo = torch.tensor([v * w.dx for v in n], device=device, dtype=torch.float64)[:, None]
-        x = lo + (ci + torch.rand((3, cnt), generator=gen, device=device, dtype=torch.float64)) * w.dx
-        del ci
-        ut, cdf = wl._mj_table(s.temperature)
-        ut = torch.tensor(ut, device=device)
-        cdf = torch.tensor(cdf, device=device)
-        r = torch.rand(cnt, generator=gen, device=device, dtype=torch.float64)
-        k = torch.searchsorted(cdf, r).clamp(1, len(cdf) - 1)
-        t = ((r - cdf[k - 1]) / (cdf[k] - cdf[k - 1]).clamp_min(1e-300)).clamp(0, 1)
-        mag = ut[k - 1] + t * (ut[k] - ut[k - 1])
-        del r, k, t
-        cost = 2 * torch.rand(cnt, generator=gen, device=device, dtype=torch.float64) - 1
-        phi = 2 * math.pi * torch.rand(cnt, generator=gen, device=device, dtype=torch.float64)
-        sint = torch.sqrt((1 - cost * cost).clamp_min(0))
-        u = torch.stack([mag * sint * torch.cos(phi), mag * sint * torch.sin(phi), mag * cost])
-        del cost, phi, sint, mag
-        if s.drift_gamma > 1.0:  # boost along the drift axis (volume-transform flipping, as workloads.boost)
-            beta = s.drift_sign * math.sqrt(1.0 - 1.0 / (s.drift_gamma * s.drift_gamma))
-            g = torch.sqrt(1.0 + (u * u).sum(0))
-            ua = u[s.drift_axis]
-            flip = (-beta * ua / g) > torch.rand(cnt, generator=gen, device=device, dtype=torch.float64)
-            ua = torch.where(flip, -ua, ua)
-            u[s.drift_axis] = s.drift_gamma * (ua + beta * g)
-            del g, ua, flip
-        p = u * s.mass
-        del u
-        xs.append(x)
-        ps.append(p)
-        counts.append(cnt)
-    x = torch.cat(xs, 1).contiguous()
-    p = torch.cat(ps, 1).contiguous()
-    return x, p, counts
-
-
 def cpu_baseline(seconds: float):
     """The FP64 oracle (a single-threaded C++ restatement of the reference step) on a
     bounded sample of the same physics: the reference's CPU case, 32x32x8 cells with
@@ -158,35 +111,6 @@ def cpu_baseline(seconds: float):
             "sample": "oracle FP64 step, 32x32x8 cells x 40 ppc (%d particles), %d steps in %.1f s" % (npart, steps, el)}
 
 
-def open_box(w, torch, dev, **kw):
-    """A context on the current torch stream holding w's synthetic initial state (E = 0, B = B0
-    uniform, particles generated on the device), after the pre-loop sequence."""
-    from picpp_amd import workloads as wl
-    from picpp_amd.simulation import Simulation
-
-    stream = torch.cuda.current_stream(dev)
-    sim = Simulation(w, capacity=w.capacity_needed(), stream=stream.cuda_stream, **kw)
-    N = w.N
-    E = np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3))
-    E[...] = np.array(w.e0)
-    if w.boundary == "wall":  # no tangential E at and behind the wall (PIC++/initialization.cpp:5282-5287)
-        E[:3, :, :, 1:] = 0.0
-    B = np.zeros(N + (3,))
-    B[...] = np.array(w.b0)
-    st = wl.InitialState(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros((0, 3)), np.zeros((0, 3)), E, B)
-    sim.load_state(st)
-    x, p, counts = make_state_on_device(w, torch, dev)
-    torch.cuda.synchronize()
-    cnt = (ctypes.c_longlong * len(counts))(*counts)
-    _D = ctypes.POINTER(ctypes.c_double)
-    rc = sim.lib.pic_upload_particles_device(sim.h, cnt, ctypes.cast(x.data_ptr(), _D), ctypes.cast(p.data_ptr(), _D), None)
-    sim._chk(rc, "pic_upload_particles_device")
-    del x, p
-    torch.cuda.empty_cache()
-    sim.initialize()
-    return sim
-
-
 def train_network(args, torch, dev, w=None):
     """Trains the learned preconditioner on pairs logged by the mixed-precision solves of a
     train_cells^3 box with the same physics (by default the benchmark's, 16 ppc per species).
@@ -196,7 +120,7 @@ def train_network(args, torch, dev, w=None):
 
     if w is None:
         w = wl.config2(n=(args.train_cells,) * 3, ppc=16, steps=0)
-    sim = open_box(w, torch, dev, precision="fp32", stop_rule="true_residual", solver="mixed")
+    sim = open_synthetic_box(w, torch, dev, precision="fp32", stop_rule="true_residual", solver="mixed")
     sim.precond_create()
     sim.precond_log(args.train_pairs)
     t0 = time.perf_counter()
@@ -286,7 +210,7 @@ def energy_drift(args, w, torch, dev):
     series = {}
     for name, kw in (("chosen", dict(precision=args.precision, stop_rule=args.stop_rule, solver=args.solver)),
                      ("fp64", dict(precision="fp64", stop_rule="true_residual", solver="fp64"))):
-        sim = open_box(w, torch, dev, pressure=args.pressure or "reference", **kw)
+        sim = open_synthetic_box(w, torch, dev, pressure=args.pressure or "reference", **kw)
         e = [sim.updateEnergy()[3]]
         for i in range(args.drift_steps):
             sim.step()
@@ -331,7 +255,7 @@ def weibel_check(args, torch, dev):
     solver_stats = {}
     for name, kw in (("chosen", dict(precision=args.precision, stop_rule=args.stop_rule, solver=args.solver)),
                      ("fp64", dict(precision="fp64", stop_rule="true_residual", solver="fp64"))):
-        sim = open_box(w, torch, dev, pressure=args.pressure or "reference", **kw)
+        sim = open_synthetic_box(w, torch, dev, pressure=args.pressure or "reference", **kw)
         if name == "chosen" and learned:
             sim.precond_create()
             sim.set_precond_params(params, mu_scale)
@@ -384,7 +308,7 @@ def explicit_run(args, torch, dev):
     from picpp_amd import workloads as wl
 
     w = wl.config2(n=(args.cells,) * 3, ppc=args.ppc, steps=args.explicit)
-    sim = open_box(w, torch, dev, precision="fp64", scheme="explicit")
+    sim = open_synthetic_box(w, torch, dev, precision="fp64", scheme="explicit")
     sim.load_yee(wl.yee_initial_fields(w))
     npart = sim.num_particles()
     print("explicit: %s, %d particles" % (w.name, npart), file=sys.stderr, flush=True)
@@ -414,7 +338,7 @@ def shock_run(args, torch, dev):
 
     w = wl.shock(n=(args.shock_cells, 64, 64), ppc=args.shock_ppc, steps=args.shock)
     pressure = args.pressure or "cell"
-    sim = open_box(w, torch, dev, precision=args.precision, stop_rule=args.stop_rule, solver=args.solver, pressure=pressure)
+    sim = open_synthetic_box(w, torch, dev, precision=args.precision, stop_rule=args.stop_rule, solver=args.solver, pressure=pressure)
     n0 = sim.num_particles()
     print("shock: %s, %d particles, precision %s, solver %s" % (w.name, n0, args.precision, args.solver), file=sys.stderr,
           flush=True)
@@ -492,7 +416,7 @@ def main():
         params, mu_scale, training = train_network(args, torch, dev)
     n = (args.cells,) * 3
     w = wl.config2(n=n, ppc=args.ppc, steps=args.steps)
-    sim = open_box(w, torch, dev, precision=args.precision, stop_rule=args.stop_rule, ortho=args.ortho, solver=args.solver,
+    sim = open_synthetic_box(w, torch, dev, precision=args.precision, stop_rule=args.stop_rule, ortho=args.ortho, solver=args.solver,
                    pressure=args.pressure or "reference")
     npart = sim.num_particles()
     print("bench: %s, %d particles, precision %s, solver %s" % (w.name, npart, args.precision, args.solver), file=sys.stderr,
@@ -546,7 +470,7 @@ def main():
         "energy": {"particles": energy[0], "E": energy[1], "B": energy[2], "total": energy[3]},
     }
     if not args.no_fp64_compare and args.precision == "fp32":
-        sim = open_box(w, torch, dev, precision="fp64", stop_rule="true_residual", solver="fp64",
+        sim = open_synthetic_box(w, torch, dev, precision="fp64", stop_rule="true_residual", solver="fp64",
                        pressure=args.pressure or "reference")
         el64, solves64 = time_steps(sim, args)
         sim.close()

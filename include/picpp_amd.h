@@ -243,6 +243,15 @@ int pic_step(pic_ctx* ctx, pic_solve_info* info);
 int pic_gmres_solve(pic_ctx* ctx, const double* rhs, double* x, pic_solve_info* info);
 /* y = A v on device vectors [3][n0*n1*n2] (multiplySpecialMatrixVector, specialmath.cpp:202-252) */
 int pic_apply_operator(pic_ctx* ctx, const double* v, double* y);
+/* The same operator in the factored form that PIC_SOLVER_MIXED applies (multiplySpecialMatrixVector,
+   PIC++/specialmath.cpp:202-252, on the rows of createInternalEquation{X,Y,Z},
+   PIC++/fields.cpp:1396-2632, and the wall rows of fields.cpp:690-781): w = mu v on the nodes
+   around the unknowns, then (A v)_c = v_c - w_c - C lap v_c + C d2_c w_c + the mixed second
+   differences of w. precision PIC_PRECISION_FP64: v, y are device double vectors
+   [3][n0*n1*n2]; PIC_PRECISION_FP32: device float vectors, mu rounded to float (needs a context
+   created with PIC_SOLVER_MIXED). Exposes the mixed solver's operator so that it can be checked
+   against the 37-entry rows. */
+int pic_apply_operator_factored(pic_ctx* ctx, int precision, const void* v, void* y);
 
 /* Philox4x64-10, the generator of the injection: out (device, 4 n words) receives the
    outputs for counters (c0, c1 + i, c2, c3), i < n, under key (k0, k1). For tests of the

@@ -84,6 +84,7 @@ SIGNATURES = {
     "pic_step": (ctypes.c_int, [_V, ctypes.POINTER(PicSolveInfo)]),
     "pic_gmres_solve": (ctypes.c_int, [_V, _D, _D, ctypes.POINTER(PicSolveInfo)]),
     "pic_apply_operator": (ctypes.c_int, [_V, _D, _D]),
+    "pic_apply_operator_factored": (ctypes.c_int, [_V, ctypes.c_int, _V, _V]),
     "pic_synchronize": (ctypes.c_int, [_V]),
     "pic_philox": (ctypes.c_int, [_V] + [ctypes.c_ulonglong] * 6 + [ctypes.c_longlong, _V]),
     "pic_upload_particles_device": (ctypes.c_int, [_V, ctypes.POINTER(ctypes.c_longlong), _D, _D, _I]),

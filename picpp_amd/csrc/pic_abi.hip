@@ -1519,6 +1519,31 @@ int pic_apply_operator(pic_ctx* c, const double* v, double* y) {
 	return PIC_OK;
 }
 
+int pic_apply_operator_factored(pic_ctx* c, int precision, const void* v, void* y) {
+	if (!c || !v || !y) return PIC_ERR_ARGUMENT;
+	if (precision != PIC_PRECISION_FP64 && precision != PIC_PRECISION_FP32) return PIC_ERR_ARGUMENT;
+	if (precision == PIC_PRECISION_FP32 && c->mu32h == nullptr) {
+		c->error = "the FP32 factored operator needs a context created with PIC_SOLVER_MIXED";
+		return PIC_ERR_ARGUMENT;
+	}
+	c->fused = 0;
+	try {
+		launch_pack_mu(c->g, c->mu, c->muc, c->stream);
+		launch_pack_mu_halo(c->g, c->mu, c->muh, c->stream);
+		if (precision == PIC_PRECISION_FP64) {
+			launch_apply_factored<double>(c->g, c->muh, static_cast<const double*>(v), c->w64, operator_C(c),
+			                              static_cast<double*>(y), c->stream);
+		} else {
+			launch_to_f32(c->muh, c->mu32h, 9 * halo_count(c->g), c->stream);
+			launch_apply_factored<float>(c->g, c->mu32h, static_cast<const float*>(v), c->w32, operator_C(c),
+			                             static_cast<float*>(y), c->stream);
+		}
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
 int pic_update_b(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	c->fused = 0;

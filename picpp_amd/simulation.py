@@ -268,6 +268,12 @@ class Simulation:
         self._chk(self.lib.pic_apply_operator(self.h, ctypes.cast(v_ptr, _D), ctypes.cast(y_ptr, _D)),
                   "pic_apply_operator")
 
+    def apply_operator_factored(self, v_ptr: int, y_ptr: int, precision: str = "fp64"):
+        """y = A v in the factored form of the mixed-precision solver (device vectors of doubles
+        for "fp64", of floats for "fp32")."""
+        self._chk(self.lib.pic_apply_operator_factored(self.h, PRECISIONS[precision], ctypes.c_void_p(v_ptr),
+                                                       ctypes.c_void_p(y_ptr)), "pic_apply_operator_factored")
+
     def gmres_solve(self, rhs_ptr: int, x_ptr: int) -> Dict[str, float]:
         info = _lib.PicSolveInfo()
         self._chk(self.lib.pic_gmres_solve(self.h, ctypes.cast(rhs_ptr, _D), ctypes.cast(x_ptr, _D), ctypes.byref(info)),
@@ -321,3 +327,31 @@ def _info_dict(info) -> Dict[str, float]:
     return dict(iterations=info.iterations, restarts=info.restarts, estimated_residual=info.estimated_residual,
                 true_residual=info.true_residual, seconds=info.seconds, preconditioned=info.preconditioned,
                 fallbacks=info.fallbacks)
+
+
+def open_synthetic_box(w, torch, dev, **kw):
+    """A context on the current torch stream holding w's synthetic initial state (E = 0, B = B0
+    uniform, particles generated on the device), after the pre-loop sequence."""
+    from . import workloads as wl
+
+    stream = torch.cuda.current_stream(dev)
+    sim = Simulation(w, capacity=w.capacity_needed(), stream=stream.cuda_stream, **kw)
+    N = w.N
+    E = np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3))
+    E[...] = np.array(w.e0)
+    if w.boundary == "wall":  # no tangential E at and behind the wall (PIC++/initialization.cpp:5282-5287)
+        E[:3, :, :, 1:] = 0.0
+    B = np.zeros(N + (3,))
+    B[...] = np.array(w.b0)
+    st = wl.InitialState(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros((0, 3)), np.zeros((0, 3)), E, B)
+    sim.load_state(st)
+    x, p, counts = wl.make_state_on_device(w, torch, dev)
+    torch.cuda.synchronize()
+    cnt = (ctypes.c_longlong * len(counts))(*counts)
+    _D = ctypes.POINTER(ctypes.c_double)
+    rc = sim.lib.pic_upload_particles_device(sim.h, cnt, ctypes.cast(x.data_ptr(), _D), ctypes.cast(p.data_ptr(), _D), None)
+    sim._chk(rc, "pic_upload_particles_device")
+    del x, p
+    torch.cuda.empty_cache()
+    sim.initialize()
+    return sim

@@ -334,3 +334,50 @@ def species_arrays(w: Workload):
             np.array([s.charge for s in w.species], np.float64),
             np.array([w.weight(s) for s in w.species], np.float64),
             np.array([s.ptype for s in w.species], np.int32))
+
+
+def make_state_on_device(w, torch, device):
+    """Seeded synthetic state on the GPU: uniform positions in each cell, Maxwell-Juettner
+    momenta (tabulated inverse CDF, as picpp_amd.workloads does on the host)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(w.seed)
+    n = w.n
+    ncell = n[0] * n[1] * n[2]
+    xs, ps, counts = [], [], []
+    for s in w.species:
+        cnt = ncell * s.ppc
+        cid = torch.arange(ncell, device=device, dtype=torch.int64).repeat_interleave(s.ppc)
+        ci = torch.stack([cid // (n[1] * n[2]), (cid // n[2]) % n[1], cid % n[2]]).to(torch.float64)
+        del cid
+        lo = torch.tensor([v * w.dx for v in n], device=device, dtype=torch.float64)[:, None]
+        x = lo + (ci + torch.rand((3, cnt), generator=gen, device=device, dtype=torch.float64)) * w.dx
+        del ci
+        ut, cdf = _mj_table(s.temperature)
+        ut = torch.tensor(ut, device=device)
+        cdf = torch.tensor(cdf, device=device)
+        r = torch.rand(cnt, generator=gen, device=device, dtype=torch.float64)
+        k = torch.searchsorted(cdf, r).clamp(1, len(cdf) - 1)
+        t = ((r - cdf[k - 1]) / (cdf[k] - cdf[k - 1]).clamp_min(1e-300)).clamp(0, 1)
+        mag = ut[k - 1] + t * (ut[k] - ut[k - 1])
+        del r, k, t
+        cost = 2 * torch.rand(cnt, generator=gen, device=device, dtype=torch.float64) - 1
+        phi = 2 * math.pi * torch.rand(cnt, generator=gen, device=device, dtype=torch.float64)
+        sint = torch.sqrt((1 - cost * cost).clamp_min(0))
+        u = torch.stack([mag * sint * torch.cos(phi), mag * sint * torch.sin(phi), mag * cost])
+        del cost, phi, sint, mag
+        if s.drift_gamma > 1.0:  # boost along the drift axis (volume-transform flipping, as workloads.boost)
+            beta = s.drift_sign * math.sqrt(1.0 - 1.0 / (s.drift_gamma * s.drift_gamma))
+            g = torch.sqrt(1.0 + (u * u).sum(0))
+            ua = u[s.drift_axis]
+            flip = (-beta * ua / g) > torch.rand(cnt, generator=gen, device=device, dtype=torch.float64)
+            ua = torch.where(flip, -ua, ua)
+            u[s.drift_axis] = s.drift_gamma * (ua + beta * g)
+            del g, ua, flip
+        p = u * s.mass
+        del u
+        xs.append(x)
+        ps.append(p)
+        counts.append(cnt)
+    x = torch.cat(xs, 1).contiguous()
+    p = torch.cat(ps, 1).contiguous()
+    return x, p, counts
