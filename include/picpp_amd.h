@@ -259,6 +259,24 @@ int pic_apply_operator_factored(pic_ctx* ctx, int precision, const void* v, void
 int pic_philox(pic_ctx* ctx, unsigned long long c0, unsigned long long c1, unsigned long long c2, unsigned long long c3,
                unsigned long long k0, unsigned long long k1, long long n, unsigned long long* out);
 
+/* updateEnergy (the seven values of pic_energy, PIC++/simulation.cpp:1506-1640) of caller-owned
+   device state, with no context and no allocation of any kind. No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc). cfg supplies the grid (nx, ny, nz, dx, dy, dz,
+   boundary_x) and the species (nspecies, mass, weight); its other fields are ignored. E: ghosted
+   node field [3][(N0+1)(N1+1)(N2+1)] and B: ghosted cell field [3][N0 N1 N2] (device, the layouts
+   of PIC_ARRAY_E / PIC_ARRAY_B); p: device momenta [3][np] (component planes, m gamma v) of the
+   particles grouped by species, counts[s] (host array) of species s; out7 (device, 7 doubles)
+   receives particle, E, B, total energy, momentum x, y, z. Work is queued on `stream` (NULL: the
+   null stream) and the call returns without waiting. The partial sums use a 37 KB device array
+   that the library holds on every device it runs on until the process ends (9 x 512 block sums
+   and 9 totals, the same whatever the sizes, so no size is refused for lack of room); every call
+   overwrites it before reading it. Calls that overlap on one device race on that array: order
+   them on one stream. PIC_ERR_ARGUMENT for a bad argument, PIC_ERR_DEVICE when the device array
+   or a launch fails; pic_last_error(NULL) describes this thread's last failure. */
+int pic_energy_noalloc(const pic_config* cfg, const double* E, const double* B, const double* p, const long long* counts,
+                       double* out7, void* stream);
+
 /* Blocks until the context's stream is idle. */
 int pic_synchronize(pic_ctx* ctx);
 

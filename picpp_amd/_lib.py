@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libpicpp_amd.so")
 
 PIC_OK = 0
 ERRORS = {-1: "argument", -2: "device", -3: "memory", -4: "particle", -5: "solver"}
+ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
 
 PRECISION_FP64, PRECISION_FP32 = 0, 1
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
@@ -86,6 +87,7 @@ SIGNATURES = {
     "pic_apply_operator": (ctypes.c_int, [_V, _D, _D]),
     "pic_apply_operator_factored": (ctypes.c_int, [_V, ctypes.c_int, _V, _V]),
     "pic_synchronize": (ctypes.c_int, [_V]),
+    "pic_energy_noalloc": (ctypes.c_int, [ctypes.POINTER(PicConfig), _V, _V, _V, ctypes.POINTER(ctypes.c_longlong), _V, _V]),
     "pic_philox": (ctypes.c_int, [_V] + [ctypes.c_ulonglong] * 6 + [ctypes.c_longlong, _V]),
     "pic_upload_particles_device": (ctypes.c_int, [_V, ctypes.POINTER(ctypes.c_longlong), _D, _D, _I]),
     "pic_profile_enable": (ctypes.c_int, [_V, ctypes.c_int]),

@@ -710,7 +710,7 @@ int gmres_cycle(pic_ctx* c, double* b, double* x, double target, int max_iterati
 
 extern "C" {
 
-const char* pic_last_error(const pic_ctx* c) { return c ? c->error.c_str() : "null context"; }
+const char* pic_last_error(const pic_ctx* c) { return c ? c->error.c_str() : noalloc_last_error(); }
 
 int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	if (!cfg || !out) return PIC_ERR_ARGUMENT;

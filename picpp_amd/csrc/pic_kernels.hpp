@@ -85,4 +85,7 @@ void launch_rows_final(const double* partial, int n, int rows, double* out, hipS
 
 constexpr int kDotBlocks = 1024;
 
+// the failure text of this thread's last context-free entry point (pic_energy_noalloc)
+const char* noalloc_last_error();
+
 }  // namespace picgpu
