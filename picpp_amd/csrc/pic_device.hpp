@@ -155,38 +155,4 @@ __device__ inline void gather_cell(const Grid& g, const double* __restrict__ f, 
 			}
 }
 
-// Particle::gammaFactor and getVelocity (PIC++/particle.cpp:216-235, :381-390).
-__device__ inline void velocity_gamma(const double p[3], double m, double v[3], double& g) {
-	double p2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
-	g = sqrt(p2 / (m * m) + 1.0);
-	v[0] = p[0] / (m * g);
-	v[1] = p[1] / (m * g);
-	v[2] = p[2] / (m * g);
-}
-
-// evaluateAlphaRotationTensor (PIC++/mover.cpp:851-908): row-major 3x3.
-// alpha_ij = [delta_ij + bs^2 B_i B_j - bs sum_l eps_jil B_l] / (G (1 + bs^2 B^2)),
-// G = beta (E.v) + gamma, bs = beta / G.
-__device__ inline void alpha_tensor(double beta, const double v[3], double gamma, const double E[3], const double B[3],
-                                    double a[9]) {
-	// Levi-Civita eps[j][i][l] as indexed by the reference (PIC++/initialization.cpp:779-793).
-	constexpr double eps[3][3][3] = {{{0, 0, 0}, {0, 0, 1}, {0, -1, 0}},
-	                                 {{0, 0, -1}, {0, 0, 0}, {1, 0, 0}},
-	                                 {{0, 1, 0}, {-1, 0, 0}, {0, 0, 0}}};
-	double G = (beta * (E[0] * v[0] + E[1] * v[1] + E[2] * v[2])) + gamma;
-	double bs = beta / G;
-	double b2 = bs * bs;
-	double den = G * (1 + b2 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]));
-#pragma unroll
-	for (int i = 0; i < 3; ++i)
-#pragma unroll
-		for (int j = 0; j < 3; ++j) {
-			double r = (i == j ? 1.0 : 0.0) + (b2 * B[i] * B[j]);
-#pragma unroll
-			for (int l = 0; l < 3; ++l)
-				if (eps[j][i][l] != 0) r -= (bs * eps[j][i][l] * B[l]);
-			a[i * 3 + j] = r / den;
-		}
-}
-
 }  // namespace picgpu

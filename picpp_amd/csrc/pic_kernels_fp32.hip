@@ -10,10 +10,11 @@
 // Deposits are deterministic without atomics. Particles are sorted by key (species, cell,
 // octant); a group of G lanes serves one cell (node moments) or one dual cell (cell moments,
 // density) and takes its particles round-robin. Each lane sums its particles in that fixed
-// order, the group then reduces with a fixed xor-shuffle tree and writes 8 corner sums.
+// order, the group then reduces through LDS in a fixed lane order and writes 8 corner sums.
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
 #include "pic_kernels_fp32.hpp"
+#include "pic_tiles.hpp"
 
 namespace picgpu {
 
@@ -278,24 +279,9 @@ constexpr int kMaxTile = 16;  // cells per deposit workgroup
 // Deposit tiles: a workgroup serves T cells b = (i, j, k0 .. k0+T-1) of one z row, with a
 // group of G lanes per cell. Lanes of a group take the cell's particles of each species
 // round-robin (lane l: l, l+G, l+2G, ...), so they stay evenly loaded whatever the octant
-// counts. Each lane sums its particles in that fixed order and the group reduces with a
-// fixed xor-shuffle tree: the sums are deterministic.
-struct DualTile {
-	int i, j, k0;
-};
+// counts. Each lane sums its particles in that fixed order and the group reduces through LDS
+// in a fixed lane order (group_sum8): the sums are deterministic.
 
-__device__ inline DualTile dual_tile(const Grid& g, int T) {
-	int nt = (g.N[2] + T - 1) / T;
-	int b = blockIdx.x;
-	DualTile t;
-	t.k0 = (b % nt) * T;
-	b /= nt;
-	t.j = b % g.N[1];
-	t.i = b / g.N[1];
-	return t;
-}
-
-__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 
 // Node moments (J, mu) per cell. The E stencil (the cell's 8 nodes) is shared by all its
@@ -399,57 +385,6 @@ __global__ void __launch_bounds__(256) k_node_moments32(Grid g, SpeciesTable sp,
 		for (int a = 0; a < 8; ++a) v8[a] = acc[a][q];
 		const float sum = group_sum8(v8, red, lt, lane, G);
 		if (cell_ok && lane < 8) out[((long long)q * 8 + lane) * g.ncells + c] = sum;
-	}
-}
-
-// Dual cells: base b collects the segments (species, cell b + a, octant 1 - a), a in {0,1}^3.
-// A dual-cell workgroup serves Tj x Tk bases (i, j0 .. j0+Tj-1, k0 .. k0+Tk-1) of one x plane.
-// The octants of a source cell with one x bit are contiguous (octant = x<<2 | y<<1 | z), and
-// each x half feeds exactly one base plane, so a plane tile reads half cells once; only the
-// tile's j and k edges are shared with its neighbours. Workgroups are renumbered so that each
-// XCD (blocks are dealt to the 8 XCDs in turn) walks a contiguous run of tiles and finds the
-// shared edges in its own L2.
-struct PlaneTile {
-	int i, j0, k0;
-	bool ok;
-};
-
-__device__ inline PlaneTile plane_tile(const Grid& g, int Tj, int Tk) {
-	const int nj = (g.N[1] + Tj - 1) / Tj, nk = (g.N[2] + Tk - 1) / Tk;
-	const long long n = (long long)g.N[0] * nj * nk;
-	const long long per = (n + 7) / 8;
-	long long L = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
-	PlaneTile t;
-	t.ok = L < n;
-	if (!t.ok) L = 0;
-	t.k0 = (int)(L % nk) * Tk;
-	L /= nk;
-	t.j0 = (int)(L % nj) * Tj;
-	t.i = (int)(L / nj);
-	return t;
-}
-
-// The 8 segments of one (dual cell, species): combined index v in [0, p[8]) lies in segment
-// a = max{q : p[q] <= v}; its particle index is v + off[a].
-struct DualSegments {
-	int p[9];
-	int off[8];
-};
-
-__device__ inline void dual_segments(const Grid& g, const int* __restrict__ start, int bi, int bj, int bk, int s, DualSegments& d) {
-	d.p[0] = 0;
-#pragma unroll
-	for (int a = 0; a < 8; ++a) {
-		const int ax = (a >> 2) & 1, ay = (a >> 1) & 1, az = a & 1;
-		const int ci = bi + ax, cj = bj + ay, ck = bk + az;
-		int lo = 0, len = 0;
-		if (ci < g.N[0] && cj < g.N[1] && ck < g.N[2]) {
-			long long key = (((long long)s * g.ncells + cell_id(g, ci, cj, ck)) << 3) | (7 - a);
-			lo = start[key];
-			len = start[key + 1] - lo;
-		}
-		d.p[a + 1] = d.p[a] + len;
-		d.off[a] = lo - d.p[a];
 	}
 }
 

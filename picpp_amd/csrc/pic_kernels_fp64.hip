@@ -13,6 +13,7 @@
 
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
+#include "pic_recip.hpp"
 
 namespace picgpu {
 
@@ -50,63 +51,6 @@ __device__ inline bool maps3(const Grid& g, const double x[3], AxisMap m[3]) {
 }
 
 // ------------------------------------------------------------------ moments at x^n
-// evaluateParticlesRotationTensor + the node loop of updateElectroMagneticParameters
-// (PIC++/mover.cpp:810-841, PIC++/momentum_collecting.cpp:397-480): per cell, the 8
-// corner sums of J (3) and of the dielectric tensor mu (9).
-__global__ void __launch_bounds__(128) k_node_moments(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ start,
-                                                      const double* __restrict__ E, const double* __restrict__ B,
-                                                      double dt, double theta, double* __restrict__ out) {
-	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= g.ncells) return;
-	double acc[8][12];
-#pragma unroll
-	for (int a = 0; a < 8; ++a)
-#pragma unroll
-		for (int q = 0; q < 12; ++q) acc[a][q] = 0.0;
-	for (int s = 0; s < sp.count; ++s) {
-		long long key0 = ((long long)s * g.ncells + c) << 3;
-		int lo = start[key0], hi = start[key0 + 8];
-		double m = sp.mass[s], q = sp.charge[s], w = sp.weight[s];
-		double beta = 0.5 * q * dt / m;
-		double omega = w * theta * dt * dt * 2 * kPi * q * q / m;
-		double pc = q * w;
-		for (int i = lo; i < hi; ++i) {
-			double x[3], p[3];
-			load_particle(P, i, x, p);
-			AxisMap mp[3];
-			maps3(g, x, mp);
-			double Ef[3], Bf[3], v[3], gm, a[9];
-			gather_node(g, E, mp, Ef);
-			gather_cell(g, B, mp, Bf);
-			velocity_gamma(p, m, v, gm);
-			alpha_tensor(beta, v, gm, Ef, Bf, a);
-			double u[3] = {v[0] * gm, v[1] * gm, v[2] * gm};
-			double rv[3];
-#pragma unroll
-			for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
-#pragma unroll
-			for (int ci = 0; ci < 2; ++ci)
-#pragma unroll
-				for (int cj = 0; cj < 2; ++cj)
-#pragma unroll
-					for (int ck = 0; ck < 2; ++ck) {
-						int corner = ci * 4 + cj * 2 + ck;
-						double corr = mp[0].nw[ci] * mp[1].nw[cj] * mp[2].nw[ck] / g.volume;
-						double f = pc * corr;
-#pragma unroll
-						for (int r = 0; r < 3; ++r) acc[corner][r] += rv[r] * f;
-						double o = omega * corr;
-#pragma unroll
-						for (int r = 0; r < 9; ++r) acc[corner][3 + r] = acc[corner][3 + r] - a[r] * o;
-					}
-		}
-	}
-#pragma unroll
-	for (int a = 0; a < 8; ++a)
-#pragma unroll
-		for (int q = 0; q < 12; ++q) out[((long long)q * 8 + a) * g.ncells + c] = acc[a][q];
-}
-
 // Node values from the corner sums of the 8 cells around each node, written to the
 // ghosted node arrays J (3 planes) and mu (9 planes).
 __global__ void k_node_assemble(Grid g, const double* __restrict__ cs, double* __restrict__ J, double* __restrict__ mu) {
@@ -132,75 +76,6 @@ __global__ void k_node_assemble(Grid g, const double* __restrict__ cs, double* _
 	for (int q = 0; q < 3; ++q) J[q * g.nnodes + nid] = s[q];
 #pragma unroll
 	for (int q = 0; q < 9; ++q) mu[q * g.nnodes + nid] = s[3 + q];
-}
-
-// Cell loop of updateElectroMagneticParameters (PIC++/momentum_collecting.cpp:521-584):
-// per dual cell b (cell-map base), the 8 corner sums of rhohat and the pressure tensor
-// contributions. The reference writes the pressure tensor at [curI][j][k] with the
-// stencil indices j, k (momentum_collecting.cpp:565, :577); the 48 partials kept here
-// are indexed (x stencil index, j, k, 6 symmetric components).
-__global__ void __launch_bounds__(128) k_cell_moments(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ start,
-                                                      const double* __restrict__ E, const double* __restrict__ B,
-                                                      double dt, double* __restrict__ rho_out, double* __restrict__ pi_out) {
-	long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (b >= g.ncells) return;
-	int bk = (int)(b % g.N[2]);
-	int bj = (int)((b / g.N[2]) % g.N[1]);
-	int bi = (int)(b / ((long long)g.N[2] * g.N[1]));
-	double ar[8], ap[8][6];
-#pragma unroll
-	for (int t = 0; t < 8; ++t) {
-		ar[t] = 0.0;
-#pragma unroll
-		for (int q = 0; q < 6; ++q) ap[t][q] = 0.0;
-	}
-	for (int s = 0; s < sp.count; ++s) {
-		double m = sp.mass[s], q = sp.charge[s], w = sp.weight[s];
-		double beta = 0.5 * q * dt / m;
-		double pc = q * w;
-		for (int seg = 0; seg < 8; ++seg) {
-			int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
-			int mi = bi + ax, mj = bj + ay, mk = bk + az;
-			if (mi >= g.N[0] || mj >= g.N[1] || mk >= g.N[2]) continue;
-			long long key = (((long long)s * g.ncells + cell_id(g, mi, mj, mk)) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) | (1 - az);
-			int lo = start[key], hi = start[key + 1];
-			for (int i = lo; i < hi; ++i) {
-				double x[3], p[3];
-				load_particle(P, i, x, p);
-				AxisMap mp[3];
-				maps3(g, x, mp);
-				double Ef[3], Bf[3], v[3], gm, a[9];
-				gather_node(g, E, mp, Ef);
-				gather_cell(g, B, mp, Bf);
-				velocity_gamma(p, m, v, gm);
-				alpha_tensor(beta, v, gm, Ef, Bf, a);
-				double u[3] = {v[0] * gm, v[1] * gm, v[2] * gm};
-				double rv[3];
-#pragma unroll
-				for (int r = 0; r < 3; ++r) rv[r] = a[r * 3 + 0] * u[0] + a[r * 3 + 1] * u[1] + a[r * 3 + 2] * u[2];
-				double T[6] = {rv[0] * rv[0], rv[0] * rv[1], rv[0] * rv[2], rv[1] * rv[1], rv[1] * rv[2], rv[2] * rv[2]};
-#pragma unroll
-				for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-					for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-						for (int tk = 0; tk < 2; ++tk) {
-							int t = ti * 4 + tj * 2 + tk;
-							double corr = mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk] / g.volume;
-							double f = pc * corr;
-							ar[t] += f;
-#pragma unroll
-							for (int r = 0; r < 6; ++r) ap[t][r] += f * T[r];
-						}
-			}
-		}
-	}
-#pragma unroll
-	for (int t = 0; t < 8; ++t) {
-		rho_out[(long long)t * g.ncells + b] = ar[t];
-#pragma unroll
-		for (int r = 0; r < 6; ++r) pi_out[((long long)t * 6 + r) * g.ncells + b] = ap[t][r];
-	}
 }
 
 // rhohat per cell from the corner sums of the 8 dual cells around it.
@@ -829,17 +704,6 @@ __global__ void k_update_b(Grid g, const double* __restrict__ Et, const double* 
 }
 
 // ------------------------------------------------------------------ mover
-// Particle::evaluateVelocity with its nonrelativistic branch (particle.cpp:366-379).
-__device__ inline void evaluate_velocity(const double p[3], double m, double v[3]) {
-	double p2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
-	if (p2 < 0.0001 * m * m) {
-		for (int a = 0; a < 3; ++a) v[a] = p[a] / m;
-	} else {
-		double g = sqrt(p2 + m * m) / m;
-		for (int a = 0; a < 3; ++a) v[a] = p[a] / (m * g);
-	}
-}
-
 __device__ inline void cross(const double a[3], const double b[3], double out[3]) {
 	out[0] = a[1] * b[2] - a[2] * b[1];
 	out[1] = -a[0] * b[2] + a[2] * b[0];
@@ -853,9 +717,11 @@ __device__ inline double dnorm(const double a[3], const double b[3]) {
 
 // moveParticle (PIC++/mover.cpp:194-491) with the one-rank periodic wrap of exchangeParticles
 // (mover.cpp:1007-1021), followed by the new sort key. alpha is recomputed from E^n, B^n at x^n
-// exactly as evaluateParticlesRotationTensor computed it. One thread per particle.
+// as evaluateParticlesRotationTensor computed it (to rounding). The maps of the predictor and
+// Picard positions, the velocities and alpha take reciprocals in place of divisions
+// (pic_recip.hpp); the new key keeps the reference's divisions. One thread per particle.
 template <bool WALL>
-__global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start, const double* __restrict__ E,
+__global__ void k_push(Grid g, SpeciesTable sp, Recip R, ParticleArrays P, const int* __restrict__ species_start, const double* __restrict__ E,
                        const double* __restrict__ Etheta, const double* __restrict__ B, double dt, double eta,
                        uint32_t* __restrict__ keys, int* __restrict__ status, int* __restrict__ dead,
                        uint32_t* __restrict__ esc) {
@@ -870,13 +736,13 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 	double x[3], p[3];
 	load_particle(P, i, x, p);
 	AxisMap mp[3];
-	if (!maps3(g, x, mp)) atomicOr(status, 8);
+	if (!maps3_fast(g, R, x, mp)) atomicOr(status, 8);
 	double En[3], Bf[3], v[3], gm, a[9];
 	gather_node(g, E, mp, En);
 	gather_cell(g, B, mp, Bf);
-	velocity_gamma(p, m, v, gm);
+	velocity_gamma_r(p, m, R.im2[s], v, gm);
 	double beta = 0.5 * q * dt / m;
-	alpha_tensor(beta, v, gm, En, Bf, a);
+	alpha_tensor_r(beta, v, gm, En, Bf, a);
 	double Ef[3];
 	gather_node(g, Etheta, mp, Ef);
 	double oldP[3] = {p[0], p[1], p[2]};
@@ -887,7 +753,7 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 	cross(v, Bf, vxb);
 	for (int c = 0; c < 3; ++c) newP[c] += (Ef[c] + vxb[c]) * q * dt;
 	double newV[3];
-	evaluate_velocity(newP, m, newV);
+	evaluate_velocity_r(newP, m, R.im2[s], newV);
 	double mid[3];
 	for (int c = 0; c < 3; ++c) mid[c] = v[c] * (1 - eta) + newV[c] * eta;
 	for (int c = 0; c < 3; ++c) x[c] += mid[c] * etaDt;
@@ -933,7 +799,7 @@ __global__ void k_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 			reflect_at_wall();
 			break;
 		}
-		if (!maps3(g, x, mp)) atomicOr(status, 8);
+		if (!maps3_fast(g, R, x, mp)) atomicOr(status, 8);
 		gather_node(g, Etheta, mp, Ef);
 		gather_cell(g, B, mp, Bf);
 		double mxb[3];
@@ -1031,65 +897,6 @@ __global__ void k_segments(const uint32_t* __restrict__ keys, long long count, l
 	for (long long k = prev + 1; k <= cur; ++k) start[k] = (int)i;
 }
 
-// ------------------------------------------------------------------ post-push density
-// updateDensityParameters (PIC++/momentum_collecting.cpp:760-1020): per dual cell and
-// species, the corner sums of concentration, energy (w gamma m), bulk momentum and charge.
-__global__ void __launch_bounds__(128) k_density(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ start,
-                                                 double* __restrict__ out) {
-	long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (b >= g.ncells) return;
-	int bk = (int)(b % g.N[2]);
-	int bj = (int)((b / g.N[2]) % g.N[1]);
-	int bi = (int)(b / ((long long)g.N[2] * g.N[1]));
-	double rho[8];
-	for (int t = 0; t < 8; ++t) rho[t] = 0.0;
-	for (int s = 0; s < sp.count; ++s) {
-		double m = sp.mass[s], q = sp.charge[s], w = sp.weight[s];
-		double pc = q * w;
-		double conc[8], en[8], bulk[8][3];
-		for (int t = 0; t < 8; ++t) {
-			conc[t] = en[t] = 0.0;
-			bulk[t][0] = bulk[t][1] = bulk[t][2] = 0.0;
-		}
-		for (int seg = 0; seg < 8; ++seg) {
-			int ax = (seg >> 2) & 1, ay = (seg >> 1) & 1, az = seg & 1;
-			int mi = bi + ax, mj = bj + ay, mk = bk + az;
-			if (mi >= g.N[0] || mj >= g.N[1] || mk >= g.N[2]) continue;
-			long long key = (((long long)s * g.ncells + cell_id(g, mi, mj, mk)) << 3) | ((1 - ax) << 2) | ((1 - ay) << 1) | (1 - az);
-			int lo = start[key], hi = start[key + 1];
-			for (int i = lo; i < hi; ++i) {
-				double x[3], p[3];
-				load_particle(P, i, x, p);
-				AxisMap mp[3];
-				maps3(g, x, mp);
-				double v[3], gm;
-				velocity_gamma(p, m, v, gm);
-				double full = gm * m;
-#pragma unroll
-				for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-					for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-						for (int tk = 0; tk < 2; ++tk) {
-							int t = ti * 4 + tj * 2 + tk;
-							double corr = mp[0].cw[ti] * mp[1].cw[tj] * mp[2].cw[tk] / g.volume;
-							for (int c = 0; c < 3; ++c) bulk[t][c] += p[c] * w * corr;
-							rho[t] += pc * corr;
-							conc[t] += w * corr;
-							en[t] += w * full * corr;
-						}
-			}
-		}
-		long long base = (long long)s * 5 * 8;
-		for (int t = 0; t < 8; ++t) {
-			out[((base + 0 * 8 + t)) * g.ncells + b] = conc[t];
-			out[((base + 1 * 8 + t)) * g.ncells + b] = en[t];
-			for (int c = 0; c < 3; ++c) out[((base + (2 + c) * 8 + t)) * g.ncells + b] = bulk[t][c];
-		}
-	}
-	long long base = (long long)sp.count * 5 * 8;
-	for (int t = 0; t < 8; ++t) out[(base + t) * g.ncells + b] = rho[t];
-}
 
 // Cell values of the density quantities: nq planes of 8 corner sums -> nq cell planes.
 __global__ void k_density_assemble(Grid g, const double* __restrict__ cs, int nq, double* __restrict__ out) {
@@ -1244,17 +1051,8 @@ namespace {
 inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 }  // namespace
 
-void launch_node_moments(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* start, const double* E,
-                         const double* B, double dt, double theta, double* scratch, hipStream_t s) {
-	hipLaunchKernelGGL(k_node_moments, dim3(blocks(g.ncells, 128)), dim3(128), 0, s, g, sp, P, start, E, B, dt, theta, scratch);
-}
 void launch_node_assemble(const Grid& g, const double* scratch, double* J, double* mu, hipStream_t s) {
 	hipLaunchKernelGGL(k_node_assemble, dim3(blocks(g.nnodes, 256)), dim3(256), 0, s, g, scratch, J, mu);
-}
-void launch_cell_moments(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* start, const double* E,
-                         const double* B, double dt, double* rho_scratch, double* pi_scratch, hipStream_t s) {
-	hipLaunchKernelGGL(k_cell_moments, dim3(blocks(g.ncells, 128)), dim3(128), 0, s, g, sp, P, start, E, B, dt, rho_scratch,
-	                   pi_scratch);
 }
 void launch_cell_assemble(const Grid& g, const double* rho_scratch, double* rho, hipStream_t s) {
 	hipLaunchKernelGGL(k_cell_assemble, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, rho_scratch, rho);
@@ -1355,7 +1153,8 @@ void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P,
                  const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, int* dead,
                  uint32_t* esc, hipStream_t s) {
 	if (P.count == 0) return;
-	hipLaunchKernelGGL(g.wall ? k_push<true> : k_push<false>, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, E,
+	hipLaunchKernelGGL(g.wall ? k_push<true> : k_push<false>, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, make_recip(g, sp), P,
+	                   species_start, E,
 	                   Etheta, B, dt, eta, keys,
 	                   status, dead, esc);
 }
@@ -1387,10 +1186,6 @@ void launch_count_keys(const uint32_t* keys, long long count, int* counts, hipSt
 void launch_iota(uint32_t* v, long long count, hipStream_t s) {
 	if (count == 0) return;
 	hipLaunchKernelGGL(k_iota, dim3(blocks(count, 256)), dim3(256), 0, s, v, count);
-}
-void launch_density(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* start, double* scratch,
-                    hipStream_t s) {
-	hipLaunchKernelGGL(k_density, dim3(blocks(g.ncells, 128)), dim3(128), 0, s, g, sp, P, start, scratch);
 }
 void launch_density_assemble(const Grid& g, const double* scratch, int nq, double* out, hipStream_t s) {
 	hipLaunchKernelGGL(k_density_assemble, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, scratch, nq, out);
