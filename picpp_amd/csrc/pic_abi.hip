@@ -309,8 +309,8 @@ void rebin_particles(pic_ctx* c, int removed = 0, bool resized = false);
 
 // After a push with the wall boundary (PIC++/simulation.cpp:258-289 order: erase and exchange,
 // injection, preserve-charge replacement): particles that left through the right end are
-// dropped, or with injection on replaced in place; the injected slabs are appended; the FP32
-// rebin (or one full sort) restores the cell order and the species ranges.
+// dropped, or with injection on replaced in place; the injected slabs are appended; the rebin
+// restores the cell order and the species ranges.
 void wall_particles(pic_ctx* c) {
 	hipStream_t s = c->stream;
 	int removed = 0;
@@ -339,7 +339,7 @@ void wall_particles(pic_ctx* c) {
 				at += b.count * ncols;
 				c->next_id += b.count * ncols;
 			}
-			if (c->fp32) launch_mark_movers(c->mask, n, n + ninj, s);  // the new particles arrive in the rebin
+			launch_mark_movers(c->mask, n, n + ninj, s);  // the new particles arrive in the rebin
 			if (ninj > 0) {
 				set_count(c, n + ninj);
 				resized = true;
@@ -356,14 +356,11 @@ void wall_particles(pic_ctx* c) {
 		}
 		c->inj_step++;
 	}
-	if (c->fp32)
-		rebin_particles(c, removed, resized);
-	else
-		sort_particles(c, removed, resized);
+	rebin_particles(c, removed, resized);
 }
 
 // Restores the cell-sorted order after a push that left one mover bit per particle in
-// c->mask (the FP32 push, the explicit push), without a full sort (pic_kernels_fp32.hip, rebin
+// c->mask (the FP64 and FP32 pushes, the explicit push), without a full sort (pic_kernels_fp32.hip, rebin
 // kernels); more than a quarter of the particles changing segment takes the full sort
 // instead. Same result as a stable sort by key within each new segment's stayers and
 // arrivals. With the wall boundary the `removed` particles (key 0xFFFFFFFF, flagged as movers
@@ -857,7 +854,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c->keys, c->keys_sorted, c->perm, c->perm_sorted,
 		                                             (int)std::max<long long>(c->capacity, 1), 0, c->end_bit, c->stream));
 		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->counts, c->start, (int)(c->nkeys + 1), c->stream));
-		if (c->fp32 || c->explicit_scheme) {  // the rebin after a push with mover bits
+		{  // the rebin after a push with mover bits
 			const long long words = c->capacity / 64 + 2;
 			c->mask = dalloc<uint64_t>(words);
 			c->wcount = dalloc<int>(words);
@@ -1573,16 +1570,14 @@ int pic_push(pic_ctx* c) {
 			              c->mask, c->status, c->dead, c->esc, c->stream);
 		} else {
 			Timed t(c, "push", (double)c->P.count * 100 + ((double)c->g.nnodes * 6 + (double)c->g.ncells * 3) * 8);
-			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->status,
+			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->mask, c->status,
 			            c->dead, c->esc, c->stream);
 		}
 		check_status(c);
-		if (c->g.wall)  // removals, replacements and injection, then the full sort
+		if (c->g.wall)  // removals, replacements and injection, then the rebin
 			wall_particles(c);
-		else if (c->fp32)
-			rebin_particles(c);
 		else
-			sort_particles(c);
+			rebin_particles(c);
 		density(c, true);
 	} catch (const DeviceError& e) {
 		return fail(c, e);

@@ -720,13 +720,13 @@ __device__ inline double dnorm(const double a[3], const double b[3]) {
 // as evaluateParticlesRotationTensor computed it (to rounding). The maps of the predictor and
 // Picard positions, the velocities and alpha take reciprocals in place of divisions
 // (pic_recip.hpp); the new key keeps the reference's divisions. One thread per particle.
+// Returns whether the particle changed segment (its sort key) or was removed.
 template <bool WALL>
-__global__ void k_push(Grid g, SpeciesTable sp, Recip R, ParticleArrays P, const int* __restrict__ species_start, const double* __restrict__ E,
-                       const double* __restrict__ Etheta, const double* __restrict__ B, double dt, double eta,
-                       uint32_t* __restrict__ keys, int* __restrict__ status, int* __restrict__ dead,
-                       uint32_t* __restrict__ esc) {
-	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= P.count) return;
+__device__ inline bool push_particle(const Grid& g, const SpeciesTable& sp, const Recip& R, const ParticleArrays& P,
+                                     const int* __restrict__ species_start, const double* __restrict__ E,
+                                     const double* __restrict__ Etheta, const double* __restrict__ B, double dt, double eta,
+                                     uint32_t* __restrict__ keys, int* __restrict__ status, int* __restrict__ dead,
+                                     uint32_t* __restrict__ esc, long long i) {
 	const double xw = g.x0[0] + 2 * g.d[0];  // the wall, xgrid[2]
 	const double rest = (1.0 - eta) * dt;
 	bool reflected = false;
@@ -735,6 +735,7 @@ __global__ void k_push(Grid g, SpeciesTable sp, Recip R, ParticleArrays P, const
 	double m = sp.mass[s], q = sp.charge[s];
 	double x[3], p[3];
 	load_particle(P, i, x, p);
+	const uint32_t key0 = position_key(g, s, x);
 	AxisMap mp[3];
 	if (!maps3_fast(g, R, x, mp)) atomicOr(status, 8);
 	double En[3], Bf[3], v[3], gm, a[9];
@@ -829,7 +830,7 @@ __global__ void k_push(Grid g, SpeciesTable sp, Recip R, ParticleArrays P, const
 			}
 			keys[i] = 0xFFFFFFFFu;
 			esc[atomicAdd(dead, 1)] = (uint32_t)i;  // listed for replacement (y, z left unwrapped)
-			return;
+			return true;
 		}
 	}
 	// one-rank periodic wrap (mover.cpp:435-490 classification, :1007-1021 shift)
@@ -852,7 +853,24 @@ __global__ void k_push(Grid g, SpeciesTable sp, Recip R, ParticleArrays P, const
 		bits[c] = mp[c].cbase - mp[c].nbase + 1;
 		if (bits[c] < 0 || bits[c] > 1) atomicOr(status, 2);
 	}
-	keys[i] = particle_key(g, s, mp);
+	const uint32_t key = particle_key(g, s, mp);
+	keys[i] = key;
+	return key != key0;
+}
+
+template <bool WALL>
+__global__ void __launch_bounds__(256) k_push(Grid g, SpeciesTable sp, Recip R, ParticleArrays P, const int* __restrict__ species_start,
+                                              const double* __restrict__ E, const double* __restrict__ Etheta,
+                                              const double* __restrict__ B, double dt, double eta, uint32_t* __restrict__ keys,
+                                              uint64_t* __restrict__ mask, int* __restrict__ status, int* __restrict__ dead,
+                                              uint32_t* __restrict__ esc) {
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	bool mv = false;
+	if (i < P.count) mv = push_particle<WALL>(g, sp, R, P, species_start, E, Etheta, B, dt, eta, keys, status, dead, esc, i);
+	// one mover bit per particle: bit i & 63 of word i >> 6, written by the wave's first lane
+	// (waves are 64 consecutive particles; lanes past the end contribute zeros)
+	const unsigned long long bits = __ballot(mv);
+	if ((threadIdx.x & 63) == 0 && i < P.count) mask[i >> 6] = bits;
 }
 
 // Sort key of every particle at its current position (updateParticleCorrelationMaps).
@@ -1150,13 +1168,11 @@ void launch_update_b(const Grid& g, const double* Etheta, const double* B, doubl
 	hipLaunchKernelGGL(k_update_b, dim3(blocks(g.ncells, 256)), dim3(256), 0, s, g, Etheta, B, dt, Bnew);
 }
 void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const double* E,
-                 const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, int* status, int* dead,
-                 uint32_t* esc, hipStream_t s) {
+                 const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, uint64_t* mask, int* status,
+                 int* dead, uint32_t* esc, hipStream_t s) {
 	if (P.count == 0) return;
 	hipLaunchKernelGGL(g.wall ? k_push<true> : k_push<false>, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, make_recip(g, sp), P,
-	                   species_start, E,
-	                   Etheta, B, dt, eta, keys,
-	                   status, dead, esc);
+	                   species_start, E, Etheta, B, dt, eta, keys, mask, status, dead, esc);
 }
 void launch_wall_mirror_cells(const Grid& g, double* f, int ncomp, const int* flip, hipStream_t s) {
 	long long n = (long long)g.N[1] * g.N[2] * ncomp;

@@ -142,3 +142,4 @@ def test_true_residual_stop_rule():
     info = sim.evaluateElectricField()
     assert 0 <= info["true_residual"] <= 1e-8
     sim.close()
+
