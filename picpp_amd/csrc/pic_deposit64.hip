@@ -46,29 +46,6 @@ __device__ inline double group_sum8_64(const double v[8], double* red, int lt, i
 	return s;
 }
 
-// A box of a field's values staged in LDS: extent e[3] from the origin o[3] (ghosted indices),
-// 3 components per point.
-struct Box {
-	int o[3], e[3];
-	__device__ int at(int i, int j, int k) const { return ((i - o[0]) * e[1] + (j - o[1])) * e[2] + (k - o[2]); }
-	__device__ bool holds(int i, int j, int k) const {
-		return i >= o[0] && i + 1 < o[0] + e[0] && j >= o[1] && j + 1 < o[1] + e[1] && k >= o[2] && k + 1 < o[2] + e[2];
-	}
-};
-
-// Stages the box (indices clamped into [0, lim]) of a component-plane field into LDS.
-__device__ inline void stage_box(const Box& b, const double* __restrict__ f, long long plane, int lim0, int lim1, int lim2,
-                                 bool nodes, const Grid& g, double* lds) {
-	const int n = b.e[0] * b.e[1] * b.e[2];
-	for (int q = threadIdx.x; q < n; q += blockDim.x) {
-		const int dk = q % b.e[2], dj = (q / b.e[2]) % b.e[1], di = q / (b.e[2] * b.e[1]);
-		const int i = clampi(b.o[0] + di, 0, lim0), j = clampi(b.o[1] + dj, 0, lim1), k = clampi(b.o[2] + dk, 0, lim2);
-		const long long id = nodes ? node_id(g, i, j, k) : cell_id(g, i, j, k);
-		lds[3 * q + 0] = f[id];
-		lds[3 * q + 1] = f[plane + id];
-		lds[3 * q + 2] = f[2 * plane + id];
-	}
-}
 
 // The segment of combined index v among a dual cell's 8 segments and its particle index. The
 // segment bounds p[] are nondecreasing, so the sums of the comparisons telescope to segment a

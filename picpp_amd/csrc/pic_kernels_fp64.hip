@@ -720,13 +720,43 @@ __device__ inline double dnorm(const double a[3], const double b[3]) {
 // as evaluateParticlesRotationTensor computed it (to rounding). The maps of the predictor and
 // Picard positions, the velocities and alpha take reciprocals in place of divisions
 // (pic_recip.hpp); the new key keeps the reference's divisions. One thread per particle.
-// Returns whether the particle changed segment (its sort key) or was removed.
+// 8-point gathers of the mover (correlationGeneralEfield / correlationBfield,
+// PIC++/correlation.cpp:23-58, :103-164) with fused multiply-adds
+__device__ inline void gather_node_f(const Grid& g, const double* __restrict__ f, const AxisMap* m, double out[3]) {
+	out[0] = out[1] = out[2] = 0.0;
+#pragma unroll
+	for (int t = 0; t < 8; ++t) {
+		const int i = (t >> 2) & 1, j = (t >> 1) & 1, k = t & 1;
+		const long long id = node_id(g, m[0].nbase + i, m[1].nbase + j, m[2].nbase + k);
+		const double c = m[0].nw[i] * m[1].nw[j] * m[2].nw[k];
+		out[0] = fma(f[id], c, out[0]);
+		out[1] = fma(f[g.nnodes + id], c, out[1]);
+		out[2] = fma(f[2 * g.nnodes + id], c, out[2]);
+	}
+}
+
+__device__ inline void gather_cell_f(const Grid& g, const double* __restrict__ f, const AxisMap* m, double out[3]) {
+	out[0] = out[1] = out[2] = 0.0;
+#pragma unroll
+	for (int t = 0; t < 8; ++t) {
+		const int i = (t >> 2) & 1, j = (t >> 1) & 1, k = t & 1;
+		const long long id = cell_id(g, m[0].cbase + i, m[1].cbase + j, m[2].cbase + k);
+		const double c = m[0].cw[i] * m[1].cw[j] * m[2].cw[k];
+		out[0] = fma(f[id], c, out[0]);
+		out[1] = fma(f[g.ncells + id], c, out[1]);
+		out[2] = fma(f[2 * g.ncells + id], c, out[2]);
+	}
+}
+
+// Returns whether the particle changed segment (its sort key) or was removed. The arithmetic is
+// contracted to fused multiply-adds (within an ulp of the reference's separate roundings).
 template <bool WALL>
 __device__ inline bool push_particle(const Grid& g, const SpeciesTable& sp, const Recip& R, const ParticleArrays& P,
                                      const int* __restrict__ species_start, const double* __restrict__ E,
                                      const double* __restrict__ Etheta, const double* __restrict__ B, double dt, double eta,
                                      uint32_t* __restrict__ keys, int* __restrict__ status, int* __restrict__ dead,
                                      uint32_t* __restrict__ esc, long long i) {
+#pragma clang fp contract(fast)
 	const double xw = g.x0[0] + 2 * g.d[0];  // the wall, xgrid[2]
 	const double rest = (1.0 - eta) * dt;
 	bool reflected = false;
@@ -739,13 +769,13 @@ __device__ inline bool push_particle(const Grid& g, const SpeciesTable& sp, cons
 	AxisMap mp[3];
 	if (!maps3_fast(g, R, x, mp)) atomicOr(status, 8);
 	double En[3], Bf[3], v[3], gm, a[9];
-	gather_node(g, E, mp, En);
-	gather_cell(g, B, mp, Bf);
+	gather_node_f(g, E, mp, En);
+	gather_cell_f(g, B, mp, Bf);
 	velocity_gamma_r(p, m, R.im2[s], v, gm);
 	double beta = 0.5 * q * dt / m;
 	alpha_tensor_r(beta, v, gm, En, Bf, a);
 	double Ef[3];
-	gather_node(g, Etheta, mp, Ef);
+	gather_node_f(g, Etheta, mp, Ef);
 	double oldP[3] = {p[0], p[1], p[2]};
 	double newP[3] = {p[0], p[1], p[2]};
 	double oldX[3] = {x[0], x[1], x[2]};
@@ -801,8 +831,8 @@ __device__ inline bool push_particle(const Grid& g, const SpeciesTable& sp, cons
 			break;
 		}
 		if (!maps3_fast(g, R, x, mp)) atomicOr(status, 8);
-		gather_node(g, Etheta, mp, Ef);
-		gather_cell(g, B, mp, Bf);
+		gather_node_f(g, Etheta, mp, Ef);
+		gather_cell_f(g, B, mp, Bf);
 		double mxb[3];
 		cross(mid, Bf, mxb);
 		for (int c = 0; c < 3; ++c) newP[c] = oldP[c] + (Ef[c] + mxb[c]) * (q * dt);
