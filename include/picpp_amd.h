@@ -136,6 +136,13 @@ typedef struct {
 	int smoothing_count;
 	double smoothing_parameter;
 	int scheme;              /* PIC_SCHEME_* */
+	/* 1: pic_step runs pic_cleanup_divergence after the push, at the reference's call position
+	   (after updateDensityParameters and the newE exchange, before updateFields; the reference's
+	   call, PIC++/simulation.cpp:117, :348, runs every divergenceCleanUpParameter = 1 steps and is
+	   commented out in its loop). Implicit scheme, periodic boundary. */
+	int divergence_cleanup;
+	/* > 0: pic_step then runs pic_filter_fields(ctx, filter_cut_wave_number) on newE and newB */
+	int filter_cut_wave_number;
 } pic_config;
 
 typedef struct {
@@ -252,6 +259,25 @@ int pic_apply_operator(pic_ctx* ctx, const double* v, double* y);
    created with PIC_SOLVER_MIXED). Exposes the mixed solver's operator so that it can be checked
    against the 37-entry rows. */
 int pic_apply_operator_factored(pic_ctx* ctx, int precision, const void* v, void* y);
+
+/* Simulation::cleanupDivergence(newEfield, chargeDensity) (PIC++/divergencecleanup.cpp:20-392, its
+   GMRES branch): solves L phi = div newE - 4 pi rho on the owned cells, L the 27-point operator of
+   createDivergenceCleanupInternalEquation (:620-813), with the reference's GMRES (stop rule of
+   PIC_STOP_REFERENCE with one component, at most maxDivergenceCleanupIterations = 100,
+   PIC++/constants.h:56), then newE -= grad phi (evaluateDivergenceCleaningField,
+   updateFieldByCleaning, :531-618) and the newE ghost exchange. rho is the density array's charge
+   plane (the last pic_push's updateDensityParameters). After it div newE = 4 pi rho to the solver's
+   tolerance. Implicit scheme, periodic boundary (PIC_ERR_ARGUMENT otherwise). info (may be NULL)
+   receives the GMRES iterations and the Givens estimate. */
+int pic_cleanup_divergence(pic_ctx* ctx, pic_solve_info* info);
+/* Simulation::filterFields(cutWaveNumber), periodic branch (PIC++/fields_filtering.cpp:21-295):
+   every component of newE and newB over the owned points is transformed with the separable
+   discrete Fourier transform of fourierTranslation (PIC++/fourier.cpp:13-192), the modes with
+   |k| > 2 pi / (cut_wave_number dx) are multiplied by -1 (the reference's filterFieldGeneral; k from
+   the unfolded mode indices 0 .. n-1, and the y component's k_y from the z mode index, as there),
+   transformed back and written to the owned points; then the newE and newB ghost exchanges.
+   Implicit scheme, periodic boundary, cut_wave_number > 0. */
+int pic_filter_fields(pic_ctx* ctx, int cut_wave_number);
 
 /* Philox4x64-10, the generator of the injection: out (device, 4 n words) receives the
    outputs for counters (c0, c1 + i, c2, c3), i < n, under key (k0, k1). For tests of the

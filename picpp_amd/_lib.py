@@ -45,7 +45,7 @@ class PicConfig(ctypes.Structure):
                 ("b_right", ctypes.c_double * 3), ("inject", ctypes.c_int), ("flow", ctypes.c_double * 3),
                 ("ppc", ctypes.c_int * 8), ("temperature", ctypes.c_double * 8), ("seed", ctypes.c_ulonglong),
                 ("pressure", ctypes.c_int), ("smoothing_count", ctypes.c_int), ("smoothing_parameter", ctypes.c_double),
-                ("scheme", ctypes.c_int)]
+                ("scheme", ctypes.c_int), ("divergence_cleanup", ctypes.c_int), ("filter_cut_wave_number", ctypes.c_int)]
 
 
 class PicSolveInfo(ctypes.Structure):
@@ -87,6 +87,8 @@ SIGNATURES = {
     "pic_apply_operator": (ctypes.c_int, [_V, _D, _D]),
     "pic_apply_operator_factored": (ctypes.c_int, [_V, ctypes.c_int, _V, _V]),
     "pic_synchronize": (ctypes.c_int, [_V]),
+    "pic_cleanup_divergence": (ctypes.c_int, [_V, ctypes.POINTER(PicSolveInfo)]),
+    "pic_filter_fields": (ctypes.c_int, [_V, ctypes.c_int]),
     "pic_energy_noalloc": (ctypes.c_int, [ctypes.POINTER(PicConfig), _V, _V, _V, ctypes.POINTER(ctypes.c_longlong), _V, _V]),
     "pic_philox": (ctypes.c_int, [_V] + [ctypes.c_ulonglong] * 6 + [ctypes.c_longlong, _V]),
     "pic_upload_particles_device": (ctypes.c_int, [_V, ctypes.POINTER(ctypes.c_longlong), _D, _D, _I]),

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <random>
@@ -21,6 +22,7 @@
 #include "pic_inject.hpp"
 #include "pic_precond.hpp"
 #include "pic_solver.hpp"
+#include "pic_cleanup.hpp"
 
 using namespace picgpu;
 
@@ -111,6 +113,12 @@ struct pic_ctx {
 	long long next_id = 0;
 	// smoothing scratch: 9 node planes
 	double* smooth_tmp = nullptr;
+	// divergence cleanup (allocated by its first call): the GMRES basis and vectors over the owned
+	// cells; Fourier filtering: two complex owned-size buffers and the twiddles of each axis
+	std::vector<double*> Vcl;
+	double** Vcl_dev = nullptr;
+	double *cl_rhs = nullptr, *cl_phi = nullptr;
+	double *ft_re = nullptr, *ft_im = nullptr, *ft_re2 = nullptr, *ft_im2 = nullptr, *ft_tw[3] = {nullptr, nullptr, nullptr};
 	// Krylov solver
 	double* muh = nullptr;    // mu on the halo layout of the operators
 	float* mu32h = nullptr;
@@ -625,11 +633,26 @@ struct Givens {
 	}
 };
 
+// The operator and Krylov storage of one GMRES: op(v, w) sets w = A v on vectors of length len;
+// V holds up to max_basis + 1 basis vectors (Vdev: their device addresses).
+struct KrylovSystem {
+	long long len;
+	std::function<void(const double*, double*)> op;
+	const char* op_name;
+	double op_bytes;  // algorithmic bytes of one application
+	std::vector<double*>* V;
+	double** Vdev;
+	int max_basis;
+	int ortho;
+};
+
 // One unrestarted GMRES cycle on A x = b (x0 = 0), as generalizedMinimalResidualMethod:
 // b is normalised in place and restored. Returns Arnoldi steps; rel = Givens estimate.
-int gmres_cycle(pic_ctx* c, double* b, double* x, double target, int max_iterations, int dim, double& rel) {
+int gmres_cycle_on(pic_ctx* c, const KrylovSystem& K, double* b, double* x, double target, int max_iterations, int dim,
+                   double& rel) {
 	hipStream_t s = c->stream;
-	long long len = c->nown3();
+	const long long len = K.len;
+	std::vector<double*>& V = *K.V;
 	double* d_norm = c->dscal + 0;
 	double* d_normr = c->dscal + 1;
 	double* d_h = c->dscal + 8;
@@ -642,46 +665,45 @@ int gmres_cycle(pic_ctx* c, double* b, double* x, double target, int max_iterati
 		HIP_CHECK(hipMemsetAsync(x, 0, len * sizeof(double), s));
 		return 0;
 	}
-	launch_div_copy(b, c->V[0], d_norm, len, s);
+	launch_div_copy(b, V[0], d_norm, len, s);
 	launch_dot(b, b, len, c->partial, d_normr, true, s);
 	double normr = 0;
 	HIP_CHECK(hipMemcpyAsync(&normr, d_normr, sizeof(double), hipMemcpyDeviceToHost, s));
 	HIP_CHECK(hipStreamSynchronize(s));
-	const double C = operator_C(c);
 	Givens gv;
 	int nn = 2;
 	rel = 1.0;
-	int limit = std::min(std::min(max_iterations, dim + 3), c->max_basis + 1);
+	int limit = std::min(std::min(max_iterations, dim + 3), K.max_basis + 1);
 	std::vector<double> hcol;
 	while (rel > target && nn < limit) {
 		int j = nn - 2;
-		double* w = c->V[j + 1];
+		double* w = V[j + 1];
 		{
-			Timed t(c, "apply_operator", (double)c->g.nown * 15 * 8);
-			launch_apply_operator(c->g, c->muh, c->V[j], C, w, s);
+			Timed t(c, K.op_name, K.op_bytes);
+			K.op(V[j], w);
 		}
 		{
 			// algorithmic bytes: MGS reads v_m and w twice and writes w once per basis vector
-			Timed tg(c, "orthogonalize", (double)len * 8 * (c->cfg.ortho == PIC_ORTHO_MGS ? 5.0 * (j + 1) + 3 : 6.0 * (j + 1) + 3));
-			if (c->cfg.ortho == PIC_ORTHO_MGS) {
+			Timed tg(c, "orthogonalize", (double)len * 8 * (K.ortho == PIC_ORTHO_MGS ? 5.0 * (j + 1) + 3 : 6.0 * (j + 1) + 3));
+			if (K.ortho == PIC_ORTHO_MGS) {
 				for (int m = 0; m <= j; ++m) {
-					launch_dot(c->V[m], w, len, c->partial, d_h + m, false, s);
-					launch_axpy_dev(w, c->V[m], d_h + m, len, s);
+					launch_dot(V[m], w, len, c->partial, d_h + m, false, s);
+					launch_axpy_dev(w, V[m], d_h + m, len, s);
 				}
 			} else {
 				// CGS2: two passes of h = V^T w, w -= V h (same span; reorthogonalised)
 				for (int pass = 0; pass < 2; ++pass) {
 					double* hp = d_h + (pass == 0 ? 0 : 64);
 					for (int m = 0; m <= j; ++m)
-						launch_dot(c->V[m], w, len, c->partial + (m + 1) * kDotBlocks, hp + m, false, s);
-					for (int m = 0; m <= j; ++m) launch_axpy_dev(w, c->V[m], hp + m, len, s);
+						launch_dot(V[m], w, len, c->partial + (m + 1) * kDotBlocks, hp + m, false, s);
+					for (int m = 0; m <= j; ++m) launch_axpy_dev(w, V[m], hp + m, len, s);
 				}
 			}
 			launch_dot(w, w, len, c->partial, d_h + j + 1, true, s);
 			launch_normalize_dev(w, d_h + j + 1, len, s);
 		}
 		hcol.assign(nn, 0.0);
-		if (c->cfg.ortho == PIC_ORTHO_MGS) {
+		if (K.ortho == PIC_ORTHO_MGS) {
 			HIP_CHECK(hipMemcpyAsync(hcol.data(), d_h, nn * sizeof(double), hipMemcpyDeviceToHost, s));
 			HIP_CHECK(hipStreamSynchronize(s));
 		} else {
@@ -699,8 +721,16 @@ int gmres_cycle(pic_ctx* c, double* b, double* x, double target, int max_iterati
 	int steps = nn - 2;
 	HIP_CHECK(hipMemcpyAsync(c->ydev, gv.y.data(), gv.y.size() * sizeof(double), hipMemcpyHostToDevice, s));
 	Timed t(c, "combine", (double)len * 8 * (gv.y.size() + 1));
-	launch_combine(x, c->Vdev, c->ydev, (int)gv.y.size(), d_norm, len, s);
+	launch_combine(x, K.Vdev, c->ydev, (int)gv.y.size(), d_norm, len, s);
 	return steps;
+}
+
+// GMRES on the Maxwell operator (the 37-entry rows) with the context's basis
+int gmres_cycle(pic_ctx* c, double* b, double* x, double target, int max_iterations, int dim, double& rel) {
+	const double C = operator_C(c);
+	KrylovSystem K{c->nown3(), [c, C](const double* v, double* w) { launch_apply_operator(c->g, c->muh, v, C, w, c->stream); },
+	               "apply_operator", (double)c->g.nown * 15 * 8, &c->V, c->Vdev, c->max_basis, c->cfg.ortho};
+	return gmres_cycle_on(c, K, b, x, target, max_iterations, dim, rel);
 }
 
 }  // namespace
@@ -721,6 +751,10 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		return PIC_ERR_ARGUMENT;
 	if (cfg->boundary_x != PIC_BOUNDARY_PERIODIC && cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
 	if (cfg->scheme != PIC_SCHEME_IMPLICIT && cfg->scheme != PIC_SCHEME_EXPLICIT) return PIC_ERR_ARGUMENT;
+	if ((cfg->divergence_cleanup || cfg->filter_cut_wave_number != 0) &&  // built for the periodic implicit step
+	    (cfg->scheme != PIC_SCHEME_IMPLICIT || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->filter_cut_wave_number < 0 ||
+	     (cfg->divergence_cleanup != 0 && cfg->divergence_cleanup != 1)))
+		return PIC_ERR_ARGUMENT;
 	if (cfg->scheme == PIC_SCHEME_EXPLICIT &&  // the explicit step is built FP64, periodic, without smoothing or injection
 	    (cfg->precision != PIC_PRECISION_FP64 || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->smoothing_count != 0 || cfg->inject))
 		return PIC_ERR_ARGUMENT;
@@ -994,6 +1028,10 @@ void pic_destroy(pic_ctx* c) {
 		f(p);
 	for (float* v : c->Z32) f(v);
 	for (float* v : c->log_mu) f(v);
+	for (double* v : c->Vcl) f(v);
+	for (void* p : {(void*)c->Vcl_dev, (void*)c->cl_rhs, (void*)c->cl_phi, (void*)c->ft_re, (void*)c->ft_im, (void*)c->ft_re2,
+	                (void*)c->ft_im2, (void*)c->ft_tw[0], (void*)c->ft_tw[1], (void*)c->ft_tw[2]})
+		f(p);
 	c->net.reset();
 	for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
 	if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
@@ -1541,6 +1579,99 @@ int pic_apply_operator_factored(pic_ctx* c, int precision, const void* v, void* 
 	return PIC_OK;
 }
 
+// maxDivergenceCleanupIterations (PIC++/constants.h:56)
+constexpr int kCleanupMaxIterations = 100;
+
+int pic_cleanup_divergence(pic_ctx* c, pic_solve_info* info) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	if (c->explicit_scheme || c->g.wall) {
+		c->error = "divergence cleanup is built for the implicit scheme on the periodic box";
+		return PIC_ERR_ARGUMENT;
+	}
+	c->fused = 0;
+	try {
+		const Grid& g = c->g;
+		hipStream_t s = c->stream;
+		const long long len = (long long)g.n[0] * g.n[1] * g.n[2];
+		const int max_basis = kCleanupMaxIterations - 2;  // n < maxIteration Arnoldi steps (specialmath.cpp:481)
+		if (c->Vcl.empty()) {
+			c->Vcl.resize(max_basis + 1);
+			for (auto& v : c->Vcl) v = dalloc<double>(len);
+			c->Vcl_dev = dalloc<double*>(c->Vcl.size());
+			HIP_CHECK(hipMemcpy(c->Vcl_dev, c->Vcl.data(), c->Vcl.size() * sizeof(double*), hipMemcpyHostToDevice));
+			c->cl_rhs = dalloc<double>(len);
+			c->cl_phi = dalloc<double>(len);
+		}
+		{
+			Timed t(c, "cleanup_rhs", (double)g.nnodes * 24 + (double)g.ncells * 8 + (double)len * 8);
+			launch_cleanup_rhs(g, c->Enew, c->dens + (long long)5 * c->sp.count * g.ncells, c->cl_rhs, s);
+		}
+		const CleanupStencil S = cleanup_stencil(g);
+		KrylovSystem K{len, [c, S](const double* v, double* w) { launch_cleanup_apply(c->g, S, v, w, c->stream); }, "cleanup_apply",
+		               (double)len * 16, &c->Vcl, c->Vcl_dev, max_basis, PIC_ORTHO_MGS};
+		const int dim = (int)len;  // one component (generalizedMinimalResidualMethod with lnumber = 1)
+		const double target = std::max(c->cfg.gmres_tolerance / dim, 1e-15);
+		double rel = 0;
+		pic_solve_info inf{0, 0, 0, -1, 0, 0, 0};
+		HIP_CHECK(hipStreamSynchronize(s));
+		auto t0 = std::chrono::steady_clock::now();
+		inf.iterations = gmres_cycle_on(c, K, c->cl_rhs, c->cl_phi, target, kCleanupMaxIterations, dim, rel);
+		inf.estimated_residual = rel;
+		{
+			Timed t(c, "cleanup_correct", (double)len * 8 + (double)g.nnodes * 48);
+			launch_cleanup_correct(g, c->cl_phi, c->Enew, s);
+			launch_ghost_fill(g, c->Enew, 3, true, s);  // exchangeGeneralEfield
+		}
+		HIP_CHECK(hipStreamSynchronize(s));
+		inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+		if (info) *info = inf;
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_filter_fields(pic_ctx* c, int cut_wave_number) {
+	if (!c || cut_wave_number <= 0) return PIC_ERR_ARGUMENT;
+	if (c->explicit_scheme || c->g.wall) {
+		c->error = "Fourier filtering is built for the implicit scheme on the periodic box";
+		return PIC_ERR_ARGUMENT;
+	}
+	c->fused = 0;
+	try {
+		const Grid& g = c->g;
+		hipStream_t s = c->stream;
+		const long long len = (long long)g.n[0] * g.n[1] * g.n[2];
+		if (!c->ft_re) {
+			for (double** p : {&c->ft_re, &c->ft_im, &c->ft_re2, &c->ft_im2}) *p = dalloc<double>(len);
+			for (int a = 0; a < 3; ++a) {
+				// (cos, sin)(2 pi q / n), q < n, from the exact remainder q (the reference's recurrence
+				// of complexExp products gives the same factors to rounding)
+				std::vector<double> tw(2 * g.n[a]);
+				for (int q = 0; q < g.n[a]; ++q) {
+					const double ang = 2.0 * 3.141592653589793115997963468544185161590576171875 * q / g.n[a];
+					tw[2 * q] = std::cos(ang);
+					tw[2 * q + 1] = std::sin(ang);
+				}
+				c->ft_tw[a] = dalloc<double>(tw.size());
+				HIP_CHECK(hipMemcpy(c->ft_tw[a], tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
+			}
+		}
+		const double kcut = 2 * 3.141592653589793115997963468544185161590576171875 / (cut_wave_number * g.d[0]);
+		const double* tw[3] = {c->ft_tw[0], c->ft_tw[1], c->ft_tw[2]};
+		Timed t(c, "filter_fields", (double)len * 8 * 2 * 6 * (g.n[0] + g.n[1] + g.n[2]));
+		for (int comp = 0; comp < 3; ++comp)
+			launch_filter_component(g, c->Enew, comp * g.nnodes, true, comp, kcut, tw, c->ft_re, c->ft_im, c->ft_re2, c->ft_im2, s);
+		for (int comp = 0; comp < 3; ++comp)
+			launch_filter_component(g, c->Bnew, comp * g.ncells, false, comp, kcut, tw, c->ft_re, c->ft_im, c->ft_re2, c->ft_im2, s);
+		launch_ghost_fill(g, c->Enew, 3, true, s);   // exchangeGeneralEfield(newEfield)
+		launch_ghost_fill(g, c->Bnew, 3, false, s);  // exchangeGeneralBfield(newBfield)
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
 int pic_update_b(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
@@ -1714,6 +1845,10 @@ int pic_step(pic_ctx* c, pic_solve_info* info) {
 	if ((rc = pic_maxwell_solve(c, info))) return rc;
 	if ((rc = pic_update_b(c))) return rc;
 	if ((rc = pic_push(c))) return rc;
+	// the reference's call position of cleanupDivergence (PIC++/simulation.cpp:348): after the
+	// push's updateDensityParameters, before updateFields
+	if (c->cfg.divergence_cleanup && (rc = pic_cleanup_divergence(c, nullptr))) return rc;
+	if (c->cfg.filter_cut_wave_number > 0 && (rc = pic_filter_fields(c, c->cfg.filter_cut_wave_number))) return rc;
 	return pic_commit_fields(c);
 }
 

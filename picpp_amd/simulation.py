@@ -61,6 +61,8 @@ class Simulation:
         cfg.smoothing_count = workload.smoothing_count
         cfg.smoothing_parameter = workload.smoothing_parameter
         cfg.scheme = {"implicit": _lib.SCHEME_IMPLICIT, "explicit": _lib.SCHEME_EXPLICIT}[scheme]
+        cfg.divergence_cleanup = 1 if workload.divergence_cleanup else 0
+        cfg.filter_cut_wave_number = workload.filter_cut_wave_number
         for a in range(3):
             cfg.e_right[a] = workload.e_right[a]
             cfg.b_right[a] = workload.b_right[a]
@@ -251,6 +253,16 @@ class Simulation:
                 setattr(bi, key, val)
         paths = [os.fsencode(p) for p in (general_file, e_file, b_file, particles_file)]
         self._chk(self.lib.pic_write_backup(self.h, ctypes.byref(bi), *paths), "pic_write_backup")
+
+    def cleanupDivergence(self) -> Dict[str, float]:
+        """cleanupDivergence(newEfield, chargeDensity) (PIC++/divergencecleanup.cpp:20-392)."""
+        info = _lib.PicSolveInfo()
+        self._chk(self.lib.pic_cleanup_divergence(self.h, ctypes.byref(info)), "pic_cleanup_divergence")
+        return _info_dict(info)
+
+    def filterFields(self, cut_wave_number: int):
+        """filterFields(cutWaveNumber) on newE and newB (PIC++/fields_filtering.cpp:21-295)."""
+        self._chk(self.lib.pic_filter_fields(self.h, int(cut_wave_number)), "pic_filter_fields")
 
     def synchronize(self):
         self._chk(self.lib.pic_synchronize(self.h), "pic_synchronize")
