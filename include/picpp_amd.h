@@ -136,10 +136,11 @@ typedef struct {
 	int smoothing_count;
 	double smoothing_parameter;
 	int scheme;              /* PIC_SCHEME_* */
-	/* 1: pic_step runs pic_cleanup_divergence after the push, at the reference's call position
+	/* bit 1: pic_step runs pic_cleanup_divergence after the push, at the reference's call position
 	   (after updateDensityParameters and the newE exchange, before updateFields; the reference's
-	   call, PIC++/simulation.cpp:117, :348, runs every divergenceCleanUpParameter = 1 steps and is
-	   commented out in its loop). Implicit scheme, periodic boundary. */
+	   calls, PIC++/simulation.cpp:117-118, :348-349, run every divergenceCleanUpParameter = 1 steps
+	   and are commented out in its loop); bit 2: then pic_cleanup_divergence_magnetic. Implicit
+	   scheme, periodic boundary. */
 	int divergence_cleanup;
 	/* > 0: pic_step then runs pic_filter_fields(ctx, filter_cut_wave_number) on newE and newB */
 	int filter_cut_wave_number;
@@ -270,6 +271,14 @@ int pic_apply_operator_factored(pic_ctx* ctx, int precision, const void* v, void
    tolerance. Implicit scheme, periodic boundary (PIC_ERR_ARGUMENT otherwise). info (may be NULL)
    receives the GMRES iterations and the Givens estimate. */
 int pic_cleanup_divergence(pic_ctx* ctx, pic_solve_info* info);
+/* Simulation::cleanupDivergenceMagnetic (PIC++/divergencecleanup.cpp:939-1306): the same 27-point
+   equation for a node potential with div newB at the nodes (evaluateDivBgeneral, fields.cpp:3019-3030)
+   as right part, then newB -= grad phi at the cells and the newB ghost exchange, so that div newB = 0
+   to the solver's tolerance. The reference solves it with BiCGStab to maxCleanupErrorLevel = 1e-8
+   (PIC++/specialmath.cpp:1714), which reaches rounding level on these systems; this build uses the
+   GMRES of pic_cleanup_divergence run to its 1e-15 floor (at most 98 Arnoldi steps). B from pic_update_b is solenoidal to rounding (the
+   cell curl of node E), so on the implicit step this corrects rounding only. */
+int pic_cleanup_divergence_magnetic(pic_ctx* ctx, pic_solve_info* info);
 /* Simulation::filterFields(cutWaveNumber), periodic branch (PIC++/fields_filtering.cpp:21-295):
    every component of newE and newB over the owned points is transformed with the separable
    discrete Fourier transform of fourierTranslation (PIC++/fourier.cpp:13-192), the modes with

@@ -11,6 +11,9 @@ TEST INFRASTRUCTURE ONLY: used by tests/ as the checker; the product path never 
   relative tolerance; this restatement solves the periodic system exactly in Fourier space (the
   operator is circulant: its eigenvalues are its stencil's symbol), with the constant mode of the
   potential set to zero. The two agree to the reference's solver tolerance.
+- cleanup_divergence_magnetic follows Simulation::cleanupDivergenceMagnetic (:939-1306): the same
+  operator for a node potential with div B at the nodes as right part, and the cell gradient
+  subtracted from B; solved exactly here, with BiCGStab in the reference.
 - filter_fields follows Simulation::filterFields / filterFieldGeneral, periodic branch
   (PIC++/fields_filtering.cpp:21-295) with fourierTranslation (PIC++/fourier.cpp:13-192): forward
   DFT of each component over the owned points, modes with |k| > 2 pi / (cut dx) multiplied by -1
@@ -95,6 +98,25 @@ def cleanup_divergence(E, rho, d):
     the potential on the owned cells."""
     o = slice(2, -2)
     rhs = -(4 * np.pi * rho[o, o, o] - div_e(E, d)[o, o, o])
+    phi = _solve_periodic(rhs, d)
+    owned = E[2:-3, 2:-3, 2:-3] + grad_correction(phi, d)
+    return _periodic_nodes(owned), phi
+
+
+def div_b(B, d):
+    """evaluateDivBgeneral at every node i = 1 .. N-1 of a ghosted cell field B (N, N, N, 3):
+    (N-1, N-1, N-1), index i - 1 for node i."""
+    Bx, By, Bz = B[..., 0], B[..., 1], B[..., 2]
+    rx = (Bx[1:, :-1, :-1] + Bx[1:, 1:, :-1] + Bx[1:, :-1, 1:] + Bx[1:, 1:, 1:]) / 4.0
+    lx = (Bx[:-1, :-1, :-1] + Bx[:-1, 1:, :-1] + Bx[:-1, :-1, 1:] + Bx[:-1, 1:, 1:]) / 4.0
+    ry = (By[:-1, 1:, :-1] + By[1:, 1:, :-1] + By[:-1, 1:, 1:] + By[1:, 1:, 1:]) / 4.0
+    ly = (By[:-1, :-1, :-1] + By[1:, :-1, :-1] + By[:-1, :-1, 1:] + By[1:, :-1, 1:]) / 4.0
+    rz = (Bz[:-1, :-1, 1:] + Bz[1:, :-1, 1:] + Bz[:-1, 1:, 1:] + Bz[1:, 1:, 1:]) / 4.0
+    lz = (Bz[:-1, :-1, :-1] + Bz[1:, :-1, :-1] + Bz[:-1, 1:, :-1] + Bz[1:, 1:, :-1]) / 4.0
+    return (rx - lx) / d[0] + (ry - ly) / d[1] + (rz - lz) / d[2]
+
+
+def _solve_periodic(rhs, d):
     n = rhs.shape
     w = cleanup_stencil(d)
     k = np.meshgrid(*[2 * np.pi * np.fft.fftfreq(v) for v in n], indexing="ij")
@@ -107,9 +129,27 @@ def cleanup_divergence(E, rho, d):
     small = np.abs(lam) < 1e-12 * np.max(np.abs(lam))
     lam[small] = 1.0
     R[small] = 0.0
-    phi = np.fft.ifftn(R / lam).real
-    owned = E[2:-3, 2:-3, 2:-3] + grad_correction(phi, d)
-    return _periodic_nodes(owned), phi
+    return np.fft.ifftn(R / lam).real
+
+
+def cleanup_divergence_magnetic(B, d):
+    """Simulation::cleanupDivergenceMagnetic (PIC++/divergencecleanup.cpp:939-1306) of a ghosted cell
+    field B: right part div B at the owned nodes (-cleanUpRightPartMagnetic), the same 27-point
+    operator for a node potential, B -= grad phi at the owned cells
+    (evaluateDivergenceCleaningFieldMagnetic); exact periodic solve as in cleanup_divergence (the
+    reference uses BiCGStab to 1e-8). Returns the cleaned B (ghosts exchanged) and phi."""
+    o = slice(1, -2)  # div_b index i - 1: owned nodes 2 .. n+1
+    phi = _solve_periodic(div_b(B, d)[o, o, o], d)
+    P = {}
+    for di in (0, 1):
+        for dj in (0, 1):
+            for dk in (0, 1):
+                P[di, dj, dk] = np.roll(phi, (-di, -dj, -dk), axis=(0, 1, 2))
+    cx = -(P[1, 0, 0] + P[1, 1, 0] + P[1, 0, 1] + P[1, 1, 1] - P[0, 0, 0] - P[0, 1, 0] - P[0, 0, 1] - P[0, 1, 1]) / (4 * d[0])
+    cy = -(P[0, 1, 0] + P[1, 1, 0] + P[0, 1, 1] + P[1, 1, 1] - P[0, 0, 0] - P[1, 0, 0] - P[0, 0, 1] - P[1, 0, 1]) / (4 * d[1])
+    cz = -(P[0, 0, 1] + P[0, 1, 1] + P[1, 0, 1] + P[1, 1, 1] - P[0, 0, 0] - P[0, 1, 0] - P[1, 0, 0] - P[1, 1, 0]) / (4 * d[2])
+    owned = B[2:-2, 2:-2, 2:-2] + np.stack([cx, cy, cz], -1)
+    return _periodic_cells(owned), phi
 
 
 def filter_fields(E, B, cut, d):

@@ -304,9 +304,10 @@ int main(int argc, char** argv) {
 	} else {
 		s.smoothingCount = 0;
 	}
-	// optional trailer: divergence cleanup (1: cleanupDivergence(newEfield, chargeDensity) every step
-	// at its call position in the loop, PIC++/simulation.cpp:348, after updateDensityParameters and
-	// the newE exchange) and the cut wave number of filterFields (> 0: filterFields(cut) next)
+	// optional trailer: divergence cleanup (bit 1: cleanupDivergence(newEfield, chargeDensity), bit 2:
+	// cleanupDivergenceMagnetic(), every step at their call position in the loop,
+	// PIC++/simulation.cpp:348-349, after updateDensityParameters and the newE exchange) and the cut
+	// wave number of filterFields (> 0: filterFields(cut) next)
 	int32_t cleanup = 0, filter_cut = 0;
 	if (fread(&cleanup, sizeof(int32_t), 1, in) == 1) filter_cut = rd<int32_t>(in);
 	fclose(in);
@@ -384,9 +385,10 @@ int main(int argc, char** argv) {
 		s.updateParticleCorrelationMaps();
 		s.updateDensityParameters();
 		s.exchangeGeneralEfield(s.newEfield);
-		if (cleanup == 1) s.cleanupDivergence(s.newEfield, s.chargeDensity);
+		if (cleanup & 1) s.cleanupDivergence(s.newEfield, s.chargeDensity);
+		if (cleanup & 2) s.cleanupDivergenceMagnetic();
 		if (filter_cut > 0) s.filterFields(filter_cut);
-		if ((cleanup == 1 || filter_cut > 0) && full) {
+		if ((cleanup != 0 || filter_cut > 0) && full) {
 			dump_node_vec(s, s.newEfield, tag("Enew_post", step));
 			dump_cell_vec(s, s.newBfield, tag("Bnew_post", step));
 		}

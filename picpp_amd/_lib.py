@@ -88,6 +88,7 @@ SIGNATURES = {
     "pic_apply_operator_factored": (ctypes.c_int, [_V, ctypes.c_int, _V, _V]),
     "pic_synchronize": (ctypes.c_int, [_V]),
     "pic_cleanup_divergence": (ctypes.c_int, [_V, ctypes.POINTER(PicSolveInfo)]),
+    "pic_cleanup_divergence_magnetic": (ctypes.c_int, [_V, ctypes.POINTER(PicSolveInfo)]),
     "pic_filter_fields": (ctypes.c_int, [_V, ctypes.c_int]),
     "pic_energy_noalloc": (ctypes.c_int, [ctypes.POINTER(PicConfig), _V, _V, _V, ctypes.POINTER(ctypes.c_longlong), _V, _V]),
     "pic_philox": (ctypes.c_int, [_V] + [ctypes.c_ulonglong] * 6 + [ctypes.c_longlong, _V]),

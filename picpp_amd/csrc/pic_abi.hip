@@ -753,7 +753,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	if (cfg->scheme != PIC_SCHEME_IMPLICIT && cfg->scheme != PIC_SCHEME_EXPLICIT) return PIC_ERR_ARGUMENT;
 	if ((cfg->divergence_cleanup || cfg->filter_cut_wave_number != 0) &&  // built for the periodic implicit step
 	    (cfg->scheme != PIC_SCHEME_IMPLICIT || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->filter_cut_wave_number < 0 ||
-	     (cfg->divergence_cleanup != 0 && cfg->divergence_cleanup != 1)))
+	     cfg->divergence_cleanup < 0 || cfg->divergence_cleanup > 3))
 		return PIC_ERR_ARGUMENT;
 	if (cfg->scheme == PIC_SCHEME_EXPLICIT &&  // the explicit step is built FP64, periodic, without smoothing or injection
 	    (cfg->precision != PIC_PRECISION_FP64 || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->smoothing_count != 0 || cfg->inject))
@@ -1582,7 +1582,11 @@ int pic_apply_operator_factored(pic_ctx* c, int precision, const void* v, void* 
 // maxDivergenceCleanupIterations (PIC++/constants.h:56)
 constexpr int kCleanupMaxIterations = 100;
 
-int pic_cleanup_divergence(pic_ctx* c, pic_solve_info* info) {
+// One divergence cleanup: the right part over the owned points (rhs_fn), the 27-point potential
+// equation solved by GMRES with the reference's stop rule, the correction (correct_fn).
+// target: the Givens estimate to stop at (0: the reference's GMRES rule).
+static int cleanup_solve(pic_ctx* c, pic_solve_info* info, const std::function<void(double*)>& rhs_fn,
+                         const std::function<void(const double*)>& correct_fn, double target = 0.0) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	if (c->explicit_scheme || c->g.wall) {
 		c->error = "divergence cleanup is built for the implicit scheme on the periodic box";
@@ -1604,13 +1608,13 @@ int pic_cleanup_divergence(pic_ctx* c, pic_solve_info* info) {
 		}
 		{
 			Timed t(c, "cleanup_rhs", (double)g.nnodes * 24 + (double)g.ncells * 8 + (double)len * 8);
-			launch_cleanup_rhs(g, c->Enew, c->dens + (long long)5 * c->sp.count * g.ncells, c->cl_rhs, s);
+			rhs_fn(c->cl_rhs);
 		}
 		const CleanupStencil S = cleanup_stencil(g);
 		KrylovSystem K{len, [c, S](const double* v, double* w) { launch_cleanup_apply(c->g, S, v, w, c->stream); }, "cleanup_apply",
 		               (double)len * 16, &c->Vcl, c->Vcl_dev, max_basis, PIC_ORTHO_MGS};
 		const int dim = (int)len;  // one component (generalizedMinimalResidualMethod with lnumber = 1)
-		const double target = std::max(c->cfg.gmres_tolerance / dim, 1e-15);
+		if (target <= 0) target = std::max(c->cfg.gmres_tolerance / dim, 1e-15);
 		double rel = 0;
 		pic_solve_info inf{0, 0, 0, -1, 0, 0, 0};
 		HIP_CHECK(hipStreamSynchronize(s));
@@ -1619,8 +1623,7 @@ int pic_cleanup_divergence(pic_ctx* c, pic_solve_info* info) {
 		inf.estimated_residual = rel;
 		{
 			Timed t(c, "cleanup_correct", (double)len * 8 + (double)g.nnodes * 48);
-			launch_cleanup_correct(g, c->cl_phi, c->Enew, s);
-			launch_ghost_fill(g, c->Enew, 3, true, s);  // exchangeGeneralEfield
+			correct_fn(c->cl_phi);
 		}
 		HIP_CHECK(hipStreamSynchronize(s));
 		inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1629,6 +1632,29 @@ int pic_cleanup_divergence(pic_ctx* c, pic_solve_info* info) {
 		return fail(c, e);
 	}
 	return PIC_OK;
+}
+
+int pic_cleanup_divergence(pic_ctx* c, pic_solve_info* info) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	return cleanup_solve(
+	    c, info,
+	    [c](double* rhs) { launch_cleanup_rhs(c->g, c->Enew, c->dens + (long long)5 * c->sp.count * c->g.ncells, rhs, c->stream); },
+	    [c](const double* phi) {
+		    launch_cleanup_correct(c->g, phi, c->Enew, c->stream);
+		    launch_ghost_fill(c->g, c->Enew, 3, true, c->stream);  // exchangeGeneralEfield
+	    });
+}
+
+int pic_cleanup_divergence_magnetic(pic_ctx* c, pic_solve_info* info) {
+	if (!c) return PIC_ERR_ARGUMENT;
+	return cleanup_solve(
+	    c, info, [c](double* rhs) { launch_cleanup_rhs_b(c->g, c->Bnew, rhs, c->stream); },
+	    [c](const double* phi) {
+		    launch_cleanup_correct_b(c->g, phi, c->Bnew, c->stream);
+		    launch_ghost_fill(c->g, c->Bnew, 3, false, c->stream);  // exchangeGeneralBfield
+	    },
+	    // the reference's BiCGStab reaches rounding level on these systems: the GMRES floor
+	    1e-15);
 }
 
 int pic_filter_fields(pic_ctx* c, int cut_wave_number) {
@@ -1847,7 +1873,8 @@ int pic_step(pic_ctx* c, pic_solve_info* info) {
 	if ((rc = pic_push(c))) return rc;
 	// the reference's call position of cleanupDivergence (PIC++/simulation.cpp:348): after the
 	// push's updateDensityParameters, before updateFields
-	if (c->cfg.divergence_cleanup && (rc = pic_cleanup_divergence(c, nullptr))) return rc;
+	if ((c->cfg.divergence_cleanup & 1) && (rc = pic_cleanup_divergence(c, nullptr))) return rc;
+	if ((c->cfg.divergence_cleanup & 2) && (rc = pic_cleanup_divergence_magnetic(c, nullptr))) return rc;
 	if (c->cfg.filter_cut_wave_number > 0 && (rc = pic_filter_fields(c, c->cfg.filter_cut_wave_number))) return rc;
 	return pic_commit_fields(c);
 }

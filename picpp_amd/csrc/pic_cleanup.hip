@@ -102,6 +102,53 @@ __global__ void k_cleanup_correct(Grid g, const double* __restrict__ phi, double
 	E[2 * g.nnodes + id] += cz;
 }
 
+// right part of the magnetic cleanup per owned node: div newB (evaluateDivBgeneral,
+// PIC++/fields.cpp:3019-3030, of the 8 cells around the node; -cleanUpRightPartMagnetic,
+// PIC++/divergencecleanup.cpp:1302-1306)
+__global__ void k_cleanup_rhs_b(Grid g, const double* __restrict__ B, double* __restrict__ rhs) {
+	const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
+	int a, b, c;
+	owned_ijk(g, o, a, b, c);
+	const int i = a + 2, j = b + 2, k = c + 2;
+	const double* Bx = B;
+	const double* By = B + g.ncells;
+	const double* Bz = B + 2 * g.ncells;
+	auto q = [&](int ii, int jj, int kk) { return cell_id(g, ii, jj, kk); };
+	const double rx = (Bx[q(i, j - 1, k - 1)] + Bx[q(i, j, k - 1)] + Bx[q(i, j - 1, k)] + Bx[q(i, j, k)]) / 4.0;
+	const double lx = (Bx[q(i - 1, j - 1, k - 1)] + Bx[q(i - 1, j, k - 1)] + Bx[q(i - 1, j - 1, k)] + Bx[q(i - 1, j, k)]) / 4.0;
+	const double ry = (By[q(i - 1, j, k - 1)] + By[q(i, j, k - 1)] + By[q(i - 1, j, k)] + By[q(i, j, k)]) / 4.0;
+	const double ly = (By[q(i - 1, j - 1, k - 1)] + By[q(i, j - 1, k - 1)] + By[q(i - 1, j - 1, k)] + By[q(i, j - 1, k)]) / 4.0;
+	const double rz = (Bz[q(i - 1, j - 1, k)] + Bz[q(i, j - 1, k)] + Bz[q(i - 1, j, k)] + Bz[q(i, j, k)]) / 4.0;
+	const double lz = (Bz[q(i - 1, j - 1, k - 1)] + Bz[q(i, j - 1, k - 1)] + Bz[q(i - 1, j, k - 1)] + Bz[q(i, j, k - 1)]) / 4.0;
+	rhs[o] = ((rx - lx) / g.d[0]) + ((ry - ly) / g.d[1]) + ((rz - lz) / g.d[2]);
+}
+
+// B -= grad phi at the owned cells from the node potential at the cell's 8 corners
+// (evaluateDivergenceCleaningFieldMagnetic and updateFieldByCleaningMagnetic,
+// PIC++/divergencecleanup.cpp:1076-1159); phi on the owned nodes, its ghosts the periodic images
+__global__ void k_cleanup_correct_b(Grid g, const double* __restrict__ phi, double* __restrict__ B) {
+	const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= (long long)g.n[0] * g.n[1] * g.n[2]) return;
+	int a, b, c;
+	owned_ijk(g, o, a, b, c);
+	// cell a sits between owned nodes a and a + 1
+	auto P = [&](int di, int dj, int dk) {
+		const int ii = wrapn(a + di, g.n[0]), jj = wrapn(b + dj, g.n[1]), kk = wrapn(c + dk, g.n[2]);
+		return phi[((long long)ii * g.n[1] + jj) * g.n[2] + kk];
+	};
+	const double cx = -(P(1, 0, 0) + P(1, 1, 0) + P(1, 0, 1) + P(1, 1, 1) - P(0, 0, 0) - P(0, 1, 0) - P(0, 0, 1) - P(0, 1, 1)) /
+	                  (4 * g.d[0]);
+	const double cy = -(P(0, 1, 0) + P(1, 1, 0) + P(0, 1, 1) + P(1, 1, 1) - P(0, 0, 0) - P(1, 0, 0) - P(0, 0, 1) - P(1, 0, 1)) /
+	                  (4 * g.d[1]);
+	const double cz = -(P(0, 0, 1) + P(0, 1, 1) + P(1, 0, 1) + P(1, 1, 1) - P(0, 0, 0) - P(0, 1, 0) - P(1, 0, 0) - P(1, 1, 0)) /
+	                  (4 * g.d[2]);
+	const long long id = cell_id(g, a + 2, b + 2, c + 2);
+	B[id] += cx;
+	B[g.ncells + id] += cy;
+	B[2 * g.ncells + id] += cz;
+}
+
 // ---------------------------------------------------------------- Fourier filtering
 // component comp of a ghosted field (nodes or cells) at the owned points -> re, im = 0
 __global__ void k_dft_load(Grid g, const double* __restrict__ f, long long plane, int nodes, double* __restrict__ re,
@@ -206,6 +253,14 @@ void launch_cleanup_apply(const Grid& g, const CleanupStencil& S, const double* 
 void launch_cleanup_correct(const Grid& g, const double* phi, double* E, hipStream_t s) {
 	const long long n = (long long)g.n[0] * g.n[1] * g.n[2];
 	hipLaunchKernelGGL(k_cleanup_correct, dim3(nb(n, 256)), dim3(256), 0, s, g, phi, E);
+}
+void launch_cleanup_rhs_b(const Grid& g, const double* B, double* rhs, hipStream_t s) {
+	const long long n = (long long)g.n[0] * g.n[1] * g.n[2];
+	hipLaunchKernelGGL(k_cleanup_rhs_b, dim3(nb(n, 256)), dim3(256), 0, s, g, B, rhs);
+}
+void launch_cleanup_correct_b(const Grid& g, const double* phi, double* B, hipStream_t s) {
+	const long long n = (long long)g.n[0] * g.n[1] * g.n[2];
+	hipLaunchKernelGGL(k_cleanup_correct_b, dim3(nb(n, 256)), dim3(256), 0, s, g, phi, B);
 }
 void launch_filter_component(const Grid& g, double* f, long long plane, bool nodes, int comp, double kcut, const double* const tw[3],
                              double* re, double* im, double* re2, double* im2, hipStream_t s) {

@@ -18,6 +18,9 @@ void launch_cleanup_rhs(const Grid& g, const double* E, const double* rho, doubl
 void launch_cleanup_apply(const Grid& g, const CleanupStencil& S, const double* v, double* out, hipStream_t s);
 // E -= grad phi at the owned nodes (ghosts are left for the caller's exchange)
 void launch_cleanup_correct(const Grid& g, const double* phi, double* E, hipStream_t s);
+// magnetic cleanup: rhs (owned nodes) = div B at the nodes; B -= grad phi at the owned cells
+void launch_cleanup_rhs_b(const Grid& g, const double* B, double* rhs, hipStream_t s);
+void launch_cleanup_correct_b(const Grid& g, const double* phi, double* B, hipStream_t s);
 // filterFieldGeneral on component comp (plane offset `plane`) of a ghosted node or cell field:
 // tw[a] holds (cos, sin)(2 pi q / n_a), q < n_a; re, im, re2, im2 are owned-size scratch
 void launch_filter_component(const Grid& g, double* f, long long plane, bool nodes, int comp, double kcut, const double* const tw[3],

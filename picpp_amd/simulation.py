@@ -61,7 +61,7 @@ class Simulation:
         cfg.smoothing_count = workload.smoothing_count
         cfg.smoothing_parameter = workload.smoothing_parameter
         cfg.scheme = {"implicit": _lib.SCHEME_IMPLICIT, "explicit": _lib.SCHEME_EXPLICIT}[scheme]
-        cfg.divergence_cleanup = 1 if workload.divergence_cleanup else 0
+        cfg.divergence_cleanup = int(workload.divergence_cleanup)
         cfg.filter_cut_wave_number = workload.filter_cut_wave_number
         for a in range(3):
             cfg.e_right[a] = workload.e_right[a]
@@ -258,6 +258,12 @@ class Simulation:
         """cleanupDivergence(newEfield, chargeDensity) (PIC++/divergencecleanup.cpp:20-392)."""
         info = _lib.PicSolveInfo()
         self._chk(self.lib.pic_cleanup_divergence(self.h, ctypes.byref(info)), "pic_cleanup_divergence")
+        return _info_dict(info)
+
+    def cleanupDivergenceMagnetic(self) -> Dict[str, float]:
+        """cleanupDivergenceMagnetic() on newB (PIC++/divergencecleanup.cpp:939-1306)."""
+        info = _lib.PicSolveInfo()
+        self._chk(self.lib.pic_cleanup_divergence_magnetic(self.h, ctypes.byref(info)), "pic_cleanup_divergence_magnetic")
         return _info_dict(info)
 
     def filterFields(self, cut_wave_number: int):

@@ -73,9 +73,10 @@ class Workload:
     # of the 27-point kernel with weights (s/2, 1-s, s/2) per axis, PIC++/smoothing.cpp)
     smoothing_count: int = 0
     smoothing_parameter: float = 0.5
-    # divergence cleanup of newE after the push (PIC++/divergencecleanup.cpp:20-392) and the cut
-    # wave number of the Fourier filtering of newE, newB (PIC++/fields_filtering.cpp:21-295; 0: off)
-    divergence_cleanup: bool = False
+    # divergence cleanup after the push: 1 newE (PIC++/divergencecleanup.cpp:20-392), 2 newB
+    # (:939-1306), 3 both; and the cut wave number of the Fourier filtering of newE, newB
+    # (PIC++/fields_filtering.cpp:21-295; 0: off)
+    divergence_cleanup: int = 0
     filter_cut_wave_number: int = 0
 
     @property

@@ -20,9 +20,10 @@ fixtures next to this script:
   wall_long_ref.npz the shock set-up (gamma 10 e-/p+ flow onto the wall, mass ratio 64,
                    sigma 0.1) at 16x8x8 cells, 8 ppc per species, 150 steps of dt = 0.1, no
                    injection: energies and E, B, rho norms every step, a particle sample at the end
-  cleanup_ref.npz, filter_ref.npz  the mini case with the divergence cleanup of newE
-                   (cleanupDivergence) after every push, and with the Fourier filtering of newE, newB
-                   (filterFields, cut wave number 2), 5 steps each
+  cleanup_ref.npz, filter_ref.npz, cleanup_eb_ref.npz  the mini case with the divergence cleanup
+                   of newE (cleanupDivergence) after every push, with the Fourier filtering of newE,
+                   newB (filterFields, cut wave number 2), and with the cleanup of newE and then newB
+                   (cleanupDivergenceMagnetic), 5 steps each
   explicit_ref.npz the explicit (BUNEMAN / Tristan) step (oracle/_ref/ref_harness_explicit):
                    the mini grid at dt = 0.4 with fast electrons (u ~ N(0, 0.5) per axis), so that
                    many zig-zag paths cross cells, perturbed Yee fields, 6 steps
@@ -78,7 +79,7 @@ def write_input(path, w, st, steps):
         else:
             f.write(struct.pack("<i", 0))
         f.write(struct.pack("<id", w.smoothing_count, w.smoothing_parameter))
-        f.write(struct.pack("<ii", 1 if w.divergence_cleanup else 0, w.filter_cut_wave_number))
+        f.write(struct.pack("<ii", int(w.divergence_cleanup), w.filter_cut_wave_number))
 
 
 def read_output(path):
@@ -227,8 +228,9 @@ def cleanup_cases():
     filtering of newE and newB (cut wave number 2), each 5 steps."""
     w = wl.mini()
     st = wl.make_initial_state(w, perturb=0.05)
-    return ((dataclasses.replace(w, divergence_cleanup=True, name="mini-cleanup"), st, "cleanup_ref.npz"),
-            (dataclasses.replace(w, filter_cut_wave_number=2, name="mini-filter"), st, "filter_ref.npz"))
+    return ((dataclasses.replace(w, divergence_cleanup=1, name="mini-cleanup"), st, "cleanup_ref.npz"),
+            (dataclasses.replace(w, filter_cut_wave_number=2, name="mini-filter"), st, "filter_ref.npz"),
+            (dataclasses.replace(w, divergence_cleanup=3, name="mini-cleanup-eb"), st, "cleanup_eb_ref.npz"))
 
 
 def make_cleanup():
