@@ -81,6 +81,11 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB between two device buffers three times (the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--no-held-out", action="store_true", help="skip the held-out solver comparison after the timed steps")
+    ap.add_argument("--pmc-summary", default=None, metavar="DIR",
+                    help="summarise FETCH_SIZE / WRITE_SIZE counter runs under DIR into profiles/pmc_traffic.json")
+    ap.add_argument("--trace-summary", default=None, metavar="DIR",
+                    help="per-kernel, per-grid-size dispatch statistics of the rocprofv3 kernel traces under DIR (markdown)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     return ap.parse_args()
@@ -107,8 +112,19 @@ def cpu_baseline(seconds: float):
         if el >= seconds or steps >= 50:
             break
     npart = len(st.species)
-    return {"value": npart * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "oracle FP64 step, 32x32x8 cells x 40 ppc (%d particles), %d steps in %.1f s" % (npart, steps, el)}
+    out = {"value": npart * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+           "sample": "oracle FP64 step (single-threaded), 32x32x8 cells, 20 ppc per species (%d particles), %d steps in %.1f s"
+                     % (npart, steps, el)}
+    # the compiled reference itself cannot run on this machine (its sources are not here); its
+    # timing on the same workload, taken where it was built, is committed beside the profiles
+    path = os.path.join(ROOT, "profiles", "reference_cpu_implicit_32x32x8_ppc20.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            ref = json.load(fh)
+        out["compiled_reference"] = {"cpu": ref["cpu"], "steps": ref["steps"],
+                                     "particle_steps_per_s_by_ranks": {str(r["ranks"]): r["particle_steps_per_s"] for r in ref["runs"]},
+                                     "source": "profiles/reference_cpu_implicit_32x32x8_ppc20.md"}
+    return out
 
 
 def train_network(args, torch, dev, w=None):
@@ -203,6 +219,23 @@ def read_profile(sim, steps):
     return kernels
 
 
+def step_roofline(npart, n, ms, args, state_bytes=28):
+    """The whole step against the HBM roofline, per the step's algorithmic bytes: each particle's
+    state read once and written once (2 S: 56 B in FP32, 96 B in FP64) plus 45 FP64 words per node
+    (E^{n+theta}, B^n, E^{n+1}, B^{n+1} read; J, mu, rhohat, Pi, rho and the two species' five
+    density planes written). With the committed counter summary of this workload, also the
+    measured HBM bytes of one step and their ratio to the algorithmic bytes."""
+    nodes = n[0] * n[1] * n[2]
+    alg = npart * 2.0 * state_bytes + nodes * 360.0
+    out = {"algorithmic_bytes": alg, "achieved_gbs": alg / (ms * 1e-3) / 1e9,
+           "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "unit": "GB/s", "peak": HBM_PEAK_GBS}
+    rec = pmc_record(args) if state_bytes == 28 else None
+    if rec and "step_traffic_bytes" in rec:
+        out["counter_traffic_bytes"] = rec["step_traffic_bytes"]
+        out["counter_traffic_ratio"] = rec["step_traffic_bytes"] / alg
+    return out
+
+
 def energy_drift(args, w, torch, dev):
     """Total energy of the chosen form and of the plain FP64 step over drift_steps steps from
     the same initial state: the largest relative difference between the two series, and each
@@ -227,15 +260,150 @@ def energy_drift(args, w, torch, dev):
             "fp64_rel_change": float(np.max(np.abs(b - b[0]) / abs(b[0])))}
 
 
+PMC_PATH = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+# the step kernels' names in the counter runs, by the bench's per-kernel names
+PMC_SLOTS = {"push": ("k_push32<",), "density_cell_moments": ("k_cell_moments32<true>",),
+             "node_moments": ("k_node_moments32",), "permute": ("k_permute32",),
+             "cell_moments": ("k_cell_moments32<false>",), "density": ("k_density32",)}
+
+
+def pmc_record(args):
+    """The committed counter summary, when it was taken on this workload (FP32, 128^3, 64 ppc)."""
+    if args.precision != "fp32" or args.cells != 128 or args.ppc != 64 or not os.path.exists(PMC_PATH):
+        return None
+    with open(PMC_PATH) as fh:
+        return json.load(fh)
+
+
 def measured_traffic(kernel, args):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE counter passes
     (scaled by a plain copy kernel's counters), when they were taken on this workload."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if args.precision != "fp32" or args.cells != 128 or args.ppc != 64 or not os.path.exists(path):
-        return None
-    with open(path) as fh:
-        rec = json.load(fh)["kernels"].get(kernel)
+    rec = pmc_record(args)
+    rec = rec["kernels"].get(kernel) if rec else None
     return rec["traffic_bytes"] if rec else None
+
+
+def _counter_rows(path):
+    """(dispatch id, kernel name, counter value) of every row of the rocprofv3 counter CSVs under path."""
+    import csv
+    import glob
+
+    rows = []
+    for f in sorted(glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True)):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                rows.append((int(r.get("Dispatch_Id", r.get("Correlation_Id", 0))), r["Kernel_Name"],
+                             float(r["Counter_Value"])))
+    if not rows:
+        raise SystemExit("no counter CSV under %s" % path)
+    return rows
+
+
+def _per_dispatch(path):
+    tot = {}
+    for d, name, v in _counter_rows(path):
+        k = (d, name)
+        tot[k] = tot.get(k, 0.0) + v  # a counter may come per XCD / instance
+    return tot
+
+
+def short_kernel_name(nm):
+    nm = nm.replace("(anonymous namespace)::", "")
+    depth, out = 0, []
+    for ch in nm:  # drop the argument list, keep template arguments
+        if ch == "(" and depth == 0 and out:
+            break
+        depth += ch == "<"
+        depth -= ch == ">"
+        out.append(ch)
+    return "".join(out).replace("void ", "", 1)
+
+
+def trace_summary(d, top=40):
+    """Dispatch statistics per kernel and grid size from the rocprofv3 kernel-trace CSVs under d:
+    the launches of one kernel on boxes of different sizes (the bench's 128^3 box, the learned
+    preconditioner's 32^3 training box) are kept apart."""
+    import csv
+    import glob
+
+    stats = {}
+    for f in sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                grid = 1
+                for ax in ("X", "Y", "Z"):
+                    grid *= int(r.get("Grid_Size_" + ax, 1) or 1)
+                if "Grid_Size" in r:
+                    grid = int(r["Grid_Size"])
+                k = (short_kernel_name(r["Kernel_Name"]), grid)
+                ns = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+                c, t = stats.get(k, (0, 0))
+                stats[k] = (c + 1, t + ns)
+    total = sum(t for c, t in stats.values())
+    rows = sorted(stats.items(), key=lambda kv: -kv[1][1])
+    lines = ["| Kernel | Grid (work-items) | Calls | Average ms | Total ms | % |", "|---|---|---|---|---|---|"]
+    for (nm, grid), (c, t) in rows[:top]:
+        lines.append("| `%s` | %d | %d | %.3f | %.3f | %.1f |" % (nm[:80], grid, c, t / c / 1e6, t / 1e6, 100.0 * t / total))
+    big = [kv for kv in rows[top:] if kv[0][1] >= (1 << 24)]
+    if big:  # the production-size launches (16 M work-items and more) not listed above
+        lines.append("| *(launches of 16 M work-items and more, below the top %d)* | | | | | |" % top)
+        for (nm, grid), (c, t) in big:
+            lines.append("| `%s` | %d | %d | %.3f | %.3f | %.1f |" % (nm[:80], grid, c, t / c / 1e6, t / 1e6, 100.0 * t / total))
+    lines.append("")
+    lines.append("All dispatches: %.1f ms, %d kernel and grid-size pairs." % (total / 1e6, len(stats)))
+    return "\n".join(lines)
+
+
+def pmc_summary(d, steps_a, steps_b):
+    """Per-launch and per-step HBM traffic from six counter runs under d: FETCH_SIZE and WRITE_SIZE
+    of the timed step at steps_a and at steps_b steps (the difference divided by steps_b - steps_a is
+    one step's traffic, since everything else in the two runs is the same), and of a plain 1 GiB
+    device copy (the scale of each counter)."""
+    cf = _per_dispatch(os.path.join(d, "copy_fetch"))
+    cw = _per_dispatch(os.path.join(d, "copy_write"))
+    gib = float(1 << 30)
+    copy_f = [v for (i, nm), v in cf.items() if "copyBuffer" in nm or "copy" in nm.lower()]
+    copy_w = [v for (i, nm), v in cw.items() if "copyBuffer" in nm or "copy" in nm.lower()]
+    copy_f = float(np.median(sorted(copy_f)[-3:]))
+    copy_w = float(np.median(sorted(copy_w)[-3:]))
+    ff = gib / (copy_f * 1024.0)  # bytes per counted byte, each counter in KiB
+    fw = gib / (copy_w * 1024.0)
+    runs = {}
+    for tag, st in (("a", steps_a), ("b", steps_b)):
+        f = _per_dispatch(os.path.join(d, "fetch_%d" % st))
+        w = _per_dispatch(os.path.join(d, "write_%d" % st))
+        runs[tag] = (f, w)
+
+    def total(f, w):
+        return sum(v * 1024.0 * ff for v in f.values()) + sum(v * 1024.0 * fw for v in w.values())
+
+    step_bytes = (total(*runs["b"]) - total(*runs["a"])) / (steps_b - steps_a)
+
+    def by_name(f, w):
+        out = {}
+        for (i, nm), v in f.items():
+            out[nm] = out.get(nm, 0.0) + v * 1024.0 * ff
+        for (i, nm), v in w.items():
+            out[nm] = out.get(nm, 0.0) + v * 1024.0 * fw
+        return out
+
+    na, nb_ = by_name(*runs["a"]), by_name(*runs["b"])
+    per_kernel = {nm: (nb_.get(nm, 0.0) - na.get(nm, 0.0)) / (steps_b - steps_a) for nm in set(na) | set(nb_)}
+    per_kernel = dict(sorted(((k[:90], v) for k, v in per_kernel.items() if abs(v) > 1e7), key=lambda kv: -kv[1]))
+    f, w = runs["b"]
+    kernels = {}
+    for slot, pats in PMC_SLOTS.items():
+        ids = sorted({k for k in f if any(p in k[1] for p in pats)})
+        if not ids:
+            continue
+        fk = sum(f[k] for k in ids)
+        wk = sum(w.get(k, 0.0) for k in ids)
+        kernels[slot] = {"fetch_kib_raw": fk / len(ids), "write_kib_raw": wk / len(ids),
+                         "traffic_bytes": (fk * ff + wk * fw) * 1024.0 / len(ids), "dispatches": len(ids)}
+    return {"fetch_factor": ff, "write_factor": fw,
+            "calibration": "1 GiB device-to-device copy: FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB per dispatch" % (copy_f, copy_w),
+            "step_traffic_bytes": step_bytes, "steps": [steps_a, steps_b], "kernels": kernels,
+            "step_traffic_by_kernel": per_kernel}
 
 
 def weibel_check(args, torch, dev):
@@ -392,6 +560,15 @@ def main():
 
     from picpp_amd import workloads as wl
 
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
+        return
+    if args.pmc_summary:
+        rec = pmc_summary(args.pmc_summary, 1, 3)
+        with open(PMC_PATH, "w") as fh:
+            json.dump(rec, fh, indent=1)
+        print(json.dumps(rec, indent=1))
+        return
     dev = torch.device("cuda:0")
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
@@ -436,7 +613,7 @@ def main():
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     traffic = measured_traffic(dom, args)
     energy = sim.updateEnergy()
-    held_out = compare_solvers(sim, torch, learned) if args.solver == "mixed" else None
+    held_out = compare_solvers(sim, torch, learned) if args.solver == "mixed" and not args.no_held_out else None
     sim.close()
     del sim
     torch.cuda.empty_cache()
@@ -466,6 +643,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": per_launch},
+        "step_roofline": step_roofline(npart, n, 1e3 * el / args.steps, args),
         "kernels_ms_per_step": {nm: kv["ms_total"] / args.steps for nm, kv in kernels.items()},
         "energy": {"particles": energy[0], "E": energy[1], "B": energy[2], "total": energy[3]},
     }
@@ -478,7 +656,8 @@ def main():
         torch.cuda.empty_cache()
         out["fp64_step"] = {"ms_per_step": 1e3 * el64 / args.steps, "value": npart * args.steps / el64,
                             "gmres_ms_per_solve": 1e3 * float(np.mean([s["seconds"] for s in solves64])),
-                            "stop_rule": "true_residual", "speedup_of_chosen_form": el64 / el}
+                            "stop_rule": "true_residual", "speedup_of_chosen_form": el64 / el,
+                            "step_roofline": step_roofline(npart, n, 1e3 * el64 / args.steps, args, state_bytes=48)}
         print("fp64 step: %.2f ms (chosen form %.2f ms, %.2fx)" % (1e3 * el64 / args.steps, 1e3 * el / args.steps, el64 / el),
               file=sys.stderr)
     if args.explicit_steps > 0 and args.precision == "fp32":

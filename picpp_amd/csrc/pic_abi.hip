@@ -1,6 +1,6 @@
 // C ABI of the implicit PIC hot path: context, transfers and the Simulation-level phases.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <climits>
 
 #include <algorithm>
 #include <chrono>
@@ -13,6 +13,10 @@
 #include <random>
 #include <string>
 #include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "../../include/picpp_amd.h"
 #include "pic_device.hpp"
@@ -89,12 +93,17 @@ struct pic_ctx {
 	// made E, B equal to them, and the next pic_rotation_and_moments uses them. Every other call
 	// that can change particles or fields resets it to 0.
 	int fused = 0;
-	// PIC_SCHEME_EXPLICIT: the Yee arrays and the largest particle count of a cell
+	// PIC_SCHEME_EXPLICIT: the Yee arrays and the bounds of the current's fixed-point scale.
+	// bounds->u2max holds each species' largest |u|^2 as the last explicit push left it
+	// (u2_valid, in u2_slots); every entry point that can change particles clears u2_valid, and the next step
+	// then reads it from the particles.
 	bool explicit_scheme = false;
 	YeeGrid yee{};
-	int* maxcount = nullptr;
-	void* cub_temp = nullptr;
-	size_t cub_temp_bytes = 0;
+	ExplicitBounds* bounds = nullptr;
+	unsigned long long* u2_slots = nullptr;  // kU2Slots x kMaxSpecies
+	bool u2_valid = false;
+	void* sort_temp = nullptr;
+	size_t sort_temp_bytes = 0;
 	std::vector<int> species_start_h;
 	int* species_start = nullptr;
 	int* status = nullptr;
@@ -225,6 +234,9 @@ void check_status(pic_ctx* c) {
 		if (st[0] & 2) why += " particle maps inconsistent (moved more than half a cell?);";
 		if (st[0] & 4) why += " particle outside the owned region;";
 		if (st[0] & 8) why += " particle stencil outside the ghost layers;";
+		if (st[0] & 16)
+			why += " a particle's new momentum exceeds the bound the explicit current's fixed-point scale was set for "
+			       "(momenta changed through device pointers since the last step?), the current is not valid;";
 		if (st[1] && c->fp32) {  // the FP32 mover's first failing particle, still in its pre-push state
 			const long long i = st[3];
 			int cell = 0;
@@ -272,9 +284,9 @@ void sort_particles(pic_ctx* c, int removed = 0, bool resized = false) {
 		{
 			Timed t(c, "sort_radix", (double)n * 16);
 			launch_iota(c->perm, n, s);
-			size_t bytes = c->cub_temp_bytes;
-			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_temp, bytes, c->keys, c->keys_sorted, c->perm, c->perm_sorted, (int)n,
-			                                             0, c->end_bit, s));
+			size_t bytes = c->sort_temp_bytes;
+			HIP_CHECK(rocprim::radix_sort_pairs(c->sort_temp, bytes, c->keys, c->keys_sorted, c->perm, c->perm_sorted,
+			                                    (size_t)n, 0, c->end_bit, s));
 		}
 		if (c->fp32) {
 			Timed t(c, "permute", (double)n * (2 * 32 + 4));
@@ -294,8 +306,8 @@ void sort_particles(pic_ctx* c, int removed = 0, bool resized = false) {
 	Timed t(c, "segments", (double)n * 4 + (double)(c->nkeys + 1) * 12);
 	HIP_CHECK(hipMemsetAsync(c->counts, 0, (c->nkeys + 1) * sizeof(int), s));
 	launch_count_keys(c->keys_sorted, n, c->counts, s);
-	size_t bytes = c->cub_temp_bytes;
-	HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start, (int)(c->nkeys + 1), s));
+	size_t bytes = c->sort_temp_bytes;
+	HIP_CHECK(rocprim::exclusive_scan(c->sort_temp, bytes, c->counts, c->start, 0, (size_t)(c->nkeys + 1), rocprim::plus<int>(), s));
 	if (removed > 0 || resized) species_ranges(c, n);
 }
 
@@ -355,8 +367,8 @@ void wall_particles(pic_ctx* c) {
 		}
 		if (removed > 0) {
 			Timed t(c, "replace", 0);
-			size_t bytes = c->cub_temp_bytes;
-			HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(c->cub_temp, bytes, c->esc, c->esc_sorted, removed, 0, c->esc_bits, s));
+			size_t bytes = c->sort_temp_bytes;
+			HIP_CHECK(rocprim::radix_sort_keys(c->sort_temp, bytes, c->esc, c->esc_sorted, (size_t)removed, 0, c->esc_bits, s));
 			ph.id_base = (int)c->next_id;
 			launch_replace(c->g, c->sp, ph, c->species_start, c->esc_sorted, removed, c->P, c->Q, c->fp32, c->keys, s);
 			c->next_id += removed;
@@ -382,8 +394,8 @@ void rebin_particles(pic_ctx* c, int removed, bool resized) {
 		Timed t(c, "rebin_scan", (double)n / 64 * 16);
 		const long long nw = (n + 63) / 64;
 		launch_mask_counts(c->mask, nw, c->wcount, s);
-		size_t bytes = c->cub_temp_bytes;
-		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->wcount, c->wprefix, (int)(nw + 1), s));
+		size_t bytes = c->sort_temp_bytes;
+		HIP_CHECK(rocprim::exclusive_scan(c->sort_temp, bytes, c->wcount, c->wprefix, 0, (size_t)(nw + 1), rocprim::plus<int>(), s));
 		HIP_CHECK(hipMemcpyAsync(&M, c->wprefix + nw, sizeof(int), hipMemcpyDeviceToHost, s));
 		HIP_CHECK(hipStreamSynchronize(s));
 	}
@@ -395,8 +407,8 @@ void rebin_particles(pic_ctx* c, int removed, bool resized) {
 		Timed t(c, "rebin_movers", (double)n / 64 * 12 + (double)M * 36 + (double)(c->nkeys + 1) * 4);
 		launch_gather_movers(c->mask, c->wprefix, c->keys, n, c->nkeys, c->mk, c->mi, s);
 		if (M > 0) {
-			size_t bytes = c->cub_temp_bytes;
-			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->cub_temp, bytes, c->mk, c->mk_s, c->mi, c->mi_s, M, 0, c->mover_bit, s));
+			size_t bytes = c->sort_temp_bytes;
+			HIP_CHECK(rocprim::radix_sort_pairs(c->sort_temp, bytes, c->mk, c->mk_s, c->mi, c->mi_s, (size_t)M, 0, c->mover_bit, s));
 		}
 		HIP_CHECK(hipMemsetAsync(c->arrive, 0, (c->nkeys + 1) * sizeof(int), s));
 		launch_count_keys(c->mk_s, M, c->arrive, s);
@@ -405,8 +417,9 @@ void rebin_particles(pic_ctx* c, int removed, bool resized) {
 	{
 		Timed t(c, "rebin_segments", (double)(c->nkeys + 1) * 4 * 6);
 		launch_rebin_counts(c->start, c->mask, c->wprefix, c->arrive, c->nkeys, c->stay, c->counts, s);
-		size_t bytes = c->cub_temp_bytes;
-		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->cub_temp, bytes, c->counts, c->start_new, (int)(c->nkeys + 1), s));
+		size_t bytes = c->sort_temp_bytes;
+		HIP_CHECK(rocprim::exclusive_scan(c->sort_temp, bytes, c->counts, c->start_new, 0, (size_t)(c->nkeys + 1),
+		                                  rocprim::plus<int>(), s));
 	}
 	{
 		Timed t(c, "rebin_perm", (double)n * (4 + 4) + (double)c->nkeys * 12);
@@ -818,7 +831,8 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 				HIP_CHECK(hipMemset(c->yee.J[comp], 0, yee_count(c->yee.dE[comp]) * sizeof(double)));
 				HIP_CHECK(hipMemset(c->yee.B[comp], 0, yee_count(c->yee.dB[comp]) * sizeof(double)));
 			}
-			c->maxcount = dalloc<int>(1);
+			c->bounds = dalloc<ExplicitBounds>(1);
+			c->u2_slots = dalloc<unsigned long long>((size_t)kU2Slots * kMaxSpecies);
 		}
 		c->nkeys = (long long)cfg->nspecies * g.ncells * 8;
 		if (c->nkeys >= (1LL << 32) - 1) throw DeviceError{PIC_ERR_ARGUMENT, "grid too large for 32-bit particle keys"};
@@ -885,9 +899,10 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->counts = dalloc<int>(c->nkeys + 1);
 		HIP_CHECK(hipMemset(c->start, 0, (c->nkeys + 1) * sizeof(int)));
 		size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
-		HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c->keys, c->keys_sorted, c->perm, c->perm_sorted,
-		                                             (int)std::max<long long>(c->capacity, 1), 0, c->end_bit, c->stream));
-		HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->counts, c->start, (int)(c->nkeys + 1), c->stream));
+		HIP_CHECK(rocprim::radix_sort_pairs(nullptr, b1, c->keys, c->keys_sorted, c->perm, c->perm_sorted,
+		                                    (size_t)std::max<long long>(c->capacity, 1), 0, c->end_bit, c->stream));
+		HIP_CHECK(rocprim::exclusive_scan(nullptr, b2, c->counts, c->start, 0, (size_t)(c->nkeys + 1),
+		                                  rocprim::plus<int>(), c->stream));
 		{  // the rebin after a push with mover bits
 			const long long words = c->capacity / 64 + 2;
 			c->mask = dalloc<uint64_t>(words);
@@ -899,12 +914,12 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 			c->first = dalloc<int>(c->nkeys + 1);
 			c->mover_capacity = c->capacity / 4 + 1;
 			for (uint32_t** p : {&c->mk, &c->mi, &c->mk_s, &c->mi_s}) *p = dalloc<uint32_t>(c->mover_capacity);
-			HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, c->wcount, c->wprefix, (int)words, c->stream));
-			HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b4, c->mk, c->mk_s, c->mi, c->mi_s, (int)c->mover_capacity, 0,
-			                                             c->mover_bit, c->stream));
+			HIP_CHECK(rocprim::exclusive_scan(nullptr, b3, c->wcount, c->wprefix, 0, (size_t)words, rocprim::plus<int>(), c->stream));
+			HIP_CHECK(rocprim::radix_sort_pairs(nullptr, b4, c->mk, c->mk_s, c->mi, c->mi_s,
+			                                    (size_t)c->mover_capacity, 0, c->mover_bit, c->stream));
 		}
-		c->cub_temp_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
-		c->cub_temp = dalloc<char>(c->cub_temp_bytes);
+		c->sort_temp_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
+		c->sort_temp = dalloc<char>(c->sort_temp_bytes);
 		c->species_start = dalloc<int>(kMaxSpecies + 1);
 		c->species_start_h.assign(kMaxSpecies + 1, 0);
 		HIP_CHECK(hipMemset(c->species_start, 0, (kMaxSpecies + 1) * sizeof(int)));
@@ -917,12 +932,12 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 			c->esc_sorted = dalloc<uint32_t>(c->capacity);
 			while ((1LL << c->esc_bits) < c->capacity) ++c->esc_bits;
 			size_t b4 = 0;
-			HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, b4, c->esc, c->esc_sorted, (int)std::max<long long>(c->capacity, 1), 0,
-			                                            c->esc_bits, c->stream));
-			if (b4 > c->cub_temp_bytes) {
-				hipFree(c->cub_temp);
-				c->cub_temp_bytes = b4;
-				c->cub_temp = dalloc<char>(c->cub_temp_bytes);
+			HIP_CHECK(rocprim::radix_sort_keys(nullptr, b4, c->esc, c->esc_sorted,
+			                                   (size_t)std::max<long long>(c->capacity, 1), 0, c->esc_bits, c->stream));
+			if (b4 > c->sort_temp_bytes) {
+				hipFree(c->sort_temp);
+				c->sort_temp_bytes = b4;
+				c->sort_temp = dalloc<char>(c->sort_temp_bytes);
 			}
 			for (int s = 0; s < cfg->nspecies; ++s)  // injectionLength at the loop start (PIC++/simulation.cpp:125-127)
 				if (cfg->inject && cfg->ppc[s] > 0) c->inj_len[s] = cfg->dx / cfg->ppc[s] - 0.0001 * cfg->dx;
@@ -991,7 +1006,7 @@ void pic_destroy(pic_ctx* c) {
 	for (void* p : {(void*)c->E, (void*)c->Etheta, (void*)c->Enew, (void*)c->B, (void*)c->Bnew, (void*)c->J, (void*)c->mu,
 	                (void*)c->divPi, (void*)c->rhohat, (void*)c->Pi, (void*)c->dens, (void*)c->node_scratch,
 	                (void*)c->rho_scratch, (void*)c->pi_scratch, (void*)c->dens_scratch, (void*)c->keys, (void*)c->keys_sorted,
-	                (void*)c->perm, (void*)c->perm_sorted, (void*)c->start, (void*)c->counts, c->cub_temp,
+	                (void*)c->perm, (void*)c->perm_sorted, (void*)c->start, (void*)c->counts, c->sort_temp,
 	                (void*)c->species_start, (void*)c->status, (void*)c->dead, (void*)c->dens_flip, (void*)c->dens_zero, (void*)c->esc, (void*)c->esc_sorted, (void*)c->smooth_tmp, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
 	                (void*)c->Vdev, (void*)c->partial, (void*)c->dscal, (void*)c->ydev, (void*)c->epartial, (void*)c->eout})
 		f(p);
@@ -1008,7 +1023,8 @@ void pic_destroy(pic_ctx* c) {
 		f(c->yee.J[comp]);
 		f(c->yee.Jacc[comp]);
 	}
-	f(c->maxcount);
+	f(c->bounds);
+	f(c->u2_slots);
 	for (ParticleArrays32* Q : {&c->Q, &c->Qalt}) {
 		f(Q->cell);
 		for (int a = 0; a < 3; ++a) {
@@ -1041,6 +1057,7 @@ void pic_destroy(pic_ctx* c) {
 int pic_upload_fields(pic_ctx* c, const double* E, const double* B) {
 	if (!c || !E || !B) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		const Grid& g = c->g;
 		HIP_CHECK(hipMemcpyAsync(c->E, E, 3 * g.nnodes * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1077,9 +1094,9 @@ void load_particles(pic_ctx* c, const std::vector<int>& cnt, const double* x, co
 		if (np > 0) {
 			int* dmax = dalloc<int>(1);
 			size_t bytes = 0;
-			HIP_CHECK(hipcub::DeviceReduce::Max(nullptr, bytes, ids, dmax, (int)np, s));
+			HIP_CHECK(rocprim::reduce(nullptr, bytes, ids, dmax, INT_MIN, (size_t)np, rocprim::maximum<int>(), s));
 			char* tmp = dalloc<char>(bytes);
-			HIP_CHECK(hipcub::DeviceReduce::Max(tmp, bytes, ids, dmax, (int)np, s));
+			HIP_CHECK(rocprim::reduce(tmp, bytes, ids, dmax, INT_MIN, (size_t)np, rocprim::maximum<int>(), s));
 			int mx = 0;
 			HIP_CHECK(hipMemcpyAsync(&mx, dmax, sizeof(int), hipMemcpyDeviceToHost, s));
 			HIP_CHECK(hipStreamSynchronize(s));
@@ -1121,6 +1138,7 @@ int pic_upload_particles(pic_ctx* c, long long np, const int* species, const int
 		return PIC_ERR_ARGUMENT;
 	}
 	c->fused = 0;
+	c->u2_valid = false;
 	double* dx = nullptr;
 	double* dp = nullptr;
 	int* di = nullptr;
@@ -1165,6 +1183,7 @@ int pic_upload_particles(pic_ctx* c, long long np, const int* species, const int
 int pic_upload_particles_device(pic_ctx* c, const long long* counts, const double* x, const double* p, const int* ids) {
 	if (!c || !counts || !x || !p) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		std::vector<int> cnt(c->sp.count + 1, 0);
 		long long np = 0;
@@ -1259,6 +1278,7 @@ static int array_ptr(pic_ctx* c, int which, double** ptr, long long* count) {
 int pic_device_array(pic_ctx* c, int which, double** ptr, long long* count) {
 	if (!c || !ptr || !count) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	return array_ptr(c, which, ptr, count);
 }
 
@@ -1281,6 +1301,7 @@ long long pic_download_array(pic_ctx* c, int which, double* out) {
 int pic_initialize(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		const Grid& g = c->g;
 		hipStream_t s = c->stream;
@@ -1313,6 +1334,7 @@ int pic_rotation_and_moments(pic_ctx* c) {
 int pic_field_rhs(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		Timed t(c, "rhs", (double)c->g.nown * (3 + 3 + 3 + 1 + 3 + 9 + 9) * 8);
 		launch_rhs(c->g, c->E, c->B, c->J, c->rhohat, c->theta, c->cfg.dt, c->rhs, c->stream);
@@ -1518,6 +1540,7 @@ static int solve_impl(pic_ctx* c, double* b, double* x, pic_solve_info* info) {
 int pic_maxwell_solve(pic_ctx* c, pic_solve_info* info) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		int rc = pic_field_rhs(c);
 		if (rc) return rc;
@@ -1534,6 +1557,7 @@ int pic_maxwell_solve(pic_ctx* c, pic_solve_info* info) {
 int pic_gmres_solve(pic_ctx* c, const double* rhs, double* x, pic_solve_info* info) {
 	if (!c || !rhs || !x) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		return solve_impl(c, const_cast<double*>(rhs), x, info);
 	} catch (const DeviceError& e) {
@@ -1544,6 +1568,7 @@ int pic_gmres_solve(pic_ctx* c, const double* rhs, double* x, pic_solve_info* in
 int pic_apply_operator(pic_ctx* c, const double* v, double* y) {
 	if (!c || !v || !y) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		launch_pack_mu(c->g, c->mu, c->muc, c->stream);
 		launch_pack_mu_halo(c->g, c->mu, c->muh, c->stream);
@@ -1562,6 +1587,7 @@ int pic_apply_operator_factored(pic_ctx* c, int precision, const void* v, void* 
 		return PIC_ERR_ARGUMENT;
 	}
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		launch_pack_mu(c->g, c->mu, c->muc, c->stream);
 		launch_pack_mu_halo(c->g, c->mu, c->muh, c->stream);
@@ -1593,6 +1619,7 @@ static int cleanup_solve(pic_ctx* c, pic_solve_info* info, const std::function<v
 		return PIC_ERR_ARGUMENT;
 	}
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		const Grid& g = c->g;
 		hipStream_t s = c->stream;
@@ -1664,6 +1691,7 @@ int pic_filter_fields(pic_ctx* c, int cut_wave_number) {
 		return PIC_ERR_ARGUMENT;
 	}
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		const Grid& g = c->g;
 		hipStream_t s = c->stream;
@@ -1701,6 +1729,7 @@ int pic_filter_fields(pic_ctx* c, int cut_wave_number) {
 int pic_update_b(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		Timed t(c, "update_b", (double)c->g.ncells * 6 * 8 + (double)c->g.nnodes * 3 * 8);
 		launch_update_b(c->g, c->Etheta, c->B, c->cfg.dt, c->Bnew, c->stream);
@@ -1713,6 +1742,7 @@ int pic_update_b(pic_ctx* c) {
 int pic_push(pic_ctx* c) {
 	if (!c) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		if (c->fp32) {
 			const Grid& g = c->g;
@@ -1722,11 +1752,13 @@ int pic_push(pic_ctx* c) {
 				launch_f64_to_f32(c->Etheta, c->Et32, g.nnodes, c->stream);
 				launch_f64_to_f32(c->B, c->B32, g.ncells, c->stream);
 			}
-			Timed t(c, "push", (double)c->np() * 60 + ((double)g.nnodes * 6 + (double)g.ncells * 3) * 4);
+			// algorithmic bytes: the particle state read once and written once, 2 x 28 B (the
+			// per-particle unit of the step; the key, the mover bits and the field reads are extra)
+			Timed t(c, "push", (double)c->np() * 56);
 			launch_push32(g, c->sp, c->Q, c->species_start, c->E32, c->Et32, c->B32, (float)c->cfg.dt, (float)c->eta, c->keys,
 			              c->mask, c->status, c->dead, c->esc, c->stream);
 		} else {
-			Timed t(c, "push", (double)c->P.count * 100 + ((double)c->g.nnodes * 6 + (double)c->g.ncells * 3) * 8);
+			Timed t(c, "push", (double)c->P.count * 96);  // 2 x 48 B of particle state
 			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->mask, c->status,
 			            c->dead, c->esc, c->stream);
 		}
@@ -1819,29 +1851,57 @@ int pic_explicit_step(pic_ctx* c) {
 			launch_tristan_b_half(g, c->yee, dt, s);
 			launch_yee_exchange(g, c->yee, true, s);
 		}
-		// fixed-point scale of the current: an entry takes at most 2 parts from each particle
+		// Fixed-point scale of the current. A particle's part of a zig-zag path holds at most
+		// 2 |q w / V| |v| (v its new velocity): an entry takes at most 2 parts from each particle
 		// whose cell before or after the move touches it (4 + 4 cells; the particles of the 4
-		// cells after the move come from their 27 neighbours), each part at most 2 |q w / V|:
-		// below 512 m |q w / V| with m particles in the fullest cell
-		int maxcount = 0;
-		HIP_CHECK(hipMemsetAsync(c->maxcount, 0, sizeof(int), s));
-		launch_max_cell_count(g, c->sp.count, c->start, c->maxcount, s);
-		HIP_CHECK(hipMemcpyAsync(&maxcount, c->maxcount, sizeof(int), hipMemcpyDeviceToHost, s));
+		// cells after the move come from their 27 neighbours), so it stays below
+		// 512 m max(|q w / V| |v|) with m particles in the fullest cell. The new momentum obeys
+		// |u_new| <= |u| + |q| dt |E| / m (the rotation keeps |u|; each half kick adds at most
+		// |q| dt |E| / 2m, |E| bounded by the components' largest grid values) and
+		// |v| = |u| / sqrt(1 + |u|^2). The scale is the largest power of two keeping that sum
+		// below 2^62 and one contribution below 2^50 (the push's rounding to integers): the
+		// rounding of a contribution is then at most 2^-62 of the sum's bound, i.e. relative to
+		// the particles' actual speeds, not to c.
+		ExplicitBounds hb{};
+		const size_t slot_bytes = sizeof(unsigned long long) * kU2Slots * kMaxSpecies;
+		HIP_CHECK(hipMemsetAsync(c->bounds, 0, sizeof(ExplicitBounds), s));
+		if (!c->u2_valid) HIP_CHECK(hipMemsetAsync(c->u2_slots, 0, slot_bytes, s));  // else the last push's maxima
+		launch_explicit_bounds(g, c->sp, c->P, c->species_start, c->start, c->yee, !c->u2_valid, c->u2_slots, c->bounds, s);
+		HIP_CHECK(hipMemcpyAsync(&hb, c->bounds, sizeof(hb), hipMemcpyDeviceToHost, s));
 		HIP_CHECK(hipStreamSynchronize(s));
-		double fcd = 0;
-		for (int q = 0; q < c->sp.count; ++q) fcd = std::max(fcd, std::fabs(c->sp.charge[q] * c->sp.weight[q] / g.volume));
-		const double bound = 512.0 * std::max(maxcount, 1) * std::max(fcd, 1e-300);
-		// and a single contribution (below 2 |q w / V| as |v| < 1) stays below 2^50, for the
-		// push's rounding to integers
-		const int e_sum = (int)std::floor(62.0 - std::log2(bound));
-		const int e_one = (int)std::floor(49.0 - std::log2(std::max(fcd, 1e-300)));
-		const double scale = std::ldexp(1.0, std::min(e_sum, e_one));
+		auto bits_to_double = [](unsigned long long b) {
+			double v;
+			memcpy(&v, &b, sizeof(v));
+			return v;
+		};
+		double e2 = 0;
+		for (int a = 0; a < 3; ++a) e2 += bits_to_double(hb.emax[a]) * bits_to_double(hb.emax[a]);
+		const double emag = std::sqrt(e2);
+		double fv = 0, u2_limit[kMaxSpecies] = {};
+		for (int q = 0; q < c->sp.count; ++q) {
+			const double U = (std::sqrt(bits_to_double(hb.u2max[q])) + std::fabs(c->sp.charge[q]) * dt * emag / c->sp.mass[q]) *
+			                 (1.0 + 1e-6);  // margin for the rounding of the particles' own arithmetic
+			u2_limit[q] = U * U;
+			fv = std::max(fv, std::fabs(c->sp.charge[q] * c->sp.weight[q] / g.volume) * (U / std::sqrt(1.0 + U * U)));
+		}
+		if (!std::isfinite(fv) || !std::isfinite(emag))
+			throw DeviceError{PIC_ERR_PARTICLE, "pic_explicit_step: a particle momentum or the electric field is not finite"};
+		int e = 0;
+		if (fv > 0) {
+			const double bound = 512.0 * std::max(hb.maxcount, 1) * fv;
+			e = std::min((int)std::floor(62.0 - std::log2(bound)), (int)std::floor(49.0 - std::log2(fv)));
+			e = std::max(-1000, std::min(1000, e));
+		}
+		const double scale = std::ldexp(1.0, e);
+		HIP_CHECK(hipMemsetAsync(c->u2_slots, 0, slot_bytes, s));  // for the push's maxima
 		for (int comp = 0; comp < 3; ++comp)
 			HIP_CHECK(hipMemsetAsync(c->yee.Jacc[comp], 0, yee_count(c->yee.dE[comp]) * sizeof(long long), s));
 		{
 			Timed t(c, "tristan_push", (double)c->P.count * 100 + (double)g.ncells * 6 * 8 * 8);
-			launch_tristan_push(g, c->sp, c->P, c->species_start, c->yee, dt, scale, c->keys, c->mask, c->status, s);
+			launch_tristan_push(g, c->sp, c->P, c->species_start, c->yee, dt, scale, u2_limit, c->keys, c->mask, c->status,
+			                    c->u2_slots, s);
 		}
+		c->u2_valid = true;
 		check_status(c);
 		{
 			Timed t(c, "yee_b_half", (double)g.ncells * 3 * 8 * 3);
@@ -2026,6 +2086,7 @@ int pic_checkpoint_save(pic_ctx* c, const char* path) {
 int pic_checkpoint_load(pic_ctx* c, const char* path) {
 	if (!c || !path) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
+	c->u2_valid = false;
 	try {
 		HIP_CHECK(hipStreamSynchronize(c->stream));
 		File f(path, "rb");

@@ -9,12 +9,16 @@
 //   current of the same move (addParticleFluxZigzag, momentum_collecting.cpp:85-249).
 //
 // The zig-zag current is summed without floating-point atomics: every contribution is
-// rounded to a 64-bit fixed-point value (a power-of-two scale chosen per step from the
-// largest number of particles in a cell, so that no entry can overflow) and added with
-// integer atomics. Integer addition is associative, so the sums, and everything after them,
-// are the same in every run whatever the order of the atomics. Each contribution is rounded
-// by at most half a unit of 2^-62 times the bound on an entry, far below FP64 rounding of the
-// entry itself at these counts.
+// rounded to a 64-bit fixed-point value and added with integer atomics. Integer addition is
+// associative, so the sums, and everything after them, are the same in every run whatever the
+// order of the atomics. The scale, a power of two, is chosen per step from the largest number
+// of particles in a cell and a bound on the particles' new speeds (their largest |u| before the
+// step, as the previous push recorded it, plus what the largest |E| can add), so that no entry
+// can overflow: each contribution is rounded by at most half a unit of 2^-62 times that bound
+// on an entry. With m particles in the fullest cell and speeds up to v_max that unit is about
+// 2^-53 m v_max |q w / V|, against contributions of |q w / V| |v|: the relative rounding of a
+// particle's contribution is about 1e-16 m v_max / |v|, and the sums' errors average out over
+// the particles of an entry.
 #include <math.h>
 
 #include "pic_explicit.hpp"
@@ -169,15 +173,107 @@ __global__ void k_fixed_to_double(const long long* __restrict__ acc, double* __r
 	J[t] = (double)acc[t] * inv_scale;
 }
 
+// A 64-bit DPP move: lanes whose source is outside the pattern or whose row is masked off
+// read 0.
+template <int CTRL, int ROWS>
+__device__ inline long long dpp64(long long v) {
+	const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(unsigned long long)v, CTRL, ROWS, 0xF, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)((unsigned long long)v >> 32), CTRL, ROWS, 0xF, false);
+	return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ inline double wave_max(double v) {
+	for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+	return v;
+}
+
+// The largest of the wave's 64 non-negative doubles, given and returned as bit patterns (which
+// order like the values), in every lane: the DPP steps of wave_run_sums below with max for the
+// sum (a lane whose source is outside a pattern reads 0, which changes no maximum), after which
+// lane 63 holds the maximum. Needs all 64 lanes of the wave active.
+__device__ inline long long wave_max_bits(long long b) {
+	b = max(b, dpp64<0x111, 0xF>(b));  // row_shr:1
+	b = max(b, dpp64<0x112, 0xF>(b));  // row_shr:2
+	b = max(b, dpp64<0x114, 0xF>(b));  // row_shr:4
+	b = max(b, dpp64<0x118, 0xF>(b));  // row_shr:8
+	b = max(b, dpp64<0x142, 0xA>(b));  // row_bcast:15
+	b = max(b, dpp64<0x143, 0xC>(b));  // row_bcast:31
+	const int lo = __builtin_amdgcn_readlane((int)(unsigned)(unsigned long long)b, 63);
+	const int hi = __builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)b >> 32), 63);
+	return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ inline int wave_max(int v) {
+	for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
+	return v;
+}
+
+// Largest particle count of a cell: one atomic per wave.
 __global__ void k_max_cell_count(Grid g, int nspecies, const int* __restrict__ start, int* __restrict__ out) {
 	long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= g.ncells) return;
 	int n = 0;
-	for (int s = 0; s < nspecies; ++s) {
-		const long long k0 = ((long long)s * g.ncells + c) << 3;
-		n += start[k0 + 8] - start[k0];
+	if (c < g.ncells)
+		for (int s = 0; s < nspecies; ++s) {
+			const long long k0 = ((long long)s * g.ncells + c) << 3;
+			n += start[k0 + 8] - start[k0];
+		}
+	n = wave_max(n);
+	if ((threadIdx.x & 63) == 0 && n > 0) atomicMax(out, n);
+}
+
+// Largest |v| of n values into *out (the bits of a non-negative double): a grid-stride loop,
+// then one atomic per wave.
+__global__ void k_abs_max(const double* __restrict__ v, long long n, unsigned long long* __restrict__ out) {
+	double m = 0;
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+		m = fmax(m, fabs(v[i]));
+	m = wave_max(m);
+	if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// Per species the largest |u|^2 of its particles: the wave's maximum with one atomic when all
+// its lanes hold one species (lanes without a particle pass species -1), else one atomic per
+// lane. The atomics go to one of kU2Slots rows picked by the workgroup (a single address taking
+// one atomic from every wave of a 268 M particle launch serialises them: 48 ms instead of 17 ms
+// for the push), folded by k_fold_u2_slots.
+__device__ inline void species_u2_max(int s, double u2, unsigned long long* slots) {
+	unsigned long long* row = slots + (blockIdx.x % kU2Slots) * kMaxSpecies;
+	const int s0 = __builtin_amdgcn_readfirstlane(s);
+	if (__all(s == s0)) {
+		const long long m = wave_max_bits(__double_as_longlong(u2));
+		if ((threadIdx.x & 63) == 0 && s0 >= 0) atomicMax(row + s0, (unsigned long long)m);
+	} else if (s >= 0) {
+		atomicMax(row + s, (unsigned long long)__double_as_longlong(u2));
 	}
-	if (n > 0) atomicMax(out, n);
+}
+
+// The slots' maxima per species into out[kMaxSpecies]: one workgroup of kU2Slots threads.
+__global__ void k_fold_u2_slots(const unsigned long long* __restrict__ slots, unsigned long long* __restrict__ out) {
+	__shared__ unsigned long long r[kU2Slots];
+	for (int s = 0; s < kMaxSpecies; ++s) {
+		r[threadIdx.x] = slots[threadIdx.x * kMaxSpecies + s];
+		__syncthreads();
+		for (int h = kU2Slots / 2; h > 0; h >>= 1) {
+			if ((int)threadIdx.x < h) r[threadIdx.x] = max(r[threadIdx.x], r[threadIdx.x + h]);
+			__syncthreads();
+		}
+		if (threadIdx.x == 0) out[s] = r[0];
+		__syncthreads();
+	}
+}
+
+__global__ void k_species_u2(SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start,
+                             unsigned long long* __restrict__ out) {
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	int s = -1;
+	double u2 = 0;
+	if (i < P.count) {
+		s = 0;
+		while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+		const double p0 = P.p[0][i], p1 = P.p[1][i], p2 = P.p[2][i];
+		u2 = (p0 * p0 + p1 * p1 + p2 * p2) / (sp.mass[s] * sp.mass[s]);
+	}
+	species_u2_max(s, u2, out);
 }
 
 // The workgroup's current box in LDS. A workgroup's 256 particles are consecutive in the
@@ -211,6 +307,7 @@ struct TristanConsts {
 	double fcd[kMaxSpecies];     // q w / V
 	double fcd_dt[kMaxSpecies];  // q w / (V dt)
 	double scale;                // fixed-point scale of the current, a power of two
+	double u2_limit[kMaxSpecies];  // the new |u|^2 the scale allows
 };
 
 // v * scale rounded to the nearest integer, ties to even (llrint). The scale is a power of two,
@@ -289,16 +386,9 @@ struct PartSum {
 	int a, b, c;
 	long long q[12];
 	bool moved;  // the particle's sort key changed
+	int species;  // -1 for no particle
+	double u2;    // |p|^2 / m^2 after the move
 };
-
-// A 64-bit DPP move: lanes whose source is outside the pattern or whose row is masked off
-// read 0.
-template <int CTRL, int ROWS>
-__device__ inline long long dpp64(long long v) {
-	const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(unsigned long long)v, CTRL, ROWS, 0xF, false);
-	const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)((unsigned long long)v >> 32), CTRL, ROWS, 0xF, false);
-	return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
 
 template <int CTRL, int ROWS>
 __device__ inline void seg_step(long long q[12], bool take) {
@@ -330,7 +420,8 @@ __device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const Pa
 
 __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start,
                                                       YeeGrid y, double dt, TristanConsts K, uint32_t* __restrict__ keys,
-                                                      uint64_t* __restrict__ mask, int* __restrict__ status) {
+                                                      uint64_t* __restrict__ mask, int* __restrict__ status,
+                                                      unsigned long long* __restrict__ u2_slots) {
 	__shared__ long long box[kJBoxX + kJBoxY + kJBoxZ];
 	__shared__ int row_ends[2][3];
 	const long long b0 = (long long)blockIdx.x * blockDim.x;
@@ -352,7 +443,10 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 	PartSum ps;
 	ps.cell = -1;
 	ps.moved = false;
+	ps.species = -1;
+	ps.u2 = 0;
 	tristan_particle(g, sp, P, species_start, y, dt, K, keys, status, idx, box, jb, ps);
+	species_u2_max(ps.species, ps.u2, u2_slots);
 	{
 		const int lane = (int)(threadIdx.x & 63);
 		// one mover bit per particle: bit i & 63 of word i >> 6 (waves are 64 consecutive
@@ -487,7 +581,11 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 	double c2[3] = {tm[1] * Bf[2] - tm[2] * Bf[1], -tm[0] * Bf[2] + tm[2] * Bf[0], tm[0] * Bf[1] - tm[1] * Bf[0]};
 	const double fb = f * bs;
 	for (int a = 0; a < 3; ++a) p[a] += c2[a] * fb + dp[a];
-	const double gn = sqrt((p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * K.inv_m2[s] + 1.0);
+	const double u2 = (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * K.inv_m2[s];
+	ps.species = s;
+	ps.u2 = u2;
+	if (u2 > K.u2_limit[s]) atomicOr(status, 16);  // beyond what the current's scale was set for
+	const double gn = sqrt(u2 + 1.0);
 	const double iv = 1.0 / (m * gn);
 	double v[3];
 	for (int a = 0; a < 3; ++a) {
@@ -603,13 +701,22 @@ void launch_yee_current(const Grid& g, const YeeGrid& y, double inv_scale, hipSt
 	yee_axes(g, q, true, s);
 }
 
-void launch_max_cell_count(const Grid& g, int nspecies, const int* start, int* out, hipStream_t s) {
-	hipLaunchKernelGGL(k_max_cell_count, dim3(nb(g.ncells, 256)), dim3(256), 0, s, g, nspecies, start, out);
+void launch_explicit_bounds(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
+                            const int* start, const YeeGrid& y, bool particles, unsigned long long* u2_slots,
+                            ExplicitBounds* out, hipStream_t s) {
+	hipLaunchKernelGGL(k_max_cell_count, dim3(nb(g.ncells, 256)), dim3(256), 0, s, g, sp.count, start, &out->maxcount);
+	for (int c = 0; c < 3; ++c) {
+		const long long n = yee_count(y.dE[c]);
+		hipLaunchKernelGGL(k_abs_max, dim3(std::min(nb(n, 256), 1024u)), dim3(256), 0, s, y.E[c], n, out->emax + c);
+	}
+	if (particles && P.count > 0)
+		hipLaunchKernelGGL(k_species_u2, dim3(nb(P.count, 256)), dim3(256), 0, s, sp, P, species_start, u2_slots);
+	hipLaunchKernelGGL(k_fold_u2_slots, dim3(1), dim3(kU2Slots), 0, s, u2_slots, out->u2max);
 }
 
 void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
-                         const YeeGrid& y, double dt, double scale, uint32_t* keys, uint64_t* mask, int* status,
-                         hipStream_t s) {
+                         const YeeGrid& y, double dt, double scale, const double* u2_limit, uint32_t* keys, uint64_t* mask,
+                         int* status, unsigned long long* u2max_out, hipStream_t s) {
 	if (P.count == 0) return;
 	TristanConsts K{};
 	for (int a = 0; a < 3; ++a) {
@@ -625,8 +732,9 @@ void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleAr
 		K.fcd_dt[q] = K.fcd[q] / dt;
 	}
 	K.scale = scale;
+	for (int q = 0; q < kMaxSpecies; ++q) K.u2_limit[q] = q < sp.count ? u2_limit[q] : 0.0;
 	hipLaunchKernelGGL(k_tristan_push, dim3(nb(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, y, dt, K, keys, mask,
-	                   status);
+	                   status, u2max_out);
 }
 
 }  // namespace picgpu

@@ -30,13 +30,32 @@ void launch_tristan_e(const Grid& g, const YeeGrid& y, double dt, hipStream_t s)
 void launch_yee_exchange(const Grid& g, const YeeGrid& y, bool b_field, hipStream_t s);
 // the fixed-point current to J, then exchangeBunemanFlux's periodic folds
 void launch_yee_current(const Grid& g, const YeeGrid& y, double inv_scale, hipStream_t s);
-// largest number of particles of all species in one cell (start: segment starts per key)
-void launch_max_cell_count(const Grid& g, int nspecies, const int* start, int* out, hipStream_t s);
+// What the fixed-point scale of the explicit push's current is set from: the largest number of
+// particles of all species in one cell, per species the largest |u|^2 = |p|^2 / m^2 of its
+// particles, and per component the largest |E| on the Yee grid (ghosts included). The maxima of
+// non-negative doubles are kept as their bit patterns, which order like the values.
+struct ExplicitBounds {
+	int maxcount;
+	int pad;
+	unsigned long long u2max[kMaxSpecies];
+	unsigned long long emax[3];
+};
+// The per-species maxima are gathered in kU2Slots rows of kMaxSpecies (atomics spread over the
+// rows by workgroup) and folded into ExplicitBounds::u2max.
+constexpr int kU2Slots = 256;
+// Fills out (cleared by the caller): the cell count (start: segment starts per key) and |E|;
+// when `particles`, the species' |u|^2 from the particles into u2_slots (cleared by the caller;
+// else the slots hold what the last push left); then u2max from the slots.
+void launch_explicit_bounds(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
+                            const int* start, const YeeGrid& y, bool particles, unsigned long long* u2_slots,
+                            ExplicitBounds* out, hipStream_t s);
 // moveParticleTristan for every particle, the zig-zag current of the move into Jacc (fixed
 // point, scale a power of two), the one-rank periodic wrap, the new sort key and one mover bit
-// per particle in mask (bit i & 63 of word i >> 6: the key changed)
+// per particle in mask (bit i & 63 of word i >> 6: the key changed). Per species, the largest
+// new |u|^2 goes to the rows of u2_slots (cleared by the caller) for the next step's scale; a particle whose
+// new |u|^2 exceeds u2_limit[species] (the bound the scale was set for) sets status bit 16.
 void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
-                         const YeeGrid& y, double dt, double scale, uint32_t* keys, uint64_t* mask, int* status,
-                         hipStream_t s);
+                         const YeeGrid& y, double dt, double scale, const double* u2_limit, uint32_t* keys, uint64_t* mask,
+                         int* status, unsigned long long* u2_slots, hipStream_t s);
 
 }  // namespace picgpu

@@ -131,6 +131,44 @@ def test_explicit_step_matches_oracle_with_workgroup_boxes(n, ppc, steps):
     sim.close()
 
 
+def test_explicit_cold_plasma_matches_oracle():
+    """A cold plasma (kT/mc^2 = 1e-4: electrons at u_th = 0.01, protons at 0.001) at 64 ppc per
+    species. The fixed-point quantum of the current follows the particles' actual speeds (their
+    largest |u| plus what the largest |E| can add in a step), not the speed of light, so the small
+    contributions of slow particles keep FP64 accuracy: J, E, B, x and p within 1e-12 of the
+    oracle's FP64 sums."""
+    n, ppc, steps = (16, 16, 16), 64, 3
+    w = wl.Workload("explicit-cold-%dx%dx%d-ppc%d" % (n + (ppc,)), n, 1.0, 0.4, steps, wl.electron_proton(ppc, 1e-4, 100.0),
+                    b0=(wl.b_for_gyro(0.5), 0.0, 0.0), seed=9)
+    st = wl.make_initial_state(w)
+    yee = wl.yee_initial_fields(w, perturb=0.01)
+    print("case", w.name, "particles", len(st.ids), flush=True)
+    sim = Simulation(w, precision="fp64", scheme="explicit")
+    sim.load_state(st)
+    sim.load_yee(yee)
+    o = OracleSim(w, st)
+    o.yee_init()
+    for name in wl.YEE_NAMES:
+        o.set_yee(name, yee[name])
+    o.yee_exchange()
+    worst = {}
+    for step in range(1, steps + 1):
+        sim.explicit_step()
+        o.explicit_step()
+        p = sim.particles()
+        ids, x, mom = by_id(p["ids"], p["x"], p["p"])
+        rids, rx, rp = by_id(*(o.particles()[k] for k in ("ids", "x", "p")))
+        assert np.array_equal(ids, rids), step
+        worst["x"] = max(worst.get("x", 0.0), relerr(x, rx))
+        worst["p"] = max(worst.get("p", 0.0), relerr(mom, rp))
+        for name in wl.YEE_NAMES + ("Jx", "Jy", "Jz"):
+            worst[name] = max(worst.get(name, 0.0), relerr(sim.yee(name), o.yee(name)))
+    print("cold plasma, explicit step vs oracle:", {k: "%.1e" % v for k, v in worst.items()})
+    for k, v in worst.items():
+        assert v <= 1e-12, (k, v)
+    sim.close()
+
+
 def test_explicit_step_without_particles():
     # no particles: the Yee leapfrog alone (a vacuum), against the oracle; the current stays 0
     w, st, yee = box_case((6, 5, 7), 1, 3)
