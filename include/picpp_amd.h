@@ -371,6 +371,31 @@ typedef struct pic_backup_info {
 int pic_write_backup(pic_ctx* ctx, const pic_backup_info* info, const char* general_file, const char* e_file,
                      const char* b_file, const char* particles_file);
 
+/* ---- general.dat rows ----
+   The numbers of one row of the reference's general.dat (outputGeneral, PIC++/output.cpp:3215-3250)
+   for the context's current state, as the step loop keeps them (updateEnergy,
+   PIC++/simulation.cpp:1506-1640; updateParameters :1707-1880; the theoretical energy and momentum
+   of updateTheoreticalEnergy :1399-1504 and of the mover's reflections, the injection and the
+   replacement of escapes, PIC++/mover.cpp:256, 994, PIC++/initialization.cpp:8319). The context
+   starts the theoretical totals from the energy and momentum of the state that the first step,
+   pic_initialize or pic_general after an upload finds, and updates them in every pic_push and
+   pic_explicit_step. out[PIC_GENERAL_VALUES], in the row's order without the iteration and time
+   columns, with the row's unit factors (energies x (scale_factor / plasma_period)^2, momenta x
+   scale_factor / plasma_period, field maxima and rms over plasma_period sqrt(scale_factor)):
+     0 particle energy, 1 E energy, 2 B energy, 3 total energy, 4-6 total momentum,
+     7 theoretical energy, 8-10 theoretical momentum, 11 largest |E|, 12 largest |B|,
+     13 particles numbered so far (particlesNumber), 14-16 rms of E_x, E_y, E_z,
+     17-19 E_x, E_y, E_z energies, 20-22 B_x, B_y, B_z energies, 23-25 field momentum,
+     26-28 particle momentum.
+   From info: plasma_period, scale_factor, B0 (updateParameters starts its |B| maximum from
+   B[0][0][0] - B0), time and current_iteration (pic_write_general only). */
+#define PIC_GENERAL_VALUES 29
+int pic_general(pic_ctx* ctx, const pic_backup_info* info, double* out);
+/* Appends the row to general_file in outputGeneral's format (two printf calls, "%15.10g"
+   fields); shockWaveX (-1) and the derivative points (0) keep their initial values, their
+   updates being switched off in the reference's loop (PIC++/simulation.cpp:108-111). */
+int pic_write_general(pic_ctx* ctx, const pic_backup_info* info, const char* general_file);
+
 /* ---- Learned preconditioner ----
    No counterpart in the reference, whose GMRES is unpreconditioned
    (PIC++/specialmath.cpp:367-671). A 3-D convolution U-Net (3 levels, 16/32/64 channels,

@@ -156,6 +156,20 @@ void dump_energy(Simulation& s, int step) {
 	dump(tag("energy", step).c_str(), d, {7});
 }
 
+// The numbers of one general.dat row (outputGeneral, PIC++/output.cpp:3215-3250), unscaled, in
+// the row's order without the iteration and the time columns.
+void dump_general(Simulation& s, int step) {
+	std::vector<double> d = {s.particleEnergy, s.electricFieldEnergy, s.magneticFieldEnergy, s.energy,
+	                         s.globalMomentum.x, s.globalMomentum.y, s.globalMomentum.z, s.generalTheoreticalEnergy,
+	                         s.generalTheoreticalMomentum.x, s.generalTheoreticalMomentum.y, s.generalTheoreticalMomentum.z,
+	                         s.maxEfield.norm(), s.maxBfield.norm(), (double)s.particlesNumber, s.meanSquaredEfield[0],
+	                         s.meanSquaredEfield[1], s.meanSquaredEfield[2], s.electricFieldEnergyX, s.electricFieldEnergyY,
+	                         s.electricFieldEnergyZ, s.magneticFieldEnergyX, s.magneticFieldEnergyY, s.magneticFieldEnergyZ,
+	                         s.electromagneticMomentum.x, s.electromagneticMomentum.y, s.electromagneticMomentum.z,
+	                         s.particleMomentum.x, s.particleMomentum.y, s.particleMomentum.z};
+	dump(tag("general", step).c_str(), d, {d.size()});
+}
+
 template <class T>
 T rd(FILE* f) {
 	T v;
@@ -310,6 +324,12 @@ int main(int argc, char** argv) {
 	// wave number of filterFields (> 0: filterFields(cut) next)
 	int32_t cleanup = 0, filter_cut = 0;
 	if (fread(&cleanup, sizeof(int32_t), 1, in) == 1) filter_cut = rd<int32_t>(in);
+	// optional trailer: general rows (1: the loop's diagnostics for general.dat, as simulate()
+	// keeps them: updateParameters in the pre-loop sequence and after every step,
+	// updateTheoreticalEnergy between eraseEscapedPaticles and exchangeParticles, outputGeneral
+	// after every updateEnergy; PIC++/simulation.cpp:96, 138-148, 266-272, 381-384)
+	int32_t general = 0;
+	if (fread(&general, sizeof(int32_t), 1, in) != 1) general = 0;
 	fclose(in);
 	s.particlesNumber = (int)np;
 	if (argc > 3) {
@@ -330,12 +350,20 @@ int main(int argc, char** argv) {
 	s.exchangeGeneralBfield(s.Bfield);
 	s.exchangeGeneralBfield(s.newBfield);
 	s.updateParticleCorrelationMaps();
+	if (general) s.updateParameters();
 	s.evaluateParticlesRotationTensor();
 	s.updateElectroMagneticParameters();
 	s.updateDensityParameters();
 	s.updateFields();
 	s.currentIteration = 0;
 	s.updateEnergy();
+	const std::string general_file = std::string(argv[2]) + ".general";
+	if (general) {
+		FILE* gf = fopen(general_file.c_str(), "w");  // outputGeneral appends
+		if (gf) fclose(gf);
+		outputGeneral(general_file.c_str(), &s);
+		dump_general(s, 0);
+	}
 	dump_particles(s, tag("particles", 0));
 	dump_node_vec(s, s.Efield, tag("E", 0));
 	dump_cell_vec(s, s.Bfield, tag("B", 0));
@@ -376,6 +404,7 @@ int main(int argc, char** argv) {
 		}
 		s.moveParticles();
 		s.eraseEscapedPaticles();
+		if (general) s.updateTheoreticalEnergy();
 		s.exchangeParticles();
 		if (inject == 1) {  // PIC++/simulation.cpp:281-289
 			for (int t = 0; t < s.typesNumber; ++t) s.types[t].injectionLength += fabs(s.V0.x * s.deltaT);
@@ -396,10 +425,15 @@ int main(int argc, char** argv) {
 		s.exchangeGeneralEfield(s.newEfield);
 		s.exchangeGeneralEfield(s.Efield);
 		s.exchangeGeneralBfield(s.Bfield);
+		if (general) s.updateParameters();
 		s.removeEscapedParticles();
 		s.time += s.deltaT;
 		s.currentIteration++;
 		s.updateEnergy();
+		if (general) {
+			outputGeneral(general_file.c_str(), &s);
+			dump_general(s, step);
+		}
 		dump_particles(s, tag("particles", step));
 		if (inject == 1) {  // reference type of every particle, in the same order
 			std::vector<double> ty;

@@ -54,6 +54,9 @@ class PicSolveInfo(ctypes.Structure):
                 ("fallbacks", ctypes.c_int)]
 
 
+GENERAL_VALUES = 29  # PIC_GENERAL_VALUES
+
+
 class PicBackupInfo(ctypes.Structure):
     _fields_ = [("input_type", ctypes.c_int), ("particles_per_bin", ctypes.c_int * 6),
                 ("concentration", ctypes.c_double * 6), ("temperature", ctypes.c_double),
@@ -109,6 +112,8 @@ SIGNATURES = {
     "pic_checkpoint_save": (ctypes.c_int, [_V, ctypes.c_char_p]),
     "pic_checkpoint_load": (ctypes.c_int, [_V, ctypes.c_char_p]),
     "pic_write_backup": (ctypes.c_int, [_V, ctypes.POINTER(PicBackupInfo)] + [ctypes.c_char_p] * 4),
+    "pic_general": (ctypes.c_int, [_V, ctypes.POINTER(PicBackupInfo), ctypes.POINTER(ctypes.c_double)]),
+    "pic_write_general": (ctypes.c_int, [_V, ctypes.POINTER(PicBackupInfo), ctypes.c_char_p]),
 }
 
 _lib = None

@@ -27,6 +27,7 @@
 #include "pic_precond.hpp"
 #include "pic_solver.hpp"
 #include "pic_cleanup.hpp"
+#include "pic_general.hpp"
 
 using namespace picgpu;
 
@@ -102,6 +103,15 @@ struct pic_ctx {
 	ExplicitBounds* bounds = nullptr;
 	unsigned long long* u2_slots = nullptr;  // kU2Slots x kMaxSpecies
 	bool u2_valid = false;
+	// general.dat bookkeeping: theo = the theoretical energy and momentum (4 doubles, device),
+	// formed from the energies when the run starts (theo_init) and updated by every push;
+	// gpartial / gout the field sums of pic_general; bounce one bit per particle reflected at
+	// the wall by the last push
+	double* theo = nullptr;
+	bool theo_init = false;
+	double* gpartial = nullptr;
+	double* gout = nullptr;
+	uint64_t* bounce = nullptr;
 	void* sort_temp = nullptr;
 	size_t sort_temp_bytes = 0;
 	std::vector<int> species_start_h;
@@ -338,6 +348,15 @@ void wall_particles(pic_ctx* c) {
 	HIP_CHECK(hipStreamSynchronize(s));
 	HIP_CHECK(hipMemsetAsync(c->dead, 0, sizeof(int), s));
 	bool resized = removed > 0;
+	const int escaped = removed;
+	if (escaped > 0) {
+		// the escapes in index order (the push lists them in atomic order), and their energy and
+		// momentum out of the theoretical totals (updateTheoreticalEnergy's escapedParticlesRight)
+		size_t bytes = c->sort_temp_bytes;
+		HIP_CHECK(rocprim::radix_sort_keys(c->sort_temp, bytes, c->esc, c->esc_sorted, (size_t)escaped, 0, c->esc_bits, s));
+		launch_theo_particles(c->sp, c->P, c->Q, c->fp32, c->species_start, c->keys, c->g.ncells, c->esc_sorted, 0, escaped, -1.0,
+		                      c->theo, s);
+	}
 	if (c->cfg.inject) {
 		long long n = c->np();
 		InjectPhysics ph = inject_physics(c);
@@ -356,6 +375,9 @@ void wall_particles(pic_ctx* c) {
 			for (const InjectSlabs& b : plan) {
 				ph.id_base = (int)c->next_id;
 				launch_inject(c->g, c->sp, b, ph, c->P, c->Q, c->fp32, at, c->keys, s);
+				// the new particles into the theoretical totals (injectNewParticles, PIC++/mover.cpp:994-996)
+				launch_theo_particles(c->sp, c->P, c->Q, c->fp32, c->species_start, c->keys, c->g.ncells, nullptr, at,
+				                      b.count * ncols, 1.0, c->theo, s);
 				at += b.count * ncols;
 				c->next_id += b.count * ncols;
 			}
@@ -367,10 +389,11 @@ void wall_particles(pic_ctx* c) {
 		}
 		if (removed > 0) {
 			Timed t(c, "replace", 0);
-			size_t bytes = c->sort_temp_bytes;
-			HIP_CHECK(rocprim::radix_sort_keys(c->sort_temp, bytes, c->esc, c->esc_sorted, (size_t)removed, 0, c->esc_bits, s));
 			ph.id_base = (int)c->next_id;
 			launch_replace(c->g, c->sp, ph, c->species_start, c->esc_sorted, removed, c->P, c->Q, c->fp32, c->keys, s);
+			// addToPreserveChargeGlobal's new particles (PIC++/initialization.cpp:8319-8321)
+			launch_theo_particles(c->sp, c->P, c->Q, c->fp32, c->species_start, c->keys, c->g.ncells, c->esc_sorted, 0, removed,
+			                      1.0, c->theo, s);
 			c->next_id += removed;
 			removed = 0;
 		}
@@ -746,6 +769,45 @@ int gmres_cycle(pic_ctx* c, double* b, double* x, double target, int max_iterati
 	return gmres_cycle_on(c, K, b, x, target, max_iterations, dim, rel);
 }
 
+// updateEnergy's sums (PIC++/simulation.cpp:1506-1640) without the unit factors: h[0] E energy,
+// h[1] B energy, h[2..4] field momentum, h[5] particle energy, h[6..8] particle momentum. The
+// explicit scheme takes the Yee components at one (i, j, k) as the reference does
+// (getBunemanElectricField / getBunemanMagneticField, PIC++/fields.cpp:3429-3435).
+void energy_sums(pic_ctx* c, double h[9]) {
+	hipStream_t s = c->stream;
+	const Grid& g = c->g;
+	if (!c->explicit_scheme) launch_field_energy(g, c->E, c->B, c->epartial, s);
+	if (c->fp32)
+		launch_particle_energy32(c->sp, c->Q, c->species_start, c->epartial + 5 * kEnergyBlocks, s);
+	else
+		launch_particle_energy(c->sp, c->P, c->species_start, c->epartial + 5 * kEnergyBlocks, s);
+	launch_rows_final(c->epartial, kEnergyBlocks, 9, c->eout, s);
+	HIP_CHECK(hipMemcpyAsync(h, c->eout, 9 * sizeof(double), hipMemcpyDeviceToHost, s));
+	double gr[kGeneralRows] = {};
+	if (c->explicit_scheme) {
+		launch_general_fields(g, yee_view(c->yee), c->gpartial, c->gout, s);
+		HIP_CHECK(hipMemcpyAsync(gr, c->gout, sizeof(gr), hipMemcpyDeviceToHost, s));
+	}
+	HIP_CHECK(hipStreamSynchronize(s));
+	if (c->explicit_scheme) {
+		const double pi = 4 * std::atan2(1.0, 1.0);
+		h[0] = (gr[0] + gr[1] + gr[2]) * g.volume / (8 * pi);
+		h[1] = (gr[3] + gr[4] + gr[5]) * g.volume / (8 * pi);
+		for (int a = 0; a < 3; ++a) h[2 + a] = gr[6 + a] * g.volume / (4 * pi);
+	}
+}
+
+// The theoretical energy and momentum at the start of the run: the energy and the total
+// momentum (updateEnergy at iteration 0, PIC++/simulation.cpp:1630-1633).
+void init_theo(pic_ctx* c) {
+	double h[9];
+	energy_sums(c, h);
+	const double t[4] = {h[5] + h[0] + h[1], h[2] + h[6], h[3] + h[7], h[4] + h[8]};
+	HIP_CHECK(hipMemcpyAsync(c->theo, t, sizeof(t), hipMemcpyHostToDevice, c->stream));
+	HIP_CHECK(hipStreamSynchronize(c->stream));
+	c->theo_init = true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -906,6 +968,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		{  // the rebin after a push with mover bits
 			const long long words = c->capacity / 64 + 2;
 			c->mask = dalloc<uint64_t>(words);
+			if (g.wall) c->bounce = dalloc<uint64_t>(words);
 			c->wcount = dalloc<int>(words);
 			c->wprefix = dalloc<int>(words);
 			c->start_new = dalloc<int>(c->nkeys + 1);
@@ -985,6 +1048,10 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->ydev = dalloc<double>(128);
 		c->epartial = dalloc<double>(9 * kEnergyBlocks);
 		c->eout = dalloc<double>(16);
+		c->theo = dalloc<double>(4);
+		HIP_CHECK(hipMemset(c->theo, 0, 4 * sizeof(double)));
+		c->gpartial = dalloc<double>((size_t)kGeneralRows * kEnergyBlocks);
+		c->gout = dalloc<double>(kGeneralRows);
 		HIP_CHECK(hipDeviceSynchronize());
 	} catch (const DeviceError& e) {
 		std::string msg = e.what;
@@ -1008,7 +1075,8 @@ void pic_destroy(pic_ctx* c) {
 	                (void*)c->rho_scratch, (void*)c->pi_scratch, (void*)c->dens_scratch, (void*)c->keys, (void*)c->keys_sorted,
 	                (void*)c->perm, (void*)c->perm_sorted, (void*)c->start, (void*)c->counts, c->sort_temp,
 	                (void*)c->species_start, (void*)c->status, (void*)c->dead, (void*)c->dens_flip, (void*)c->dens_zero, (void*)c->esc, (void*)c->esc_sorted, (void*)c->smooth_tmp, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
-	                (void*)c->Vdev, (void*)c->partial, (void*)c->dscal, (void*)c->ydev, (void*)c->epartial, (void*)c->eout})
+	                (void*)c->Vdev, (void*)c->partial, (void*)c->dscal, (void*)c->ydev, (void*)c->epartial, (void*)c->eout,
+	                (void*)c->theo, (void*)c->gpartial, (void*)c->gout, (void*)c->bounce})
 		f(p);
 	for (ParticleArrays* P : {&c->P, &c->Palt}) {
 		for (int a = 0; a < 3; ++a) {
@@ -1058,6 +1126,7 @@ int pic_upload_fields(pic_ctx* c, const double* E, const double* B) {
 	if (!c || !E || !B) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
 	c->u2_valid = false;
+	c->theo_init = false;  // the run starts again from this state
 	try {
 		const Grid& g = c->g;
 		HIP_CHECK(hipMemcpyAsync(c->E, E, 3 * g.nnodes * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1139,6 +1208,7 @@ int pic_upload_particles(pic_ctx* c, long long np, const int* species, const int
 	}
 	c->fused = 0;
 	c->u2_valid = false;
+	c->theo_init = false;  // the run starts again from this state
 	double* dx = nullptr;
 	double* dp = nullptr;
 	int* di = nullptr;
@@ -1184,6 +1254,7 @@ int pic_upload_particles_device(pic_ctx* c, const long long* counts, const doubl
 	if (!c || !counts || !x || !p) return PIC_ERR_ARGUMENT;
 	c->fused = 0;
 	c->u2_valid = false;
+	c->theo_init = false;  // the run starts again from this state
 	try {
 		std::vector<int> cnt(c->sp.count + 1, 0);
 		long long np = 0;
@@ -1314,6 +1385,7 @@ int pic_initialize(pic_ctx* c) {
 		HIP_CHECK(hipMemcpyAsync(c->E, c->Enew, 3 * g.nnodes * sizeof(double), hipMemcpyDeviceToDevice, s));
 		HIP_CHECK(hipMemcpyAsync(c->B, c->Bnew, 3 * g.ncells * sizeof(double), hipMemcpyDeviceToDevice, s));
 		HIP_CHECK(hipStreamSynchronize(s));
+		init_theo(c);
 	} catch (const DeviceError& e) {
 		return fail(c, e);
 	}
@@ -1744,6 +1816,7 @@ int pic_push(pic_ctx* c) {
 	c->fused = 0;
 	c->u2_valid = false;
 	try {
+		if (!c->theo_init) init_theo(c);
 		if (c->fp32) {
 			const Grid& g = c->g;
 			{
@@ -1756,13 +1829,24 @@ int pic_push(pic_ctx* c) {
 			// per-particle unit of the step; the key, the mover bits and the field reads are extra)
 			Timed t(c, "push", (double)c->np() * 56);
 			launch_push32(g, c->sp, c->Q, c->species_start, c->E32, c->Et32, c->B32, (float)c->cfg.dt, (float)c->eta, c->keys,
-			              c->mask, c->status, c->dead, c->esc, c->stream);
+			              c->mask, c->status, c->dead, c->esc, c->bounce, c->stream);
 		} else {
 			Timed t(c, "push", (double)c->P.count * 96);  // 2 x 48 B of particle state
 			launch_push(c->g, c->sp, c->P, c->species_start, c->E, c->Etheta, c->B, c->cfg.dt, c->eta, c->keys, c->mask, c->status,
-			            c->dead, c->esc, c->stream);
+			            c->dead, c->esc, c->bounce, c->stream);
 		}
 		check_status(c);
+		{
+			// the theoretical energy and momentum (updateTheoreticalEnergy, PIC++/simulation.cpp:1399-1504,
+			// after the mover; E and B are still E^n, B^n): the wall's reflections, then the
+			// Poynting flux through the x boundaries (zero with periodic x: the same values on both
+			// sides); escapes and injected particles follow in wall_particles
+			Timed t(c, "theoretical_energy", 0);
+			if (c->g.wall)
+				launch_theo_bounce(c->sp, c->P, c->Q, c->fp32, c->species_start, c->bounce, c->np(), c->epartial, c->theo,
+				                   c->stream);
+			launch_theo_poynting(c->g, implicit_view(c->g, c->E, c->B), c->cfg.dt, c->theo, c->stream);
+		}
 		if (c->g.wall)  // removals, replacements and injection, then the rebin
 			wall_particles(c);
 		else
@@ -1790,16 +1874,8 @@ int pic_commit_fields(pic_ctx* c) {
 int pic_energy(pic_ctx* c, double* out7) {
 	if (!c || !out7) return PIC_ERR_ARGUMENT;
 	try {
-		hipStream_t s = c->stream;
-		launch_field_energy(c->g, c->E, c->B, c->epartial, s);
-		if (c->fp32)
-			launch_particle_energy32(c->sp, c->Q, c->species_start, c->epartial + 5 * kEnergyBlocks, s);
-		else
-			launch_particle_energy(c->sp, c->P, c->species_start, c->epartial + 5 * kEnergyBlocks, s);
-		launch_rows_final(c->epartial, kEnergyBlocks, 9, c->eout, s);
 		double h[9];
-		HIP_CHECK(hipMemcpyAsync(h, c->eout, 9 * sizeof(double), hipMemcpyDeviceToHost, s));
-		HIP_CHECK(hipStreamSynchronize(s));
+		energy_sums(c, h);
 		out7[0] = h[5];
 		out7[1] = h[0];
 		out7[2] = h[1];
@@ -1813,12 +1889,94 @@ int pic_energy(pic_ctx* c, double* out7) {
 	return PIC_OK;
 }
 
+int pic_general(pic_ctx* c, const pic_backup_info* info, double* out) {
+	if (!c || !info || !out) return PIC_ERR_ARGUMENT;
+	try {
+		const Grid& g = c->g;
+		hipStream_t s = c->stream;
+		if (!c->theo_init) init_theo(c);
+		double h[9];
+		energy_sums(c, h);
+		const FieldView v = c->explicit_scheme ? yee_view(c->yee) : implicit_view(g, c->E, c->B);
+		launch_general_fields(g, v, c->gpartial, c->gout, s);
+		double gr[kGeneralRows], th[4], e000[3], b000[3];
+		HIP_CHECK(hipMemcpyAsync(gr, c->gout, sizeof(gr), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipMemcpyAsync(th, c->theo, sizeof(th), hipMemcpyDeviceToHost, s));
+		for (int a = 0; a < 3; ++a) {  // the values updateParameters starts from: E[0][0][0], B[0][0][0] - B0
+			HIP_CHECK(hipMemcpyAsync(&e000[a], v.e[a], sizeof(double), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipMemcpyAsync(&b000[a], v.b[a], sizeof(double), hipMemcpyDeviceToHost, s));
+		}
+		HIP_CHECK(hipStreamSynchronize(s));
+		const double pi = 4 * std::atan2(1.0, 1.0);
+		const double ms = info->scale_factor / info->plasma_period, es = ms * ms;
+		const double ff = info->plasma_period * std::sqrt(info->scale_factor);
+		double e0 = 0, b0 = 0;
+		for (int a = 0; a < 3; ++a) {
+			e0 += e000[a] * e000[a];
+			b0 += (b000[a] - info->B0[a]) * (b000[a] - info->B0[a]);
+		}
+		out[0] = h[5] * es;
+		out[1] = h[0] * es;
+		out[2] = h[1] * es;
+		out[3] = (h[5] + h[0] + h[1]) * es;
+		for (int a = 0; a < 3; ++a) out[4 + a] = (h[2 + a] + h[6 + a]) * ms;
+		out[7] = th[0] * es;
+		for (int a = 0; a < 3; ++a) out[8 + a] = th[1 + a] * ms;
+		out[11] = std::sqrt(std::max(e0, gr[9])) / ff;  // the largest |E| of updateParameters
+		out[12] = std::sqrt(std::max(b0, gr[10])) / ff;
+		out[13] = (double)c->next_id;  // particlesNumber: the particles numbered so far
+		const double ncell = (double)g.n[0] * g.n[1] * g.n[2];
+		for (int a = 0; a < 3; ++a) {
+			out[14 + a] = std::sqrt(gr[a] / ncell) / ff;
+			out[17 + a] = gr[a] * g.volume / (8 * pi) * es;
+			out[20 + a] = gr[3 + a] * g.volume / (8 * pi) * es;
+			out[23 + a] = h[2 + a] * ms;
+			out[26 + a] = h[6 + a] * ms;
+		}
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
+int pic_write_general(pic_ctx* c, const pic_backup_info* info, const char* path) {
+	if (!c || !info || !path) return PIC_ERR_ARGUMENT;
+	double r[PIC_GENERAL_VALUES];
+	const int rc = pic_general(c, info, r);
+	if (rc) return rc;
+	try {
+		FILE* f = std::fopen(path, "a");
+		if (!f) throw DeviceError{PIC_ERR_ARGUMENT, std::string("cannot open ") + path};
+		const double t = info->time, dx = c->g.d[0];
+		// outputGeneral's two fprintf calls (PIC++/output.cpp:3229-3249); shockWaveX and the
+		// derivative points keep their initial values, their updates being off in the loop
+		if (std::fprintf(f,
+		                 "%d %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g "
+		                 "%15.10g %15.10g %15.10g %15.10g %d %15.10g %15.10g %15.10g %15.10g ",
+		                 info->current_iteration, t, t * info->plasma_period, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8],
+		                 r[9], r[10], r[11], r[12], c->cfg.dt, (int)r[13], -1.0, r[14], r[15], r[16]) < 0 ||
+		    std::fprintf(f,
+		                 "%15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %15.10g %d %15.10g "
+		                 "%d %15.10g\n",
+		                 r[17], r[18], r[19], r[20], r[21], r[22], r[23], r[24], r[25], r[26], r[27], r[28], 0,
+		                 0 * dx * info->scale_factor, 0, 0 * dx * info->scale_factor) < 0) {
+			std::fclose(f);
+			throw DeviceError{PIC_ERR_ARGUMENT, std::string("cannot write ") + path};
+		}
+		if (std::fclose(f) != 0) throw DeviceError{PIC_ERR_ARGUMENT, std::string("cannot close ") + path};
+	} catch (const DeviceError& e) {
+		return fail(c, e);
+	}
+	return PIC_OK;
+}
+
 int pic_upload_yee(pic_ctx* c, const double* const* fields6) {
 	if (!c || !fields6) return PIC_ERR_ARGUMENT;
 	if (!c->explicit_scheme) {
 		c->error = "pic_upload_yee: the context was not created with PIC_SCHEME_EXPLICIT";
 		return PIC_ERR_ARGUMENT;
 	}
+	c->theo_init = false;  // the run starts again from this state
 	try {
 		for (int k = 0; k < 6; ++k) {
 			const int comp = k % 3;
@@ -1846,6 +2004,7 @@ int pic_explicit_step(pic_ctx* c) {
 		const Grid& g = c->g;
 		hipStream_t s = c->stream;
 		const double dt = c->cfg.dt;
+		if (!c->theo_init) init_theo(c);
 		{
 			Timed t(c, "yee_b_half", (double)g.ncells * 3 * 8 * 3);
 			launch_tristan_b_half(g, c->yee, dt, s);
@@ -1917,6 +2076,9 @@ int pic_explicit_step(pic_ctx* c) {
 			launch_tristan_e(g, c->yee, dt, s);
 			launch_yee_exchange(g, c->yee, false, s);
 		}
+		// the Poynting flux through the x boundaries with the new fields (updateTheoreticalEnergy
+		// after the BUNEMAN branch, PIC++/simulation.cpp:272); zero with periodic x
+		launch_theo_poynting(g, yee_view(c->yee), dt, c->theo, s);
 		rebin_particles(c);
 	} catch (const DeviceError& e) {
 		return fail(c, e);
@@ -1947,7 +2109,7 @@ namespace {
 
 constexpr char kCkptMagic[8] = {'P', 'I', 'C', 'A', 'M', 'D', 'C', 'K'};
 constexpr char kCkptEnd[8] = {'C', 'K', 'P', 'T', 'E', 'N', 'D', '\0'};
-constexpr uint32_t kCkptVersion = 1;
+constexpr uint32_t kCkptVersion = 2;  // 2: the theoretical energy and momentum
 constexpr size_t kStage = size_t(64) << 20;  // host staging per copy, bytes
 
 struct File {
@@ -2073,6 +2235,14 @@ int pic_checkpoint_save(pic_ctx* c, const char* path) {
 		f.put<uint32_t>(c->inj_slabs);
 		for (int s = 0; s < kMaxSpecies; ++s) f.put<double>(c->inj_len[s]);
 		for (int s = 0; s <= kMaxSpecies; ++s) f.put<int32_t>(c->species_start_h[s]);
+		double th[4] = {0, 0, 0, 0};
+		if (c->theo_init) {
+			HIP_CHECK(hipMemcpy(th, c->theo, sizeof(th), hipMemcpyDeviceToHost));
+		}
+		f.put<int32_t>(c->theo_init ? 1 : 0);
+		for (double v : th) f.put<double>(v);
+		// FP32: whether the next step takes its cell moments from the last push's density pass
+		f.put<int32_t>(c->fused == 2 ? 1 : 0);
 		std::vector<char> stage(kStage);
 		for (const auto& a : checkpoint_arrays(c, np)) put_device(f, a.first, a.second, stage);
 		f.put(kCkptEnd, 8);
@@ -2104,13 +2274,31 @@ int pic_checkpoint_load(pic_ctx* c, const char* path) {
 		for (int s = 0; s < kMaxSpecies; ++s) inj_len[s] = f.get<double>();
 		std::vector<int> start(kMaxSpecies + 1);
 		for (int s = 0; s <= kMaxSpecies; ++s) start[s] = f.get<int32_t>();
+		if (start[0] != 0 || start[c->sp.count] != np) throw DeviceError{PIC_ERR_ARGUMENT, "bad species ranges"};
 		for (int s = 0; s < c->sp.count; ++s)
 			if (start[s] < 0 || start[s] > start[s + 1] || start[s + 1] > np) throw DeviceError{PIC_ERR_ARGUMENT, "bad species ranges"};
-		std::vector<char> stage(kStage);
-		for (const auto& a : checkpoint_arrays(c, np)) get_device(f, a.first, a.second, stage);
+		const int32_t theo_init = f.get<int32_t>();
+		double th[4];
+		for (double& v : th) v = f.get<double>();
+		const int32_t fused_cache = f.get<int32_t>();
+		// Everything is checked before the first device write, so a refused file leaves the
+		// context as it was: the length the arrays need, and the end marker.
+		const auto arrays = checkpoint_arrays(c, np);
+		long long need = 8;
+		for (const auto& a : arrays) need += (long long)a.second;
+		const long long here = std::ftell(f.f);
+		if (here < 0 || std::fseek(f.f, 0, SEEK_END) != 0) throw DeviceError{PIC_ERR_ARGUMENT, "cannot size the checkpoint"};
+		const long long size = std::ftell(f.f);
+		if (size != here + need) throw DeviceError{PIC_ERR_ARGUMENT, "checkpoint truncated or corrupt (length)"};
 		char end[8];
+		if (std::fseek(f.f, size - 8, SEEK_SET) != 0) throw DeviceError{PIC_ERR_ARGUMENT, "cannot read the checkpoint's end"};
 		f.get(end, 8);
 		if (std::memcmp(end, kCkptEnd, 8) != 0) throw DeviceError{PIC_ERR_ARGUMENT, "checkpoint truncated or corrupt"};
+		if (std::fseek(f.f, here, SEEK_SET) != 0) throw DeviceError{PIC_ERR_ARGUMENT, "cannot read the checkpoint"};
+		std::vector<char> stage(kStage);
+		for (const auto& a : arrays) get_device(f, a.first, a.second, stage);
+		HIP_CHECK(hipMemcpyAsync(c->theo, th, sizeof(th), hipMemcpyHostToDevice, c->stream));
+		c->theo_init = theo_init != 0;
 		set_count(c, np);
 		c->next_id = next_id;
 		c->inj_step = inj_step;
@@ -2129,6 +2317,13 @@ int pic_checkpoint_load(pic_ctx* c, const char* path) {
 			check_status(c);
 		}
 		sort_particles(c);
+		if (c->fp32 && fused_cache) {
+			// the saved run's next step takes its cell moments from the post-push pass: form them
+			// again with the same kernel (from the saved particles and Enew, Bnew, which equal E, B
+			// after the commit), so that the resumed run repeats its bits
+			density(c, true);
+			c->fused = 2;
+		}
 		HIP_CHECK(hipStreamSynchronize(s));
 	} catch (const DeviceError& e) {
 		return fail(c, e);

@@ -60,7 +60,7 @@ void launch_update_b(const Grid& g, const double* Etheta, const double* B, doubl
 // the push writes the new sort keys and one mover bit per particle (mask, as the FP32 push)
 void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const double* E,
                  const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, uint64_t* mask, int* status,
-                 int* dead, uint32_t* esc, hipStream_t s);
+                 int* dead, uint32_t* esc, uint64_t* bounce_mask, hipStream_t s);
 // wall boundary helpers: cell planes 0, 1 folded into their mirrors 3, 2 (flip[c] negates
 // component c), the right-end fold of cells N-2, N-1 into N-3 (zero_mask[c] zeroes them), and
 // zeroing of node planes i0..i1 for components [c0, c1)

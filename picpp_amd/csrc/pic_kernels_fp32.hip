@@ -659,7 +659,8 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
                                                 const int* __restrict__ species_start, const float4* __restrict__ E,
                                                 const float4* __restrict__ Et, const float4* __restrict__ B, float dt, float eta,
                                                 uint32_t* __restrict__ keys, uint64_t* __restrict__ mask, int* __restrict__ status,
-                                                int* __restrict__ dead, uint32_t* __restrict__ esc) {
+                                                int* __restrict__ dead, uint32_t* __restrict__ esc,
+                                                uint64_t* __restrict__ bounce_mask) {
 	constexpr int oEt = Box32<2>::size, oB = Box32<2>::size + Box32<4>::size;
 	__shared__ float4 boxes[Box32<2>::size + Box32<4>::size + Box32<5>::size];  // E^n, E^{n+theta}, B
 	__shared__ int row_ends[2];
@@ -818,6 +819,7 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 		if (WALL && pos[0] < (float)(2 - cc[0])) {  // final position behind the wall (mover.cpp:435-441)
 			pos[0] = (float)(4 - 2 * cc[0]) - pos[0];
 			unew[0] = -unew[0];
+			reflected = true;
 		}
 	}
 	// Out through the free right end: removed (mover.cpp:447-458). The reference removes x > node
@@ -827,9 +829,12 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 	if (WALL && pos[0] >= (float)(g.n[0] + 2 - cc[0])) {
 		keys[i] = 0xFFFFFFFFu;
 		mv = true;
-		// listed for replacement, with its absolute y, z in cell units
+		// listed for replacement, with its absolute y, z in cell units, and with its new momentum
+		// (what leaves the box, for the theoretical energy)
 		P.off[1][i] = (float)cc[1] + pos[1];
 		P.off[2][i] = (float)cc[2] + pos[2];
+#pragma unroll
+		for (int c = 0; c < 3; ++c) P.u[c][i] = unew[c];
 		esc[atomicAdd(dead, 1)] = (uint32_t)i;
 	} else {
 		// x^{n+1} = x^n + mid dt, carried into the integer cell; one-rank periodic wrap (x too
@@ -879,6 +884,10 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 	// (waves are 64 consecutive particles; lanes past the end contribute zeros)
 	const unsigned long long bits = __ballot(mv);
 	if ((threadIdx.x & 63) == 0) mask[i >> 6] = bits;
+	if (WALL) {  // and one bit per particle reflected at the wall in this push
+		const unsigned long long rb = __ballot(reflected);
+		if ((threadIdx.x & 63) == 0) bounce_mask[i >> 6] = rb;
+	}
 }
 
 // ------------------------------------------------------------------ rebin after a push
@@ -1225,7 +1234,7 @@ static void magic_divisor(uint32_t d, uint32_t& m, int& l) {
 
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, uint64_t* mask, int* status, int* dead,
-                   uint32_t* esc, hipStream_t s) {
+                   uint32_t* esc, uint64_t* bounce_mask, hipStream_t s) {
 	if (P.count == 0) return;
 	Push32Consts K{};
 	for (int q = 0; q < sp.count; ++q) {  // the same float arithmetic as q dt / m on the device
@@ -1241,7 +1250,7 @@ void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32
 	magic_divisor((uint32_t)g.N[1], K.m1, K.l1);
 	hipLaunchKernelGGL(g.wall ? k_push32<true> : k_push32<false>, dim3(blocks32(P.count, 256)), dim3(256), 0, s, g, sp, K, P, species_start,
 	                   reinterpret_cast<const float4*>(E), reinterpret_cast<const float4*>(Et), reinterpret_cast<const float4*>(B), dt,
-	                   eta, keys, mask, status, dead, esc);
+	                   eta, keys, mask, status, dead, esc, bounce_mask);
 }
 void launch_mask_counts(const uint64_t* mask, long long nwords, int* counts, hipStream_t s) {
 	hipLaunchKernelGGL(k_mask_counts, dim3(blocks32(nwords + 1, 256)), dim3(256), 0, s, mask, nwords, counts);

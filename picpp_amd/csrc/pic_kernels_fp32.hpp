@@ -40,7 +40,7 @@ struct Push32Consts {
 // wave of 64 particles.
 void launch_push32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, const float* E,
                    const float* Et, const float* B, float dt, float eta, uint32_t* keys, uint64_t* mask, int* status, int* dead,
-                   uint32_t* esc, hipStream_t s);
+                   uint32_t* esc, uint64_t* bounce_mask, hipStream_t s);
 // counts[w] = population of mask word w < nwords, counts[nwords] = 0
 void launch_mask_counts(const uint64_t* mask, long long nwords, int* counts, hipStream_t s);
 // mover bits of the particles lo .. hi-1 appended after the push

@@ -755,7 +755,7 @@ __device__ inline bool push_particle(const Grid& g, const SpeciesTable& sp, cons
                                      const int* __restrict__ species_start, const double* __restrict__ E,
                                      const double* __restrict__ Etheta, const double* __restrict__ B, double dt, double eta,
                                      uint32_t* __restrict__ keys, int* __restrict__ status, int* __restrict__ dead,
-                                     uint32_t* __restrict__ esc, long long i) {
+                                     uint32_t* __restrict__ esc, long long i, bool& bounced) {
 #pragma clang fp contract(fast)
 	const double xw = g.x0[0] + 2 * g.d[0];  // the wall, xgrid[2]
 	const double rest = (1.0 - eta) * dt;
@@ -847,8 +847,10 @@ __device__ inline bool push_particle(const Grid& g, const SpeciesTable& sp, cons
 		if (WALL && x[0] < xw) {  // final position behind the wall (mover.cpp:435-441)
 			x[0] = 2 * xw - x[0];
 			p[0] = -p[0];
+			bounced = true;
 		}
 	}
+	if (reflected) bounced = true;
 	if (WALL) {
 		// leaving through the free right end removes the particle (mover.cpp:447-458, exchangeParticles
 		// keeps no right escapes without periodicity, :1007-1021)
@@ -893,14 +895,18 @@ __global__ void __launch_bounds__(256) k_push(Grid g, SpeciesTable sp, Recip R, 
                                               const double* __restrict__ E, const double* __restrict__ Etheta,
                                               const double* __restrict__ B, double dt, double eta, uint32_t* __restrict__ keys,
                                               uint64_t* __restrict__ mask, int* __restrict__ status, int* __restrict__ dead,
-                                              uint32_t* __restrict__ esc) {
+                                              uint32_t* __restrict__ esc, uint64_t* __restrict__ bounce_mask) {
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	bool mv = false;
-	if (i < P.count) mv = push_particle<WALL>(g, sp, R, P, species_start, E, Etheta, B, dt, eta, keys, status, dead, esc, i);
+	bool mv = false, bounced = false;
+	if (i < P.count) mv = push_particle<WALL>(g, sp, R, P, species_start, E, Etheta, B, dt, eta, keys, status, dead, esc, i, bounced);
 	// one mover bit per particle: bit i & 63 of word i >> 6, written by the wave's first lane
 	// (waves are 64 consecutive particles; lanes past the end contribute zeros)
 	const unsigned long long bits = __ballot(mv);
 	if ((threadIdx.x & 63) == 0 && i < P.count) mask[i >> 6] = bits;
+	if (WALL) {  // and one bit per particle reflected at the wall in this push
+		const unsigned long long rb = __ballot(bounced);
+		if ((threadIdx.x & 63) == 0 && i < P.count) bounce_mask[i >> 6] = rb;
+	}
 }
 
 // Sort key of every particle at its current position (updateParticleCorrelationMaps).
@@ -1199,10 +1205,10 @@ void launch_update_b(const Grid& g, const double* Etheta, const double* B, doubl
 }
 void launch_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const double* E,
                  const double* Etheta, const double* B, double dt, double eta, uint32_t* keys, uint64_t* mask, int* status,
-                 int* dead, uint32_t* esc, hipStream_t s) {
+                 int* dead, uint32_t* esc, uint64_t* bounce_mask, hipStream_t s) {
 	if (P.count == 0) return;
 	hipLaunchKernelGGL(g.wall ? k_push<true> : k_push<false>, dim3(blocks(P.count, 256)), dim3(256), 0, s, g, sp, make_recip(g, sp), P,
-	                   species_start, E, Etheta, B, dt, eta, keys, mask, status, dead, esc);
+	                   species_start, E, Etheta, B, dt, eta, keys, mask, status, dead, esc, bounce_mask);
 }
 void launch_wall_mirror_cells(const Grid& g, double* f, int ncomp, const int* flip, hipStream_t s) {
 	long long n = (long long)g.N[1] * g.N[2] * ncomp;

@@ -237,12 +237,8 @@ class Simulation:
         """Restores a checkpoint written by a context of the same configuration."""
         self._chk(self.lib.pic_checkpoint_load(self.h, os.fsencode(path)), "pic_checkpoint_load")
 
-    def outputSimulationBackup(self, general_file: str, e_file: str, b_file: str, particles_file: str, **info):
-        """The reference's backup files (PIC++/output.cpp:3432-3602) of the current state. The
-        keyword arguments fill pic_backup_info (input_type, particles_per_bin, concentration,
-        temperature, plasma_period, scale_factor, time, max_time, current_iteration,
-        max_iteration, size, debug_mode, maxwell_matrix_size, ext_j, V0, E0, B0, species_type,
-        particle_dx); missing ones are 0."""
+    @staticmethod
+    def _backup_info(info) -> "_lib.PicBackupInfo":
         bi = _lib.PicBackupInfo()
         for key, val in info.items():
             cur = getattr(bi, key)
@@ -251,6 +247,32 @@ class Simulation:
                     cur[i] = v
             else:
                 setattr(bi, key, val)
+        return bi
+
+    def general(self, plasma_period: float = 1.0, scale_factor: float = 1.0, **info) -> np.ndarray:
+        """The numbers of a general.dat row (outputGeneral, PIC++/output.cpp:3215-3250) for the
+        current state, in the row's order without the iteration and the time (PIC_GENERAL_VALUES
+        of them, the order of include/picpp_amd.h). Keyword arguments as outputSimulationBackup's
+        (B0 is used)."""
+        bi = self._backup_info(dict(info, plasma_period=plasma_period, scale_factor=scale_factor))
+        out = np.zeros(_lib.GENERAL_VALUES)
+        self._chk(self.lib.pic_general(self.h, ctypes.byref(bi), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                  "pic_general")
+        return out
+
+    def outputGeneral(self, general_file: str, plasma_period: float = 1.0, scale_factor: float = 1.0, **info):
+        """Appends the general.dat row of the current state in the reference's format (time and
+        current_iteration from the keyword arguments)."""
+        bi = self._backup_info(dict(info, plasma_period=plasma_period, scale_factor=scale_factor))
+        self._chk(self.lib.pic_write_general(self.h, ctypes.byref(bi), os.fsencode(general_file)), "pic_write_general")
+
+    def outputSimulationBackup(self, general_file: str, e_file: str, b_file: str, particles_file: str, **info):
+        """The reference's backup files (PIC++/output.cpp:3432-3602) of the current state. The
+        keyword arguments fill pic_backup_info (input_type, particles_per_bin, concentration,
+        temperature, plasma_period, scale_factor, time, max_time, current_iteration,
+        max_iteration, size, debug_mode, maxwell_matrix_size, ext_j, V0, E0, B0, species_type,
+        particle_dx); missing ones are 0."""
+        bi = self._backup_info(info)
         paths = [os.fsencode(p) for p in (general_file, e_file, b_file, particles_file)]
         self._chk(self.lib.pic_write_backup(self.h, ctypes.byref(bi), *paths), "pic_write_backup")
 

@@ -24,6 +24,9 @@ fixtures next to this script:
                    of newE (cleanupDivergence) after every push, with the Fourier filtering of newE,
                    newB (filterFields, cut wave number 2), and with the cleanup of newE and then newB
                    (cleanupDivergenceMagnetic), 5 steps each
+  general_ref.npz, general_wall_ref.npz  general.dat rows (outputGeneral, with the loop's
+                   updateTheoreticalEnergy and updateParameters) of the perturbed mini case and of the
+                   wall escape case, 5 steps each: the reference's text and the rows' numbers
   explicit_ref.npz the explicit (BUNEMAN / Tristan) step (oracle/_ref/ref_harness_explicit):
                    the mini grid at dt = 0.4 with fast electrons (u ~ N(0, 0.5) per axis), so that
                    many zig-zag paths cross cells, perturbed Yee fields, 6 steps
@@ -51,7 +54,7 @@ HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 HARNESS_EXPLICIT = os.path.join(ROOT, "oracle", "_ref", "ref_harness_explicit")
 
 
-def write_input(path, w, st, steps):
+def write_input(path, w, st, steps, general=False):
     m, q, wt, ty = wl.species_arrays(w)
     with open(path, "wb") as f:
         f.write(struct.pack("<iii", *w.n))
@@ -80,6 +83,7 @@ def write_input(path, w, st, steps):
             f.write(struct.pack("<i", 0))
         f.write(struct.pack("<id", w.smoothing_count, w.smoothing_parameter))
         f.write(struct.pack("<ii", int(w.divergence_cleanup), w.filter_cut_wave_number))
+        f.write(struct.pack("<i", 1 if general else 0))
 
 
 def read_output(path):
@@ -108,15 +112,19 @@ REF_DIRS = ["C:/users/Vadik/Documents/Visual Studio 2010/Projects/PICpp/PIC++/" 
             for d in ("output", "reduced_output", "input", "backup")]
 
 
-def run(w, st, steps):
+def run(w, st, steps, general=False):
     with tempfile.TemporaryDirectory() as td:
         for d in REF_DIRS:
             os.makedirs(os.path.join(td, d), exist_ok=True)
         fi = os.path.join(td, "in.bin")
         fo = os.path.join(td, "out.bin")
-        write_input(fi, w, st, steps)
+        write_input(fi, w, st, steps, general)
         subprocess.run([HARNESS, fi, fo], check=True, stdout=subprocess.DEVNULL, cwd=td)
-        return read_output(fo)
+        out = read_output(fo)
+        if general:  # the general.dat text the reference wrote, one row per line
+            with open(fo + ".general") as fh:
+                out["general_text"] = np.array(fh.read().splitlines())
+        return out
 
 
 def inputs(st):
@@ -239,6 +247,28 @@ def make_cleanup():
         np.savez_compressed(os.path.join(HERE, fname), **inputs(st), **out)
 
 
+def general_cases():
+    """general.dat rows (outputGeneral after every updateEnergy, with the loop's
+    updateTheoreticalEnergy and updateParameters): the perturbed mini case, and the wall case with
+    reflections at the wall and escapes through the free right end (no injection: the injected
+    momenta are the reference's rand() samples), 5 steps each."""
+    w = dataclasses.replace(wl.mini(), name="mini-general")
+    ww, stw = wall_out_case()
+    return ((w, wl.make_initial_state(w, perturb=0.05), "general_ref.npz"),
+            (dataclasses.replace(ww, name="wall-out-general"), stw, "general_wall_ref.npz"))
+
+
+def make_general():
+    for w, st, fname in general_cases():
+        out = run(w, st, w.steps, general=True)
+        keep = {k: v for k, v in out.items() if k.endswith("/general") or k.endswith("/energy") or k in ("meta", "general_text")}
+        for s in range(w.steps + 1):
+            keep["s%d/particles" % s] = out["s%d/particles" % s]
+            keep["s%d/E" % s] = out["s%d/E" % s]
+            keep["s%d/B" % s] = out["s%d/B" % s]
+        np.savez_compressed(os.path.join(HERE, fname), **inputs(st), **keep)
+
+
 def explicit_case():
     w = dataclasses.replace(wl.mini(), dt=0.4, steps=6, name="explicit")
     st = wl.make_initial_state(w)
@@ -306,7 +336,7 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref", "-j8"], check=True)
     for name in which:
-        {"mini": make_mini, "config1": make_config1, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject,
+        {"mini": make_mini, "config1": make_config1, "general": make_general, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject,
          "wall_long": make_wall_long, "smooth": make_smooth, "explicit": make_explicit,
          "backup": make_backup, "cleanup": make_cleanup}[name]()
         print("wrote", name)

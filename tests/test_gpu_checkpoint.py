@@ -2,11 +2,13 @@
 and the reference's backup files (pic_write_backup, outputSimulationBackup).
 
 - A run interrupted by a checkpoint and resumed in a fresh context repeats the uninterrupted
-  run's bits: FP64 implicit steps (mini case) and explicit steps (the explicit golden case).
-  FP32 agrees to FP32 rounding: its first resumed step forms the cell moments afresh instead
-  of taking them from the post-push pass.
-- A checkpoint does not load into a context of another configuration, and a truncated file is
-  refused.
+  run's bits: FP64 and FP32 implicit steps (mini case) and explicit steps (the explicit golden
+  case). The FP32 load forms the post-push pass's cell moments again, so the first resumed step
+  uses the same moments as the uninterrupted run.
+- A checkpoint does not load into a context of another configuration; a truncated file and one
+  with a damaged end are refused before anything is written, so the context keeps its state.
+- The wall case with injection resumes bit for bit (the injection's random stream and counters
+  travel with the checkpoint).
 - The backup files of the mini case's initial state are byte for byte the compiled
   reference's (tests/golden/backup_ref.npz, written by oracle/ref_harness.cpp when given a
   path prefix).
@@ -17,7 +19,6 @@ import os
 import numpy as np
 import pytest
 
-from conftest import relerr
 from picpp_amd import _lib
 from picpp_amd import workloads as wl
 from picpp_amd.simulation import Simulation
@@ -63,15 +64,7 @@ def test_checkpoint_resume_implicit(tmp_path, precision):
     assert_bitwise(saved, state(b))
     for _ in range(3):
         b.step()
-    sa, sb = state(a), state(b)
-    if precision == "fp64":
-        assert_bitwise(sa, sb)
-    else:
-        assert np.array_equal(sa["ids"], sb["ids"])
-        errs = {k: relerr(sb[k], sa[k]) for k in ("x", "p", "E", "B")}
-        print("fp32 resumed vs uninterrupted:", {k: "%.1e" % v for k, v in errs.items()})
-        for k, v in errs.items():
-            assert v <= 1e-5, (k, v)
+    assert_bitwise(state(a), state(b))
     a.close()
     b.close()
 
@@ -118,11 +111,52 @@ def test_checkpoint_refuses_other_configuration_and_truncation(tmp_path):
         data = f.read()
     with open(cut, "wb") as f:
         f.write(data[: len(data) - 100])
+    bad_end = str(tmp_path / "bad_end.ckpt")  # the right length, a damaged end marker
+    with open(bad_end, "wb") as f:
+        f.write(data[:-1] + bytes([data[-1] ^ 0xFF]))
+    # a refused file leaves the context as it was: a running context keeps its state
     d = Simulation(w, precision="fp64")
-    with pytest.raises(_lib.PicError, match="short read|truncated"):
-        d.load_checkpoint(cut)
+    d.load_state(wl.make_initial_state(dataclasses.replace(w, seed=w.seed + 1), perturb=0.05))
+    d.initialize()
+    d.step()
+    before = state(d)
+    for bad in (cut, bad_end):
+        with pytest.raises(_lib.PicError, match="short read|truncated"):
+            d.load_checkpoint(bad)
+        assert_bitwise(state(d), before)
+    d.step()  # and still steps
     d.close()
     a.close()
+
+
+def test_checkpoint_resume_wall_with_injection(tmp_path):
+    """The wall case with escapes, injection and replacement (the injection golden's state):
+    the checkpoint carries the injection's stream step, slab count, injection length and next id,
+    so a resumed run injects the same particles and repeats the uninterrupted run's bits."""
+    d = np.load(os.path.join(HERE, "golden", "inject_ref.npz"))
+    w = wl.shock(n=(12, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=5, seed=19, name="inject")
+    st = wl.InitialState(d["in/species"].astype(np.int32), d["in/ids"].astype(np.int32), d["in/x"], d["in/p"], d["in/E"],
+                         d["in/B"])
+    print("case", w.name, flush=True)
+    a = Simulation(w, precision="fp64")
+    a.load_state(st)
+    a.initialize()
+    for _ in range(2):
+        a.step()
+    path = str(tmp_path / "inject.ckpt")
+    a.save_checkpoint(path)
+    for _ in range(3):
+        a.step()
+    want = state(a)
+    n_want = a.num_particles()
+    a.close()
+    b = Simulation(w, precision="fp64")
+    b.load_checkpoint(path)
+    for _ in range(3):
+        b.step()
+    assert b.num_particles() == n_want
+    assert_bitwise(state(b), want)
+    b.close()
 
 
 def parse_general(text):
