@@ -168,6 +168,16 @@ int main(int argc, char** argv) {
 		dump_yee(*ey[c], tag(en[c], 0));
 		dump_yee(*by[c], tag(bn[c], 0));
 	}
+	// updateEnergy with the Yee fields (getBunemanElectricField / getBunemanMagneticField,
+	// PIC++/simulation.cpp:1506-1640): particle, E, B and total energy, total momentum
+	auto dump_energy = [&](int step) {
+		s.updateEnergy();
+		std::vector<double> e = {s.particleEnergy, s.electricFieldEnergy, s.magneticFieldEnergy, s.energy,
+		                         s.globalMomentum.x, s.globalMomentum.y, s.globalMomentum.z};
+		dump(tag("energy", step), e, {7});
+	};
+	s.currentIteration = 0;
+	dump_energy(0);
 	for (int step = 1; step <= nsteps; ++step) {
 		// PIC++/simulation.cpp:163-197, BUNEMAN branch, smoothingCount = 0
 		s.tristanEvaluateBhalfStep();
@@ -191,6 +201,7 @@ int main(int argc, char** argv) {
 			dump_yee(*ey[c], tag(en[c], step));
 			dump_yee(*by[c], tag(bn[c], step));
 		}
+		dump_energy(step);
 	}
 	fclose(g_out);
 	MPI_Finalize();

@@ -5,9 +5,10 @@ bit for bit, tests/test_explicit_oracle.py).
 
 The case: 6x5x5 cells, 4 ppc per species, dt = 0.4, fast electrons (u ~ N(0, 0.5) per axis)
 so that many zig-zag paths cross cells and some particles leave through every face, perturbed
-Yee fields, 6 steps. Particles (matched by id), the Yee E and B after every step and the Yee
-current of the first two steps must agree to 1e-12 normwise; the fixed-point current makes
-two runs bitwise identical.
+Yee fields, 6 steps. Particles (matched by id), the Yee E and B after every step, the Yee
+current of the first two steps and the energies and total momentum of updateEnergy (from the
+Yee components) must agree to 1e-12 normwise; the fixed-point current makes two runs bitwise
+identical.
 
 A second case checks the paths that the small case cannot reach against the CPU restatement
 (oracle/explicit_oracle.cpp): at 8x8x24 cells with 32 ppc per species, a workgroup's 256
@@ -53,8 +54,17 @@ def test_explicit_step_matches_reference():
     worst = {}
     for name in wl.YEE_NAMES:
         worst[name + "0"] = relerr(sim.yee(name), d["s0/" + name])
+
+    def energy_errors(step):
+        # updateEnergy with the Yee components (the explicit scheme's getBuneman* fields)
+        e, want = np.asarray(sim.updateEnergy()), d["s%d/energy" % step]
+        worst["energy"] = max(worst.get("energy", 0.0), relerr(e[:4], want[:4]))
+        worst["momentum"] = max(worst.get("momentum", 0.0), relerr(e[4:], want[4:]))
+
+    energy_errors(0)
     for step in range(1, w.steps + 1):
         sim.explicit_step()
+        energy_errors(step)
         p = sim.particles()
         ids, x, mom = by_id(p["ids"], p["x"], p["p"])
         ref = d["s%d/particles" % step]
