@@ -84,8 +84,8 @@ enum {
 enum {
 	PIC_SCHEME_IMPLICIT = 0, /* the theta-implicit step (the reference's IMPLICIT solver) */
 	/* The reference's BUNEMAN solver: explicit Yee fields, the Tristan mover and the zig-zag
-	   current (PIC++/simulation.cpp:163-197). FP64, periodic, no smoothing; fields are the Yee
-	   arrays (PIC_ARRAY_YEE_*), stepped by pic_explicit_step. */
+	   current (PIC++/simulation.cpp:163-197). FP64, periodic; smoothing_count passes smooth the
+	   current; fields are the Yee arrays (PIC_ARRAY_YEE_*), stepped by pic_explicit_step. */
 	PIC_SCHEME_EXPLICIT = 1
 };
 
@@ -132,7 +132,9 @@ typedef struct {
 	/* Smoothing after the moments (PIC++/simulation.cpp:213-217): smoothing_count passes over
 	   chargeDensityHat, electricFlux and the dielectric tensor of the 27-point kernel with
 	   weights (s/2, 1 - s, s/2) per axis, s = smoothing_parameter (PIC++/smoothing.cpp). The
-	   shipped input uses 8 passes of s = 0.5; 0 passes is the default. */
+	   shipped input uses 8 passes of s = 0.5; 0 passes is the default. With PIC_SCHEME_EXPLICIT the
+	   passes smooth the Yee current after its summation instead (smoothBunemanEfieldGeneral,
+	   PIC++/simulation.cpp:185-188, PIC++/smoothing.cpp:535-733). */
 	int smoothing_count;
 	double smoothing_parameter;
 	int scheme;              /* PIC_SCHEME_* */
@@ -198,12 +200,13 @@ int pic_upload_fields(pic_ctx* ctx, const double* E, const double* B);
    exchangeBunemanEfield (PIC++/simulation.cpp:83-85). Replaces pic_upload_fields and
    pic_initialize for that scheme. */
 int pic_upload_yee(pic_ctx* ctx, const double* const* fields6);
-/* PIC_SCHEME_EXPLICIT: one step of the BUNEMAN loop body (PIC++/simulation.cpp:163-197 with
-   smoothingCount = 0, then eraseEscapedPaticles / exchangeParticles): tristanEvaluateBhalfStep
-   and exchangeBunemanBfield, moveParticleTristan for every particle with its zig-zag current
-   (addParticleFluxZigzag, summed in 64-bit fixed point: the same sums in every run), the
-   second half step, exchangeBunemanFlux, tristanEvaluateE and exchangeBunemanEfield, the
-   one-rank periodic wrap. Replaces the implicit phases for that scheme. */
+/* PIC_SCHEME_EXPLICIT: one step of the BUNEMAN loop body (PIC++/simulation.cpp:163-197, then
+   eraseEscapedPaticles / exchangeParticles): tristanEvaluateBhalfStep and exchangeBunemanBfield,
+   moveParticleTristan for every particle with its zig-zag current (addParticleFluxZigzag, summed
+   in 64-bit fixed point: the same sums in every run), the second half step,
+   exchangeBunemanFlux, smoothing_count passes of smoothBunemanEfieldGeneral on the current,
+   tristanEvaluateE and exchangeBunemanEfield, the one-rank periodic wrap. Replaces the implicit
+   phases for that scheme. */
 int pic_explicit_step(pic_ctx* ctx);
 /* Particles: species[np], ids[np], x[3][np], p[3][np] (component planes, absolute
    coordinates in [L, 2L] per axis, momentum p = m gamma v). */

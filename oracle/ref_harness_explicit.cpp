@@ -146,6 +146,14 @@ int main(int argc, char** argv) {
 		                           s.deltaZ / 2, dt);
 		s.particles.push_back(q);
 	}
+	// optional trailer: smoothing passes of the current every step (smoothingCount,
+	// smoothingParameter; smoothBunemanEfieldGeneral after tristanUpdateFlux,
+	// PIC++/simulation.cpp:185-188, PIC++/smoothing.cpp:535)
+	int32_t smooth_count = 0;
+	double smooth_param = 0.0;
+	if (fread(&smooth_count, sizeof(int32_t), 1, in) == 1) smooth_param = rd<double>(in);
+	s.smoothingCount = smooth_count;
+	s.smoothingParameter = smooth_param;
 	fclose(in);
 	s.particlesNumber = (int)np;
 
@@ -179,13 +187,14 @@ int main(int argc, char** argv) {
 	s.currentIteration = 0;
 	dump_energy(0);
 	for (int step = 1; step <= nsteps; ++step) {
-		// PIC++/simulation.cpp:163-197, BUNEMAN branch, smoothingCount = 0
+		// PIC++/simulation.cpp:163-197, BUNEMAN branch
 		s.tristanEvaluateBhalfStep();
 		s.exchangeBunemanBfield(s.bunemanBx, s.bunemanBy, s.bunemanBz);
 		s.moveParticles();
 		s.tristanEvaluateBhalfStep();
 		s.exchangeBunemanBfield(s.bunemanBx, s.bunemanBy, s.bunemanBz);
 		s.tristanUpdateFlux();
+		for (int n = 0; n < s.smoothingCount; ++n) s.smoothBunemanEfieldGeneral(s.bunemanJx, s.bunemanJy, s.bunemanJz);
 		if (step <= 2)
 			for (int c = 0; c < 3; ++c) dump_yee(*jy[c], tag(jn[c], step));
 		s.tristanEvaluateE();

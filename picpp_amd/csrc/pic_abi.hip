@@ -830,8 +830,8 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	    (cfg->scheme != PIC_SCHEME_IMPLICIT || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->filter_cut_wave_number < 0 ||
 	     cfg->divergence_cleanup < 0 || cfg->divergence_cleanup > 3))
 		return PIC_ERR_ARGUMENT;
-	if (cfg->scheme == PIC_SCHEME_EXPLICIT &&  // the explicit step is built FP64, periodic, without smoothing or injection
-	    (cfg->precision != PIC_PRECISION_FP64 || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->smoothing_count != 0 || cfg->inject))
+	if (cfg->scheme == PIC_SCHEME_EXPLICIT &&  // the explicit step is built FP64, periodic, without injection
+	    (cfg->precision != PIC_PRECISION_FP64 || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->inject))
 		return PIC_ERR_ARGUMENT;
 	if (cfg->inject) {  // injection needs the wall set-up and createParticle's Maxwellian branch
 		if (cfg->inject != 1 || cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
@@ -987,7 +987,7 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		c->species_start_h.assign(kMaxSpecies + 1, 0);
 		HIP_CHECK(hipMemset(c->species_start, 0, (kMaxSpecies + 1) * sizeof(int)));
 		c->status = dalloc<int>(kStatusWords);
-		if (cfg->smoothing_count > 0) c->smooth_tmp = dalloc<double>(9 * g.nnodes);
+		if (cfg->smoothing_count > 0 && cfg->scheme != PIC_SCHEME_EXPLICIT) c->smooth_tmp = dalloc<double>(9 * g.nnodes);
 		c->dead = dalloc<int>(1);
 		HIP_CHECK(hipMemset(c->dead, 0, sizeof(int)));
 		if (g.wall) {
@@ -2070,6 +2070,15 @@ int pic_explicit_step(pic_ctx* c) {
 		{
 			Timed t(c, "yee_current", (double)g.ncells * 3 * 16);
 			launch_yee_current(g, c->yee, 1.0 / scale, s);
+		}
+		if (c->cfg.smoothing_count > 0) {
+			// smoothBunemanEfieldGeneral passes (PIC++/simulation.cpp:185-188); the fixed-point
+			// accumulators, done with for this step, hold the scratch copies
+			Timed t(c, "smoothing", (double)c->cfg.smoothing_count * g.ncells * 3 * 8 * 3);
+			const Smooth27 sm = smoothing_matrix(c->cfg.smoothing_parameter);
+			double* const tmp[3] = {reinterpret_cast<double*>(c->yee.Jacc[0]), reinterpret_cast<double*>(c->yee.Jacc[1]),
+			                        reinterpret_cast<double*>(c->yee.Jacc[2])};
+			for (int n = 0; n < c->cfg.smoothing_count; ++n) launch_smooth_yee_current(g, c->yee, sm, tmp, s);
 		}
 		{
 			Timed t(c, "yee_e", (double)g.ncells * 3 * 8 * 4);

@@ -690,6 +690,38 @@ void launch_yee_exchange(const Grid& g, const YeeGrid& y, bool b_field, hipStrea
 	yee_axes(g, q, false, s);
 }
 
+// smoothBunemanEfieldGeneral (PIC++/smoothing.cpp:535-733) on current component c: the 27-point
+// kernel of createSmoothingMatrix over the component's range (along its own axis [2, N - 2),
+// along the others [2, N - 2]), read from src and written to dst; with the wall the plane i = 2
+// (and i = N0 - 2 for the y and z components) is copied instead (the reference's non-periodic
+// branches). The sum runs over the 27 neighbours in the reference's order.
+__global__ void k_smooth_yee(Grid g, const double* __restrict__ src, double* __restrict__ dst, int comp, Smooth27 sm, int wall,
+                             int d1, int d2, int r1, int r2, long long n) {
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	const int k = 2 + (int)(t % r2), j = 2 + (int)((t / r2) % r1), i = 2 + (int)(t / ((long long)r2 * r1));
+	const long long o = ((long long)i * d1 + j) * d2 + k;
+	if (wall && (i == 2 || (comp != 0 && i == g.N[0] - 2))) {
+		dst[o] = src[o];
+		return;
+	}
+	double acc = 0;
+	for (int a = -1; a <= 1; ++a)
+		for (int b = -1; b <= 1; ++b)
+			for (int c = -1; c <= 1; ++c)
+				acc += src[((long long)(i + a) * d1 + (j + b)) * d2 + (k + c)] * sm.m[(a + 1) * 9 + (b + 1) * 3 + (c + 1)];
+	dst[o] = acc;
+}
+
+__global__ void k_copy_range(const double* __restrict__ src, double* __restrict__ dst, int d1, int d2, int r1, int r2,
+                             long long n) {
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	const int k = 2 + (int)(t % r2), j = 2 + (int)((t / r2) % r1), i = 2 + (int)(t / ((long long)r2 * r1));
+	const long long o = ((long long)i * d1 + j) * d2 + k;
+	dst[o] = src[o];
+}
+
 void launch_yee_current(const Grid& g, const YeeGrid& y, double inv_scale, hipStream_t s) {
 	Yee3 q;
 	for (int c = 0; c < 3; ++c) {
@@ -699,6 +731,21 @@ void launch_yee_current(const Grid& g, const YeeGrid& y, double inv_scale, hipSt
 		for (int a = 0; a < 3; ++a) q.d[c][a] = y.dE[c][a];
 	}
 	yee_axes(g, q, true, s);
+}
+
+void launch_smooth_yee_current(const Grid& g, const YeeGrid& y, const Smooth27& sm, double* const tmp[3], hipStream_t s) {
+	Yee3 q;
+	for (int c = 0; c < 3; ++c) {
+		const int* d = y.dE[c];
+		int r[3];
+		for (int a = 0; a < 3; ++a) r[a] = a == c ? g.N[a] - 4 : g.N[a] - 3;
+		const long long n = (long long)r[0] * r[1] * r[2];
+		hipLaunchKernelGGL(k_smooth_yee, dim3(nb(n, 256)), dim3(256), 0, s, g, y.J[c], tmp[c], c, sm, g.wall, d[1], d[2], r[1], r[2], n);
+		hipLaunchKernelGGL(k_copy_range, dim3(nb(n, 256)), dim3(256), 0, s, tmp[c], y.J[c], d[1], d[2], r[1], r[2], n);
+		q.f[c] = y.J[c];
+		for (int a = 0; a < 3; ++a) q.d[c][a] = d[a];
+	}
+	yee_axes(g, q, false, s);  // exchangeBunemanEfield on the current
 }
 
 void launch_explicit_bounds(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,

@@ -30,6 +30,9 @@ void launch_tristan_e(const Grid& g, const YeeGrid& y, double dt, hipStream_t s)
 void launch_yee_exchange(const Grid& g, const YeeGrid& y, bool b_field, hipStream_t s);
 // the fixed-point current to J, then exchangeBunemanFlux's periodic folds
 void launch_yee_current(const Grid& g, const YeeGrid& y, double inv_scale, hipStream_t s);
+// one pass of smoothBunemanEfieldGeneral on J (tmp: three scratch arrays of J's shapes), then
+// exchangeBunemanEfield on J
+void launch_smooth_yee_current(const Grid& g, const YeeGrid& y, const Smooth27& sm, double* const tmp[3], hipStream_t s);
 // What the fixed-point scale of the explicit push's current is set from: the largest number of
 // particles of all species in one cell, per species the largest |u|^2 = |p|^2 / m^2 of its
 // particles, and per component the largest |E| on the Yee grid (ghosts included). The maxima of

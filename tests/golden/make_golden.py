@@ -30,6 +30,7 @@ fixtures next to this script:
   explicit_ref.npz the explicit (BUNEMAN / Tristan) step (oracle/_ref/ref_harness_explicit):
                    the mini grid at dt = 0.4 with fast electrons (u ~ N(0, 0.5) per axis), so that
                    many zig-zag paths cross cells, perturbed Yee fields, 6 steps
+  explicit_smooth_ref.npz  the same with 8 smoothing passes (s = 0.5) of the current every step
 
 Usage: python tests/golden/make_golden.py [mini|config1]...
 """
@@ -295,10 +296,10 @@ def write_input_explicit(path, w, st, yee, steps):
         rec["x"] = st.x
         rec["p"] = st.p
         f.write(rec.tobytes())
+        f.write(struct.pack("<id", w.smoothing_count, w.smoothing_parameter))
 
 
-def make_explicit():
-    w, st, yee = explicit_case()
+def run_explicit(w, st, yee):
     with tempfile.TemporaryDirectory() as td:
         for d in REF_DIRS:
             os.makedirs(os.path.join(td, d), exist_ok=True)
@@ -308,7 +309,17 @@ def make_explicit():
         out = read_output(fo)
     ins = {"in/species": st.species, "in/ids": st.ids, "in/x": st.x, "in/p": st.p}
     ins.update({"in/" + k: v for k, v in yee.items()})
+    return ins, out
+
+
+def make_explicit():
+    w, st, yee = explicit_case()
+    ins, out = run_explicit(w, st, yee)
     np.savez_compressed(os.path.join(HERE, "explicit_ref.npz"), **ins, **out)
+    # the same with the shipped input's current smoothing (8 passes of s = 0.5 every step)
+    ws = dataclasses.replace(w, smoothing_count=8, smoothing_parameter=0.5, name="explicit-smooth")
+    ins, out = run_explicit(ws, st, yee)
+    np.savez_compressed(os.path.join(HERE, "explicit_smooth_ref.npz"), **ins, **out)
 
 
 def make_config1():
