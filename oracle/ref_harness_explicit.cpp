@@ -13,6 +13,7 @@
 // Output: named float64 arrays, as oracle/ref_harness.cpp writes them.
 #include <mpi.h>
 #include <cstdint>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -20,6 +21,7 @@
 
 #include "particle.h"
 #include "simulation.h"
+#include "boundaryFieldEvaluator.h"
 #include "vector3d.h"
 
 namespace {
@@ -154,6 +156,54 @@ int main(int argc, char** argv) {
 	if (fread(&smooth_count, sizeof(int32_t), 1, in) == 1) smooth_param = rd<double>(in);
 	s.smoothingCount = smooth_count;
 	s.smoothingParameter = smooth_param;
+	// optional trailer: x boundary (1: SUPER_CONDUCTOR_LEFT with the fields E0, B0 at the free
+	// right end, as initializeFluxFromRight sets up, PIC++/initialization.cpp:5201-5210; the
+	// Tristan mover reflects at xgrid[reflectingWallPoint]), then injection (V0, per species the
+	// particles per bin and kT / (m c^2)), each step injectNewParticles and addToPreserveChargeGlobal
+	// after the particle exchange (PIC++/simulation.cpp:274-289)
+	int32_t boundary = 0, inject = 0;
+	if (fread(&boundary, sizeof(int32_t), 1, in) != 1) boundary = 0;
+	if (boundary == 1) {
+		Vector3d e_right, b_right;
+		e_right.x = rd<double>(in);
+		e_right.y = rd<double>(in);
+		e_right.z = rd<double>(in);
+		b_right.x = rd<double>(in);
+		b_right.y = rd<double>(in);
+		b_right.z = rd<double>(in);
+		s.boundaryConditionTypeX = SUPER_CONDUCTOR_LEFT;
+		s.E0 = e_right;  // tristanEvaluateE holds E0 at the free end (PIC++/fields.cpp:117-141)
+		s.B0 = b_right;
+		s.rightBoundaryFieldEvaluator = new ConstantBoundaryFieldEvaluator(e_right, b_right);
+		if (fread(&inject, sizeof(int32_t), 1, in) != 1) inject = 0;
+	}
+	if (inject == 1) {
+		s.V0.x = rd<double>(in);
+		s.V0.y = rd<double>(in);
+		s.V0.z = rd<double>(in);
+		for (int t = 0; t < s.typesNumber; ++t) s.types[t].particlesPerBin = 0;
+		for (int q = 0; q < nspecies; ++q) {
+			const int ppc = rd<int32_t>(in);
+			const double theta = rd<double>(in);
+			int t = 0;
+			while (t < s.typesNumber && s.types[t].type != (ParticleTypes)sp_type[q]) ++t;
+			if (t == s.typesNumber) {
+				fprintf(stderr, "species type %d not found\n", sp_type[q]);
+				return 2;
+			}
+			ParticleTypeContainer& tc = s.types[t];
+			tc.mass = sp_mass[q];
+			tc.charge = sp_charge[q];
+			tc.chargeCount = sp_charge[q] > 0 ? 1 : -1;
+			tc.particlesPerBin = ppc;
+			tc.particesDeltaX = s.deltaX / ppc;
+			tc.injectionLength = tc.particesDeltaX - 0.0001 * s.deltaX;  // PIC++/simulation.cpp:125-127
+			const double T = theta * sp_mass[q] / s.kBoltzman_normalized;
+			tc.temperatureX = tc.temperatureY = tc.temperatureZ = T;
+			tc.concentration = sp_weight[q] * ppc / s.volumeB();
+		}
+		s.preserveChargeGlobal = true;
+	}
 	fclose(in);
 	s.particlesNumber = (int)np;
 
@@ -199,9 +249,14 @@ int main(int argc, char** argv) {
 			for (int c = 0; c < 3; ++c) dump_yee(*jy[c], tag(jn[c], step));
 		s.tristanEvaluateE();
 		s.exchangeBunemanEfield(s.bunemanEx, s.bunemanEy, s.bunemanEz);
-		// :258-263, :388
+		// :258-263, :274-289, :388
 		s.eraseEscapedPaticles();
 		s.exchangeParticles();
+		if (inject == 1) {
+			for (int t = 0; t < s.typesNumber; ++t) s.types[t].injectionLength += fabs(s.V0.x * s.deltaT);
+			s.injectNewParticles();
+			s.addToPreserveChargeGlobal();
+		}
 		s.removeEscapedParticles();
 		s.time += s.deltaT;
 		s.currentIteration++;

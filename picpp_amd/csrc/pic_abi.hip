@@ -378,6 +378,9 @@ void wall_particles(pic_ctx* c) {
 				// the new particles into the theoretical totals (injectNewParticles, PIC++/mover.cpp:994-996)
 				launch_theo_particles(c->sp, c->P, c->Q, c->fp32, c->species_start, c->keys, c->g.ncells, nullptr, at,
 				                      b.count * ncols, 1.0, c->theo, s);
+				if (c->explicit_scheme)  // the next step's current scale covers them
+					launch_species_u2_added(c->sp, c->P, c->species_start, c->keys, c->g.ncells, nullptr, at, b.count * ncols,
+					                        c->u2_slots, s);
 				at += b.count * ncols;
 				c->next_id += b.count * ncols;
 			}
@@ -394,6 +397,9 @@ void wall_particles(pic_ctx* c) {
 			// addToPreserveChargeGlobal's new particles (PIC++/initialization.cpp:8319-8321)
 			launch_theo_particles(c->sp, c->P, c->Q, c->fp32, c->species_start, c->keys, c->g.ncells, c->esc_sorted, 0, removed,
 			                      1.0, c->theo, s);
+			if (c->explicit_scheme)
+				launch_species_u2_added(c->sp, c->P, c->species_start, c->keys, c->g.ncells, c->esc_sorted, 0, removed, c->u2_slots,
+				                        s);
 			c->next_id += removed;
 			removed = 0;
 		}
@@ -830,8 +836,9 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	    (cfg->scheme != PIC_SCHEME_IMPLICIT || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->filter_cut_wave_number < 0 ||
 	     cfg->divergence_cleanup < 0 || cfg->divergence_cleanup > 3))
 		return PIC_ERR_ARGUMENT;
-	if (cfg->scheme == PIC_SCHEME_EXPLICIT &&  // the explicit step is built FP64, periodic, without injection
-	    (cfg->precision != PIC_PRECISION_FP64 || cfg->boundary_x != PIC_BOUNDARY_PERIODIC || cfg->inject))
+	if (cfg->scheme == PIC_SCHEME_EXPLICIT &&  // the explicit step is built FP64; its wall needs the particles' wall
+	                                          // (cell 20) inside the box
+	    (cfg->precision != PIC_PRECISION_FP64 || (cfg->boundary_x == PIC_BOUNDARY_WALL && cfg->nx < 20)))
 		return PIC_ERR_ARGUMENT;
 	if (cfg->inject) {  // injection needs the wall set-up and createParticle's Maxwellian branch
 		if (cfg->inject != 1 || cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
@@ -2058,10 +2065,12 @@ int pic_explicit_step(pic_ctx* c) {
 		{
 			Timed t(c, "tristan_push", (double)c->P.count * 100 + (double)g.ncells * 6 * 8 * 8);
 			launch_tristan_push(g, c->sp, c->P, c->species_start, c->yee, dt, scale, u2_limit, c->keys, c->mask, c->status,
-			                    c->u2_slots, s);
+			                    c->u2_slots, c->dead, c->esc, c->bounce, s);
 		}
 		c->u2_valid = true;
 		check_status(c);
+		if (g.wall)  // the reflections' momentum for the theoretical totals (mover.cpp:749)
+			launch_theo_bounce(c->sp, c->P, c->Q, c->fp32, c->species_start, c->bounce, c->np(), c->epartial, c->theo, s);
 		{
 			Timed t(c, "yee_b_half", (double)g.ncells * 3 * 8 * 3);
 			launch_tristan_b_half(g, c->yee, dt, s);
@@ -2088,7 +2097,10 @@ int pic_explicit_step(pic_ctx* c) {
 		// the Poynting flux through the x boundaries with the new fields (updateTheoreticalEnergy
 		// after the BUNEMAN branch, PIC++/simulation.cpp:272); zero with periodic x
 		launch_theo_poynting(g, yee_view(c->yee), dt, c->theo, s);
-		rebin_particles(c);
+		if (g.wall)  // removals, replacements and injection, then the rebin
+			wall_particles(c);
+		else
+			rebin_particles(c);
 	} catch (const DeviceError& e) {
 		return fail(c, e);
 	}

@@ -67,7 +67,7 @@ __global__ void k_b_half(Grid g, YeeGrid y, double cdtd2) {
 
 // tristanEvaluateE over the owned box: Ex on i < N-2, Ey on j < N-2, Ez on k < N-2
 // (fields.cpp:81-150, periodic x).
-__global__ void k_e(Grid g, YeeGrid y, double dt, double four_pi) {
+__global__ void k_e(Grid g, YeeGrid y, double dt, double four_pi, double e0y, double e0z) {
 	const int ni = g.N[0] - 3, nj = g.N[1] - 3, nk = g.N[2] - 3;
 	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= (long long)ni * nj * nk) return;
@@ -88,12 +88,17 @@ __global__ void k_e(Grid g, YeeGrid y, double dt, double four_pi) {
 		r += (Bx[yi(y.dB[0], i, j, k)] - Bx[yi(y.dB[0], i, j, k - 1)]) / g.d[2];
 		const long long q = yi(y.dE[1], i, j, k);
 		y.E[1][q] = y.E[1][q] + (kCprimeX * r - four_pi * y.J[1][q]) * dt;
+		// the wall: zero at the conductor node 2 and E0 at the free end (fields.cpp:117-122)
+		if (g.wall && i <= 2) y.E[1][q] = 0;
+		if (g.wall && i >= g.N[0] - 2) y.E[1][q] = e0y;
 	}
 	if (k < hz) {  // evaluateBunemanRotBz (:3329-3338)
 		double r = (By[yi(y.dB[1], i, j, k)] - By[yi(y.dB[1], i - 1, j, k)]) / g.d[0];
 		r -= (Bx[yi(y.dB[0], i, j, k)] - Bx[yi(y.dB[0], i, j - 1, k)]) / g.d[1];
 		const long long q = yi(y.dE[2], i, j, k);
 		y.E[2][q] = y.E[2][q] + (kCprimeX * r - four_pi * y.J[2][q]) * dt;
+		if (g.wall && i <= 2) y.E[2][q] = 0;  // (fields.cpp:136-141)
+		if (g.wall && i >= g.N[0] - 2) y.E[2][q] = e0z;
 	}
 }
 
@@ -139,6 +144,20 @@ __global__ void k_yee_exchange(Yee3 y, int ax, int Nax) {
 		at(i) = at(Nax - 4 + i);
 	}
 	if (node) at(Nax - 2) = at(2);
+}
+
+// The wall's ghost planes along x (the one-rank non-periodic branches of exchangeBuneman*AlongX,
+// PIC++/fields_exchange.cpp:2537-4215, SUPER_CONDUCTOR_LEFT): planes 0 and 1 take zero for the
+// components in zero_mask (bit c) and plane 2's value for the others; the right end keeps its
+// values.
+__global__ void k_yee_wall_x(Yee3 y, int zero_mask) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	int c, idx[3];
+	if (!line_of(y, 0, t, c, idx)) return;
+	const int* d = y.d[c];
+	double* f = y.f[c];
+	const double v = (zero_mask >> c) & 1 ? 0.0 : f[yi(d, 2, idx[1], idx[2])];
+	for (int i = 0; i <= 1; ++i) f[yi(d, i, idx[1], idx[2])] = v;
 }
 
 // Periodic fold of the current along ax: N points f[N-1-i] += f[3-i], f[3-i] = f[N-1-i]
@@ -245,6 +264,29 @@ __device__ inline void species_u2_max(int s, double u2, unsigned long long* slot
 	} else if (s >= 0) {
 		atomicMax(row + s, (unsigned long long)__double_as_longlong(u2));
 	}
+}
+
+// The same for particles added after a push (injected: species from the sort key; replacements
+// in escaped slots: species from the ranges below species_start[nspecies]), idx[0 .. count) or
+// first .. first + count - 1.
+__global__ void k_species_u2_added(SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start,
+                                   const uint32_t* __restrict__ keys, long long ncells, const uint32_t* __restrict__ idx,
+                                   long long first, long long count, unsigned long long* __restrict__ out) {
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	int s = -1;
+	double u2 = 0;
+	if (t < count) {
+		const long long i = idx ? (long long)idx[t] : first + t;
+		if (i >= species_start[sp.count]) {
+			s = (int)((long long)(keys[i] >> 3) / ncells);
+		} else {
+			s = 0;
+			while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
+		}
+		const double p0 = P.p[0][i], p1 = P.p[1][i], p2 = P.p[2][i];
+		u2 = (p0 * p0 + p1 * p1 + p2 * p2) / (sp.mass[s] * sp.mass[s]);
+	}
+	species_u2_max(s, u2, out);
 }
 
 // The slots' maxima per species into out[kMaxSpecies]: one workgroup of kU2Slots threads.
@@ -388,7 +430,12 @@ struct PartSum {
 	bool moved;  // the particle's sort key changed
 	int species;  // -1 for no particle
 	double u2;    // |p|^2 / m^2 after the move
+	bool bounced;  // reflected at the wall in this move
 };
+
+// The particles' wall of the Tristan mover with SUPER_CONDUCTOR_LEFT: x = xgrid[reflectingWallPoint]
+// (PIC++/constants.h:64), cell 20 (the fields' conductor stays at node 2).
+constexpr int kWallCell = 20;
 
 template <int CTRL, int ROWS>
 __device__ inline void seg_step(long long q[12], bool take) {
@@ -414,14 +461,18 @@ __device__ inline void wave_run_sums(long long q[12], int lane, int dist) {
 	seg_step<0x143, 0xC>(q, dist > lane - 32);
 }
 
+template <bool WALL>
 __device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* __restrict__ species_start,
                                  const YeeGrid& y, double dt, const TristanConsts& K, uint32_t* __restrict__ keys,
-                                 int* __restrict__ status, long long i, long long* box, const JBox& jb, PartSum& ps);
+                                 int* __restrict__ status, int* __restrict__ dead, uint32_t* __restrict__ esc, long long i,
+                                 long long* box, const JBox& jb, PartSum& ps);
 
+template <bool WALL>
 __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start,
                                                       YeeGrid y, double dt, TristanConsts K, uint32_t* __restrict__ keys,
                                                       uint64_t* __restrict__ mask, int* __restrict__ status,
-                                                      unsigned long long* __restrict__ u2_slots) {
+                                                      unsigned long long* __restrict__ u2_slots, int* __restrict__ dead,
+                                                      uint32_t* __restrict__ esc, uint64_t* __restrict__ bounce_mask) {
 	__shared__ long long box[kJBoxX + kJBoxY + kJBoxZ];
 	__shared__ int row_ends[2][3];
 	const long long b0 = (long long)blockIdx.x * blockDim.x;
@@ -445,8 +496,13 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 	ps.moved = false;
 	ps.species = -1;
 	ps.u2 = 0;
-	tristan_particle(g, sp, P, species_start, y, dt, K, keys, status, idx, box, jb, ps);
+	ps.bounced = false;
+	tristan_particle<WALL>(g, sp, P, species_start, y, dt, K, keys, status, dead, esc, idx, box, jb, ps);
 	species_u2_max(ps.species, ps.u2, u2_slots);
+	if (WALL) {  // one bit per particle reflected at the wall
+		const unsigned long long rb = __ballot(ps.bounced);
+		if ((threadIdx.x & 63) == 0 && idx < P.count) bounce_mask[idx >> 6] = rb;
+	}
 	{
 		const int lane = (int)(threadIdx.x & 63);
 		// one mover bit per particle: bit i & 63 of word i >> 6 (waves are 64 consecutive
@@ -480,10 +536,12 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 // changes cell goes to the sums here; the first part, with the second when both share the
 // cell, is left in ps for the wave's run sums. Divisions by the cell sizes, by dt and by m^2 are
 // products with the host's reciprocals.
+template <bool WALL>
 __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P,
                                                  const int* __restrict__ species_start, const YeeGrid& y, double dt,
                                                  const TristanConsts& K, uint32_t* __restrict__ keys, int* __restrict__ status,
-                                                 long long i, long long* box, const JBox& jb, PartSum& ps) {
+                                                 int* __restrict__ dead, uint32_t* __restrict__ esc, long long i, long long* box,
+                                                 const JBox& jb, PartSum& ps) {
 	for (int v = 0; v < 12; ++v) ps.q[v] = 0;
 	if (i >= P.count) return;
 	int s = 0;
@@ -592,6 +650,13 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 		v[a] = p[a] * iv;
 		x[a] += v[a] * dt;
 	}
+	const double xw = g.x0[0] + kWallCell * g.d[0];  // xgrid[reflectingWallPoint]
+	if (WALL && x[0] < xw) {  // reflected at the wall (mover.cpp:746-750, reflectMomentumX)
+		x[0] = 2 * xw - x[0];
+		p[0] = -p[0];
+		v[0] = -v[0];
+		ps.bounced = true;
+	}
 	// addParticleFluxZigzag at the unwrapped position
 	double loc[3], prev[3], r[3];
 	int cur[3], pre[3];
@@ -599,6 +664,11 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 		loc[a] = x[a] - g.x0[a];
 		cur[a] = (int)floor(loc[a] * K.inv_d[a]);
 		prev[a] = loc[a] - v[a] * dt;
+	}
+	// the wall cell's path of a reflected particle starts from the mirror of its previous
+	// position (momentum_collecting.cpp:93-99)
+	if (WALL && cur[0] == kWallCell && v[0] > 0 && prev[0] < kWallCell * g.d[0]) prev[0] = 2 * kWallCell * g.d[0] - prev[0];
+	for (int a = 0; a < 3; ++a) {
 		pre[a] = (int)floor(prev[a] * K.inv_d[a]);
 		r[a] = (loc[a] + prev[a]) / 2;
 		if (cur[a] != pre[a]) r[a] = fmax(g.x0[a] + pre[a] * g.d[a], g.x0[a] + cur[a] * g.d[a]) - g.x0[a];
@@ -634,8 +704,21 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 	ps.b = pre[1];
 	ps.c = pre[2];
 	ps.cell = (int)(((long long)pre[0] * g.N[1] + pre[1]) * g.N[2] + pre[2]);
-	// one-rank periodic wrap (mover.cpp:770-806 classification, exchangeParticles' shift)
-	for (int a = 0; a < 3; ++a) {
+	if (WALL && x[0] > g.x0[0] + (g.N[0] - 2) * g.d[0]) {
+		// out through the free right end (mover.cpp:760-767): its current is summed above, as the
+		// reference's flux loop still holds it; listed for removal or replacement (y, z unwrapped)
+		for (int a = 0; a < 3; ++a) {
+			P.x[a][i] = x[a];
+			P.p[a][i] = p[a];
+		}
+		keys[i] = 0xFFFFFFFFu;
+		esc[atomicAdd(dead, 1)] = (uint32_t)i;
+		ps.moved = true;
+		return;
+	}
+	// one-rank periodic wrap (mover.cpp:770-806 classification, exchangeParticles' shift); x too
+	// unless walled
+	for (int a = WALL ? 1 : 0; a < 3; ++a) {
 		const double lo = g.x0[a] + 2 * g.d[a], hi = g.x0[a] + (g.N[a] - 2) * g.d[a];
 		if (x[a] < lo)
 			x[a] += g.L[a];
@@ -666,15 +749,19 @@ void launch_tristan_b_half(const Grid& g, const YeeGrid& y, double dt, hipStream
 void launch_tristan_e(const Grid& g, const YeeGrid& y, double dt, hipStream_t s) {
 	const long long n = (long long)(g.N[0] - 3) * (g.N[1] - 3) * (g.N[2] - 3);
 	const double pi = 4 * atan2(1.0, 1.0);  // PIC++/constants.h:28-29
-	hipLaunchKernelGGL(k_e, dim3(nb(n, 256)), dim3(256), 0, s, g, y, dt, 4 * pi);
+	hipLaunchKernelGGL(k_e, dim3(nb(n, 256)), dim3(256), 0, s, g, y, dt, 4 * pi, g.e_right[1], g.e_right[2]);
 }
 
-static void yee_axes(const Grid& g, Yee3& q, bool fold, hipStream_t s) {
+// The exchanges (or, with fold, the current's folds) along the three axes. With the wall, x
+// has no folds and its ghost planes follow the wall's rule (zero_mask: the components zeroed).
+static void yee_axes(const Grid& g, Yee3& q, bool fold, hipStream_t s, int zero_mask = 0) {
 	for (int ax = 0; ax < 3; ++ax) {
 		const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
 		long long lines = 0;
 		for (int c = 0; c < 3; ++c) lines += (long long)q.d[c][o1] * q.d[c][o2];
-		if (fold)
+		if (g.wall && ax == 0) {
+			if (!fold) hipLaunchKernelGGL(k_yee_wall_x, dim3(nb(lines, 256)), dim3(256), 0, s, q, zero_mask);
+		} else if (fold)
 			hipLaunchKernelGGL(k_yee_fold, dim3(nb(lines, 256)), dim3(256), 0, s, q, ax, g.N[ax]);
 		else
 			hipLaunchKernelGGL(k_yee_exchange, dim3(nb(lines, 256)), dim3(256), 0, s, q, ax, g.N[ax]);
@@ -687,7 +774,7 @@ void launch_yee_exchange(const Grid& g, const YeeGrid& y, bool b_field, hipStrea
 		q.f[c] = b_field ? y.B[c] : y.E[c];
 		for (int a = 0; a < 3; ++a) q.d[c][a] = b_field ? y.dB[c][a] : y.dE[c][a];
 	}
-	yee_axes(g, q, false, s);
+	yee_axes(g, q, false, s, b_field ? 6 : 1);  // the wall zeroes By, Bz and Ex behind it
 }
 
 // smoothBunemanEfieldGeneral (PIC++/smoothing.cpp:535-733) on current component c: the 27-point
@@ -745,7 +832,15 @@ void launch_smooth_yee_current(const Grid& g, const YeeGrid& y, const Smooth27& 
 		q.f[c] = y.J[c];
 		for (int a = 0; a < 3; ++a) q.d[c][a] = d[a];
 	}
-	yee_axes(g, q, false, s);  // exchangeBunemanEfield on the current
+	yee_axes(g, q, false, s, 1);  // exchangeBunemanEfield on the current
+}
+
+void launch_species_u2_added(const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const uint32_t* keys,
+                             long long ncells, const uint32_t* idx, long long first, long long count, unsigned long long* u2_slots,
+                             hipStream_t s) {
+	if (count <= 0) return;
+	hipLaunchKernelGGL(k_species_u2_added, dim3(nb(count, 256)), dim3(256), 0, s, sp, P, species_start, keys, ncells, idx, first,
+	                   count, u2_slots);
 }
 
 void launch_explicit_bounds(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
@@ -763,7 +858,8 @@ void launch_explicit_bounds(const Grid& g, const SpeciesTable& sp, const Particl
 
 void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
                          const YeeGrid& y, double dt, double scale, const double* u2_limit, uint32_t* keys, uint64_t* mask,
-                         int* status, unsigned long long* u2max_out, hipStream_t s) {
+                         int* status, unsigned long long* u2_slots, int* dead, uint32_t* esc, uint64_t* bounce_mask,
+                         hipStream_t s) {
 	if (P.count == 0) return;
 	TristanConsts K{};
 	for (int a = 0; a < 3; ++a) {
@@ -780,8 +876,8 @@ void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleAr
 	}
 	K.scale = scale;
 	for (int q = 0; q < kMaxSpecies; ++q) K.u2_limit[q] = q < sp.count ? u2_limit[q] : 0.0;
-	hipLaunchKernelGGL(k_tristan_push, dim3(nb(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, y, dt, K, keys, mask,
-	                   status, u2max_out);
+	hipLaunchKernelGGL(g.wall ? k_tristan_push<true> : k_tristan_push<false>, dim3(nb(P.count, 256)), dim3(256), 0, s, g, sp, P,
+	                   species_start, y, dt, K, keys, mask, status, u2_slots, dead, esc, bounce_mask);
 }
 
 }  // namespace picgpu

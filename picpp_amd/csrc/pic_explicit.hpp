@@ -52,13 +52,21 @@ constexpr int kU2Slots = 256;
 void launch_explicit_bounds(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
                             const int* start, const YeeGrid& y, bool particles, unsigned long long* u2_slots,
                             ExplicitBounds* out, hipStream_t s);
+// The |u|^2 of particles added after a push (injection, replacement) into the same rows, so that
+// the next step's bound covers them: idx[0 .. count) or first .. first + count - 1.
+void launch_species_u2_added(const SpeciesTable& sp, const ParticleArrays& P, const int* species_start, const uint32_t* keys,
+                             long long ncells, const uint32_t* idx, long long first, long long count, unsigned long long* u2_slots,
+                             hipStream_t s);
 // moveParticleTristan for every particle, the zig-zag current of the move into Jacc (fixed
 // point, scale a power of two), the one-rank periodic wrap, the new sort key and one mover bit
 // per particle in mask (bit i & 63 of word i >> 6: the key changed). Per species, the largest
 // new |u|^2 goes to the rows of u2_slots (cleared by the caller) for the next step's scale; a particle whose
 // new |u|^2 exceeds u2_limit[species] (the bound the scale was set for) sets status bit 16.
+// With the wall (g.wall): particles reflect at x = xgrid[20] (a bit in bounce_mask), and those
+// past node N - 2 leave: key 0xFFFFFFFF, listed in esc (count in dead), y and z unwrapped.
 void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* species_start,
                          const YeeGrid& y, double dt, double scale, const double* u2_limit, uint32_t* keys, uint64_t* mask,
-                         int* status, unsigned long long* u2_slots, hipStream_t s);
+                         int* status, unsigned long long* u2_slots, int* dead, uint32_t* esc, uint64_t* bounce_mask,
+                         hipStream_t s);
 
 }  // namespace picgpu

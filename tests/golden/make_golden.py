@@ -31,6 +31,10 @@ fixtures next to this script:
                    the mini grid at dt = 0.4 with fast electrons (u ~ N(0, 0.5) per axis), so that
                    many zig-zag paths cross cells, perturbed Yee fields, 6 steps
   explicit_smooth_ref.npz  the same with 8 smoothing passes (s = 0.5) of the current every step
+  explicit_wall_ref.npz, explicit_inject_ref.npz  the explicit step with the wall (the particles'
+                   wall at cell 20, the fields' conductor at node 2, the free right end): a gamma 2 flow
+                   at 28x5x5 cells with escapes through the right end, and the same with injection and
+                   replacement, 6 steps
 
 Usage: python tests/golden/make_golden.py [mini|config1]...
 """
@@ -297,6 +301,34 @@ def write_input_explicit(path, w, st, yee, steps):
         rec["p"] = st.p
         f.write(rec.tobytes())
         f.write(struct.pack("<id", w.smoothing_count, w.smoothing_parameter))
+        if w.boundary == "wall":
+            f.write(struct.pack("<i", 1))
+            f.write(struct.pack("<6d", *w.e_right, *w.b_right))
+            f.write(struct.pack("<i", 1 if w.inject else 0))
+            if w.inject:
+                f.write(struct.pack("<3d", *w.flow))
+                for sp in w.species:
+                    f.write(struct.pack("<id", sp.ppc, sp.temperature))
+        else:
+            f.write(struct.pack("<i", 0))
+
+
+def explicit_wall_case(inject=False):
+    """The explicit step with the wall: a gamma 2 flow toward the particles' wall at
+    xgrid[20] (the Tristan mover's reflectingWallPoint) at 28x5x5 cells, the particles right of
+    it (cells 20 .. 29), those of the right half turned to +x so that some leave through the free
+    right end; perturbed Yee fields; 6 steps of dt = 0.5; with `inject` the flow is injected at the
+    right end and every escape replaced."""
+    w = wl.shock(n=(28, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=6, seed=23,
+                 name="explicit-wall", inject=inject)
+    st = wl.make_initial_state(w)
+    x0 = w.origin()[0]
+    keep = st.x[:, 0] >= x0 + 20.25 * w.dx
+    n = int(keep.sum())
+    st = wl.InitialState(st.species[keep], np.arange(n, dtype=np.int32), st.x[keep], st.p[keep], st.E, st.B)
+    right = st.x[:, 0] > x0 + 26 * w.dx
+    st.p[right, 0] = np.abs(st.p[right, 0])
+    return w, st, wl.yee_initial_fields(w, perturb=0.05)
 
 
 def run_explicit(w, st, yee):
@@ -320,6 +352,10 @@ def make_explicit():
     ws = dataclasses.replace(w, smoothing_count=8, smoothing_parameter=0.5, name="explicit-smooth")
     ins, out = run_explicit(ws, st, yee)
     np.savez_compressed(os.path.join(HERE, "explicit_smooth_ref.npz"), **ins, **out)
+    for inject, fname in ((False, "explicit_wall_ref.npz"), (True, "explicit_inject_ref.npz")):
+        ww, sw, yw = explicit_wall_case(inject)
+        ins, out = run_explicit(ww, sw, yw)
+        np.savez_compressed(os.path.join(HERE, fname), **ins, **out)
 
 
 def make_config1():

@@ -112,6 +112,84 @@ def test_explicit_step_with_current_smoothing_matches_reference():
     sim.close()
 
 
+def explicit_wall_workload(inject):
+    return wl.shock(n=(28, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=6, seed=23,
+                    name="explicit-wall", inject=inject)
+
+
+def open_explicit_wall(d, w):
+    N = tuple(v + 4 for v in w.n)
+    st = wl.InitialState(d["in/species"].astype(np.int32), d["in/ids"].astype(np.int32), d["in/x"], d["in/p"],
+                         np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3)), np.zeros(N + (3,)))
+    sim = Simulation(w, precision="fp64", scheme="explicit")
+    sim.load_yee({name: d["in/" + name] for name in wl.YEE_NAMES})
+    sim.load_state(st)
+    return sim
+
+
+def test_explicit_wall_matches_reference():
+    """The wall (tests/golden/explicit_wall_ref.npz): the Tristan mover reflects at xgrid[20]
+    (reflectingWallPoint) and starts the reflected path's current from the mirrored previous
+    position; particles past node N - 2 leave through the free right end (their current still
+    counted); the fields' conductor at node 2 zeroes E_y, E_z there, the free end holds E0, and the
+    x ghost planes follow the one-rank non-periodic exchanges (zero or plane 2 per component, no
+    current folds along x). Particles, Yee fields, current and energies within 1e-12 over 6 steps
+    with about 45 escapes per step."""
+    d = np.load(os.path.join(HERE, "golden", "explicit_wall_ref.npz"))
+    w = explicit_wall_workload(False)
+    print("case", w.name, "particles", len(d["in/ids"]), flush=True)
+    sim = open_explicit_wall(d, w)
+    worst = {}
+    for step in range(1, w.steps + 1):
+        sim.explicit_step()
+        p = sim.particles()
+        ids, x, mom = by_id(p["ids"], p["x"], p["p"])
+        ref = d["s%d/particles" % step]
+        ref = ref[np.argsort(ref[:, 0])]
+        assert np.array_equal(ids.astype(np.float64), ref[:, 0]), step
+        worst["x"] = max(worst.get("x", 0.0), relerr(x, ref[:, 1:4]))
+        worst["p"] = max(worst.get("p", 0.0), relerr(mom, ref[:, 4:7]))
+        for name in wl.YEE_NAMES + (("Jx", "Jy", "Jz") if step <= 2 else ()):
+            worst[name] = max(worst.get(name, 0.0), relerr(sim.yee(name), d["s%d/%s" % (step, name)]))
+        e = np.asarray(sim.updateEnergy())
+        worst["energy"] = max(worst.get("energy", 0.0), relerr(e[:4], d["s%d/energy" % step][:4]))
+    print("explicit wall vs reference:", {k: "%.1e" % v for k, v in worst.items()})
+    for k, v in worst.items():
+        assert v <= 1e-12, (k, v)
+    sim.close()
+
+
+def test_explicit_injection_matches_reference():
+    """Injection at the right end and replacement of the escapes (explicit_inject_ref.npz, the
+    reference's own injectNewParticles and addToPreserveChargeGlobal): every step adds as many
+    particles as the reference; in the first step, before the new particles' current feeds back,
+    the kept particles and the fields match within 1e-12 and the new particles' x positions are
+    the reference's bit for bit (their momenta follow the reference's distribution, not its
+    rand() samples)."""
+    d = np.load(os.path.join(HERE, "golden", "explicit_inject_ref.npz"))
+    w = explicit_wall_workload(True)
+    n0 = len(d["in/ids"])
+    print("case", w.name + "-inject", "particles", n0, flush=True)
+    sim = open_explicit_wall(d, w)
+    for step in range(1, w.steps + 1):
+        sim.explicit_step()
+        p = sim.particles()
+        ref = d["s%d/particles" % step]
+        assert len(p["ids"]) == len(ref), (step, len(p["ids"]), len(ref))
+        if step == 1:
+            old = p["ids"] < n0
+            ids, x, mom = by_id(p["ids"][old], p["x"][old], p["p"][old])
+            r_old = ref[ref[:, 0] < n0]
+            r_old = r_old[np.argsort(r_old[:, 0])]
+            assert np.array_equal(ids.astype(np.float64), r_old[:, 0])
+            assert relerr(x, r_old[:, 1:4]) <= 1e-12 and relerr(mom, r_old[:, 4:7]) <= 1e-12
+            assert np.array_equal(np.sort(p["x"][~old][:, 0]), np.sort(ref[ref[:, 0] >= n0][:, 1]))
+            for name in wl.YEE_NAMES:
+                assert relerr(sim.yee(name), d["s1/" + name]) <= 1e-12, name
+    print("particles per step:", [len(d["s%d/particles" % s]) for s in range(w.steps + 1)])
+    sim.close()
+
+
 def test_explicit_step_deterministic_bitwise():
     d = np.load(os.path.join(HERE, "golden", "explicit_ref.npz"))
     w = explicit_workload()
