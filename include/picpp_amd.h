@@ -84,8 +84,9 @@ enum {
 enum {
 	PIC_SCHEME_IMPLICIT = 0, /* the theta-implicit step (the reference's IMPLICIT solver) */
 	/* The reference's BUNEMAN solver: explicit Yee fields, the Tristan mover and the zig-zag
-	   current (PIC++/simulation.cpp:163-197). FP64, periodic; smoothing_count passes smooth the
-	   current; fields are the Yee arrays (PIC_ARRAY_YEE_*), stepped by pic_explicit_step. */
+	   current (PIC++/simulation.cpp:163-197). FP64; periodic, or PIC_BOUNDARY_WALL with nx >= 20
+	   (the mover's wall at xgrid[20]) and injection; smoothing_count passes smooth the current;
+	   fields are the Yee arrays (PIC_ARRAY_YEE_*), stepped by pic_explicit_step. */
 	PIC_SCHEME_EXPLICIT = 1
 };
 
@@ -125,8 +126,9 @@ typedef struct {
 	int inject;
 	double flow[3];          /* V0, the flow velocity (|V0| < 1) */
 	int ppc[8];              /* particlesPerBin of each species; 0 = not injected */
-	double temperature[8];   /* kT / (m c^2) of each injected species in the flow frame, < 0.1
-	                            (createParticle's Maxwellian branch, initialization.cpp:8507-8514) */
+	double temperature[8];   /* kT / (m c^2) of each injected species in the flow frame
+	                            (createParticle, initialization.cpp:8479-8570: the Maxwellian
+	                            branch below 0.1, the isotropic Juttner branches from 0.1 on) */
 	unsigned long long seed;
 	int pressure;            /* PIC_PRESSURE_* */
 	/* Smoothing after the moments (PIC++/simulation.cpp:213-217): smoothing_count passes over
@@ -146,6 +148,12 @@ typedef struct {
 	int divergence_cleanup;
 	/* > 0: pic_step then runs pic_filter_fields(ctx, filter_cut_wave_number) on newE and newB */
 	int filter_cut_wave_number;
+	/* kBoltzman_normalized x the run's temperature (Simulation::temperature), in units of mass c^2:
+	   createParticle's Juttner branches for species with temperature[s] >= 0.1 choose between
+	   the tabulated ("cold") and the rejection ("hot") sampler by run_kT / mass[s] < 2, a
+	   quirk of the reference that reads the run's temperature there, not the species'
+	   (PIC++/initialization.cpp:8515-8527). */
+	double run_kT;
 } pic_config;
 
 typedef struct {

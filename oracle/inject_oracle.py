@@ -4,8 +4,9 @@ injection kernels; the product never does).
 
 Reference: Simulation::injectNewParticles (PIC++/mover.cpp:910-1000), the injectionLength
 update of the step loop (PIC++/simulation.cpp:125-127, :281-283), addToPreserveChargeGlobal
-(PIC++/initialization.cpp:8281-8338), createParticle's Maxwellian branch
-(PIC++/initialization.cpp:8479-8514), Particle::getVelocity / addVelocity / setMomentumByV
+(PIC++/initialization.cpp:8281-8338), createParticle's Maxwellian and isotropic Juttner branches
+(PIC++/initialization.cpp:8479-8535) with the tabulated CDF of evaluateJuttnerFunction
+(:7372-7394) and the samplers of PIC++/random.cpp:43-66, Particle::getVelocity / addVelocity / setMomentumByV
 (PIC++/particle.cpp:216-232, :291-331, :337-364).
 
 The reference draws from rand(); the HIP path and this restatement draw from Philox4x64-10
@@ -17,9 +18,13 @@ sample. What pins each part:
   (tests/golden/inject_ref.npz), exactly;
 - the placement of slab particles and replacements (columns, common in-cell offset of a
   slab, x of replacements, kept y and z): the same golden vectors;
-- flow_momentum: closed forms -- with V0 = 0 the momenta are sqrt(theta) m N(0, 1) per axis;
-  the boost preserves the invariant mass (gamma^2 - u^2 = 1) and maps the rest-frame mean
-  to gamma_V V along V0; with theta = 0 every particle gets exactly gamma_V m V0.
+- flow_momentum / boost: closed forms -- with V0 = 0 the momenta are sqrt(theta) m N(0, 1) per
+  axis; the boost preserves the invariant mass (gamma^2 - u^2 = 1) and maps the rest-frame mean
+  to gamma_V V along V0; with theta = 0 every particle gets exactly gamma_V m V0;
+- the Juttner branches (rest_u from theta = 0.1 on): the mean Lorentz factor of the samples
+  against the Maxwell-Juttner closed form <gamma> = K1(1/theta) / K2(1/theta) + 3 theta (tabulated
+  branch) and the sampler's own target (rejection branch), isotropy, and the compiled
+  reference's own hot injection (the means and spreads of its momenta).
 """
 from __future__ import annotations
 
@@ -57,14 +62,87 @@ def box_muller(wa: int, wb: int) -> Tuple[float, float]:
     return r * math.cos(a), r * math.sin(a)
 
 
+JUTTNER_POINTS = 4096  # randomParameter, PIC++/constants.h:8
+
+
+def juttner_table(theta: float) -> Tuple[np.ndarray, np.ndarray]:
+    """evaluateJuttnerFunction (PIC++/initialization.cpp:7372-7394): u_i = i 20 theta / (points - 1)
+    and the running sum of u^2 exp(-sqrt(1 + u^2) / theta) du / 10 over 10 sub-steps per interval,
+    normalised to end at exactly 1."""
+    u = np.zeros(JUTTNER_POINTS)
+    F = np.zeros(JUTTNER_POINTS)
+    du = 20 * theta / (JUTTNER_POINTS - 1)
+    sdu = du / 10
+    for i in range(1, JUTTNER_POINTS):
+        u[i] = u[i - 1] + du
+        f = F[i - 1]
+        lu = u[i - 1]
+        for _ in range(10):
+            f += math.exp(-math.sqrt(1 + lu * lu) / theta) * lu * lu * sdu
+            lu += sdu
+        F[i] = f
+    F /= F[-1]
+    F[-1] = 1.0
+    return u, F
+
+
+def juttner_inverse(table: Tuple[np.ndarray, np.ndarray], y: float) -> float:
+    """dichotomySolver (PIC++/specialmath.cpp:931-947) on the table: halve [0, points - 1] until
+    one step wide, then interpolate linearly."""
+    u, F = table
+    lo, hi = 0, JUTTNER_POINTS - 1
+    while lo + 1 != hi:
+        mid = (lo + hi) // 2
+        if F[mid] > y:
+            hi = mid
+        elif F[mid] < y:
+            lo = mid
+        else:
+            return float(u[mid])
+    return float((u[hi] - u[lo]) * (y - F[lo]) / (F[hi] - F[lo]) + u[lo])
+
+
+def rest_u(theta: float, theta_run: float, table, ctr: Sequence[int], key: Sequence[int]) -> List[float]:
+    """createParticle's rest-frame u = p / (m c) (PIC++/initialization.cpp:8505-8535): below
+    theta = 0.1 u_i = sqrt(theta) N(0, 1); from 0.1 on an isotropic direction and |u| from the
+    table (theta_run < 2, maxwellJuttnerMomentumColdDistribution, PIC++/random.cpp:43-48) or from
+    the rejection sampler (maxwellJuttnerMomentumHotDistribution, :50-66, U1 kept over the
+    attempts), with the counters of picpp_amd/csrc/pic_inject.hip."""
+    w = philox4x64_10(ctr, key)
+    if theta < 0.1:
+        n0, n1 = box_muller(w[0], w[1])
+        n2, _ = box_muller(w[2], w[3])
+        st = math.sqrt(theta)
+        return [st * n0, st * n1, st * n2]
+    if theta_run < 2:
+        a = juttner_inverse(table, uniform53(w[0]))
+    else:
+        x1 = uniform53(w[0])
+        a = -theta_run * math.log(uniform53(w[1]) * uniform53(w[2]) * uniform53(w[3]))
+        att = 1
+        while att < 64 and math.exp((a - math.sqrt(1 + a * a)) / theta_run) < x1:
+            w = philox4x64_10((ctr[0], ctr[1], ctr[2], att), key)
+            a = -theta_run * math.log(uniform53(w[1]) * uniform53(w[2]) * uniform53(w[3]))
+            att += 1
+    d = philox4x64_10((ctr[0], ctr[1], ctr[2], 1 << 40), key)
+    uz = a * (2 * uniform53(d[0]) - 1)
+    phi = TWO_PI * uniform53(d[1])
+    un = math.sqrt(a * a - uz * uz)
+    return [un * math.cos(phi), un * math.sin(phi), uz]
+
+
 def flow_momentum(flow: Sequence[float], m: float, theta: float, w: Sequence[int]) -> np.ndarray:
     """p = m gamma v of a Maxwellian particle (p_i = sqrt(m k T) N(0,1),
-    PIC++/initialization.cpp:8507-8514) boosted by V0 with the relativistic velocity addition
-    (PIC++/particle.cpp:291-331), p from v as setMomentumByV (particle.cpp:337-364)."""
+    PIC++/initialization.cpp:8507-8514) from the generator words w, boosted by V0."""
     n0, n1 = box_muller(w[0], w[1])
     n2, _ = box_muller(w[2], w[3])
     st = math.sqrt(theta)
-    u = [st * n0, st * n1, st * n2]
+    return boost(flow, m, [st * n0, st * n1, st * n2])
+
+
+def boost(flow: Sequence[float], m: float, u: Sequence[float]) -> np.ndarray:
+    """The rest-frame u boosted by V0 with the relativistic velocity addition
+    (PIC++/particle.cpp:291-331), p from v as setMomentumByV (particle.cpp:337-364)."""
     g1 = math.sqrt((u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) + 1.0)
     v = [u[0] / g1, u[1] / g1, u[2] / g1]
     V2 = flow[0] * flow[0] + flow[1] * flow[1] + flow[2] * flow[2]
@@ -95,6 +173,8 @@ class Injector:
         self.ppc = [s.ppc for s in w.species]
         self.theta = [s.temperature for s in w.species]
         self.mass = [s.mass for s in w.species]
+        self.theta_run = [getattr(w, "run_kT", 0.0) / s.mass for s in w.species]
+        self.tables = [juttner_table(t) if t >= 0.1 and p > 0 else None for t, p in zip(self.theta, [s.ppc for s in w.species])]
         self.flow = w.flow
         self.seed = w.seed
         # injectionLength at the loop start: particesDeltaX - 1e-4 dx, particesDeltaX = dx / ppc
@@ -138,12 +218,12 @@ class Injector:
                 j, k = 2 + col // n2, 2 + col % n2
                 y = (self.x0[1] + j * self.d) + dy
                 z = (self.x0[2] + k * self.d) + dz
-                c = philox4x64_10((self.step, (0 << 32) | slab, col, 0), key)
+                u = rest_u(self.theta[s], self.theta_run[s], self.tables[s], (self.step, (0 << 32) | slab, col, 0), key)
                 sp.append(s)
                 ids.append(self.next_id)
                 self.next_id += 1
                 xs.append((x, y, z))
-                ps.append(flow_momentum(self.flow, self.mass[s], self.theta[s], c))
+                ps.append(boost(self.flow, self.mass[s], u))
                 kind.append(0)
         xr = self.right_edge()
         for r, (s, y, z) in enumerate(escaped):
@@ -155,12 +235,12 @@ class Injector:
                     pos[a] = lo + self.d * uniform53(wy[2 * (a - 1)])
                 if pos[a] > hi:
                     pos[a] = hi - self.d * uniform53(wy[2 * (a - 1) + 1])
-            c = philox4x64_10((self.step, (1 << 32) | r, 0, 0), key)
+            u = rest_u(self.theta[s], self.theta_run[s], self.tables[s], (self.step, (1 << 32) | r, 0, 0), key)
             sp.append(s)
             ids.append(self.next_id)
             self.next_id += 1
             xs.append(tuple(pos))
-            ps.append(flow_momentum(self.flow, self.mass[s], self.theta[s], c))
+            ps.append(boost(self.flow, self.mass[s], u))
             kind.append(1)
         self.step += 1
         return {"species": np.array(sp, np.int64), "ids": np.array(ids, np.int64),

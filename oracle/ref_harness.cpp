@@ -304,6 +304,9 @@ int main(int argc, char** argv) {
 			tc.temperatureX = tc.temperatureY = tc.temperatureZ = T;
 			tc.concentration = sp_weight[q] * ppc / s.volumeB();  // weight = concentration / ppc * volumeB
 		}
+		// the tabulated Juttner CDFs of the types, from their temperatures (as the reference's
+		// set-ups do before creating particles, PIC++/initialization.cpp:5393)
+		s.evaluateJuttnerFunctions();
 		s.preserveChargeGlobal = true;
 	}
 	// optional trailer: smoothing passes and parameter (smoothingCount, smoothingParameter;
@@ -330,6 +333,11 @@ int main(int argc, char** argv) {
 	// after every updateEnergy; PIC++/simulation.cpp:96, 138-148, 266-272, 381-384)
 	int32_t general = 0;
 	if (fread(&general, sizeof(int32_t), 1, in) != 1) general = 0;
+	// optional trailer: kBoltzman_normalized x the run's temperature in m c^2 units
+	// (Simulation::temperature, which createParticle's Juttner branch choice reads,
+	// PIC++/initialization.cpp:8515-8527)
+	double run_kT = 0.0;
+	if (fread(&run_kT, sizeof(double), 1, in) == 1) s.temperature = run_kT / s.kBoltzman_normalized;
 	fclose(in);
 	s.particlesNumber = (int)np;
 	if (argc > 3) {

@@ -45,7 +45,7 @@ class PicConfig(ctypes.Structure):
                 ("b_right", ctypes.c_double * 3), ("inject", ctypes.c_int), ("flow", ctypes.c_double * 3),
                 ("ppc", ctypes.c_int * 8), ("temperature", ctypes.c_double * 8), ("seed", ctypes.c_ulonglong),
                 ("pressure", ctypes.c_int), ("smoothing_count", ctypes.c_int), ("smoothing_parameter", ctypes.c_double),
-                ("scheme", ctypes.c_int), ("divergence_cleanup", ctypes.c_int), ("filter_cut_wave_number", ctypes.c_int)]
+                ("scheme", ctypes.c_int), ("divergence_cleanup", ctypes.c_int), ("filter_cut_wave_number", ctypes.c_int), ("run_kT", ctypes.c_double)]
 
 
 class PicSolveInfo(ctypes.Structure):

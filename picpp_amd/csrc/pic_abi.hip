@@ -109,6 +109,7 @@ struct pic_ctx {
 	// the wall by the last push
 	double* theo = nullptr;
 	bool theo_init = false;
+	double* juttner = nullptr;  // the injected species' Juttner tables (kMaxSpecies x 2 x kJuttnerPoints)
 	double* gpartial = nullptr;
 	double* gout = nullptr;
 	uint64_t* bounce = nullptr;
@@ -329,7 +330,11 @@ void set_count(pic_ctx* c, long long n) {
 InjectPhysics inject_physics(const pic_ctx* c) {
 	InjectPhysics ph{};
 	for (int a = 0; a < 3; ++a) ph.flow[a] = c->cfg.flow[a];
-	for (int s = 0; s < kMaxSpecies; ++s) ph.theta[s] = s < c->sp.count ? c->cfg.temperature[s] : 0.0;
+	for (int s = 0; s < kMaxSpecies; ++s) {
+		ph.theta[s] = s < c->sp.count ? c->cfg.temperature[s] : 0.0;
+		ph.theta_run[s] = s < c->sp.count ? c->cfg.run_kT / c->sp.mass[s] : 0.0;
+	}
+	ph.juttner = c->juttner;
 	ph.seed = c->cfg.seed;
 	ph.step = c->inj_step;
 	return ph;
@@ -840,12 +845,13 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	                                          // (cell 20) inside the box
 	    (cfg->precision != PIC_PRECISION_FP64 || (cfg->boundary_x == PIC_BOUNDARY_WALL && cfg->nx < 20)))
 		return PIC_ERR_ARGUMENT;
-	if (cfg->inject) {  // injection needs the wall set-up and createParticle's Maxwellian branch
+	if (cfg->inject) {  // injection needs the wall set-up, a flow below c and finite temperatures
 		if (cfg->inject != 1 || cfg->boundary_x != PIC_BOUNDARY_WALL) return PIC_ERR_ARGUMENT;
+		if (!(cfg->run_kT >= 0.0 && std::isfinite(cfg->run_kT))) return PIC_ERR_ARGUMENT;
 		double v2 = cfg->flow[0] * cfg->flow[0] + cfg->flow[1] * cfg->flow[1] + cfg->flow[2] * cfg->flow[2];
 		if (!(v2 < 1.0)) return PIC_ERR_ARGUMENT;
 		for (int s = 0; s < cfg->nspecies; ++s)
-			if (cfg->ppc[s] < 0 || (cfg->ppc[s] > 0 && !(cfg->temperature[s] >= 0.0 && cfg->temperature[s] < 0.1)))
+			if (cfg->ppc[s] < 0 || (cfg->ppc[s] > 0 && !(cfg->temperature[s] >= 0.0 && std::isfinite(cfg->temperature[s]))))
 				return PIC_ERR_ARGUMENT;
 	}
 	pic_ctx* c = new pic_ctx();
@@ -1011,6 +1017,15 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 			}
 			for (int s = 0; s < cfg->nspecies; ++s)  // injectionLength at the loop start (PIC++/simulation.cpp:125-127)
 				if (cfg->inject && cfg->ppc[s] > 0) c->inj_len[s] = cfg->dx / cfg->ppc[s] - 0.0001 * cfg->dx;
+			bool hot = false;  // the tabulated Juttner CDFs of the species injected from kT / mc^2 = 0.1 on
+			for (int s = 0; s < cfg->nspecies; ++s) hot = hot || (cfg->inject && cfg->ppc[s] > 0 && cfg->temperature[s] >= 0.1);
+			if (hot) {
+				std::vector<double> tab((size_t)kMaxSpecies * 2 * kJuttnerPoints, 0.0);
+				for (int s = 0; s < cfg->nspecies; ++s)
+					if (cfg->ppc[s] > 0 && cfg->temperature[s] >= 0.1) juttner_table(cfg->temperature[s], &tab[(size_t)s * 2 * kJuttnerPoints]);
+				c->juttner = dalloc<double>(tab.size());
+				HIP_CHECK(hipMemcpy(c->juttner, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+			}
 		}
 		{
 			int nq = 5 * cfg->nspecies + 1;
@@ -1083,7 +1098,7 @@ void pic_destroy(pic_ctx* c) {
 	                (void*)c->perm, (void*)c->perm_sorted, (void*)c->start, (void*)c->counts, c->sort_temp,
 	                (void*)c->species_start, (void*)c->status, (void*)c->dead, (void*)c->dens_flip, (void*)c->dens_zero, (void*)c->esc, (void*)c->esc_sorted, (void*)c->smooth_tmp, (void*)c->rhs, (void*)c->muc, (void*)c->xsol, (void*)c->work, (void*)c->work2,
 	                (void*)c->Vdev, (void*)c->partial, (void*)c->dscal, (void*)c->ydev, (void*)c->epartial, (void*)c->eout,
-	                (void*)c->theo, (void*)c->gpartial, (void*)c->gout, (void*)c->bounce})
+	                (void*)c->theo, (void*)c->gpartial, (void*)c->gout, (void*)c->bounce, (void*)c->juttner})
 		f(p);
 	for (ParticleArrays* P : {&c->P, &c->Palt}) {
 		for (int a = 0; a < 3; ++a) {

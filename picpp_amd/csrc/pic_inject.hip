@@ -10,6 +10,8 @@
 //   (step, 0 << 32 | slab, col, 0)   the thermal momentum of the slab's particle in column col
 //   (step, 1 << 32 | rank, 0, 0)     the thermal momentum of the rank-th replacement
 //   (step, 1 << 32 | rank, 1, 0)     its y, z redraws when the escaped particle was outside
+// and, for species from kT / mc^2 = 0.1 on, the fourth word numbering the Juttner sampler's
+// attempts, 1 << 40 for the direction (rest_u).
 // A 64-bit output w becomes the uniform (w >> 11) 2^-53 in [0, 1); normals come in pairs from
 // the Box-Muller transform of two uniforms.
 //
@@ -56,19 +58,83 @@ __device__ inline void box_muller(uint64_t wa, uint64_t wb, double& n0, double& 
 	n1 = r * sin(a);
 }
 
-// Momentum p = m gamma v of a particle of the flowing plasma: a Maxwellian momentum of
-// temperature theta in the flow frame (createParticle, PIC++/initialization.cpp:8505-8514:
-// p_i = sqrt(m k T) N(0, 1), the branch for kT / mc^2 < 0.1), its velocity v' = p / (m gamma')
-// (Particle::getVelocity, particle.cpp:216-232), the relativistic velocity addition of V0
-// (Particle::addVelocity, particle.cpp:291-331: along V0, (V + v'_par) / (1 + V v'_par);
-// across, v'_perp / (gamma_V (1 + V v'_par))) and p from the new velocity
-// (Particle::setMomentumByV, particle.cpp:337-364).
-__device__ inline void flow_momentum(const InjectPhysics& ph, double m, double theta, const uint64_t w[4], double p[3]) {
-	double n[4];
-	box_muller(w[0], w[1], n[0], n[1]);
-	box_muller(w[2], w[3], n[2], n[3]);
-	const double st = sqrt(theta);
-	double u[3] = {st * n[0], st * n[1], st * n[2]};
+// |u| = |p| / (m c) from a species' tabulated Juttner CDF for the uniform y
+// (dichotomySolver, PIC++/specialmath.cpp:931-947: halve the index interval [0, points - 1]
+// until it is one step wide, then interpolate linearly; an exact hit returns the point).
+__device__ inline double juttner_inverse(const double* tab, double y) {
+	const double* u = tab;
+	const double* F = tab + kJuttnerPoints;
+	int lo = 0, hi = kJuttnerPoints - 1;
+	while (lo + 1 != hi) {
+		const int mid = (lo + hi) / 2;
+		if (F[mid] > y)
+			hi = mid;
+		else if (F[mid] < y)
+			lo = mid;
+		else
+			return u[mid];
+	}
+	return (u[hi] - u[lo]) * (y - F[lo]) / (F[hi] - F[lo]) + u[lo];
+}
+
+// The rest-frame u = p / (m c) of createParticle (PIC++/initialization.cpp:8505-8535). Below
+// kT / mc^2 = 0.1 the Maxwellian u_i = sqrt(theta) N(0, 1) from w (two Box-Muller pairs).
+// From 0.1 on, an isotropic direction (u_z = |u| (2 U - 1), phi = 2 pi U) and |u| from the
+// species' table when the run's theta is below 2 (maxwellJuttnerMomentumColdDistribution,
+// PIC++/random.cpp:43-48), else from the rejection sampler of maxwellJuttnerMomentumHotDistribution
+// (:50-66: |u| = -theta log(U2 U3 U4), kept when exp((|u| - sqrt(1 + u^2)) / theta) >= U1).
+// Counters: (c0, c1, c2, a) for attempt a of the rejection (a = 0 also the table's uniform),
+// (c0, c1, c2, 1 << 40) for the direction.
+__device__ inline void rest_u(const InjectPhysics& ph, int s, const uint64_t c[4], double u[3]) {
+	const double theta = ph.theta[s];
+	uint64_t w[4] = {c[0], c[1], c[2], c[3]};
+	philox4x64_10(w, ph.seed, 0);
+	if (theta < 0.1) {
+		double n[4];
+		box_muller(w[0], w[1], n[0], n[1]);
+		box_muller(w[2], w[3], n[2], n[3]);
+		const double st = sqrt(theta);
+		u[0] = st * n[0];
+		u[1] = st * n[1];
+		u[2] = st * n[2];
+		return;
+	}
+	double a;
+	const double t1 = ph.theta_run[s];
+	if (t1 < 2) {
+		a = juttner_inverse(ph.juttner + (size_t)s * 2 * kJuttnerPoints, uniform53(w[0]));
+	} else {
+		// U1 is drawn once and kept for every attempt, as the reference does; the loop ends after
+		// 64 attempts at most
+		const double x1 = uniform53(w[0]);
+		a = -t1 * log(uniform53(w[1]) * uniform53(w[2]) * uniform53(w[3]));
+		for (uint64_t att = 1; att < 64 && exp((a - sqrt(1 + a * a)) / t1) < x1; ++att) {
+			w[0] = c[0];
+			w[1] = c[1];
+			w[2] = c[2];
+			w[3] = att;
+			philox4x64_10(w, ph.seed, 0);
+			a = -t1 * log(uniform53(w[1]) * uniform53(w[2]) * uniform53(w[3]));
+		}
+	}
+	uint64_t d[4] = {c[0], c[1], c[2], 1ull << 40};
+	philox4x64_10(d, ph.seed, 0);
+	const double uz = a * (2 * uniform53(d[0]) - 1);
+	const double phi = kTwoPi * uniform53(d[1]);
+	const double un = sqrt(a * a - uz * uz);
+	u[0] = un * cos(phi);
+	u[1] = un * sin(phi);
+	u[2] = uz;
+}
+
+// Momentum p = m gamma v of a particle of the flowing plasma: the rest-frame u of createParticle
+// (rest_u), its velocity v' = u / gamma' (Particle::getVelocity, particle.cpp:216-232), the
+// relativistic velocity addition of V0 (Particle::addVelocity, particle.cpp:291-331: along V0,
+// (V + v'_par) / (1 + V v'_par); across, v'_perp / (gamma_V (1 + V v'_par))) and p from the new
+// velocity (Particle::setMomentumByV, particle.cpp:337-364).
+__device__ inline void flow_momentum(const InjectPhysics& ph, double m, int s, const uint64_t c[4], double p[3]) {
+	double u[3];
+	rest_u(ph, s, c, u);
 	const double g1 = sqrt((u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) + 1.0);
 	double v[3] = {u[0] / g1, u[1] / g1, u[2] / g1};
 	const double V2 = ph.flow[0] * ph.flow[0] + ph.flow[1] * ph.flow[1] + ph.flow[2] * ph.flow[2];
@@ -154,10 +220,9 @@ __global__ void k_inject(Grid g, SpeciesTable sp, InjectSlabs S, InjectPhysics p
 	x[0] = S.x[sl];
 	x[1] = (g.x0[1] + j * g.d[1]) + dy;  // ygrid[j] + tempDeltaY (PIC++/mover.cpp:976-977)
 	x[2] = (g.x0[2] + k * g.d[2]) + dz;
-	uint64_t c[4] = {ph.step, (0ull << 32) | S.slab[sl], (uint64_t)col, 0};
-	philox4x64_10(c, ph.seed, 0);
+	const uint64_t c[4] = {ph.step, (0ull << 32) | S.slab[sl], (uint64_t)col, 0};
 	double p[3];
-	flow_momentum(ph, sp.mass[s], ph.theta[s], c, p);
+	flow_momentum(ph, sp.mass[s], s, c, p);
 	store_particle(g, s, sp, x, p, ph.id_base + (int)t, P, Q, fp32 != 0, first + t, keys);
 }
 
@@ -188,10 +253,9 @@ __global__ void k_replace(Grid g, SpeciesTable sp, InjectPhysics ph, const int* 
 		if (x[a] < lo) x[a] = lo + g.d[a] * uniform53(wy[2 * (a - 1)]);
 		if (x[a] > hi) x[a] = hi - g.d[a] * uniform53(wy[2 * (a - 1) + 1]);
 	}
-	uint64_t c[4] = {ph.step, (1ull << 32) | (uint64_t)r, 0, 0};
-	philox4x64_10(c, ph.seed, 0);
+	const uint64_t c[4] = {ph.step, (1ull << 32) | (uint64_t)r, 0, 0};
 	double p[3];
-	flow_momentum(ph, sp.mass[s], ph.theta[s], c, p);
+	flow_momentum(ph, sp.mass[s], s, c, p);
 	store_particle(g, s, sp, x, p, ph.id_base + r, P, Q, fp32 != 0, i, keys);
 }
 
@@ -208,6 +272,29 @@ __global__ void k_philox(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uin
 inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
+
+void juttner_table(double theta, double* out) {
+	double* u = out;
+	double* F = out + kJuttnerPoints;
+	const double umax = 20 * theta;
+	const double du = umax / (kJuttnerPoints - 1);
+	const int sub = 10;
+	const double sdu = du / sub;
+	u[0] = 0;
+	F[0] = 0;
+	for (int i = 1; i < kJuttnerPoints; ++i) {
+		u[i] = u[i - 1] + du;
+		F[i] = F[i - 1];
+		double lu = u[i - 1];
+		for (int j = 0; j < sub; ++j) {  // juttnerFunction, PIC++/specialmath.cpp:949-951
+			F[i] += std::exp(-std::sqrt(1 + lu * lu) / theta) * lu * lu * sdu;
+			lu += sdu;
+		}
+	}
+	const double last = F[kJuttnerPoints - 1];
+	for (int i = 0; i < kJuttnerPoints; ++i) F[i] /= last;
+	F[kJuttnerPoints - 1] = 1.0;
+}
 
 std::vector<InjectSlabs> plan_injection(const Grid& g, int nspecies, const int* ppc, double flow_x, double dt, double* len,
                                         unsigned& slab_counter) {

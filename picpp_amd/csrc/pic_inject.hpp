@@ -22,13 +22,25 @@ struct InjectSlabs {
 };
 
 // Physics and random streams shared by injection and replacement.
+// Points of the tabulated Maxwell-Juttner CDF of a species (randomParameter, PIC++/constants.h:8).
+constexpr int kJuttnerPoints = 4096;
+
 struct InjectPhysics {
 	double flow[3];               // V0, |V0| < 1
-	double theta[kMaxSpecies];    // kT / (m c^2) of each species, < 0.1
+	double theta[kMaxSpecies];    // kT / (m c^2) of each species
+	double theta_run[kMaxSpecies];  // the run's kT / (m_s c^2): the Juttner branch choice
+	// for species with theta >= 0.1 and theta_run < 2: kJuttnerPoints values u then the
+	// normalised CDF per species (evaluateJuttnerFunction, PIC++/initialization.cpp:7372-7394)
+	const double* juttner;
 	unsigned long long seed;      // Philox key
 	unsigned long long step;      // step number: first counter word
 	int id_base;                  // id of the first particle this launch creates
 };
+
+// The table of one species: u_i = i 20 theta / (points - 1) and the running integral of
+// u^2 exp(-sqrt(1 + u^2) / theta) over 10 sub-steps per interval, normalised to end at 1
+// (evaluateJuttnerFunction's arithmetic). out: values then CDF, 2 kJuttnerPoints doubles.
+void juttner_table(double theta, double* out);
 
 // The slabs of one step, in the reference's order (species, then slab), for species with
 // ppc[s] > 0: injectionLength len[s] grows by |V0_x| dt, and while ppc len >= dx a slab is

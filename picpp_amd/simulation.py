@@ -63,6 +63,7 @@ class Simulation:
         cfg.scheme = {"implicit": _lib.SCHEME_IMPLICIT, "explicit": _lib.SCHEME_EXPLICIT}[scheme]
         cfg.divergence_cleanup = int(workload.divergence_cleanup)
         cfg.filter_cut_wave_number = workload.filter_cut_wave_number
+        cfg.run_kT = workload.run_kT
         for a in range(3):
             cfg.e_right[a] = workload.e_right[a]
             cfg.b_right[a] = workload.b_right[a]

@@ -78,6 +78,7 @@ class Workload:
     # (PIC++/fields_filtering.cpp:21-295; 0: off)
     divergence_cleanup: int = 0
     filter_cut_wave_number: int = 0
+    run_kT: float = 0.0  # kB T of the run (Simulation::temperature) in m c^2 units: the Juttner branch choice
 
     @property
     def N(self) -> Tuple[int, int, int]:

@@ -15,6 +15,9 @@ fixtures next to this script:
                    +x, so that they leave through the free right end
   inject_ref.npz   the escape case with the flow injected at the right end every step and
                    every particle leaving there replaced (the reference's own injection)
+  inject_hot_ref.npz, inject_hotrun_ref.npz  the same with hot electrons (kT / m_e c^2 = 0.5,
+                   createParticle's Juttner branch), with the run's temperature 0 (tabulated sampler)
+                   and with kT_run / m_e c^2 = 3 (rejection sampler)
   smooth_ref.npz, smooth_wall_ref.npz  the mini case and the escape case with 8 smoothing
                    passes of s = 0.5 (the shipped input.dat's setting) every step
   wall_long_ref.npz the shock set-up (gamma 10 e-/p+ flow onto the wall, mass ratio 64,
@@ -89,6 +92,7 @@ def write_input(path, w, st, steps, general=False):
         f.write(struct.pack("<id", w.smoothing_count, w.smoothing_parameter))
         f.write(struct.pack("<ii", int(w.divergence_cleanup), w.filter_cut_wave_number))
         f.write(struct.pack("<i", 1 if general else 0))
+        f.write(struct.pack("<d", w.run_kT))
 
 
 def read_output(path):
@@ -193,6 +197,24 @@ def inject_case():
     leave there."""
     w, st = wall_out_case()
     return dataclasses.replace(w, inject=True, name="inject"), st
+
+
+def inject_hot_cases():
+    """The injection case with hot electrons (kT / m_e c^2 = 0.5: createParticle's isotropic Juttner
+    branch; protons at the same kT stay Maxwellian): with the run's temperature 0 (the
+    tabulated sampler) and with kT_run / m_e c^2 = 3 (the rejection sampler for electrons)."""
+    w, st = wall_out_case()
+    w = dataclasses.replace(w, inject=True, name="inject-hot")
+    w = dataclasses.replace(w, species=wl.electron_proton(w.species[0].ppc, 0.5, 100.0))
+    for sp in w.species:
+        sp.drift_gamma, sp.drift_axis, sp.drift_sign = 2.0, 0, -1
+    return ((w, st, "inject_hot_ref.npz"), (dataclasses.replace(w, run_kT=3.0, name="inject-hotrun"), st, "inject_hotrun_ref.npz"))
+
+
+def make_inject_hot():
+    for w, st, fname in inject_hot_cases():
+        out = run(w, st, w.steps)
+        np.savez_compressed(os.path.join(HERE, fname), **inputs(st), **out)
 
 
 def make_inject():
@@ -383,7 +405,7 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref", "-j8"], check=True)
     for name in which:
-        {"mini": make_mini, "config1": make_config1, "general": make_general, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject,
+        {"mini": make_mini, "config1": make_config1, "general": make_general, "inject_hot": make_inject_hot, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject,
          "wall_long": make_wall_long, "smooth": make_smooth, "explicit": make_explicit,
          "backup": make_backup, "cleanup": make_cleanup}[name]()
         print("wrote", name)

@@ -14,6 +14,10 @@ per species, dt = 0.5, 5 steps. Every step must
   particle as the reference does (1e-12).
 The FP32 form injects the same particles, stored as cell + FP32 offset and FP32 u: positions
 within 1e-6 cells, momenta within 1e-6.
+The same holds with hot injected electrons (tests/golden/inject_hot_ref.npz: kT / m_e c^2 = 0.5,
+momenta from the tabulated Maxwell-Juttner inverse; inject_hotrun_ref.npz: the run's
+kT / m_e c^2 = 3, the rejection sampler): per-species counts as the reference's, the oracle's
+particles, and the first-step push of the others.
 """
 import os
 
@@ -52,9 +56,20 @@ def test_philox_matches_oracle():
     sim.close()
 
 
-def run_case(precision):
-    d = np.load(os.path.join(HERE, "golden", "inject_ref.npz"))
-    w = inject_workload()
+def hot_workload(run_kT=0.0):
+    """Hot injected electrons (kT / m_e c^2 = 0.5: the tabulated Maxwell-Juttner sampler; with
+    the run's kT / m_e c^2 = 3 the rejection sampler), as inject_hot_ref.npz and
+    inject_hotrun_ref.npz were made."""
+    import dataclasses
+    sp = wl.electron_proton(4, 0.5, 100.0)
+    for s in sp:
+        s.drift_gamma, s.drift_axis, s.drift_sign = 2.0, 0, -1
+    return dataclasses.replace(inject_workload(), name="inject-hot", run_kT=run_kT, species=sp)
+
+
+def run_case(precision, fixture="inject_ref.npz", w=None):
+    d = np.load(os.path.join(HERE, "golden", fixture))
+    w = w or inject_workload()
     st = GoldenState(d)
     print("case", w.name, precision, "particles", len(st.species), flush=True)
     sim = Simulation(w, precision=precision)
@@ -127,3 +142,9 @@ def test_injection_fp64_matches_oracle_and_reference():
 
 def test_injection_fp32_matches_oracle_and_reference():
     run_case("fp32")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("fixture,run_kT", [("inject_hot_ref.npz", 0.0), ("inject_hotrun_ref.npz", 3.0)])
+def test_hot_injection_matches_oracle_and_reference(precision, fixture, run_kT):
+    run_case(precision, fixture, hot_workload(run_kT))
