@@ -81,10 +81,10 @@ struct pic_ctx {
 	int *start = nullptr, *counts = nullptr;
 	long long nkeys = 0;
 	// rebin of the FP32 particles after a push: mover flags, their prefix count, the movers
-	// (key, old index) and per-key stay / arrival counts
+	// (key, old index) and per-key arrival counts
 	uint64_t* mask = nullptr;  // FP32 push: mover bit per particle
 	int *wcount = nullptr, *wprefix = nullptr;  // mask word populations and their exclusive scan
-	int *start_new = nullptr, *stay = nullptr, *arrive = nullptr, *first = nullptr;
+	int *start_new = nullptr, *arrive = nullptr, *first = nullptr;
 	uint32_t *mk = nullptr, *mi = nullptr, *mk_s = nullptr, *mi_s = nullptr;
 	long long mover_capacity = 0;
 	int end_bit = 0;
@@ -450,16 +450,14 @@ void rebin_particles(pic_ctx* c, int removed, bool resized) {
 	}
 	{
 		Timed t(c, "rebin_segments", (double)(c->nkeys + 1) * 4 * 6);
-		launch_rebin_counts(c->start, c->mask, c->wprefix, c->arrive, c->nkeys, c->stay, c->counts, s);
+		launch_rebin_counts(c->start, c->mask, c->wprefix, c->arrive, c->nkeys, c->counts, s);
 		size_t bytes = c->sort_temp_bytes;
 		HIP_CHECK(rocprim::exclusive_scan(c->sort_temp, bytes, c->counts, c->start_new, 0, (size_t)(c->nkeys + 1),
 		                                  rocprim::plus<int>(), s));
 	}
 	{
-		Timed t(c, "rebin_perm", (double)n * (4 + 4) + (double)c->nkeys * 12);
-		// the arrival counts are folded into start_new by now: their array takes the per-key shifts
-		launch_rebin_perm(n, c->mask, c->wprefix, c->keys, c->start, c->start_new, c->nkeys, c->arrive, c->mk_s, c->mi_s, M, c->stay,
-		                  c->first, c->perm_sorted, s);
+		Timed t(c, "rebin_perm", (double)n * 4 + (double)c->nkeys * 12);
+		launch_rebin_place(c->start, c->start_new, c->mask, c->arrive, c->first, c->mi_s, c->nkeys, c->perm_sorted, s);
 	}
 	const long long n_new = n - removed;
 	set_count(c, n_new);  // the permutation fills the first n_new places
@@ -985,7 +983,6 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 			c->wcount = dalloc<int>(words);
 			c->wprefix = dalloc<int>(words);
 			c->start_new = dalloc<int>(c->nkeys + 1);
-			c->stay = dalloc<int>(c->nkeys + 1);
 			c->arrive = dalloc<int>(c->nkeys + 1);
 			c->first = dalloc<int>(c->nkeys + 1);
 			c->mover_capacity = c->capacity / 4 + 1;
@@ -1129,7 +1126,7 @@ void pic_destroy(pic_ctx* c) {
 	for (double* v : c->V) f(v);
 	for (float* v : c->V32) f(v);
 	for (void* p : {(void*)c->V32dev, (void*)c->mu32, (void*)c->mu32h, (void*)c->muh, (void*)c->w32, (void*)c->w64, (void*)c->tmp32, (void*)c->Z32dev,
-	                (void*)c->log_r, (void*)c->log_e, (void*)c->mask, (void*)c->wcount, (void*)c->wprefix, (void*)c->start_new, (void*)c->stay,
+	                (void*)c->log_r, (void*)c->log_e, (void*)c->mask, (void*)c->wcount, (void*)c->wprefix, (void*)c->start_new,
 	                (void*)c->arrive, (void*)c->first, (void*)c->mk, (void*)c->mi, (void*)c->mk_s, (void*)c->mi_s})
 		f(p);
 	for (float* v : c->Z32) f(v);

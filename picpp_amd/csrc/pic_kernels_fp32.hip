@@ -927,15 +927,23 @@ __global__ void k_mark_movers(uint64_t* __restrict__ mask, long long lo, long lo
 }
 
 // movers in old-index order: mk[F(i)] = key, mi[F(i)] = i; removed particles (keys >= nkeys)
-// take the key nkeys, which has no segment
+// take the key nkeys, which has no segment. One thread per mask word: a thread per particle
+// would launch a million workgroups for a few percent of movers.
 __global__ void k_gather_movers(const uint64_t* __restrict__ mask, const int* __restrict__ prefix, const uint32_t* __restrict__ keys,
-                                long long n, uint32_t nkeys, uint32_t* __restrict__ mk, uint32_t* __restrict__ mi) {
-	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n || !is_mover(mask, i)) return;
-	int m = movers_before(mask, prefix, i);
-	const uint32_t k = keys[i];
-	mk[m] = k < nkeys ? k : nkeys;
-	mi[m] = (uint32_t)i;
+                                long long n, long long nwords, uint32_t nkeys, uint32_t* __restrict__ mk, uint32_t* __restrict__ mi) {
+	const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (w >= nwords) return;
+	uint64_t bits = mask[w];
+	int m = prefix[w];
+	while (bits) {
+		const long long i = w * 64 + (__ffsll((unsigned long long)bits) - 1);  // lowest set bit first
+		bits &= bits - 1;
+		if (i >= n) break;
+		const uint32_t k = keys[i];
+		mk[m] = k < nkeys ? k : nkeys;
+		mi[m] = (uint32_t)i;
+		++m;
+	}
 }
 
 // first[k] = index of the first sorted mover with key k
@@ -945,9 +953,9 @@ __global__ void k_first_of_run(const uint32_t* __restrict__ mk, long long M, int
 	if (m == 0 || mk[m - 1] != mk[m]) first[mk[m]] = (int)m;
 }
 
-// stay[k] = old segment length minus its leavers; counts[k] = stay[k] + arrivals[k]
+// counts[k] = old segment length minus its leavers, plus its arrivals; counts[nkeys] = 0
 __global__ void k_rebin_counts(const int* __restrict__ start_old, const uint64_t* __restrict__ mask, const int* __restrict__ prefix,
-                               const int* __restrict__ arrive, long long nkeys, int* __restrict__ stay, int* __restrict__ counts) {
+                               const int* __restrict__ arrive, long long nkeys, int* __restrict__ counts) {
 	long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k > nkeys) return;
 	if (k == nkeys) {
@@ -955,38 +963,57 @@ __global__ void k_rebin_counts(const int* __restrict__ start_old, const uint64_t
 		return;
 	}
 	int lo = start_old[k], hi = start_old[k + 1];
-	int st = (hi - lo) - (movers_before(mask, prefix, hi) - movers_before(mask, prefix, lo));
-	stay[k] = st;
-	counts[k] = st + arrive[k];
-}
-
-// shift[k] = start_new[k] - start_old[k] + F(start_old[k]): a stayer i of segment k lands at
-// i - F(i) + shift[k] (its old offset in the segment minus the leavers before it)
-__global__ void k_rebin_shift(const int* __restrict__ start_old, const int* __restrict__ start_new, const uint64_t* __restrict__ mask,
-                              const int* __restrict__ prefix, long long nkeys, int* __restrict__ shift) {
-	long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= nkeys) return;
-	int so = start_old[k];
-	shift[k] = start_new[k] - so + movers_before(mask, prefix, so);
+	counts[k] = (hi - lo) - (movers_before(mask, prefix, hi) - movers_before(mask, prefix, lo)) + arrive[k];
 }
 
 // The destinations are written as a source index per new position (perm[to] = i), and the
 // particles then move with one coalesced gather (k_permute32): scattering the 32-byte
-// particles directly writes partial cache lines at every segment boundary.
-__global__ void k_rebin_stayers(long long n, const uint64_t* __restrict__ mask, const int* __restrict__ prefix,
-                                const uint32_t* __restrict__ keys, const int* __restrict__ shift, uint32_t* __restrict__ perm) {
-	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n || is_mover(mask, i)) return;
-	perm[i - movers_before(mask, prefix, i) + shift[keys[i]]] = (uint32_t)i;
-}
-
-__global__ void k_rebin_movers(const uint32_t* __restrict__ mk, const uint32_t* __restrict__ mi, long long M, const int* __restrict__ start_new,
-                               const int* __restrict__ stay, const int* __restrict__ first, uint32_t nkeys, uint32_t* __restrict__ perm) {
-	long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (m >= M) return;
-	uint32_t k = mk[m];
-	if (k >= nkeys) return;  // removed: no place in the new order
-	perm[(long long)start_new[k] + stay[k] + (m - first[k])] = mi[m];
+// particles directly writes partial cache lines at every segment boundary. One thread per
+// segment k writes its places from start_new[k] on: the stayers (old indexes start_old[k] ..
+// start_old[k+1]-1 without a mover bit) in order, then the arrivals mi[first[k] ..
+// first[k]+arrive[k]-1] (the movers sorted by new key, stable in old index). A workgroup's
+// places form one range of perm; when it fits kPlaceStage entries it is assembled in LDS and
+// written out coalesced (the per-thread writes are strided by the segment lengths), otherwise
+// the threads write perm directly.
+constexpr int kPlaceStage = 4096;
+__global__ void __launch_bounds__(256) k_rebin_place(const int* __restrict__ start_old, const int* __restrict__ start_new,
+                                                     const uint64_t* __restrict__ mask, const int* __restrict__ arrive,
+                                                     const int* __restrict__ first, const uint32_t* __restrict__ mi, long long nkeys,
+                                                     uint32_t* __restrict__ perm) {
+	__shared__ uint32_t stage[kPlaceStage];
+	const long long k0 = (long long)blockIdx.x * blockDim.x;
+	const long long k = k0 + threadIdx.x;
+	const long long kend = k0 + blockDim.x < nkeys ? k0 + blockDim.x : nkeys;
+	const long long base = start_new[k0], top = start_new[kend];  // start_new has nkeys + 1 entries
+	const bool staged = top - base <= kPlaceStage;  // the same for the whole workgroup
+	auto put = [&](long long d, uint32_t v) {
+		if (staged)
+			stage[d - base] = v;
+		else
+			perm[d] = v;
+	};
+	if (k < nkeys) {
+		const int lo = start_old[k], hi = start_old[k + 1];
+		long long dst = start_new[k];
+		for (long long w = lo >> 6; w * 64 < hi; ++w) {
+			// the word's bits inside [lo, hi) that are not movers
+			const int a = lo > w * 64 ? (int)(lo - w * 64) : 0, b = hi < (w + 1) * 64 ? (int)(hi - w * 64) : 64;
+			uint64_t stay = ~mask[w] & ((b == 64 ? ~0ull : ((1ull << b) - 1)) & ~((1ull << a) - 1));
+			while (stay) {
+				put(dst++, (uint32_t)(w * 64 + (__ffsll((unsigned long long)stay) - 1)));
+				stay &= stay - 1;
+			}
+		}
+		const int na = arrive[k];
+		if (na > 0) {
+			const int f = first[k];
+			for (int r = 0; r < na; ++r) put(dst++, mi[f + r]);
+		}
+	}
+	if (staged) {
+		__syncthreads();
+		for (long long d = base + threadIdx.x; d < top; d += blockDim.x) perm[d] = stage[d - base];
+	}
 }
 
 __global__ void k_keys32(Grid g, SpeciesTable sp, ParticleArrays32 P, const int* __restrict__ species_start, uint32_t* __restrict__ keys) {
@@ -1263,24 +1290,22 @@ void launch_mark_movers(uint64_t* mask, long long lo, long long hi, hipStream_t 
 void launch_gather_movers(const uint64_t* mask, const int* prefix, const uint32_t* keys, long long n, long long nkeys, uint32_t* mk,
                           uint32_t* mi, hipStream_t s) {
 	if (n == 0) return;
-	hipLaunchKernelGGL(k_gather_movers, dim3(blocks32(n, 256)), dim3(256), 0, s, mask, prefix, keys, n, (uint32_t)nkeys, mk, mi);
+	const long long nwords = (n + 63) / 64;
+	hipLaunchKernelGGL(k_gather_movers, dim3(blocks32(nwords, 256)), dim3(256), 0, s, mask, prefix, keys, n, nwords, (uint32_t)nkeys,
+	                   mk, mi);
 }
 void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_t s) {
 	if (M == 0) return;
 	hipLaunchKernelGGL(k_first_of_run, dim3(blocks32(M, 256)), dim3(256), 0, s, mk, M, first);
 }
-void launch_rebin_counts(const int* start_old, const uint64_t* mask, const int* prefix, const int* arrive, long long nkeys, int* stay,
-                         int* counts, hipStream_t s) {
-	hipLaunchKernelGGL(k_rebin_counts, dim3(blocks32(nkeys + 1, 256)), dim3(256), 0, s, start_old, mask, prefix, arrive, nkeys, stay,
-	                   counts);
+void launch_rebin_counts(const int* start_old, const uint64_t* mask, const int* prefix, const int* arrive, long long nkeys, int* counts,
+                         hipStream_t s) {
+	hipLaunchKernelGGL(k_rebin_counts, dim3(blocks32(nkeys + 1, 256)), dim3(256), 0, s, start_old, mask, prefix, arrive, nkeys, counts);
 }
-void launch_rebin_perm(long long n, const uint64_t* mask, const int* prefix, const uint32_t* keys, const int* start_old,
-                       const int* start_new, long long nkeys, int* shift, const uint32_t* mk, const uint32_t* mi, long long M,
-                       const int* stay, const int* first, uint32_t* perm, hipStream_t s) {
-	hipLaunchKernelGGL(k_rebin_shift, dim3(blocks32(nkeys, 256)), dim3(256), 0, s, start_old, start_new, mask, prefix, nkeys, shift);
-	if (n > 0) hipLaunchKernelGGL(k_rebin_stayers, dim3(blocks32(n, 256)), dim3(256), 0, s, n, mask, prefix, keys, shift, perm);
-	if (M > 0)
-		hipLaunchKernelGGL(k_rebin_movers, dim3(blocks32(M, 256)), dim3(256), 0, s, mk, mi, M, start_new, stay, first, (uint32_t)nkeys, perm);
+void launch_rebin_place(const int* start_old, const int* start_new, const uint64_t* mask, const int* arrive, const int* first,
+                        const uint32_t* mi, long long nkeys, uint32_t* perm, hipStream_t s) {
+	hipLaunchKernelGGL(k_rebin_place, dim3(blocks32(nkeys, 256)), dim3(256), 0, s, start_old, start_new, mask, arrive, first, mi, nkeys,
+	                   perm);
 }
 void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
                    hipStream_t s) {

@@ -49,12 +49,11 @@ void launch_mark_movers(uint64_t* mask, long long lo, long long hi, hipStream_t 
 void launch_gather_movers(const uint64_t* mask, const int* prefix, const uint32_t* keys, long long n, long long nkeys, uint32_t* mk,
                           uint32_t* mi, hipStream_t s);
 void launch_first_of_run(const uint32_t* mk, long long M, int* first, hipStream_t s);
-void launch_rebin_counts(const int* start_old, const uint64_t* mask, const int* prefix, const int* arrive, long long nkeys, int* stay,
-                         int* counts, hipStream_t s);
-// perm[new position] = old index; shift: scratch of nkeys ints (overwritten)
-void launch_rebin_perm(long long n, const uint64_t* mask, const int* prefix, const uint32_t* keys, const int* start_old,
-                       const int* start_new, long long nkeys, int* shift, const uint32_t* mk, const uint32_t* mi, long long M,
-                       const int* stay, const int* first, uint32_t* perm, hipStream_t s);
+void launch_rebin_counts(const int* start_old, const uint64_t* mask, const int* prefix, const int* arrive, long long nkeys, int* counts,
+                         hipStream_t s);
+// perm[new position] = old index: per segment its stayers, then its arrivals mi[first[k] ..]
+void launch_rebin_place(const int* start_old, const int* start_new, const uint64_t* mask, const int* arrive, const int* first,
+                        const uint32_t* mi, long long nkeys, uint32_t* perm, hipStream_t s);
 void launch_keys32(const Grid& g, const SpeciesTable& sp, const ParticleArrays32& P, const int* species_start, uint32_t* keys,
                    hipStream_t s);
 void launch_permute32(const ParticleArrays32& src, const ParticleArrays32& dst, const uint32_t* perm, hipStream_t s);
