@@ -446,16 +446,40 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DENS ?
 			const float beta = 0.5f * q * dt / m;
 			const float wv = w * inv_v;
 			const float pc = q * wv;
-			for (int v = lane; v < d.p[8]; v += G) {
-				int a = 0, off = d.off[0];
+			// The fused pass (DENS, 2 waves per SIMD) is software pipelined: the next particle's
+			// loads are issued before this one's work (a lane past the end reloads a valid particle
+			// and drops it). The plain pass keeps 3 waves only without the extra registers.
+			const int nv = d.p[8];
+			auto fetch = [&](int v, float f[3], float u[3], int& a) {
+				v = v < nv ? v : nv - 1;
+				int off = d.off[0];
+				a = 0;
 #pragma unroll
 				for (int qq = 1; qq < 8; ++qq)
 					if (v >= d.p[qq]) {
 						a = qq;
 						off = d.off[qq];
 					}
-				float f[3], u[3];
 				load32(P, v + off, f, u);
+			};
+			float fq[3], uq[3];
+			int aq = 0;
+			if constexpr (DENS) fetch(lane, fq, uq, aq);
+			for (int v = lane; v < nv; v += G) {
+				float f[3], u[3];
+				int a;
+				if constexpr (DENS) {
+					a = aq;
+#pragma unroll
+					for (int c3 = 0; c3 < 3; ++c3) {
+						f[c3] = fq[c3];
+						u[c3] = uq[c3];
+					}
+					fetch(v + G, fq, uq, aq);
+					__builtin_amdgcn_sched_barrier(0);  // issue those loads here
+				} else {
+					fetch(v, f, u, a);
+				}
 				const int ax = (a >> 2) & 1, ay = (a >> 1) & 1, az = a & 1;
 				const int bit[3] = {1 - ax, 1 - ay, 1 - az};
 				float nw[3][2], cw[3][2];
@@ -507,6 +531,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DENS ?
 								for (int r = 0; r < 3; ++r) bulk[tt][r] += c * u[r] * m;
 							}
 						}
+				// the next particle's values are needed from here on only: the loads stay ahead of
+				// this particle's work instead of being sunk into the next iteration
+				if constexpr (DENS) asm volatile("" : "+v"(fq[0]), "+v"(fq[1]), "+v"(fq[2]), "+v"(uq[0]), "+v"(uq[1]), "+v"(uq[2]));
 			}
 		}
 		if (DENS) {  // planes per species [conc, energy, bulk x3] x 8 corners, as k_density32

@@ -57,20 +57,26 @@ struct DualSegments {
 	int off[8];
 };
 
+// The 16 loads are issued together: a source cell outside the grid reads key 0 and takes an
+// empty segment (a load under a per-segment branch would wait for each in turn).
 __device__ inline void dual_segments(const Grid& g, const int* __restrict__ start, int bi, int bj, int bk, int s, DualSegments& d) {
-	d.p[0] = 0;
+	int lo[8], hi[8];
+	bool in[8];
 #pragma unroll
 	for (int a = 0; a < 8; ++a) {
 		const int ax = (a >> 2) & 1, ay = (a >> 1) & 1, az = a & 1;
 		const int ci = bi + ax, cj = bj + ay, ck = bk + az;
-		int lo = 0, len = 0;
-		if (ci < g.N[0] && cj < g.N[1] && ck < g.N[2]) {
-			long long key = (((long long)s * g.ncells + cell_id(g, ci, cj, ck)) << 3) | (7 - a);
-			lo = start[key];
-			len = start[key + 1] - lo;
-		}
+		in[a] = ci < g.N[0] && cj < g.N[1] && ck < g.N[2];
+		const long long key = in[a] ? ((((long long)s * g.ncells + cell_id(g, ci, cj, ck)) << 3) | (7 - a)) : 0;
+		lo[a] = start[key];
+		hi[a] = start[key + 1];
+	}
+	d.p[0] = 0;
+#pragma unroll
+	for (int a = 0; a < 8; ++a) {
+		const int l = in[a] ? lo[a] : 0, len = in[a] ? hi[a] - lo[a] : 0;
 		d.p[a + 1] = d.p[a] + len;
-		d.off[a] = lo - d.p[a];
+		d.off[a] = l - d.p[a];
 	}
 }
 
