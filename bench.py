@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+from picpp_amd._lib import PicError  # noqa: E402
 from picpp_amd.simulation import open_synthetic_box  # noqa: E402
 
 
@@ -72,6 +73,11 @@ def parse():
     ap.add_argument("--explicit-steps", type=int, default=5,
                     help="with the default benchmark, also time this many explicit (BUNEMAN / Tristan) FP64 steps on the "
                          "same box and report them as explicit_step (0: skip)")
+    ap.add_argument("--precond-dt-sweep", default=None, metavar="DT,DT,...",
+                    help="only compare plain and learned-preconditioned mixed GMRES at each time step (in 1/omega_pe): "
+                         "the network is trained at that dt on the train_cells^3 box, then both solve held-out systems of "
+                         "the benchmark box (--cells, --ppc) to the 1e-8 true residual")
+    ap.add_argument("--sweep-held-out", type=int, default=3, help="held-out systems per time step of --precond-dt-sweep")
     ap.add_argument("--shock-cells", type=int, default=1024, help="cells along x (64 x 64 across)")
     ap.add_argument("--shock-ppc", type=int, default=16, help="particles per cell per species at injection")
     ap.add_argument("--pressure", default=None, choices=["reference", "cell"],
@@ -182,6 +188,51 @@ def compare_solvers(sim, torch, learned, reps=3):
     if learned:
         sim.precond_use(False)
     return out
+
+
+def precond_dt_sweep(args, torch, dev):
+    """Plain against learned-preconditioned mixed GMRES as the implicit system stiffens: per time
+    step, the network is trained at that dt (train_network on the train_cells^3 box), the
+    benchmark box is stepped twice at that dt, and each of sweep_held_out systems (one per
+    further step, none seen in training) is solved by both, to the 1e-8 true residual."""
+    import dataclasses
+
+    from picpp_amd import workloads as wl
+
+    rows = []
+    for dt in [float(v) for v in args.precond_dt_sweep.split(",")]:
+        tw = dataclasses.replace(wl.config2(n=(args.train_cells,) * 3, ppc=16, steps=0), dt=dt)
+        params, mu_scale, training = train_network(args, torch, dev, tw)
+        w = dataclasses.replace(wl.config2(n=(args.cells,) * 3, ppc=args.ppc, steps=0), dt=dt)
+        sim = open_synthetic_box(w, torch, dev, precision="fp32", stop_rule="true_residual", solver="mixed")
+        sim.precond_create()
+        sim.set_precond_params(params, mu_scale)
+        sim.precond_use(False)
+        held = []
+        try:
+            for _ in range(2):
+                sim.step()
+            for _ in range(args.sweep_held_out):
+                held.append(compare_solvers(sim, torch, True))
+                sim.step()
+        except PicError as e:  # the step's own checks refuse this time step: reported, the sweep goes on
+            print("dt %.3g: %s" % (dt, e), file=sys.stderr, flush=True)
+            rows.append({"dt": dt, "training": training, "failed": str(e)})
+            continue
+        finally:
+            sim.close()
+            del sim
+            torch.cuda.empty_cache()
+        row = {"dt": dt, "training": training}
+        for name in ("plain", "learned"):
+            row[name] = {k: float(np.mean([h[name][k] for h in held])) for k in ("ms", "iterations", "true_residual")}
+            row[name]["fallbacks"] = int(sum(h[name]["fallbacks"] for h in held))
+        row["learned_over_plain_time"] = row["learned"]["ms"] / row["plain"]["ms"]
+        print("dt %.3g: plain %.1f iterations %.3f ms, learned %.1f iterations %.3f ms" % (
+            dt, row["plain"]["iterations"], row["plain"]["ms"], row["learned"]["iterations"], row["learned"]["ms"]),
+              file=sys.stderr, flush=True)
+        rows.append(row)
+    return {"cells": [args.cells] * 3, "ppc_per_species": args.ppc, "held_out_per_dt": args.sweep_held_out, "rows": rows}
 
 
 def time_steps(sim, args):
@@ -583,6 +634,9 @@ def main():
         return
     if args.shock > 0:
         print(json.dumps({"shock": shock_run(args, torch, dev)}))
+        return
+    if args.precond_dt_sweep:
+        print(json.dumps({"precond_dt_sweep": precond_dt_sweep(args, torch, dev)}))
         return
     if args.explicit > 0:
         print(json.dumps({"explicit": explicit_run(args, torch, dev)}))
