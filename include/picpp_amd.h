@@ -84,9 +84,10 @@ enum {
 enum {
 	PIC_SCHEME_IMPLICIT = 0, /* the theta-implicit step (the reference's IMPLICIT solver) */
 	/* The reference's BUNEMAN solver: explicit Yee fields, the Tristan mover and the zig-zag
-	   current (PIC++/simulation.cpp:163-197). FP64; periodic, or PIC_BOUNDARY_WALL with nx >= 20
-	   (the mover's wall at xgrid[20]) and injection; smoothing_count passes smooth the current;
-	   fields are the Yee arrays (PIC_ARRAY_YEE_*), stepped by pic_explicit_step. */
+	   current (PIC++/simulation.cpp:163-197). FP64; 3-D, or 2-D XY with nz = 1; periodic, or
+	   PIC_BOUNDARY_WALL with nx >= 20 (the mover's wall at xgrid[20]) and injection;
+	   smoothing_count passes smooth the current; fields are the Yee arrays (PIC_ARRAY_YEE_*),
+	   stepped by pic_explicit_step. */
 	PIC_SCHEME_EXPLICIT = 1
 };
 
@@ -97,7 +98,10 @@ enum {
 };
 
 typedef struct {
-	int nx, ny, nz;       /* physical cells per axis, each >= 5 */
+	/* physical cells per axis, each >= 5; nz = 1 with PIC_SCHEME_EXPLICIT is the reference's 2-D XY
+	   grid (znumberGeneral = 1, PIC++/initialization.cpp:170-174): no ghost layers along z, the Yee
+	   arrays have 1 or 2 planes along z as the reference allocates them, particles keep z unwrapped */
+	int nx, ny, nz;
 	double dx, dy, dz;    /* cell sizes */
 	double dt;            /* time step (deltaT) */
 	int nspecies;         /* <= 8 (PIC++ supports 8 types, initialization.cpp:596-606) */

@@ -826,9 +826,12 @@ const char* pic_last_error(const pic_ctx* c) { return c ? c->error.c_str() : noa
 int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 	if (!cfg || !out) return PIC_ERR_ARGUMENT;
 	*out = nullptr;
-	if (cfg->nx < 5 || cfg->ny < 5 || cfg->nz < 5 || cfg->nspecies < 1 || cfg->nspecies > kMaxSpecies || cfg->dt <= 0 ||
+	// nz = 1: the reference's 2-D XY grid (no ghost layers along z), built for the explicit scheme
+	const bool cz = cfg->nz == 1;
+	if (cfg->nx < 5 || cfg->ny < 5 || (cfg->nz < 5 && !cz) || cfg->nspecies < 1 || cfg->nspecies > kMaxSpecies || cfg->dt <= 0 ||
 	    cfg->dx <= 0 || cfg->dy <= 0 || cfg->dz <= 0 || cfg->capacity < 0)
 		return PIC_ERR_ARGUMENT;
+	if (cz && cfg->scheme != PIC_SCHEME_EXPLICIT) return PIC_ERR_ARGUMENT;
 	if (cfg->precision != PIC_PRECISION_FP64 && cfg->precision != PIC_PRECISION_FP32) return PIC_ERR_ARGUMENT;
 	if (cfg->pressure != PIC_PRESSURE_REFERENCE && cfg->pressure != PIC_PRESSURE_CELL) return PIC_ERR_ARGUMENT;
 	if (cfg->smoothing_count < 0 || (cfg->smoothing_count > 0 && !(cfg->smoothing_parameter >= 0.0 && cfg->smoothing_parameter <= 1.0)))
@@ -860,12 +863,13 @@ int pic_create(const pic_config* cfg, void* stream, pic_ctx** out) {
 		Grid& g = c->g;
 		int n[3] = {cfg->nx, cfg->ny, cfg->nz};
 		double d[3] = {cfg->dx, cfg->dy, cfg->dz};
+		g.cz = cz ? 1 : 0;
 		for (int a = 0; a < 3; ++a) {
 			g.n[a] = n[a];
-			g.N[a] = n[a] + 4;
+			g.N[a] = n[a] + 2 * owned_lo(g, a);  // two ghost layers each side, none along a collapsed z
 			g.d[a] = d[a];
 			g.L[a] = n[a] * d[a];
-			g.x0[a] = g.L[a] + (-2) * d[a];
+			g.x0[a] = g.L[a] - owned_lo(g, a) * d[a];  // node 0; the first owned node at L
 		}
 		g.volume = d[0] * d[1] * d[2];
 		g.nnodes = (long long)(g.N[0] + 1) * (g.N[1] + 1) * (g.N[2] + 1);
@@ -2096,7 +2100,7 @@ int pic_explicit_step(pic_ctx* c) {
 			// smoothBunemanEfieldGeneral passes (PIC++/simulation.cpp:185-188); the fixed-point
 			// accumulators, done with for this step, hold the scratch copies
 			Timed t(c, "smoothing", (double)c->cfg.smoothing_count * g.ncells * 3 * 8 * 3);
-			const Smooth27 sm = smoothing_matrix(c->cfg.smoothing_parameter);
+			const Smooth27 sm = smoothing_matrix(c->cfg.smoothing_parameter, g.cz != 0);
 			double* const tmp[3] = {reinterpret_cast<double*>(c->yee.Jacc[0]), reinterpret_cast<double*>(c->yee.Jacc[1]),
 			                        reinterpret_cast<double*>(c->yee.Jacc[2])};
 			for (int n = 0; n < c->cfg.smoothing_count; ++n) launch_smooth_yee_current(g, c->yee, sm, tmp, s);

@@ -28,7 +28,14 @@ struct Grid {
 	// (nu0 = n0 + 2: nodes 2 .. n0+3, the wall node 2 and the boundary nodes n0+2, n0+3)
 	int wall, nu0;
 	double e_right[3], b_right[3];
+	// 1: one cell along z and no ghost layers there (the reference's 2-D XY grids, znumberAdded =
+	// 1, PIC++/initialization.cpp:170-174): N[2] = 1, every z stencil collapses onto the plane
+	// k = 0 and particles keep their z unwrapped. Explicit scheme only.
+	int cz;
 };
+
+// First owned index along axis a: 2 behind the two ghost layers, 0 along a collapsed z.
+__host__ __device__ inline int owned_lo(const Grid& g, int a) { return (a == 2 && g.cz) ? 0 : 2; }
 
 struct SpeciesTable {
 	int count;
@@ -124,6 +131,10 @@ __device__ inline uint32_t particle_key(const Grid& g, int sp, const AxisMap* m)
 __device__ inline uint32_t position_key(const Grid& g, int sp, const double x[3]) {
 	AxisMap m[3];
 	for (int a = 0; a < 3; ++a) axis_bases(x[a], g.x0[a], g.d[a], m[a].nbase, m[a].cbase);
+	if (g.cz) {  // one cell along z: k = 0, octant bit 0, whatever the unwrapped z
+		m[2].nbase = 0;
+		m[2].cbase = -1;
+	}
 	return particle_key(g, sp, m);
 }
 

@@ -34,26 +34,27 @@ inline unsigned nb(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 __device__ inline long long yi(const int d[3], int i, int j, int k) { return ((long long)i * d[1] + j) * d[2] + k; }
 
 // tristanEvaluateBhalfStep over the owned box i, j, k in [2, N-2]: Bx on i <= N-2, j, k < N-2;
-// By on j <= N-2; Bz on k <= N-2 (fields.cpp:26-74).
+// By on j <= N-2; Bz on k <= N-2 (fields.cpp:26-74). Along a collapsed z only k = 0, without
+// the z derivatives (fields.cpp:36-39, :3280-3282).
 __global__ void k_b_half(Grid g, YeeGrid y, double cdtd2) {
-	const int ni = g.N[0] - 3, nj = g.N[1] - 3, nk = g.N[2] - 3;
+	const int ni = g.N[0] - 3, nj = g.N[1] - 3, nk = g.cz ? 1 : g.N[2] - 3;
 	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= (long long)ni * nj * nk) return;
-	const int k = 2 + (int)(t % nk), j = 2 + (int)((t / nk) % nj), i = 2 + (int)(t / ((long long)nk * nj));
-	const int hx = g.N[0] - 2, hy = g.N[1] - 2, hz = g.N[2] - 2;
+	const int k = owned_lo(g, 2) + (int)(t % nk), j = 2 + (int)((t / nk) % nj), i = 2 + (int)(t / ((long long)nk * nj));
+	const int hx = g.N[0] - 2, hy = g.N[1] - 2, hz = g.cz ? 1 : g.N[2] - 2;
 	const double* Ex = y.E[0];
 	const double* Ey = y.E[1];
 	const double* Ez = y.E[2];
 	if (j < hy && k < hz) {  // evaluateBunemanRotEx (fields.cpp:3273-3284)
 		double r = 0;
 		r += (Ez[yi(y.dE[2], i, j + 1, k)] - Ez[yi(y.dE[2], i, j, k)]) / g.d[1];
-		r -= (Ey[yi(y.dE[1], i, j, k + 1)] - Ey[yi(y.dE[1], i, j, k)]) / g.d[2];
+		if (!g.cz) r -= (Ey[yi(y.dE[1], i, j, k + 1)] - Ey[yi(y.dE[1], i, j, k)]) / g.d[2];
 		double& b = y.B[0][yi(y.dB[0], i, j, k)];
 		b = b - r * cdtd2;
 	}
 	if (i < hx && k < hz) {  // evaluateBunemanRotEy (:3286-3294)
 		double r = -((Ez[yi(y.dE[2], i + 1, j, k)] - Ez[yi(y.dE[2], i, j, k)]) / g.d[0]);
-		r += (Ex[yi(y.dE[0], i, j, k + 1)] - Ex[yi(y.dE[0], i, j, k)]) / g.d[2];
+		if (!g.cz) r += (Ex[yi(y.dE[0], i, j, k + 1)] - Ex[yi(y.dE[0], i, j, k)]) / g.d[2];
 		double& b = y.B[1][yi(y.dB[1], i, j, k)];
 		b = b - r * cdtd2;
 	}
@@ -66,26 +67,27 @@ __global__ void k_b_half(Grid g, YeeGrid y, double cdtd2) {
 }
 
 // tristanEvaluateE over the owned box: Ex on i < N-2, Ey on j < N-2, Ez on k < N-2
-// (fields.cpp:81-150, periodic x).
+// (fields.cpp:81-150, periodic x). Along a collapsed z only k = 0, without the z derivatives
+// (fields.cpp:96-99, :3313-3315, :3323-3325).
 __global__ void k_e(Grid g, YeeGrid y, double dt, double four_pi, double e0y, double e0z) {
-	const int ni = g.N[0] - 3, nj = g.N[1] - 3, nk = g.N[2] - 3;
+	const int ni = g.N[0] - 3, nj = g.N[1] - 3, nk = g.cz ? 1 : g.N[2] - 3;
 	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= (long long)ni * nj * nk) return;
-	const int k = 2 + (int)(t % nk), j = 2 + (int)((t / nk) % nj), i = 2 + (int)(t / ((long long)nk * nj));
-	const int hx = g.N[0] - 2, hy = g.N[1] - 2, hz = g.N[2] - 2;
+	const int k = owned_lo(g, 2) + (int)(t % nk), j = 2 + (int)((t / nk) % nj), i = 2 + (int)(t / ((long long)nk * nj));
+	const int hx = g.N[0] - 2, hy = g.N[1] - 2, hz = g.cz ? 1 : g.N[2] - 2;
 	const double* Bx = y.B[0];
 	const double* By = y.B[1];
 	const double* Bz = y.B[2];
 	if (i < hx) {  // evaluateBunemanRotBx (fields.cpp:3306-3317)
 		double r = 0;
 		r += (Bz[yi(y.dB[2], i, j, k)] - Bz[yi(y.dB[2], i, j - 1, k)]) / g.d[1];
-		r -= (By[yi(y.dB[1], i, j, k)] - By[yi(y.dB[1], i, j, k - 1)]) / g.d[2];
+		if (!g.cz) r -= (By[yi(y.dB[1], i, j, k)] - By[yi(y.dB[1], i, j, k - 1)]) / g.d[2];
 		const long long q = yi(y.dE[0], i, j, k);
 		y.E[0][q] = y.E[0][q] + (kCprimeX * r - four_pi * y.J[0][q]) * dt;
 	}
 	if (j < hy) {  // evaluateBunemanRotBy (:3319-3327)
 		double r = -((Bz[yi(y.dB[2], i, j, k)] - Bz[yi(y.dB[2], i - 1, j, k)]) / g.d[0]);
-		r += (Bx[yi(y.dB[0], i, j, k)] - Bx[yi(y.dB[0], i, j, k - 1)]) / g.d[2];
+		if (!g.cz) r += (Bx[yi(y.dB[0], i, j, k)] - Bx[yi(y.dB[0], i, j, k - 1)]) / g.d[2];
 		const long long q = yi(y.dE[1], i, j, k);
 		y.E[1][q] = y.E[1][q] + (kCprimeX * r - four_pi * y.J[1][q]) * dt;
 		// the wall: zero at the conductor node 2 and E0 at the free end (fields.cpp:117-122)
@@ -184,6 +186,20 @@ __global__ void k_yee_fold(Yee3 y, int ax, int Nax) {
 			at(3 - i) = at(Nax - 1 - i);
 		}
 	}
+}
+
+// Along a collapsed z every exchange and every current sum copies plane 0 onto plane 1 of the
+// arrays with two planes (Ex, Ey, Jx, Jy, Bz; the 2-D branches of exchangeBuneman*AlongZ and
+// sumBuneman*AlongZ, PIC++/fields_exchange.cpp:2823-2829, momentum_exchange.cpp:1980-1986); the
+// others have one plane.
+__global__ void k_yee_copy_z(Yee3 y) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	int c, idx[3];
+	if (!line_of(y, 2, t, c, idx)) return;
+	const int* d = y.d[c];
+	if (d[2] != 2) return;
+	double* f = y.f[c];
+	f[yi(d, idx[0], idx[1], 1)] = f[yi(d, idx[0], idx[1], 0)];
 }
 
 __global__ void k_fixed_to_double(const long long* __restrict__ acc, double* __restrict__ J, long long n, double inv_scale) {
@@ -487,7 +503,7 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 	JBox jb;
 	{
 		const int K = row_ends[1][2] - row_ends[0][2] + 1;
-		jb.use = row_ends[0][0] == row_ends[1][0] && row_ends[0][1] == row_ends[1][1] && K >= 1 && K <= kJBoxK;
+		jb.use = !g.cz && row_ends[0][0] == row_ends[1][0] && row_ends[0][1] == row_ends[1][1] && K >= 1 && K <= kJBoxK;
 		for (int a = 0; a < 3; ++a) jb.o[a] = row_ends[0][a] - 1;
 		jb.kext = K + 3;
 	}
@@ -582,9 +598,16 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 			wn[a][0] = 0.5 + (((g0 + cnt * d) + (g0 + (cnt + 1) * d)) / 2 - x[a]) * id;
 			wn[a][1] = 1.0 - wn[a][0];
 		}
+		if (a == 2 && g.cz) {  // one plane along z: weight 1 at k = 0 (the TWO_D_XY sums, correlation.cpp:627-661)
+			lc[a] = ln[a] = 0;
+			wc[a][0] = wn[a][0] = 1.0;
+			wc[a][1] = wn[a][1] = 0.0;
+			continue;
+		}
 		// stencils inside the arrays: cells 0 .. N-1, nodes 0 .. N
 		if (lc[a] < 0 || lc[a] + 1 > g.N[a] - 1 || ln[a] < 0 || ln[a] + 1 > g.N[a]) ok = false;
 	}
+	const int zs = g.cz ? 0 : 1;  // the block's second z plane (plane 0 again, at weight 0, when collapsed)
 	if (!ok) {  // no access outside the arrays: report and leave the particle
 		atomicOr(status, 8);
 		return;
@@ -617,8 +640,8 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 			for (int sy = 0; sy < 2; ++sy)
 #pragma unroll
 				for (int sz = 0; sz < 2; ++sz) {
-					e += pe[sx * se + sy * dE[2] + sz] * we[0][sx] * we[1][sy] * we[2][sz];
-					b += pb[sx * sb + sy * dB[2] + sz] * wb[0][sx] * wb[1][sy] * wb[2][sz];
+					e += pe[sx * se + sy * dE[2] + sz * zs] * we[0][sx] * we[1][sy] * we[2][sz];
+					b += pb[sx * sb + sy * dB[2] + sz * zs] * wb[0][sx] * wb[1][sy] * wb[2][sz];
 				}
 		Ef[c] = e;
 		Bf[c] = b;
@@ -671,6 +694,10 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 	for (int a = 0; a < 3; ++a) {
 		pre[a] = (int)floor(prev[a] * K.inv_d[a]);
 		r[a] = (loc[a] + prev[a]) / 2;
+		if (a == 2 && g.cz) {  // the TWO_D_XY path (momentum_collecting.cpp:235-277): the z midpoint, plane 0
+			cur[a] = pre[a] = 0;
+			continue;
+		}
 		if (cur[a] != pre[a]) r[a] = fmax(g.x0[a] + pre[a] * g.d[a], g.x0[a] + cur[a] * g.d[a]) - g.x0[a];
 		// J stencils of both parts inside J's arrays (cells 0 .. N-2 along the component's
 		// own axis, nodes 0 .. N-1 along the others)
@@ -690,6 +717,7 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 		W1[a] = (prev[a] + r[a]) * K.inv_2d[a] - pre[a];
 		W2[a] = (loc[a] + r[a]) * K.inv_2d[a] - cur[a];
 	}
+	if (g.cz) W1[2] = W2[2] = 0.0;  // Jx, Jy without a z factor: all on plane 0
 	quantize_part(F1, W1, K.scale, ps.q);
 	{
 		long long q2[12];
@@ -718,7 +746,7 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 	}
 	// one-rank periodic wrap (mover.cpp:770-806 classification, exchangeParticles' shift); x too
 	// unless walled
-	for (int a = WALL ? 1 : 0; a < 3; ++a) {
+	for (int a = WALL ? 1 : 0; a < (g.cz ? 2 : 3); ++a) {  // z unwrapped when collapsed (mover.cpp:794)
 		const double lo = g.x0[a] + 2 * g.d[a], hi = g.x0[a] + (g.N[a] - 2) * g.d[a];
 		if (x[a] < lo)
 			x[a] += g.L[a];
@@ -742,12 +770,12 @@ void yee_dims(const Grid& g, bool b_field, int c, int dims[3]) {
 }
 
 void launch_tristan_b_half(const Grid& g, const YeeGrid& y, double dt, hipStream_t s) {
-	const long long n = (long long)(g.N[0] - 3) * (g.N[1] - 3) * (g.N[2] - 3);
+	const long long n = (long long)(g.N[0] - 3) * (g.N[1] - 3) * (g.cz ? 1 : g.N[2] - 3);
 	hipLaunchKernelGGL(k_b_half, dim3(nb(n, 256)), dim3(256), 0, s, g, y, kCprimeX * dt / 2.0);
 }
 
 void launch_tristan_e(const Grid& g, const YeeGrid& y, double dt, hipStream_t s) {
-	const long long n = (long long)(g.N[0] - 3) * (g.N[1] - 3) * (g.N[2] - 3);
+	const long long n = (long long)(g.N[0] - 3) * (g.N[1] - 3) * (g.cz ? 1 : g.N[2] - 3);
 	const double pi = 4 * atan2(1.0, 1.0);  // PIC++/constants.h:28-29
 	hipLaunchKernelGGL(k_e, dim3(nb(n, 256)), dim3(256), 0, s, g, y, dt, 4 * pi, g.e_right[1], g.e_right[2]);
 }
@@ -761,7 +789,9 @@ static void yee_axes(const Grid& g, Yee3& q, bool fold, hipStream_t s, int zero_
 		for (int c = 0; c < 3; ++c) lines += (long long)q.d[c][o1] * q.d[c][o2];
 		if (g.wall && ax == 0) {
 			if (!fold) hipLaunchKernelGGL(k_yee_wall_x, dim3(nb(lines, 256)), dim3(256), 0, s, q, zero_mask);
-		} else if (fold)
+		} else if (g.cz && ax == 2)
+			hipLaunchKernelGGL(k_yee_copy_z, dim3(nb(lines, 256)), dim3(256), 0, s, q);
+		else if (fold)
 			hipLaunchKernelGGL(k_yee_fold, dim3(nb(lines, 256)), dim3(256), 0, s, q, ax, g.N[ax]);
 		else
 			hipLaunchKernelGGL(k_yee_exchange, dim3(nb(lines, 256)), dim3(256), 0, s, q, ax, g.N[ax]);
@@ -781,12 +811,14 @@ void launch_yee_exchange(const Grid& g, const YeeGrid& y, bool b_field, hipStrea
 // kernel of createSmoothingMatrix over the component's range (along its own axis [2, N - 2),
 // along the others [2, N - 2]), read from src and written to dst; with the wall the plane i = 2
 // (and i = N0 - 2 for the y and z components) is copied instead (the reference's non-periodic
-// branches). The sum runs over the 27 neighbours in the reference's order.
+// branches). The sum runs over the 27 neighbours in the reference's order; along a collapsed z
+// over the 9 of the plane k = 0 (kwidth = 0, smoothing.cpp:546-561).
 __global__ void k_smooth_yee(Grid g, const double* __restrict__ src, double* __restrict__ dst, int comp, Smooth27 sm, int wall,
                              int d1, int d2, int r1, int r2, long long n) {
 	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n) return;
-	const int k = 2 + (int)(t % r2), j = 2 + (int)((t / r2) % r1), i = 2 + (int)(t / ((long long)r2 * r1));
+	const int k = owned_lo(g, 2) + (int)(t % r2), j = 2 + (int)((t / r2) % r1), i = 2 + (int)(t / ((long long)r2 * r1));
+	const int cw = g.cz ? 0 : 1;
 	const long long o = ((long long)i * d1 + j) * d2 + k;
 	if (wall && (i == 2 || (comp != 0 && i == g.N[0] - 2))) {
 		dst[o] = src[o];
@@ -795,16 +827,16 @@ __global__ void k_smooth_yee(Grid g, const double* __restrict__ src, double* __r
 	double acc = 0;
 	for (int a = -1; a <= 1; ++a)
 		for (int b = -1; b <= 1; ++b)
-			for (int c = -1; c <= 1; ++c)
+			for (int c = -cw; c <= cw; ++c)
 				acc += src[((long long)(i + a) * d1 + (j + b)) * d2 + (k + c)] * sm.m[(a + 1) * 9 + (b + 1) * 3 + (c + 1)];
 	dst[o] = acc;
 }
 
 __global__ void k_copy_range(const double* __restrict__ src, double* __restrict__ dst, int d1, int d2, int r1, int r2,
-                             long long n) {
+                             long long n, int k0) {
 	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n) return;
-	const int k = 2 + (int)(t % r2), j = 2 + (int)((t / r2) % r1), i = 2 + (int)(t / ((long long)r2 * r1));
+	const int k = k0 + (int)(t % r2), j = 2 + (int)((t / r2) % r1), i = 2 + (int)(t / ((long long)r2 * r1));
 	const long long o = ((long long)i * d1 + j) * d2 + k;
 	dst[o] = src[o];
 }
@@ -825,10 +857,10 @@ void launch_smooth_yee_current(const Grid& g, const YeeGrid& y, const Smooth27& 
 	for (int c = 0; c < 3; ++c) {
 		const int* d = y.dE[c];
 		int r[3];
-		for (int a = 0; a < 3; ++a) r[a] = a == c ? g.N[a] - 4 : g.N[a] - 3;
+		for (int a = 0; a < 3; ++a) r[a] = (a == 2 && g.cz) ? 1 : (a == c ? g.N[a] - 4 : g.N[a] - 3);
 		const long long n = (long long)r[0] * r[1] * r[2];
 		hipLaunchKernelGGL(k_smooth_yee, dim3(nb(n, 256)), dim3(256), 0, s, g, y.J[c], tmp[c], c, sm, g.wall, d[1], d[2], r[1], r[2], n);
-		hipLaunchKernelGGL(k_copy_range, dim3(nb(n, 256)), dim3(256), 0, s, tmp[c], y.J[c], d[1], d[2], r[1], r[2], n);
+		hipLaunchKernelGGL(k_copy_range, dim3(nb(n, 256)), dim3(256), 0, s, tmp[c], y.J[c], d[1], d[2], r[1], r[2], n, owned_lo(g, 2));
 		q.f[c] = y.J[c];
 		for (int a = 0; a < 3; ++a) q.d[c][a] = d[a];
 	}

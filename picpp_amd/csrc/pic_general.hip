@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256) k_general_fields(Grid g, FieldView v, dou
 	const int nx = g.wall ? g.N[0] - 1 : g.n[0], x_lo = g.wall ? 0 : 2;
 	const long long cols = (long long)nx * g.n[1] * g.n[2];
 	for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < cols; o += (long long)gridDim.x * blockDim.x) {
-		const int k = (int)(o % g.n[2]) + 2;
+		const int k = (int)(o % g.n[2]) + owned_lo(g, 2);
 		const int j = (int)((o / g.n[2]) % g.n[1]) + 2;
 		const int i = (int)(o / ((long long)g.n[2] * g.n[1])) + x_lo;
 		double e[3], b[3], ea[3];
@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(256) k_theo_poynting(Grid g, FieldView v, doub
 	const long long cols = (long long)g.n[1] * g.n[2];
 	const int il = 2, ir = g.N[0] - 2;
 	for (long long o = threadIdx.x; o < cols; o += blockDim.x) {
-		const int k = (int)(o % g.n[2]) + 2, j = (int)(o / g.n[2]) + 2;
+		const int k = (int)(o % g.n[2]) + owned_lo(g, 2), j = (int)(o / g.n[2]) + 2;
 		double el[3], bl[3], er[3], br[3], cl[3], cr[3];
 		field_e(v, il, j, k, el);
 		field_b(v, il, j, k, bl);

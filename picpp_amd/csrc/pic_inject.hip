@@ -193,15 +193,13 @@ __device__ inline void store_particle(const Grid& g, int s, const SpeciesTable& 
 		keys[i] = (uint32_t)((((long long)s * g.ncells + cell) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) |
 		                     (f[2] >= 0.5f));
 	} else {
-		AxisMap m[3];
 #pragma unroll
 		for (int a = 0; a < 3; ++a) {
 			P.x[a][i] = x[a];
 			P.p[a][i] = p[a];
-			m[a] = axis_map(x[a], g.x0[a], g.d[a]);
 		}
 		P.id[i] = id;
-		keys[i] = particle_key(g, s, m);
+		keys[i] = position_key(g, s, x);
 	}
 }
 
@@ -212,7 +210,7 @@ __global__ void k_inject(Grid g, SpeciesTable sp, InjectSlabs S, InjectPhysics p
 	if (t >= (long long)S.count * ncols) return;
 	const int sl = (int)(t / ncols), col = (int)(t % ncols);
 	const int s = S.species[sl];
-	const int j = 2 + col / g.n[2], k = 2 + col % g.n[2];
+	const int j = 2 + col / g.n[2], k = owned_lo(g, 2) + col % g.n[2];
 	uint64_t w[4] = {ph.step, (2ull << 32) | S.slab[sl], 0, 0};
 	philox4x64_10(w, ph.seed, 0);
 	const double dy = g.d[1] * uniform53(w[0]), dz = g.d[2] * uniform53(w[1]);
@@ -246,9 +244,9 @@ __global__ void k_replace(Grid g, SpeciesTable sp, InjectPhysics ph, const int* 
 	uint64_t wy[4] = {ph.step, (1ull << 32) | (uint64_t)r, 1, 0};
 	philox4x64_10(wy, ph.seed, 0);
 	// y, z of the escaped particle; outside the box, redrawn inside its first or last cell
-	// (PIC++/initialization.cpp:8294-8310)
+	// (PIC++/initialization.cpp:8294-8310); a collapsed z is kept as it is
 #pragma unroll
-	for (int a = 1; a < 3; ++a) {
+	for (int a = 1; a < (g.cz ? 2 : 3); ++a) {
 		const double lo = g.x0[a] + 2 * g.d[a], hi = g.x0[a] + (g.N[a] - 2) * g.d[a];
 		if (x[a] < lo) x[a] = lo + g.d[a] * uniform53(wy[2 * (a - 1)]);
 		if (x[a] > hi) x[a] = hi - g.d[a] * uniform53(wy[2 * (a - 1) + 1]);

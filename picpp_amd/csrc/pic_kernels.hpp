@@ -33,7 +33,7 @@ void launch_ghost_fill(const Grid& g, double* f, int ncomp, bool nodes, hipStrea
 struct Smooth27 {
 	double m[27];
 };
-Smooth27 smoothing_matrix(double smoothing_parameter);
+Smooth27 smoothing_matrix(double smoothing_parameter, bool collapsed_z = false);
 void launch_smooth_cells(const Grid& g, double* f, double* tmp, const Smooth27& m, hipStream_t s);
 void launch_smooth_nodes(const Grid& g, double* f, int ncomp, double* tmp, const Smooth27& m, hipStream_t s);
 void launch_div_pressure_update_j(const Grid& g, const double* Pi, double* J, double* divPi, double eta, double dt, hipStream_t s);

@@ -919,9 +919,18 @@ __global__ void k_keys(Grid g, SpeciesTable sp, ParticleArrays P, const int* __r
 	double x[3];
 	for (int c = 0; c < 3; ++c) {
 		x[c] = P.x[c][i];
+		if (c == 2 && g.cz) continue;  // z unwrapped along a collapsed axis
 		double lo = g.x0[c] + 2 * g.d[c];
 		double hi = g.x0[c] + (g.N[c] - 2) * g.d[c];
 		if (!(x[c] >= lo && x[c] <= hi)) atomicOr(status, 4);
+	}
+	if (g.cz) {  // one cell along z: the stencils of x and y, the key of plane 0
+		for (int c = 0; c < 2; ++c) {
+			const AxisMap m = axis_map(x[c], g.x0[c], g.d[c]);
+			if (!(m.nbase >= 0 && m.nbase <= g.N[c] - 1 && m.cbase >= 0 && m.cbase <= g.N[c] - 2)) atomicOr(status, 8);
+		}
+		keys[i] = position_key(g, s, x);
+		return;
 	}
 	AxisMap mp[3];
 	if (!maps3(g, x, mp)) atomicOr(status, 8);
@@ -1138,12 +1147,13 @@ void launch_ghost_fill(const Grid& g, double* f, int ncomp, bool nodes, hipStrea
 	hipLaunchKernelGGL(k_ghost_fill, dim3(blocks(nodes ? g.nnodes : g.ncells, 256)), dim3(256), 0, s, g, f, ncomp, nodes ? 1 : 0,
 	                   wall_zero ? 1 : 0);
 }
-Smooth27 smoothing_matrix(double sp) {
+Smooth27 smoothing_matrix(double sp, bool collapsed_z) {
 	Smooth27 m;
 	const double f[3] = {sp / 2, 1 - sp, sp / 2};  // createSmoothingMatrix, smoothing.cpp:105-137
+	const double fz[3] = {0.0, 1.0, 0.0};          // its factors along a one-cell z (:121-130)
 	for (int a = 0; a < 3; ++a)
 		for (int b = 0; b < 3; ++b)
-			for (int c = 0; c < 3; ++c) m.m[a * 9 + b * 3 + c] = f[a] * f[b] * f[c];
+			for (int c = 0; c < 3; ++c) m.m[a * 9 + b * 3 + c] = f[a] * f[b] * (collapsed_z ? fz[c] : f[c]);
 	return m;
 }
 void launch_smooth_cells(const Grid& g, double* f, double* tmp, const Smooth27& m, hipStream_t s) {

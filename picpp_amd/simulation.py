@@ -80,7 +80,7 @@ class Simulation:
         _lib.check(self.lib.pic_create(ctypes.byref(cfg), ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)),
                    None, "pic_create")
         self.h = h
-        self.N = tuple(v + 4 for v in workload.n)
+        self.N = workload.N
         self.last_solve: Dict[str, float] = {}
 
     def close(self):

@@ -82,7 +82,9 @@ class Workload:
 
     @property
     def N(self) -> Tuple[int, int, int]:
-        return tuple(v + 4 for v in self.n)
+        """Ghosted extent per axis: n + 4, but 1 along a one-cell z (the reference's 2-D XY
+        grids have no ghost layers there, PIC++/initialization.cpp:170-174)."""
+        return tuple(1 if (a == 2 and v == 1) else v + 4 for a, v in enumerate(self.n))
 
     @property
     def cell_volume(self) -> float:
@@ -117,8 +119,9 @@ class Workload:
         return tuple(v)
 
     def origin(self) -> Tuple[float, float, float]:
-        """Coordinate of node 0 per axis: L - 2 dx (PIC++/initialization.cpp:194)."""
-        return tuple(v * self.dx + (-2) * self.dx for v in self.n)
+        """Coordinate of node 0 per axis: L - 2 dx, or L along a one-cell z
+        (PIC++/initialization.cpp:115-131, :194)."""
+        return tuple(v * self.dx - (0 if (a == 2 and v == 1) else 2) * self.dx for a, v in enumerate(self.n))
 
 
 def electron_proton(ppc: int, theta_e: float, mass_ratio: float, theta_p: Optional[float] = None) -> List[Species]:
@@ -299,7 +302,7 @@ def yee_dims(w: Workload, name: str):
     """Shape of a ghosted Yee (Buneman) array of the explicit solver, as the reference
     allocates it (PIC++/initialization.cpp:7146-7174): E and J component c have N points along
     axis c and N + 1 along the others; B component c the opposite."""
-    N = [v + 4 for v in w.n]
+    N = w.N
     c = "xyz".index(name[1])
     b = name[0] == "B"
     return tuple(N[a] if (a == c) != b else N[a] + 1 for a in range(3))

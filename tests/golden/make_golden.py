@@ -38,6 +38,11 @@ fixtures next to this script:
                    wall at cell 20, the fields' conductor at node 2, the free right end): a gamma 2 flow
                    at 28x5x5 cells with escapes through the right end, and the same with injection and
                    replacement, 6 steps
+  explicit2d_ref.npz, explicit2d_shock_ref.npz, explicit2d_inject_ref.npz  the explicit step on
+                   2-D XY grids (one cell along z, the reference's znumberGeneral = 1 paths): the
+                   8x6x1 fast-electron case, and the shipped input's kind of run on 28x6x1 cells
+                   (a gamma 1.5 flow, sigma 0.03, mass ratio 25, 10 ppc per species, the wall,
+                   escapes, 8 smoothing passes), without and with injection, 6 steps each
 
 Usage: python tests/golden/make_golden.py [mini|config1]...
 """
@@ -380,6 +385,47 @@ def make_explicit():
         np.savez_compressed(os.path.join(HERE, fname), **ins, **out)
 
 
+def explicit2d_case():
+    """The explicit case on a 2-D XY grid (8x6x1 cells, the reference's znumberGeneral = 1 paths):
+    fast electrons, perturbed Yee fields, 6 steps of dt = 0.4."""
+    w = dataclasses.replace(wl.mini(n=(8, 6, 1), ppc=4), dt=0.4, steps=6, name="explicit-2d")
+    st = wl.make_initial_state(w)
+    rng = np.random.default_rng(31)
+    e = st.species == 0
+    st.p[e] = w.species[0].mass * rng.normal(0.0, 0.5, (int(e.sum()), 3))
+    return w, st, wl.yee_initial_fields(w, perturb=0.05)
+
+
+def explicit2d_shock_case(inject=False):
+    """The shipped input's kind of run (PIC++/input/input.dat: BUNEMAN on a 2-D XY grid, a
+    gamma 1.5 flow with magnetisation 0.03 and B along z, electron mass 0.04 proton masses, 10
+    particles per cell per species, 8 smoothing passes of s = 0.5) on 28x6x1 cells: the particles
+    right of the particles' wall at xgrid[20], those right of cell 26 turned to +x so that some
+    leave through the free right end; with `inject` the flow is injected and every escape
+    replaced. 6 steps of dt = 0.5."""
+    w = wl.shock(n=(28, 6, 1), ppc=10, gamma0=1.5, sigma=0.03, theta=1.15e-3, mass_ratio=25.0, dt=0.5, steps=6, seed=37,
+                 name="explicit-2d-shock", inject=inject)
+    w = dataclasses.replace(w, smoothing_count=8, smoothing_parameter=0.5)
+    st = wl.make_initial_state(w)
+    x0 = w.origin()[0]
+    keep = st.x[:, 0] >= x0 + 20.25 * w.dx
+    n = int(keep.sum())
+    st = wl.InitialState(st.species[keep], np.arange(n, dtype=np.int32), st.x[keep], st.p[keep], st.E, st.B)
+    right = st.x[:, 0] > x0 + 26 * w.dx
+    st.p[right, 0] = np.abs(st.p[right, 0])
+    return w, st, wl.yee_initial_fields(w, perturb=0.05)
+
+
+def make_explicit2d():
+    w, st, yee = explicit2d_case()
+    ins, out = run_explicit(w, st, yee)
+    np.savez_compressed(os.path.join(HERE, "explicit2d_ref.npz"), **ins, **out)
+    for inject, fname in ((False, "explicit2d_shock_ref.npz"), (True, "explicit2d_inject_ref.npz")):
+        ww, sw, yw = explicit2d_shock_case(inject)
+        ins, out = run_explicit(ww, sw, yw)
+        np.savez_compressed(os.path.join(HERE, fname), **ins, **out)
+
+
 def make_config1():
     w = wl.config1()
     st = wl.make_initial_state(w)
@@ -406,6 +452,6 @@ if __name__ == "__main__":
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref", "-j8"], check=True)
     for name in which:
         {"mini": make_mini, "config1": make_config1, "general": make_general, "inject_hot": make_inject_hot, "wall": make_wall, "wall_out": make_wall_out, "inject": make_inject,
-         "wall_long": make_wall_long, "smooth": make_smooth, "explicit": make_explicit,
+         "wall_long": make_wall_long, "smooth": make_smooth, "explicit": make_explicit, "explicit2d": make_explicit2d,
          "backup": make_backup, "cleanup": make_cleanup}[name]()
         print("wrote", name)

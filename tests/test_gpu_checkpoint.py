@@ -89,6 +89,30 @@ def test_checkpoint_resume_explicit(tmp_path):
     b.close()
 
 
+def test_checkpoint_resume_explicit_2d_shock_with_injection(tmp_path):
+    """The 2-D XY shock run (one cell along z: wall, injection, replacement, 8 smoothing passes;
+    tests/golden/explicit2d_inject_ref.npz): saved after 2 steps, resumed in a fresh context,
+    3 more steps give the uninterrupted run's bits, the injected particles included."""
+    from test_gpu_explicit import explicit2d_shock_workload, open_explicit_wall
+    d = np.load(os.path.join(HERE, "golden", "explicit2d_inject_ref.npz"))
+    w = explicit2d_shock_workload(True)
+    print("case", w.name, flush=True)
+    a = open_explicit_wall(d, w)
+    for _ in range(2):
+        a.explicit_step()
+    path = str(tmp_path / "run.ckpt")
+    a.save_checkpoint(path)
+    for _ in range(3):
+        a.explicit_step()
+    b = Simulation(w, precision="fp64", scheme="explicit")
+    b.load_checkpoint(path)
+    for _ in range(3):
+        b.explicit_step()
+    assert_bitwise(state(a, True), state(b, True))
+    a.close()
+    b.close()
+
+
 def test_checkpoint_refuses_other_configuration_and_truncation(tmp_path):
     w = wl.mini()
     st = wl.make_initial_state(w, perturb=0.05)

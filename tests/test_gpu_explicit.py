@@ -33,7 +33,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def open_explicit(d, w):
-    N = tuple(v + 4 for v in w.n)
+    N = w.N
     st = wl.InitialState(d["in/species"].astype(np.int32), d["in/ids"].astype(np.int32), d["in/x"], d["in/p"],
                          np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3)), np.zeros(N + (3,)))
     sim = Simulation(w, precision="fp64", scheme="explicit")
@@ -112,13 +112,32 @@ def test_explicit_step_with_current_smoothing_matches_reference():
     sim.close()
 
 
+def compare_steps(sim, d, w, worst):
+    """Steps sim through w.steps and compares every step with the golden d: particles by id,
+    the Yee fields, the current of the first two steps and updateEnergy's components."""
+    for step in range(1, w.steps + 1):
+        sim.explicit_step()
+        p = sim.particles()
+        ids, x, mom = by_id(p["ids"], p["x"], p["p"])
+        ref = d["s%d/particles" % step]
+        ref = ref[np.argsort(ref[:, 0])]
+        assert np.array_equal(ids.astype(np.float64), ref[:, 0]), step
+        worst["x"] = max(worst.get("x", 0.0), relerr(x, ref[:, 1:4]))
+        worst["p"] = max(worst.get("p", 0.0), relerr(mom, ref[:, 4:7]))
+        for name in wl.YEE_NAMES + (("Jx", "Jy", "Jz") if step <= 2 else ()):
+            worst[name] = max(worst.get(name, 0.0), relerr(sim.yee(name), d["s%d/%s" % (step, name)]))
+        e = np.asarray(sim.updateEnergy())
+        worst["energy"] = max(worst.get("energy", 0.0), relerr(e[:4], d["s%d/energy" % step][:4]))
+    return worst
+
+
 def explicit_wall_workload(inject):
     return wl.shock(n=(28, 5, 5), ppc=4, gamma0=2.0, theta=0.01, mass_ratio=100.0, dt=0.5, steps=6, seed=23,
                     name="explicit-wall", inject=inject)
 
 
 def open_explicit_wall(d, w):
-    N = tuple(v + 4 for v in w.n)
+    N = w.N
     st = wl.InitialState(d["in/species"].astype(np.int32), d["in/ids"].astype(np.int32), d["in/x"], d["in/p"],
                          np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3)), np.zeros(N + (3,)))
     sim = Simulation(w, precision="fp64", scheme="explicit")
@@ -309,3 +328,89 @@ def test_explicit_step_without_particles():
         for name in ("Jx", "Jy", "Jz"):
             assert not np.any(sim.yee(name)), name
     sim.close()
+
+
+def explicit2d_workload():
+    return dataclasses.replace(wl.mini(n=(8, 6, 1), ppc=4), dt=0.4, steps=6, name="explicit-2d")
+
+
+def explicit2d_shock_workload(inject):
+    w = wl.shock(n=(28, 6, 1), ppc=10, gamma0=1.5, sigma=0.03, theta=1.15e-3, mass_ratio=25.0, dt=0.5, steps=6, seed=37,
+                 name="explicit-2d-shock", inject=inject)
+    return dataclasses.replace(w, smoothing_count=8, smoothing_parameter=0.5)
+
+
+def test_explicit_2d_matches_reference():
+    """The 2-D XY grid (8x6x1 cells; tests/golden/explicit2d_ref.npz, the reference with
+    znumberGeneral = 1): one plane along z without ghost layers, no z derivatives in the curls,
+    the TWO_D_XY gather and zig-zag current (weight 1 on plane 0, the z relay at the midpoint),
+    plane 1 of the two-plane arrays copied from plane 0 by every exchange, z unwrapped. Particles,
+    Yee fields, current and energies within 1e-12 over 6 steps."""
+    d = np.load(os.path.join(HERE, "golden", "explicit2d_ref.npz"))
+    w = explicit2d_workload()
+    print("case", w.name, "particles", len(d["in/ids"]), flush=True)
+    sim = open_explicit(d, w)
+    worst = {name + "0": relerr(sim.yee(name), d["s0/" + name]) for name in wl.YEE_NAMES}
+    compare_steps(sim, d, w, worst)
+    print("explicit 2-D vs reference:", {k: "%.1e" % v for k, v in worst.items()})
+    for k, v in worst.items():
+        assert v <= 1e-12, (k, v)
+    sim.close()
+
+
+def test_explicit_2d_shock_matches_reference():
+    """The shipped input's kind of run on a 2-D XY grid (tests/golden/explicit2d_shock_ref.npz:
+    28x6x1 cells, a gamma 1.5 flow, B along z, mass ratio 25, 10 ppc per species, the particles'
+    wall at xgrid[20], escapes through the free right end, 8 smoothing passes of s = 0.5 every
+    step, 2-D smoothing kernel without z neighbours): within 1e-12 over 6 steps."""
+    d = np.load(os.path.join(HERE, "golden", "explicit2d_shock_ref.npz"))
+    w = explicit2d_shock_workload(False)
+    print("case", w.name, "particles", len(d["in/ids"]), flush=True)
+    sim = open_explicit_wall(d, w)
+    worst = compare_steps(sim, d, w, {})
+    print("explicit 2-D shock vs reference:", {k: "%.1e" % v for k, v in worst.items()},
+          "particles", [len(d["s%d/particles" % s]) for s in range(w.steps + 1)])
+    for k, v in worst.items():
+        assert v <= 1e-12, (k, v)
+    sim.close()
+
+
+def test_explicit_2d_injection_matches_reference():
+    """The same with injection and replacement (tests/golden/explicit2d_inject_ref.npz): the
+    reference's particle count every step; in the first step the kept particles and the fields
+    within 1e-12 and the new particles' x positions bit for bit; the new particles' z in the one
+    cell along z (ygrid[j] + dy, zgrid[0] + dz), the replacements' z their escaped particle's."""
+    d = np.load(os.path.join(HERE, "golden", "explicit2d_inject_ref.npz"))
+    w = explicit2d_shock_workload(True)
+    n0 = len(d["in/ids"])
+    print("case", w.name + "-inject", "particles", n0, flush=True)
+    sim = open_explicit_wall(d, w)
+    for step in range(1, w.steps + 1):
+        sim.explicit_step()
+        p = sim.particles()
+        ref = d["s%d/particles" % step]
+        assert len(p["ids"]) == len(ref), (step, len(p["ids"]), len(ref))
+        if step == 1:
+            old = p["ids"] < n0
+            ids, x, mom = by_id(p["ids"][old], p["x"][old], p["p"][old])
+            r_old = ref[ref[:, 0] < n0]
+            r_old = r_old[np.argsort(r_old[:, 0])]
+            assert np.array_equal(ids.astype(np.float64), r_old[:, 0])
+            assert relerr(x, r_old[:, 1:4]) <= 1e-12 and relerr(mom, r_old[:, 4:7]) <= 1e-12
+            assert np.array_equal(np.sort(p["x"][~old][:, 0]), np.sort(ref[ref[:, 0] >= n0][:, 1]))
+            for name in wl.YEE_NAMES:
+                assert relerr(sim.yee(name), d["s1/" + name]) <= 1e-12, name
+            # the slabs injected in this step have not moved yet: z in the one cell along z
+            z0 = w.origin()[2]
+            inj = ~old & (p["x"][:, 0] < w.origin()[0] + (w.N[0] - 2) * w.dx)
+            assert inj.sum() > 0 and np.all((p["x"][inj, 2] >= z0) & (p["x"][inj, 2] < z0 + w.dx))
+    print("particles per step:", [len(d["s%d/particles" % s]) for s in range(w.steps + 1)])
+    sim.close()
+
+
+def test_2d_grid_needs_the_explicit_scheme():
+    """A one-cell z is built for the explicit scheme only: the implicit scheme refuses it."""
+    from picpp_amd._lib import PicError
+    w = dataclasses.replace(wl.mini(n=(8, 6, 1), ppc=1), name="implicit-2d")
+    with pytest.raises(PicError):
+        Simulation(w, precision="fp64")
