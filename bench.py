@@ -78,6 +78,10 @@ def parse():
                          "the network is trained at that dt on the train_cells^3 box, then both solve held-out systems of "
                          "the benchmark box (--cells, --ppc) to the 1e-8 true residual")
     ap.add_argument("--sweep-held-out", type=int, default=3, help="held-out systems per time step of --precond-dt-sweep")
+    ap.add_argument("--precond-production", type=int, default=0, metavar="HELD_OUT",
+                    help="only train the learned preconditioner on the benchmark box's own solves (--cells, --ppc; the "
+                         "first --train-pairs pairs that the mixed-precision solver logs), then compare plain and learned "
+                         "GMRES on the systems of the next HELD_OUT steps, each to the 1e-8 true residual")
     ap.add_argument("--shock-cells", type=int, default=1024, help="cells along x (64 x 64 across)")
     ap.add_argument("--shock-ppc", type=int, default=16, help="particles per cell per species at injection")
     ap.add_argument("--pressure", default=None, choices=["reference", "cell"],
@@ -233,6 +237,58 @@ def precond_dt_sweep(args, torch, dev):
               file=sys.stderr, flush=True)
         rows.append(row)
     return {"cells": [args.cells] * 3, "ppc_per_species": args.ppc, "held_out_per_dt": args.sweep_held_out, "rows": rows}
+
+
+def precond_production(args, torch, dev):
+    """The learned preconditioner trained on the production box itself: the benchmark box
+    (--cells, --ppc) is stepped in the chosen form while the mixed-precision solver logs its
+    first train_pairs pairs (every Arnoldi step gives (A v_j, v_j), every solve (b, x)); the
+    network is trained on them (10 % held out); then each of the next `precond_production`
+    steps' systems, none seen in training, is solved by plain and by learned-preconditioned
+    GMRES to the 1e-8 true residual."""
+    from picpp_amd import workloads as wl
+
+    w = wl.config2(n=(args.cells,) * 3, ppc=args.ppc, steps=0)
+    sim = open_synthetic_box(w, torch, dev, precision="fp32", stop_rule="true_residual", solver="mixed")
+    npart = sim.num_particles()
+    sim.precond_create()
+    sim.precond_log(args.train_pairs)
+    t0 = time.perf_counter()
+    steps = 0
+    while sim.precond_log(-1) < args.train_pairs and steps < 4000:
+        sim.step()
+        steps += 1
+    torch.cuda.synchronize()
+    t_log = time.perf_counter() - t0
+    npairs = sim.precond_log(-1)
+    print("logged %d pairs from %d steps in %.1f s" % (npairs, steps, t_log), file=sys.stderr, flush=True)
+    _, identity = sim.precond_train(0, 8, 2e-3, 0.1)
+    t0 = time.perf_counter()
+    tl, hl = sim.precond_train(args.train_epochs, 8, 2e-3, 0.1)
+    t_train = time.perf_counter() - t0
+    print("trained %d epochs in %.1f s: held-out loss %.3e (identity %.3e)" % (args.train_epochs, t_train, hl, identity),
+          file=sys.stderr, flush=True)
+    sim.precond_log(0)  # frees the logged pairs
+    held = []
+    for k in range(args.precond_production):
+        h = compare_solvers(sim, torch, True)
+        h["step"] = steps + k + 1
+        held.append(h)
+        print("held-out step %d: plain %.1f iterations %.3f ms, learned %.1f iterations %.3f ms (fallbacks %d)" % (
+            h["step"], h["plain"]["iterations"], h["plain"]["ms"], h["learned"]["iterations"], h["learned"]["ms"],
+            h["learned"]["fallbacks"]), file=sys.stderr, flush=True)
+        sim.step()
+    sim.close()
+    summary = {}
+    for name in ("plain", "learned"):
+        summary[name] = {q: float(np.mean([h[name][q] for h in held])) for q in ("ms", "iterations")}
+        summary[name]["true_residual_max"] = float(max(h[name]["true_residual"] for h in held))
+        summary[name]["fallbacks"] = int(sum(h[name]["fallbacks"] for h in held))
+    return {"cells": [args.cells] * 3, "ppc_per_species": args.ppc, "particles": npart, "dt": w.dt,
+            "training": {"pairs": npairs, "solves": steps, "log_seconds": t_log, "epochs": args.train_epochs,
+                         "train_seconds": t_train, "train_loss": tl, "holdout_loss": hl, "identity_holdout_loss": identity},
+            "held_out_steps": held, "mean": summary,
+            "learned_over_plain_time": summary["learned"]["ms"] / summary["plain"]["ms"]}
 
 
 def time_steps(sim, args):
@@ -637,6 +693,9 @@ def main():
         return
     if args.precond_dt_sweep:
         print(json.dumps({"precond_dt_sweep": precond_dt_sweep(args, torch, dev)}))
+        return
+    if args.precond_production > 0:
+        print(json.dumps({"precond_production": precond_production(args, torch, dev)}))
         return
     if args.explicit > 0:
         print(json.dumps({"explicit": explicit_run(args, torch, dev)}))

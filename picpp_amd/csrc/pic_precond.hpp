@@ -49,6 +49,8 @@ private:
 	uint16_t *x0_ = nullptr, *e1_ = nullptr, *p1_ = nullptr, *e2_ = nullptr, *p2_ = nullptr, *e3_ = nullptr, *d3_ = nullptr,
 	         *u2_ = nullptr, *d2_ = nullptr, *u1_ = nullptr, *d1_ = nullptr;
 	uint16_t* dz_ = nullptr;                            // masked gradient, bf16, largest level
+	uint16_t *dzc_ = nullptr, *xc_ = nullptr;           // channel-major copies of dz_ and a layer's input
+	float* rtmp_ = nullptr;                             // slice sums of the weight-gradient reductions
 	float *g0_ = nullptr, *g1_ = nullptr, *g2_ = nullptr;  // float32 gradient scratch per level
 	float *g0b_ = nullptr, *g1b_ = nullptr;                 // skip-gradient copies
 	float* z_ = nullptr;                                  // forward output for the loss

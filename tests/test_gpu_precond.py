@@ -40,12 +40,16 @@ def rand_params(seed):
     return p.astype(np.float32)
 
 
-def test_network_forward_and_gradients_match_cpu():
+@pytest.mark.parametrize("grid", [(8, 8, 8), (16, 16, 64)])
+def test_network_forward_and_gradients_match_cpu(grid):
+    """8^3: every convolution on the voxel-row kernel (the coarse levels are narrower than a
+    16-voxel row); 16 x 16 x 64: every convolution, forward and adjoint, on the LDS brick
+    kernel (levels 16 x 16 x 64, 8 x 8 x 32, 4 x 4 x 16)."""
     torch = _torch()
-    w = wl.mini(n=(8, 8, 8), ppc=4, seed=11)
+    w = wl.mini(n=grid, ppc=4, seed=11)
     d = w.n
     V = int(np.prod(d))
-    print("case: network on an 8x8x8 grid, random parameters", flush=True)
+    print("case: network on a %dx%dx%d grid, random parameters" % d, flush=True)
     sim = Simulation(w, precision="fp32", solver="mixed", stop_rule="true_residual")
     sim.precond_create()
     p = rand_params(3)
