@@ -477,13 +477,15 @@ __device__ inline void wave_run_sums(long long q[12], int lane, int dist) {
 	seg_step<0x143, 0xC>(q, dist > lane - 32);
 }
 
-template <bool WALL>
+template <bool WALL, bool FLAT>
 __device__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P, const int* __restrict__ species_start,
                                  const YeeGrid& y, double dt, const TristanConsts& K, uint32_t* __restrict__ keys,
                                  int* __restrict__ status, int* __restrict__ dead, uint32_t* __restrict__ esc, long long i,
                                  long long* box, const JBox& jb, PartSum& ps);
 
-template <bool WALL>
+// FLAT: one cell along z (2-D XY grids), a template parameter so that the 3-D stencils keep
+// constant offsets
+template <bool WALL, bool FLAT>
 __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, ParticleArrays P, const int* __restrict__ species_start,
                                                       YeeGrid y, double dt, TristanConsts K, uint32_t* __restrict__ keys,
                                                       uint64_t* __restrict__ mask, int* __restrict__ status,
@@ -503,7 +505,7 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 	JBox jb;
 	{
 		const int K = row_ends[1][2] - row_ends[0][2] + 1;
-		jb.use = !g.cz && row_ends[0][0] == row_ends[1][0] && row_ends[0][1] == row_ends[1][1] && K >= 1 && K <= kJBoxK;
+		jb.use = !FLAT && row_ends[0][0] == row_ends[1][0] && row_ends[0][1] == row_ends[1][1] && K >= 1 && K <= kJBoxK;
 		for (int a = 0; a < 3; ++a) jb.o[a] = row_ends[0][a] - 1;
 		jb.kext = K + 3;
 	}
@@ -513,7 +515,7 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 	ps.species = -1;
 	ps.u2 = 0;
 	ps.bounced = false;
-	tristan_particle<WALL>(g, sp, P, species_start, y, dt, K, keys, status, dead, esc, idx, box, jb, ps);
+	tristan_particle<WALL, FLAT>(g, sp, P, species_start, y, dt, K, keys, status, dead, esc, idx, box, jb, ps);
 	species_u2_max(ps.species, ps.u2, u2_slots);
 	if (WALL) {  // one bit per particle reflected at the wall
 		const unsigned long long rb = __ballot(ps.bounced);
@@ -552,7 +554,7 @@ __global__ void __launch_bounds__(256) k_tristan_push(Grid g, SpeciesTable sp, P
 // changes cell goes to the sums here; the first part, with the second when both share the
 // cell, is left in ps for the wave's run sums. Divisions by the cell sizes, by dt and by m^2 are
 // products with the host's reciprocals.
-template <bool WALL>
+template <bool WALL, bool FLAT>
 __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTable& sp, const ParticleArrays& P,
                                                  const int* __restrict__ species_start, const YeeGrid& y, double dt,
                                                  const TristanConsts& K, uint32_t* __restrict__ keys, int* __restrict__ status,
@@ -598,7 +600,7 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 			wn[a][0] = 0.5 + (((g0 + cnt * d) + (g0 + (cnt + 1) * d)) / 2 - x[a]) * id;
 			wn[a][1] = 1.0 - wn[a][0];
 		}
-		if (a == 2 && g.cz) {  // one plane along z: weight 1 at k = 0 (the TWO_D_XY sums, correlation.cpp:627-661)
+		if (FLAT && a == 2) {  // one plane along z: weight 1 at k = 0 (the TWO_D_XY sums, correlation.cpp:627-661)
 			lc[a] = ln[a] = 0;
 			wc[a][0] = wn[a][0] = 1.0;
 			wc[a][1] = wn[a][1] = 0.0;
@@ -607,7 +609,7 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 		// stencils inside the arrays: cells 0 .. N-1, nodes 0 .. N
 		if (lc[a] < 0 || lc[a] + 1 > g.N[a] - 1 || ln[a] < 0 || ln[a] + 1 > g.N[a]) ok = false;
 	}
-	const int zs = g.cz ? 0 : 1;  // the block's second z plane (plane 0 again, at weight 0, when collapsed)
+	constexpr int zs = FLAT ? 0 : 1;  // the block's second z plane (plane 0 again, at weight 0, when collapsed)
 	if (!ok) {  // no access outside the arrays: report and leave the particle
 		atomicOr(status, 8);
 		return;
@@ -694,7 +696,7 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 	for (int a = 0; a < 3; ++a) {
 		pre[a] = (int)floor(prev[a] * K.inv_d[a]);
 		r[a] = (loc[a] + prev[a]) / 2;
-		if (a == 2 && g.cz) {  // the TWO_D_XY path (momentum_collecting.cpp:235-277): the z midpoint, plane 0
+		if (FLAT && a == 2) {  // the TWO_D_XY path (momentum_collecting.cpp:235-277): the z midpoint, plane 0
 			cur[a] = pre[a] = 0;
 			continue;
 		}
@@ -717,7 +719,7 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 		W1[a] = (prev[a] + r[a]) * K.inv_2d[a] - pre[a];
 		W2[a] = (loc[a] + r[a]) * K.inv_2d[a] - cur[a];
 	}
-	if (g.cz) W1[2] = W2[2] = 0.0;  // Jx, Jy without a z factor: all on plane 0
+	if (FLAT) W1[2] = W2[2] = 0.0;  // Jx, Jy without a z factor: all on plane 0
 	quantize_part(F1, W1, K.scale, ps.q);
 	{
 		long long q2[12];
@@ -746,7 +748,7 @@ __device__ __forceinline__ void tristan_particle(const Grid& g, const SpeciesTab
 	}
 	// one-rank periodic wrap (mover.cpp:770-806 classification, exchangeParticles' shift); x too
 	// unless walled
-	for (int a = WALL ? 1 : 0; a < (g.cz ? 2 : 3); ++a) {  // z unwrapped when collapsed (mover.cpp:794)
+	for (int a = WALL ? 1 : 0; a < (FLAT ? 2 : 3); ++a) {  // z unwrapped when collapsed (mover.cpp:794)
 		const double lo = g.x0[a] + 2 * g.d[a], hi = g.x0[a] + (g.N[a] - 2) * g.d[a];
 		if (x[a] < lo)
 			x[a] += g.L[a];
@@ -908,8 +910,10 @@ void launch_tristan_push(const Grid& g, const SpeciesTable& sp, const ParticleAr
 	}
 	K.scale = scale;
 	for (int q = 0; q < kMaxSpecies; ++q) K.u2_limit[q] = q < sp.count ? u2_limit[q] : 0.0;
-	hipLaunchKernelGGL(g.wall ? k_tristan_push<true> : k_tristan_push<false>, dim3(nb(P.count, 256)), dim3(256), 0, s, g, sp, P,
-	                   species_start, y, dt, K, keys, mask, status, u2_slots, dead, esc, bounce_mask);
+	auto kern = g.wall ? (g.cz ? k_tristan_push<true, true> : k_tristan_push<true, false>)
+	                   : (g.cz ? k_tristan_push<false, true> : k_tristan_push<false, false>);
+	hipLaunchKernelGGL(kern, dim3(nb(P.count, 256)), dim3(256), 0, s, g, sp, P, species_start, y, dt, K, keys, mask, status, u2_slots,
+	                   dead, esc, bounce_mask);
 }
 
 }  // namespace picgpu

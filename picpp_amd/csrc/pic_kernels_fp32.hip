@@ -706,34 +706,44 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 	const Box32<4> bEt = make_box<4>(rf[0], rf[1], rf[2], KB);
 	const Box32<5> bB = make_box<5>(rf[0], rf[1], rf[2], KB);
 	if (use_box) {
-		// Stage the boxes: 4, 16 and 25 rows along k of K + W - 1 points (W = 2, 4, 5), one
-		// point per thread and pass. Point e of a box is row r = e / len, point ld = e - r len
-		// (the quotient from a float reciprocal, exact for these sizes: the fraction of
-		// (e + 1/2) / len is at least 1 / (2 len) away from an integer); a row r of a box of
-		// width W is (la, lc) = (r / W, r % W), r / 5 = (13 r) >> 6 for r < 25. Ghosted indexes
-		// are clamped into the arrays.
+		// Stage the boxes: 4, 16 and 25 rows along k of K + W - 1 points (W = 2, 4, 5). Thread t
+		// loads point t of each box in the same pass, so its three loads are in flight together.
+		// Point e of a box is row r = e / len, point ld = e - r len (the quotient from a float
+		// reciprocal, exact for these sizes: the fraction of (e + 1/2) / len is at least
+		// 1 / (2 len) away from an integer); a row r of a box of width W is (la, lc) = (r / W,
+		// r % W), r / 5 = (13 r) >> 6 for r < 25. Ghosted indexes are clamped into the arrays.
 		const int len0 = KB + 1, len1 = KB + 3, len2 = KB + 4;
-		const int n0 = 4 * len0, n1 = n0 + 16 * len1, n2 = n1 + 25 * len2;
+		const int n0 = 4 * len0, n1 = 16 * len1, n2 = 25 * len2;
 		const float rl0 = rcp32((float)len0), rl1 = rcp32((float)len1), rl2 = rcp32((float)len2);
-		for (int e = threadIdx.x; e < n2; e += blockDim.x) {
-			const int kind = e < n0 ? 0 : (e < n1 ? 1 : 2);  // E^n, E^{n+theta}, B
-			const int eb = e - (kind == 0 ? 0 : (kind == 1 ? n0 : n1));
-			const int len = kind == 0 ? len0 : (kind == 1 ? len1 : len2);
-			const float rl = kind == 0 ? rl0 : (kind == 1 ? rl1 : rl2);
-			const int r = (int)(((float)eb + 0.5f) * rl);
-			const int ld = eb - r * len;
-			const int w = kind == 0 ? 2 : (kind == 1 ? 4 : 5);
-			const int la = kind == 2 ? (r * 13) >> 6 : r >> (kind + 1);
-			const int lc = r - la * w;
-			const bool node = kind < 2;
-			const int lim = node ? 0 : 1;  // nodes 0..N, cells 0..N-1
-			const int a = clampi(rf[0] - kind + la, 0, g.N[0] - lim), c = clampi(rf[1] - kind + lc, 0, g.N[1] - lim),
-			          d = clampi(rf[2] - kind + ld, 0, g.N[2] - lim);
-			const int S1 = node ? g.N[1] + 1 : g.N[1], S2 = node ? g.N[2] + 1 : g.N[2];
-			const float4* src = kind == 0 ? E : (kind == 1 ? Et : B);
-			const int sj = kind == 0 ? Box32<2>::sj : (kind == 1 ? Box32<4>::sj : Box32<5>::sj);
-			const int lo = kind == 0 ? 0 : (kind == 1 ? oEt : oB);
-			boxes[lo + r * sj + ld] = src[(a * S1 + c) * S2 + d];
+		const int nj = g.N[1] + 1, nk = g.N[2] + 1;  // node strides (cells: N[1], N[2])
+		for (int e = threadIdx.x; e < n2; e += blockDim.x) {  // n2 is the largest
+			float4 v0, v1, v2;
+			int o0 = 0, o1 = 0, o2 = 0;
+			if (e < n0) {  // E^n on the row's nodes
+				const int r = (int)(((float)e + 0.5f) * rl0), ld = e - r * len0;
+				const int a = clampi(rf[0] + (r >> 1), 0, g.N[0]), c = clampi(rf[1] + (r & 1), 0, g.N[1]),
+				          d = clampi(rf[2] + ld, 0, g.N[2]);
+				v0 = E[(a * nj + c) * nk + d];
+				o0 = r * Box32<2>::sj + ld;
+			}
+			if (e < n1) {  // E^{n+theta}, one node of margin
+				const int r = (int)(((float)e + 0.5f) * rl1), ld = e - r * len1;
+				const int a = clampi(rf[0] - 1 + (r >> 2), 0, g.N[0]), c = clampi(rf[1] - 1 + (r & 3), 0, g.N[1]),
+				          d = clampi(rf[2] - 1 + ld, 0, g.N[2]);
+				v1 = Et[(a * nj + c) * nk + d];
+				o1 = oEt + r * Box32<4>::sj + ld;
+			}
+			{  // B, two cells of margin
+				const int r = (int)(((float)e + 0.5f) * rl2), ld = e - r * len2;
+				const int la = (r * 13) >> 6, lc = r - 5 * la;
+				const int a = clampi(rf[0] - 2 + la, 0, g.N[0] - 1), c = clampi(rf[1] - 2 + lc, 0, g.N[1] - 1),
+				          d = clampi(rf[2] - 2 + ld, 0, g.N[2] - 1);
+				v2 = B[(a * g.N[1] + c) * g.N[2] + d];
+				o2 = oB + r * Box32<5>::sj + ld;
+			}
+			if (e < n0) boxes[o0] = v0;
+			if (e < n1) boxes[o1] = v1;
+			boxes[o2] = v2;
 		}
 	}
 	__syncthreads();
