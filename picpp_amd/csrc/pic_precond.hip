@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(256) k_conv3(const uint16_t* __restrict__ X, c
 // The same convolution on bricks staged in LDS. A workgroup of BX waves computes a brick of
 // BX x BY x 16 output voxels: wave w takes x = x0 + w and the BY rows y = y0 + r, each row
 // the 16 voxels z0 .. z0 + 15 along z. The input box around the brick, (BX + 2) x (BY + 2) x
-// 18 voxels with periodic wrap, is loaded once into LDS (channels innermost, 16-byte pieces),
+// 18 voxels with periodic wrap, is loaded once into LDS (16-byte pieces of 8 channels),
 // so the 27 taps read LDS at offsets that are constants of the unrolled chunk loop, and each
 // chunk's weights (one 16-byte load per lane, shared by the BY rows) come from L1. The MFMA
 // takes the weights as its first operand, so a lane of the result holds 4 consecutive output
@@ -160,68 +160,84 @@ __global__ void __launch_bounds__(64 * BX) k_conv3_brick(const uint16_t* __restr
 	const int z0 = (b % nbz) * 16;
 	b /= nbz;
 	const int y0 = (b % nby) * BY, x0 = (b / nby) * BX;
-	// one voxel (CV pieces) per thread and pass
+	// one voxel (CV pieces) per thread and pass; in LDS each row (hx, hy) holds its HZ voxels'
+	// piece 0, then their piece 1, ...: the 16 lanes of a lane group read 16 consecutive
+	// voxels' pieces, 256 contiguous bytes (no bank conflicts)
 	for (int vox = threadIdx.x; vox < NVOX; vox += 64 * BX) {
 		const int hz = vox % HZ, hxy = vox / HZ;
 		const int hy = hxy % HY, hx = hxy / HY;
 		const int gx = wrap(x0 - 1 + hx, d0), gy = wrap(y0 - 1 + hy, d1), gz = wrap(z0 - 1 + hz, d2);
 		const uint4* src = reinterpret_cast<const uint4*>(X + (((long long)gx * d1 + gy) * d2 + gz) * CIN);
 #pragma unroll
-		for (int q = 0; q < CV; ++q) box[vox * CV + q] = src[q];
+		for (int q = 0; q < CV; ++q) box[(hxy * CV + q) * HZ + hz] = src[q];
 	}
 	__syncthreads();
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	const int r16 = lane & 15, kg = lane >> 4;
-	// the lane's piece of tap (0, 0, 0) of row 0; chunk c adds the tap's constant offset
-	const int lane_base = (((w + 1) * HY + 1) * HZ + 1 + r16) * CV;
+	// the lane's voxel of tap (0, 0, 0) of row 0, piece 0; chunk c adds the tap's and the
+	// piece's constant offsets
+	const int lane_base = ((w + 1) * HY + 1) * CV * HZ + 1 + r16;
 	f32x4 acc[BY][NT];
 #pragma unroll
 	for (int r = 0; r < BY; ++r)
 #pragma unroll
 		for (int n = 0; n < NT; ++n) acc[r][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-	for (int c = 0; c < NCH; ++c) {
+	// chunk c's operands: A pieces of the BY rows from LDS, the weights from L1
+	auto load_chunk = [&](const int c, bf16x8 (&a)[BY], bf16x8 (&bw)[NT]) {
 		// kf = 32 c + 8 kg = t CIN + ci: for CIN = 16 the two halves of the wave take taps 2c
 		// and 2c + 1; otherwise the chunk holds one tap
 		const int h = CIN == 16 ? kg >> 1 : 0;
 		const int t = CIN == 16 ? 2 * c + h : (c * 32) / CIN;
 		const int ci = CIN == 16 ? (kg & 1) * 8 : (c * 32) % CIN + kg * 8;
 		const int t0 = CIN == 16 ? 2 * c : t, t1 = CIN == 16 ? 2 * c + 1 : t;
-		constexpr int sx = HY * HZ;
-		const int o0 = ((t0 / 9 - 1) * sx + ((t0 / 3) % 3 - 1) * HZ + (t0 % 3 - 1)) * CV;
-		const int o1 = ((t1 / 9 - 1) * sx + ((t1 / 3) % 3 - 1) * HZ + (t1 % 3 - 1)) * CV;
-		const int off = (h ? o1 : o0) + ci / 8;
+		constexpr int sy = CV * HZ, sx = HY * sy;
+		const int o0 = (t0 / 9 - 1) * sx + ((t0 / 3) % 3 - 1) * sy + (t0 % 3 - 1);
+		const int o1 = (t1 / 9 - 1) * sx + ((t1 / 3) % 3 - 1) * sy + (t1 % 3 - 1);
+		const int off = (h ? o1 : o0) + (ci / 8) * HZ;
 		const bool valid = t < 27;
-		bf16x8 a[BY], bw[NT];
 #pragma unroll
 		for (int r = 0; r < BY; ++r)
-			a[r] = valid ? __builtin_bit_cast(bf16x8, box[lane_base + off + r * HZ * CV]) : bf16x8{};
+			a[r] = valid ? __builtin_bit_cast(bf16x8, box[lane_base + off + r * sy]) : bf16x8{};
 #pragma unroll
 		for (int n = 0; n < NT; ++n)
 			bw[n] = valid ? load8(W + ((long long)(valid ? t : 0) * COUT + n * 16 + r16) * CIN + ci) : bf16x8{};
+	};
+	// double-buffered: chunk c + 1's loads are issued before chunk c's MFMAs
+	bf16x8 a[2][BY], bw[2][NT];
+	load_chunk(0, a[0], bw[0]);
+#pragma unroll
+	for (int c = 0; c < NCH; ++c) {
+		const int cur = c & 1;
+		if (c + 1 < NCH) load_chunk(c + 1, a[cur ^ 1], bw[cur ^ 1]);
 #pragma unroll
 		for (int r = 0; r < BY; ++r)
 #pragma unroll
-			for (int n = 0; n < NT; ++n) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[n], a[r], acc[r][n], 0, 0, 0);
+			for (int n = 0; n < NT; ++n)
+				acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[cur][n], a[cur][r], acc[r][n], 0, 0, 0);
 	}
 	// result layout: lane (kg, r16) holds output channels n 16 + 4 kg + i of voxel z0 + r16
+	float bv[NT][4];
+#pragma unroll
+	for (int n = 0; n < NT; ++n)
+#pragma unroll
+		for (int i = 0; i < 4; ++i) bv[n][i] = bias ? bias[n * 16 + 4 * kg + i] : 0.0f;
+	OutT* y0p = Y + ((((long long)(x0 + w) * d1 + y0) * d2 + z0 + r16) * COUT + 4 * kg);
 #pragma unroll
 	for (int r = 0; r < BY; ++r) {
-		const long long v = ((long long)(x0 + w) * d1 + y0 + r) * d2 + z0 + r16;
+		OutT* yp = y0p + (long long)r * d2 * COUT;
 #pragma unroll
 		for (int n = 0; n < NT; ++n) {
-			const int co = n * 16 + 4 * kg;
 			float val[4];
 #pragma unroll
 			for (int i = 0; i < 4; ++i) {
-				val[i] = acc[r][n][i] + (bias ? bias[co + i] : 0.0f);
+				val[i] = acc[r][n][i] + bv[n][i];
 				if (relu) val[i] = val[i] > 0.0f ? val[i] : 0.0f;
 			}
 			if constexpr (sizeof(OutT) == 2) {
 				uint2 pk = make_uint2(f2bf(val[0]) | (uint32_t)f2bf(val[1]) << 16, f2bf(val[2]) | (uint32_t)f2bf(val[3]) << 16);
-				*reinterpret_cast<uint2*>(Y + v * COUT + co) = pk;
+				*reinterpret_cast<uint2*>(yp + n * 16) = pk;
 			} else {
-				*reinterpret_cast<float4*>(Y + v * COUT + co) = make_float4(val[0], val[1], val[2], val[3]);
+				*reinterpret_cast<float4*>(yp + n * 16) = make_float4(val[0], val[1], val[2], val[3]);
 			}
 		}
 	}
@@ -232,7 +248,7 @@ __global__ void __launch_bounds__(64 * BX) k_conv3_brick(const uint16_t* __restr
 // layers (the coarsest level).
 template <int CIN, int COUT>
 struct BrickShape {
-	static constexpr int BX = CIN == 64 || COUT == 64 ? 2 : 4;
+	static constexpr int BX = CIN == 64 || COUT == 64 ? 2 : (CIN == 16 && COUT == 16 ? 8 : 4);
 	static constexpr int BY = CIN == 64 || COUT == 64 ? 2 : 4;
 };
 
