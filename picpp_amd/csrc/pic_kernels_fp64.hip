@@ -161,23 +161,34 @@ __global__ void k_pressure_cell(Grid g, const T* __restrict__ ps, double* __rest
 // One-rank periodic folds, one axis per launch (momentum_exchange.cpp:172-195, :446-460,
 // :807-811, :1657-1660). Node arrays: pairs (N - i, 4 - i), i = 0..4. Cell arrays: pairs
 // (N-1-i, lo-i), i < pairs. Every thread owns one (line, pair, component).
+// Thread order: along x and y folds the z index varies fastest (the planes' rows are
+// contiguous); along z the pair index, then y.
 __global__ void k_fold(Grid g, double* __restrict__ f, int ncomp, int axis, int nodes, int pairs, int hi0, int lo0) {
 	int ext[3];
 	for (int a = 0; a < 3; ++a) ext[a] = g.N[a] + (nodes ? 1 : 0);
-	int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
-	long long lines = (long long)ext[o1] * ext[o2];
 	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	long long total = lines * pairs * ncomp;
+	const int p1 = axis == 0 ? 1 : 0;  // the slower of the two plane axes (x or y)
+	long long total = (long long)ext[p1] * ext[axis == 2 ? 1 : 2] * pairs * ncomp;
 	if (t >= total) return;
-	int comp = (int)(t % ncomp);
-	int pr = (int)((t / ncomp) % pairs);
-	long long line = t / ((long long)ncomp * pairs);
-	int a = (int)(line / ext[o2]), bb = (int)(line % ext[o2]);
-	int hi[3], lo[3];
+	int hi[3], lo[3], pr;
+	if (axis == 2) {
+		pr = (int)(t % pairs);
+		long long r = t / pairs;
+		hi[1] = lo[1] = (int)(r % ext[1]);
+		r /= ext[1];
+		hi[0] = lo[0] = (int)(r % ext[0]);
+		t = r / ext[0];
+	} else {
+		hi[2] = lo[2] = (int)(t % ext[2]);
+		long long r = t / ext[2];
+		hi[p1] = lo[p1] = (int)(r % ext[p1]);
+		r /= ext[p1];
+		pr = (int)(r % pairs);
+		t = r / pairs;
+	}
+	const int comp = (int)t;
 	hi[axis] = hi0 - pr;
 	lo[axis] = lo0 - pr;
-	hi[o1] = lo[o1] = a;
-	hi[o2] = lo[o2] = bb;
 	long long len = nodes ? g.nnodes : g.ncells;
 	long long ih = nodes ? node_id(g, hi[0], hi[1], hi[2]) : cell_id(g, hi[0], hi[1], hi[2]);
 	long long il = nodes ? node_id(g, lo[0], lo[1], lo[2]) : cell_id(g, lo[0], lo[1], lo[2]);
