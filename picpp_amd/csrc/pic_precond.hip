@@ -137,21 +137,21 @@ __global__ void __launch_bounds__(256) k_conv3(const uint16_t* __restrict__ X, c
 			}
 }
 
-// The same convolution on bricks staged in LDS. A workgroup of BX waves computes a brick of
-// BX x BY x 16 output voxels: wave w takes x = x0 + w and the BY rows y = y0 + r, each row
-// the 16 voxels z0 .. z0 + 15 along z. The input box around the brick, (BX + 2) x (BY + 2) x
+// The same convolution on bricks staged in LDS. A workgroup of BX x NG waves computes a brick
+// of BX x BY x 16 output voxels: wave (w, g) takes x = x0 + w and the BY rows y = y0 + r, each
+// row the 16 voxels z0 .. z0 + 15 along z, for the g-th NG-th of the output channels. The input box around the brick, (BX + 2) x (BY + 2) x
 // 18 voxels with periodic wrap, is loaded once into LDS (16-byte pieces of 8 channels),
 // so the 27 taps read LDS at offsets that are constants of the unrolled chunk loop, and each
 // chunk's weights (one 16-byte load per lane, shared by the BY rows) come from L1. The MFMA
 // takes the weights as its first operand, so a lane of the result holds 4 consecutive output
 // channels of one voxel and stores them at once. Same sums in the same order as k_conv3.
 // Needs d0 % BX = d1 % BY = d2 % 16 = 0.
-template <int CIN, int COUT, int BX, int BY, class OutT>
-__global__ void __launch_bounds__(64 * BX) k_conv3_brick(const uint16_t* __restrict__ X, const uint16_t* __restrict__ W,
+template <int CIN, int COUT, int BX, int BY, int NG, class OutT>
+__global__ void __launch_bounds__(64 * BX * NG) k_conv3_brick(const uint16_t* __restrict__ X, const uint16_t* __restrict__ W,
                                                          const float* __restrict__ bias, int relu, OutT* __restrict__ Y, int d0, int d1,
                                                          int d2) {
 	constexpr int HX = BX + 2, HY = BY + 2, HZ = 18, CV = CIN / 8;
-	constexpr int NT = COUT / 16;
+	constexpr int NT = COUT / 16 / NG;  // output-channel tiles per wave
 	constexpr int NCH = (27 * CIN + 31) / 32;
 	constexpr int NVOX = HX * HY * HZ;
 	__shared__ uint4 box[NVOX * CV];
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(64 * BX) k_conv3_brick(const uint16_t* __restr
 	// one voxel (CV pieces) per thread and pass; in LDS each row (hx, hy) holds its HZ voxels'
 	// piece 0, then their piece 1, ...: the 16 lanes of a lane group read 16 consecutive
 	// voxels' pieces, 256 contiguous bytes (no bank conflicts)
-	for (int vox = threadIdx.x; vox < NVOX; vox += 64 * BX) {
+	for (int vox = threadIdx.x; vox < NVOX; vox += 64 * BX * NG) {
 		const int hz = vox % HZ, hxy = vox / HZ;
 		const int hy = hxy % HY, hx = hxy / HY;
 		const int gx = wrap(x0 - 1 + hx, d0), gy = wrap(y0 - 1 + hy, d1), gz = wrap(z0 - 1 + hz, d2);
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(64 * BX) k_conv3_brick(const uint16_t* __restr
 		for (int q = 0; q < CV; ++q) box[(hxy * CV + q) * HZ + hz] = src[q];
 	}
 	__syncthreads();
-	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) % BX, ng = (threadIdx.x >> 6) / BX;
 	const int r16 = lane & 15, kg = lane >> 4;
 	// the lane's voxel of tap (0, 0, 0) of row 0, piece 0; chunk c adds the tap's and the
 	// piece's constant offsets
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(64 * BX) k_conv3_brick(const uint16_t* __restr
 			a[r] = valid ? __builtin_bit_cast(bf16x8, box[lane_base + off + r * sy]) : bf16x8{};
 #pragma unroll
 		for (int n = 0; n < NT; ++n)
-			bw[n] = valid ? load8(W + ((long long)(valid ? t : 0) * COUT + n * 16 + r16) * CIN + ci) : bf16x8{};
+			bw[n] = valid ? load8(W + ((long long)(valid ? t : 0) * COUT + (ng * NT + n) * 16 + r16) * CIN + ci) : bf16x8{};
 	};
 	// double-buffered: chunk c + 1's loads are issued before chunk c's MFMAs
 	bf16x8 a[2][BY], bw[2][NT];
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(64 * BX) k_conv3_brick(const uint16_t* __restr
 #pragma unroll
 	for (int n = 0; n < NT; ++n)
 #pragma unroll
-		for (int i = 0; i < 4; ++i) bv[n][i] = bias ? bias[n * 16 + 4 * kg + i] : 0.0f;
+		for (int i = 0; i < 4; ++i) bv[n][i] = bias ? bias[(ng * NT + n) * 16 + 4 * kg + i] : 0.0f;
 	OutT* y0p = Y + ((((long long)(x0 + w) * d1 + y0) * d2 + z0 + r16) * COUT + 4 * kg);
 #pragma unroll
 	for (int r = 0; r < BY; ++r) {
@@ -235,31 +235,35 @@ __global__ void __launch_bounds__(64 * BX) k_conv3_brick(const uint16_t* __restr
 			}
 			if constexpr (sizeof(OutT) == 2) {
 				uint2 pk = make_uint2(f2bf(val[0]) | (uint32_t)f2bf(val[1]) << 16, f2bf(val[2]) | (uint32_t)f2bf(val[3]) << 16);
-				*reinterpret_cast<uint2*>(yp + n * 16) = pk;
+				*reinterpret_cast<uint2*>(yp + (ng * NT + n) * 16) = pk;
 			} else {
-				*reinterpret_cast<float4*>(yp + n * 16) = make_float4(val[0], val[1], val[2], val[3]);
+				*reinterpret_cast<float4*>(yp + (ng * NT + n) * 16) = make_float4(val[0], val[1], val[2], val[3]);
 			}
 		}
 	}
 }
 
-// Brick shapes: 4 waves of 4 rows where the input box fits LDS comfortably (measured at 128^3
-// for 16 -> 16 channels: 51.7 us against 54.5 us with 8 rows), 2 x 2 for the 64-channel
-// layers (the coarsest level).
+// Brick shapes: BX x NG waves, the NG wave groups splitting the output channels. 8 waves of 4
+// rows for 16 -> 16 channels (measured at 128^3: 44.8 us against 52.2 us with 4 waves and
+// 54.5 us with 4 waves of 8 rows), 4 waves of 4 rows for the middle level, 2 x 2 bricks for
+// the 64-channel layers (the coarsest level: 32^3 at 128^3), with the output tiles spread
+// over the waves so that the few bricks still give every SIMD several waves.
 template <int CIN, int COUT>
 struct BrickShape {
-	static constexpr int BX = CIN == 64 || COUT == 64 ? 2 : (CIN == 16 && COUT == 16 ? 8 : 4);
-	static constexpr int BY = CIN == 64 || COUT == 64 ? 2 : 4;
+	static constexpr bool coarse = CIN == 64 || COUT == 64;
+	static constexpr int BX = coarse ? 2 : (CIN == 16 && COUT == 16 ? 8 : 4);
+	static constexpr int BY = coarse ? 2 : 4;
+	static constexpr int NG = coarse ? COUT / 16 : 1;
 };
 
 template <int CIN, int COUT, class OutT>
 void conv3(const uint16_t* X, const uint16_t* W, const float* b, bool relu, OutT* Y, const int d[3], hipStream_t s) {
 	long long V = (long long)d[0] * d[1] * d[2];
-	constexpr int BX = BrickShape<CIN, COUT>::BX, BY = BrickShape<CIN, COUT>::BY;
+	constexpr int BX = BrickShape<CIN, COUT>::BX, BY = BrickShape<CIN, COUT>::BY, NG = BrickShape<CIN, COUT>::NG;
 	if (d[0] % BX == 0 && d[1] % BY == 0 && d[2] % 16 == 0) {
 		unsigned blocks = (unsigned)(V / (BX * BY * 16));
-		hipLaunchKernelGGL((k_conv3_brick<CIN, COUT, BX, BY, OutT>), dim3(blocks), dim3(64 * BX), 0, s, X, W, b, relu ? 1 : 0, Y,
-		                   d[0], d[1], d[2]);
+		hipLaunchKernelGGL((k_conv3_brick<CIN, COUT, BX, BY, NG, OutT>), dim3(blocks), dim3(64 * BX * NG), 0, s, X, W, b, relu ? 1 : 0,
+		                   Y, d[0], d[1], d[2]);
 		return;
 	}
 	unsigned blocks = (unsigned)((V + 127) / 128);
