@@ -7,11 +7,12 @@ plain GMRES. Workload: the one-rank production box, 128^3 cells, 64 particles pe
 per species (electron-proton, m_p/m_e = 100, kT/mc^2 = 0.01, omega_ce/omega_pe = 0.5,
 dt = 0.1/omega_pe), synthetic seeded initial state generated on the GPU.
 
-Default run: the chosen form (FP32 mover/deposit, mixed-precision GMRES) is timed for
-`--steps` steps and gives `value`; the learned preconditioner is trained on a 32^3 box of
-the same physics and compared with plain GMRES on a held-out 128^3 system; the plain FP64
-step (FP64 GMRES to the same residual) is timed beside it; the CPU baseline is the FP64
-oracle on the reference's CPU case.
+Default run: the chosen form (FP32 mover/deposit, mixed-precision GMRES) is timed for `--steps`
+steps from the initial state and gives `value`; the learned preconditioner is then trained on
+the box's own solves (2000 pairs logged over the following steps, untimed), and the systems of
+the next `--held-out-steps` steps are solved by plain and by learned-preconditioned GMRES; the
+plain FP64 step (FP64 GMRES to the same residual) and the explicit step are timed beside it
+from the same initial state; the CPU baseline is the FP64 oracle on the reference's CPU case.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 128] [--ppc 64]
                        [--precision fp32|fp64] [--solver mixed|fp64] [--learned off|compare|use]
@@ -52,7 +53,13 @@ def parse():
     ap.add_argument("--learned", default="compare", choices=["off", "compare", "use"],
                     help="learned preconditioner: off; trained and compared with plain GMRES on held-out systems "
                          "(compare); or also used in the timed steps (use)")
-    ap.add_argument("--train-cells", type=int, default=32, help="cells per axis of the box the network is trained on")
+    ap.add_argument("--train-on", default="production", choices=["production", "small"],
+                    help="learned preconditioner's training pairs: the first --train-pairs that the benchmark box's own "
+                         "solves log (production), or a train_cells^3 box of the same physics (small)")
+    ap.add_argument("--held-out-steps", type=int, default=3,
+                    help="after the timed steps, the systems of this many further steps are solved by plain and by "
+                         "learned-preconditioned GMRES (none seen in training)")
+    ap.add_argument("--train-cells", type=int, default=32, help="cells per axis of the small training box")
     ap.add_argument("--train-pairs", type=int, default=2000)
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--no-fp64-compare", action="store_true", help="skip timing the plain FP64 step beside the chosen form")
@@ -239,18 +246,11 @@ def precond_dt_sweep(args, torch, dev):
     return {"cells": [args.cells] * 3, "ppc_per_species": args.ppc, "held_out_per_dt": args.sweep_held_out, "rows": rows}
 
 
-def precond_production(args, torch, dev):
-    """The learned preconditioner trained on the production box itself: the benchmark box
-    (--cells, --ppc) is stepped in the chosen form while the mixed-precision solver logs its
-    first train_pairs pairs (every Arnoldi step gives (A v_j, v_j), every solve (b, x)); the
-    network is trained on them (10 % held out); then each of the next `precond_production`
-    steps' systems, none seen in training, is solved by plain and by learned-preconditioned
-    GMRES to the 1e-8 true residual."""
-    from picpp_amd import workloads as wl
-
-    w = wl.config2(n=(args.cells,) * 3, ppc=args.ppc, steps=0)
-    sim = open_synthetic_box(w, torch, dev, precision="fp32", stop_rule="true_residual", solver="mixed")
-    npart = sim.num_particles()
+def log_and_train(sim, args, torch):
+    """Trains the learned preconditioner of `sim` on its own solves: the box is stepped in the
+    chosen form while the mixed-precision solver logs its first train_pairs pairs (every Arnoldi
+    step gives (A v_j, v_j), every solve (b, x)); the network is trained on them with 10 % held
+    out, and the logged pairs are freed."""
     sim.precond_create()
     sim.precond_log(args.train_pairs)
     t0 = time.perf_counter()
@@ -269,26 +269,50 @@ def precond_production(args, torch, dev):
     print("trained %d epochs in %.1f s: held-out loss %.3e (identity %.3e)" % (args.train_epochs, t_train, hl, identity),
           file=sys.stderr, flush=True)
     sim.precond_log(0)  # frees the logged pairs
+    return {"box": list(sim.w.n), "pairs": npairs, "solves": steps, "log_seconds": t_log, "epochs": args.train_epochs,
+            "train_seconds": t_train, "train_loss": tl, "holdout_loss": hl, "identity_holdout_loss": identity}
+
+
+def held_out_steps(sim, torch, learned, count):
+    """The systems of the next `count` steps, none seen in training, each solved by plain and (if
+    trained) learned-preconditioned mixed GMRES to the 1e-8 true residual: per step and mean."""
     held = []
-    for k in range(args.precond_production):
-        h = compare_solvers(sim, torch, True)
-        h["step"] = steps + k + 1
+    for k in range(count):
+        h = compare_solvers(sim, torch, learned)
         held.append(h)
-        print("held-out step %d: plain %.1f iterations %.3f ms, learned %.1f iterations %.3f ms (fallbacks %d)" % (
-            h["step"], h["plain"]["iterations"], h["plain"]["ms"], h["learned"]["iterations"], h["learned"]["ms"],
-            h["learned"]["fallbacks"]), file=sys.stderr, flush=True)
-        sim.step()
-    sim.close()
+        print("held-out step %d: plain %.1f iterations %.3f ms" % (k + 1, h["plain"]["iterations"], h["plain"]["ms"])
+              + (", learned %.1f iterations %.3f ms (fallbacks %d)" % (h["learned"]["iterations"], h["learned"]["ms"],
+                                                                       h["learned"]["fallbacks"]) if learned else ""),
+              file=sys.stderr, flush=True)
+        if k + 1 < count:
+            sim.step()
     summary = {}
     for name in ("plain", "learned"):
+        if name not in held[0]:
+            continue
         summary[name] = {q: float(np.mean([h[name][q] for h in held])) for q in ("ms", "iterations")}
         summary[name]["true_residual_max"] = float(max(h[name]["true_residual"] for h in held))
         summary[name]["fallbacks"] = int(sum(h[name]["fallbacks"] for h in held))
-    return {"cells": [args.cells] * 3, "ppc_per_species": args.ppc, "particles": npart, "dt": w.dt,
-            "training": {"pairs": npairs, "solves": steps, "log_seconds": t_log, "epochs": args.train_epochs,
-                         "train_seconds": t_train, "train_loss": tl, "holdout_loss": hl, "identity_holdout_loss": identity},
-            "held_out_steps": held, "mean": summary,
-            "learned_over_plain_time": summary["learned"]["ms"] / summary["plain"]["ms"]}
+    out = {"steps": held, "mean": summary}
+    if "learned" in summary:
+        out["learned_over_plain_time"] = summary["learned"]["ms"] / summary["plain"]["ms"]
+    return out
+
+
+def precond_production(args, torch, dev):
+    """The learned preconditioner trained on the production box itself (log_and_train on the
+    benchmark box, --cells, --ppc), then plain against learned-preconditioned GMRES on the
+    systems of the next `precond_production` steps, each to the 1e-8 true residual."""
+    from picpp_amd import workloads as wl
+
+    w = wl.config2(n=(args.cells,) * 3, ppc=args.ppc, steps=0)
+    sim = open_synthetic_box(w, torch, dev, precision="fp32", stop_rule="true_residual", solver="mixed")
+    npart = sim.num_particles()
+    training = log_and_train(sim, args, torch)
+    held = held_out_steps(sim, torch, True, args.precond_production)
+    sim.close()
+    return {"cells": [args.cells] * 3, "ppc_per_species": args.ppc, "particles": npart, "dt": w.dt, "training": training,
+            "held_out": held}
 
 
 def time_steps(sim, args):
@@ -700,9 +724,9 @@ def main():
     if args.explicit > 0:
         print(json.dumps({"explicit": explicit_run(args, torch, dev)}))
         return
-    learned = args.learned != "off" and args.solver == "mixed"
+    learned = args.learned != "off" and args.solver == "mixed" and args.precision == "fp32"
     training = None
-    if learned:
+    if learned and args.train_on == "small":
         params, mu_scale, training = train_network(args, torch, dev)
     n = (args.cells,) * 3
     w = wl.config2(n=n, ppc=args.ppc, steps=args.steps)
@@ -711,10 +735,21 @@ def main():
     npart = sim.num_particles()
     print("bench: %s, %d particles, precision %s, solver %s" % (w.name, npart, args.precision, args.solver), file=sys.stderr,
           flush=True)
-    if learned:
-        sim.precond_create()
-        sim.set_precond_params(params, mu_scale)
+    def setup_learned():
+        if args.train_on == "production":  # the box's own solves (untimed)
+            t = log_and_train(sim, args, torch)
+        else:
+            t = training
+            sim.precond_create()
+            sim.set_precond_params(params, mu_scale)
         sim.precond_use(args.learned == "use")
+        return t
+
+    # With the network in the timed steps it is trained first; for the comparison alone the
+    # timed steps start from the initial state, as the plain FP64 step's do, and the network
+    # is trained on the solves of the steps that follow them.
+    if learned and args.learned == "use":
+        training = setup_learned()
     el, solves = time_steps(sim, args)
     for i, s in enumerate(solves):
         print("step %d: gmres %d iterations, %.3f ms" % (i, s["iterations"], 1e3 * s["seconds"]), file=sys.stderr)
@@ -726,7 +761,11 @@ def main():
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     traffic = measured_traffic(dom, args)
     energy = sim.updateEnergy()
-    held_out = compare_solvers(sim, torch, learned) if args.solver == "mixed" and not args.no_held_out else None
+    if learned and args.learned != "use":
+        sim._chk(sim.lib.pic_profile_enable(sim.h, 0), "pic_profile_enable")
+        training = setup_learned()
+    held_out = (held_out_steps(sim, torch, learned, args.held_out_steps)
+                if args.solver == "mixed" and not args.no_held_out and args.held_out_steps > 0 else None)
     sim.close()
     del sim
     torch.cuda.empty_cache()
@@ -752,7 +791,7 @@ def main():
         "gmres": {"ms_per_solve": 1e3 * float(np.mean([s["seconds"] for s in solves])),
                   "iterations_per_solve": float(np.mean([s["iterations"] for s in solves])),
                   "true_residual": float(max(s["true_residual"] for s in solves)), "stop_rule": args.stop_rule,
-                  "held_out_system": held_out, "training": training},
+                  "held_out": held_out, "training": training},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": per_launch},

@@ -512,12 +512,14 @@ int wgrad_chunk(long long V) {
 	return (int)(c < 64 ? 64 : c);
 }
 
-// Workgroups of the MFMA weight gradient: contiguous ranges of 32-voxel groups, at most 1024
-// (256 for the widest layers) so that the partial sums stay small.
+// Workgroups of the MFMA weight gradient: contiguous ranges of at least 16 groups of 32
+// voxels, at most 1024 workgroups (256 for the widest layers), so that the partial sums stay
+// small next to the work.
 long long wgrad_blocks(long long V, long long stride, long long* per) {
 	const long long ng = V / 32;
 	const long long cap = stride <= 16384 ? 1024 : 256;
-	const long long target = ng < cap ? ng : cap;
+	const long long want = ng / 16 > 1 ? ng / 16 : 1;
+	const long long target = want < cap ? want : cap;
 	*per = (ng + target - 1) / target;
 	return (ng + *per - 1) / *per;
 }
