@@ -450,14 +450,19 @@ def short_kernel_name(nm):
     return "".join(out).replace("void ", "", 1)
 
 
-def trace_summary(d, top=40):
+def trace_summary(d, top=40, skip=2, first=5):
     """Dispatch statistics per kernel and grid size from the rocprofv3 kernel-trace CSVs under d:
     the launches of one kernel on boxes of different sizes (the bench's 128^3 box, the learned
-    preconditioner's 32^3 training box) are kept apart."""
+    preconditioner's 32^3 box) are kept apart. For the launches of 16 M work-items and more
+    (the production box's particle and grid kernels) a second table gives the average of
+    launches skip + 1 .. skip + first of each kernel: in the default run those are the chosen
+    form's timed steps (after 2 warmup steps; the FP32 kernels' first launches), which come
+    before the steps that log the learned preconditioner's training pairs."""
     import csv
     import glob
 
     stats = {}
+    launches = {}
     for f in sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)):
         with open(f) as fh:
             for r in csv.DictReader(fh):
@@ -470,6 +475,8 @@ def trace_summary(d, top=40):
                 ns = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
                 c, t = stats.get(k, (0, 0))
                 stats[k] = (c + 1, t + ns)
+                if grid >= (1 << 24):
+                    launches.setdefault(k, []).append((int(r["Start_Timestamp"]), ns))
     total = sum(t for c, t in stats.values())
     rows = sorted(stats.items(), key=lambda kv: -kv[1][1])
     lines = ["| Kernel | Grid (work-items) | Calls | Average ms | Total ms | % |", "|---|---|---|---|---|---|"]
@@ -482,6 +489,14 @@ def trace_summary(d, top=40):
             lines.append("| `%s` | %d | %d | %.3f | %.3f | %.1f |" % (nm[:80], grid, c, t / c / 1e6, t / 1e6, 100.0 * t / total))
     lines.append("")
     lines.append("All dispatches: %.1f ms, %d kernel and grid-size pairs." % (total / 1e6, len(stats)))
+    lines.append("")
+    lines.append("Launches %d to %d of each kernel of 16 M work-items and more (by start time):" % (skip + 1, skip + first))
+    lines.append("")
+    lines.append("| Kernel | Grid (work-items) | Launches averaged | Average ms |")
+    lines.append("|---|---|---|---|")
+    window = {k: sorted(v)[skip:skip + first] or sorted(v)[:first] for k, v in launches.items()}
+    for (nm, grid), ls in sorted(window.items(), key=lambda kv: -sum(x[1] for x in kv[1])):
+        lines.append("| `%s` | %d | %d | %.3f |" % (nm[:80], grid, len(ls), sum(x[1] for x in ls) / len(ls) / 1e6))
     return "\n".join(lines)
 
 
