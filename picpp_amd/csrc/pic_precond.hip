@@ -471,6 +471,120 @@ __global__ void __launch_bounds__(256) k_wgrad_mfma(const uint16_t* __restrict__
 	if (threadIdx.x < COUT) out[27LL * NP + threadIdx.x] = bsum[0][threadIdx.x] + bsum[1][threadIdx.x] + bsum[2][threadIdx.x] + bsum[3][threadIdx.x];
 }
 
+// Weight gradient of the 16 -> 16 channel layers (the finest level, most of the training's
+// FLOPs) on bricks staged in LDS: the operands of k_wgrad_mfma, but each workgroup loads its
+// brick's rows once, 2 x 4 rows of 32 voxels along z, with the input's halo rows (4 x 6) and one
+// 8-voxel piece of margin at each end, from the channel-major copies; the 27 taps' reads then
+// come from LDS. Wave w takes brick rows 2w and 2w + 1 (one group of 32 voxels each) and all
+// 9 (dx, dy) units; the waves' sums are added in wave order through LDS, and each workgroup
+// writes one partial vector (taps, then the bias) for the fixed-order reduction. Needs
+// d0 % 2 = d1 % 4 = d2 % 32 = 0.
+constexpr int kWBX = 2, kWBY = 4;
+__global__ void __launch_bounds__(256) k_wgrad_brick16(const uint16_t* __restrict__ dZc, const uint16_t* __restrict__ Xc, int d0, int d1,
+                                                       int d2, int bricks_per_block, float* __restrict__ partial) {
+	constexpr int C = 16, NP = C * C;
+	constexpr int HX = kWBX + 2, HY = kWBY + 2, NR = kWBX * kWBY, NPC = 6;
+	constexpr long long STRIDE = 27LL * NP + C;
+	__shared__ uint4 xs[C * HX * HY * NPC];  // [ci][halo row][piece]: pieces z0 - 8 .. z0 + 39
+	__shared__ uint4 zs[C * NR * 4];         // [co][row][piece]: z0 .. z0 + 31
+	__shared__ float bsum[4][4][C];          // [wave][kg][co]
+	const long long V = (long long)d0 * d1 * d2;
+	const int np8 = d2 / 8, nbz = d2 / 32, nby = d1 / kWBY, nbx = d0 / kWBX;
+	const int nbricks = nbx * nby * nbz;
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const int r16 = lane & 15, kg = lane >> 4;
+	f32x4 acc[9][3];
+#pragma unroll
+	for (int u = 0; u < 9; ++u)
+#pragma unroll
+		for (int q = 0; q < 3; ++q) acc[u][q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+	float bacc = 0.0f;
+	const int b0 = blockIdx.x * bricks_per_block;
+	const int b1 = b0 + bricks_per_block < nbricks ? b0 + bricks_per_block : nbricks;
+	for (int b = b0; b < b1; ++b) {
+		const int z0 = (b % nbz) * 32, y0 = ((b / nbz) % nby) * kWBY, x0 = (b / nbz / nby) * kWBX;
+		__syncthreads();  // the previous brick's reads are done
+		// one (channel, halo row) per thread and pass, its NPC pieces loaded together (only the
+		// first and the last can wrap along z)
+		const int zp = z0 / 8;
+		for (int e = threadIdx.x; e < C * HX * HY; e += 256) {
+			const int hrow = e % (HX * HY), ci = e / (HX * HY);
+			const int gx = wrap(x0 - 1 + hrow / HY, d0), gy = wrap(y0 - 1 + hrow % HY, d1);
+			const uint16_t* src = Xc + (long long)ci * V + ((long long)gx * d1 + gy) * d2;
+			uint4 v[NPC];
+#pragma unroll
+			for (int pc = 0; pc < NPC; ++pc) {
+				const int piece = pc == 0 ? (zp == 0 ? np8 - 1 : zp - 1) : (pc == NPC - 1 ? (zp + 4 == np8 ? 0 : zp + 4) : zp + pc - 1);
+				v[pc] = *reinterpret_cast<const uint4*>(src + piece * 8);
+			}
+#pragma unroll
+			for (int pc = 0; pc < NPC; ++pc) xs[e * NPC + pc] = v[pc];
+		}
+		if (threadIdx.x < C * NR) {
+			const int row = threadIdx.x % NR, co = threadIdx.x / NR;
+			const uint4* src = reinterpret_cast<const uint4*>(dZc + (long long)co * V + ((long long)(x0 + row / kWBY) * d1 + y0 + row % kWBY) * d2 + z0);
+			uint4 v[4];
+#pragma unroll
+			for (int pc = 0; pc < 4; ++pc) v[pc] = src[pc];
+#pragma unroll
+			for (int pc = 0; pc < 4; ++pc) zs[threadIdx.x * 4 + pc] = v[pc];
+		}
+		__syncthreads();
+#pragma unroll
+		for (int rr = 0; rr < 2; ++rr) {
+			const int row = 2 * w + rr, rx = row / kWBY, ry = row % kWBY;
+			const uint4 aq = zs[(r16 * NR + row) * 4 + kg];
+			const bf16x8 A = __builtin_bit_cast(bf16x8, aq);
+			{
+				float f[8];
+				unpack8(aq, f);
+#pragma unroll
+				for (int j = 0; j < 8; ++j) bacc += f[j];
+			}
+#pragma unroll
+			for (int u = 0; u < 9; ++u) {
+				const int hrow = (rx + u / 3) * HY + ry + u % 3;  // (rx + 1 + dx, ry + 1 + dy)
+				const int base = (r16 * HX * HY + hrow) * NPC + kg + 1;
+				const uint4 Lm = xs[base - 1], L0 = xs[base], Lp = xs[base + 1];
+				const bf16x8 Bm = __builtin_bit_cast(bf16x8, make_uint4(__builtin_amdgcn_alignbit(L0.x, Lm.w, 16), __builtin_amdgcn_alignbit(L0.y, L0.x, 16),
+				                                                       __builtin_amdgcn_alignbit(L0.z, L0.y, 16), __builtin_amdgcn_alignbit(L0.w, L0.z, 16)));
+				const bf16x8 Bc = __builtin_bit_cast(bf16x8, L0);
+				const bf16x8 Bp = __builtin_bit_cast(bf16x8, make_uint4(__builtin_amdgcn_alignbit(L0.y, L0.x, 16), __builtin_amdgcn_alignbit(L0.z, L0.y, 16),
+				                                                       __builtin_amdgcn_alignbit(L0.w, L0.z, 16), __builtin_amdgcn_alignbit(Lp.x, L0.w, 16)));
+				acc[u][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bm, acc[u][0], 0, 0, 0);
+				acc[u][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bc, acc[u][1], 0, 0, 0);
+				acc[u][2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bp, acc[u][2], 0, 0, 0);
+			}
+		}
+	}
+	// the waves' sums, added in wave order in LDS (the staging buffer), then written coalesced
+	__syncthreads();
+	float* red = reinterpret_cast<float*>(xs);  // 27 x 256 floats
+	for (int wv = 0; wv < 4; ++wv) {
+		if (w == wv) {
+#pragma unroll
+			for (int u = 0; u < 9; ++u)
+#pragma unroll
+				for (int q = 0; q < 3; ++q)
+#pragma unroll
+					for (int i = 0; i < 4; ++i) {
+						const int idx = (u * 3 + q) * NP + (4 * kg + i) * C + r16;  // [t][co][ci]
+						red[idx] = (wv == 0 ? 0.0f : red[idx]) + acc[u][q][i];
+					}
+			bsum[wv][kg][r16] = bacc;
+		}
+		__syncthreads();
+	}
+	float* out = partial + (long long)blockIdx.x * STRIDE;
+	for (int e = threadIdx.x; e < 27 * NP; e += 256) out[e] = red[e];
+	if (threadIdx.x < C) {
+		float sb = 0.0f;
+		for (int wv = 0; wv < 4; ++wv)
+			for (int k = 0; k < 4; ++k) sb += bsum[wv][k][threadIdx.x];
+		out[27 * NP + threadIdx.x] = sb;
+	}
+}
+
 // out[p] += sum_c partial[c][p], c in order
 __global__ void k_reduce_chunks(const float* __restrict__ partial, int nchunk, long long stride, float* __restrict__ out) {
 	long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -537,6 +651,14 @@ void wgrad(const uint16_t* dZ, const uint16_t* X, const int d[3], float* partial
 	if (wgrad_on_mfma(d)) {
 		hipLaunchKernelGGL((k_channel_major<COUT>), dim3((unsigned)(V / 64)), dim3(256), 0, s, dZ, V, dzc);
 		hipLaunchKernelGGL((k_channel_major<CIN>), dim3((unsigned)(V / 64)), dim3(256), 0, s, X, V, xc);
+		if (CIN == 16 && COUT == 16 && d[0] % kWBX == 0 && d[1] % kWBY == 0 && d[2] % 32 == 0) {
+			const int nbricks = (d[0] / kWBX) * (d[1] / kWBY) * (d[2] / 32);
+			const int per = (nbricks + 1023) / 1024;
+			const int nblk = (nbricks + per - 1) / per;
+			hipLaunchKernelGGL(k_wgrad_brick16, dim3((unsigned)nblk), dim3(256), 0, s, dzc, xc, d[0], d[1], d[2], per, partial);
+			reduce_partials(partial, nblk, stride, tmp, grad_w, s);
+			return;
+		}
 		long long per = 0;
 		const long long nblk = wgrad_blocks(V, stride, &per);
 		hipLaunchKernelGGL((k_wgrad_mfma<CIN, COUT>), dim3((unsigned)nblk), dim3(256), 0, s, dzc, xc, d[0], d[1], d[2], per, partial);
