@@ -599,11 +599,12 @@ def weibel_check(args, torch, dev):
            "pressure": args.pressure or "reference", "learned_preconditioner": learned, "training": training}
     for name, (e, el, npart) in res.items():
         lb = np.log(np.maximum(e[:, 2], 1e-300))
-        win = 20
         # B starts at zero and the first steps only bring up the particle noise level: the
-        # search for the exponential phase starts after the first 10 % of the steps
+        # search for the exponential phase starts after the first 10 % of the steps (short runs
+        # fit shorter windows)
         k0 = max(1, len(t) // 10)
-        slopes = [np.polyfit(t[k:k + win], lb[k:k + win], 1)[0] for k in range(k0, len(t) - win)]
+        win = max(2, min(20, len(t) - k0 - 1))
+        slopes = [np.polyfit(t[k:k + win], lb[k:k + win], 1)[0] for k in range(k0, len(t) - win + 1)]
         k = k0 + int(np.argmax(slopes))
         out[name] = {**solver_stats[name], "ms_per_step": 1e3 * el / args.weibel, "particle_steps_per_s": npart * args.weibel / el,
                      "magnetic_energy_growth_rate": float(slopes[k - k0]), "growth_window_steps": [k, k + win],

@@ -643,41 +643,65 @@ bool wgrad_on_mfma(const int d[3]) {
 	return d[2] % 8 == 0 && V % 64 == 0;
 }
 
-template <int CIN, int COUT>
-void wgrad(const uint16_t* dZ, const uint16_t* X, const int d[3], float* partial, float* grad_w, uint16_t* dzc, uint16_t* xc,
-           float* tmp, hipStream_t s) {
-	long long V = (long long)d[0] * d[1] * d[2];
-	long long stride = 27LL * COUT * CIN + COUT;
-	if (wgrad_on_mfma(d)) {
-		hipLaunchKernelGGL((k_channel_major<COUT>), dim3((unsigned)(V / 64)), dim3(256), 0, s, dZ, V, dzc);
-		hipLaunchKernelGGL((k_channel_major<CIN>), dim3((unsigned)(V / 64)), dim3(256), 0, s, X, V, xc);
-		if (CIN == 16 && COUT == 16 && d[0] % kWBX == 0 && d[1] % kWBY == 0 && d[2] % 32 == 0) {
-			const int nbricks = (d[0] / kWBX) * (d[1] / kWBY) * (d[2] / 32);
-			const int per = (nbricks + 1023) / 1024;
-			const int nblk = (nbricks + per - 1) / per;
-			hipLaunchKernelGGL(k_wgrad_brick16, dim3((unsigned)nblk), dim3(256), 0, s, dzc, xc, d[0], d[1], d[2], per, partial);
-			reduce_partials(partial, nblk, stride, tmp, grad_w, s);
-			return;
-		}
-		long long per = 0;
-		const long long nblk = wgrad_blocks(V, stride, &per);
-		hipLaunchKernelGGL((k_wgrad_mfma<CIN, COUT>), dim3((unsigned)nblk), dim3(256), 0, s, dzc, xc, d[0], d[1], d[2], per, partial);
-		reduce_partials(partial, (int)nblk, stride, tmp, grad_w, s);
-		return;
-	}
-	int chunk = wgrad_chunk(V);
-	int nchunk = (int)((V + chunk - 1) / chunk);
-	hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(28, nchunk), dim3(256), 0, s, dZ, X, d[0], d[1], d[2], chunk, partial);
-	reduce_partials(partial, nchunk, stride, tmp, grad_w, s);
+// The weight gradient's kernel for a layer: the LDS brick kernel (16 -> 16 channels), the
+// MFMA kernel, or the FP32 kernel; and the number of partial vectors it writes (one per
+// workgroup, or per chunk), with `per` bricks, voxel groups or voxels per workgroup. The
+// partial buffer is sized from the same count.
+enum class WgradKind { Brick16, Mfma, Fp32 };
+
+WgradKind wgrad_kind(int cin, int cout, const int d[3]) {
+	if (!wgrad_on_mfma(d)) return WgradKind::Fp32;
+	if (cin == 16 && cout == 16 && d[0] % kWBX == 0 && d[1] % kWBY == 0 && d[2] % 32 == 0) return WgradKind::Brick16;
+	return WgradKind::Mfma;
 }
 
-void wgrad_any(int cin, int cout, const uint16_t* dZ, const uint16_t* X, const int d[3], float* partial, float* grad_w,
-               uint16_t* dzc, uint16_t* xc, float* tmp, hipStream_t s) {
-	if (cin == 16 && cout == 16) return wgrad<16, 16>(dZ, X, d, partial, grad_w, dzc, xc, tmp, s);
-	if (cin == 16 && cout == 32) return wgrad<16, 32>(dZ, X, d, partial, grad_w, dzc, xc, tmp, s);
-	if (cin == 32 && cout == 64) return wgrad<32, 64>(dZ, X, d, partial, grad_w, dzc, xc, tmp, s);
-	if (cin == 64 && cout == 32) return wgrad<64, 32>(dZ, X, d, partial, grad_w, dzc, xc, tmp, s);
-	if (cin == 32 && cout == 16) return wgrad<32, 16>(dZ, X, d, partial, grad_w, dzc, xc, tmp, s);
+long long wgrad_partials(int cin, int cout, const int d[3], long long* per) {
+	const long long V = (long long)d[0] * d[1] * d[2];
+	const long long stride = 27LL * cin * cout + cout;
+	switch (wgrad_kind(cin, cout, d)) {
+	case WgradKind::Brick16: {
+		const long long nbricks = (long long)(d[0] / kWBX) * (d[1] / kWBY) * (d[2] / 32);
+		*per = (nbricks + 1023) / 1024;
+		return (nbricks + *per - 1) / *per;
+	}
+	case WgradKind::Mfma:
+		return wgrad_blocks(V, stride, per);
+	default:
+		*per = wgrad_chunk(V);
+		return (V + *per - 1) / *per;
+	}
+}
+
+template <int CIN, int COUT>
+void wgrad(const uint16_t* dZ, const uint16_t* X, const int d[3], float* partial, long long partial_len, float* grad_w,
+           uint16_t* dzc, uint16_t* xc, float* tmp, hipStream_t s) {
+	const long long V = (long long)d[0] * d[1] * d[2];
+	const long long stride = 27LL * COUT * CIN + COUT;
+	long long per = 0;
+	const long long nblk = wgrad_partials(CIN, COUT, d, &per);
+	if (nblk * stride > partial_len)
+		throw std::runtime_error("learned preconditioner: weight-gradient partials exceed their buffer");
+	const WgradKind kind = wgrad_kind(CIN, COUT, d);
+	if (kind != WgradKind::Fp32) {
+		hipLaunchKernelGGL((k_channel_major<COUT>), dim3((unsigned)(V / 64)), dim3(256), 0, s, dZ, V, dzc);
+		hipLaunchKernelGGL((k_channel_major<CIN>), dim3((unsigned)(V / 64)), dim3(256), 0, s, X, V, xc);
+	}
+	if (kind == WgradKind::Brick16)
+		hipLaunchKernelGGL(k_wgrad_brick16, dim3((unsigned)nblk), dim3(256), 0, s, dzc, xc, d[0], d[1], d[2], (int)per, partial);
+	else if (kind == WgradKind::Mfma)
+		hipLaunchKernelGGL((k_wgrad_mfma<CIN, COUT>), dim3((unsigned)nblk), dim3(256), 0, s, dzc, xc, d[0], d[1], d[2], per, partial);
+	else
+		hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(28, (unsigned)nblk), dim3(256), 0, s, dZ, X, d[0], d[1], d[2], (int)per, partial);
+	reduce_partials(partial, (int)nblk, stride, tmp, grad_w, s);
+}
+
+void wgrad_any(int cin, int cout, const uint16_t* dZ, const uint16_t* X, const int d[3], float* partial, long long partial_len,
+               float* grad_w, uint16_t* dzc, uint16_t* xc, float* tmp, hipStream_t s) {
+	if (cin == 16 && cout == 16) return wgrad<16, 16>(dZ, X, d, partial, partial_len, grad_w, dzc, xc, tmp, s);
+	if (cin == 16 && cout == 32) return wgrad<16, 32>(dZ, X, d, partial, partial_len, grad_w, dzc, xc, tmp, s);
+	if (cin == 32 && cout == 64) return wgrad<32, 64>(dZ, X, d, partial, partial_len, grad_w, dzc, xc, tmp, s);
+	if (cin == 64 && cout == 32) return wgrad<64, 32>(dZ, X, d, partial, partial_len, grad_w, dzc, xc, tmp, s);
+	if (cin == 32 && cout == 16) return wgrad<32, 16>(dZ, X, d, partial, partial_len, grad_w, dzc, xc, tmp, s);
 	throw std::runtime_error("wgrad: unsupported channel pair");
 }
 
@@ -999,18 +1023,10 @@ UNet::UNet(const int d[3]) {
 	z_ = dmalloc<float>(3 * V);
 	long long worst = 0;
 	for (auto& c : convs_) {
-		long long Vl = V >> (3 * c.level);
 		const long long stride = 27LL * c.cin * c.cout + c.cout;
 		const int dl[3] = {d[0] >> c.level, d[1] >> c.level, d[2] >> c.level};
-		long long nchunk;
-		if (wgrad_on_mfma(dl)) {
-			long long per = 0;
-			nchunk = wgrad_blocks(Vl, stride, &per);
-		} else {
-			long long chunk = wgrad_chunk(Vl);
-			nchunk = (Vl + chunk - 1) / chunk;
-		}
-		long long need = nchunk * stride;
+		long long per = 0;
+		const long long need = wgrad_partials(c.cin, c.cout, dl, &per) * stride;
 		if (need > worst) worst = need;
 	}
 	dzc_ = dmalloc<uint16_t>(16 * V);  // channel-major copies for the weight gradients (largest: level 0, 16 channels)
@@ -1097,7 +1113,7 @@ void UNet::conv_backward(const ConvSpec& c, const float* dout, const uint16_t* a
 	const int dl[3] = {d_[0] >> lvl, d_[1] >> lvl, d_[2] >> lvl};
 	long long Vl = level_voxels(lvl);
 	hipLaunchKernelGGL(k_relu_mask, dim3(nb(Vl * c.cout / 8, 256)), dim3(256), 0, s, dout, act, Vl * c.cout, dz_);
-	wgrad_any(c.cin, c.cout, dz_, input, dl, partial_, grads_ + c.w_off, dzc_, xc_, rtmp_, s);
+	wgrad_any(c.cin, c.cout, dz_, input, dl, partial_, partial_len_, grads_ + c.w_off, dzc_, xc_, rtmp_, s);
 	if (din) {
 		size_t k = &c - convs_.data();
 		conv3_any<float>(c.cout, c.cin, dz_, wb_ + weight_offset(convs_, k), nullptr, false, din, dl, s);

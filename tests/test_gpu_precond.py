@@ -40,11 +40,13 @@ def rand_params(seed):
     return p.astype(np.float32)
 
 
-@pytest.mark.parametrize("grid", [(8, 8, 8), (16, 16, 64)])
+@pytest.mark.parametrize("grid", [(8, 8, 8), (16, 16, 64), (32, 32, 32)])
 def test_network_forward_and_gradients_match_cpu(grid):
     """8^3: every convolution on the voxel-row kernel (the coarse levels are narrower than a
     16-voxel row); 16 x 16 x 64: every convolution, forward and adjoint, on the LDS brick
-    kernel (levels 16 x 16 x 64, 8 x 8 x 32, 4 x 4 x 16)."""
+    kernel (levels 16 x 16 x 64, 8 x 8 x 32, 4 x 4 x 16); 32^3 (the size of the small training
+    box): the 16 -> 16 weight gradients on the brick kernel with more workgroups than the
+    MFMA kernel would use, which the partial-sum buffer must hold."""
     torch = _torch()
     w = wl.mini(n=grid, ppc=4, seed=11)
     d = w.n
