@@ -11,6 +11,8 @@
 // octant); a group of G lanes serves one cell (node moments) or one dual cell (cell moments,
 // density) and takes its particles round-robin. Each lane sums its particles in that fixed
 // order, the group then reduces through LDS in a fixed lane order and writes 8 corner sums.
+#include <type_traits>
+
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
 #include "pic_kernels_fp32.hpp"
@@ -196,6 +198,187 @@ __device__ inline void gather_boxed(const Grid& g, const float4* __restrict__ f,
 	sum8(e, w0, w1, w2, out);
 }
 
+// The FP32 kernels take reciprocals and inverse square roots from the hardware's
+// v_rcp_f32 / v_rsq_f32 (about 1 ulp) instead of correctly rounded divisions and square
+// roots, which cost about ten instructions each.
+__device__ inline float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float rsq32(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// ------------------------------------------------------------------ coefficient boxes
+// The CIC gather of a 2 x 2 x 2 stencil with corner values p[4i + 2j + k] at local
+// coordinates (x, y, z) in [0, 1) (the weight of corner 1 along an axis) is the trilinear
+// interpolant
+//   F = (c0 + z cz) + y (cy + z cyz) + x ((cx + z cxz) + y (cxy + z cxyz)),
+// seven fused multiply-adds per component once the 8 coefficients are formed, against 12
+// weight products and 24 multiply-adds for the weighted corner sum of 3 components. The
+// node map's local coordinate is the offset t in the cell, the cell map's t + 1/2 from the
+// cell below (t < 1/2) or t - 1/2 from its own cell: the weights (1 - x, x) are the closed
+// forms of correlationBspline (PIC++/correlation.cpp:824-835).
+//
+// A push workgroup whose particles lie in one (i, j) row of at most kCoefK cells stages, in
+// LDS, the coefficients of every stencil origin that its gathers can use: E^n at the row's
+// own nodes, E^{n+theta} with one node and B with two cells of margin (Picard iterates up
+// to one cell away). A gather whose origin lies in its box reads 24 floats; any other forms
+// the coefficients from the 8 corner values in HBM with the same operations, so both paths
+// give the same bits.
+constexpr int kCoefK = 12;
+
+// p: the 8 corner values (x, y, z per point); c[2r], c[2r + 1] = component r's
+// (c0, cz, cy, cyz), (cx, cxz, cxy, cxyz)
+__device__ inline void trilinear_coefs(const float4 p[8], float4 c[6]) {
+#pragma unroll
+	for (int r = 0; r < 3; ++r) {
+		float q[8];
+#pragma unroll
+		for (int t = 0; t < 8; ++t) q[t] = r == 0 ? p[t].x : r == 1 ? p[t].y : p[t].z;
+		const float cz = q[1] - q[0], cy = q[2] - q[0], cx = q[4] - q[0];
+		const float cyz = (q[3] - q[2]) - cz;
+		const float cxz = (q[5] - q[4]) - cz;
+		const float cxy = (q[6] - q[4]) - cy;
+		const float cxyz = ((q[7] - q[6]) - (q[5] - q[4])) - cyz;
+		c[2 * r] = make_float4(q[0], cz, cy, cyz);
+		c[2 * r + 1] = make_float4(cx, cxz, cxy, cxyz);
+	}
+}
+
+__device__ inline float trilinear_eval(const float a[4], const float b[4], float x, float y, float z) {
+	const float g = fmaf(y, fmaf(z, a[3], a[2]), fmaf(z, a[1], a[0]));
+	const float h = fmaf(y, fmaf(z, b[3], b[2]), fmaf(z, b[1], b[0]));
+	return fmaf(x, h, g);
+}
+
+// node (nb, tn) and cell (cb, tc) map bases and local coordinates of a point at offset f
+// from the lower corner of cell c (f may leave [0, 1) during the Picard iterations)
+struct Loc32 {
+	int nb, cb;
+	float tn, tc;
+};
+
+__device__ inline void locs32(const int c[3], const float f[3], Loc32 L[3]) {
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		const float fl = floorf(f[a]);
+		const int cc = c[a] + (int)fl;
+		const float t = f[a] - fl;
+		const bool lo = t < 0.5f;
+		L[a].nb = cc;
+		L[a].tn = t;
+		L[a].cb = lo ? cc - 1 : cc;
+		L[a].tc = lo ? t + 0.5f : t - 0.5f;
+	}
+}
+
+// (W - 1) x (W - 1) x (K + W - 2) stencil origins of a W-point box, 24 floats each
+template <int W>
+struct CBox32 {
+	int o[3];    // ghosted index of origin (0, 0, 0)
+	int limk;    // last origin along k: K + W - 3
+	int sj, si;  // origin strides along j and i
+	int off;     // first float of the box in LDS
+};
+
+template <int W>
+__device__ inline CBox32<W> make_cbox(int i0, int j0, int k0, int K, int off) {
+	CBox32<W> b;
+	constexpr int margin = W == 2 ? 0 : W - 3;
+	b.o[0] = i0 - margin;
+	b.o[1] = j0 - margin;
+	b.o[2] = k0 - margin;
+	b.limk = K + W - 3;
+	b.sj = K + W - 2;
+	b.si = (W - 1) * b.sj;
+	b.off = off;
+	return b;
+}
+
+// Gather with coefficients: from the box when the workgroup has one (use) and the origin
+// lies in it, else from HBM with the bases clamped into the ghosted arrays (a clamp sets
+// flag in why).
+template <bool NODE, int W>
+__device__ inline void gather_coef(const Grid& g, const float4* __restrict__ f, lds_float* lds, const CBox32<W>& b, bool use,
+                                   const Loc32 L[3], int flag, int& why, float out[3]) {
+	int a = NODE ? L[0].nb : L[0].cb, c = NODE ? L[1].nb : L[1].cb, d = NODE ? L[2].nb : L[2].cb;
+	const float x = NODE ? L[0].tn : L[0].tc, y = NODE ? L[1].tn : L[1].tc, z = NODE ? L[2].tn : L[2].tc;
+	const int la = a - b.o[0], lc = c - b.o[1], ld = d - b.o[2];
+	if (use && (unsigned)la <= (unsigned)(W - 2) && (unsigned)lc <= (unsigned)(W - 2) && (unsigned)ld <= (unsigned)b.limk) {
+		// one component's 8 coefficients at a time (fewer registers live)
+		lds_float* p = lds + b.off + 24 * (la * b.si + lc * b.sj + ld);
+#pragma unroll
+		for (int r = 0; r < 3; ++r) {
+			float c8[8];
+#pragma unroll
+			for (int t = 0; t < 8; ++t) c8[t] = p[8 * r + t];
+			out[r] = trilinear_eval(c8, c8 + 4, x, y, z);
+		}
+		return;
+	}
+	const int lim0 = NODE ? g.N[0] - 1 : g.N[0] - 2, lim1 = NODE ? g.N[1] - 1 : g.N[1] - 2, lim2 = NODE ? g.N[2] - 1 : g.N[2] - 2;
+	if (a < 0 || a > lim0 || c < 0 || c > lim1 || d < 0 || d > lim2) why |= flag;
+	a = clampi(a, 0, lim0);
+	c = clampi(c, 0, lim1);
+	d = clampi(d, 0, lim2);
+	const int sj = NODE ? g.N[2] + 1 : g.N[2], si = (NODE ? g.N[1] + 1 : g.N[1]) * sj;
+	const int base = a * si + c * sj + d;
+	float4 p[8], cf[6];
+#pragma unroll
+	for (int t = 0; t < 8; ++t) p[t] = f[base + ((t >> 2) & 1) * si + ((t >> 1) & 1) * sj + (t & 1)];
+	trilinear_coefs(p, cf);
+#pragma unroll
+	for (int r = 0; r < 3; ++r) {
+		const float c8[8] = {cf[2 * r].x, cf[2 * r].y, cf[2 * r].z, cf[2 * r].w,
+		                     cf[2 * r + 1].x, cf[2 * r + 1].y, cf[2 * r + 1].z, cf[2 * r + 1].w};
+		out[r] = trilinear_eval(c8, c8 + 4, x, y, z);
+	}
+}
+
+// Stages the three coefficient boxes of a row of K cells from cell (i0, j0, k0): E^n (K
+// origins), E^{n+theta} (9 (K + 2)) and B (16 (K + 3)), concatenated in LDS in origin order,
+// so that origin o of the concatenation is at float 24 o. Corner indexes are clamped into
+// the arrays (the boxes of a row of owned cells lie inside them).
+__device__ inline void stage_coef_boxes(const Grid& g, int i0, int j0, int k0, int K, const float4* __restrict__ E,
+                                        const float4* __restrict__ Et, const float4* __restrict__ B, float4* lds) {
+	const int nEn = K, zt = K + 2, zb = K + 3;
+	const int nEt = 9 * zt, total = nEn + nEt + 16 * zb;
+	const float rzt = rcp32((float)zt), rzb = rcp32((float)zb);
+	for (int o = threadIdx.x; o < total; o += blockDim.x) {
+		// Origin o's box and local (la, lc, ld); row quotients from a float reciprocal, exact
+		// for these sizes (the fraction of (e + 1/2) / len is at least 1 / (2 len) away from
+		// an integer), and r / 3 = (11 r) >> 5 for r < 9.
+		int a, c, d;
+		bool node = true;
+		const float4* src = E;
+		if (o < nEn) {
+			a = i0;
+			c = j0;
+			d = k0 + o;
+		} else if (o < nEn + nEt) {
+			const int e = o - nEn, r = (int)(((float)e + 0.5f) * rzt), la = (r * 11) >> 5;
+			a = i0 - 1 + la;
+			c = j0 - 1 + (r - 3 * la);
+			d = k0 - 1 + (e - r * zt);
+			src = Et;
+		} else {
+			const int e = o - nEn - nEt, r = (int)(((float)e + 0.5f) * rzb);
+			a = i0 - 2 + (r >> 2);
+			c = j0 - 2 + (r & 3);
+			d = k0 - 2 + (e - r * zb);
+			node = false;
+			src = B;
+		}
+		const int l0 = node ? g.N[0] : g.N[0] - 1, l1 = node ? g.N[1] : g.N[1] - 1, l2 = node ? g.N[2] : g.N[2] - 1;
+		const int sj = node ? g.N[2] + 1 : g.N[2], si = (node ? g.N[1] + 1 : g.N[1]) * sj;
+		const int ia[2] = {clampi(a, 0, l0) * si, clampi(a + 1, 0, l0) * si};
+		const int ic[2] = {clampi(c, 0, l1) * sj, clampi(c + 1, 0, l1) * sj};
+		const int id[2] = {clampi(d, 0, l2), clampi(d + 1, 0, l2)};
+		float4 p[8], cf[6];
+#pragma unroll
+		for (int t = 0; t < 8; ++t) p[t] = src[ia[(t >> 2) & 1] + ic[(t >> 1) & 1] + id[t & 1]];
+		trilinear_coefs(p, cf);
+#pragma unroll
+		for (int r = 0; r < 6; ++r) lds[6 * o + r] = cf[r];
+	}
+}
+
 // The 8 stencil values of a fixed stencil, loaded once and reused for every particle of
 // a segment (all particles of one key share both stencil bases).
 struct Stencil8 {
@@ -222,12 +405,6 @@ __device__ inline void interp8(const Stencil8& s, const float w0[2], const float
 		out[2] += s.v[t][2] * c;
 	}
 }
-
-// The FP32 kernels take reciprocals and inverse square roots from the hardware's
-// v_rcp_f32 / v_rsq_f32 (about 1 ulp) instead of correctly rounded divisions and square
-// roots, which cost about ten instructions each.
-__device__ inline float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ inline float rsq32(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // alpha tensor (PIC++/mover.cpp:851-908) in FP32, with v and gamma from u = gamma v.
 __device__ inline void alpha32(float beta, const float v[3], float gamma, const float E[3], const float B[3], float a[9]) {
@@ -682,14 +859,17 @@ __device__ inline void cell_ijk_magic(const Grid& g, const Push32Consts& K, int 
 // moveParticle (PIC++/mover.cpp:194-491) on cell offsets, one thread per particle; alpha is
 // recomputed from E^n, B^n at x^n. Momentum-space constants: du = (q dt / m) (E + v x B).
 template <bool WALL>
-__global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32Consts K, ParticleArrays32 P,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WALL ? 1 : 4))) k_push32(Grid g, SpeciesTable sp, Push32Consts K, ParticleArrays32 P,
                                                 const int* __restrict__ species_start, const float4* __restrict__ E,
                                                 const float4* __restrict__ Et, const float4* __restrict__ B, float dt, float eta,
                                                 uint32_t* __restrict__ keys, uint64_t* __restrict__ mask, int* __restrict__ status,
                                                 int* __restrict__ dead, uint32_t* __restrict__ esc,
                                                 uint64_t* __restrict__ bounce_mask) {
 	constexpr int oEt = Box32<2>::size, oB = Box32<2>::size + Box32<4>::size;
-	__shared__ float4 boxes[Box32<2>::size + Box32<4>::size + Box32<5>::size];  // E^n, E^{n+theta}, B
+	constexpr int kPointF4 = Box32<2>::size + Box32<4>::size + Box32<5>::size;
+	constexpr int kCoefF4 = 6 * (kCoefK + 9 * (kCoefK + 2) + 16 * (kCoefK + 3));
+	// E^n, E^{n+theta}, B: point boxes or coefficient boxes
+	__shared__ float4 boxes[kPointF4 > kCoefF4 ? kPointF4 : kCoefF4];
 	__shared__ int row_ends[2];
 	const long long b0 = (long long)blockIdx.x * blockDim.x;
 	const long long idx = b0 + threadIdx.x;
@@ -701,11 +881,21 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 	cell_ijk_magic(g, K, row_ends[0], rf);
 	cell_ijk_magic(g, K, row_ends[1], rl);
 	const int KB = rl[2] - rf[2] + 1;
-	const bool use_box = rf[0] == rl[0] && rf[1] == rl[1] && KB >= 1 && KB <= kBoxK;
+	// Rows of at most kCoefK cells gather with coefficient boxes, rows of up to kBoxK cells
+	// with point boxes (the coefficient boxes of a long row would not fit the LDS budget of
+	// four workgroups per CU), any other workgroup from HBM.
+	const bool one_row = rf[0] == rl[0] && rf[1] == rl[1] && KB >= 1;
+	const bool coef = one_row && KB <= kCoefK;
+	const bool use_box = one_row && !coef && KB <= kBoxK;
 	const Box32<2> bEn = make_box<2>(rf[0], rf[1], rf[2], KB);
 	const Box32<4> bEt = make_box<4>(rf[0], rf[1], rf[2], KB);
 	const Box32<5> bB = make_box<5>(rf[0], rf[1], rf[2], KB);
-	if (use_box) {
+	const CBox32<2> cEn = make_cbox<2>(rf[0], rf[1], rf[2], KB, 0);
+	const CBox32<4> cEt = make_cbox<4>(rf[0], rf[1], rf[2], KB, 24 * KB);
+	const CBox32<5> cB = make_cbox<5>(rf[0], rf[1], rf[2], KB, 24 * (KB + 9 * (KB + 2)));
+	if (coef) {
+		stage_coef_boxes(g, rf[0], rf[1], rf[2], KB, E, Et, B, boxes);
+	} else if (use_box) {
 		// Stage the boxes: 4, 16 and 25 rows along k of K + W - 1 points (W = 2, 4, 5). Thread t
 		// loads point t of each box in the same pass, so its three loads are in flight together.
 		// Point e of a box is row r = e / len, point ld = e - r len (the quotient from a float
@@ -748,6 +938,14 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 	}
 	__syncthreads();
 	if (idx >= P.count) return;
+	lds_float* lEn = (lds_float*)boxes;
+	lds_float* lEt = (lds_float*)(boxes + oEt);
+	lds_float* lB = (lds_float*)(boxes + oB);
+	lds_float* lco = (lds_float*)boxes;
+	// the particle's push, with the coefficient-box (COEF) or point-box gathers; coef is the
+	// same for the whole workgroup
+	auto body = [&](auto coef_tag) {
+	constexpr bool COEF = decltype(coef_tag)::value;
 	int i = (int)idx;
 	int s = 0;
 	while (s + 1 < sp.count && i >= species_start[s + 1]) ++s;
@@ -761,14 +959,20 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 	const uint32_t key_s = (uint32_t)s * (uint32_t)g.ncells;  // keys fit 32 bits (checked on the host)
 	const uint32_t key0 = ((key_s + (uint32_t)cell0) << 3) | ((f[0] >= 0.5f) << 2) | ((f[1] >= 0.5f) << 1) | (f[2] >= 0.5f);
 	Map32 mp[3];
-	int why = maps32(g, cc, f, mp) ? 0 : 1;  // failed checks: 1 start stencil, 2 Picard stencil, 4 << c new cell
+	Loc32 L[3];
+	int why = 0;  // failed checks: 1 start stencil, 2 Picard stencil, 4 << c new cell
 	float En[3], Bf[3], a[9], Ef[3];
-	lds_float* lEn = (lds_float*)boxes;
-	lds_float* lEt = (lds_float*)(boxes + oEt);
-	lds_float* lB = (lds_float*)(boxes + oB);
-	gather_boxed<true, 2>(g, E, lEn, bEn, use_box, mp, En);
-	gather_boxed<false, 5>(g, B, lB, bB, use_box, mp, Bf);
-	gather_boxed<true, 4>(g, Et, lEt, bEt, use_box, mp, Ef);
+	if constexpr (COEF) {
+		locs32(cc, f, L);
+		gather_coef<true, 2>(g, E, lco, cEn, true, L, 1, why, En);
+		gather_coef<false, 5>(g, B, lco, cB, true, L, 1, why, Bf);
+		gather_coef<true, 4>(g, Et, lco, cEt, true, L, 1, why, Ef);
+	} else {
+		if (!maps32(g, cc, f, mp)) why = 1;
+		gather_boxed<true, 2>(g, E, lEn, bEn, use_box, mp, En);
+		gather_boxed<false, 5>(g, B, lB, bB, use_box, mp, Bf);
+		gather_boxed<true, 4>(g, Et, lEt, bEt, use_box, mp, Ef);
+	}
 	float u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
 	float ig = rsq32(1.0f + u2);
 	float gm = (1.0f + u2) * ig;
@@ -834,9 +1038,15 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 			reflect_at_wall(nx);
 			break;
 		}
-		if (!maps32(g, cc, nx, mp)) why |= 2;
-		gather_boxed<true, 4>(g, Et, lEt, bEt, use_box, mp, Ef);
-		gather_boxed<false, 5>(g, B, lB, bB, use_box, mp, Bf);
+		if constexpr (COEF) {
+			locs32(cc, nx, L);
+			gather_coef<true, 4>(g, Et, lco, cEt, true, L, 2, why, Ef);
+			gather_coef<false, 5>(g, B, lco, cB, true, L, 2, why, Bf);
+		} else {
+			if (!maps32(g, cc, nx, mp)) why |= 2;
+			gather_boxed<true, 4>(g, Et, lEt, bEt, use_box, mp, Ef);
+			gather_boxed<false, 5>(g, B, lB, bB, use_box, mp, Bf);
+		}
 		nu[0] = u[0] + (Ef[0] + (mid[1] * Bf[2] - mid[2] * Bf[1])) * qdm;
 		nu[1] = u[1] + (Ef[1] + (-mid[0] * Bf[2] + mid[2] * Bf[0])) * qdm;
 		nu[2] = u[2] + (Ef[2] + (mid[0] * Bf[1] - mid[1] * Bf[0])) * qdm;
@@ -925,6 +1135,11 @@ __global__ void __launch_bounds__(256) k_push32(Grid g, SpeciesTable sp, Push32C
 		const unsigned long long rb = __ballot(reflected);
 		if ((threadIdx.x & 63) == 0) bounce_mask[i >> 6] = rb;
 	}
+	};
+	if (coef)
+		body(std::integral_constant<bool, true>());
+	else
+		body(std::integral_constant<bool, false>());
 }
 
 // ------------------------------------------------------------------ rebin after a push
