@@ -87,6 +87,20 @@ def test_fp32_nonuniform_density():
         assert v <= TOL32, (k, v)
 
 
+def test_fp32_coefficient_box_gathers():
+    """48 ppc per species in rows of 16 cells: a push workgroup's 256 particles span at most
+    7 cells of a row, so it gathers from trilinear coefficient boxes in LDS; the workgroups
+    that straddle two rows (about a third) gather from HBM with the same coefficients. Every
+    quantity must agree with the oracle to the FP32 bar."""
+    w = wl.mini(n=(8, 6, 16), ppc=48, seed=29)
+    st = wl.make_initial_state(w, perturb=0.05)
+    print("case", w.name, flush=True)
+    errs = first_step_errors(w, st)
+    print("fp32 vs fp64 oracle:", {k: "%.2e" % v for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v <= TOL32, (k, v)
+
+
 def test_fp32_deterministic_bitwise():
     w = wl.mini(n=(10, 8, 6), ppc=6, seed=3)
     st = wl.make_initial_state(w, perturb=0.05)
